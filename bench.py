@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 post-reconstruction filter stage.
+
+One step = the whole in-loop filter chain (plus film grain when the frame has
+grain) over one synthetic 10-bit 4:2:0 3840x2160 frame whose input planes are
+already resident in HBM.  Prints one JSON line (see DESIGN.md for the metric,
+the algorithmic byte counts and the CPU baseline).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 4k10|1080p10|cif8]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import rav2d_amd as R  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    # name: (w, h, bitdepth, sb128, tools)
+    "4k10": (3840, 2160, 10, 1, R.TOOL_ALL),
+    "1080p10": (1920, 1080, 10, 1, R.TOOL_ALL),
+    "cif8": (352, 288, 8, 0, R.TOOL_ALL & ~R.TOOL_GRAIN),
+}
+
+PASS_NAMES = ["deblock_v", "deblock_h", "cdef_ccso", "loop_restoration", "film_grain"]
+
+
+def frame_samples(w, h):
+    return w * h + 2 * (w // 2) * (h // 2)
+
+
+def cpu_baseline(w, h, bd, sb128, tools, seed, budget_s=12.0):
+    """The CPU oracle (scalar restatement of rav2d's chain), one thread, timed on
+    a bounded sample of the same workload: whole frames until ~budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O  # the oracle is only the CPU-baseline leg here
+
+    sf = R.SynthFrame(w, h, bd, sb128, seed, tools=tools)
+    O.filter_frame(sf)  # warm
+    n, t0 = 0, time.perf_counter()
+    while True:
+        O.filter_frame(sf)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s or n >= 200:
+            break
+    sf.close()
+    return {"value": round(n * w * h / 1e6 / dt, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": "%d whole %dx%d %d-bit frames through the oracle chain in %.1f s" % (n, w, h, bd, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="4k10", choices=sorted(CONFIGS))
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    w, h, bd, sb128, tools = CONFIGS[args.config]
+    print("bench: building %s workload %dx%d %d-bit" % (args.config, w, h, bd), file=sys.stderr, flush=True)
+    sf = R.SynthFrame(w, h, bd, sb128, args.seed, tools=tools)
+    ctx = R.GpuContext(0)
+    f = R.GpuFrame.from_synth(ctx, sf)
+    grain = bool(sf.params.grain_present)
+
+    def step():
+        f.filter()
+        if grain:
+            f.apply_grain()
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    # timed region: K steps, inputs resident in HBM
+    wall0 = time.perf_counter()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        step()
+    ms = ctx.timer_stop()
+    wall = (time.perf_counter() - wall0) * 1e3
+    ms_per_step = ms / args.steps
+    mpix = w * h * args.steps / (ms / 1e3) / 1e6
+
+    # per-pass kernel durations over a second run of the same K steps
+    ctx.set_profiling(True)
+    ctx.pass_times(reset=True)
+    for _ in range(args.steps):
+        step()
+    times = ctx.pass_times(reset=True)
+    ctx.set_profiling(False)
+    bps = 2 if bd > 8 else 1
+    S = frame_samples(w, h)
+    passes = {}
+    for i, name in enumerate(PASS_NAMES):
+        tot, cnt = times[i]
+        if cnt:
+            passes[name] = round(tot / cnt * 1e3, 2)  # us per launch
+    # dominant kernel: the pass with the largest total time; its algorithmic
+    # bytes are one read of its input planes and one write of its output planes
+    dom = max(range(5), key=lambda i: times[i][0])
+    dom_us = times[dom][0] / max(times[dom][1], 1) * 1e3
+    alg_bytes = 2 * S * bps
+    achieved = alg_bytes / (dom_us * 1e-6) / 1e9
+    frame_bytes = S * bps * (2 + (1 if grain else 0))
+    chain_gbs = frame_bytes / (ms_per_step * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        print("bench: CPU baseline leg", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(w, h, bd, sb128, tools, args.seed)
+
+    line = {
+        "metric": "bit-exact filtered Mpix/s, 10-bit 4K, batched vs per-frame; p99 ms/frame",
+        "value": round(mpix, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16" if bd > 8 else "u8",
+        "data": "synthetic",
+        "config": {"workload": "per-frame launch, %dx%d %d-bit 4:2:0, sb%d, in-loop chain%s" % (
+            w, h, bd, 64 << sb128, " + film grain" if grain else ""),
+            "passes_built": [n for n in PASS_NAMES if n in passes],
+            "wall_ms_timed_region": round(wall, 2),
+            "chain_GBps_algorithmic": round(chain_gbs, 1),
+            "pass_us": passes},
+        "roofline": {"bound": "hbm", "kernel": PASS_NAMES[dom], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    f.close()
+    ctx.close()
+    sf.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

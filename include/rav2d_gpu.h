@@ -1,0 +1,270 @@
+/*
+ * rav2d_gpu.h — C ABI of the MI355X (gfx950) post-reconstruction filter stage.
+ *
+ * The library replaces rav2d's filter stage, which today runs on the host:
+ *   - the per-superblock-row filter loop of decode_frame_main
+ *     (crates/rav2d/src/decode.rs:3597-3625, body filter_sbrow decode.rs:3718-4058:
+ *     deblock cols -> deblock rows -> copy_db -> CDEF(+CCSO) -> loop restoration);
+ *   - film-grain synthesis on the output copy, apply_grain_to_picture_mt
+ *     (crates/rav2d/src/decode.rs:4672, called from Decoder::output_image
+ *     crates/rav2d/src/decoder.rs:430-435).
+ * The side-info structs below keep the memory layout of the reference's Rust types
+ * field for field (a #[repr(C)] mirror on the Rust side binds them unchanged), so
+ * the masks the host decoder already builds are handed over as they are.
+ *
+ * Conventions: every entry point returns 0 on success or a negative RAV2D_E*
+ * code and never aborts.  Plane strides are in BYTES, as in rav2d's
+ * PictureAllocation (crates/rav2d/src/picture.rs:43-48).  Streams are hipStream_t
+ * passed as void*; NULL means the context's own stream.
+ */
+#ifndef RAV2D_GPU_H
+#define RAV2D_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define RAV2D_OK 0
+#define RAV2D_EINVAL (-22)  /* invalid or unsupported parameter            */
+#define RAV2D_ENOMEM (-12)  /* device or host allocation failed            */
+#define RAV2D_EDEVICE (-5)  /* a HIP call failed                           */
+#define RAV2D_EAGAIN (-11)  /* admission rejected (memory budget / slots)  */
+#define RAV2D_ENOENT (-2)   /* unknown ticket / stream                     */
+
+/* ---- enums with the reference's values --------------------------------- */
+/* PixelLayout, crates/rav2d/src/headers.rs:112-118 */
+#define RAV2D_PIXEL_LAYOUT_I400 0
+#define RAV2D_PIXEL_LAYOUT_I420 1
+#define RAV2D_PIXEL_LAYOUT_I422 2
+#define RAV2D_PIXEL_LAYOUT_I444 3
+
+/* InloopFilterType bits, crates/rav2d/src/decoder.rs:22-41 and
+ * crates/rav2d/src/looprestoration.rs:1697-1706 */
+#define RAV2D_INLOOPFILTER_DEBLOCK (1u << 0)
+#define RAV2D_INLOOPFILTER_CDEF (1u << 1)
+#define RAV2D_INLOOPFILTER_CCSO (1u << 2)
+#define RAV2D_INLOOPFILTER_WIENER (1u << 3)
+#define RAV2D_INLOOPFILTER_GDF (1u << 4)
+#define RAV2D_INLOOPFILTER_ALL 31u
+
+/* RestorationType, crates/rav2d/src/looprestoration.rs:1671-1676 */
+#define RAV2D_RESTORATION_NONE 0
+#define RAV2D_RESTORATION_PC_WIENER 1
+#define RAV2D_RESTORATION_NS_WIENER 2
+#define RAV2D_RESTORATION_SWITCHABLE 3
+
+/* ---- side info: layout-identical mirrors of the reference's types -------- */
+
+/* Av2Filter, crates/rav2d/src/lf_mask.rs:7-19: one per 256x256 luma region,
+ * row-major over the frame's SB256 grid.  filter_y/uv[dir][x4|y4][level 0..3,
+ * 4 = sub-PU][sb64 index] are the deblock edge masks; the rest is CDEF, CCSO,
+ * GDF and LR side info. */
+typedef struct Rav2dAv2Filter {
+    uint16_t filter_y[2][64][5][4];
+    uint16_t filter_uv[2][64][5][4];
+    uint16_t qidx[16];
+    uint8_t gdf[16];
+    int8_t cdef_idx[16];
+    uint8_t ccso[3];
+    uint8_t _pad0;
+    uint16_t noskip_mask[32][4];
+    uint16_t lr_noskip_mask[64][4];
+    uint16_t lossless_mask_y[64][4];
+    uint16_t lossless_mask_uv[64][4];
+} Rav2dAv2Filter;
+
+/* Av2RestorationUnit / Av2Restoration, crates/rav2d/src/lf_mask.rs:1-5, 38-41 */
+typedef struct Rav2dRestorationUnit {
+    uint8_t restoration_type;
+    int8_t ns_filter[16][32];
+} Rav2dRestorationUnit;
+
+typedef struct Rav2dAv2Restoration {
+    Rav2dRestorationUnit lr[3][16];
+} Rav2dAv2Restoration;
+
+/* FilmGrainData, crates/rav2d/src/headers.rs:509-525 */
+typedef struct Rav2dFilmGrainData {
+    int32_t chroma_scaling_from_luma;
+    int32_t num_points[3];
+    uint8_t points[3][14][2];
+    int32_t scaling_shift;
+    int32_t ar_coeff_lag;
+    int8_t ar_coeffs[3][28];
+    uint64_t ar_coeff_shift;
+    int32_t grain_scale_shift;
+    int32_t uv_mult[2];
+    int32_t uv_luma_mult[2];
+    int32_t uv_offset[2];
+    int32_t overlap_flag;
+    int32_t clip_to_restricted_range;
+    int32_t mc_identity;
+    int32_t block_size;
+} Rav2dFilmGrainData;
+
+/* FhCcsoPlane, crates/rav2d/src/headers.rs:650-662 (fields the filter reads) */
+typedef struct Rav2dCcsoPlane {
+    uint8_t enabled;
+    uint8_t bo_only;
+    uint8_t scale_idx;
+    uint8_t quant_idx;
+    uint8_t ext_filter_support;
+    uint8_t edge_clf;
+    uint8_t max_band_log2;
+    uint8_t _pad;
+    uint8_t filter_off[64];
+} Rav2dCcsoPlane;
+
+/* FhRestorationPlane + NSWienerPlane, crates/rav2d/src/headers.rs:628-643 */
+typedef struct Rav2dRestorationPlane {
+    uint8_t restoration_type;
+    uint8_t frame_filters_on;
+    uint8_t num_classes_idx;
+    uint8_t num_classes;
+    int8_t filter[16][18];
+} Rav2dRestorationPlane;
+
+/* The frame-header and sequence-header fields the filter stage reads
+ * (FilterFrameParams decode.rs:3645-3709, LrContext looprestoration.rs:1785-1818,
+ * FhDeblock/FhCdef/FhCcso/FhRestoration/FhGdf headers.rs:589-690). */
+typedef struct Rav2dFrameParams {
+    int32_t width;           /* frame width in luma samples            */
+    int32_t height;          /* frame height in luma samples           */
+    int32_t bitdepth;        /* 8 or 10                                 */
+    int32_t layout;          /* RAV2D_PIXEL_LAYOUT_*; I420 supported    */
+    int32_t sb128;           /* superblock size 128 (1) or 64 (0)       */
+    uint32_t inloop_filters; /* RAV2D_INLOOPFILTER_* mask               */
+    /* sequence header */
+    int32_t seq_cdef;             /* SequenceHeader.cdef                */
+    int32_t cfl_ds_filter_index;  /* 0 box 2x2, 1 vertical avg, 2 copy  */
+    /* quantizer: FhQuant */
+    int32_t yac;
+    int32_t uac_delta;
+    int32_t vac_delta;
+    /* segmentation must be disabled (segmap input not supported) */
+    int32_t seg_enabled;
+    /* FhDeblock */
+    uint8_t db_level_y[2];
+    uint8_t db_level_u;
+    uint8_t db_level_v;
+    int8_t db_delta_q_y[2];
+    int8_t db_delta_q_u;
+    int8_t db_delta_q_v;
+    /* FhCdef */
+    uint8_t cdef_damping;
+    uint8_t cdef_n_strengths;
+    uint8_t cdef_on_skiptx;
+    uint8_t _pad0;
+    uint8_t cdef_y_strength[8];
+    uint8_t cdef_uv_strength[8];
+    /* FhCcso */
+    Rav2dCcsoPlane ccso[3];
+    /* FhRestoration */
+    Rav2dRestorationPlane restoration[3];
+    uint8_t lr_unit_size[2]; /* log2, luma / chroma                     */
+    uint8_t _pad1[2];
+    /* FhGdf + derived index (compute_gdf_ref_dst_idx decode.rs:67-81) */
+    int32_t gdf_enabled;
+    int32_t gdf_qp_idx;
+    int32_t gdf_scale;
+    int32_t gdf_ref_dst_idx;
+    /* film grain: FhFilmGrain.present/seed + FilmGrainData */
+    int32_t grain_present;
+    uint32_t grain_seed;
+    Rav2dFilmGrainData grain;
+} Rav2dFrameParams;
+
+/* Plane set in the PictureAllocation convention: 3 plane pointers, byte strides
+ * (luma, chroma). */
+typedef struct Rav2dPlanes {
+    void *data[3];
+    ptrdiff_t stride[2];
+} Rav2dPlanes;
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct Rav2dGpuCtx Rav2dGpuCtx;
+typedef struct Rav2dGpuFrame Rav2dGpuFrame;
+
+/* Number of HIP devices visible (0 when none or the runtime is absent). */
+int rav2d_gpu_device_count(void);
+
+/* Create a context on HIP device `device`.  hbm_budget_bytes bounds what the
+ * frame pool and the serving admission may hold (0 = no bound). */
+int rav2d_gpu_ctx_create(int device, uint64_t hbm_budget_bytes, uint32_t flags,
+                         Rav2dGpuCtx **out);
+int rav2d_gpu_ctx_destroy(Rav2dGpuCtx *ctx);
+int rav2d_gpu_ctx_synchronize(Rav2dGpuCtx *ctx);
+/* Bytes currently held by frames of this context. */
+uint64_t rav2d_gpu_ctx_bytes_in_use(const Rav2dGpuCtx *ctx);
+
+/* ---- device-resident frames ---------------------------------------------
+ * A frame owns: the reconstructed input planes (F0), the filtered output
+ * planes (in-loop output, F4), optional display planes for the grained copy,
+ * intermediate planes, and device copies of the side info.  This is the
+ * PicAllocator-compatible pool of crates/rav2d/src/picture.rs:38-41, 77-125:
+ * w/h are padded to 128 and strides get +64 B when a multiple of 1024 B. */
+int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params,
+                           Rav2dGpuFrame **out);
+int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f);
+/* Upload side info: n_mask = sb256 rows * sb256 cols Av2Filter records,
+ * n_lr = the same number of Av2Restoration records (lr may be NULL when no
+ * plane uses Wiener restoration). */
+int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask,
+                                  size_t n_mask, const Rav2dAv2Restoration *lr,
+                                  size_t n_lr, void *stream);
+/* Replace the frame parameters (same size, bit depth and layout). */
+int rav2d_gpu_frame_set_params(Rav2dGpuFrame *f, const Rav2dFrameParams *params);
+/* Device planes of the frame: which = 0 input F0, 1 output, 2 display. */
+int rav2d_gpu_frame_planes(const Rav2dGpuFrame *f, int which, Rav2dPlanes *out);
+/* Host <-> device copies of tightly packed planes (stride = width samples). */
+int rav2d_gpu_frame_upload(Rav2dGpuFrame *f, const void *y, const void *u,
+                           const void *v, void *stream);
+int rav2d_gpu_frame_download(Rav2dGpuFrame *f, int which, void *y, void *u, void *v,
+                             void *stream);
+
+/* ---- frame-level entry points --------------------------------------------
+ * rav2d_gpu_filter_frame replaces the PHASE-B loop (decode.rs:3597-3625):
+ * input planes F0 -> output planes, all in-loop filters enabled by
+ * params->inloop_filters.  rav2d_gpu_apply_grain replaces
+ * apply_grain_to_picture_mt (decode.rs:4672): output planes -> display planes;
+ * a no-op copy when the frame carries no grain (picture_has_grain). */
+int rav2d_gpu_filter_frame(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream);
+int rav2d_gpu_apply_grain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream);
+
+/* ---- per-pass entry points (unit parity and per-kernel profiling) -------- */
+#define RAV2D_PASS_DEBLOCK_V 0
+#define RAV2D_PASS_DEBLOCK_H 1
+#define RAV2D_PASS_CDEF 2
+#define RAV2D_PASS_LR 3
+#define RAV2D_PASS_GRAIN 4
+int rav2d_gpu_run_pass(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int pass, void *stream);
+
+/* ---- timing with HIP events on the context stream ----------------------- */
+/* When profiling is on, every pass launched through this context on its own
+ * stream is bracketed by HIP events; rav2d_gpu_pass_times returns, per pass
+ * (RAV2D_PASS_*), the summed milliseconds and the launch count since the last
+ * reset (synchronizes). */
+#define RAV2D_NUM_PASSES 5
+int rav2d_gpu_set_profiling(Rav2dGpuCtx *ctx, int on);
+int rav2d_gpu_pass_times(Rav2dGpuCtx *ctx, double *ms, int64_t *count, int reset);
+int rav2d_gpu_timer_start(Rav2dGpuCtx *ctx);
+/* Milliseconds between the last timer_start and now (synchronizes). */
+int rav2d_gpu_timer_stop(Rav2dGpuCtx *ctx, float *ms);
+
+/* sizeof() of the ABI structs, so bindings can check their mirrors:
+ * 0 Rav2dAv2Filter, 1 Rav2dAv2Restoration, 2 Rav2dFrameParams,
+ * 3 Rav2dFilmGrainData, 4 Rav2dPlanes; 0 for an unknown index. */
+size_t rav2d_gpu_struct_size(int which);
+
+const char *rav2d_gpu_version(void);
+/* Text of the last error recorded on this thread. */
+const char *rav2d_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAV2D_GPU_H */

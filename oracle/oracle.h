@@ -1,0 +1,29 @@
+/*
+ * C entry points of the CPU oracle (TEST INFRASTRUCTURE ONLY: loaded by the
+ * tests, the smoke check and the benchmark's CPU-baseline leg).
+ */
+#pragma once
+#include "../include/rav2d_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Extra bit of `inloop_mask`: stop deblocking after the vertical-edge pass. */
+#define ORACLE_DEBLOCK_COLS_ONLY (1u << 8)
+
+/* Run the in-loop filter chain in place on host planes (strides in bytes),
+ * in rav2d's per-superblock-row order.  `inloop_mask` is ANDed with
+ * params->inloop_filters (use RAV2D_INLOOPFILTER_ALL for the full chain). */
+int oracle_filter_frame(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask, size_t n_mask,
+                        const Rav2dAv2Restoration *lr, size_t n_lr, void *y, void *u, void *v,
+                        ptrdiff_t y_stride, ptrdiff_t uv_stride, uint32_t inloop_mask);
+
+/* Known-answer hooks into the restated helpers. */
+int oracle_dq_lookup(int qidx);
+uint32_t oracle_deblock_quant_thr(int hbd, int qidx);
+uint32_t oracle_deblock_side_thr(int hbd, int qidx);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,323 @@
+"""rav2d_amd: MI355X (gfx950) post-reconstruction filter stage of the rav2d AV2 decoder.
+
+Python bindings over the C ABI in ``include/rav2d_gpu.h`` (``librav2d_gpu.so``)
+and over the seeded workload generator (``librav2d_synth.so``).  The compute
+path is the HIP library; nothing here filters pixels on the CPU.  Loading the
+GPU library fails loudly if it was not built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+RAV2D_OK = 0
+INLOOP_DEBLOCK, INLOOP_CDEF, INLOOP_CCSO, INLOOP_WIENER, INLOOP_GDF = 1, 2, 4, 8, 16
+INLOOP_ALL = 31
+PASS_DEBLOCK_V, PASS_DEBLOCK_H, PASS_CDEF, PASS_LR, PASS_GRAIN = range(5)
+
+TOOL_DEBLOCK, TOOL_CDEF, TOOL_CCSO, TOOL_WIENER, TOOL_GDF, TOOL_GRAIN = 1, 2, 4, 8, 16, 32
+TOOL_ALL = 63
+
+
+class FilmGrainData(C.Structure):
+    _fields_ = [
+        ("chroma_scaling_from_luma", C.c_int32),
+        ("num_points", C.c_int32 * 3),
+        ("points", (C.c_uint8 * 2 * 14) * 3),
+        ("scaling_shift", C.c_int32),
+        ("ar_coeff_lag", C.c_int32),
+        ("ar_coeffs", (C.c_int8 * 28) * 3),
+        ("ar_coeff_shift", C.c_uint64),
+        ("grain_scale_shift", C.c_int32),
+        ("uv_mult", C.c_int32 * 2),
+        ("uv_luma_mult", C.c_int32 * 2),
+        ("uv_offset", C.c_int32 * 2),
+        ("overlap_flag", C.c_int32),
+        ("clip_to_restricted_range", C.c_int32),
+        ("mc_identity", C.c_int32),
+        ("block_size", C.c_int32),
+    ]
+
+
+class CcsoPlane(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_uint8), ("bo_only", C.c_uint8), ("scale_idx", C.c_uint8),
+        ("quant_idx", C.c_uint8), ("ext_filter_support", C.c_uint8), ("edge_clf", C.c_uint8),
+        ("max_band_log2", C.c_uint8), ("_pad", C.c_uint8), ("filter_off", C.c_uint8 * 64),
+    ]
+
+
+class RestorationPlane(C.Structure):
+    _fields_ = [
+        ("restoration_type", C.c_uint8), ("frame_filters_on", C.c_uint8),
+        ("num_classes_idx", C.c_uint8), ("num_classes", C.c_uint8),
+        ("filter", (C.c_int8 * 18) * 16),
+    ]
+
+
+class FrameParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("bitdepth", C.c_int32),
+        ("layout", C.c_int32), ("sb128", C.c_int32), ("inloop_filters", C.c_uint32),
+        ("seq_cdef", C.c_int32), ("cfl_ds_filter_index", C.c_int32),
+        ("yac", C.c_int32), ("uac_delta", C.c_int32), ("vac_delta", C.c_int32),
+        ("seg_enabled", C.c_int32),
+        ("db_level_y", C.c_uint8 * 2), ("db_level_u", C.c_uint8), ("db_level_v", C.c_uint8),
+        ("db_delta_q_y", C.c_int8 * 2), ("db_delta_q_u", C.c_int8), ("db_delta_q_v", C.c_int8),
+        ("cdef_damping", C.c_uint8), ("cdef_n_strengths", C.c_uint8), ("cdef_on_skiptx", C.c_uint8),
+        ("_pad0", C.c_uint8),
+        ("cdef_y_strength", C.c_uint8 * 8), ("cdef_uv_strength", C.c_uint8 * 8),
+        ("ccso", CcsoPlane * 3),
+        ("restoration", RestorationPlane * 3),
+        ("lr_unit_size", C.c_uint8 * 2), ("_pad1", C.c_uint8 * 2),
+        ("gdf_enabled", C.c_int32), ("gdf_qp_idx", C.c_int32), ("gdf_scale", C.c_int32),
+        ("gdf_ref_dst_idx", C.c_int32),
+        ("grain_present", C.c_int32), ("grain_seed", C.c_uint32),
+        ("grain", FilmGrainData),
+    ]
+
+
+class Planes(C.Structure):
+    _fields_ = [("data", C.c_void_p * 3), ("stride", C.c_ssize_t * 2)]
+
+
+class SynthConfig(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("bitdepth", C.c_int32), ("sb128", C.c_int32),
+        ("seed", C.c_uint64), ("tools", C.c_uint32), ("lossless_pct", C.c_int32),
+        ("skip_pct", C.c_int32), ("pc_gdf_pct", C.c_int32),
+    ]
+
+
+AV2FILTER_SIZE = 12100
+AV2RESTORATION_SIZE = 24624
+
+_gpu = None
+_synth = None
+
+
+def _declare(lib, name, restype, argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+    return f
+
+
+def gpu_lib_path():
+    return os.path.join(HERE, "librav2d_gpu.so")
+
+
+def load_gpu():
+    """Load the HIP library; raises if it was not built (no CPU fallback exists)."""
+    global _gpu
+    if _gpu is not None:
+        return _gpu
+    path = gpu_lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError("librav2d_gpu.so not built: run `python -m rav2d_amd.build`")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    _declare(lib, "rav2d_gpu_version", C.c_char_p, [])
+    _declare(lib, "rav2d_gpu_last_error", C.c_char_p, [])
+    _declare(lib, "rav2d_gpu_ctx_create", C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(P)])
+    _declare(lib, "rav2d_gpu_ctx_destroy", C.c_int, [P])
+    _declare(lib, "rav2d_gpu_ctx_synchronize", C.c_int, [P])
+    _declare(lib, "rav2d_gpu_ctx_bytes_in_use", C.c_uint64, [P])
+    _declare(lib, "rav2d_gpu_frame_create", C.c_int, [P, C.POINTER(FrameParams), C.POINTER(P)])
+    _declare(lib, "rav2d_gpu_frame_destroy", C.c_int, [P])
+    _declare(lib, "rav2d_gpu_frame_set_side_info", C.c_int, [P, P, C.c_size_t, P, C.c_size_t, P])
+    _declare(lib, "rav2d_gpu_frame_set_params", C.c_int, [P, C.POINTER(FrameParams)])
+    _declare(lib, "rav2d_gpu_frame_planes", C.c_int, [P, C.c_int, C.POINTER(Planes)])
+    _declare(lib, "rav2d_gpu_frame_upload", C.c_int, [P, P, P, P, P])
+    _declare(lib, "rav2d_gpu_frame_download", C.c_int, [P, C.c_int, P, P, P, P])
+    _declare(lib, "rav2d_gpu_filter_frame", C.c_int, [P, P, P])
+    _declare(lib, "rav2d_gpu_apply_grain", C.c_int, [P, P, P])
+    _declare(lib, "rav2d_gpu_run_pass", C.c_int, [P, P, C.c_int, P])
+    _declare(lib, "rav2d_gpu_timer_start", C.c_int, [P])
+    _declare(lib, "rav2d_gpu_timer_stop", C.c_int, [P, C.POINTER(C.c_float)])
+    _declare(lib, "rav2d_gpu_struct_size", C.c_size_t, [C.c_int])
+    _declare(lib, "rav2d_gpu_device_count", C.c_int, [])
+    _declare(lib, "rav2d_gpu_set_profiling", C.c_int, [P, C.c_int])
+    _declare(lib, "rav2d_gpu_pass_times", C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int])
+    _gpu = lib
+    return lib
+
+
+def load_synth():
+    global _synth
+    if _synth is not None:
+        return _synth
+    path = os.path.join(HERE, "librav2d_synth.so")
+    if not os.path.exists(path):
+        raise RuntimeError("librav2d_synth.so not built: run `python -m rav2d_amd.build`")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    _declare(lib, "r2s_frame_create", P, [C.POINTER(SynthConfig)])
+    _declare(lib, "r2s_frame_destroy", None, [P])
+    _declare(lib, "r2s_params", C.POINTER(FrameParams), [P])
+    _declare(lib, "r2s_plane", P, [P, C.c_int])
+    _declare(lib, "r2s_plane_width", C.c_int32, [P, C.c_int])
+    _declare(lib, "r2s_plane_height", C.c_int32, [P, C.c_int])
+    _declare(lib, "r2s_plane_bytes", C.c_size_t, [P, C.c_int])
+    _declare(lib, "r2s_mask", P, [P, C.POINTER(C.c_size_t)])
+    _declare(lib, "r2s_lr", P, [P, C.POINTER(C.c_size_t)])
+    _synth = lib
+    return lib
+
+
+def device_count():
+    """HIP devices visible to the library (0 without a GPU)."""
+    return load_gpu().rav2d_gpu_device_count()
+
+
+class Rav2dError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != RAV2D_OK:
+        msg = load_gpu().rav2d_gpu_last_error()
+        raise Rav2dError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+class SynthFrame:
+    """A seeded synthetic reconstructed frame plus its filter side info."""
+
+    def __init__(self, width, height, bitdepth=10, sb128=0, seed=1, tools=TOOL_ALL,
+                 lossless_pct=0, skip_pct=30, pc_gdf_pct=50):
+        lib = load_synth()
+        cfg = SynthConfig(width, height, bitdepth, sb128, seed, tools, lossless_pct, skip_pct, pc_gdf_pct)
+        self._lib = lib
+        self._h = lib.r2s_frame_create(C.byref(cfg))
+        if not self._h:
+            raise ValueError("invalid synthetic frame config")
+        self.width, self.height, self.bitdepth = width, height, bitdepth
+        self.params = lib.r2s_params(self._h).contents
+        n = C.c_size_t()
+        self.mask_ptr = lib.r2s_mask(self._h, C.byref(n))
+        self.n_mask = n.value
+        self.lr_ptr = lib.r2s_lr(self._h, C.byref(n))
+        self.n_lr = n.value
+
+    def plane_ptr(self, pl):
+        return self._lib.r2s_plane(self._h, pl)
+
+    def plane_shape(self, pl):
+        return (self._lib.r2s_plane_height(self._h, pl), self._lib.r2s_plane_width(self._h, pl))
+
+    @property
+    def dtype(self):
+        return np.uint16 if self.bitdepth > 8 else np.uint8
+
+    def planes(self):
+        """Copies of the three input planes as numpy arrays."""
+        out = []
+        for pl in range(3):
+            h, w = self.plane_shape(pl)
+            n = h * w * np.dtype(self.dtype).itemsize
+            buf = (C.c_uint8 * n).from_address(self.plane_ptr(pl))
+            out.append(np.frombuffer(buf, dtype=self.dtype).reshape(h, w).copy())
+        return out
+
+    def mask_bytes(self):
+        return (C.c_uint8 * (self.n_mask * AV2FILTER_SIZE)).from_address(self.mask_ptr)
+
+    def close(self):
+        if self._h:
+            self._lib.r2s_frame_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuContext:
+    def __init__(self, device=0, hbm_budget=0):
+        self.lib = load_gpu()
+        h = C.c_void_p()
+        _check(self.lib.rav2d_gpu_ctx_create(device, hbm_budget, 0, C.byref(h)), "ctx_create")
+        self.h = h
+
+    def synchronize(self):
+        _check(self.lib.rav2d_gpu_ctx_synchronize(self.h), "synchronize")
+
+    def bytes_in_use(self):
+        return self.lib.rav2d_gpu_ctx_bytes_in_use(self.h)
+
+    def set_profiling(self, on):
+        _check(self.lib.rav2d_gpu_set_profiling(self.h, 1 if on else 0), "set_profiling")
+
+    def pass_times(self, reset=True):
+        """{pass index: (total ms, launches)} since the last reset."""
+        ms = (C.c_double * 5)()
+        cnt = (C.c_int64 * 5)()
+        _check(self.lib.rav2d_gpu_pass_times(self.h, ms, cnt, 1 if reset else 0), "pass_times")
+        return {i: (ms[i], cnt[i]) for i in range(5)}
+
+    def timer_start(self):
+        _check(self.lib.rav2d_gpu_timer_start(self.h), "timer_start")
+
+    def timer_stop(self):
+        ms = C.c_float()
+        _check(self.lib.rav2d_gpu_timer_stop(self.h, C.byref(ms)), "timer_stop")
+        return ms.value
+
+    def close(self):
+        if self.h:
+            self.lib.rav2d_gpu_ctx_destroy(self.h)
+            self.h = None
+
+
+class GpuFrame:
+    """A device-resident frame: input planes, output planes, display planes, side info."""
+
+    def __init__(self, ctx, params):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = C.c_void_p()
+        _check(self.lib.rav2d_gpu_frame_create(ctx.h, C.byref(params), C.byref(h)), "frame_create")
+        self.h = h
+        self.width, self.height, self.bitdepth = params.width, params.height, params.bitdepth
+
+    @classmethod
+    def from_synth(cls, ctx, sf):
+        f = cls(ctx, sf.params)
+        f.set_side_info(sf.mask_ptr, sf.n_mask, sf.lr_ptr, sf.n_lr)
+        _check(f.lib.rav2d_gpu_frame_upload(f.h, sf.plane_ptr(0), sf.plane_ptr(1), sf.plane_ptr(2), None),
+               "upload")
+        return f
+
+    def set_side_info(self, mask_ptr, n_mask, lr_ptr, n_lr):
+        _check(self.lib.rav2d_gpu_frame_set_side_info(self.h, mask_ptr, n_mask, lr_ptr, n_lr, None),
+               "set_side_info")
+
+    def set_params(self, params):
+        _check(self.lib.rav2d_gpu_frame_set_params(self.h, C.byref(params)), "set_params")
+
+    def filter(self):
+        _check(self.lib.rav2d_gpu_filter_frame(self.ctx.h, self.h, None), "filter_frame")
+
+    def apply_grain(self):
+        _check(self.lib.rav2d_gpu_apply_grain(self.ctx.h, self.h, None), "apply_grain")
+
+    def run_pass(self, upto):
+        _check(self.lib.rav2d_gpu_run_pass(self.ctx.h, self.h, upto, None), "run_pass")
+
+    def download(self, which=1):
+        dt = np.uint16 if self.bitdepth > 8 else np.uint8
+        shapes = [(self.height, self.width), (self.height // 2, self.width // 2),
+                  (self.height // 2, self.width // 2)]
+        arrs = [np.empty(s, dtype=dt) for s in shapes]
+        _check(self.lib.rav2d_gpu_frame_download(self.h, which, arrs[0].ctypes.data, arrs[1].ctypes.data,
+                                                 arrs[2].ctypes.data, None), "download")
+        return arrs
+
+    def close(self):
+        if self.h:
+            self.lib.rav2d_gpu_frame_destroy(self.h)
+            self.h = None

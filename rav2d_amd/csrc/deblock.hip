@@ -1,0 +1,290 @@
+// AV2 deblocking on gfx950: vertical-edge pass, then horizontal-edge pass.
+//
+// Arithmetic follows crates/rav2d/src/deblock.rs: filter_choice_bd (:81-186),
+// deblock_bd (:217-283), the per-column/row walkers deblock_{h,v}_sb64{y,uv}_bd
+// (:303-551) and the threshold set-up setup_thr_cols/rows + edge_thr
+// (:1466-1567) driven by deblock64_cols/rows (:1623-2140).
+//
+// Layout: one workgroup filters a 64x32 (V) or 64x32 (H) tile of one plane.
+// The tile plus a 16-sample halo across the edge direction is staged in LDS
+// as 16-bit samples, every edge whose footprint reaches the tile is filtered
+// in LDS by one lane per (edge, 4-line segment), and the tile interior is
+// written to the output plane.  Edges of one pass are filtered concurrently:
+// an edge of level L (transform >= 4<<L on both sides) reads at most 8
+// samples on each side and its writes never reach a sample another edge of
+// the same pass reads, so the result equals the reference's sequential walk.
+#include "internal.h"
+
+namespace r2g {
+
+__constant__ int8_t kMaxWidthY[4] = {1, 3, 6, 8};
+__constant__ int8_t kMaxWidthUV[3] = {1, 3, 4};
+__constant__ int8_t kQFirst[3] = {45, 40, 32};
+__constant__ int8_t kQThreshMults[8] = {32, 25, 19, 19, 0, 18, 0, 17};
+__constant__ int8_t kWMult[8] = {85, 51, 37, 28, 0, 20, 0, 15};
+
+// filter_choice_bd: `s` = line 0, `t` = line 3 of the segment, `st` steps
+// across the edge (positive side = +st).
+__device__ __forceinline__ int filter_choice(const uint16_t *s, const uint16_t *t, int st, int mwn, int mwp,
+                                             uint32_t q_thr, uint32_t side_thr) {
+    auto A = [&](const uint16_t *b, int k) -> int { return (int)b[k * st]; };
+    uint32_t sd[4];
+#pragma unroll
+    for (int d = -2; d < 2; d++) {
+        uint32_t ds = (uint32_t)abs(A(s, d - 1) - 2 * A(s, d) + A(s, d + 1));
+        uint32_t dt = (uint32_t)abs(A(t, d - 1) - 2 * A(t, d) + A(t, d + 1));
+        sd[d + 2] = (ds + dt + 1) >> 1;
+    }
+    uint32_t high = dmax(sd[0], sd[3]);
+    if (high > side_thr) return 0;
+    if (mwp == 1) return 1;
+    uint32_t transition = sd[1] + sd[2];
+    if (high > (side_thr >> 2)) return 1;
+    if (transition > q_thr * 4) return 1;
+    if (high > (side_thr >> 3)) return 2;
+    if (transition > q_thr * 3) return 2;
+    uint32_t end_thr = (side_thr * 3) >> 4;
+    if (mwn >= 3) {
+        uint32_t ds = (uint32_t)abs(A(s, -1) - A(s, -4) - 3 * (A(s, -1) - A(s, -2)));
+        uint32_t dt = (uint32_t)abs(A(t, -1) - A(t, -4) - 3 * (A(t, -1) - A(t, -2)));
+        if (((ds + dt + 1) >> 1) > end_thr) return 2;
+    }
+    {
+        uint32_t ds = (uint32_t)abs(A(s, 0) - A(s, 3) - 3 * (A(s, 0) - A(s, 1)));
+        uint32_t dt = (uint32_t)abs(A(t, 0) - A(t, 3) - 3 * (A(t, 0) - A(t, 1)));
+        if (((ds + dt + 1) >> 1) > end_thr) return 2;
+    }
+    if (mwp == 3) return 3;
+    transition <<= 4;
+    int prev = 3;
+    for (int dist = 4; dist <= mwp; dist += 2) {
+        uint32_t q4 = q_thr * (uint32_t)kQFirst[(dist - 4) >> 1];
+        uint32_t e4 = (side_thr * (uint32_t)dist) >> 4;
+        if (transition > q4) return prev;
+        int d2 = dmin(7, dist);
+        if (mwn >= d2) {
+            uint32_t ds = (uint32_t)abs(A(s, -1) - A(s, -d2 - 1) - d2 * (A(s, -1) - A(s, -2)));
+            uint32_t dt = (uint32_t)abs(A(t, -1) - A(t, -d2 - 1) - d2 * (A(t, -1) - A(t, -2)));
+            if (((ds + dt + 1) >> 1) > e4) return prev;
+        }
+        uint32_t ds = (uint32_t)abs(A(s, 0) - A(s, d2) - d2 * (A(s, 0) - A(s, 1)));
+        uint32_t dt = (uint32_t)abs(A(t, 0) - A(t, d2) - d2 * (A(t, 0) - A(t, 1)));
+        if (((ds + dt + 1) >> 1) > e4) return prev;
+        prev = dist;
+    }
+    return mwp;
+}
+
+// deblock_bd on an LDS segment: `p` = first sample on the positive side of
+// line 0, `sa` steps along the edge, `sb` across it.
+__device__ __forceinline__ void filter_edge(uint16_t *p, int sa, int sb, int bdmax, uint32_t q_thr,
+                                            uint32_t side_thr, int mwp, int mwn, bool pos_ll, bool neg_ll) {
+    int width = filter_choice(p, p + 3 * sa, sb, mwn, mwp, q_thr, side_thr);
+    int wn = dmin(width, mwn), wp = width;
+    if (wp < 1) return;
+    int clampv = (int)q_thr * kQThreshMults[width - 1];
+    int mneg = kWMult[wn - 1], mpos = kWMult[wp - 1];
+    for (int i = 0; i < 4; i++) {
+        uint16_t *d = p + i * sa;
+        int d0 = d[0], dm1 = d[-sb], dp1 = d[sb], dm2 = d[-2 * sb];
+        int delta = iclip(4 * (3 * (d0 - dm1) - (dp1 - dm2)), -clampv, clampv);
+        if (!neg_ll) {
+            int dn = delta * mneg;
+            for (int j = 0; j < wn; j++) {
+                int v = d[(-j - 1) * sb] + ((dn * (wn - j) + 1024) >> 11);
+                d[(-j - 1) * sb] = (uint16_t)iclip(v, 0, bdmax);
+            }
+        }
+        if (!pos_ll) {
+            int dp = delta * mpos;
+            for (int j = 0; j < wp; j++) {
+                int v = d[j * sb] - ((dp * (wp - j) + 1024) >> 11);
+                d[j * sb] = (uint16_t)iclip(v, 0, bdmax);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ int edge_thr(int cur, int prev) {
+    return (cur && prev) ? (cur + prev + 1) >> 1 : (cur | prev);
+}
+
+__device__ __forceinline__ const Rav2dAv2Filter &sb_at(const Rav2dAv2Filter *m, int sb256w, int lx4, int ly4) {
+    return m[(ly4 >> 6) * sb256w + (lx4 >> 6)];
+}
+
+// qidx of the 64x64 luma block holding luma 4px unit (lx4, ly4).
+__device__ __forceinline__ int qidx_at(const Rav2dAv2Filter *m, int sb256w, int lx4, int ly4) {
+    const Rav2dAv2Filter &f = sb_at(m, sb256w, lx4, ly4);
+    return dmin((int)f.qidx[((ly4 >> 4) & 3) * 4 + ((lx4 >> 4) & 3)], 511);
+}
+
+struct EdgeDesc {
+    bool on;
+    int mwp, mwn;
+    uint32_t q, side;
+    bool pos_ll, neg_ll;
+};
+
+// Edge of the given pass (vert = vertical edges, cols pass) at the plane's 4px
+// unit (x4, y4); pl selects luma or chroma indexing.
+__device__ EdgeDesc edge_desc(const DeblockArgs &a, int pl, bool vert, int x4, int y4) {
+    EdgeDesc e;
+    e.on = false;
+    const int sbw = a.g.sb256w;
+    const bool ch = pl != 0;
+    // luma 4px units of this sample (4:2:0 chroma)
+    const int lx4 = ch ? x4 * 2 : x4, ly4 = ch ? y4 * 2 : y4;
+    const Rav2dAv2Filter &f = sb_at(a.mask, sbw, lx4, ly4);
+    const uint16_t(*fm)[5][4] = ch ? f.filter_uv[vert ? 0 : 1] : f.filter_y[vert ? 0 : 1];
+    int along = vert ? x4 : y4;   // index of the edge line within the SB256
+    int across = vert ? y4 : x4;  // position along the edge
+    int line = ch ? (along & 31) : (along & 63);
+    int sidx = ch ? ((across >> 4) & 1) : ((across >> 4) & 3);
+    uint32_t bit = 1u << (across & 15);
+    int idx;
+    if (ch) {
+        uint32_t m0 = fm[line][0][sidx], m1 = fm[line][1][sidx], m2 = fm[line][2][sidx];
+        if (!((m0 | m1 | m2) & bit)) return e;
+        idx = (m2 & bit) ? 2 : ((m1 & bit) ? 1 : 0);
+        e.mwp = kMaxWidthUV[idx];
+    } else {
+        uint32_t m0 = fm[line][0][sidx], m1 = fm[line][1][sidx], m2 = fm[line][2][sidx], m3 = fm[line][3][sidx];
+        if (!((m0 | m1 | m2 | m3) & bit)) return e;
+        idx = (m3 & bit) ? 3 : (m2 & bit) ? 2 : ((m1 & bit) ? 1 : 0);
+        e.mwp = kMaxWidthY[idx];
+    }
+    int subpu = 3 * ((fm[line][4][sidx] & bit) ? 1 : 0);
+    // the rows pass clamps the negative side at the top of every 64-luma-row band
+    bool band_top = !vert && ((ch ? (y4 & 7) : (y4 & 15)) == 0);
+    e.mwn = band_top ? dmin(ch ? 2 : 6, e.mwp) : e.mwp;
+    // thresholds: this block's and the neighbour's (left or above) LUT values
+    int kind = ch ? 1 + pl : (vert ? 0 : 1);
+    int nx4 = vert ? x4 - 1 : x4, ny4 = vert ? y4 : y4 - 1;
+    int lnx4 = ch ? nx4 * 2 : nx4, lny4 = ch ? ny4 * 2 : ny4;
+    int qc = qidx_at(a.mask, sbw, lx4, ly4), qp = qidx_at(a.mask, sbw, lnx4, lny4);
+    e.q = (uint32_t)(edge_thr(a.lut->q[kind][qc], a.lut->q[kind][qp]) >> subpu);
+    e.side = (uint32_t)(edge_thr(a.lut->side[kind][qc], a.lut->side[kind][qp]) >> subpu);
+    // lossless sides
+    const Rav2dAv2Filter &fn = sb_at(a.mask, sbw, lnx4, lny4);
+    if (ch) {
+        e.pos_ll = (f.lossless_mask_uv[y4 & 31][(x4 >> 3) & 3] >> (x4 & 7)) & 1;
+        e.neg_ll = (fn.lossless_mask_uv[ny4 & 31][(nx4 >> 3) & 3] >> (nx4 & 7)) & 1;
+    } else {
+        e.pos_ll = (f.lossless_mask_y[y4 & 63][(x4 >> 4) & 3] >> (x4 & 15)) & 1;
+        e.neg_ll = (fn.lossless_mask_y[ny4 & 63][(nx4 >> 4) & 3] >> (nx4 & 15)) & 1;
+    }
+    e.on = true;
+    return e;
+}
+
+constexpr int kTW = 64, kTH = 32, kHalo = 16, kThreads = 256;
+
+template <typename P>
+__global__ __launch_bounds__(kThreads) void deblock_v_kernel(DeblockArgs a) {
+    __shared__ uint16_t tile[kTH][kTW + 2 * kHalo];
+    int pl, tx, ty;
+    decode_tile(a.map, blockIdx.x, pl, tx, ty);
+    const DevPlane &src = a.src[pl];
+    const DevPlane &dst = a.dst[pl];
+    const int x0 = tx * kTW, y0 = ty * kTH;
+    const int W = src.w, H = src.h;
+    constexpr int LW = kTW + 2 * kHalo;
+    for (int i = threadIdx.x; i < kTH * LW; i += kThreads) {
+        int r = i / LW, c = i - r * LW;
+        int gx = x0 - kHalo + c, gy = y0 + r;
+        uint16_t v = 0;
+        if (gy < H && gx >= 0 && gx < W) v = row_ptr<const P>(src, gy)[gx];
+        tile[r][c] = v;
+    }
+    __syncthreads();
+    if (a.plane_on[pl]) {
+        constexpr int NE = kTW / 4 + 3;  // edge columns x0-4 .. x0+TW+4
+        for (int i = threadIdx.x; i < NE * (kTH / 4); i += kThreads) {
+            int rs = i / NE, ec = i - rs * NE;
+            int ex = x0 - 4 + ec * 4, ey = y0 + rs * 4;
+            if (ex <= 0 || ex >= W || ey >= H) continue;  // no filtering at the frame's left edge
+            EdgeDesc e = edge_desc(a, pl, true, ex >> 2, ey >> 2);
+            if (!e.on) continue;
+            filter_edge(&tile[rs * 4][ex - x0 + kHalo], LW, 1, a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll,
+                        e.neg_ll);
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < kTH * kTW; i += kThreads) {
+        int r = i / kTW, c = i - r * kTW;
+        int gx = x0 + c, gy = y0 + r;
+        if (gy < H && gx < W) row_ptr<P>(dst, gy)[gx] = (P)tile[r][c + kHalo];
+    }
+}
+
+template <typename P>
+__global__ __launch_bounds__(kThreads) void deblock_h_kernel(DeblockArgs a) {
+    __shared__ uint16_t tile[kTH + 2 * kHalo][kTW];
+    int pl, tx, ty;
+    decode_tile(a.map, blockIdx.x, pl, tx, ty);
+    const DevPlane &src = a.src[pl];
+    const DevPlane &dst = a.dst[pl];
+    const int x0 = tx * kTW, y0 = ty * kTH;
+    const int W = src.w, H = src.h;
+    constexpr int LH = kTH + 2 * kHalo;
+    for (int i = threadIdx.x; i < LH * kTW; i += kThreads) {
+        int r = i / kTW, c = i - r * kTW;
+        int gx = x0 + c, gy = y0 - kHalo + r;
+        uint16_t v = 0;
+        if (gy >= 0 && gy < H && gx < W) v = row_ptr<const P>(src, gy)[gx];
+        tile[r][c] = v;
+    }
+    __syncthreads();
+    if (a.plane_on[pl]) {
+        constexpr int NE = kTH / 4 + 3;  // edge rows y0-4 .. y0+TH+4
+        for (int i = threadIdx.x; i < NE * (kTW / 4); i += kThreads) {
+            int er = i / (kTW / 4), cs = i - er * (kTW / 4);
+            int ey = y0 - 4 + er * 4, ex = x0 + cs * 4;
+            if (ey <= 0 || ey >= H || ex >= W) continue;  // no filtering at the frame's top edge
+            EdgeDesc e = edge_desc(a, pl, false, ex >> 2, ey >> 2);
+            if (!e.on) continue;
+            filter_edge(&tile[ey - y0 + kHalo][cs * 4], 1, kTW, a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll,
+                        e.neg_ll);
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < kTH * kTW; i += kThreads) {
+        int r = i / kTW, c = i - r * kTW;
+        int gx = x0 + c, gy = y0 + r;
+        if (gy < H && gx < W) row_ptr<P>(dst, gy)[gx] = (P)tile[r + kHalo][c];
+    }
+}
+
+static TileMap make_map(const DeblockArgs &a) {
+    TileMap m;
+    m.first[0] = 0;
+    for (int p = 0; p < 3; p++) {
+        m.tiles_x[p] = (a.src[p].w + kTW - 1) / kTW;
+        m.tiles_y[p] = (a.src[p].h + kTH - 1) / kTH;
+        m.first[p + 1] = m.first[p] + m.tiles_x[p] * m.tiles_y[p];
+    }
+    return m;
+}
+
+void launch_deblock_v(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
+    DeblockArgs a = a0;
+    a.map = make_map(a);
+    if (a.map.first[3] == 0) return;
+    if (bitdepth > 8)
+        hipLaunchKernelGGL(deblock_v_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(deblock_v_kernel<uint8_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+}
+
+void launch_deblock_h(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
+    DeblockArgs a = a0;
+    a.map = make_map(a);
+    if (a.map.first[3] == 0) return;
+    if (bitdepth > 8)
+        hipLaunchKernelGGL(deblock_h_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(deblock_h_kernel<uint8_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+}
+
+}  // namespace r2g

@@ -1,0 +1,592 @@
+// Host runtime and C ABI of the gfx950 filter stage (include/rav2d_gpu.h).
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "tables.h"
+
+namespace r2g {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            set_last_error(std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+            return RAV2D_EDEVICE;                                                              \
+        }                                                                                      \
+    } while (0)
+
+}  // namespace r2g
+
+using namespace r2g;
+
+struct Rav2dGpuCtx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t budget = 0;
+    uint64_t in_use = 0;
+    std::mutex mu;
+    // per-pass profiling: event pairs recorded around launches, resolved lazily
+    bool profiling = false;
+    struct Rec {
+        int pass;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> free_events;
+    double pass_ms[RAV2D_NUM_PASSES] = {0};
+    int64_t pass_count[RAV2D_NUM_PASSES] = {0};
+};
+
+namespace {
+
+hipEvent_t take_event(Rav2dGpuCtx *c) {
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one pass launch with events when profiling is on.
+struct PassTimer {
+    Rav2dGpuCtx *c;
+    int pass;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    PassTimer(Rav2dGpuCtx *c_, int pass_, hipStream_t s_) : c(c_), pass(pass_), s(s_) {
+        if (c->profiling && s == c->stream) {
+            a = take_event(c);
+            if (a) (void)hipEventRecord(a, s);
+        }
+    }
+    ~PassTimer() {
+        if (!a) return;
+        hipEvent_t b = take_event(c);
+        if (!b) return;
+        (void)hipEventRecord(b, s);
+        c->recs.push_back({pass, a, b});
+    }
+};
+
+}  // namespace
+
+// Buffer roles of a frame.  F0 is never written by a pass.
+enum { BUF_IN = 0, BUF_OUT = 1, BUF_DISP = 2, BUF_T1 = 3, BUF_T2 = 4, NBUF = 5 };
+
+struct Rav2dGpuFrame {
+    Rav2dGpuCtx *ctx = nullptr;
+    Rav2dFrameParams params;
+    DevGeom g;
+    int pw[3], ph[3], bps;
+    int64_t stride[3];            // bytes, per plane
+    size_t plane_bytes[3];
+    uint8_t *block = nullptr;     // one allocation: all plane buffers
+    uint8_t *buf[NBUF][3];
+    Rav2dAv2Filter *d_mask = nullptr;
+    size_t n_mask = 0;
+    Rav2dAv2Restoration *d_lr = nullptr;
+    size_t n_lr = 0;
+    DbLut *d_dblut = nullptr;
+    size_t bytes = 0;
+    bool side_ok = false;
+};
+
+namespace {
+
+uint32_t quant_thr(int hbd, int qidx) {
+    int qmax = 255 + 48 * hbd;
+    qidx = std::min(std::max(qidx, 0), qmax);
+    int q = std::min(std::max(qidx, 0), 255);
+    int dq = q == 0 ? 64 : ((int)R2T_DQ_LOOKUP[(q - 1) % 24] << ((q - 1) / 24));
+    return (uint32_t)((dq + 4) >> 9);
+}
+
+uint32_t side_thr(int hbd, int qidx) {
+    int bdm8 = 2 * hbd;
+    int q_ind = std::min(std::max(qidx - 24 * bdm8, 0), 295);
+    int s = R2T_DEBLOCK_SIDE_THRESHOLDS[q_ind] + ((1 << 4) >> bdm8);
+    return (uint32_t)std::max(s, 0) >> (5 - bdm8);
+}
+
+// init_deblock_thr_lut_y/uv (deblock.rs:13-57), segmentation disabled.
+void build_db_lut(const Rav2dFrameParams &p, int hbd, DbLut &lut) {
+    for (int q = 0; q < 512; q++) {
+        for (int dir = 0; dir < 2; dir++) {
+            int v = q + 8 * p.db_delta_q_y[dir];
+            lut.q[dir][q] = (uint16_t)quant_thr(hbd, v);
+            lut.side[dir][q] = (uint16_t)side_thr(hbd, v);
+        }
+        int u = q + p.uac_delta + 8 * p.db_delta_q_u;
+        int v = q + p.vac_delta + 8 * p.db_delta_q_v;
+        lut.q[2][q] = (uint16_t)quant_thr(hbd, u);
+        lut.side[2][q] = (uint16_t)side_thr(hbd, u);
+        lut.q[3][q] = (uint16_t)quant_thr(hbd, v);
+        lut.side[3][q] = (uint16_t)side_thr(hbd, v);
+    }
+}
+
+// Bottom-of-frame crop of 32-long transform edges (deblock.rs:1572-1606),
+// applied once to the library's copy for every superblock row.
+void crop_bottom(std::vector<Rav2dAv2Filter> &mask, const DevGeom &g) {
+    int sbh = (g.bh + (16 << g.sb128) - 1) >> (4 + g.sb128);
+    for (int sby = 0; sby < sbh; sby++) {
+        size_t mask_row = (size_t)((sby >> (2 - g.sb128)) * g.sb256w);
+        int y64_start = sby << g.sb128;
+        int y64_end = std::min((sby + 1) << g.sb128, (g.bh + 15) >> 4);
+        for (int y64 = y64_start; y64 < y64_end; y64++) {
+            if ((y64 + 1) * 16 + 4 <= g.bh) continue;
+            int starty4 = (y64 * 16) & 0x30;
+            int h4 = std::min(g.bh - y64 * 16, 16);
+            int crop = starty4 + h4 - 2;
+            if (crop < 0) continue;
+            for (int x256 = 0; x256 < g.sb256w; x256++) {
+                if (mask_row + x256 >= mask.size()) break;
+                int w = std::min(64, g.bw - x256 * 64);
+                uint16_t(*yv)[4] = mask[mask_row + x256].filter_y[1][crop];
+                for (int i = 0; i < ((w + 15) >> 4); i++) {
+                    uint16_t m = yv[3][i];
+                    yv[3][i] = 0;
+                    yv[2][i] |= m;
+                }
+            }
+        }
+    }
+}
+
+int validate(const Rav2dFrameParams *p) {
+    if (!p) return RAV2D_EINVAL;
+    if (p->width <= 0 || p->height <= 0 || (p->width & 7) || (p->height & 7)) {
+        set_last_error("frame size must be a positive multiple of 8");
+        return RAV2D_EINVAL;
+    }
+    if (p->bitdepth != 8 && p->bitdepth != 10) {
+        set_last_error("bit depth must be 8 or 10");
+        return RAV2D_EINVAL;
+    }
+    if (p->layout != RAV2D_PIXEL_LAYOUT_I420) {
+        set_last_error("only 4:2:0 is supported");
+        return RAV2D_EINVAL;
+    }
+    if (p->seg_enabled) {
+        set_last_error("segmentation is not supported");
+        return RAV2D_EINVAL;
+    }
+    if (p->sb128 != 0 && p->sb128 != 1) {
+        set_last_error("sb128 must be 0 or 1");
+        return RAV2D_EINVAL;
+    }
+    return RAV2D_OK;
+}
+
+DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl) {
+    DevPlane d;
+    d.ptr = f->buf[b][pl];
+    d.stride = f->stride[pl];
+    d.w = f->pw[pl];
+    d.h = f->ph[pl];
+    return d;
+}
+
+int copy_plane(Rav2dGpuFrame *f, int from, int to, int pl, hipStream_t s) {
+    HIP_TRY(hipMemcpy2DAsync(f->buf[to][pl], f->stride[pl], f->buf[from][pl], f->stride[pl],
+                             (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyDeviceToDevice, s));
+    return RAV2D_OK;
+}
+
+hipStream_t pick_stream(Rav2dGpuCtx *ctx, void *stream) { return stream ? (hipStream_t)stream : ctx->stream; }
+
+// Which passes touch which plane.
+struct Plan {
+    bool dbv[3], dbh[3];
+};
+
+Plan make_plan(const Rav2dFrameParams &p) {
+    Plan pl;
+    bool db_on = (p.inloop_filters & RAV2D_INLOOPFILTER_DEBLOCK) && (p.db_level_y[0] || p.db_level_y[1]);
+    pl.dbv[0] = db_on && p.db_level_y[0];
+    pl.dbh[0] = db_on && p.db_level_y[1];
+    for (int c = 1; c < 3; c++) {
+        bool lv = c == 1 ? p.db_level_u : p.db_level_v;
+        pl.dbv[c] = db_on && lv;
+        pl.dbh[c] = db_on && lv;
+    }
+    return pl;
+}
+
+// Runs the in-loop chain F0 -> OUT.  `upto` limits the passes (for per-pass
+// parity): passes with index > upto are skipped.
+int run_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, hipStream_t s) {
+    if (!f->side_ok) {
+        set_last_error("side info not set");
+        return RAV2D_EINVAL;
+    }
+    Plan plan = make_plan(f->params);
+    int cur[3] = {BUF_IN, BUF_IN, BUF_IN};
+    // output buffer of a pass for a plane: a temporary other than `cur`
+    auto next_tmp = [&](int c) { return c == BUF_T1 ? BUF_T2 : BUF_T1; };
+    // deblock V
+    if (upto >= RAV2D_PASS_DEBLOCK_V && (plan.dbv[0] || plan.dbv[1] || plan.dbv[2])) {
+        DeblockArgs a;
+        memset(&a, 0, sizeof(a));
+        a.g = f->g;
+        a.mask = f->d_mask;
+        a.lut = f->d_dblut;
+        for (int pl = 0; pl < 3; pl++) {
+            a.plane_on[pl] = plan.dbv[pl];
+            int out = next_tmp(cur[pl]);
+            a.src[pl] = plane_of(f, cur[pl], pl);
+            a.dst[pl] = plane_of(f, out, pl);
+            cur[pl] = out;
+        }
+        PassTimer pt(ctx, RAV2D_PASS_DEBLOCK_V, s);
+        launch_deblock_v(a, f->g.bitdepth, s);
+        HIP_TRY(hipGetLastError());
+    }
+    if (upto >= RAV2D_PASS_DEBLOCK_H && (plan.dbh[0] || plan.dbh[1] || plan.dbh[2])) {
+        DeblockArgs a;
+        memset(&a, 0, sizeof(a));
+        a.g = f->g;
+        a.mask = f->d_mask;
+        a.lut = f->d_dblut;
+        for (int pl = 0; pl < 3; pl++) {
+            a.plane_on[pl] = plan.dbh[pl];
+            int out = next_tmp(cur[pl]);
+            a.src[pl] = plane_of(f, cur[pl], pl);
+            a.dst[pl] = plane_of(f, out, pl);
+            cur[pl] = out;
+        }
+        PassTimer pt(ctx, RAV2D_PASS_DEBLOCK_H, s);
+        launch_deblock_h(a, f->g.bitdepth, s);
+        HIP_TRY(hipGetLastError());
+    }
+    for (int pl = 0; pl < 3; pl++) {
+        int r = copy_plane(f, cur[pl], BUF_OUT, pl, s);
+        if (r) return r;
+    }
+    return RAV2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t rav2d_gpu_struct_size(int which) {
+    switch (which) {
+        case 0: return sizeof(Rav2dAv2Filter);
+        case 1: return sizeof(Rav2dAv2Restoration);
+        case 2: return sizeof(Rav2dFrameParams);
+        case 3: return sizeof(Rav2dFilmGrainData);
+        case 4: return sizeof(Rav2dPlanes);
+        default: return 0;
+    }
+}
+
+const char *rav2d_gpu_version(void) { return "rav2d-gpu 0.1 (gfx950)"; }
+const char *rav2d_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int rav2d_gpu_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int rav2d_gpu_ctx_create(int device, uint64_t hbm_budget_bytes, uint32_t flags, Rav2dGpuCtx **out) {
+    (void)flags;
+    if (!out) return RAV2D_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        set_last_error("no such HIP device");
+        return RAV2D_EINVAL;
+    }
+    HIP_TRY(hipSetDevice(device));
+    Rav2dGpuCtx *c = new Rav2dGpuCtx();
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        set_last_error("stream/event creation failed");
+        delete c;
+        return RAV2D_EDEVICE;
+    }
+    *out = c;
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_ctx_destroy(Rav2dGpuCtx *ctx) {
+    if (!ctx) return RAV2D_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &r : ctx->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_ctx_synchronize(Rav2dGpuCtx *ctx) {
+    if (!ctx) return RAV2D_EINVAL;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RAV2D_OK;
+}
+
+uint64_t rav2d_gpu_ctx_bytes_in_use(const Rav2dGpuCtx *ctx) { return ctx ? ctx->in_use : 0; }
+
+int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav2dGpuFrame **out) {
+    if (!ctx || !out) return RAV2D_EINVAL;
+    *out = nullptr;
+    int r = validate(params);
+    if (r) return r;
+    HIP_TRY(hipSetDevice(ctx->device));
+    Rav2dGpuFrame *f = new Rav2dGpuFrame();
+    f->ctx = ctx;
+    f->params = *params;
+    DevGeom &g = f->g;
+    g.bw = ((params->width + 7) >> 3) << 1;
+    g.bh = ((params->height + 7) >> 3) << 1;
+    g.sb256w = (g.bw + 63) >> 6;
+    g.sb256h = (g.bh + 63) >> 6;
+    g.sb128 = params->sb128;
+    g.bitdepth = params->bitdepth;
+    g.bdmax = (1 << params->bitdepth) - 1;
+    g.hbd = params->bitdepth > 8;
+    g.ss_hor = 1;
+    g.ss_ver = 1;
+    f->bps = params->bitdepth > 8 ? 2 : 1;
+    f->pw[0] = params->width;
+    f->ph[0] = params->height;
+    f->pw[1] = f->pw[2] = (params->width + 1) >> 1;
+    f->ph[1] = f->ph[2] = (params->height + 1) >> 1;
+    size_t total = 0;
+    for (int pl = 0; pl < 3; pl++) {
+        int64_t s = ((int64_t)f->pw[pl] * f->bps + 255) & ~(int64_t)255;
+        if ((s & 1023) == 0) s += 256;  // keep rows off the same channel stride
+        f->stride[pl] = s;
+        f->plane_bytes[pl] = (size_t)s * (f->ph[pl] + 8);
+        total += f->plane_bytes[pl];
+    }
+    size_t need = total * NBUF + sizeof(DbLut);
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (ctx->budget && ctx->in_use + need > ctx->budget) {
+            delete f;
+            set_last_error("HBM budget exceeded");
+            return RAV2D_EAGAIN;
+        }
+        ctx->in_use += need;
+    }
+    f->bytes = need;
+    if (hipMalloc((void **)&f->block, total * NBUF) != hipSuccess ||
+        hipMalloc((void **)&f->d_dblut, sizeof(DbLut)) != hipSuccess) {
+        if (f->block) (void)hipFree(f->block);
+        {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            ctx->in_use -= need;
+        }
+        delete f;
+        set_last_error("hipMalloc failed");
+        return RAV2D_ENOMEM;
+    }
+    uint8_t *p = f->block;
+    for (int b = 0; b < NBUF; b++)
+        for (int pl = 0; pl < 3; pl++) {
+            f->buf[b][pl] = p;
+            p += f->plane_bytes[pl];
+        }
+    DbLut lut;
+    build_db_lut(*params, g.hbd, lut);
+    HIP_TRY(hipMemcpy(f->d_dblut, &lut, sizeof(lut), hipMemcpyHostToDevice));
+    *out = f;
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f) {
+    if (!f) return RAV2D_EINVAL;
+    Rav2dGpuCtx *ctx = f->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(f->block);
+    (void)hipFree(f->d_dblut);
+    if (f->d_mask) (void)hipFree(f->d_mask);
+    if (f->d_lr) (void)hipFree(f->d_lr);
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->in_use -= f->bytes;
+    }
+    delete f;
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_frame_set_params(Rav2dGpuFrame *f, const Rav2dFrameParams *params) {
+    if (!f) return RAV2D_EINVAL;
+    int r = validate(params);
+    if (r) return r;
+    if (params->width != f->params.width || params->height != f->params.height ||
+        params->bitdepth != f->params.bitdepth || params->layout != f->params.layout ||
+        params->sb128 != f->params.sb128) {
+        set_last_error("frame geometry differs");
+        return RAV2D_EINVAL;
+    }
+    f->params = *params;
+    DbLut lut;
+    build_db_lut(*params, f->g.hbd, lut);
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    HIP_TRY(hipMemcpy(f->d_dblut, &lut, sizeof(lut), hipMemcpyHostToDevice));
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, size_t n_mask,
+                                  const Rav2dAv2Restoration *lr, size_t n_lr, void *stream) {
+    if (!f || !mask) return RAV2D_EINVAL;
+    size_t need = (size_t)f->g.sb256w * f->g.sb256h;
+    if (n_mask != need || (lr && n_lr != need)) {
+        set_last_error("side info count must equal the SB256 grid size");
+        return RAV2D_EINVAL;
+    }
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    hipStream_t s = pick_stream(f->ctx, stream);
+    if (!f->d_mask) {
+        HIP_TRY(hipMalloc((void **)&f->d_mask, need * sizeof(Rav2dAv2Filter)));
+        std::lock_guard<std::mutex> lk(f->ctx->mu);
+        f->ctx->in_use += need * sizeof(Rav2dAv2Filter);
+        f->bytes += need * sizeof(Rav2dAv2Filter);
+    }
+    std::vector<Rav2dAv2Filter> copy(mask, mask + n_mask);
+    crop_bottom(copy, f->g);
+    HIP_TRY(hipMemcpyAsync(f->d_mask, copy.data(), need * sizeof(Rav2dAv2Filter), hipMemcpyHostToDevice, s));
+    if (lr) {
+        if (!f->d_lr) {
+            HIP_TRY(hipMalloc((void **)&f->d_lr, need * sizeof(Rav2dAv2Restoration)));
+            std::lock_guard<std::mutex> lk(f->ctx->mu);
+            f->ctx->in_use += need * sizeof(Rav2dAv2Restoration);
+            f->bytes += need * sizeof(Rav2dAv2Restoration);
+        }
+        HIP_TRY(hipMemcpyAsync(f->d_lr, lr, need * sizeof(Rav2dAv2Restoration), hipMemcpyHostToDevice, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));  // `copy` is a pageable temporary
+    f->side_ok = true;
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_frame_planes(const Rav2dGpuFrame *f, int which, Rav2dPlanes *out) {
+    if (!f || !out || which < 0 || which > 2) return RAV2D_EINVAL;
+    int b = which == 0 ? BUF_IN : which == 1 ? BUF_OUT : BUF_DISP;
+    for (int pl = 0; pl < 3; pl++) out->data[pl] = f->buf[b][pl];
+    out->stride[0] = f->stride[0];
+    out->stride[1] = f->stride[1];
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_frame_upload(Rav2dGpuFrame *f, const void *y, const void *u, const void *v, void *stream) {
+    if (!f || !y || !u || !v) return RAV2D_EINVAL;
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    hipStream_t s = pick_stream(f->ctx, stream);
+    const void *src[3] = {y, u, v};
+    for (int pl = 0; pl < 3; pl++)
+        HIP_TRY(hipMemcpy2DAsync(f->buf[BUF_IN][pl], f->stride[pl], src[pl], (size_t)f->pw[pl] * f->bps,
+                                 (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_frame_download(Rav2dGpuFrame *f, int which, void *y, void *u, void *v, void *stream) {
+    if (!f || !y || !u || !v || which < 0 || which > 2) return RAV2D_EINVAL;
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    hipStream_t s = pick_stream(f->ctx, stream);
+    int b = which == 0 ? BUF_IN : which == 1 ? BUF_OUT : BUF_DISP;
+    void *dst[3] = {y, u, v};
+    for (int pl = 0; pl < 3; pl++)
+        HIP_TRY(hipMemcpy2DAsync(dst[pl], (size_t)f->pw[pl] * f->bps, f->buf[b][pl], f->stride[pl],
+                                 (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_filter_frame(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream) {
+    if (!ctx || !f || f->ctx != ctx) return RAV2D_EINVAL;
+    HIP_TRY(hipSetDevice(ctx->device));
+    return run_chain(ctx, f, RAV2D_PASS_GRAIN, pick_stream(ctx, stream));
+}
+
+int rav2d_gpu_apply_grain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream) {
+    if (!ctx || !f || f->ctx != ctx) return RAV2D_EINVAL;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = pick_stream(ctx, stream);
+    for (int pl = 0; pl < 3; pl++) {
+        int r = copy_plane(f, BUF_OUT, BUF_DISP, pl, s);
+        if (r) return r;
+    }
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_run_pass(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int pass, void *stream) {
+    if (!ctx || !f || pass < 0 || pass > RAV2D_PASS_GRAIN) return RAV2D_EINVAL;
+    HIP_TRY(hipSetDevice(ctx->device));
+    return run_chain(ctx, f, pass, pick_stream(ctx, stream));
+}
+
+int rav2d_gpu_set_profiling(Rav2dGpuCtx *ctx, int on) {
+    if (!ctx) return RAV2D_EINVAL;
+    ctx->profiling = on != 0;
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_pass_times(Rav2dGpuCtx *ctx, double *ms, int64_t *count, int reset) {
+    if (!ctx) return RAV2D_EINVAL;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto &r : ctx->recs) {
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
+        ctx->pass_ms[r.pass] += t;
+        ctx->pass_count[r.pass] += 1;
+        ctx->free_events.push_back(r.a);
+        ctx->free_events.push_back(r.b);
+    }
+    ctx->recs.clear();
+    for (int i = 0; i < RAV2D_NUM_PASSES; i++) {
+        if (ms) ms[i] = ctx->pass_ms[i];
+        if (count) count[i] = ctx->pass_count[i];
+        if (reset) {
+            ctx->pass_ms[i] = 0;
+            ctx->pass_count[i] = 0;
+        }
+    }
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_timer_start(Rav2dGpuCtx *ctx) {
+    if (!ctx) return RAV2D_EINVAL;
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_timer_stop(Rav2dGpuCtx *ctx, float *ms) {
+    if (!ctx || !ms) return RAV2D_EINVAL;
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    HIP_TRY(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return RAV2D_OK;
+}
+
+}  // extern "C"

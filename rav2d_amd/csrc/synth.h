@@ -1,0 +1,59 @@
+/*
+ * Seeded synthetic filter-stage workloads: reconstructed planes plus the side
+ * info a decoder would hand to the filter stage (Av2Filter / Av2Restoration
+ * records and frame parameters).  Used by the tests and the benchmark; not
+ * part of the drop-in ABI in rav2d_gpu.h.
+ *
+ * Planes: smooth 2-D gradient + Gaussian noise (sigma 4 at 8 bit, 16 at 10 bit)
+ * + per-transform-block DC offsets on a random quadtree of 8x8..64x64 blocks with
+ * square transforms 4..64.  Edge masks are produced the way create_db_mask
+ * (crates/rav2d/src/lf_mask.rs:399-502) does for intra blocks without transform
+ * partitioning; parameters are drawn from the legal ranges of the frame header
+ * parser (crates/rav2d/src/obu.rs:261-328, 1293-1455, 1696-1736).
+ */
+#ifndef RAV2D_SYNTH_H
+#define RAV2D_SYNTH_H
+#include "../../include/rav2d_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* which tools the generated frame enables */
+#define R2S_TOOL_DEBLOCK (1u << 0)
+#define R2S_TOOL_CDEF (1u << 1)
+#define R2S_TOOL_CCSO (1u << 2)
+#define R2S_TOOL_WIENER (1u << 3)
+#define R2S_TOOL_GDF (1u << 4)
+#define R2S_TOOL_GRAIN (1u << 5)
+#define R2S_TOOL_ALL 63u
+
+typedef struct R2SConfig {
+    int32_t width;
+    int32_t height;
+    int32_t bitdepth;
+    int32_t sb128;
+    uint64_t seed;
+    uint32_t tools;          /* R2S_TOOL_* */
+    int32_t lossless_pct;    /* percent of blocks whose masks are marked lossless */
+    int32_t skip_pct;        /* percent of blocks without residual (noskip = 0)   */
+    int32_t pc_gdf_pct;      /* percent of restoration units / GDF blocks enabled */
+} R2SConfig;
+
+typedef struct R2SFrame R2SFrame;
+
+R2SFrame *r2s_frame_create(const R2SConfig *cfg);
+void r2s_frame_destroy(R2SFrame *f);
+Rav2dFrameParams *r2s_params(R2SFrame *f);
+/* tightly packed planes: stride = plane width in samples */
+void *r2s_plane(R2SFrame *f, int pl);
+int32_t r2s_plane_width(const R2SFrame *f, int pl);
+int32_t r2s_plane_height(const R2SFrame *f, int pl);
+size_t r2s_plane_bytes(const R2SFrame *f, int pl);
+Rav2dAv2Filter *r2s_mask(R2SFrame *f, size_t *n);
+Rav2dAv2Restoration *r2s_lr(R2SFrame *f, size_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

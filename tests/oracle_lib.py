@@ -1,0 +1,46 @@
+"""ctypes loader for the CPU oracle (oracle/liboracle.so): the checker the
+parity tests compare the HIP path against."""
+import ctypes as C
+import os
+
+import numpy as np
+
+import rav2d_amd as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so not built: run `python -m rav2d_amd.build`")
+        lib = C.CDLL(path)
+        P = C.c_void_p
+        lib.oracle_filter_frame.restype = C.c_int
+        lib.oracle_filter_frame.argtypes = [C.POINTER(R.FrameParams), P, C.c_size_t, P, C.c_size_t,
+                                            P, P, P, C.c_ssize_t, C.c_ssize_t, C.c_uint32]
+        lib.oracle_dq_lookup.restype = C.c_int
+        lib.oracle_dq_lookup.argtypes = [C.c_int]
+        lib.oracle_deblock_quant_thr.restype = C.c_uint32
+        lib.oracle_deblock_quant_thr.argtypes = [C.c_int, C.c_int]
+        lib.oracle_deblock_side_thr.restype = C.c_uint32
+        lib.oracle_deblock_side_thr.argtypes = [C.c_int, C.c_int]
+        _lib = lib
+    return _lib
+
+
+def filter_frame(sf, inloop_mask=R.INLOOP_ALL, params=None):
+    """Run the oracle chain on copies of a SynthFrame's planes; returns [y, u, v]."""
+    lib = load()
+    planes = sf.planes()
+    bps = planes[0].itemsize
+    p = params if params is not None else sf.params
+    rc = lib.oracle_filter_frame(C.byref(p), sf.mask_ptr, sf.n_mask, sf.lr_ptr, sf.n_lr,
+                                 planes[0].ctypes.data, planes[1].ctypes.data, planes[2].ctypes.data,
+                                 planes[0].shape[1] * bps, planes[1].shape[1] * bps, inloop_mask)
+    if rc != 0:
+        raise RuntimeError("oracle_filter_frame failed: %d" % rc)
+    return planes
