@@ -5,6 +5,9 @@
  */
 #include "oracle_common.h"
 #include "oracle.h"
+#include "oracle_cdef.h"
+#include "oracle_lr.h"
+#include "../rav2d_amd/csrc/tables.h"
 
 namespace oracle {
 template <typename P>
@@ -26,9 +29,78 @@ static int run_frame(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask_in, s
     Frame<P> f{y, u, v, ys, uvs};
     uint32_t inloop = p->inloop_filters & stop_mask;
     bool deblock_on = (inloop & RAV2D_INLOOPFILTER_DEBLOCK) && (p->db_level_y[0] || p->db_level_y[1]);
+    /* CDEF (+CCSO) set-up, decode.rs:3858-3913 */
+    bool cdef_stage = p->seq_cdef && (inloop & (RAV2D_INLOOPFILTER_CDEF | RAV2D_INLOOPFILTER_CCSO));
+    CdefBrowParams cp;
+    memset(&cp, 0, sizeof(cp));
+    cp.bw = g.bw;
+    cp.bh = g.bh;
+    cp.damping = p->cdef_damping;
+    cp.on_skip_tx = p->cdef_on_skiptx != 0;
+    cp.cdef_on = (inloop & RAV2D_INLOOPFILTER_CDEF) != 0;
+    cp.ccso_on = (inloop & RAV2D_INLOOPFILTER_CCSO) && (p->ccso[0].enabled || p->ccso[1].enabled || p->ccso[2].enabled);
+    cp.y_strength = p->cdef_y_strength;
+    cp.uv_strength = p->cdef_uv_strength;
+    cp.mask = mask.data();
+    cp.n_mask = mask.size();
+    cp.sb256w = g.sb256w;
+    for (int pl = 0; pl < 3; pl++) {
+        const Rav2dCcsoPlane &c = p->ccso[pl];
+        cp.ccso[pl].max_band_log2 = c.max_band_log2;
+        cp.ccso[pl].ext_filter = c.ext_filter_support;
+        cp.ccso[pl].quant_step = R2T_CCSO_QUANT_SZ[c.scale_idx][c.quant_idx];
+        cp.ccso[pl].edge_clf = c.edge_clf;
+        cp.ccso[pl].bo_only = c.bo_only != 0;
+        cp.ccso[pl].scale_idx = c.scale_idx;
+        memcpy(cp.ccso[pl].filter_off, c.filter_off, 64);
+    }
+    CdefLines<P> lines;
+    for (int b = 0; b < 2; b++) {
+        lines.bank[b][0].assign((size_t)(2 * ys), 0);
+        lines.bank[b][1].assign((size_t)(2 * uvs), 0);
+        lines.bank[b][2].assign((size_t)(2 * uvs), 0);
+    }
+    lines.toggle = 0;
+    /* loop restoration set-up, decode.rs:3812-3853, 3999-4057 */
+    LrCtx<P> lc;
+    lc.p = p;
+    lc.g = &g;
+    lc.mask = mask.data();
+    lc.n_mask = mask.size();
+    lc.lr = lr;
+    lc.n_lr = lr ? n_lr : 0;
+    lc.ys = ys;
+    lc.uvs = uvs;
+    lc.inloop = inloop;
+    lr_init(lc, p);
+    for (int pl = 0; pl < 3; pl++) lc.lr_db[pl].assign((size_t)(20 * (pl ? uvs : ys)), 0);
+    bool lr_stage = lc.restore_planes && (inloop & (RAV2D_INLOOPFILTER_WIENER | RAV2D_INLOOPFILTER_GDF));
+    std::vector<P> snapshot;
     for (int sby = 0; sby < g.sbh; sby++) {
         if (deblock_on)
             deblock_sbrow_impl(p, g, mask.data(), mask.size(), f, sby, (stop_mask & ORACLE_DEBLOCK_COLS_ONLY) != 0);
+        /* copy_db (decode.rs:3818-3853) */
+        if (lr_stage) copy_db(lc, f, sby);
+        if (cdef_stage) {
+            int start = sby * g.sb_step;
+            if (sby > 0) {
+                /* cross-row seam with the previous mask row (decode.rs:3914-3956) */
+                CdefBrowParams prev = cp;
+                prev.row = (size_t)(((sby - 1) >> (2 - g.sb128)) * g.sb256w);
+                cdef_brow(prev, f, lines, start - 2, start, g.bitdepth);
+            }
+            CdefBrowParams cur = cp;
+            cur.row = (size_t)((sby >> (2 - g.sb128)) * g.sb256w);
+            int n_blks = g.sb_step - 2 * (sby + 1 < g.sbh);
+            int end = imin(start + n_blks, g.bh);
+            cdef_brow(cur, f, lines, start, end, g.bitdepth);
+        }
+        if (lr_stage) {
+            /* chroma reads the pre-luma-LR luma of this sbrow (decode.rs:4004-4007) */
+            snapshot.assign(f.y, f.y + (size_t)ys * ((g.bh * 4) + 0));
+            lc.luma_snapshot = snapshot.data();
+            lr_sbrow_apply(lc, f, sby);
+        }
     }
     return 0;
 }

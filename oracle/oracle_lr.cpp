@@ -1,0 +1,821 @@
+/*
+ * Oracle: loop restoration (NS-Wiener single luma / chroma with cross-component
+ * taps, NS-Wiener multi-class, PC-Wiener, GDF) restated from
+ * crates/rav2d/src/looprestoration.rs, with the deblocked-line store of
+ * copy_db/backup_db (crates/rav2d/src/deblock.rs:688-729, 2192-2264) and the
+ * per-frame derivations of init_wiener / compute_restore_planes
+ * (crates/rav2d/src/decode.rs:27-65).  The 8-bit SIMD kernels of simd.rs are
+ * equal to these scalar forms by the reference's own tests (simd.rs:1129-1450),
+ * so only the scalar forms are restated.  Single-tile frames only: the
+ * *_INTEGRATED edge flags and the cross-tile CDEF line are never set.
+ * TEST INFRASTRUCTURE ONLY (see oracle_common.h).
+ */
+#include "oracle_lr.h"
+#include "../rav2d_amd/csrc/tables.h"
+
+namespace oracle {
+
+static const uint8_t LR_HAVE_LEFT = 1, LR_HAVE_RIGHT = 2, LR_HAVE_TOP = 4, LR_HAVE_BOTTOM = 8;
+
+/* looprestoration.rs:13-22 */
+static const uint16_t PC_WIENER_NORMALIZER[4] = {3739, 3273, 3074, 7};
+static const int16_t MODE_WEIGHTS[4][3] = {{-527, 15325, 321}, {26436, -17705, 17905}, {366, -147, -194}, {202, -267, -179}};
+static const int16_t MODE_OFFSETS[4] = {-547, -21565, -573, -680};
+
+/* looprestoration.rs:334-406 */
+static const int8_t GDF_COORDS[18][2] = {{6, 0}, {5, 0}, {4, 0}, {3, 0}, {2, 1}, {2, 0}, {2, -1}, {1, 2}, {1, 1},
+                                         {1, 0}, {1, -1}, {1, -2}, {0, 6}, {0, 5}, {0, 4}, {0, 3}, {0, 2}, {0, 1}};
+static const int8_t WIENER_NS_CONFIG_UV[6][2] = {{1, 0}, {0, 1}, {1, 1}, {-1, 1}, {2, 0}, {0, 2}};
+static const int8_t WIENER_NS_CONFIG_UV_FROM_Y[12][2] = {{1, 0},  {-1, 0},  {0, 1}, {0, -1}, {1, 1}, {-1, -1},
+                                                         {-1, 1}, {1, -1}, {2, 0}, {-2, 0}, {0, 2}, {0, -2}};
+static const int8_t PC_WIENER_CONFIG[12][2] = {{1, 0},  {0, 1},  {2, 0}, {0, 2}, {1, 1}, {-1, 1},
+                                               {2, 1},  {2, -1}, {1, 2}, {1, -2}, {3, 0}, {0, 3}};
+static const int8_t WIENER_NS_CONFIG_Y[16][2] = {{1, 0}, {0, 1}, {2, 0}, {0, 2}, {1, 1}, {-1, 1}, {2, 1}, {2, -1},
+                                                 {1, 2}, {1, -2}, {3, 0}, {0, 3}, {4, 0}, {0, 4}, {3, 3}, {3, -3}};
+
+static const int RUS = 76;   /* REST_UNIT_STRIDE */
+static const int ORIG = 6;   /* ROW_ORIGIN */
+static const int LBS = 76 + 64;  /* LUMA_BUF_STRIDE */
+
+/* looprestoration.rs:40-49 */
+int get_qval_given_tskip(int qstep, int tskip, int i, int bdm8) {
+    qstep = (qstep + ((1 << bdm8) >> 1)) >> bdm8;
+    int prod = (tskip * qstep + 128) >> 8;
+    int qval = MODE_WEIGHTS[i][0] * (tskip << 5) + MODE_WEIGHTS[i][1] * qstep + MODE_WEIGHTS[i][2] * prod;
+    int a = qval < 0 ? -qval : qval;
+    qval = apply_sign((a + (1 << 12)) >> 13, qval);
+    return 255 * (MODE_OFFSETS[i] + qval);
+}
+
+/* looprestoration.rs:54-80 */
+template <typename P>
+static void backup_row(P *dst, const P *src, const P *left6, int w, int edge_len, uint8_t edges) {
+    if (edges & LR_HAVE_LEFT)
+        for (int x = 0; x < edge_len; x++) dst[ORIG - edge_len + x] = left6[6 - edge_len + x];
+    else
+        for (int x = 0; x < edge_len; x++) dst[ORIG - edge_len + x] = src[0];
+    for (int x = 0; x < w; x++) dst[ORIG + x] = src[x];
+    if (edges & LR_HAVE_RIGHT)
+        for (int x = 0; x < edge_len; x++) dst[ORIG + w + x] = src[w + x];
+    else
+        for (int x = 0; x < edge_len; x++) dst[ORIG + w + x] = src[w - 1];
+}
+
+/* looprestoration.rs:85-109 */
+template <typename P> static void backup_row_lpf(P *dst, const P *src, int w, int edge_len, uint8_t edges) {
+    if (edges & LR_HAVE_LEFT)
+        for (int x = 0; x < edge_len; x++) dst[ORIG - edge_len + x] = src[-edge_len + x];
+    else
+        for (int x = 0; x < edge_len; x++) dst[ORIG - edge_len + x] = src[0];
+    for (int x = 0; x < w; x++) dst[ORIG + x] = src[x];
+    if (edges & LR_HAVE_RIGHT)
+        for (int x = 0; x < edge_len; x++) dst[ORIG + w + x] = src[w + x];
+    else
+        for (int x = 0; x < edge_len; x++) dst[ORIG + w + x] = src[w - 1];
+}
+
+/* looprestoration.rs:408-522 (src2 = src + src_stride) */
+template <typename P>
+static void backup_row_luma(P *dst, const P *src, ptrdiff_t src_stride, int w, uint8_t edges, int ss_hor, int ss_ver,
+                            int ds) {
+    if (!ss_ver) {
+        backup_row_lpf(dst, src, w, 4, edges);
+        return;
+    }
+    const P *src2 = src + src_stride;
+    if (ds == 0) {
+        for (int x = 0; x < w; x += 1 + ss_hor)
+            dst[ORIG + x] = (P)(((int)src[x] + src2[x] + src[x + 1] + src2[x + 1]) >> 2);
+    } else if (ds == 1) {
+        for (int x = 0; x < w; x++) dst[ORIG + x] = (P)(((int)src[x] + src2[x]) >> 1);
+    } else {
+        for (int x = 0; x < w; x++) dst[ORIG + x] = src[x];
+    }
+    if (edges & LR_HAVE_LEFT) {
+        for (int i = 0; i < 4; i++) {
+            int si = -4 + i;
+            if (ds == 0)
+                dst[ORIG - 4 + i] = (P)(((int)src[si] + src2[si] + src[si + 1] + src2[si + 1]) >> 2);
+            else if (ds == 1)
+                dst[ORIG - 4 + i] = (P)(((int)src[si] + src2[si]) >> 1);
+            else
+                dst[ORIG - 4 + i] = src[si];
+        }
+    } else {
+        P v = dst[ORIG];
+        for (int i = 0; i < 4; i++) dst[ORIG - 4 + i] = v;
+    }
+    if (edges & LR_HAVE_RIGHT) {
+        for (int i = 0; i < 4; i++) {
+            int si = w + i;
+            if (ds == 0)
+                dst[ORIG + w + i] = (P)(((int)src[si] + src2[si] + src[si + 1] + src2[si + 1]) >> 2);
+            else if (ds == 1)
+                dst[ORIG + w + i] = (P)(((int)src[si] + src2[si]) >> 1);
+            else
+                dst[ORIG + w + i] = src[si];
+        }
+    } else {
+        P v = dst[ORIG + w - 2];
+        for (int i = 0; i < 4; i++) dst[ORIG + w + i] = v;
+    }
+}
+
+/* Row buffers of a sliding window addressed through a pointer table. */
+template <typename P, int N> struct RowBufs {
+    P b[N][LBS];
+    RowBufs() { memset(b, 0, sizeof(b)); }
+};
+
+/* looprestoration.rs:524-713 */
+template <typename P>
+static void ns_wiener_single_y(P *p, ptrdiff_t stride, const P (*left)[6], const P *lpf, const P *lpf_bottom, int w,
+                               int h, const int8_t *filter, uint8_t edges, const uint16_t (*ll)[4], int bitdepth) {
+    static thread_local RowBufs<P, 9> rb;
+    int ptrs[9] = {0};
+    backup_row(rb.b[4], p, left[0], w, 4, edges);
+    ptrs[4] = 4;
+    if (edges & LR_HAVE_TOP) {
+        backup_row_lpf(rb.b[2], lpf, w, 4, edges);
+        backup_row_lpf(rb.b[3], lpf + stride, w, 4, edges);
+        ptrs[2] = 2;
+        ptrs[3] = 3;
+        ptrs[0] = 2;
+        ptrs[1] = 2;
+    } else {
+        ptrs[0] = ptrs[1] = ptrs[2] = ptrs[3] = 4;
+    }
+    for (int r = 1; r < 4; r++) {
+        backup_row(rb.b[4 + r], p + r * stride, left[r], w, 4, edges);
+        ptrs[4 + r] = 4 + r;
+    }
+    int bak = 8;
+    int bdmax = (1 << bitdepth) - 1;
+    for (int y = 0; y < h; y++) {
+        if (y + 4 < h) {
+            backup_row(rb.b[bak], p + (y + 4) * stride, left[y + 4], w, 4, edges);
+            ptrs[8] = bak;
+        } else if (y + 2 < h && (edges & LR_HAVE_BOTTOM)) {
+            backup_row_lpf(rb.b[bak], lpf_bottom + (y + 4 - h) * stride, w, 4, edges);
+            ptrs[8] = bak;
+        } else {
+            ptrs[8] = ptrs[7];
+        }
+        if (++bak == 9) bak = 0;
+        uint16_t skip = ll[y >> 2][0];
+        P *dst = p + y * stride;
+        for (int bx = 0; bx < (w >> 2); bx++) {
+            if (skip & (1 << bx)) continue;
+            for (int j = bx * 4; j < bx * 4 + 4; j++) {
+                int c = ORIG + j;
+                int m = rb.b[ptrs[4]][c];
+                int s = m << 7;
+                for (int i = 0; i < 16; i++) {
+                    int dy = WIENER_NS_CONFIG_Y[i][0], dx = WIENER_NS_CONFIG_Y[i][1];
+                    int a = rb.b[ptrs[4 + dy]][c + dx], b = rb.b[ptrs[4 - dy]][c - dx];
+                    s += (a + b - 2 * m) * filter[i];
+                }
+                dst[j] = (P)iclip((s + 64) >> 7, 0, bdmax);
+            }
+        }
+        for (int r = 0; r < 8; r++) ptrs[r] = ptrs[r + 1];
+    }
+}
+
+/* looprestoration.rs:195-271 over a 10-row window `rows` (ptr table) */
+template <typename P>
+static int get_class_lut_idx(const P *const *rows, int row_center, int col, const uint16_t *noskip, int base_q, int bx,
+                             int by, int bh, int bdm8) {
+    int f[3] = {0, 0, 0}, s = 0;
+    for (int dy = -1; dy <= 4; dy++)
+        for (int dx = -1; dx <= 4; dx++) {
+            int x = col + bx * 4 + dx;
+            int y = row_center + dy;
+            int m = rows[y][x], up = rows[y - 1][x], down = rows[y + 1][x];
+            int vert = up - 2 * m + down;
+            int anti = (int)rows[y - 1][x + 1] - 2 * m + (int)rows[y + 1][x - 1];
+            int diag = (int)rows[y - 1][x - 1] - 2 * m + (int)rows[y + 1][x + 1];
+            f[0] += abs(vert);
+            f[1] += abs(anti);
+            f[2] += abs(diag);
+        }
+    static const uint8_t num_pixels[3] = {16, 4, 1};
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+            int edge = (dy != 0) + (dx != 0);
+            int fx = iclip((bx & 15) + dx, 0, 15);
+            int fy = iclip(by + dy, 0, bh - 1);
+            s += num_pixels[edge] * (((noskip[fy] >> fx) & 1) == 0);
+        }
+    int rnd = (1 << bdm8) >> 1;
+    for (int i = 0; i < 3; i++) f[i] = (int)((uint32_t)f[i] * PC_WIENER_NORMALIZER[i] + (uint32_t)rnd) >> bdm8;
+    s *= PC_WIENER_NORMALIZER[3];
+    int qval = (imax(0, get_qval_given_tskip(base_q, s, 0, bdm8)) + (1 << 13)) >> 14;
+    qval = imin(qval, 255) >> 5;
+    int lut_idx = qval << 9;
+    for (int i = 0; i < 3; i++) {
+        qval = (imax(0, f[i] + get_qval_given_tskip(base_q, s, i + 1, bdm8)) + (1 << 13)) >> 14;
+        qval = imin(qval, 255) >> 5;
+        lut_idx |= qval << (3 * (2 - i));
+    }
+    return lut_idx;
+}
+
+/* looprestoration.rs:715-1059 (filters_user: NS multi, else PC with pretrained) */
+template <typename P>
+static void wiener_multi(P *p, ptrdiff_t stride, const P (*left)[6], const P *lpf, const P *lpf_bottom, int w, int h,
+                         const int8_t (*filters_user)[18], const int16_t (*filters_pre)[13], const uint8_t *subclass,
+                         const uint16_t *noskip, int base_q, uint8_t edges, const uint16_t (*ll)[4], int bitdepth) {
+    static thread_local RowBufs<P, 10> rb;
+    uint8_t classes[16] = {0};
+    int ptrs[10] = {0};
+    backup_row(rb.b[4], p, left[0], w, 4, edges);
+    ptrs[4] = 4;
+    if (edges & LR_HAVE_TOP) {
+        backup_row_lpf(rb.b[2], lpf, w, 4, edges);
+        backup_row_lpf(rb.b[3], lpf + stride, w, 4, edges);
+        ptrs[2] = 2;
+        ptrs[3] = 3;
+        ptrs[0] = 2;
+        ptrs[1] = 2;
+    } else {
+        ptrs[0] = ptrs[1] = ptrs[2] = ptrs[3] = 4;
+    }
+    for (int r = 1; r < 4; r++) {
+        backup_row(rb.b[4 + r], p + r * stride, left[r], w, 4, edges);
+        ptrs[4 + r] = 4 + r;
+    }
+    int bak = 8;
+    int bh = h >> 2, bw = w >> 2;
+    int bdmax = (1 << bitdepth) - 1;
+    for (int by = 0; by < bh; by++) {
+        int by4 = by << 2;
+        if (by + 1 < bh) {
+            backup_row(rb.b[bak], p + (by4 + 4) * stride, left[by4 + 4], w, 4, edges);
+            ptrs[8] = bak;
+            backup_row(rb.b[9], p + (by4 + 5) * stride, left[by4 + 5], w, 4, edges);
+            ptrs[9] = 9;
+        } else if (edges & LR_HAVE_BOTTOM) {
+            backup_row_lpf(rb.b[bak], lpf_bottom, w, 4, edges);
+            ptrs[8] = bak;
+            backup_row_lpf(rb.b[9], lpf_bottom + stride, w, 4, edges);
+            ptrs[9] = 9;
+        } else {
+            ptrs[8] = ptrs[7];
+            ptrs[9] = ptrs[7];
+        }
+        {
+            const P *rows[10];
+            for (int i = 0; i < 10; i++) rows[i] = rb.b[ptrs[i]];
+            for (int bx = 0; bx < bw; bx++) {
+                int li = get_class_lut_idx(rows, 4, ORIG, noskip, base_q, bx, by, bh, bitdepth - 8);
+                classes[bx] = subclass[R2T_PC_WIENER_LUT_TO_CLASS[li]];
+            }
+        }
+        for (int y = by4; y < by4 + 4; y++) {
+            if (y + 4 < h) {
+                backup_row(rb.b[bak], p + (y + 4) * stride, left[y + 4], w, 4, edges);
+                ptrs[8] = bak;
+            } else if (y + 2 < h && (edges & LR_HAVE_BOTTOM)) {
+                backup_row_lpf(rb.b[bak], lpf_bottom + (y + 4 - h) * stride, w, 4, edges);
+                ptrs[8] = bak;
+            } else {
+                ptrs[8] = ptrs[7];
+            }
+            if (++bak == 9) bak = 0;
+            uint16_t skip = ll[y >> 2][0];
+            P *dst = p + y * stride;
+            for (int bx = 0; bx < bw; bx++) {
+                if (skip & (1 << bx)) continue;
+                int cls = classes[bx];
+                for (int j = bx * 4; j < bx * 4 + 4; j++) {
+                    int c = ORIG + j;
+                    int m = rb.b[ptrs[4]][c];
+                    int s;
+                    if (filters_user) {
+                        const int8_t *fl = filters_user[cls];
+                        s = m << 7;
+                        for (int i = 0; i < 16; i++) {
+                            int dy = WIENER_NS_CONFIG_Y[i][0], dx = WIENER_NS_CONFIG_Y[i][1];
+                            int a = rb.b[ptrs[4 + dy]][c + dx], b = rb.b[ptrs[4 - dy]][c - dx];
+                            s += (a + b - 2 * m) * fl[i];
+                        }
+                    } else {
+                        const int16_t *fl = filters_pre[cls];
+                        s = m * fl[12];
+                        for (int i = 0; i < 12; i++) {
+                            int dy = PC_WIENER_CONFIG[i][0], dx = PC_WIENER_CONFIG[i][1];
+                            int a = rb.b[ptrs[4 + dy]][c + dx], b = rb.b[ptrs[4 - dy]][c - dx];
+                            s += (a + b) * fl[i];
+                        }
+                    }
+                    dst[j] = (P)iclip((s + 64) >> 7, 0, bdmax);
+                }
+            }
+            for (int r = 0; r < 8; r++) ptrs[r] = ptrs[r + 1];
+        }
+    }
+}
+
+/* looprestoration.rs:1145-1430 */
+template <typename P>
+static void ns_wiener_single_uv(P *p, ptrdiff_t stride, const P (*left)[6], const P *lpf, const P *lpf_bottom, int w,
+                                int h, const int8_t *filter, const P *luma, ptrdiff_t lstride, const P *luma_top,
+                                const P *luma_bottom, int ss_hor, int ss_ver, int ds, uint8_t edges,
+                                const uint16_t (*ll)[4], int bitdepth) {
+    static thread_local RowBufs<P, 5> cb;
+    static thread_local RowBufs<P, 5> lb;
+    int bdmax = (1 << bitdepth) - 1;
+    int cp[5] = {0}, lp[5] = {0};
+    int lw = w << ss_hor;
+    backup_row(cb.b[2], p, left[0], w, 2, edges);
+    cp[2] = 2;
+    if (edges & LR_HAVE_TOP) {
+        backup_row_lpf(cb.b[0], lpf, w, 2, edges);
+        backup_row_lpf(cb.b[1], lpf + stride, w, 2, edges);
+        cp[0] = 0;
+        cp[1] = 1;
+    } else {
+        cp[0] = cp[1] = 2;
+    }
+    backup_row(cb.b[3], p + stride, left[1], w, 2, edges);
+    cp[3] = 3;
+    int bak = 4;
+    backup_row_luma(lb.b[2], luma, lstride, lw, edges, ss_hor, ss_ver, ds);
+    lp[2] = 2;
+    if (edges & LR_HAVE_TOP) {
+        backup_row_luma(lb.b[0], luma_top, 0, lw, edges, ss_hor, ss_ver, ds);
+        backup_row_luma(lb.b[1], luma_top, lstride, lw, edges, ss_hor, ss_ver, ds);
+        lp[0] = 0;
+        lp[1] = 1;
+    } else {
+        lp[0] = lp[1] = 2;
+    }
+    backup_row_luma(lb.b[3], luma + (1 << ss_ver) * lstride, lstride, lw, edges, ss_hor, ss_ver, ds);
+    lp[3] = 3;
+    int lbak = 4;
+    const P *lpos = luma;
+    for (int y = 0; y < h; y++) {
+        if (y + 2 < h) {
+            backup_row(cb.b[bak], p + (y + 2) * stride, left[y + 2], w, 2, edges);
+            cp[4] = bak;
+        } else if (edges & LR_HAVE_BOTTOM) {
+            backup_row_lpf(cb.b[bak], lpf_bottom + (y + 2 - h) * stride, w, 2, edges);
+            cp[4] = bak;
+        } else {
+            cp[4] = cp[3];
+        }
+        if (++bak == 5) bak = 0;
+        if (cp[4] == cp[3]) {
+            lp[4] = lp[3];
+        } else if (y + 2 == h) {
+            backup_row_luma(lb.b[lbak], luma_bottom, lstride, lw, edges, ss_hor, ss_ver, ds);
+            lp[4] = lbak;
+        } else if (y + 1 == h) {
+            backup_row_luma(lb.b[lbak], luma_bottom + lstride, 0, lw, edges, ss_hor, ss_ver, ds);
+            lp[4] = lbak;
+        } else {
+            backup_row_luma(lb.b[lbak], lpos + (2 << ss_ver) * lstride, lstride, lw, edges, ss_hor, ss_ver, ds);
+            lp[4] = lbak;
+        }
+        if (++lbak == 5) lbak = 0;
+        for (int bx = 0; bx < (w >> 2); bx++) {
+            if (ll[y >> 2][0] & (1 << bx)) continue;
+            for (int x = bx * 4; x < bx * 4 + 4; x++) {
+                int m = cb.b[cp[2]][ORIG + x];
+                int s = m << 7;
+                for (int i = 0; i < 6; i++) {
+                    int dy = WIENER_NS_CONFIG_UV[i][0], dx = WIENER_NS_CONFIG_UV[i][1];
+                    int a = cb.b[cp[2 + dy]][ORIG + x + dx], b = cb.b[cp[2 - dy]][ORIG + x - dx];
+                    s += (a + b - 2 * m) * filter[i];
+                }
+                int l = lb.b[lp[2]][ORIG + (x << ss_hor)];
+                for (int i = 0; i < 12; i++) {
+                    int dy = WIENER_NS_CONFIG_UV_FROM_Y[i][0], dx = WIENER_NS_CONFIG_UV_FROM_Y[i][1];
+                    int lx = ORIG + (x + dx) * (1 << ss_hor);
+                    s += ((int)lb.b[lp[2 + dy]][lx] - l) * filter[6 + i];
+                }
+                p[y * stride + x] = (P)iclip((s + 64) >> 7, 0, bdmax);
+            }
+        }
+        for (int r = 0; r < 4; r++) cp[r] = cp[r + 1];
+        for (int r = 0; r < 4; r++) lp[r] = lp[r + 1];
+        lpos += lstride << ss_ver;
+    }
+}
+
+/* looprestoration.rs:114-192 (scalar form) over a 13-row window */
+template <typename P>
+static void compute_gradient_row(uint16_t (*dst)[4], const P *const *rows, int row_off, int col, int w, int shift) {
+    static const int offs[4][2] = {{1, 0}, {0, 1}, {1, 1}, {-1, 1}};
+    for (int x1 = 0; x1 < w + 2; x1 += 2) {
+        for (int d = 0; d < 4; d++) {
+            int grad = 0;
+            for (int x2 = 0; x2 < 2; x2++) {
+                int x = col + x1 + x2;
+                for (int y = 0; y < 2; y++) {
+                    int dy = offs[d][0], dx = offs[d][1];
+                    int ry = row_off + y;
+                    int a = (int)rows[ry - 1 - dy][x - 1 - dx] >> shift;
+                    int b = (int)rows[ry - 1][x - 1] >> shift;
+                    int c = (int)rows[ry - 1 + dy][x - 1 + dx] >> shift;
+                    grad += abs(b * 2 - a - c);
+                }
+            }
+            dst[x1 >> 1][d] = (uint16_t)grad;
+        }
+    }
+}
+
+/* looprestoration.rs:1432-1643 */
+template <typename P>
+static void gdf_prep(int8_t *dst, int dst_stride, const P *p, ptrdiff_t stride, const P (*left)[6], const P *lpf,
+                     const P *lpf_bottom, int w, int h, int ref_dst_idx, int qp_idx, uint8_t edges, int bitdepth) {
+    const int down = bitdepth == 12 ? 2 : 0, up = bitdepth == 8 ? 2 : 0;
+    static thread_local RowBufs<P, 13> rb;
+    int ptrs[13] = {0};
+    backup_row(rb.b[6], p, left[0], w, 6, edges);
+    ptrs[6] = 6;
+    if (edges & LR_HAVE_TOP) {
+        backup_row_lpf(rb.b[4], lpf, w, 6, edges);
+        backup_row_lpf(rb.b[5], lpf + stride, w, 6, edges);
+        ptrs[4] = 4;
+        ptrs[5] = 5;
+        ptrs[0] = ptrs[1] = ptrs[2] = ptrs[3] = 4;
+    } else {
+        for (int n = 0; n < 6; n++) ptrs[n] = 6;
+    }
+    int bak = 7;
+    for (int y = 1; y < 6; y++) {
+        backup_row(rb.b[bak], p + y * stride, left[y], w, 6, edges);
+        ptrs[bak] = bak;
+        bak++;
+    }
+    int alpha_base = ref_dst_idx * 528 + qp_idx * 88;
+    int weight_base = ref_dst_idx * 1584 + qp_idx * 264;
+    int bias_idx = ref_dst_idx * 6 + qp_idx;
+    int err_base, scale;
+    if (ref_dst_idx == 0) {
+        err_base = qp_idx * 4096;
+        scale = 8;
+    } else {
+        err_base = (ref_dst_idx - 1) * 6000 + qp_idx * 1000;
+        scale = 5;
+    }
+    static thread_local uint16_t grad[2][33][4];
+    {
+        const P *rows[13];
+        for (int i = 0; i < 13; i++) rows[i] = rb.b[ptrs[i]];
+        compute_gradient_row(grad[0], rows, 6, ORIG, w, down);
+    }
+    int gbit = 1;
+    for (int y = 0; y < h; y++) {
+        if (y + 6 < h) {
+            backup_row(rb.b[bak], p + (y + 6) * stride, left[y + 6], w, 6, edges);
+            ptrs[12] = bak;
+        } else if (y + 4 < h && (edges & LR_HAVE_BOTTOM)) {
+            backup_row_lpf(rb.b[bak], lpf_bottom + (y + 6 - h) * stride, w, 6, edges);
+            ptrs[12] = bak;
+        } else {
+            ptrs[12] = ptrs[11];
+        }
+        if (++bak == 13) bak = 0;
+        if ((y & 1) == 0) {
+            const P *rows[13];
+            for (int i = 0; i < 13; i++) rows[i] = rb.b[ptrs[i]];
+            compute_gradient_row(grad[gbit], rows, 8, ORIG, w, down);
+            gbit ^= 1;
+        }
+        for (int x1 = 0; x1 < w; x1 += 2) {
+            int gs[4];
+            int hx = x1 >> 1;
+            for (int d = 0; d < 4; d++)
+                gs[d] = grad[0][hx][d] + grad[0][hx + 1][d] + grad[1][hx][d] + grad[1][hx + 1][d];
+            int cls = (gs[0] <= gs[1]) | ((gs[2] <= gs[3]) << 1);
+            int shared[3];
+            for (int i = 0; i < 3; i++) shared[i] = R2T_GDF_BIAS[bias_idx][i];
+            for (int d = 0; d < 4; d++) {
+                int k = d + 18;
+                int alpha = R2T_GDF_ALPHA[alpha_base + k * 4 + cls];
+                int v = imin(gs[d] >> (4 - up), alpha);
+                for (int i = 0; i < 3; i++) shared[i] += v * R2T_GDF_WEIGHT[weight_base + i * 88 + k * 4 + cls];
+            }
+            for (int x2 = 0; x2 < 2; x2++) {
+                int x = x1 + x2;
+                int iv[3] = {shared[0], shared[1], shared[2]};
+                int m = (int)rb.b[ptrs[6]][ORIG + x] >> down;
+                for (int k = 0; k < 18; k++) {
+                    int alpha = R2T_GDF_ALPHA[alpha_base + k * 4 + cls];
+                    int dy = GDF_COORDS[k][0], dx = GDF_COORDS[k][1];
+                    int a = (int)rb.b[ptrs[6 - dy]][ORIG + x - dx] >> down;
+                    int b = (int)rb.b[ptrs[6 + dy]][ORIG + x + dx] >> down;
+                    int above = iclip((a - m) * (1 << up), -alpha, alpha);
+                    int below = iclip((b - m) * (1 << up), -alpha, alpha);
+                    int v = iclip(above + below, -512, 511);
+                    for (int i = 0; i < 3; i++) iv[i] += v * R2T_GDF_WEIGHT[weight_base + i * 88 + k * 4 + cls];
+                }
+                int full = 0;
+                for (int i = 0; i < 3; i++) {
+                    int sv = iv[i] * scale;
+                    int a = sv < 0 ? -sv : sv;
+                    int v = apply_sign((a + (1 << 14)) >> 15, sv);
+                    int sub = iclip(v, -scale, scale - 1) + scale;
+                    full = full * (scale * 2) + sub;
+                }
+                dst[y * dst_stride + x] =
+                    ref_dst_idx == 0 ? R2T_GDF_INTRA_ERROR[err_base + full] : R2T_GDF_INTER_ERROR[err_base + full];
+            }
+        }
+        for (int r = 0; r < 12; r++) ptrs[r] = ptrs[r + 1];
+    }
+}
+
+/* looprestoration.rs:273-329 (scalar form) */
+template <typename P>
+static void gdf_add(P *p, ptrdiff_t stride, const int8_t *err, int err_stride, int w, int h, int scale,
+                    const uint16_t (*ll)[4], int bitdepth) {
+    int shift = 12 - bitdepth;
+    int rnd = (1 << shift) >> 1;
+    int bdmax = (1 << bitdepth) - 1;
+    for (int by = 0; by < (h >> 2); by++) {
+        uint16_t skip = ll[by][0];
+        for (int bx = 0; bx < (w >> 2); bx++) {
+            if (skip & (1 << bx)) continue;
+            for (int y = by * 4; y < by * 4 + 4; y++)
+                for (int x = bx * 4; x < bx * 4 + 4; x++) {
+                    int diff = err[y * err_stride + x] * scale;
+                    int a = diff < 0 ? -diff : diff;
+                    int v = (int)p[y * stride + x] + apply_sign((a + rnd) >> shift, diff);
+                    p[y * stride + x] = (P)iclip(v, 0, bdmax);
+                }
+        }
+    }
+}
+
+/* looprestoration.rs:1847-2274 (single tile: no integrated edges, no CDEF top line) */
+template <typename P>
+static void lr_stripe(const LrCtx<P> &c, P *plane, ptrdiff_t p_off, const P (*left)[6], int x, int y, int pl, int w,
+                      int row_h, const Rav2dRestorationUnit &lr, uint8_t edges, bool first_sby) {
+    const Rav2dFrameParams *fp = c.p;
+    const Geom &g = *c.g;
+    int chroma = pl != 0;
+    int ss_ver = chroma ? g.ss_ver : 0, ss_hor = chroma ? g.ss_hor : 0;
+    ptrdiff_t stride = chroma ? c.uvs : c.ys;
+    int sby = (y + (y ? (8 << ss_ver) : 0)) >> (6 - ss_ver + g.sb128);
+    int sb256x = (x << ss_hor) >> 8;
+    int sb64x_idx = ((x << ss_hor) >> 6) & 3;
+    int stripe_h = imin((64 - 8 * first_sby) >> ss_ver, row_h - y);
+    uint8_t wtype = (c.inloop & RAV2D_INLOOPFILTER_WIENER) ? lr.restoration_type : 0;
+    enum { K_NONE, K_SINGLE, K_MULTI, K_PC } kind = K_NONE;
+    bool multi = false;
+    const Rav2dRestorationPlane &pd = fp->restoration[pl];
+    if (wtype == RAV2D_RESTORATION_NS_WIENER) {
+        if (pd.frame_filters_on && pd.num_classes != 1) {
+            kind = K_MULTI;
+            multi = true;
+        } else {
+            kind = K_SINGLE;
+        }
+    } else if (wtype == RAV2D_RESTORATION_PC_WIENER) {
+        kind = K_PC;
+        multi = true;
+    }
+    uint16_t noskip[66] = {0};
+    if (multi) {
+        int r = 0;
+        for (int by = y >> 2; by < (row_h >> 2); by++, r++) {
+            int by_idx = by & 63;
+            size_t sbi = (size_t)g.sb256w * (by >> 6) + sb256x;
+            if (sbi < c.n_mask) {
+                noskip[r] = c.mask[sbi].lr_noskip_mask[by_idx][sb64x_idx];
+                if (!(edges & LR_HAVE_RIGHT) && (w & 63)) {
+                    int shift = ((w >> 2) & 15) - 1;
+                    uint16_t mv = noskip[r];
+                    uint16_t eb = (uint16_t)(mv >> shift);
+                    noskip[r] |= (uint16_t)(eb << (shift + 1));
+                }
+            }
+        }
+    }
+    static thread_local int8_t gdf_err[64 * 64];
+    int left_idx = 0, noskip_off = 0;
+    ptrdiff_t cur = p_off;
+    const P *db = c.lr_db[pl].data();
+    ptrdiff_t lpf_off = x, llpf_off = x * 2;
+    while (y + stripe_h <= row_h) {
+        bool bottom = ((sby + 1) != g.sbh) || (y + stripe_h != row_h);
+        edges = (uint8_t)((edges & ~LR_HAVE_BOTTOM) | (bottom ? LR_HAVE_BOTTOM : 0));
+        int inc = ((edges & LR_HAVE_TOP) && y + 8 < ((g.bh * 4) >> ss_ver)) ? 8 : 0;
+        size_t sbi = (size_t)g.sb256w * ((size_t)((y << ss_ver) + inc) >> 8) + sb256x;
+        bool gdf_on = pl == 0 && (c.inloop & RAV2D_INLOOPFILTER_GDF) && sbi < c.n_mask &&
+                      c.mask[sbi].gdf[(((y + inc) >> 4) & 12) + sb64x_idx] != 0;
+        if (gdf_on)
+            gdf_prep(gdf_err, 64, plane + cur, stride, left + left_idx, db + lpf_off, db + lpf_off + 6 * stride, w,
+                     stripe_h, fp->gdf_ref_dst_idx, fp->gdf_qp_idx, edges, g.bitdepth);
+        int y4 = ((y << ss_ver) & 255) >> 2;
+        int ll_rows = imax(stripe_h >> 2, 1) + 1;
+        std::vector<uint16_t[4]> llb(ll_rows);
+        for (auto &r : llb) memset(r, 0, sizeof(r));
+        if (sbi < c.n_mask) {
+            const Rav2dAv2Filter &m = c.mask[sbi];
+            if (pl == 0 || ss_hor == 0) {
+                const uint16_t(*src)[4] = pl == 0 ? m.lossless_mask_y : m.lossless_mask_uv;
+                for (int r = 0; r < ll_rows; r++) {
+                    int yy = y4 + r;
+                    if (yy < 64)
+                        for (int k = 0; k < 4; k++)
+                            if (sb64x_idx + k < 4) llb[r][k] = src[yy][sb64x_idx + k];
+                }
+            } else {
+                int init_y = y >> 2, end_y = (y + stripe_h) >> 2;
+                int base = y4 >> ss_ver;
+                for (int yy = init_y, r = 0; yy < end_y; yy++, r++) {
+                    int sy = base + (yy - init_y);
+                    if (sy < 64 && r < ll_rows) {
+                        uint16_t lo = m.lossless_mask_uv[sy][sb64x_idx];
+                        uint16_t hi = sb64x_idx + 1 < 4 ? m.lossless_mask_uv[sy][sb64x_idx + 1] : 0;
+                        llb[r][0] = (uint16_t)((lo & 0xff) | ((hi & 0xff) << 8));
+                    }
+                }
+            }
+        }
+        const uint16_t(*ll)[4] = llb.data();
+        const P *top = db + lpf_off, *bot = db + lpf_off + 6 * stride;
+        if (kind == K_SINGLE && !chroma) {
+            const int8_t *flt = pd.frame_filters_on ? pd.filter[0] : lr.ns_filter[0];
+            ns_wiener_single_y(plane + cur, stride, left + left_idx, top, bot, w, stripe_h, flt, edges, ll, g.bitdepth);
+        } else if (kind == K_SINGLE) {
+            const int8_t *flt = pd.frame_filters_on ? pd.filter[0] : lr.ns_filter[0];
+            const P *luma = c.luma_snapshot + (x << ss_hor) + (ptrdiff_t)(y << ss_ver) * c.ys;
+            const P *ldb = c.lr_db[0].data();
+            ns_wiener_single_uv(plane + cur, stride, left + left_idx, top, bot, w, stripe_h, flt, luma, c.ys,
+                                ldb + llpf_off, ldb + llpf_off + 6 * c.ys, ss_hor, ss_ver, fp->cfl_ds_filter_index,
+                                edges, ll, g.bitdepth);
+        } else if (kind == K_MULTI) {
+            wiener_multi(plane + cur, stride, left + left_idx, top, bot, w, stripe_h, pd.filter, (const int16_t(*)[13])0,
+                         c.ns_subclass_lut, noskip + noskip_off, c.base_q, edges, ll, g.bitdepth);
+            noskip_off += stripe_h >> 2;
+        } else if (kind == K_PC) {
+            wiener_multi(plane + cur, stride, left + left_idx, top, bot, w, stripe_h, (const int8_t(*)[18])0,
+                         c.pc_filters, c.pc_subclass_lut, noskip + noskip_off, c.base_q, edges, ll, g.bitdepth);
+            noskip_off += stripe_h >> 2;
+        }
+        if (gdf_on) gdf_add(plane + cur, stride, gdf_err, 64, w, stripe_h, fp->gdf_scale, ll, g.bitdepth);
+        left_idx += stripe_h;
+        cur += stripe_h * stride;
+        y += stripe_h;
+        edges |= LR_HAVE_TOP;
+        stripe_h = imin(64 >> ss_ver, row_h - y);
+        if (stripe_h == 0) break;
+        lpf_off += 4 * stride;
+        llpf_off += 4 * c.ys;
+    }
+}
+
+/* looprestoration.rs:2276-2399 */
+template <typename P>
+static void lr_sbrow_plane(const LrCtx<P> &c, P *plane, ptrdiff_t p_off, int y, int w, int h, int row_h, int pl,
+                           bool first_sby) {
+    const Geom &g = *c.g;
+    int chroma = pl != 0;
+    int ss_ver = chroma ? g.ss_ver : 0, ss_hor = chroma ? g.ss_hor : 0;
+    ptrdiff_t stride = chroma ? c.uvs : c.ys;
+    int usl = c.p->lr_unit_size[chroma ? 1 : 0];
+    int unit = 1 << usl, half = unit >> 1;
+    int row_y = y + ((8 >> ss_ver) * first_sby);
+    int shift_hor = 8 - ss_hor;
+    std::vector<P> border(2 * 264 * 6, 0);
+    auto B = [&](int bit) { return (P(*)[6])(border.data() + bit * 264 * 6); };
+    int aligned = row_y & ~(unit - 1);
+    if (aligned && aligned + half > h) aligned -= unit;
+    aligned <<= ss_ver;
+    int sb_idx = (aligned >> 8) * g.sb256w;
+    int unit_base = ((aligned >> 6) & 3) << 2;
+    uint8_t edges = (y > 0 ? LR_HAVE_TOP : 0) | LR_HAVE_RIGHT;
+    size_t lr_sb = (size_t)sb_idx, lr_u = (size_t)unit_base;
+    int bit = 0, x = 0;
+    ptrdiff_t cp = p_off;
+    while (x + 64 < w) {
+        int next_x = x + 64;
+        int nstart = next_x & ~(unit - 1);
+        if (nstart && w - nstart < half) nstart -= unit;
+        int next_u = unit_base + ((nstart >> (shift_hor - 2)) & 3);
+        int next_sb = sb_idx + (nstart >> shift_hor);
+        int n = chroma ? 2 : 6;
+        ptrdiff_t bsrc = cp + 64;
+        for (int row = 0; row < row_h - y; row++) {
+            ptrdiff_t sr = bsrc + row * stride;
+            for (int i = 0; i < n; i++) B(bit)[row][6 - n + i] = plane[sr - n + i];
+        }
+        if (lr_sb < c.n_lr && lr_u < 16)
+            lr_stripe(c, plane, cp, B(1 - bit), x, y, pl, 64, row_h, c.lr[lr_sb].lr[pl][lr_u], edges, first_sby);
+        lr_sb = (size_t)next_sb;
+        lr_u = (size_t)next_u;
+        x = next_x;
+        cp += 64;
+        edges |= LR_HAVE_LEFT;
+        bit ^= 1;
+    }
+    edges &= ~LR_HAVE_RIGHT;
+    if (lr_sb < c.n_lr && lr_u < 16)
+        lr_stripe(c, plane, cp, B(1 - bit), x, y, pl, w - x, row_h, c.lr[lr_sb].lr[pl][lr_u], edges, first_sby);
+}
+
+/* looprestoration.rs:2401-2548, single tile (start_of_tile_row = [1, 0, 0, ...]) */
+template <typename P> void lr_sbrow_apply(const LrCtx<P> &c, Frame<P> &f, int sby) {
+    const Geom &g = *c.g;
+    int rp = c.restore_planes;
+    int not_last = sby + 1 < g.sbh;
+    bool first = sby == 0;
+    if (rp & (LR_RESTORE_U | LR_RESTORE_V)) {
+        int h = (g.bh * 4) >> g.ss_ver, w = (g.bw * 4) >> g.ss_hor;
+        int next_row_y = (sby + 1) << ((6 - g.ss_ver) + g.sb128);
+        int row_h = imin(next_row_y - (8 >> g.ss_ver) * not_last, h);
+        int offset_uv = (8 * (sby != 0)) >> g.ss_ver;
+        int ys = (sby << ((6 - g.ss_ver) + g.sb128)) - offset_uv;
+        if (rp & LR_RESTORE_U) lr_sbrow_plane(c, f.u, (ptrdiff_t)imax(ys, 0) * f.uvs, ys, w, h, row_h, 1, first);
+        if (rp & LR_RESTORE_V) lr_sbrow_plane(c, f.v, (ptrdiff_t)imax(ys, 0) * f.uvs, ys, w, h, row_h, 2, first);
+    }
+    if (rp & LR_RESTORE_Y) {
+        int h = g.bh * 4, w = g.bw * 4;
+        int next_row_y = (sby + 1) << (6 + g.sb128);
+        int row_h = imin(next_row_y - 8 * not_last, h);
+        int offset_y = 8 * (sby != 0);
+        int ys = (sby << (6 + g.sb128)) - offset_y;
+        lr_sbrow_plane(c, f.y, (ptrdiff_t)imax(ys, 0) * f.ys, ys, w, h, row_h, 0, first);
+    }
+}
+
+/* deblock.rs:688-729 (n_tc == 1, lr_backup) */
+template <typename P>
+static void backup_db(P *dst, const P *src, ptrdiff_t stride, int ss_ver, int sb128, int row, int row_h, int w) {
+    int stripe_h = (64 - 8 * (row == 0)) >> ss_ver;
+    ptrdiff_t src_off = (ptrdiff_t)(stripe_h - 2) * stride;
+    ptrdiff_t dst_off = 0;
+    if (row > 0) {
+        int top = 4 << sb128;
+        for (int i = 0; i < 4; i++) memmove(dst + i * stride, dst + (top + i) * stride, sizeof(P) * w);
+    }
+    dst_off += 4 * stride;
+    while (row + stripe_h <= row_h) {
+        for (int i = 0; i < 4; i++) {
+            memcpy(dst + dst_off, src + src_off, sizeof(P) * w);
+            dst_off += stride;
+            src_off += stride;
+        }
+        row += stripe_h;
+        stripe_h = 64 >> ss_ver;
+        src_off += (ptrdiff_t)(stripe_h - 4) * stride;
+    }
+}
+
+/* deblock.rs:2192-2264 */
+template <typename P> void copy_db(LrCtx<P> &c, const Frame<P> &f, int sby) {
+    const Geom &g = *c.g;
+    int h = g.bh * 4, w = g.bw * 4;
+    int offset = 8 * (sby != 0);
+    int y_stripe = (sby << (6 + g.sb128)) - offset;
+    int row_h = imin((sby + 1) << (6 + g.sb128), h - 1);
+    if (y_stripe < row_h) backup_db(c.lr_db[0].data(), f.y + y_stripe * f.ys, f.ys, 0, g.sb128, y_stripe, row_h, w);
+    int cw = w >> g.ss_hor, ch = h >> g.ss_ver;
+    int ouv = offset >> g.ss_ver;
+    int cy = (sby << ((6 - g.ss_ver) + g.sb128)) - ouv;
+    int crow_h = imin((sby + 1) << ((6 - g.ss_ver) + g.sb128), ch - 1);
+    if (cy < crow_h) {
+        backup_db(c.lr_db[1].data(), f.u + cy * f.uvs, f.uvs, g.ss_ver, g.sb128, cy, crow_h, cw);
+        backup_db(c.lr_db[2].data(), f.v + cy * f.uvs, f.uvs, g.ss_ver, g.sb128, cy, crow_h, cw);
+    }
+}
+
+/* decode.rs:27-65: init_wiener + compute_restore_planes for a single frame */
+template <typename P> void lr_init(LrCtx<P> &c, const Rav2dFrameParams *p) {
+    c.base_q = 0;
+    int widx = 0;
+    int ns_idx = -1;
+    uint8_t rtype = p->restoration[0].restoration_type;
+    if (rtype != RAV2D_RESTORATION_NONE) {
+        c.base_q = dq_lookup(p->yac);
+        int q = p->yac;
+        widx = q < 130 ? 0 : q < 190 ? 1 : q < 220 ? 2 : 3;
+        if (rtype == RAV2D_RESTORATION_NS_WIENER || rtype == RAV2D_RESTORATION_SWITCHABLE) {
+            int nci = p->restoration[0].num_classes_idx;
+            if (nci > 0) ns_idx = nci - 1;
+        }
+    }
+    c.pc_subclass_lut = R2T_PC_WIENER_SUB_CLASSIFY[widx];
+    c.pc_filters = R2T_PC_WIENER_FILTERS[widx];
+    c.ns_subclass_lut = R2T_PC_WIENER_SUB_CLASSIFY_NS[widx][ns_idx >= 0 ? imin(ns_idx, 6) : 0];
+    int has_y = p->restoration[0].restoration_type != RAV2D_RESTORATION_NONE || p->gdf_enabled;
+    int has_u = p->restoration[1].restoration_type != RAV2D_RESTORATION_NONE;
+    int has_v = p->restoration[2].restoration_type != RAV2D_RESTORATION_NONE;
+    c.restore_planes = has_y | (has_u << 1) | (has_v << 2);
+}
+
+template void lr_init<uint8_t>(LrCtx<uint8_t> &, const Rav2dFrameParams *);
+template void lr_init<uint16_t>(LrCtx<uint16_t> &, const Rav2dFrameParams *);
+template void copy_db<uint8_t>(LrCtx<uint8_t> &, const Frame<uint8_t> &, int);
+template void copy_db<uint16_t>(LrCtx<uint16_t> &, const Frame<uint16_t> &, int);
+template void lr_sbrow_apply<uint8_t>(const LrCtx<uint8_t> &, Frame<uint8_t> &, int);
+template void lr_sbrow_apply<uint16_t>(const LrCtx<uint16_t> &, Frame<uint16_t> &, int);
+
+}  // namespace oracle

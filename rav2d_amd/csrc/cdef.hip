@@ -1,0 +1,333 @@
+// CDEF with CCSO on gfx950: one workgroup per 64x64 luma superblock (and its
+// two 32x32 chroma blocks), F2 -> F3.
+//
+// Arithmetic follows crates/rav2d/src/cdef.rs: cdef_find_dir_bd (:134-216),
+// adjust_strength (:226-236), constrain (:117-120), cdef_filter_block with its
+// padding (:37-128, 320-460) and the per-band/superblock/block decisions of
+// cdef_brow (:702-1182); CCSO from crates/rav2d/src/ccso.rs: ccso_prep
+// (:115-186), ccso_score (:16-24) and ccso_add (:215-254).
+//
+// The reference walks 8-row bands and keeps pre-filter copies of the rows
+// above (toggled line banks) and of the two columns to the left (2x8 backups),
+// so every tap it reads is a deblocked (F2) sample and every sample outside
+// the frame is the INT16_MIN "no edge" marker.  Here the whole superblock
+// plus a 2-sample halo of F2 is staged in LDS once, with INT16_MIN outside the
+// frame, and every output sample is computed from it directly.  CCSO reads
+// the same staged luma with coordinates clamped to the frame, which is what
+// ccso_padding's edge-dependent clamps amount to.
+#include "internal.h"
+#include "tables.h"
+
+namespace r2g {
+
+__constant__ int8_t kCdefDyDx[12][2][2];  // CDEF_DIRECTIONS as (dy, dx)
+__constant__ int8_t kCcsoOffset[4][8];
+
+__device__ __forceinline__ int constrain(int diff, int threshold, int shift) {
+    int adiff = abs(diff);
+    return apply_sign(dmin(adiff, dmax(0, threshold - (adiff >> shift))), diff);
+}
+
+__device__ __forceinline__ int adjust_strength(int strength, uint32_t var) {
+    if (var == 0) return 0;
+    int i = (var >> 6) ? dmin(ulog2(var >> 6), 12) : 0;
+    return (strength * (4 + i) + 8) >> 4;
+}
+
+__device__ __forceinline__ uint32_t ccso_score(int diff, int q, uint32_t edge_clf) {
+    if (diff > q && edge_clf == 0) return 2;
+    if (diff < -q) return 0;
+    return 1;
+}
+
+// Direction search of one 8x8 luma block staged at `img` (row stride `ls`).
+__device__ int find_dir(const int16_t *img, int ls, uint32_t *var, int bdm8) {
+    int hv0[8] = {0}, hv1[8] = {0}, d0[15] = {0}, d1[15] = {0};
+    int a0[11] = {0}, a1[11] = {0}, a2[11] = {0}, a3[11] = {0};
+#pragma unroll
+    for (int y = 0; y < 8; y++) {
+#pragma unroll
+        for (int x = 0; x < 8; x++) {
+            int px = ((int)img[y * ls + x] >> bdm8) - 128;
+            d0[y + x] += px;
+            a0[y + (x >> 1)] += px;
+            hv0[y] += px;
+            a1[3 + y - (x >> 1)] += px;
+            d1[7 + y - x] += px;
+            a2[3 - (y >> 1) + x] += px;
+            hv1[x] += px;
+            a3[(y >> 1) + x] += px;
+        }
+    }
+    auto sq = [](int v) -> uint32_t { return (uint32_t)v * (uint32_t)v; };
+    const uint32_t DIV[7] = {840, 420, 280, 210, 168, 140, 120};
+    uint32_t cost[8];
+    cost[2] = 0;
+    cost[6] = 0;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        cost[2] += sq(hv0[n]);
+        cost[6] += sq(hv1[n]);
+    }
+    cost[2] *= 105;
+    cost[6] *= 105;
+    cost[0] = 0;
+    cost[4] = 0;
+#pragma unroll
+    for (int n = 0; n < 7; n++) {
+        cost[0] += (sq(d0[n]) + sq(d0[14 - n])) * DIV[n];
+        cost[4] += (sq(d1[n]) + sq(d1[14 - n])) * DIV[n];
+    }
+    cost[0] += sq(d0[7]) * 105;
+    cost[4] += sq(d1[7]) * 105;
+    int *alts[4] = {a0, a1, a2, a3};
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        int *a = alts[n];
+        uint32_t c = 0;
+#pragma unroll
+        for (int m = 0; m < 5; m++) c += sq(a[3 + m]);
+        c *= 105;
+#pragma unroll
+        for (int m = 0; m < 3; m++) c += (sq(a[m]) + sq(a[10 - m])) * DIV[2 * m + 1];
+        cost[n * 2 + 1] = c;
+    }
+    int best_dir = 0;
+    uint32_t best = cost[0];
+#pragma unroll
+    for (int n = 1; n < 8; n++)
+        if (cost[n] > best) {
+            best = cost[n];
+            best_dir = n;
+        }
+    *var = (best - cost[best_dir ^ 4]) >> 10;
+    return best_dir;
+}
+
+// One output sample of cdef_filter_block: `t` points at the sample in an LDS
+// tile with row stride `ls` whose outside-frame entries hold INT16_MIN.
+__device__ __forceinline__ int cdef_px(const int16_t *t, int ls, int pri, int sec, int dir, int damping, int bdm8) {
+    int px = t[0];
+    // direction taps as (dy, dx): the reference's offsets into a 12-wide buffer
+    auto off = [&](int d, int k) { return kCdefDyDx[d][k][0] * ls + kCdefDyDx[d][k][1]; };
+    int sum = 0;
+    if (pri) {
+        int pri_tap = 4 - ((pri >> bdm8) & 1);
+        int pri_shift = dmax(0, damping - ulog2((uint32_t)pri));
+        if (sec) {
+            int sec_shift = damping - ulog2((uint32_t)sec);
+            int mx = px, mn = px, tk = pri_tap;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                int o1 = off(dir + 2, k);
+                int p0 = t[o1], p1 = t[-o1];
+                sum += tk * constrain(p0 - px, pri, pri_shift);
+                sum += tk * constrain(p1 - px, pri, pri_shift);
+                tk = (tk & 3) | 2;
+                mn = dmin(p0, mn);
+                mx = dmax(p0, mx);
+                mn = dmin(p1, mn);
+                mx = dmax(p1, mx);
+                int o2 = off(dir + 4, k), o3 = off(dir, k);
+                int s0 = t[o2], s1 = t[-o2], s2 = t[o3], s3 = t[-o3];
+                int st = 2 - k;
+                sum += st * constrain(s0 - px, sec, sec_shift);
+                sum += st * constrain(s1 - px, sec, sec_shift);
+                sum += st * constrain(s2 - px, sec, sec_shift);
+                sum += st * constrain(s3 - px, sec, sec_shift);
+                mn = dmin(s0, mn);
+                mx = dmax(s0, mx);
+                mn = dmin(s1, mn);
+                mx = dmax(s1, mx);
+                mn = dmin(s2, mn);
+                mx = dmax(s2, mx);
+                mn = dmin(s3, mn);
+                mx = dmax(s3, mx);
+            }
+            return iclip(px + ((sum - (sum < 0) + 8) >> 4), mn, mx);
+        }
+        int tk = pri_tap;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            int o1 = off(dir + 2, k);
+            sum += tk * constrain(t[o1] - px, pri, pri_shift);
+            sum += tk * constrain(t[-o1] - px, pri, pri_shift);
+            tk = (tk & 3) | 2;
+        }
+        return px + ((sum - (sum < 0) + 8) >> 4);
+    }
+    int sec_shift = damping - ulog2((uint32_t)sec);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        int o1 = off(dir + 4, k), o2 = off(dir, k);
+        int st = 2 - k;
+        sum += st * constrain(t[o1] - px, sec, sec_shift);
+        sum += st * constrain(t[-o1] - px, sec, sec_shift);
+        sum += st * constrain(t[o2] - px, sec, sec_shift);
+        sum += st * constrain(t[-o2] - px, sec, sec_shift);
+    }
+    return px + ((sum - (sum < 0) + 8) >> 4);
+}
+
+constexpr int kThreads = 256;
+constexpr int LY = 64 + 4, LC = 32 + 4;  // staged widths: 2-sample halo each side
+
+template <typename P>
+__global__ __launch_bounds__(kThreads) void cdef_kernel(CdefArgs a) {
+    __shared__ int16_t ly[LY * LY];
+    __shared__ int16_t lc[2][LC * LC];
+    __shared__ uint8_t b_state[64];   // bit0 luma filtered, bit1 chroma filtered
+    __shared__ uint8_t b_dir[64];
+    __shared__ int16_t b_ypri[64];    // adjusted luma primary strength
+    const int sbx = blockIdx.x, sby = blockIdx.y;
+    const int x0 = sbx * 64, y0 = sby * 64;
+    const int fw = a.g.bw * 4, fh = a.g.bh * 4;
+    const int cw = fw >> 1, ch = fh >> 1;
+    const int bdm8 = a.g.bitdepth - 8;
+    for (int i = threadIdx.x; i < LY * LY; i += kThreads) {
+        int r = i / LY, c = i - r * LY;
+        int gx = x0 - 2 + c, gy = y0 - 2 + r;
+        int16_t v = INT16_MIN;
+        if (gx >= 0 && gx < fw && gy >= 0 && gy < fh) v = (int16_t)row_ptr<const P>(a.src[0], gy)[gx];
+        ly[i] = v;
+    }
+    for (int i = threadIdx.x; i < 2 * LC * LC; i += kThreads) {
+        int pl = i >= LC * LC;
+        int j = i - pl * LC * LC;
+        int r = j / LC, c = j - r * LC;
+        int gx = (x0 >> 1) - 2 + c, gy = (y0 >> 1) - 2 + r;
+        int16_t v = INT16_MIN;
+        if (gx >= 0 && gx < cw && gy >= 0 && gy < ch) v = (int16_t)row_ptr<const P>(a.src[1 + pl], gy)[gx];
+        lc[pl][j] = v;
+    }
+    const Rav2dAv2Filter &m = a.mask[(sby >> 2) * a.g.sb256w + (sbx >> 2)];
+    const int sb64_idx = ((sby & 3) << 2) + (sbx & 3);
+    const int cdef_idx = m.cdef_idx[sb64_idx];
+    const bool active = a.cdef_on && cdef_idx >= 0 &&
+                        (a.y_strength[cdef_idx] != 0 || a.uv_strength[cdef_idx] != 0);
+    const int y_lvl = active ? a.y_strength[cdef_idx] : 0;
+    const int uv_lvl = active ? a.uv_strength[cdef_idx] : 0;
+    const int y_pri = (y_lvl >> 2) << bdm8;
+    int y_sec = y_lvl & 3;
+    y_sec += y_sec == 3;
+    y_sec <<= bdm8;
+    const int uv_pri = (uv_lvl >> 2) << bdm8;
+    int uv_sec = uv_lvl & 3;
+    uv_sec += uv_sec == 3;
+    uv_sec <<= bdm8;
+    const int damping = a.damping + bdm8;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        int bxi = threadIdx.x & 7, byi = threadIdx.x >> 3;
+        int bx = (x0 >> 2) + bxi * 2, by = (y0 >> 2) + byi * 2;  // 4px units
+        uint8_t st = 0, dir = 0;
+        int16_t ypri = 0;
+        if (active && bx < a.g.bw && by < a.g.bh) {
+            uint16_t noskip = a.on_skiptx ? 0xffff : m.noskip_mask[(by & 0x3e) >> 1][sbx & 3];
+            if (noskip & (3u << (bx & 14))) {
+                uint32_t var = 0;
+                int d = (y_pri || uv_pri) ? find_dir(&ly[(byi * 8 + 2) * LY + bxi * 8 + 2], LY, &var, bdm8) : 0;
+                if (y_pri) {
+                    int adj = adjust_strength(y_pri, var);
+                    if (adj || y_sec) st |= 1;
+                    ypri = (int16_t)adj;
+                } else if (y_sec) {
+                    st |= 1;
+                }
+                if (uv_lvl) st |= 2;
+                dir = (uint8_t)d;
+            }
+        }
+        b_state[threadIdx.x] = st;
+        b_dir[threadIdx.x] = dir;
+        b_ypri[threadIdx.x] = ypri;
+    }
+    __syncthreads();
+    // luma
+    const bool cc0 = a.ccso_on && m.ccso[0];
+    for (int i = threadIdx.x; i < 64 * 64; i += kThreads) {
+        int yy = i >> 6, xx = i & 63;
+        int gx = x0 + xx, gy = y0 + yy;
+        if (gx >= fw || gy >= fh) continue;
+        int bi = (yy >> 3) * 8 + (xx >> 3);
+        const int16_t *t = &ly[(yy + 2) * LY + xx + 2];
+        int v = t[0];
+        if (b_state[bi] & 1) {
+            int pri = y_pri ? b_ypri[bi] : 0;
+            int dir = y_pri ? b_dir[bi] : 0;
+            v = cdef_px(t, LY, pri, y_sec, dir, damping, bdm8);
+        }
+        if (cc0) {
+            const CcsoDev &c = a.ccso[0];
+            int cval = t[0];
+            int idx = (int)((uint32_t)cval >> c.shift);
+            if (!c.bo_only) {
+                int nx0 = iclip(gx + c.dx, 0, fw - 1) - x0 + 2, ny0 = iclip(gy + c.dy, 0, fh - 1) - y0 + 2;
+                int nx1 = iclip(gx - c.dx, 0, fw - 1) - x0 + 2, ny1 = iclip(gy - c.dy, 0, fh - 1) - y0 + 2;
+                uint32_t c0 = ccso_score(ly[ny0 * LY + nx0] - cval, c.quant_step, c.edge_clf);
+                uint32_t c1 = ccso_score(ly[ny1 * LY + nx1] - cval, c.quant_step, c.edge_clf);
+                idx |= (int)((c0 << 5) | (c1 << 3));
+                idx &= 0xff;
+            }
+            int packed = (idx >> 1) < 64 ? c.filter_off[idx >> 1] : 0;
+            v = iclip(v + kCcsoOffset[c.scale_idx][7 & (packed >> (4 * (idx & 1)))], 0, a.g.bdmax);
+        }
+        row_ptr<P>(a.dst[0], gy)[gx] = (P)v;
+    }
+    // chroma
+    for (int i = threadIdx.x; i < 2 * 32 * 32; i += kThreads) {
+        int pl = i >> 10;
+        int j = i & 1023;
+        int yy = j >> 5, xx = j & 31;
+        int gx = (x0 >> 1) + xx, gy = (y0 >> 1) + yy;
+        if (gx >= cw || gy >= ch) continue;
+        int bi = (yy >> 2) * 8 + (xx >> 2);
+        const int16_t *t = &lc[pl][(yy + 2) * LC + xx + 2];
+        int v = t[0];
+        if (b_state[bi] & 2) {
+            int uvdir = uv_pri ? b_dir[bi] : 0;
+            v = cdef_px(t, LC, uv_pri, uv_sec, uvdir, damping - 1, bdm8);
+        }
+        if (a.ccso_on && m.ccso[1 + pl]) {
+            const CcsoDev &c = a.ccso[1 + pl];
+            int lx = gx << 1, lyy = gy << 1;
+            int cval = ly[(lyy - y0 + 2) * LY + lx - x0 + 2];
+            int idx = (int)((uint32_t)cval >> c.shift);
+            if (!c.bo_only) {
+                int nx0 = iclip(lx + c.dx, 0, fw - 1) - x0 + 2, ny0 = iclip(lyy + c.dy, 0, fh - 1) - y0 + 2;
+                int nx1 = iclip(lx - c.dx, 0, fw - 1) - x0 + 2, ny1 = iclip(lyy - c.dy, 0, fh - 1) - y0 + 2;
+                uint32_t c0 = ccso_score(ly[ny0 * LY + nx0] - cval, c.quant_step, c.edge_clf);
+                uint32_t c1 = ccso_score(ly[ny1 * LY + nx1] - cval, c.quant_step, c.edge_clf);
+                idx |= (int)((c0 << 5) | (c1 << 3));
+                idx &= 0xff;
+            }
+            int packed = (idx >> 1) < 64 ? c.filter_off[idx >> 1] : 0;
+            v = iclip(v + kCcsoOffset[c.scale_idx][7 & (packed >> (4 * (idx & 1)))], 0, a.g.bdmax);
+        }
+        row_ptr<P>(a.dst[1 + pl], gy)[gx] = (P)v;
+    }
+}
+
+void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s) {
+    static bool consts = false;
+    if (!consts) {
+        int8_t dydx[12][2][2];
+        for (int d = 0; d < 12; d++)
+            for (int k = 0; k < 2; k++) {
+                int o = R2T_CDEF_DIRECTIONS[d][k];  // dy * 12 + dx with |dx| <= 2
+                int dx = ((o + 2) % 12 + 12) % 12 - 2;
+                dydx[d][k][0] = (int8_t)((o - dx) / 12);
+                dydx[d][k][1] = (int8_t)dx;
+            }
+        (void)hipMemcpyToSymbol(HIP_SYMBOL(kCdefDyDx), dydx, sizeof(dydx));
+        (void)hipMemcpyToSymbol(HIP_SYMBOL(kCcsoOffset), R2T_CCSO_OFFSET, sizeof(R2T_CCSO_OFFSET));
+        consts = true;
+    }
+    dim3 grid((a.g.bw * 4 + 63) / 64, (a.g.bh * 4 + 63) / 64);
+    if (bitdepth > 8)
+        hipLaunchKernelGGL(cdef_kernel<uint16_t>, grid, dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(cdef_kernel<uint8_t>, grid, dim3(kThreads), 0, s, a);
+}
+
+}  // namespace r2g

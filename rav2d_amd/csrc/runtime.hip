@@ -272,6 +272,45 @@ int run_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, hipStream_t s) {
         launch_deblock_h(a, f->g.bitdepth, s);
         HIP_TRY(hipGetLastError());
     }
+    // CDEF + CCSO (decode.rs:3858-3997)
+    const Rav2dFrameParams &P = f->params;
+    bool cdef_stage = P.seq_cdef && (P.inloop_filters & (RAV2D_INLOOPFILTER_CDEF | RAV2D_INLOOPFILTER_CCSO));
+    if (upto >= RAV2D_PASS_CDEF && cdef_stage) {
+        CdefArgs a;
+        memset(&a, 0, sizeof(a));
+        a.g = f->g;
+        a.mask = f->d_mask;
+        a.cdef_on = (P.inloop_filters & RAV2D_INLOOPFILTER_CDEF) != 0;
+        a.ccso_on = (P.inloop_filters & RAV2D_INLOOPFILTER_CCSO) &&
+                    (P.ccso[0].enabled || P.ccso[1].enabled || P.ccso[2].enabled);
+        a.on_skiptx = P.cdef_on_skiptx != 0;
+        a.damping = P.cdef_damping;
+        memcpy(a.y_strength, P.cdef_y_strength, 8);
+        memcpy(a.uv_strength, P.cdef_uv_strength, 8);
+        static const int8_t pos[7][2] = {{-1, 0}, {0, -1}, {-1, -1}, {-1, 1}, {-1, -2}, {1, -2}, {0, 2}};  // ccso.rs:5-13
+        for (int pl = 0; pl < 3; pl++) {
+            const Rav2dCcsoPlane &c = P.ccso[pl];
+            CcsoDev &d = a.ccso[pl];
+            d.shift = f->g.bitdepth - c.max_band_log2;
+            int ext = c.ext_filter_support < 7 ? c.ext_filter_support : 0;
+            d.dy = pos[ext][0];
+            d.dx = pos[ext][1];
+            d.quant_step = R2T_CCSO_QUANT_SZ[c.scale_idx & 3][c.quant_idx & 3];
+            d.edge_clf = c.edge_clf;
+            d.bo_only = c.bo_only;
+            d.scale_idx = c.scale_idx & 3;
+            memcpy(d.filter_off, c.filter_off, 64);
+        }
+        for (int pl = 0; pl < 3; pl++) {
+            int out = next_tmp(cur[pl]);
+            a.src[pl] = plane_of(f, cur[pl], pl);
+            a.dst[pl] = plane_of(f, out, pl);
+            cur[pl] = out;
+        }
+        PassTimer pt(ctx, RAV2D_PASS_CDEF, s);
+        launch_cdef(a, f->g.bitdepth, s);
+        HIP_TRY(hipGetLastError());
+    }
     for (int pl = 0; pl < 3; pl++) {
         int r = copy_plane(f, cur[pl], BUF_OUT, pl, s);
         if (r) return r;
