@@ -19,6 +19,12 @@ int oracle_filter_frame(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask, s
                         const Rav2dAv2Restoration *lr, size_t n_lr, void *y, void *u, void *v,
                         ptrdiff_t y_stride, ptrdiff_t uv_stride, uint32_t inloop_mask);
 
+/* Film grain on the output copy: dst = src with grain (planes of the display
+ * copy), as apply_grain_to_picture_mt; a plain copy when the frame has none. */
+int oracle_apply_grain(const Rav2dFrameParams *p, const void *src_y, const void *src_u, const void *src_v,
+                       void *dst_y, void *dst_u, void *dst_v, ptrdiff_t y_stride, ptrdiff_t uv_stride);
+int oracle_picture_has_grain(const Rav2dFrameParams *p);
+
 /* Known-answer hooks into the restated helpers. */
 int oracle_dq_lookup(int qidx);
 uint32_t oracle_deblock_quant_thr(int hbd, int qidx);

@@ -49,6 +49,65 @@ struct CdefArgs {
 
 void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s);
 
+// Loop restoration: one work item per (plane, stripe, 64-column unit), built
+// on the host from the stripe/unit walk of lr_sbrow_apply / lr_sbrow_plane /
+// lr_stripe (crates/rav2d/src/looprestoration.rs:1847-2548).
+enum { LR_KIND_NONE = 0, LR_KIND_SINGLE = 1, LR_KIND_MULTI = 2, LR_KIND_PC = 3 };
+enum { LRE_LEFT = 1, LRE_RIGHT = 2, LRE_TOP = 4, LRE_BOTTOM = 8 };
+
+struct LrItem {
+    int16_t plane, x0, w, y0, h;  // stripe rows [y0, y0+h) x unit columns [x0, x0+w)
+    uint8_t edges, kind, gdf_on, pad;
+    int8_t filt[18];              // single-filter taps (luma 16 / chroma 18)
+    uint16_t noskip[17];          // lr_noskip rows of the stripe (multi-class kinds)
+    uint16_t ll[17];              // lossless rows of the stripe, bit per 4x4 column
+};
+
+// Device copies of the loop-restoration / GDF tables (tables.h).
+struct LrTables {
+    const uint8_t *lut_to_class;    // PC_WIENER_LUT_TO_CLASS[4096]
+    const int16_t *pc_filters;      // PC_WIENER_FILTERS[4][64][13]
+    const uint8_t *pc_sub;          // PC_WIENER_SUB_CLASSIFY[4][256]
+    const uint8_t *ns_sub;          // PC_WIENER_SUB_CLASSIFY_NS[4][7][256]
+    const uint16_t *gdf_alpha;      // GDF_ALPHA[3168]
+    const int16_t *gdf_weight;      // GDF_WEIGHT[9504]
+    const int16_t *gdf_bias;        // GDF_BIAS[36][3]
+    const int8_t *gdf_intra;        // GDF_INTRA_ERROR[24576]
+    const int8_t *gdf_inter;        // GDF_INTER_ERROR[30000]
+};
+
+struct LrArgs {
+    DevPlane f3[3], f2[3], dst[3];
+    DevGeom g;
+    const LrItem *items;
+    LrTables t;
+    int32_t cfl_ds;
+    int32_t base_q;
+    int32_t widx, ns_idx;
+    int32_t gdf_ref, gdf_qp, gdf_scale;
+    int8_t mfilt[3][16][18];  // frame filters of each plane (multi-class NS)
+};
+
+void launch_lr(const LrArgs &a, int n_items, int bitdepth, hipStream_t s);
+int lr_tables(int device, LrTables *out);
+
+// Film grain (filmgrain.rs): LUTs, scaling and block offsets live in a
+// per-frame device scratch area written by the setup kernel.
+struct GrainArgs {
+    DevPlane src[3], dst[3];
+    Rav2dFilmGrainData fg;
+    uint32_t seed;
+    int32_t bitdepth;
+    int16_t *lut;        // [3][73][82]
+    uint8_t *scaling;    // [3][1024]
+    uint8_t *offsets;    // [n_bands][n_blocks][2]
+    int32_t n_bands, n_blocks;
+    int32_t plane_on[3];
+    TileMap map;
+};
+
+void launch_grain(GrainArgs a, int bitdepth, hipStream_t s);
+
 // Which buffer a pass reads and writes, per plane.
 void launch_deblock_v(const DeblockArgs &a, int bitdepth, hipStream_t s);
 void launch_deblock_h(const DeblockArgs &a, int bitdepth, hipStream_t s);

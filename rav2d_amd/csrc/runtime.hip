@@ -102,6 +102,15 @@ struct Rav2dGpuFrame {
     DbLut *d_dblut = nullptr;
     size_t bytes = 0;
     bool side_ok = false;
+    // loop-restoration work list (rebuilt when params or side info change)
+    std::vector<Rav2dAv2Filter> h_mask;  // host copy (after the bottom crop)
+    std::vector<Rav2dAv2Restoration> h_lr;
+    std::vector<LrItem> lr_items;
+    LrItem *d_lr_items = nullptr;
+    size_t d_lr_cap = 0;
+    int lr_restore_planes = 0, lr_base_q = 0, lr_widx = 0, lr_ns_idx = 0;
+    bool lr_dirty = true;
+    uint8_t *d_grain = nullptr;  // grain LUTs + scaling + offsets scratch
 };
 
 namespace {
@@ -164,6 +173,211 @@ void crop_bottom(std::vector<Rav2dAv2Filter> &mask, const DevGeom &g) {
             }
         }
     }
+}
+
+// init_wiener + compute_restore_planes (decode.rs:27-65) for a single frame.
+void lr_frame_state(const Rav2dFrameParams &p, int &base_q, int &widx, int &ns_idx, int &restore_planes) {
+    base_q = 0;
+    widx = 0;
+    ns_idx = 0;
+    uint8_t rt = p.restoration[0].restoration_type;
+    if (rt != RAV2D_RESTORATION_NONE) {
+        int q = std::min(std::max(p.yac, 0), 255);
+        base_q = q == 0 ? 64 : ((int)R2T_DQ_LOOKUP[(q - 1) % 24] << ((q - 1) / 24));
+        widx = p.yac < 130 ? 0 : p.yac < 190 ? 1 : p.yac < 220 ? 2 : 3;
+        if ((rt == RAV2D_RESTORATION_NS_WIENER || rt == RAV2D_RESTORATION_SWITCHABLE) &&
+            p.restoration[0].num_classes_idx > 0)
+            ns_idx = std::min(p.restoration[0].num_classes_idx - 1, 6);
+    }
+    int has_y = p.restoration[0].restoration_type != RAV2D_RESTORATION_NONE || p.gdf_enabled;
+    int has_u = p.restoration[1].restoration_type != RAV2D_RESTORATION_NONE;
+    int has_v = p.restoration[2].restoration_type != RAV2D_RESTORATION_NONE;
+    restore_planes = has_y | (has_u << 1) | (has_v << 2);
+}
+
+// The stripe walk of lr_stripe (looprestoration.rs:1847-2274) for one unit
+// column, emitting one work item per stripe.
+void lr_column_items(const Rav2dGpuFrame *f, int pl, int x, int y, int w, int row_h, const Rav2dRestorationUnit *unit,
+                     uint8_t edges, bool first_sby, std::vector<LrItem> &out) {
+    const Rav2dFrameParams &p = f->params;
+    const DevGeom &g = f->g;
+    const int chroma = pl != 0, ss_ver = chroma ? g.ss_ver : 0, ss_hor = chroma ? g.ss_hor : 0;
+    const int sbh = (g.bh + (16 << g.sb128) - 1) >> (4 + g.sb128);
+    const int sby = (y + (y ? (8 << ss_ver) : 0)) >> (6 - ss_ver + g.sb128);
+    const int sb256x = (x << ss_hor) >> 8, sb64x_idx = ((x << ss_hor) >> 6) & 3;
+    int stripe_h = std::min((64 - 8 * (first_sby ? 1 : 0)) >> ss_ver, row_h - y);
+    const uint32_t inloop = p.inloop_filters;
+    uint8_t wtype = (inloop & RAV2D_INLOOPFILTER_WIENER) && unit ? unit->restoration_type : 0;
+    const Rav2dRestorationPlane &pd = p.restoration[pl];
+    int kind = LR_KIND_NONE;
+    if (wtype == RAV2D_RESTORATION_NS_WIENER)
+        kind = (pd.frame_filters_on && pd.num_classes != 1) ? LR_KIND_MULTI : LR_KIND_SINGLE;
+    else if (wtype == RAV2D_RESTORATION_PC_WIENER)
+        kind = LR_KIND_PC;
+    const bool multi = kind == LR_KIND_MULTI || kind == LR_KIND_PC;
+    uint16_t noskip[66] = {0};
+    if (multi) {
+        int r = 0;
+        for (int by = y >> 2; by < (row_h >> 2) && r < 66; by++, r++) {
+            size_t sbi = (size_t)g.sb256w * (by >> 6) + sb256x;
+            if (sbi < f->h_mask.size()) {
+                noskip[r] = f->h_mask[sbi].lr_noskip_mask[by & 63][sb64x_idx];
+                if (!(edges & LRE_RIGHT) && (w & 63)) {
+                    int shift = ((w >> 2) & 15) - 1;
+                    uint16_t eb = (uint16_t)(noskip[r] >> shift);
+                    noskip[r] |= (uint16_t)(eb << (shift + 1));
+                }
+            }
+        }
+    }
+    int noskip_off = 0;
+    const int hp = (g.bh * 4) >> ss_ver;
+    while (y + stripe_h <= row_h) {
+        bool bottom = (sby + 1) != sbh || (y + stripe_h != row_h);
+        edges = (uint8_t)((edges & ~LRE_BOTTOM) | (bottom ? LRE_BOTTOM : 0));
+        int inc = ((edges & LRE_TOP) && y + 8 < hp) ? 8 : 0;
+        size_t sbi = (size_t)g.sb256w * ((size_t)((y << ss_ver) + inc) >> 8) + sb256x;
+        LrItem it;
+        memset(&it, 0, sizeof(it));
+        it.plane = (int16_t)pl;
+        it.x0 = (int16_t)x;
+        it.w = (int16_t)w;
+        it.y0 = (int16_t)y;
+        it.h = (int16_t)stripe_h;
+        it.edges = edges;
+        it.kind = (uint8_t)kind;
+        it.gdf_on = pl == 0 && (inloop & RAV2D_INLOOPFILTER_GDF) && sbi < f->h_mask.size() &&
+                    f->h_mask[sbi].gdf[(((y + inc) >> 4) & 12) + sb64x_idx] != 0;
+        if (kind == LR_KIND_SINGLE) {
+            const int8_t *src = pd.frame_filters_on ? pd.filter[0] : unit->ns_filter[0];
+            memcpy(it.filt, src, chroma ? 18 : 16);
+        }
+        if (multi)
+            for (int r = 0; r < 17 && noskip_off + r < 66; r++) it.noskip[r] = noskip[noskip_off + r];
+        // lossless rows (looprestoration.rs:2029-2074)
+        int y4 = ((y << ss_ver) & 255) >> 2;
+        int ll_rows = std::max(stripe_h >> 2, 1) + 1;
+        if (sbi < f->h_mask.size()) {
+            const Rav2dAv2Filter &m = f->h_mask[sbi];
+            if (pl == 0 || ss_hor == 0) {
+                const uint16_t(*src)[4] = pl == 0 ? m.lossless_mask_y : m.lossless_mask_uv;
+                for (int r = 0; r < ll_rows && r < 17; r++)
+                    if (y4 + r < 64) it.ll[r] = src[y4 + r][sb64x_idx];
+            } else {
+                int init_y = y >> 2, end_y = (y + stripe_h) >> 2, base = y4 >> ss_ver;
+                for (int yy = init_y, r = 0; yy < end_y; yy++, r++) {
+                    int sy = base + (yy - init_y);
+                    if (sy < 64 && r < ll_rows && r < 17) {
+                        uint16_t lo = m.lossless_mask_uv[sy][sb64x_idx];
+                        uint16_t hi = sb64x_idx + 1 < 4 ? m.lossless_mask_uv[sy][sb64x_idx + 1] : 0;
+                        it.ll[r] = (uint16_t)((lo & 0xff) | ((hi & 0xff) << 8));
+                    }
+                }
+            }
+        }
+        out.push_back(it);
+        if (multi) noskip_off += stripe_h >> 2;
+        y += stripe_h;
+        edges |= LRE_TOP;
+        stripe_h = std::min(64 >> ss_ver, row_h - y);
+        if (stripe_h == 0) break;
+    }
+}
+
+// lr_sbrow_plane (looprestoration.rs:2276-2399): unit selection per 64 columns.
+void lr_plane_items(const Rav2dGpuFrame *f, int pl, int y, int w, int h, int row_h, bool first_sby,
+                    std::vector<LrItem> &out) {
+    const DevGeom &g = f->g;
+    const int chroma = pl != 0, ss_ver = chroma ? g.ss_ver : 0, ss_hor = chroma ? g.ss_hor : 0;
+    int usl = f->params.lr_unit_size[chroma ? 1 : 0];
+    int unit = 1 << usl, half = unit >> 1;
+    int row_y = y + ((8 >> ss_ver) * (first_sby ? 1 : 0));
+    int shift_hor = 8 - ss_hor;
+    int aligned = row_y & ~(unit - 1);
+    if (aligned && aligned + half > h) aligned -= unit;
+    aligned <<= ss_ver;
+    int sb_idx = (aligned >> 8) * g.sb256w;
+    int unit_base = ((aligned >> 6) & 3) << 2;
+    uint8_t edges = (uint8_t)((y > 0 ? LRE_TOP : 0) | LRE_RIGHT);
+    size_t lr_sb = (size_t)sb_idx, lr_u = (size_t)unit_base;
+    int x = 0;
+    auto unit_at = [&](size_t sbi, size_t u) -> const Rav2dRestorationUnit * {
+        if (sbi < f->h_lr.size() && u < 16) return &f->h_lr[sbi].lr[pl][u];
+        return nullptr;
+    };
+    auto emit = [&](int cx, int cw, uint8_t e) {
+        const Rav2dRestorationUnit *ru = unit_at(lr_sb, lr_u);
+        if (ru) {
+            lr_column_items(f, pl, cx, y, cw, row_h, ru, e, first_sby, out);
+        } else {
+            // the reference skips the column: output equals input
+            LrItem it;
+            memset(&it, 0, sizeof(it));
+            it.plane = (int16_t)pl;
+            it.x0 = (int16_t)cx;
+            it.w = (int16_t)cw;
+            it.y0 = (int16_t)y;
+            it.h = (int16_t)(row_h - y);
+            out.push_back(it);
+        }
+    };
+    while (x + 64 < w) {
+        int next_x = x + 64;
+        int nstart = next_x & ~(unit - 1);
+        if (nstart && w - nstart < half) nstart -= unit;
+        int next_u = unit_base + ((nstart >> (shift_hor - 2)) & 3);
+        int next_sb = sb_idx + (nstart >> shift_hor);
+        emit(x, 64, edges);
+        lr_sb = (size_t)next_sb;
+        lr_u = (size_t)next_u;
+        x = next_x;
+        edges |= LRE_LEFT;
+    }
+    edges &= ~LRE_RIGHT;
+    emit(x, w - x, edges);
+}
+
+// lr_sbrow_apply (looprestoration.rs:2401-2548) over every superblock row.
+void build_lr_items(Rav2dGpuFrame *f) {
+    const DevGeom &g = f->g;
+    lr_frame_state(f->params, f->lr_base_q, f->lr_widx, f->lr_ns_idx, f->lr_restore_planes);
+    f->lr_items.clear();
+    const int rp = f->lr_restore_planes;
+    const int sbh = (g.bh + (16 << g.sb128) - 1) >> (4 + g.sb128);
+    for (int sby = 0; sby < sbh; sby++) {
+        int not_last = sby + 1 < sbh;
+        bool first = sby == 0;
+        for (int pl = 1; pl < 3; pl++) {
+            if (!(rp & (1 << pl))) continue;
+            int h = (g.bh * 4) >> g.ss_ver, w = (g.bw * 4) >> g.ss_hor;
+            int next_row_y = (sby + 1) << ((6 - g.ss_ver) + g.sb128);
+            int row_h = std::min(next_row_y - (8 >> g.ss_ver) * not_last, h);
+            int off = (8 * (sby != 0)) >> g.ss_ver;
+            int ys = (sby << ((6 - g.ss_ver) + g.sb128)) - off;
+            lr_plane_items(f, pl, ys, w, h, row_h, first, f->lr_items);
+        }
+        if (rp & 1) {
+            int h = g.bh * 4, w = g.bw * 4;
+            int next_row_y = (sby + 1) << (6 + g.sb128);
+            int row_h = std::min(next_row_y - 8 * not_last, h);
+            int ys = (sby << (6 + g.sb128)) - 8 * (sby != 0);
+            lr_plane_items(f, 0, ys, w, h, row_h, first, f->lr_items);
+        }
+    }
+}
+
+int upload_lr_items(Rav2dGpuFrame *f) {
+    size_t n = f->lr_items.size();
+    if (n > f->d_lr_cap) {
+        if (f->d_lr_items) (void)hipFree(f->d_lr_items);
+        f->d_lr_items = nullptr;
+        size_t cap = n + n / 4 + 16;
+        HIP_TRY(hipMalloc((void **)&f->d_lr_items, cap * sizeof(LrItem)));
+        f->d_lr_cap = cap;
+    }
+    if (n) HIP_TRY(hipMemcpy(f->d_lr_items, f->lr_items.data(), n * sizeof(LrItem), hipMemcpyHostToDevice));
+    f->lr_dirty = false;
+    return RAV2D_OK;
 }
 
 int validate(const Rav2dFrameParams *p) {
@@ -272,6 +486,7 @@ int run_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, hipStream_t s) {
         launch_deblock_h(a, f->g.bitdepth, s);
         HIP_TRY(hipGetLastError());
     }
+    int f2buf[3] = {cur[0], cur[1], cur[2]};  // deblocked planes, read by LR at stripe edges
     // CDEF + CCSO (decode.rs:3858-3997)
     const Rav2dFrameParams &P = f->params;
     bool cdef_stage = P.seq_cdef && (P.inloop_filters & (RAV2D_INLOOPFILTER_CDEF | RAV2D_INLOOPFILTER_CCSO));
@@ -311,7 +526,41 @@ int run_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, hipStream_t s) {
         launch_cdef(a, f->g.bitdepth, s);
         HIP_TRY(hipGetLastError());
     }
+    // loop restoration (decode.rs:3999-4057), written straight into OUT
+    if (f->lr_dirty) {
+        build_lr_items(f);
+        int r = upload_lr_items(f);
+        if (r) return r;
+    }
+    bool lr_stage = f->lr_restore_planes && (P.inloop_filters & (RAV2D_INLOOPFILTER_WIENER | RAV2D_INLOOPFILTER_GDF));
+    if (upto >= RAV2D_PASS_LR && lr_stage && !f->lr_items.empty()) {
+        LrArgs a;
+        memset(&a, 0, sizeof(a));
+        a.g = f->g;
+        a.items = f->d_lr_items;
+        int r = lr_tables(ctx->device, &a.t);
+        if (r) return r;
+        a.cfl_ds = P.cfl_ds_filter_index;
+        a.base_q = f->lr_base_q;
+        a.widx = f->lr_widx;
+        a.ns_idx = f->lr_ns_idx;
+        a.gdf_ref = P.gdf_ref_dst_idx;
+        a.gdf_qp = P.gdf_qp_idx;
+        a.gdf_scale = P.gdf_scale;
+        for (int pl = 0; pl < 3; pl++) {
+            memcpy(a.mfilt[pl], P.restoration[pl].filter, sizeof(a.mfilt[pl]));
+            a.f3[pl] = plane_of(f, cur[pl], pl);
+            a.f2[pl] = plane_of(f, f2buf[pl], pl);
+            a.dst[pl] = plane_of(f, BUF_OUT, pl);
+        }
+        PassTimer pt(ctx, RAV2D_PASS_LR, s);
+        launch_lr(a, (int)f->lr_items.size(), f->g.bitdepth, s);
+        HIP_TRY(hipGetLastError());
+        for (int pl = 0; pl < 3; pl++)
+            if (f->lr_restore_planes & (1 << pl)) cur[pl] = BUF_OUT;
+    }
     for (int pl = 0; pl < 3; pl++) {
+        if (cur[pl] == BUF_OUT) continue;
         int r = copy_plane(f, cur[pl], BUF_OUT, pl, s);
         if (r) return r;
     }
@@ -424,7 +673,9 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
         f->plane_bytes[pl] = (size_t)s * (f->ph[pl] + 8);
         total += f->plane_bytes[pl];
     }
-    size_t need = total * NBUF + sizeof(DbLut);
+    const size_t grain_bytes = 3 * 73 * 82 * 2 + 3 * 1024 +
+                               (size_t)((params->height + 15) / 16) * ((params->width + 15) / 16) * 2 + 256;
+    size_t need = total * NBUF + sizeof(DbLut) + grain_bytes;
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         if (ctx->budget && ctx->in_use + need > ctx->budget) {
@@ -436,8 +687,10 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
     }
     f->bytes = need;
     if (hipMalloc((void **)&f->block, total * NBUF) != hipSuccess ||
-        hipMalloc((void **)&f->d_dblut, sizeof(DbLut)) != hipSuccess) {
+        hipMalloc((void **)&f->d_dblut, sizeof(DbLut)) != hipSuccess ||
+        hipMalloc((void **)&f->d_grain, grain_bytes) != hipSuccess) {
         if (f->block) (void)hipFree(f->block);
+        if (f->d_dblut) (void)hipFree(f->d_dblut);
         {
             std::lock_guard<std::mutex> lk(ctx->mu);
             ctx->in_use -= need;
@@ -466,8 +719,10 @@ int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(f->block);
     (void)hipFree(f->d_dblut);
+    if (f->d_grain) (void)hipFree(f->d_grain);
     if (f->d_mask) (void)hipFree(f->d_mask);
     if (f->d_lr) (void)hipFree(f->d_lr);
+    if (f->d_lr_items) (void)hipFree(f->d_lr_items);
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->in_use -= f->bytes;
@@ -487,6 +742,7 @@ int rav2d_gpu_frame_set_params(Rav2dGpuFrame *f, const Rav2dFrameParams *params)
         return RAV2D_EINVAL;
     }
     f->params = *params;
+    f->lr_dirty = true;
     DbLut lut;
     build_db_lut(*params, f->g.hbd, lut);
     HIP_TRY(hipSetDevice(f->ctx->device));
@@ -513,6 +769,12 @@ int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, 
     std::vector<Rav2dAv2Filter> copy(mask, mask + n_mask);
     crop_bottom(copy, f->g);
     HIP_TRY(hipMemcpyAsync(f->d_mask, copy.data(), need * sizeof(Rav2dAv2Filter), hipMemcpyHostToDevice, s));
+    f->h_mask.swap(copy);
+    if (lr)
+        f->h_lr.assign(lr, lr + n_lr);
+    else
+        f->h_lr.clear();
+    f->lr_dirty = true;
     if (lr) {
         if (!f->d_lr) {
             HIP_TRY(hipMalloc((void **)&f->d_lr, need * sizeof(Rav2dAv2Restoration)));
@@ -522,7 +784,7 @@ int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, 
         }
         HIP_TRY(hipMemcpyAsync(f->d_lr, lr, need * sizeof(Rav2dAv2Restoration), hipMemcpyHostToDevice, s));
     }
-    HIP_TRY(hipStreamSynchronize(s));  // `copy` is a pageable temporary
+    HIP_TRY(hipStreamSynchronize(s));  // host copies are pageable
     f->side_ok = true;
     return RAV2D_OK;
 }
@@ -571,10 +833,53 @@ int rav2d_gpu_apply_grain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream) {
     if (!ctx || !f || f->ctx != ctx) return RAV2D_EINVAL;
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = pick_stream(ctx, stream);
-    for (int pl = 0; pl < 3; pl++) {
-        int r = copy_plane(f, BUF_OUT, BUF_DISP, pl, s);
-        if (r) return r;
+    const Rav2dFrameParams &P = f->params;
+    const Rav2dFilmGrainData &d = P.grain;
+    // picture_has_grain (decode.rs:4641-4651)
+    bool has = P.grain_present && (d.num_points[0] || d.num_points[1] || d.num_points[2] ||
+                                   (d.clip_to_restricted_range && d.chroma_scaling_from_luma));
+    if (!has) {
+        for (int pl = 0; pl < 3; pl++) {
+            int r = copy_plane(f, BUF_OUT, BUF_DISP, pl, s);
+            if (r) return r;
+        }
+        return RAV2D_OK;
     }
+    if (d.ar_coeff_lag < 0 || d.ar_coeff_lag > 3 || d.block_size < 0 || d.block_size > 1 || d.scaling_shift < 8 ||
+        d.scaling_shift > 11 || d.ar_coeff_shift < 6 || d.ar_coeff_shift > 9 || d.grain_scale_shift < 0 ||
+        d.grain_scale_shift > 3) {
+        set_last_error("film grain parameters out of range");
+        return RAV2D_EINVAL;
+    }
+    for (int pl = 0; pl < 3; pl++)
+        for (int i = 0; i < d.num_points[pl] && i < 14; i++)
+            if (i && d.points[pl][i][0] < d.points[pl][i - 1][0]) {
+                set_last_error("film grain scaling points must not decrease");
+                return RAV2D_EINVAL;
+            }
+    GrainArgs a;
+    memset(&a, 0, sizeof(a));
+    a.fg = d;
+    a.seed = P.grain_seed;
+    a.bitdepth = f->g.bitdepth;
+    a.lut = (int16_t *)f->d_grain;
+    a.scaling = f->d_grain + 3 * 73 * 82 * 2;
+    a.offsets = a.scaling + 3 * 1024;
+    int bs = 16 << d.block_size;
+    a.n_bands = (P.height + bs - 1) / bs;
+    a.n_blocks = (P.width + bs - 1) / bs;
+    // apply_grain_row (filmgrain.rs:713, 744-747): a plane is grained only when
+    // its scaling LUT exists; chroma-from-luma uses the luma LUT
+    a.plane_on[0] = d.num_points[0] > 0;
+    for (int uv = 1; uv < 3; uv++)
+        a.plane_on[uv] = d.chroma_scaling_from_luma ? d.num_points[0] > 0 : d.num_points[uv] > 0;
+    for (int pl = 0; pl < 3; pl++) {
+        a.src[pl] = plane_of(f, BUF_OUT, pl);
+        a.dst[pl] = plane_of(f, BUF_DISP, pl);
+    }
+    PassTimer pt(ctx, RAV2D_PASS_GRAIN, s);
+    launch_grain(a, f->g.bitdepth, s);
+    HIP_TRY(hipGetLastError());
     return RAV2D_OK;
 }
 

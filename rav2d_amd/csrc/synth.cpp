@@ -188,7 +188,8 @@ void gen_params(Rng &r, R2SFrame *f) {
         rp.restoration_type = (uint8_t)(pl == 0 ? RAV2D_RESTORATION_SWITCHABLE : RAV2D_RESTORATION_NS_WIENER);
         if (rp.restoration_type >= RAV2D_RESTORATION_NS_WIENER) {
             rp.frame_filters_on = (uint8_t)r.range(0, 1);
-            if (rp.frame_filters_on) {
+            // multi-class frame filters are drawn for luma only
+            if (rp.frame_filters_on && pl == 0) {
                 int val = r.range(0, 7);
                 rp.num_classes_idx = (uint8_t)val;
                 rp.num_classes = (uint8_t)(1 + val + std::max(val - 3, 0) + std::max(val - 5, 0) * 2);

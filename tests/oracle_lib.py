@@ -22,6 +22,10 @@ def load():
         lib.oracle_filter_frame.restype = C.c_int
         lib.oracle_filter_frame.argtypes = [C.POINTER(R.FrameParams), P, C.c_size_t, P, C.c_size_t,
                                             P, P, P, C.c_ssize_t, C.c_ssize_t, C.c_uint32]
+        lib.oracle_apply_grain.restype = C.c_int
+        lib.oracle_apply_grain.argtypes = [C.POINTER(R.FrameParams), P, P, P, P, P, P, C.c_ssize_t, C.c_ssize_t]
+        lib.oracle_picture_has_grain.restype = C.c_int
+        lib.oracle_picture_has_grain.argtypes = [C.POINTER(R.FrameParams)]
         lib.oracle_dq_lookup.restype = C.c_int
         lib.oracle_dq_lookup.argtypes = [C.c_int]
         lib.oracle_deblock_quant_thr.restype = C.c_uint32
@@ -44,3 +48,18 @@ def filter_frame(sf, inloop_mask=R.INLOOP_ALL, params=None):
     if rc != 0:
         raise RuntimeError("oracle_filter_frame failed: %d" % rc)
     return planes
+
+
+def apply_grain(sf, planes, params=None):
+    """Film grain of the oracle on in-loop output planes; returns the display planes."""
+    import numpy as np
+    lib = load()
+    p = params if params is not None else sf.params
+    out = [np.empty_like(a) for a in planes]
+    bps = planes[0].itemsize
+    rc = lib.oracle_apply_grain(C.byref(p), planes[0].ctypes.data, planes[1].ctypes.data, planes[2].ctypes.data,
+                                out[0].ctypes.data, out[1].ctypes.data, out[2].ctypes.data,
+                                planes[0].shape[1] * bps, planes[1].shape[1] * bps)
+    if rc != 0:
+        raise RuntimeError("oracle_apply_grain failed: %d" % rc)
+    return out
