@@ -199,7 +199,9 @@ static void fgy(P *dst, const P *src, ptrdiff_t stride, const Rav2dFilmGrainData
     auto put = [&](int x, int y, int bx, int grain) {
         ptrdiff_t si = (ptrdiff_t)y * stride + x + bx;
         int sv = src[si];
-        int noise = round2(scaling[sv] * grain, (uint32_t)d.scaling_shift);
+        /* the reference indexes the LUT unchecked-in-range (a panic above the coded
+         * maximum); clamped here, as in the GPU pass */
+        int noise = round2(scaling[imin(sv, (1 << bitdepth) - 1)] * grain, (uint32_t)d.scaling_shift);
         dst[si] = (P)iclip(sv + noise, minv, maxv);
     };
     for (int bx = 0; bx < pw; bx += bs) {
@@ -266,7 +268,7 @@ static void fguv(P *dst, const P *src, ptrdiff_t stride, const Rav2dFilmGrainDat
         } else {
             val = avg;
         }
-        int noise = round2(scaling[val] * grain, (uint32_t)d.scaling_shift);
+        int noise = round2(scaling[imin(val, (1 << bitdepth) - 1)] * grain, (uint32_t)d.scaling_shift);
         dst[si] = (P)iclip(sv + noise, minv, maxv);
     };
     for (bx = 0; bx < pw; bx += bs >> sx) {

@@ -255,7 +255,9 @@ __global__ __launch_bounds__(kThreads) void cdef_kernel(CdefArgs a) {
         if (b_state[bi] & 1) {
             int pri = y_pri ? b_ypri[bi] : 0;
             int dir = y_pri ? b_dir[bi] : 0;
-            v = cdef_px(t, LY, pri, y_sec, dir, damping, bdm8);
+            // the reference stores the result through a truncating cast to the
+            // sample type (Pixel::from_i32), before CCSO reads it back
+            v = (int)(P)cdef_px(t, LY, pri, y_sec, dir, damping, bdm8);
         }
         if (cc0) {
             const CcsoDev &c = a.ccso[0];
@@ -286,7 +288,7 @@ __global__ __launch_bounds__(kThreads) void cdef_kernel(CdefArgs a) {
         int v = t[0];
         if (b_state[bi] & 2) {
             int uvdir = uv_pri ? b_dir[bi] : 0;
-            v = cdef_px(t, LC, uv_pri, uv_sec, uvdir, damping - 1, bdm8);
+            v = (int)(P)cdef_px(t, LC, uv_pri, uv_sec, uvdir, damping - 1, bdm8);
         }
         if (a.ccso_on && m.ccso[1 + pl]) {
             const CcsoDev &c = a.ccso[1 + pl];

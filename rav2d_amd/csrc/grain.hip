@@ -216,7 +216,9 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
                 val = avg;
             }
         }
-        int noise = round2(scaling[val] * g, d.scaling_shift);
+        // samples above the coded maximum (wrapped CDEF results left unrestored)
+        // would index past the scaling LUT, where the reference aborts; clamp
+        int noise = round2(scaling[dmin(val, (1 << bd) - 1)] * g, d.scaling_shift);
         row_ptr<P>(dst, y)[x] = (P)iclip(sv + noise, minv, maxv);
     }
 }
