@@ -14,6 +14,8 @@
 // block) LUT offsets (each band's LFSR walk, by one lane per band; a band's
 // "previous row" offsets are the previous band's own).  grain_apply then
 // computes every output sample independently.
+#include <string.h>
+
 #include "internal.h"
 #include "tables.h"
 
@@ -30,8 +32,28 @@ __device__ __forceinline__ uint32_t lfsr(uint32_t bits, uint32_t &s) {
 
 __device__ __forceinline__ int round2(int x, int shift) { return (x + ((1 << shift) >> 1)) >> shift; }
 
-__global__ __launch_bounds__(256) void grain_setup_kernel(GrainArgs a, const int16_t *gauss) {
+// LFSR jump-ahead: the 16-bit LFSR is linear over GF(2), so the state before
+// draw k*kChunk is M^(k*kChunk) applied to the seed.  kJump[k] holds that
+// matrix as 16 row masks (output bit i = parity(row_i & state)); the host
+// builds it once.
+constexpr int kChunk = 94;
+__constant__ uint16_t kJump[64][16];
+
+__device__ __forceinline__ uint32_t jump(int k, uint32_t s) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) r |= (uint32_t)(__popc(kJump[k][i] & s) & 1) << i;
+    return r;
+}
+
+constexpr int kSetupThreads = 128;
+
+__global__ __launch_bounds__(kSetupThreads) void grain_setup_kernel(GrainArgs a, const int16_t *gauss) {
     __shared__ int16_t lut[3][GH][GW];
+    __shared__ int16_t g_lds[2048];
+    __shared__ int8_t coef[3][28];  // AR coefficients staged out of the argument block
+    __shared__ uint8_t scl[3][1024];
+    __shared__ int8_t win[3][4][7];
     const Rav2dFilmGrainData &d = a.fg;
     const int bd = a.bitdepth, bdm8 = bd - 8;
     const int shift = 4 - bdm8 + d.grain_scale_shift;
@@ -39,63 +61,30 @@ __global__ __launch_bounds__(256) void grain_setup_kernel(GrainArgs a, const int
     const int t = threadIdx.x;
     const bool want_u = d.num_points[1] > 0 || d.chroma_scaling_from_luma;
     const bool want_v = d.num_points[2] > 0 || d.chroma_scaling_from_luma;
-    // Gaussian draws: one sequential LFSR walk per plane
-    if (t < 3) {
-        uint32_t s = a.seed;
-        int w = GW, h = GH;
-        bool on = true;
-        if (t > 0) {
-            s ^= t == 2 ? 0x49d8 : 0xb524;
-            w = SGW;
-            h = SGH;
-            on = t == 1 ? want_u : want_v;
-        }
-        for (int y = 0; y < GH; y++)
-            for (int x = 0; x < GW; x++) lut[t][y][x] = 0;
-        if (on)
-            for (int y = 0; y < h; y++)
-                for (int x = 0; x < w; x++) lut[t][y][x] = (int16_t)round2(gauss[lfsr(11, s)], shift);
-    }
-    // scaling LUTs (one lane per plane)
-    if (t >= 64 && t < 67) {
-        int pl = t - 64;
-        uint8_t *sc = a.scaling + pl * 1024;
-        int num = d.num_points[pl];
-        const uint8_t(*pts)[2] = d.points[pl];
-        int size = 1 << bd;
-        int sx = bd - 8;
-        if (num == 0) {
-            for (int i = 0; i < size; i++) sc[i] = 0;
-        } else {
-            for (int i = 0; i < (pts[0][0] << sx); i++) sc[i] = pts[0][1];
-            for (int i = 0; i < num - 1; i++) {
-                int bx = pts[i][0], by = pts[i][1], dx = pts[i + 1][0] - bx, dy = pts[i + 1][1] - by;
-                if (dx <= 0) continue;
-                int delta = dy * ((0x10000 + (dx >> 1)) / dx);
-                int dd = 0x8000;
-                for (int x = 0; x < dx; x++) {
-                    sc[(bx + x) << sx] = (uint8_t)(by + (dd >> 16));
-                    dd += delta;
-                }
-            }
-            for (int i = pts[num - 1][0] << sx; i < size; i++) sc[i] = pts[num - 1][1];
-            if (sx) {
-                int pad = 1 << sx, rnd = pad >> 1;
-                for (int i = 0; i < num - 1; i++) {
-                    int bx = pts[i][0] << sx, dx = (pts[i + 1][0] << sx) - bx;
-                    for (int x = 0; x < dx; x += pad) {
-                        int base = sc[bx + x], range = sc[bx + x + pad] - base, r = rnd;
-                        for (int n = 1; n < pad; n++) {
-                            r += range;
-                            sc[bx + x + n] = (uint8_t)(base + (r >> sx));
-                        }
-                    }
-                }
+    for (int i = t; i < 2048; i += kSetupThreads) g_lds[i] = gauss[i];
+    if (t < 3 * 28) coef[t / 28][t % 28] = d.ar_coeffs[t / 28][t % 28];
+    for (int i = t; i < 3 * GH * GW; i += kSetupThreads) (&lut[0][0][0])[i] = 0;
+    __syncthreads();
+    // Gaussian draws: lanes 0..63 walk chunks of the luma sequence, lanes
+    // 64..81 and 96..113 the U and V sequences (44x38 draws each)
+    {
+        int pl = t < 64 ? 0 : (t < 96 ? 1 : 2);
+        int k = pl == 0 ? t : (pl == 1 ? t - 64 : t - 96);
+        int w = pl ? SGW : GW, h = pl ? SGH : GH;
+        bool on = pl == 0 || (pl == 1 ? want_u : want_v);
+        int first = k * kChunk, last = dmin(first + kChunk, w * h);
+        if (on && first < w * h) {
+            uint32_t s = a.seed;
+            if (pl) s ^= pl == 2 ? 0x49d8 : 0xb524;
+            s = jump(k, s & 0xffff);
+            for (int i = first; i < last; i++) {
+                int y = i / w, x = i - y * w;
+                lut[pl][y][x] = (int16_t)round2(g_lds[lfsr(11, s)], shift);
             }
         }
     }
     // block offsets: one lane per row band
-    for (int band = t; band < a.n_bands; band += 256) {
+    for (int band = t; band < a.n_bands; band += kSetupThreads) {
         uint32_t s = a.seed;
         s ^= (uint32_t)(((band * 37 + 178) & 0xFF)) << 8;
         s ^= (uint32_t)((band * 173 + 105) & 0xFF);
@@ -109,47 +98,120 @@ __global__ __launch_bounds__(256) void grain_setup_kernel(GrainArgs a, const int
             }
     }
     __syncthreads();
-    // causal AR filter, luma: sample (y, x) at step x + 4y
+    // causal AR filter as a wavefront: sample (y, x) runs at step x + 4y, after
+    // its three left neighbours and the rows above up to x + 3.  The taps are
+    // laid out on a fixed lag-3 window (zero where the frame's lag is smaller):
+    // window row r = dy + 3 (dy = -3..0), column c = dx + 3 (dx = -3..3).
     const int lag = d.ar_coeff_lag;
+    // tap window: ci = (dy + lag) * (2 lag + 1) + (dx + lag), as the reference
+    // enumerates rows dy = -lag..0 and columns dx = -lag..lag (-lag..-1 on row 0)
+    if (t < 3 * 28) {
+        int pl = t / 28, r = (t % 28) / 7, c = t % 7;
+        int dy = r - 3, dx = c - 3;
+        int v = 0;
+        bool in = dy >= -lag && dx >= -lag && (dy < 0 ? dx <= lag : dx < 0);
+        if (in) v = coef[pl][(dy + lag) * (2 * lag + 1) + (dx + lag)];
+        win[pl][r][c] = (int8_t)v;
+    }
+    __syncthreads();
+    int wy[4][7];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 7; c++) wy[r][c] = win[0][r][c];
     for (int step = 3 + 4 * 3; step <= (GW - 4) + 4 * (GH - 1); step++) {
         int y = t + 3;
         if (y < GH) {
             int x = step - 4 * y;
             if (x >= 3 && x < GW - 3) {
-                int sum = 0, ci = 0;
-                for (int dy = y - lag; dy <= y; dy++) {
-                    int dx_end = dy == y ? x : x + lag + 1;
-                    for (int dx = x - lag; dx < dx_end; dx++) sum += d.ar_coeffs[0][ci++] * lut[0][dy][dx];
-                }
+                int sum = 0;
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+#pragma unroll
+                    for (int c = 0; c < 7; c++) sum += wy[r][c] * lut[0][y - 3 + r][x - 3 + c];
+#pragma unroll
+                for (int c = 0; c < 3; c++) sum += wy[3][c] * lut[0][y][x - 3 + c];
                 lut[0][y][x] = (int16_t)iclip(lut[0][y][x] + round2(sum, (int)d.ar_coeff_shift), gmin, gmax);
             }
         }
         __syncthreads();
     }
     // chroma (4:2:0 sub-grid), with the luma correlation term
-    for (int step = 3 + 4 * 3; step <= (SGW - 4) + 4 * (SGH - 1); step++) {
-        int uv = t >> 7, y = (t & 127) + 3;
-        bool on = uv == 0 ? want_u : want_v;
-        if (on && y < SGH) {
-            int x = step - 4 * y;
-            if (x >= 3 && x < SGW - 3) {
-                const int8_t *c = d.ar_coeffs[1 + uv];
-                int sum = 0, ci = 0;
-                for (int dy = y - lag; dy <= y; dy++) {
-                    int dx_end = dy == y ? x : x + lag + 1;
-                    for (int dx = x - lag; dx < dx_end; dx++) sum += c[ci++] * lut[1 + uv][dy][dx];
-                }
-                if (d.num_points[0] > 0) {
+    {
+        const int uv = t >> 6, y = (t & 63) + 3;
+        const bool on = uv == 0 ? want_u : want_v;
+        int wc[4][7];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 7; c++) wc[r][c] = win[1 + uv][r][c];
+        const int wl = d.num_points[0] > 0 ? coef[1 + uv][2 * lag * (lag + 1)] : 0;
+        for (int step = 3 + 4 * 3; step <= (SGW - 4) + 4 * (SGH - 1); step++) {
+            if (on && y < SGH) {
+                int x = step - 4 * y;
+                if (x >= 3 && x < SGW - 3) {
+                    int16_t(*L)[GW] = lut[1 + uv];
+                    int sum = 0;
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+#pragma unroll
+                        for (int c = 0; c < 7; c++) sum += wc[r][c] * L[y - 3 + r][x - 3 + c];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) sum += wc[3][c] * L[y][x - 3 + c];
                     int lx = ((x - 3) << 1) + 3, ly = ((y - 3) << 1) + 3;
                     int l = lut[0][ly][lx] + lut[0][ly][lx + 1] + lut[0][ly + 1][lx] + lut[0][ly + 1][lx + 1];
-                    sum += round2(l, 2) * c[ci];
+                    sum += round2(l, 2) * wl;
+                    L[y][x] = (int16_t)iclip(L[y][x] + round2(sum, (int)d.ar_coeff_shift), gmin, gmax);
                 }
-                lut[1 + uv][y][x] = (int16_t)iclip(lut[1 + uv][y][x] + round2(sum, (int)d.ar_coeff_shift), gmin, gmax);
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
-    for (int i = t; i < 3 * GH * GW; i += 256) a.lut[i] = (&lut[0][0][0])[i];
+    // scaling LUTs (generate_scaling_8bpc/hbd, filmgrain.rs:22-113), every
+    // entry in closed form: the first pass writes entries at multiples of
+    // 1 << (bd-8) (later segments overwrite earlier ones, then the tail fill),
+    // the second pass interpolates between them inside each segment.
+    for (int pl = 0; pl < 3; pl++) {
+        const int num = d.num_points[pl];
+        const uint8_t(*pts)[2] = d.points[pl];
+        const int sx = bd - 8, size = 1 << bd, pad = 1 << sx;
+        auto first_pass = [&](int c) -> int {  // value at coarse position c
+            int v = 0;
+            if (c < pts[0][0]) v = pts[0][1];
+            for (int k = 0; k < num - 1; k++) {
+                int bx = pts[k][0], dx = pts[k + 1][0] - bx;
+                if (dx > 0 && c >= bx && c < bx + dx) {
+                    int delta = (pts[k + 1][1] - pts[k][1]) * ((0x10000 + (dx >> 1)) / dx);
+                    v = (uint8_t)(pts[k][1] + ((0x8000 + (c - bx) * delta) >> 16));
+                }
+            }
+            if (c >= pts[num - 1][0]) v = pts[num - 1][1];
+            return v;
+        };
+        for (int e = t; e < size; e += kSetupThreads) {
+            int v = 0;
+            if (num > 0) {
+                int c = e >> sx, n = e & (pad - 1);
+                v = first_pass(c);
+                if (n && e >= (pts[0][0] << sx) && e < (pts[num - 1][0] << sx)) {
+                    // inside some segment: interpolate like the second pass
+                    bool in_seg = false;
+                    for (int k = 0; k < num - 1; k++) {
+                        int bx = pts[k][0] << sx, ex = pts[k + 1][0] << sx;
+                        if (e >= bx && e < ex) in_seg = true;
+                    }
+                    if (in_seg) {
+                        int base = v, range = first_pass(c + 1) - base;
+                        v = (uint8_t)(base + (((pad >> 1) + n * range) >> sx));
+                    }
+                }
+            }
+            scl[pl][e] = (uint8_t)v;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < 3 * 1024; i += kSetupThreads) a.scaling[i] = (&scl[0][0])[i];
+    for (int i = t; i < 3 * GH * GW; i += kSetupThreads) a.lut[i] = (&lut[0][0][0])[i];
 }
 
 // One output sample: S(bxo, byo) samples the block `bxo` to the left and
@@ -223,15 +285,48 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
     }
 }
 
-void launch_grain(GrainArgs a, int bitdepth, hipStream_t s) {
+static const int16_t *grain_tables() {
     static const int16_t *gauss = nullptr;
     if (!gauss) {
         int16_t *g = nullptr;
-        if (hipMalloc((void **)&g, sizeof(R2T_GAUSSIAN_SEQUENCE)) != hipSuccess) return;
+        if (hipMalloc((void **)&g, sizeof(R2T_GAUSSIAN_SEQUENCE)) != hipSuccess) return nullptr;
         (void)hipMemcpy(g, R2T_GAUSSIAN_SEQUENCE, sizeof(R2T_GAUSSIAN_SEQUENCE), hipMemcpyHostToDevice);
+        // jump-ahead matrices M^(k*kChunk), k = 0..63
+        auto mul = [](const uint16_t *A, const uint16_t *B, uint16_t *C) {
+            for (int i = 0; i < 16; i++) {
+                uint16_t r = 0;
+                for (int j = 0; j < 16; j++)
+                    if (A[i] >> j & 1) r ^= B[j];
+                C[i] = r;
+            }
+        };
+        uint16_t step[16], stepC[16], cur[16], tmp[16], jumps[64][16];
+        for (int i = 0; i < 15; i++) step[i] = (uint16_t)(1u << (i + 1));
+        step[15] = (uint16_t)((1u << 0) | (1u << 1) | (1u << 3) | (1u << 12));
+        for (int i = 0; i < 16; i++) stepC[i] = (uint16_t)(1u << i);
+        for (int n = 0; n < kChunk; n++) {  // stepC = M^kChunk
+            mul(step, stepC, tmp);
+            ::memcpy(stepC, tmp, sizeof(tmp));
+        }
+        for (int i = 0; i < 16; i++) cur[i] = (uint16_t)(1u << i);
+        for (int k = 0; k < 64; k++) {
+            ::memcpy(jumps[k], cur, sizeof(cur));
+            mul(stepC, cur, tmp);
+            ::memcpy(cur, tmp, sizeof(tmp));
+        }
+        (void)hipMemcpyToSymbol(HIP_SYMBOL(kJump), jumps, sizeof(jumps));
         gauss = g;
     }
-    hipLaunchKernelGGL(grain_setup_kernel, dim3(1), dim3(256), 0, s, a, gauss);
+    return gauss;
+}
+
+void launch_grain_setup(const GrainArgs &a, hipStream_t s) {
+    const int16_t *gauss = grain_tables();
+    if (!gauss) return;
+    hipLaunchKernelGGL(grain_setup_kernel, dim3(1), dim3(kSetupThreads), 0, s, a, gauss);
+}
+
+void launch_grain_apply(GrainArgs a, int bitdepth, hipStream_t s) {
     a.map.first[0] = 0;
     for (int p = 0; p < 3; p++) {
         a.map.tiles_x[p] = (a.src[p].w + 63) / 64;

@@ -106,7 +106,8 @@ struct GrainArgs {
     TileMap map;
 };
 
-void launch_grain(GrainArgs a, int bitdepth, hipStream_t s);
+void launch_grain_setup(const GrainArgs &a, hipStream_t s);
+void launch_grain_apply(GrainArgs a, int bitdepth, hipStream_t s);
 
 // Which buffer a pass reads and writes, per plane.
 void launch_deblock_v(const DeblockArgs &a, int bitdepth, hipStream_t s);

@@ -31,6 +31,7 @@ using namespace r2g;
 struct Rav2dGpuCtx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;  // parameter-only work (grain tables) overlapped with the passes
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t budget = 0;
     uint64_t in_use = 0;
@@ -111,6 +112,8 @@ struct Rav2dGpuFrame {
     int lr_restore_planes = 0, lr_base_q = 0, lr_widx = 0, lr_ns_idx = 0;
     bool lr_dirty = true;
     uint8_t *d_grain = nullptr;  // grain LUTs + scaling + offsets scratch
+    hipEvent_t ev_setup = nullptr, ev_free = nullptr;
+    bool grain_ready = false, grain_used = false;
 };
 
 namespace {
@@ -380,6 +383,71 @@ int upload_lr_items(Rav2dGpuFrame *f) {
     return RAV2D_OK;
 }
 
+DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl);
+
+bool frame_has_grain(const Rav2dFrameParams &P) {
+    const Rav2dFilmGrainData &d = P.grain;
+    return P.grain_present && (d.num_points[0] || d.num_points[1] || d.num_points[2] ||
+                               (d.clip_to_restricted_range && d.chroma_scaling_from_luma));
+}
+
+int grain_args(Rav2dGpuFrame *f, GrainArgs &a) {
+    const Rav2dFrameParams &P = f->params;
+    const Rav2dFilmGrainData &d = P.grain;
+    if (d.ar_coeff_lag < 0 || d.ar_coeff_lag > 3 || d.block_size < 0 || d.block_size > 1 || d.scaling_shift < 8 ||
+        d.scaling_shift > 11 || d.ar_coeff_shift < 6 || d.ar_coeff_shift > 9 || d.grain_scale_shift < 0 ||
+        d.grain_scale_shift > 3) {
+        set_last_error("film grain parameters out of range");
+        return RAV2D_EINVAL;
+    }
+    for (int pl = 0; pl < 3; pl++) {
+        if (d.num_points[pl] < 0 || d.num_points[pl] > 14) {
+            set_last_error("film grain point count out of range");
+            return RAV2D_EINVAL;
+        }
+        for (int i = 1; i < d.num_points[pl]; i++)
+            if (d.points[pl][i][0] < d.points[pl][i - 1][0]) {
+                set_last_error("film grain scaling points must not decrease");
+                return RAV2D_EINVAL;
+            }
+    }
+    memset(&a, 0, sizeof(a));
+    a.fg = d;
+    a.seed = P.grain_seed;
+    a.bitdepth = f->g.bitdepth;
+    a.lut = (int16_t *)f->d_grain;
+    a.scaling = f->d_grain + 3 * 73 * 82 * 2;
+    a.offsets = a.scaling + 3 * 1024;
+    int bs = 16 << d.block_size;
+    a.n_bands = (P.height + bs - 1) / bs;
+    a.n_blocks = (P.width + bs - 1) / bs;
+    // apply_grain_row (filmgrain.rs:713, 744-747): a plane is grained only when
+    // its scaling LUT exists; chroma-from-luma uses the luma LUT
+    a.plane_on[0] = d.num_points[0] > 0;
+    for (int uv = 1; uv < 3; uv++)
+        a.plane_on[uv] = d.chroma_scaling_from_luma ? d.num_points[0] > 0 : d.num_points[uv] > 0;
+    for (int pl = 0; pl < 3; pl++) {
+        a.src[pl] = plane_of(f, BUF_OUT, pl);
+        a.dst[pl] = plane_of(f, BUF_DISP, pl);
+    }
+    return RAV2D_OK;
+}
+
+// Grain tables depend on the frame parameters only: build them on the
+// auxiliary stream so they overlap the in-loop passes.
+int start_grain_setup(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, hipStream_t s) {
+    GrainArgs a;
+    int r = grain_args(f, a);
+    if (r) return r;
+    HIP_TRY(hipEventRecord(f->ev_free, s));  // earlier work on this frame comes first
+    HIP_TRY(hipStreamWaitEvent(ctx->aux, f->ev_free, 0));
+    launch_grain_setup(a, ctx->aux);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(f->ev_setup, ctx->aux));
+    f->grain_ready = true;
+    return RAV2D_OK;
+}
+
 int validate(const Rav2dFrameParams *p) {
     if (!p) return RAV2D_EINVAL;
     if (p->width <= 0 || p->height <= 0 || (p->width & 7) || (p->height & 7)) {
@@ -606,6 +674,7 @@ int rav2d_gpu_ctx_create(int device, uint64_t hbm_budget_bytes, uint32_t flags, 
     c->device = device;
     c->budget = hbm_budget_bytes;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         set_last_error("stream/event creation failed");
         delete c;
@@ -626,6 +695,8 @@ int rav2d_gpu_ctx_destroy(Rav2dGpuCtx *ctx) {
     for (auto e : ctx->free_events) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamSynchronize(ctx->aux);
+    (void)hipStreamDestroy(ctx->aux);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RAV2D_OK;
@@ -699,6 +770,11 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
         set_last_error("hipMalloc failed");
         return RAV2D_ENOMEM;
     }
+    if (hipEventCreateWithFlags(&f->ev_setup, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&f->ev_free, hipEventDisableTiming) != hipSuccess) {
+        set_last_error("event creation failed");
+        return RAV2D_EDEVICE;
+    }
     uint8_t *p = f->block;
     for (int b = 0; b < NBUF; b++)
         for (int pl = 0; pl < 3; pl++) {
@@ -717,8 +793,11 @@ int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f) {
     Rav2dGpuCtx *ctx = f->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->aux);
     (void)hipFree(f->block);
     (void)hipFree(f->d_dblut);
+    if (f->ev_setup) (void)hipEventDestroy(f->ev_setup);
+    if (f->ev_free) (void)hipEventDestroy(f->ev_free);
     if (f->d_grain) (void)hipFree(f->d_grain);
     if (f->d_mask) (void)hipFree(f->d_mask);
     if (f->d_lr) (void)hipFree(f->d_lr);
@@ -826,59 +905,37 @@ int rav2d_gpu_frame_download(Rav2dGpuFrame *f, int which, void *y, void *u, void
 int rav2d_gpu_filter_frame(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream) {
     if (!ctx || !f || f->ctx != ctx) return RAV2D_EINVAL;
     HIP_TRY(hipSetDevice(ctx->device));
-    return run_chain(ctx, f, RAV2D_PASS_GRAIN, pick_stream(ctx, stream));
+    hipStream_t s = pick_stream(ctx, stream);
+    if (frame_has_grain(f->params) && !f->grain_ready) {
+        int r = start_grain_setup(ctx, f, s);
+        if (r) return r;
+    }
+    return run_chain(ctx, f, RAV2D_PASS_GRAIN, s);
 }
 
 int rav2d_gpu_apply_grain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream) {
     if (!ctx || !f || f->ctx != ctx) return RAV2D_EINVAL;
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = pick_stream(ctx, stream);
-    const Rav2dFrameParams &P = f->params;
-    const Rav2dFilmGrainData &d = P.grain;
-    // picture_has_grain (decode.rs:4641-4651)
-    bool has = P.grain_present && (d.num_points[0] || d.num_points[1] || d.num_points[2] ||
-                                   (d.clip_to_restricted_range && d.chroma_scaling_from_luma));
-    if (!has) {
+    if (!frame_has_grain(f->params)) {
         for (int pl = 0; pl < 3; pl++) {
             int r = copy_plane(f, BUF_OUT, BUF_DISP, pl, s);
             if (r) return r;
         }
         return RAV2D_OK;
     }
-    if (d.ar_coeff_lag < 0 || d.ar_coeff_lag > 3 || d.block_size < 0 || d.block_size > 1 || d.scaling_shift < 8 ||
-        d.scaling_shift > 11 || d.ar_coeff_shift < 6 || d.ar_coeff_shift > 9 || d.grain_scale_shift < 0 ||
-        d.grain_scale_shift > 3) {
-        set_last_error("film grain parameters out of range");
-        return RAV2D_EINVAL;
-    }
-    for (int pl = 0; pl < 3; pl++)
-        for (int i = 0; i < d.num_points[pl] && i < 14; i++)
-            if (i && d.points[pl][i][0] < d.points[pl][i - 1][0]) {
-                set_last_error("film grain scaling points must not decrease");
-                return RAV2D_EINVAL;
-            }
     GrainArgs a;
-    memset(&a, 0, sizeof(a));
-    a.fg = d;
-    a.seed = P.grain_seed;
-    a.bitdepth = f->g.bitdepth;
-    a.lut = (int16_t *)f->d_grain;
-    a.scaling = f->d_grain + 3 * 73 * 82 * 2;
-    a.offsets = a.scaling + 3 * 1024;
-    int bs = 16 << d.block_size;
-    a.n_bands = (P.height + bs - 1) / bs;
-    a.n_blocks = (P.width + bs - 1) / bs;
-    // apply_grain_row (filmgrain.rs:713, 744-747): a plane is grained only when
-    // its scaling LUT exists; chroma-from-luma uses the luma LUT
-    a.plane_on[0] = d.num_points[0] > 0;
-    for (int uv = 1; uv < 3; uv++)
-        a.plane_on[uv] = d.chroma_scaling_from_luma ? d.num_points[0] > 0 : d.num_points[uv] > 0;
-    for (int pl = 0; pl < 3; pl++) {
-        a.src[pl] = plane_of(f, BUF_OUT, pl);
-        a.dst[pl] = plane_of(f, BUF_DISP, pl);
-    }
+    int r = grain_args(f, a);
+    if (r) return r;
     PassTimer pt(ctx, RAV2D_PASS_GRAIN, s);
-    launch_grain(a, f->g.bitdepth, s);
+    if (f->grain_ready) {
+        HIP_TRY(hipStreamWaitEvent(s, f->ev_setup, 0));
+    } else {
+        launch_grain_setup(a, s);
+        HIP_TRY(hipGetLastError());
+    }
+    f->grain_ready = false;
+    launch_grain_apply(a, f->g.bitdepth, s);
     HIP_TRY(hipGetLastError());
     return RAV2D_OK;
 }
