@@ -6,7 +6,13 @@ grain) over one synthetic 10-bit 4:2:0 3840x2160 frame whose input planes are
 already resident in HBM.  Prints one JSON line (see DESIGN.md for the metric,
 the algorithmic byte counts and the CPU baseline).
 
+A second leg times a mixed batch of 64 frames (sizes uniform over 360p..2160p,
+8/10-bit 50/50, grain on the 10-bit frames) as 64 sequential per-frame
+launches and as one launch per pass (autotuned, hipGraph replay), and checks
+both give the same planes.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 4k10|1080p10|cif8]
+                    [--batch 64] [--batch-steps 5]
 """
 import argparse
 import json
@@ -66,6 +72,109 @@ def cpu_baseline(w, h, bd, sb128, tools, seed, budget_s=12.0):
                 n, w, h, bd, " + film grain" if grain else "", dt)}
 
 
+BATCH_SIZES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
+
+
+def batch_workload(n, seed):
+    """Mixed batch: sizes uniform over 360p..2160p, 8/10-bit 50/50, every tool;
+    film grain on the 10-bit frames."""
+    import random
+    rng = random.Random(seed)
+    out = []
+    for i in range(n):
+        w, h = BATCH_SIZES[rng.randrange(len(BATCH_SIZES))]
+        bd = 10 if rng.random() < 0.5 else 8
+        tools = R.TOOL_ALL if bd == 10 else R.TOOL_ALL & ~R.TOOL_GRAIN
+        out.append((w, h, bd, rng.randrange(2), seed * 1000 + i, tools))
+    return out
+
+
+def bench_batch(ctx, n, steps, warmup, seed):
+    """64-frame mixed batch: per-frame launches vs one launch per pass (graph)."""
+    import hashlib
+    from concurrent.futures import ThreadPoolExecutor
+    spec = batch_workload(n, seed)
+    with ThreadPoolExecutor(8) as ex:
+        sfs = list(ex.map(lambda c: R.SynthFrame(c[0], c[1], c[2], c[3], c[4], tools=c[5]), spec))
+    pix = sum(c[0] * c[1] for c in spec)
+    A = [R.GpuFrame.from_synth(ctx, sf) for sf in sfs]  # per-frame form
+    B = [R.GpuFrame.from_synth(ctx, sf) for sf in sfs]  # batched form
+    grain = [bool(sf.params.grain_present) for sf in sfs]
+
+    def per_frame():
+        for f, g in zip(A, grain):
+            f.filter()
+            f.apply_grain()
+
+    for _ in range(warmup):
+        per_frame()
+    ctx.synchronize()
+    ctx.timer_start()
+    for _ in range(steps):
+        per_frame()
+    pf_ms = ctx.timer_stop() / steps
+
+    b = R.GpuBatch(ctx, B)
+    t0 = time.perf_counter()
+    cfg = b.autotune(iters=3)
+    tune_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        b.run()
+    ctx.synchronize()
+    ctx.timer_start()
+    for _ in range(steps):
+        b.run()
+    bt_ms = ctx.timer_stop() / steps
+    # per-pass durations of the batched launches (eager run of the same shapes)
+    ecfg = b.config()
+    ecfg.use_graph = 0
+    b.set_config(ecfg)
+    ctx.set_profiling(True)
+    ctx.pass_times(reset=True)
+    for _ in range(steps):
+        b.run()
+    times = ctx.pass_times(reset=True)
+    ctx.set_profiling(False)
+    pass_us, pass_bytes = {}, {}
+    for i, name in enumerate(PASS_NAMES):
+        tot, cnt = times[i]
+        if cnt:
+            pass_us[name] = round(tot / cnt * 1e3, 1)
+            pass_bytes[name] = b.info(i)[1]
+
+    def digest(frames):
+        h = hashlib.sha256()
+        for f in frames:
+            for which in (1, 2):
+                for a in f.download(which):
+                    h.update(a.tobytes())
+        return h.hexdigest()
+
+    exact = digest(A) == digest(B)
+    b.close()
+    for f in A + B:
+        f.close()
+    for sf in sfs:
+        sf.close()
+    res = {
+        "frames": n,
+        "mix": {"%dx%d" % s: sum(1 for c in spec if (c[0], c[1]) == s) for s in BATCH_SIZES},
+        "ten_bit_frames": sum(1 for c in spec if c[2] == 10),
+        "grained_frames": sum(grain),
+        "mpix_per_batch": round(pix / 1e6, 2),
+        "per_frame_ms": round(pf_ms, 3),
+        "batched_ms": round(bt_ms, 3),
+        "per_frame_mpix_s": round(pix / 1e6 / (pf_ms / 1e3), 1),
+        "batched_mpix_s": round(pix / 1e6 / (bt_ms / 1e3), 1),
+        "speedup": round(pf_ms / bt_ms, 3),
+        "batched_bit_exact_vs_per_frame": exact,
+        "autotuned": cfg.as_dict(),
+        "autotune_s": round(tune_s, 2),
+        "pass_us": pass_us,
+    }
+    return res, times, pass_bytes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +183,8 @@ def main():
     ap.add_argument("--config", default="4k10", choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--batch", type=int, default=64, help="frames of the mixed batch leg (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
@@ -126,6 +237,11 @@ def main():
     frame_bytes = S * bps * (2 + (1 if grain else 0))
     chain_gbs = frame_bytes / (ms_per_step * 1e-3) / 1e9
 
+    batch = None
+    if args.batch > 0:
+        print("bench: mixed batch of %d frames" % args.batch, file=sys.stderr, flush=True)
+        batch, _, _ = bench_batch(ctx, args.batch, args.batch_steps, 1, args.seed)
+
     cpu = None
     if not args.no_cpu:
         print("bench: CPU baseline leg", file=sys.stderr, flush=True)
@@ -154,6 +270,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None},
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     f.close()
     ctx.close()

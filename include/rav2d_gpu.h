@@ -255,9 +255,51 @@ int rav2d_gpu_timer_start(Rav2dGpuCtx *ctx);
 /* Milliseconds between the last timer_start and now (synchronizes). */
 int rav2d_gpu_timer_stop(Rav2dGpuCtx *ctx, float *ms);
 
+/* ---- batched launches -------------------------------------------------------
+ * No counterpart in the reference, which filters one frame at a time in its
+ * PHASE-B loop (decode.rs:3597-3625) and then applies grain
+ * (decode.rs:4672).  A batch runs exactly that per-frame work for many frames
+ * of any size and bit depth with one launch per pass over all of them: the
+ * output (and, with apply_grain, display) planes of every frame equal what
+ * rav2d_gpu_filter_frame + rav2d_gpu_apply_grain write.  Frames are planned
+ * when the batch is created and re-planned by the next run after their
+ * parameters or side info change; a frame must stay alive while a batch
+ * holding it exists, and must not be filtered by another call while a run
+ * of the batch is in flight (they share the frame's intermediate planes). */
+typedef struct Rav2dGpuBatch Rav2dGpuBatch;
+typedef struct Rav2dBatchConfig {
+    int32_t use_graph;                     /* capture once, replay as a hipGraph */
+    int32_t apply_grain;                   /* also write the display planes */
+    int32_t grid[RAV2D_NUM_PASSES];        /* workgroups of each pass launch
+                                              (RAV2D_PASS_*; GRAIN = apply);
+                                              0 = one per work item, else a
+                                              persistent grid that loops */
+    int32_t xcd_remap[RAV2D_NUM_PASSES];   /* give each XCD a contiguous range
+                                              of work items */
+} Rav2dBatchConfig;
+/* use_graph = 1, apply_grain = 1, one workgroup per item, no remap. */
+void rav2d_gpu_batch_default_config(Rav2dBatchConfig *cfg);
+int rav2d_gpu_batch_create(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n,
+                           const Rav2dBatchConfig *cfg, Rav2dGpuBatch **out);
+int rav2d_gpu_batch_destroy(Rav2dGpuBatch *b);
+/* Enqueue one run of the whole batch on `stream` (NULL = context stream). */
+int rav2d_gpu_batch_run(Rav2dGpuBatch *b, void *stream);
+int rav2d_gpu_batch_get_config(const Rav2dGpuBatch *b, Rav2dBatchConfig *out);
+int rav2d_gpu_batch_set_config(Rav2dGpuBatch *b, const Rav2dBatchConfig *cfg);
+/* Times every launch shape of every pass (iters runs each, on the context
+ * stream, which it synchronizes), keeps the fastest per pass and returns it. */
+int rav2d_gpu_batch_autotune(Rav2dGpuBatch *b, int iters, Rav2dBatchConfig *chosen);
+/* Work items, algorithmic bytes (one read + one write of the planes) and
+ * frames of one pass's launch. */
+int rav2d_gpu_batch_info(const Rav2dGpuBatch *b, int pass, int64_t *items, int64_t *bytes,
+                         int *frames);
+/* Luma pixels of all frames of the batch. */
+int64_t rav2d_gpu_batch_pixels(const Rav2dGpuBatch *b);
+
 /* sizeof() of the ABI structs, so bindings can check their mirrors:
  * 0 Rav2dAv2Filter, 1 Rav2dAv2Restoration, 2 Rav2dFrameParams,
- * 3 Rav2dFilmGrainData, 4 Rav2dPlanes; 0 for an unknown index. */
+ * 3 Rav2dFilmGrainData, 4 Rav2dPlanes, 5 Rav2dBatchConfig; 0 for an unknown
+ * index. */
 size_t rav2d_gpu_struct_size(int which);
 
 const char *rav2d_gpu_version(void);

@@ -83,6 +83,23 @@ class Planes(C.Structure):
     _fields_ = [("data", C.c_void_p * 3), ("stride", C.c_ssize_t * 2)]
 
 
+NUM_PASSES = 5
+PASS_NAMES = ("deblock_v", "deblock_h", "cdef_ccso", "loop_restoration", "film_grain")
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [
+        ("use_graph", C.c_int32),
+        ("apply_grain", C.c_int32),
+        ("grid", C.c_int32 * NUM_PASSES),
+        ("xcd_remap", C.c_int32 * NUM_PASSES),
+    ]
+
+    def as_dict(self):
+        return {"use_graph": self.use_graph, "apply_grain": self.apply_grain,
+                "grid": list(self.grid), "xcd_remap": list(self.xcd_remap)}
+
+
 class SynthConfig(C.Structure):
     _fields_ = [
         ("width", C.c_int32), ("height", C.c_int32), ("bitdepth", C.c_int32), ("sb128", C.c_int32),
@@ -141,6 +158,17 @@ def load_gpu():
     _declare(lib, "rav2d_gpu_device_count", C.c_int, [])
     _declare(lib, "rav2d_gpu_set_profiling", C.c_int, [P, C.c_int])
     _declare(lib, "rav2d_gpu_pass_times", C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int])
+    _declare(lib, "rav2d_gpu_batch_default_config", None, [C.POINTER(BatchConfig)])
+    _declare(lib, "rav2d_gpu_batch_create", C.c_int,
+             [P, C.POINTER(P), C.c_int, C.POINTER(BatchConfig), C.POINTER(P)])
+    _declare(lib, "rav2d_gpu_batch_destroy", C.c_int, [P])
+    _declare(lib, "rav2d_gpu_batch_run", C.c_int, [P, P])
+    _declare(lib, "rav2d_gpu_batch_get_config", C.c_int, [P, C.POINTER(BatchConfig)])
+    _declare(lib, "rav2d_gpu_batch_set_config", C.c_int, [P, C.POINTER(BatchConfig)])
+    _declare(lib, "rav2d_gpu_batch_autotune", C.c_int, [P, C.c_int, C.POINTER(BatchConfig)])
+    _declare(lib, "rav2d_gpu_batch_info", C.c_int,
+             [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)])
+    _declare(lib, "rav2d_gpu_batch_pixels", C.c_int64, [P])
     _gpu = lib
     return lib
 
@@ -320,4 +348,55 @@ class GpuFrame:
     def close(self):
         if self.h:
             self.lib.rav2d_gpu_frame_destroy(self.h)
+            self.h = None
+
+
+def default_batch_config():
+    cfg = BatchConfig()
+    load_gpu().rav2d_gpu_batch_default_config(C.byref(cfg))
+    return cfg
+
+
+class GpuBatch:
+    """Many frames filtered with one launch per pass (rav2d_gpu_batch_*)."""
+
+    def __init__(self, ctx, frames, config=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.frames = list(frames)
+        arr = (C.c_void_p * len(self.frames))(*[f.h.value for f in self.frames])
+        h = C.c_void_p()
+        cfg = config if config is not None else default_batch_config()
+        _check(self.lib.rav2d_gpu_batch_create(ctx.h, arr, len(self.frames), C.byref(cfg), C.byref(h)),
+               "batch_create")
+        self.h = h
+
+    def run(self):
+        _check(self.lib.rav2d_gpu_batch_run(self.h, None), "batch_run")
+
+    def config(self):
+        cfg = BatchConfig()
+        _check(self.lib.rav2d_gpu_batch_get_config(self.h, C.byref(cfg)), "batch_get_config")
+        return cfg
+
+    def set_config(self, cfg):
+        _check(self.lib.rav2d_gpu_batch_set_config(self.h, C.byref(cfg)), "batch_set_config")
+
+    def autotune(self, iters=3):
+        cfg = BatchConfig()
+        _check(self.lib.rav2d_gpu_batch_autotune(self.h, iters, C.byref(cfg)), "batch_autotune")
+        return cfg
+
+    def info(self, p):
+        items, nbytes, nf = C.c_int64(), C.c_int64(), C.c_int()
+        _check(self.lib.rav2d_gpu_batch_info(self.h, p, C.byref(items), C.byref(nbytes), C.byref(nf)),
+               "batch_info")
+        return items.value, nbytes.value, nf.value
+
+    def pixels(self):
+        return self.lib.rav2d_gpu_batch_pixels(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.rav2d_gpu_batch_destroy(self.h)
             self.h = None

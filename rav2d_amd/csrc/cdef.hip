@@ -172,14 +172,20 @@ __device__ __forceinline__ int cdef_px(const int16_t *t, int ls, int pri, int se
 constexpr int kThreads = 256;
 constexpr int LY = 64 + 4, LC = 32 + 4;  // staged widths: 2-sample halo each side
 
+struct CdefSmem {
+    int16_t ly[LY * LY];
+    int16_t lc[2][LC * LC];
+    uint8_t b_state[64];   // bit0 luma filtered, bit1 chroma filtered
+    uint8_t b_dir[64];
+    int16_t b_ypri[64];    // adjusted luma primary strength
+};
+
 template <typename P>
-__global__ __launch_bounds__(kThreads) void cdef_kernel(CdefArgs a) {
-    __shared__ int16_t ly[LY * LY];
-    __shared__ int16_t lc[2][LC * LC];
-    __shared__ uint8_t b_state[64];   // bit0 luma filtered, bit1 chroma filtered
-    __shared__ uint8_t b_dir[64];
-    __shared__ int16_t b_ypri[64];    // adjusted luma primary strength
-    const int sbx = blockIdx.x, sby = blockIdx.y;
+__device__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
+    int16_t *ly = sm.ly;
+    int16_t(*lc)[LC * LC] = sm.lc;
+    uint8_t *b_state = sm.b_state, *b_dir = sm.b_dir;
+    int16_t *b_ypri = sm.b_ypri;
     const int x0 = sbx * 64, y0 = sby * 64;
     const int fw = a.g.bw * 4, fh = a.g.bh * 4;
     const int cw = fw >> 1, ch = fh >> 1;
@@ -310,9 +316,35 @@ __global__ __launch_bounds__(kThreads) void cdef_kernel(CdefArgs a) {
     }
 }
 
-void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s) {
-    static bool consts = false;
-    if (!consts) {
+template <typename P> __global__ __launch_bounds__(kThreads) void cdef_kernel(CdefArgs a) {
+    __shared__ CdefSmem sm;
+    cdef_body<P>(a, blockIdx.x, blockIdx.y, sm);
+}
+
+// Batched launch: one item per 64x64 superblock of every frame.
+__global__ __launch_bounds__(kThreads) void cdef_batch(const CdefArgs *args, BatchSched s) {
+    __shared__ CdefSmem sm;
+    for_each_item(s, [&](int f, int t) {
+        const CdefArgs &a = args[f];
+        const int sbw = (a.g.bw * 4 + 63) >> 6;
+        const int sby = t / sbw, sbx = t - sby * sbw;
+        if (a.g.bitdepth > 8)
+            cdef_body<uint16_t>(a, sbx, sby, sm);
+        else
+            cdef_body<uint8_t>(a, sbx, sby, sm);
+        __syncthreads();
+    });
+}
+
+int cdef_items(const CdefArgs &a) { return ((a.g.bw * 4 + 63) >> 6) * ((a.g.bh * 4 + 63) >> 6); }
+
+// Per device: the direction offsets as (dy, dx) and the CCSO offset table.
+void cdef_init_consts() {
+    static bool done[64] = {};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64 || done[dev]) return;
+    {
         int8_t dydx[12][2][2];
         for (int d = 0; d < 12; d++)
             for (int k = 0; k < 2; k++) {
@@ -323,8 +355,18 @@ void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s) {
             }
         (void)hipMemcpyToSymbol(HIP_SYMBOL(kCdefDyDx), dydx, sizeof(dydx));
         (void)hipMemcpyToSymbol(HIP_SYMBOL(kCcsoOffset), R2T_CCSO_OFFSET, sizeof(R2T_CCSO_OFFSET));
-        consts = true;
+        done[dev] = true;
     }
+}
+
+void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, hipStream_t st) {
+    if (s.total <= 0) return;
+    cdef_init_consts();
+    hipLaunchKernelGGL(cdef_batch, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+}
+
+void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s) {
+    cdef_init_consts();
     dim3 grid((a.g.bw * 4 + 63) / 64, (a.g.bh * 4 + 63) / 64);
     if (bitdepth > 8)
         hipLaunchKernelGGL(cdef_kernel<uint16_t>, grid, dim3(kThreads), 0, s, a);

@@ -181,10 +181,9 @@ __device__ EdgeDesc edge_desc(const DeblockArgs &a, int pl, bool vert, int x4, i
 constexpr int kTW = 64, kTH = 32, kHalo = 16, kThreads = 256;
 
 template <typename P>
-__global__ __launch_bounds__(kThreads) void deblock_v_kernel(DeblockArgs a) {
-    __shared__ uint16_t tile[kTH][kTW + 2 * kHalo];
+__device__ void deblock_v_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW + 2 * kHalo]) {
     int pl, tx, ty;
-    decode_tile(a.map, blockIdx.x, pl, tx, ty);
+    decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl];
     const DevPlane &dst = a.dst[pl];
     const int x0 = tx * kTW, y0 = ty * kTH;
@@ -219,10 +218,9 @@ __global__ __launch_bounds__(kThreads) void deblock_v_kernel(DeblockArgs a) {
 }
 
 template <typename P>
-__global__ __launch_bounds__(kThreads) void deblock_h_kernel(DeblockArgs a) {
-    __shared__ uint16_t tile[kTH + 2 * kHalo][kTW];
+__device__ void deblock_h_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW]) {
     int pl, tx, ty;
-    decode_tile(a.map, blockIdx.x, pl, tx, ty);
+    decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl];
     const DevPlane &dst = a.dst[pl];
     const int x0 = tx * kTW, y0 = ty * kTH;
@@ -256,7 +254,50 @@ __global__ __launch_bounds__(kThreads) void deblock_h_kernel(DeblockArgs a) {
     }
 }
 
-static TileMap make_map(const DeblockArgs &a) {
+template <typename P> __global__ __launch_bounds__(kThreads) void deblock_v_kernel(DeblockArgs a) {
+    __shared__ uint16_t tile[kTH][kTW + 2 * kHalo];
+    deblock_v_body<P>(a, blockIdx.x, tile);
+}
+
+template <typename P> __global__ __launch_bounds__(kThreads) void deblock_h_kernel(DeblockArgs a) {
+    __shared__ uint16_t tile[kTH + 2 * kHalo][kTW];
+    deblock_h_body<P>(a, blockIdx.x, tile);
+}
+
+// Batched launches: every frame's tiles, 8- and 10-bit frames alike.
+__global__ __launch_bounds__(kThreads) void deblock_v_batch(const DeblockArgs *args, BatchSched s) {
+    __shared__ uint16_t tile[kTH][kTW + 2 * kHalo];
+    for_each_item(s, [&](int f, int t) {
+        const DeblockArgs &a = args[f];
+        if (a.g.bitdepth > 8)
+            deblock_v_body<uint16_t>(a, t, tile);
+        else
+            deblock_v_body<uint8_t>(a, t, tile);
+        __syncthreads();
+    });
+}
+
+__global__ __launch_bounds__(kThreads) void deblock_h_batch(const DeblockArgs *args, BatchSched s) {
+    __shared__ uint16_t tile[kTH + 2 * kHalo][kTW];
+    for_each_item(s, [&](int f, int t) {
+        const DeblockArgs &a = args[f];
+        if (a.g.bitdepth > 8)
+            deblock_h_body<uint16_t>(a, t, tile);
+        else
+            deblock_h_body<uint8_t>(a, t, tile);
+        __syncthreads();
+    });
+}
+
+void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, hipStream_t st) {
+    if (s.total <= 0) return;
+    if (vert)
+        hipLaunchKernelGGL(deblock_v_batch, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+    else
+        hipLaunchKernelGGL(deblock_h_batch, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+}
+
+TileMap deblock_tile_map(const DeblockArgs &a) {
     TileMap m;
     m.first[0] = 0;
     for (int p = 0; p < 3; p++) {
@@ -269,7 +310,7 @@ static TileMap make_map(const DeblockArgs &a) {
 
 void launch_deblock_v(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
     DeblockArgs a = a0;
-    a.map = make_map(a);
+    a.map = deblock_tile_map(a);
     if (a.map.first[3] == 0) return;
     if (bitdepth > 8)
         hipLaunchKernelGGL(deblock_v_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
@@ -279,7 +320,7 @@ void launch_deblock_v(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
 
 void launch_deblock_h(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
     DeblockArgs a = a0;
-    a.map = make_map(a);
+    a.map = deblock_tile_map(a);
     if (a.map.first[3] == 0) return;
     if (bitdepth > 8)
         hipLaunchKernelGGL(deblock_h_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);

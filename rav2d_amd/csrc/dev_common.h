@@ -46,4 +46,47 @@ __device__ __forceinline__ void decode_tile(const TileMap &m, int t, int &pl, in
     tx = r - ty * m.tiles_x[pl];
 }
 
+// Work schedule of a batched launch: work items of all frames back to back;
+// prefix[f] is frame f's first item.  A launch may be persistent (fewer
+// workgroups than items, each looping); with xcd_remap the items are cut into
+// 8 contiguous chunks, one per XCD (workgroup b runs on XCD b % 8), so the
+// neighbouring tiles that share halo rows share an L2.
+struct BatchSched {
+    const int32_t *prefix;
+    int32_t n_frames;
+    int32_t total;
+    int32_t xcd_remap;
+};
+
+__device__ __forceinline__ int find_frame(const int32_t *prefix, int n, int item) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Visits the items of workgroup blockIdx.x in order: fn(frame, local item).
+template <typename F> __device__ __forceinline__ void for_each_item(const BatchSched &s, F fn) {
+    const int G = gridDim.x, b = blockIdx.x;
+    if (s.xcd_remap && (G & 7) == 0) {
+        const int per_xcd = G >> 3, xcd = b & 7, slot = b >> 3;
+        const int chunk = (s.total + 7) >> 3;
+        const int lo = xcd * chunk, hi = min(s.total, lo + chunk);
+        for (int it = lo + slot; it < hi; it += per_xcd) {
+            int f = find_frame(s.prefix, s.n_frames, it);
+            fn(f, it - s.prefix[f]);
+        }
+    } else {
+        for (int it = b; it < s.total; it += G) {
+            int f = find_frame(s.prefix, s.n_frames, it);
+            fn(f, it - s.prefix[f]);
+        }
+    }
+}
+
 }  // namespace r2g

@@ -1,0 +1,130 @@
+// Host-side frame and context objects shared by the runtime sources.
+#pragma once
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace r2g {
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            set_last_error(std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+            return RAV2D_EDEVICE;                                                              \
+        }                                                                                      \
+    } while (0)
+
+
+}  // namespace r2g
+
+struct Rav2dGpuCtx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;  // parameter-only work (grain tables) overlapped with the passes
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t budget = 0;
+    uint64_t in_use = 0;
+    std::mutex mu;
+    // per-pass profiling: event pairs recorded around launches, resolved lazily
+    bool profiling = false;
+    struct Rec {
+        int pass;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> free_events;
+    double pass_ms[RAV2D_NUM_PASSES] = {0};
+    int64_t pass_count[RAV2D_NUM_PASSES] = {0};
+};
+
+namespace r2g {
+
+inline hipEvent_t take_event(Rav2dGpuCtx *c) {
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one pass launch with events when profiling is on.
+struct PassTimer {
+    Rav2dGpuCtx *c;
+    int pass;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    PassTimer(Rav2dGpuCtx *c_, int pass_, hipStream_t s_) : c(c_), pass(pass_), s(s_) {
+        if (c->profiling && s == c->stream) {
+            a = take_event(c);
+            if (a) (void)hipEventRecord(a, s);
+        }
+    }
+    ~PassTimer() {
+        if (!a) return;
+        hipEvent_t b = take_event(c);
+        if (!b) return;
+        (void)hipEventRecord(b, s);
+        c->recs.push_back({pass, a, b});
+    }
+};
+
+}  // namespace r2g
+
+// Buffer roles of a frame.  F0 is never written by a pass.
+enum { BUF_IN = 0, BUF_OUT = 1, BUF_DISP = 2, BUF_T1 = 3, BUF_T2 = 4, NBUF = 5 };
+
+struct Rav2dGpuFrame {
+    Rav2dGpuCtx *ctx = nullptr;
+    Rav2dFrameParams params;
+    r2g::DevGeom g;
+    int pw[3], ph[3], bps;
+    int64_t stride[3];            // bytes, per plane
+    size_t plane_bytes[3];
+    uint8_t *block = nullptr;     // one allocation: all plane buffers
+    uint8_t *buf[NBUF][3];
+    Rav2dAv2Filter *d_mask = nullptr;
+    size_t n_mask = 0;
+    Rav2dAv2Restoration *d_lr = nullptr;
+    size_t n_lr = 0;
+    r2g::DbLut *d_dblut = nullptr;
+    size_t bytes = 0;
+    bool side_ok = false;
+    // loop-restoration work list (rebuilt when params or side info change)
+    std::vector<Rav2dAv2Filter> h_mask;  // host copy (after the bottom crop)
+    std::vector<Rav2dAv2Restoration> h_lr;
+    std::vector<r2g::LrItem> lr_items;
+    r2g::LrItem *d_lr_items = nullptr;
+    size_t d_lr_cap = 0;
+    int lr_restore_planes = 0, lr_base_q = 0, lr_widx = 0, lr_ns_idx = 0;
+    bool lr_dirty = true;
+    uint8_t *d_grain = nullptr;  // grain LUTs + scaling + offsets scratch
+    hipEvent_t ev_setup = nullptr, ev_free = nullptr;
+    bool grain_ready = false, grain_used = false;
+    uint64_t gen = 0;  // bumped when parameters or side info change
+};
+
+namespace r2g {
+
+// Per-pass launch arguments of one frame's in-loop chain.
+struct ChainPlan {
+    bool on[4];         // deblock V, deblock H, CDEF, LR launched for this frame
+    DeblockArgs dbv, dbh;
+    CdefArgs cdef;
+    LrArgs lr;
+    int n_lr;           // restoration items (f->lr_items) of the LR launch
+    int final_buf[3];   // buffer that holds each plane after the launched passes
+};
+
+int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp);
+DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl);
+bool frame_has_grain(const Rav2dFrameParams &P);
+int grain_args(Rav2dGpuFrame *f, GrainArgs &a);
+hipStream_t pick_stream(Rav2dGpuCtx *ctx, void *stream);
+
+}  // namespace r2g

@@ -48,12 +48,20 @@ __device__ __forceinline__ uint32_t jump(int k, uint32_t s) {
 
 constexpr int kSetupThreads = 128;
 
-__global__ __launch_bounds__(kSetupThreads) void grain_setup_kernel(GrainArgs a, const int16_t *gauss) {
-    __shared__ int16_t lut[3][GH][GW];
-    __shared__ int16_t g_lds[2048];
-    __shared__ int8_t coef[3][28];  // AR coefficients staged out of the argument block
-    __shared__ uint8_t scl[3][1024];
-    __shared__ int8_t win[3][4][7];
+struct GrainSmem {
+    int16_t lut[3][GH][GW];
+    int16_t g_lds[2048];
+    int8_t coef[3][28];  // AR coefficients staged out of the argument block
+    uint8_t scl[3][1024];
+    int8_t win[3][4][7];
+};
+
+__device__ void grain_setup_body(const GrainArgs &a, const int16_t *gauss, GrainSmem &sm) {
+    auto &lut = sm.lut;
+    auto &g_lds = sm.g_lds;
+    auto &coef = sm.coef;
+    auto &scl = sm.scl;
+    auto &win = sm.win;
     const Rav2dFilmGrainData &d = a.fg;
     const int bd = a.bitdepth, bdm8 = bd - 8;
     const int shift = 4 - bdm8 + d.grain_scale_shift;
@@ -214,12 +222,23 @@ __global__ __launch_bounds__(kSetupThreads) void grain_setup_kernel(GrainArgs a,
     for (int i = t; i < 3 * GH * GW; i += kSetupThreads) a.lut[i] = (&lut[0][0][0])[i];
 }
 
+__global__ __launch_bounds__(kSetupThreads) void grain_setup_kernel(GrainArgs a, const int16_t *gauss) {
+    __shared__ GrainSmem sm;
+    grain_setup_body(a, gauss, sm);
+}
+
+// Batched set-up: one workgroup per grained frame.
+__global__ __launch_bounds__(kSetupThreads) void grain_setup_batch(const GrainArgs *args, const int16_t *gauss) {
+    __shared__ GrainSmem sm;
+    grain_setup_body(args[blockIdx.x], gauss, sm);
+}
+
 // One output sample: S(bxo, byo) samples the block `bxo` to the left and
 // `byo` above, as sample_lut with offsets[bxo][byo].
 template <typename P>
-__global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
+__device__ void grain_apply_body(const GrainArgs &a, int tile_id) {
     int pl, tx, ty;
-    decode_tile(a.map, blockIdx.x, pl, tx, ty);
+    decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl], &dst = a.dst[pl];
     const Rav2dFilmGrainData &d = a.fg;
     const int bd = a.bitdepth, bdm8 = bd - 8;
@@ -265,8 +284,6 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
         }
         int val = sv;
         if (pl) {
-            const int16_t *unused = nullptr;
-            (void)unused;
             const P *lrow = row_ptr<const P>(a.src[0], 2 * y);
             int l = lrow[2 * x];
             int avg = (l + lrow[2 * x + 1] + 1) >> 1;
@@ -285,8 +302,28 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
     }
 }
 
+template <typename P> __global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
+    grain_apply_body<P>(a, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void grain_apply_batch(const GrainArgs *args, BatchSched s) {
+    for_each_item(s, [&](int f, int t) {
+        const GrainArgs &a = args[f];
+        if (a.bitdepth > 8)
+            grain_apply_body<uint16_t>(a, t);
+        else
+            grain_apply_body<uint8_t>(a, t);
+    });
+}
+
+// Per device: the Gaussian sequence in device memory and the LFSR jump-ahead
+// matrices in constant memory.
 static const int16_t *grain_tables() {
-    static const int16_t *gauss = nullptr;
+    static const int16_t *per_dev[64] = {};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) return nullptr;
+    const int16_t *&gauss = per_dev[dev];
     if (!gauss) {
         int16_t *g = nullptr;
         if (hipMalloc((void **)&g, sizeof(R2T_GAUSSIAN_SEQUENCE)) != hipSuccess) return nullptr;
@@ -326,13 +363,30 @@ void launch_grain_setup(const GrainArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(grain_setup_kernel, dim3(1), dim3(kSetupThreads), 0, s, a, gauss);
 }
 
-void launch_grain_apply(GrainArgs a, int bitdepth, hipStream_t s) {
+bool grain_init_tables() { return grain_tables() != nullptr; }
+
+void launch_grain_setup_batch(const GrainArgs *d_args, int n, hipStream_t s) {
+    const int16_t *gauss = grain_tables();
+    if (!gauss || n <= 0) return;
+    hipLaunchKernelGGL(grain_setup_batch, dim3(n), dim3(kSetupThreads), 0, s, d_args, gauss);
+}
+
+void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &sc, int grid, hipStream_t s) {
+    if (sc.total <= 0) return;
+    hipLaunchKernelGGL(grain_apply_batch, dim3(grid), dim3(256), 0, s, d_args, sc);
+}
+
+void grain_tile_map(GrainArgs &a) {
     a.map.first[0] = 0;
     for (int p = 0; p < 3; p++) {
         a.map.tiles_x[p] = (a.src[p].w + 63) / 64;
         a.map.tiles_y[p] = (a.src[p].h + 31) / 32;
         a.map.first[p + 1] = a.map.first[p] + a.map.tiles_x[p] * a.map.tiles_y[p];
     }
+}
+
+void launch_grain_apply(GrainArgs a, int bitdepth, hipStream_t s) {
+    grain_tile_map(a);
     if (bitdepth > 8)
         hipLaunchKernelGGL(grain_apply_kernel<uint16_t>, dim3(a.map.first[3]), dim3(256), 0, s, a);
     else

@@ -46,16 +46,23 @@ __device__ __forceinline__ int qval_tskip(int qstep, int tskip, int i, int bdm8)
     return 255 * (kModeOff[i] + q);
 }
 
+struct LrSmem {
+    uint16_t V[VH * VW];
+    uint16_t Ld[(32 + 4) * (64 + 4)];
+    uint16_t cells[33 * 33 * 4];
+    uint8_t cls[16 * 16];
+    uint16_t s_alpha[88];
+    int16_t s_weight[264];
+    LrItem it;
+};
+
 template <typename P>
-__global__ __launch_bounds__(kThreads) void lr_kernel(LrArgs a) {
-    __shared__ uint16_t V[VH * VW];
-    __shared__ uint16_t Ld[(32 + 4) * (64 + 4)];
-    __shared__ uint16_t cells[33 * 33 * 4];
-    __shared__ uint8_t cls[16 * 16];
-    __shared__ uint16_t s_alpha[88];
-    __shared__ int16_t s_weight[264];
-    __shared__ LrItem it;
-    if (threadIdx.x < sizeof(LrItem) / 4) ((uint32_t *)&it)[threadIdx.x] = ((const uint32_t *)&a.items[blockIdx.x])[threadIdx.x];
+__device__ void lr_body(const LrArgs &a, int item, LrSmem &sm) {
+    uint16_t *V = sm.V, *Ld = sm.Ld, *cells = sm.cells, *s_alpha = sm.s_alpha;
+    uint8_t *cls = sm.cls;
+    int16_t *s_weight = sm.s_weight;
+    LrItem &it = sm.it;
+    if (threadIdx.x < sizeof(LrItem) / 4) ((uint32_t *)&it)[threadIdx.x] = ((const uint32_t *)&a.items[item])[threadIdx.x];
     __syncthreads();
     const int pl = it.plane, x0 = it.x0, w = it.w, y0 = it.y0, h = it.h;
     const uint8_t edges = it.edges;
@@ -295,6 +302,29 @@ __global__ __launch_bounds__(kThreads) void lr_kernel(LrArgs a) {
         }
         row_ptr<P>(dst, y0 + y)[x0 + x] = (P)out;
     }
+}
+
+template <typename P> __global__ __launch_bounds__(kThreads) void lr_kernel(LrArgs a) {
+    __shared__ LrSmem sm;
+    lr_body<P>(a, blockIdx.x, sm);
+}
+
+// Batched launch: every frame's items (restoration units and plane copies).
+__global__ __launch_bounds__(kThreads) void lr_batch(const LrArgs *args, BatchSched s) {
+    __shared__ LrSmem sm;
+    for_each_item(s, [&](int f, int t) {
+        const LrArgs &a = args[f];
+        if (a.g.bitdepth > 8)
+            lr_body<uint16_t>(a, t, sm);
+        else
+            lr_body<uint8_t>(a, t, sm);
+        __syncthreads();
+    });
+}
+
+void launch_lr_batch(const LrArgs *d_args, const BatchSched &sc, int grid, hipStream_t s) {
+    if (sc.total <= 0) return;
+    hipLaunchKernelGGL(lr_batch, dim3(grid), dim3(kThreads), 0, s, d_args, sc);
 }
 
 void launch_lr(const LrArgs &a, int n_items, int bitdepth, hipStream_t s) {

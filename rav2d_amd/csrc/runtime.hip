@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "internal.h"
+#include "frame.h"
 #include "tables.h"
 
 namespace r2g {
@@ -15,106 +15,9 @@ namespace r2g {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess) {                                                                \
-            set_last_error(std::string(#expr) + ": " + hipGetErrorString(e_));                 \
-            return RAV2D_EDEVICE;                                                              \
-        }                                                                                      \
-    } while (0)
-
 }  // namespace r2g
 
 using namespace r2g;
-
-struct Rav2dGpuCtx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t aux = nullptr;  // parameter-only work (grain tables) overlapped with the passes
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    uint64_t budget = 0;
-    uint64_t in_use = 0;
-    std::mutex mu;
-    // per-pass profiling: event pairs recorded around launches, resolved lazily
-    bool profiling = false;
-    struct Rec {
-        int pass;
-        hipEvent_t a, b;
-    };
-    std::vector<Rec> recs;
-    std::vector<hipEvent_t> free_events;
-    double pass_ms[RAV2D_NUM_PASSES] = {0};
-    int64_t pass_count[RAV2D_NUM_PASSES] = {0};
-};
-
-namespace {
-
-hipEvent_t take_event(Rav2dGpuCtx *c) {
-    if (!c->free_events.empty()) {
-        hipEvent_t e = c->free_events.back();
-        c->free_events.pop_back();
-        return e;
-    }
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    return e;
-}
-
-// Brackets one pass launch with events when profiling is on.
-struct PassTimer {
-    Rav2dGpuCtx *c;
-    int pass;
-    hipStream_t s;
-    hipEvent_t a = nullptr;
-    PassTimer(Rav2dGpuCtx *c_, int pass_, hipStream_t s_) : c(c_), pass(pass_), s(s_) {
-        if (c->profiling && s == c->stream) {
-            a = take_event(c);
-            if (a) (void)hipEventRecord(a, s);
-        }
-    }
-    ~PassTimer() {
-        if (!a) return;
-        hipEvent_t b = take_event(c);
-        if (!b) return;
-        (void)hipEventRecord(b, s);
-        c->recs.push_back({pass, a, b});
-    }
-};
-
-}  // namespace
-
-// Buffer roles of a frame.  F0 is never written by a pass.
-enum { BUF_IN = 0, BUF_OUT = 1, BUF_DISP = 2, BUF_T1 = 3, BUF_T2 = 4, NBUF = 5 };
-
-struct Rav2dGpuFrame {
-    Rav2dGpuCtx *ctx = nullptr;
-    Rav2dFrameParams params;
-    DevGeom g;
-    int pw[3], ph[3], bps;
-    int64_t stride[3];            // bytes, per plane
-    size_t plane_bytes[3];
-    uint8_t *block = nullptr;     // one allocation: all plane buffers
-    uint8_t *buf[NBUF][3];
-    Rav2dAv2Filter *d_mask = nullptr;
-    size_t n_mask = 0;
-    Rav2dAv2Restoration *d_lr = nullptr;
-    size_t n_lr = 0;
-    DbLut *d_dblut = nullptr;
-    size_t bytes = 0;
-    bool side_ok = false;
-    // loop-restoration work list (rebuilt when params or side info change)
-    std::vector<Rav2dAv2Filter> h_mask;  // host copy (after the bottom crop)
-    std::vector<Rav2dAv2Restoration> h_lr;
-    std::vector<LrItem> lr_items;
-    LrItem *d_lr_items = nullptr;
-    size_t d_lr_cap = 0;
-    int lr_restore_planes = 0, lr_base_q = 0, lr_widx = 0, lr_ns_idx = 0;
-    bool lr_dirty = true;
-    uint8_t *d_grain = nullptr;  // grain LUTs + scaling + offsets scratch
-    hipEvent_t ev_setup = nullptr, ev_free = nullptr;
-    bool grain_ready = false, grain_used = false;
-};
 
 namespace {
 
@@ -383,13 +286,23 @@ int upload_lr_items(Rav2dGpuFrame *f) {
     return RAV2D_OK;
 }
 
-DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl);
+}  // namespace
+
+namespace r2g {
 
 bool frame_has_grain(const Rav2dFrameParams &P) {
     const Rav2dFilmGrainData &d = P.grain;
     return P.grain_present && (d.num_points[0] || d.num_points[1] || d.num_points[2] ||
                                (d.clip_to_restricted_range && d.chroma_scaling_from_luma));
 }
+
+}  // namespace r2g
+
+namespace {
+
+}  // namespace
+
+namespace r2g {
 
 int grain_args(Rav2dGpuFrame *f, GrainArgs &a) {
     const Rav2dFrameParams &P = f->params;
@@ -433,6 +346,10 @@ int grain_args(Rav2dGpuFrame *f, GrainArgs &a) {
     return RAV2D_OK;
 }
 
+}  // namespace r2g
+
+namespace {
+
 // Grain tables depend on the frame parameters only: build them on the
 // auxiliary stream so they overlap the in-loop passes.
 int start_grain_setup(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, hipStream_t s) {
@@ -473,6 +390,10 @@ int validate(const Rav2dFrameParams *p) {
     return RAV2D_OK;
 }
 
+}  // namespace
+
+namespace r2g {
+
 DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl) {
     DevPlane d;
     d.ptr = f->buf[b][pl];
@@ -482,11 +403,19 @@ DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl) {
     return d;
 }
 
+}  // namespace r2g
+
+namespace {
+
 int copy_plane(Rav2dGpuFrame *f, int from, int to, int pl, hipStream_t s) {
     HIP_TRY(hipMemcpy2DAsync(f->buf[to][pl], f->stride[pl], f->buf[from][pl], f->stride[pl],
                              (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyDeviceToDevice, s));
     return RAV2D_OK;
 }
+
+}  // namespace
+
+namespace r2g {
 
 hipStream_t pick_stream(Rav2dGpuCtx *ctx, void *stream) { return stream ? (hipStream_t)stream : ctx->stream; }
 
@@ -508,59 +437,54 @@ Plan make_plan(const Rav2dFrameParams &p) {
     return pl;
 }
 
-// Runs the in-loop chain F0 -> OUT.  `upto` limits the passes (for per-pass
-// parity): passes with index > upto are skipped.
-int run_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, hipStream_t s) {
+}  // namespace r2g
+
+namespace {
+
+}  // namespace
+
+namespace r2g {
+
+// Launch arguments of the in-loop chain F0 -> OUT for one frame.  `upto`
+// limits the passes (for per-pass parity): passes with index > upto are not
+// planned.  Each launched pass reads a plane's current buffer and writes the
+// other temporary; LR writes OUT.
+int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
     if (!f->side_ok) {
         set_last_error("side info not set");
         return RAV2D_EINVAL;
     }
+    memset(&cp, 0, sizeof(cp));
     Plan plan = make_plan(f->params);
     int cur[3] = {BUF_IN, BUF_IN, BUF_IN};
-    // output buffer of a pass for a plane: a temporary other than `cur`
     auto next_tmp = [&](int c) { return c == BUF_T1 ? BUF_T2 : BUF_T1; };
-    // deblock V
-    if (upto >= RAV2D_PASS_DEBLOCK_V && (plan.dbv[0] || plan.dbv[1] || plan.dbv[2])) {
-        DeblockArgs a;
-        memset(&a, 0, sizeof(a));
+    auto deblock = [&](DeblockArgs &a, const bool *on) {
         a.g = f->g;
         a.mask = f->d_mask;
         a.lut = f->d_dblut;
         for (int pl = 0; pl < 3; pl++) {
-            a.plane_on[pl] = plan.dbv[pl];
+            a.plane_on[pl] = on[pl];
             int out = next_tmp(cur[pl]);
             a.src[pl] = plane_of(f, cur[pl], pl);
             a.dst[pl] = plane_of(f, out, pl);
             cur[pl] = out;
         }
-        PassTimer pt(ctx, RAV2D_PASS_DEBLOCK_V, s);
-        launch_deblock_v(a, f->g.bitdepth, s);
-        HIP_TRY(hipGetLastError());
+        a.map = deblock_tile_map(a);
+    };
+    if (upto >= RAV2D_PASS_DEBLOCK_V && (plan.dbv[0] || plan.dbv[1] || plan.dbv[2])) {
+        deblock(cp.dbv, plan.dbv);
+        cp.on[RAV2D_PASS_DEBLOCK_V] = true;
     }
     if (upto >= RAV2D_PASS_DEBLOCK_H && (plan.dbh[0] || plan.dbh[1] || plan.dbh[2])) {
-        DeblockArgs a;
-        memset(&a, 0, sizeof(a));
-        a.g = f->g;
-        a.mask = f->d_mask;
-        a.lut = f->d_dblut;
-        for (int pl = 0; pl < 3; pl++) {
-            a.plane_on[pl] = plan.dbh[pl];
-            int out = next_tmp(cur[pl]);
-            a.src[pl] = plane_of(f, cur[pl], pl);
-            a.dst[pl] = plane_of(f, out, pl);
-            cur[pl] = out;
-        }
-        PassTimer pt(ctx, RAV2D_PASS_DEBLOCK_H, s);
-        launch_deblock_h(a, f->g.bitdepth, s);
-        HIP_TRY(hipGetLastError());
+        deblock(cp.dbh, plan.dbh);
+        cp.on[RAV2D_PASS_DEBLOCK_H] = true;
     }
     int f2buf[3] = {cur[0], cur[1], cur[2]};  // deblocked planes, read by LR at stripe edges
     // CDEF + CCSO (decode.rs:3858-3997)
     const Rav2dFrameParams &P = f->params;
     bool cdef_stage = P.seq_cdef && (P.inloop_filters & (RAV2D_INLOOPFILTER_CDEF | RAV2D_INLOOPFILTER_CCSO));
     if (upto >= RAV2D_PASS_CDEF && cdef_stage) {
-        CdefArgs a;
-        memset(&a, 0, sizeof(a));
+        CdefArgs &a = cp.cdef;
         a.g = f->g;
         a.mask = f->d_mask;
         a.cdef_on = (P.inloop_filters & RAV2D_INLOOPFILTER_CDEF) != 0;
@@ -590,9 +514,7 @@ int run_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, hipStream_t s) {
             a.dst[pl] = plane_of(f, out, pl);
             cur[pl] = out;
         }
-        PassTimer pt(ctx, RAV2D_PASS_CDEF, s);
-        launch_cdef(a, f->g.bitdepth, s);
-        HIP_TRY(hipGetLastError());
+        cp.on[RAV2D_PASS_CDEF] = true;
     }
     // loop restoration (decode.rs:3999-4057), written straight into OUT
     if (f->lr_dirty) {
@@ -600,10 +522,8 @@ int run_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, hipStream_t s) {
         int r = upload_lr_items(f);
         if (r) return r;
     }
-    bool lr_stage = f->lr_restore_planes && (P.inloop_filters & (RAV2D_INLOOPFILTER_WIENER | RAV2D_INLOOPFILTER_GDF));
-    if (upto >= RAV2D_PASS_LR && lr_stage && !f->lr_items.empty()) {
-        LrArgs a;
-        memset(&a, 0, sizeof(a));
+    {
+        LrArgs &a = cp.lr;
         a.g = f->g;
         a.items = f->d_lr_items;
         int r = lr_tables(ctx->device, &a.t);
@@ -621,15 +541,50 @@ int run_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, hipStream_t s) {
             a.f2[pl] = plane_of(f, f2buf[pl], pl);
             a.dst[pl] = plane_of(f, BUF_OUT, pl);
         }
-        PassTimer pt(ctx, RAV2D_PASS_LR, s);
-        launch_lr(a, (int)f->lr_items.size(), f->g.bitdepth, s);
-        HIP_TRY(hipGetLastError());
+    }
+    bool lr_stage = f->lr_restore_planes && (P.inloop_filters & (RAV2D_INLOOPFILTER_WIENER | RAV2D_INLOOPFILTER_GDF));
+    if (upto >= RAV2D_PASS_LR && lr_stage && !f->lr_items.empty()) {
+        cp.on[RAV2D_PASS_LR] = true;
+        cp.n_lr = (int)f->lr_items.size();
         for (int pl = 0; pl < 3; pl++)
             if (f->lr_restore_planes & (1 << pl)) cur[pl] = BUF_OUT;
     }
+    for (int pl = 0; pl < 3; pl++) cp.final_buf[pl] = cur[pl];
+    return RAV2D_OK;
+}
+
+}  // namespace r2g
+
+namespace {
+
+int run_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, hipStream_t s) {
+    ChainPlan cp;
+    int r = plan_chain(ctx, f, upto, cp);
+    if (r) return r;
+    const int bd = f->g.bitdepth;
+    if (cp.on[RAV2D_PASS_DEBLOCK_V]) {
+        PassTimer pt(ctx, RAV2D_PASS_DEBLOCK_V, s);
+        launch_deblock_v(cp.dbv, bd, s);
+        HIP_TRY(hipGetLastError());
+    }
+    if (cp.on[RAV2D_PASS_DEBLOCK_H]) {
+        PassTimer pt(ctx, RAV2D_PASS_DEBLOCK_H, s);
+        launch_deblock_h(cp.dbh, bd, s);
+        HIP_TRY(hipGetLastError());
+    }
+    if (cp.on[RAV2D_PASS_CDEF]) {
+        PassTimer pt(ctx, RAV2D_PASS_CDEF, s);
+        launch_cdef(cp.cdef, bd, s);
+        HIP_TRY(hipGetLastError());
+    }
+    if (cp.on[RAV2D_PASS_LR]) {
+        PassTimer pt(ctx, RAV2D_PASS_LR, s);
+        launch_lr(cp.lr, cp.n_lr, bd, s);
+        HIP_TRY(hipGetLastError());
+    }
     for (int pl = 0; pl < 3; pl++) {
-        if (cur[pl] == BUF_OUT) continue;
-        int r = copy_plane(f, cur[pl], BUF_OUT, pl, s);
+        if (cp.final_buf[pl] == BUF_OUT) continue;
+        r = copy_plane(f, cp.final_buf[pl], BUF_OUT, pl, s);
         if (r) return r;
     }
     return RAV2D_OK;
@@ -646,6 +601,7 @@ size_t rav2d_gpu_struct_size(int which) {
         case 2: return sizeof(Rav2dFrameParams);
         case 3: return sizeof(Rav2dFilmGrainData);
         case 4: return sizeof(Rav2dPlanes);
+        case 5: return sizeof(Rav2dBatchConfig);
         default: return 0;
     }
 }
@@ -822,6 +778,7 @@ int rav2d_gpu_frame_set_params(Rav2dGpuFrame *f, const Rav2dFrameParams *params)
     }
     f->params = *params;
     f->lr_dirty = true;
+    f->gen++;
     DbLut lut;
     build_db_lut(*params, f->g.hbd, lut);
     HIP_TRY(hipSetDevice(f->ctx->device));
@@ -865,6 +822,7 @@ int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, 
     }
     HIP_TRY(hipStreamSynchronize(s));  // host copies are pageable
     f->side_ok = true;
+    f->gen++;
     return RAV2D_OK;
 }
 
