@@ -9,10 +9,13 @@ the algorithmic byte counts and the CPU baseline).
 A second leg times a mixed batch of 64 frames (sizes uniform over 360p..2160p,
 8/10-bit 50/50, grain on the 10-bit frames) as 64 sequential per-frame
 launches and as one launch per pass (autotuned, hipGraph replay), and checks
-both give the same planes.
+both give the same planes.  A third leg serves 10,000 jobs from 32 streams
+with Poisson arrivals (60 fps each) under an 80 GB HBM budget through the
+continuous-batching server and reports p50/p99 latency; every job's planes
+are hashed on the device and compared with the same frame filtered alone.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 4k10|1080p10|cif8]
-                    [--batch 64] [--batch-steps 5]
+                    [--batch 64] [--batch-steps 5] [--serve-jobs 10000] [--serve-streams 32]
 """
 import argparse
 import json
@@ -89,13 +92,16 @@ def batch_workload(n, seed):
     return out
 
 
-def bench_batch(ctx, n, steps, warmup, seed):
-    """64-frame mixed batch: per-frame launches vs one launch per pass (graph)."""
-    import hashlib
+def make_sources(spec):
     from concurrent.futures import ThreadPoolExecutor
-    spec = batch_workload(n, seed)
     with ThreadPoolExecutor(8) as ex:
-        sfs = list(ex.map(lambda c: R.SynthFrame(c[0], c[1], c[2], c[3], c[4], tools=c[5]), spec))
+        return list(ex.map(lambda c: R.SynthFrame(c[0], c[1], c[2], c[3], c[4], tools=c[5]), spec))
+
+
+def bench_batch(ctx, spec, sfs, steps, warmup):
+    """Mixed batch: per-frame launches vs one launch per pass (graph)."""
+    import hashlib
+    n = len(spec)
     pix = sum(c[0] * c[1] for c in spec)
     A = [R.GpuFrame.from_synth(ctx, sf) for sf in sfs]  # per-frame form
     B = [R.GpuFrame.from_synth(ctx, sf) for sf in sfs]  # batched form
@@ -154,8 +160,6 @@ def bench_batch(ctx, n, steps, warmup, seed):
     b.close()
     for f in A + B:
         f.close()
-    for sf in sfs:
-        sf.close()
     res = {
         "frames": n,
         "mix": {"%dx%d" % s: sum(1 for c in spec if (c[0], c[1]) == s) for s in BATCH_SIZES},
@@ -175,6 +179,53 @@ def bench_batch(ctx, n, steps, warmup, seed):
     return res, times, pass_bytes
 
 
+def bench_serve(spec, sfs, n_streams, slots, fps, n_jobs, seed, budget):
+    """Serving: n_streams Poisson streams (stream i = source i of the mixed
+    spec), `slots` device-resident frames each, continuous batching, every
+    job's planes hashed on the device and checked against the frame alone."""
+    import numpy as np
+    ctx = R.GpuContext(0, hbm_budget=budget)
+    frames, alone = [], []
+    try:
+        for i in range(n_streams):
+            fs = [R.GpuFrame.from_synth(ctx, sfs[i]) for _ in range(slots)]
+            frames.append(fs)
+            fs[0].filter()
+            fs[0].apply_grain()
+            alone.append((fs[0].checksum(1), fs[0].checksum(2)))
+        held = ctx.bytes_in_use()
+        cfg = R.default_serve_config()
+        cfg.checksum = 1
+        srv = R.GpuServer(ctx, cfg)
+        try:
+            res, js, jl = srv.simulate(frames, slots, fps, n_jobs, seed)
+        finally:
+            srv.close()
+        lat = np.array([r.t_done - r.t_submit for r in res]) * 1e3
+        ok = all(r.status == 0 and (r.checksum[0], r.checksum[1]) == alone[js[j]] for j, r in enumerate(res))
+        t0 = min(r.t_submit for r in res)
+        t1 = max(r.t_done for r in res)
+        pix = sum(spec[js[j]][0] * spec[js[j]][1] for j in range(n_jobs))
+        bsz = np.array([r.batch_frames for r in res])
+        return {
+            "streams": n_streams, "slots_per_stream": slots, "fps_per_stream": fps, "jobs": n_jobs,
+            "hbm_budget_GB": budget / 1e9, "hbm_held_GB": round(held / 1e9, 3),
+            "max_batch": cfg.max_batch, "max_inflight": cfg.max_inflight,
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "max_ms": round(float(lat.max()), 3),
+            "mpix_s": round(pix / 1e6 / (t1 - t0), 1),
+            "offered_mpix_s": round(pix / 1e6 / (max(r.t_submit for r in res) - t0), 1),
+            "mean_batch": round(float(bsz.mean()), 2),
+            "all_jobs_equal_alone": bool(ok),
+        }
+    finally:
+        for fs in frames:
+            for f in fs:
+                f.close()
+        ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +236,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--batch", type=int, default=64, help="frames of the mixed batch leg (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--serve-jobs", type=int, default=10000, help="jobs of the serving leg (0 = skip)")
+    ap.add_argument("--serve-streams", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
@@ -237,10 +290,21 @@ def main():
     frame_bytes = S * bps * (2 + (1 if grain else 0))
     chain_gbs = frame_bytes / (ms_per_step * 1e-3) / 1e9
 
-    batch = None
-    if args.batch > 0:
-        print("bench: mixed batch of %d frames" % args.batch, file=sys.stderr, flush=True)
-        batch, _, _ = bench_batch(ctx, args.batch, args.batch_steps, 1, args.seed)
+    batch = serve = None
+    n_src = max(args.batch, args.serve_streams if args.serve_jobs > 0 else 0)
+    if n_src > 0:
+        spec = batch_workload(n_src, args.seed)
+        print("bench: %d synthetic mixed frames" % n_src, file=sys.stderr, flush=True)
+        sfs = make_sources(spec)
+        if args.batch > 0:
+            print("bench: mixed batch of %d frames" % args.batch, file=sys.stderr, flush=True)
+            batch, _, _ = bench_batch(ctx, spec[:args.batch], sfs[:args.batch], args.batch_steps, 1)
+        if args.serve_jobs > 0:
+            print("bench: serving %d jobs from %d streams" % (args.serve_jobs, args.serve_streams),
+                  file=sys.stderr, flush=True)
+            serve = bench_serve(spec, sfs, args.serve_streams, 2, 60.0, args.serve_jobs, args.seed, 80 * 10**9)
+        for x in sfs:
+            x.close()
 
     cpu = None
     if not args.no_cpu:
@@ -271,6 +335,7 @@ def main():
                      "traffic": None},
         "cpu_baseline": cpu,
         "batch": batch,
+        "serving": serve,
     }
     f.close()
     ctx.close()

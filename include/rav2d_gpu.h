@@ -296,10 +296,68 @@ int rav2d_gpu_batch_info(const Rav2dGpuBatch *b, int pass, int64_t *items, int64
 /* Luma pixels of all frames of the batch. */
 int64_t rav2d_gpu_batch_pixels(const Rav2dGpuBatch *b);
 
+/* ---- serving ----------------------------------------------------------------
+ * No counterpart in the reference (a decoder library: one Decoder per stream,
+ * frames filtered in decode order).  A server takes filter jobs from many
+ * streams as they arrive.  A job names a device-resident frame whose input
+ * planes, side info and parameters are set; it produces what
+ * rav2d_gpu_filter_frame (+ rav2d_gpu_apply_grain with apply_grain) would.
+ * A dispatcher thread owns max_inflight launch slots (one HIP stream each);
+ * whenever a slot is free it gathers up to max_batch waiting jobs, oldest
+ * first, into one batched launch per pass, so jobs that arrive while other
+ * batches run are admitted into the next batch and overlap them on the
+ * device (continuous batching).  A job waits while an earlier job of the same
+ * frame is queued or running.  Memory admission is the context's HBM budget:
+ * rav2d_gpu_frame_create returns RAV2D_EAGAIN once the budget cannot hold
+ * another frame.  The caller must not change a frame while it has jobs. */
+typedef struct Rav2dGpuServer Rav2dGpuServer;
+typedef struct Rav2dServeConfig {
+    int32_t max_batch;     /* jobs per batched launch (1 = one frame per launch) */
+    int32_t max_inflight;  /* batches on the device at once, 1..8 */
+    int32_t apply_grain;   /* also write the display planes */
+    int32_t checksum;      /* hash each job's output and display planes */
+} Rav2dServeConfig;
+typedef struct Rav2dServeResult {
+    uint64_t tag;
+    double t_submit;       /* arrival, seconds on the server clock */
+    double t_launch;       /* its batch was enqueued */
+    double t_done;         /* its batch was seen complete */
+    uint64_t checksum[2];  /* output, display planes (rav2d_gpu_frame_checksum) */
+    int32_t status;        /* RAV2D_OK or the error of its batch */
+    int32_t batch_frames;  /* jobs in its batch */
+} Rav2dServeResult;
+/* max_batch 16, max_inflight 3, apply_grain 1, checksum 0. */
+void rav2d_gpu_server_default_config(Rav2dServeConfig *cfg);
+int rav2d_gpu_server_create(Rav2dGpuCtx *ctx, const Rav2dServeConfig *cfg, Rav2dGpuServer **out);
+/* Finishes every submitted job, then stops the dispatcher. */
+int rav2d_gpu_server_destroy(Rav2dGpuServer *s);
+/* Seconds since the server was created. */
+double rav2d_gpu_server_clock(const Rav2dGpuServer *s);
+/* Queue a job; t_arrival < 0 stamps it with the current clock. */
+int rav2d_gpu_server_submit(Rav2dGpuServer *s, Rav2dGpuFrame *f, uint64_t tag, double t_arrival);
+/* Up to max completed jobs (waiting up to wait_ms for the first); returns the
+ * number written, or a negative error code. */
+int rav2d_gpu_server_poll(Rav2dGpuServer *s, Rav2dServeResult *out, int max, int wait_ms);
+/* Blocks until every submitted job has completed. */
+int rav2d_gpu_server_drain(Rav2dGpuServer *s);
+/* Simulated streams: stream i owns frames[i*slots .. i*slots + slots-1],
+ * set up by the caller.  Each stream's frames arrive as a Poisson process of
+ * rate fps (seeded); job j filters its stream's next slot in turn.  Submits
+ * n_jobs jobs at their arrival times and returns when all completed;
+ * results[j] is job j's record, job_stream/job_slot (optional) its stream
+ * and slot. */
+int rav2d_gpu_server_simulate(Rav2dGpuServer *s, Rav2dGpuFrame *const *frames, int n_streams,
+                              int slots, double fps, int n_jobs, uint64_t seed,
+                              Rav2dServeResult *results, int32_t *job_stream, int32_t *job_slot);
+/* Order-independent 64-bit hash of a frame's planes (which = 0 input,
+ * 1 output, 2 display): the sum over samples of splitmix64(plane << 60 |
+ * y << 40 | x << 20 | value).  Synchronizes the context stream. */
+int rav2d_gpu_frame_checksum(Rav2dGpuFrame *f, int which, uint64_t *out);
+
 /* sizeof() of the ABI structs, so bindings can check their mirrors:
  * 0 Rav2dAv2Filter, 1 Rav2dAv2Restoration, 2 Rav2dFrameParams,
- * 3 Rav2dFilmGrainData, 4 Rav2dPlanes, 5 Rav2dBatchConfig; 0 for an unknown
- * index. */
+ * 3 Rav2dFilmGrainData, 4 Rav2dPlanes, 5 Rav2dBatchConfig, 6 Rav2dServeConfig,
+ * 7 Rav2dServeResult; 0 for an unknown index. */
 size_t rav2d_gpu_struct_size(int which);
 
 const char *rav2d_gpu_version(void);

@@ -100,6 +100,27 @@ class BatchConfig(C.Structure):
                 "grid": list(self.grid), "xcd_remap": list(self.xcd_remap)}
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [
+        ("max_batch", C.c_int32),
+        ("max_inflight", C.c_int32),
+        ("apply_grain", C.c_int32),
+        ("checksum", C.c_int32),
+    ]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [
+        ("tag", C.c_uint64),
+        ("t_submit", C.c_double),
+        ("t_launch", C.c_double),
+        ("t_done", C.c_double),
+        ("checksum", C.c_uint64 * 2),
+        ("status", C.c_int32),
+        ("batch_frames", C.c_int32),
+    ]
+
+
 class SynthConfig(C.Structure):
     _fields_ = [
         ("width", C.c_int32), ("height", C.c_int32), ("bitdepth", C.c_int32), ("sb128", C.c_int32),
@@ -169,6 +190,17 @@ def load_gpu():
     _declare(lib, "rav2d_gpu_batch_info", C.c_int,
              [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)])
     _declare(lib, "rav2d_gpu_batch_pixels", C.c_int64, [P])
+    _declare(lib, "rav2d_gpu_server_default_config", None, [C.POINTER(ServeConfig)])
+    _declare(lib, "rav2d_gpu_server_create", C.c_int, [P, C.POINTER(ServeConfig), C.POINTER(P)])
+    _declare(lib, "rav2d_gpu_server_destroy", C.c_int, [P])
+    _declare(lib, "rav2d_gpu_server_clock", C.c_double, [P])
+    _declare(lib, "rav2d_gpu_server_submit", C.c_int, [P, P, C.c_uint64, C.c_double])
+    _declare(lib, "rav2d_gpu_server_poll", C.c_int, [P, C.POINTER(ServeResult), C.c_int, C.c_int])
+    _declare(lib, "rav2d_gpu_server_drain", C.c_int, [P])
+    _declare(lib, "rav2d_gpu_server_simulate", C.c_int,
+             [P, C.POINTER(P), C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, C.POINTER(ServeResult),
+              C.POINTER(C.c_int32), C.POINTER(C.c_int32)])
+    _declare(lib, "rav2d_gpu_frame_checksum", C.c_int, [P, C.c_int, C.POINTER(C.c_uint64)])
     _gpu = lib
     return lib
 
@@ -345,6 +377,11 @@ class GpuFrame:
                                                  arrs[2].ctypes.data, None), "download")
         return arrs
 
+    def checksum(self, which=1):
+        out = C.c_uint64()
+        _check(self.lib.rav2d_gpu_frame_checksum(self.h, which, C.byref(out)), "frame_checksum")
+        return out.value
+
     def close(self):
         if self.h:
             self.lib.rav2d_gpu_frame_destroy(self.h)
@@ -399,4 +436,76 @@ class GpuBatch:
     def close(self):
         if self.h:
             self.lib.rav2d_gpu_batch_destroy(self.h)
+            self.h = None
+
+
+def plane_checksum(planes):
+    """Host restatement of rav2d_gpu_frame_checksum over numpy planes [y, u, v]."""
+    M = np.uint64((1 << 64) - 1)
+    total = 0
+    for pl, a in enumerate(planes):
+        h, w = a.shape
+        y = np.arange(h, dtype=np.uint64)[:, None]
+        x = np.arange(w, dtype=np.uint64)[None, :]
+        z = (np.uint64(pl) << np.uint64(60)) | (y << np.uint64(40)) | (x << np.uint64(20)) | a.astype(np.uint64)
+        with np.errstate(over="ignore"):
+            z = z + np.uint64(0x9e3779b97f4a7c15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+            z = z ^ (z >> np.uint64(31))
+        total = (total + int(z.sum(dtype=np.uint64))) & int(M)
+    return total
+
+
+def default_serve_config():
+    cfg = ServeConfig()
+    load_gpu().rav2d_gpu_server_default_config(C.byref(cfg))
+    return cfg
+
+
+class GpuServer:
+    """Continuous-batching request loop (rav2d_gpu_server_*)."""
+
+    def __init__(self, ctx, config=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        cfg = config if config is not None else default_serve_config()
+        h = C.c_void_p()
+        _check(self.lib.rav2d_gpu_server_create(ctx.h, C.byref(cfg), C.byref(h)), "server_create")
+        self.h = h
+
+    def clock(self):
+        return self.lib.rav2d_gpu_server_clock(self.h)
+
+    def submit(self, frame, tag, t_arrival=-1.0):
+        _check(self.lib.rav2d_gpu_server_submit(self.h, frame.h, tag, t_arrival), "server_submit")
+
+    def poll(self, max_results=256, wait_ms=0):
+        buf = (ServeResult * max_results)()
+        n = self.lib.rav2d_gpu_server_poll(self.h, buf, max_results, wait_ms)
+        if n < 0:
+            _check(n, "server_poll")
+        return [buf[i] for i in range(n)]
+
+    def drain(self):
+        _check(self.lib.rav2d_gpu_server_drain(self.h), "server_drain")
+
+    def simulate(self, stream_frames, slots, fps, n_jobs, seed):
+        """stream_frames: list (per stream) of `slots` GpuFrames.  Returns
+        (results, job_stream, job_slot) as numpy arrays / ServeResult list."""
+        n_streams = len(stream_frames)
+        flat = [f for fs in stream_frames for f in fs]
+        assert all(len(fs) == slots for fs in stream_frames)
+        arr = (C.c_void_p * len(flat))(*[f.h.value for f in flat])
+        res = (ServeResult * n_jobs)()
+        js = np.zeros(n_jobs, dtype=np.int32)
+        jl = np.zeros(n_jobs, dtype=np.int32)
+        _check(self.lib.rav2d_gpu_server_simulate(
+            self.h, arr, n_streams, slots, fps, n_jobs, seed, res,
+            js.ctypes.data_as(C.POINTER(C.c_int32)), jl.ctypes.data_as(C.POINTER(C.c_int32))), "server_simulate")
+        return res, js, jl
+
+    def close(self):
+        if self.h:
+            self.lib.rav2d_gpu_server_destroy(self.h)
             self.h = None

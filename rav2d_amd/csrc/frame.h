@@ -60,7 +60,7 @@ struct PassTimer {
     hipStream_t s;
     hipEvent_t a = nullptr;
     PassTimer(Rav2dGpuCtx *c_, int pass_, hipStream_t s_) : c(c_), pass(pass_), s(s_) {
-        if (c->profiling && s == c->stream) {
+        if (c && c->profiling && s == c->stream) {
             a = take_event(c);
             if (a) (void)hipEventRecord(a, s);
         }
@@ -102,6 +102,7 @@ struct Rav2dGpuFrame {
     r2g::LrItem *d_lr_items = nullptr;
     size_t d_lr_cap = 0;
     int lr_restore_planes = 0, lr_base_q = 0, lr_widx = 0, lr_ns_idx = 0;
+    int lr_n_restore = 0;  // lr_items: restoration items first, then plane copies
     bool lr_dirty = true;
     uint8_t *d_grain = nullptr;  // grain LUTs + scaling + offsets scratch
     hipEvent_t ev_setup = nullptr, ev_free = nullptr;
@@ -117,7 +118,7 @@ struct ChainPlan {
     DeblockArgs dbv, dbh;
     CdefArgs cdef;
     LrArgs lr;
-    int n_lr;           // restoration items (f->lr_items) of the LR launch
+    int n_lr;           // items of the LR launch (restoration units and plane copies)
     int final_buf[3];   // buffer that holds each plane after the launched passes
 };
 

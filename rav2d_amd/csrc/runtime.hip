@@ -243,6 +243,23 @@ void lr_plane_items(const Rav2dGpuFrame *f, int pl, int y, int w, int h, int row
     emit(x, w - x, edges);
 }
 
+// Copy items for plane `pl`: the whole plane from the LR launch's input to
+// OUT in 256x64 blocks (items of kind NONE without GDF).
+void copy_items(const Rav2dGpuFrame *f, int pl, std::vector<LrItem> &out) {
+    for (int y = 0; y < f->ph[pl]; y += 64)
+        for (int x = 0; x < f->pw[pl]; x += 256) {
+            LrItem it;
+            memset(&it, 0, sizeof(it));
+            it.plane = (int16_t)pl;
+            it.x0 = (int16_t)x;
+            it.w = (int16_t)std::min(256, f->pw[pl] - x);
+            it.y0 = (int16_t)y;
+            it.h = (int16_t)std::min(64, f->ph[pl] - y);
+            it.kind = LR_KIND_NONE;
+            out.push_back(it);
+        }
+}
+
 // lr_sbrow_apply (looprestoration.rs:2401-2548) over every superblock row.
 void build_lr_items(Rav2dGpuFrame *f) {
     const DevGeom &g = f->g;
@@ -270,15 +287,32 @@ void build_lr_items(Rav2dGpuFrame *f) {
             lr_plane_items(f, 0, ys, w, h, row_h, first, f->lr_items);
         }
     }
+    // planes the LR launch does not restore get copy items, so that one launch
+    // always leaves all three output planes written
+    const Rav2dFrameParams &P = f->params;
+    const bool lr_stage = rp && (P.inloop_filters & (RAV2D_INLOOPFILTER_WIENER | RAV2D_INLOOPFILTER_GDF));
+    if (!lr_stage) f->lr_items.clear();
+    f->lr_n_restore = (int)f->lr_items.size();
+    for (int pl = 0; pl < 3; pl++)
+        if (!lr_stage || !(rp & (1 << pl))) copy_items(f, pl, f->lr_items);
 }
 
 int upload_lr_items(Rav2dGpuFrame *f) {
     size_t n = f->lr_items.size();
     if (n > f->d_lr_cap) {
-        if (f->d_lr_items) (void)hipFree(f->d_lr_items);
+        if (f->d_lr_items) {
+            (void)hipFree(f->d_lr_items);
+            std::lock_guard<std::mutex> lk(f->ctx->mu);
+            f->ctx->in_use -= f->d_lr_cap * sizeof(LrItem);
+            f->bytes -= f->d_lr_cap * sizeof(LrItem);
+        }
         f->d_lr_items = nullptr;
+        f->d_lr_cap = 0;
         size_t cap = n + n / 4 + 16;
         HIP_TRY(hipMalloc((void **)&f->d_lr_items, cap * sizeof(LrItem)));
+        std::lock_guard<std::mutex> lk(f->ctx->mu);
+        f->ctx->in_use += cap * sizeof(LrItem);
+        f->bytes += cap * sizeof(LrItem);
         f->d_lr_cap = cap;
     }
     if (n) HIP_TRY(hipMemcpy(f->d_lr_items, f->lr_items.data(), n * sizeof(LrItem), hipMemcpyHostToDevice));
@@ -542,12 +576,11 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
             a.dst[pl] = plane_of(f, BUF_OUT, pl);
         }
     }
-    bool lr_stage = f->lr_restore_planes && (P.inloop_filters & (RAV2D_INLOOPFILTER_WIENER | RAV2D_INLOOPFILTER_GDF));
-    if (upto >= RAV2D_PASS_LR && lr_stage && !f->lr_items.empty()) {
+    // the LR launch restores what the stage restores and copies the rest
+    if (upto >= RAV2D_PASS_LR) {
         cp.on[RAV2D_PASS_LR] = true;
         cp.n_lr = (int)f->lr_items.size();
-        for (int pl = 0; pl < 3; pl++)
-            if (f->lr_restore_planes & (1 << pl)) cur[pl] = BUF_OUT;
+        for (int pl = 0; pl < 3; pl++) cur[pl] = BUF_OUT;
     }
     for (int pl = 0; pl < 3; pl++) cp.final_buf[pl] = cur[pl];
     return RAV2D_OK;
@@ -602,6 +635,8 @@ size_t rav2d_gpu_struct_size(int which) {
         case 3: return sizeof(Rav2dFilmGrainData);
         case 4: return sizeof(Rav2dPlanes);
         case 5: return sizeof(Rav2dBatchConfig);
+        case 6: return sizeof(Rav2dServeConfig);
+        case 7: return sizeof(Rav2dServeResult);
         default: return 0;
     }
 }
@@ -702,7 +737,16 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
     }
     const size_t grain_bytes = 3 * 73 * 82 * 2 + 3 * 1024 +
                                (size_t)((params->height + 15) / 16) * ((params->width + 15) / 16) * 2 + 256;
-    size_t need = total * NBUF + sizeof(DbLut) + grain_bytes;
+    // side info and the LR work list are reserved here too, so that the budget
+    // check covers everything the frame will hold
+    const size_t n_sb = (size_t)g.sb256w * g.sb256h;
+    const int sbrows = (params->height + (64 << params->sb128) - 1) / (64 << params->sb128);
+    size_t lr_cap = 16;
+    for (int pl = 0; pl < 3; pl++)
+        lr_cap += (size_t)((f->pw[pl] + 63) / 64) * sbrows * 3 +
+                  (size_t)((f->pw[pl] + 255) / 256) * ((f->ph[pl] + 63) / 64);
+    const size_t side_bytes = n_sb * (sizeof(Rav2dAv2Filter) + sizeof(Rav2dAv2Restoration));
+    size_t need = total * NBUF + sizeof(DbLut) + grain_bytes + side_bytes + lr_cap * sizeof(LrItem);
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         if (ctx->budget && ctx->in_use + need > ctx->budget) {
@@ -715,9 +759,13 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
     f->bytes = need;
     if (hipMalloc((void **)&f->block, total * NBUF) != hipSuccess ||
         hipMalloc((void **)&f->d_dblut, sizeof(DbLut)) != hipSuccess ||
-        hipMalloc((void **)&f->d_grain, grain_bytes) != hipSuccess) {
-        if (f->block) (void)hipFree(f->block);
-        if (f->d_dblut) (void)hipFree(f->d_dblut);
+        hipMalloc((void **)&f->d_grain, grain_bytes) != hipSuccess ||
+        hipMalloc((void **)&f->d_mask, n_sb * sizeof(Rav2dAv2Filter)) != hipSuccess ||
+        hipMalloc((void **)&f->d_lr, n_sb * sizeof(Rav2dAv2Restoration)) != hipSuccess ||
+        hipMalloc((void **)&f->d_lr_items, lr_cap * sizeof(LrItem)) != hipSuccess) {
+        for (void *p : {(void *)f->block, (void *)f->d_dblut, (void *)f->d_grain, (void *)f->d_mask, (void *)f->d_lr,
+                        (void *)f->d_lr_items})
+            if (p) (void)hipFree(p);
         {
             std::lock_guard<std::mutex> lk(ctx->mu);
             ctx->in_use -= need;
@@ -731,6 +779,7 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
         set_last_error("event creation failed");
         return RAV2D_EDEVICE;
     }
+    f->d_lr_cap = lr_cap;
     uint8_t *p = f->block;
     for (int b = 0; b < NBUF; b++)
         for (int pl = 0; pl < 3; pl++) {
@@ -796,12 +845,6 @@ int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, 
     }
     HIP_TRY(hipSetDevice(f->ctx->device));
     hipStream_t s = pick_stream(f->ctx, stream);
-    if (!f->d_mask) {
-        HIP_TRY(hipMalloc((void **)&f->d_mask, need * sizeof(Rav2dAv2Filter)));
-        std::lock_guard<std::mutex> lk(f->ctx->mu);
-        f->ctx->in_use += need * sizeof(Rav2dAv2Filter);
-        f->bytes += need * sizeof(Rav2dAv2Filter);
-    }
     std::vector<Rav2dAv2Filter> copy(mask, mask + n_mask);
     crop_bottom(copy, f->g);
     HIP_TRY(hipMemcpyAsync(f->d_mask, copy.data(), need * sizeof(Rav2dAv2Filter), hipMemcpyHostToDevice, s));
@@ -812,12 +855,6 @@ int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, 
         f->h_lr.clear();
     f->lr_dirty = true;
     if (lr) {
-        if (!f->d_lr) {
-            HIP_TRY(hipMalloc((void **)&f->d_lr, need * sizeof(Rav2dAv2Restoration)));
-            std::lock_guard<std::mutex> lk(f->ctx->mu);
-            f->ctx->in_use += need * sizeof(Rav2dAv2Restoration);
-            f->bytes += need * sizeof(Rav2dAv2Restoration);
-        }
         HIP_TRY(hipMemcpyAsync(f->d_lr, lr, need * sizeof(Rav2dAv2Restoration), hipMemcpyHostToDevice, s));
     }
     HIP_TRY(hipStreamSynchronize(s));  // host copies are pageable
