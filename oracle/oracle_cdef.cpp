@@ -501,3 +501,27 @@ template int cdef_find_dir<uint8_t>(const uint8_t *, ptrdiff_t, uint32_t *, int)
 template int cdef_find_dir<uint16_t>(const uint16_t *, ptrdiff_t, uint32_t *, int);
 
 }  // namespace oracle
+
+/* Known-answer hooks (cdef.rs:117-120, 134-216, 226-236; ccso.rs:16-24, 115-186, 215-254). */
+extern "C" int oracle_cdef_constrain(int diff, int threshold, int shift) { return oracle::constrain(diff, threshold, shift); }
+extern "C" int oracle_cdef_adjust_strength(int strength, uint32_t var) { return oracle::cdef_adjust_strength(strength, var); }
+extern "C" int oracle_cdef_find_dir(const void *img, int hbd, ptrdiff_t stride, int bdm8, uint32_t *var) {
+    if (hbd) return oracle::cdef_find_dir((const uint16_t *)img, stride, var, bdm8);
+    return oracle::cdef_find_dir((const uint8_t *)img, stride, var, bdm8);
+}
+extern "C" uint32_t oracle_ccso_score(int diff, int quant_step, uint32_t edge_clf) {
+    return oracle::ccso_score(diff, quant_step, edge_clf);
+}
+extern "C" void oracle_ccso_prep_8(uint8_t *dst, int dst_stride, const uint8_t *src, ptrdiff_t ss, ptrdiff_t src_off,
+                                   const uint8_t (*left)[2], const uint8_t *top, ptrdiff_t top_off,
+                                   const uint8_t *bottom, ptrdiff_t bot_off, uint32_t max_band_log2, int ext,
+                                   int quant_step, uint32_t edge_clf, int bo_only, int edges, int w, int h,
+                                   int ss_hor, int ss_ver) {
+    oracle::ccso_prep(dst, dst_stride, src, ss, src_off, left, top, top_off, bottom, bot_off, max_band_log2, ext,
+                      quant_step, edge_clf, bo_only != 0, edges, w, h, ss_hor, ss_ver, 8);
+}
+extern "C" void oracle_ccso_add_8(uint8_t *dst, ptrdiff_t ds, const uint8_t *idx, int idx_stride,
+                                  const uint8_t *offs, const int8_t *lut, int w, int h, const uint16_t (*ll)[4]) {
+    oracle::ccso_add(dst, ds, idx, idx_stride, offs, lut, w, h, ll, 255);
+}
+

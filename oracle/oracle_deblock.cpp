@@ -500,3 +500,14 @@ template void deblock_sbrow_impl<uint16_t>(const Rav2dFrameParams *, const Geom 
 extern "C" int oracle_dq_lookup(int qidx) { return oracle::dq_lookup(qidx); }
 extern "C" uint32_t oracle_deblock_quant_thr(int hbd, int qidx) { return oracle::deblock_quant_thr(hbd, qidx); }
 extern "C" uint32_t oracle_deblock_side_thr(int hbd, int qidx) { return oracle::deblock_side_thr(hbd, qidx); }
+
+/* Known-answer hooks (deblock.rs:81-186, 1466-1472). */
+extern "C" int oracle_filter_choice(const void *buf, int hbd, ptrdiff_t s, ptrdiff_t t, ptrdiff_t stride,
+                                    int max_width_neg, int max_width_pos, uint32_t q_thr, uint32_t side_thr) {
+    if (hbd)
+        return oracle::filter_choice((const uint16_t *)buf, s, t, stride, max_width_neg, max_width_pos, q_thr,
+                                     side_thr);
+    return oracle::filter_choice((const uint8_t *)buf, s, t, stride, max_width_neg, max_width_pos, q_thr, side_thr);
+}
+extern "C" int oracle_edge_thr(int cur, int prev) { return oracle::edge_thr(cur, prev); }
+

@@ -391,3 +391,24 @@ extern "C" int oracle_apply_grain(const Rav2dFrameParams *p, const void *src_y, 
                              (const uint8_t *)src_u, (const uint8_t *)src_v, y_stride, uv_stride);
     return 0;
 }
+
+/* Known-answer hooks (filmgrain.rs:11-229). */
+extern "C" uint32_t oracle_grain_random(uint32_t bits, uint32_t *state) { return oracle::get_random_number(bits, state); }
+extern "C" int oracle_grain_round2(int x, uint32_t shift) { return oracle::round2(x, shift); }
+/* points: num (x, y) pairs; scaling: 256 entries (8-bit) or 1 << bitdepth */
+extern "C" void oracle_generate_scaling(int bitdepth, const uint8_t (*pts)[2], int num, uint8_t *scaling) {
+    if (bitdepth == 8)
+        oracle::generate_scaling_8bpc(pts, num, scaling);
+    else
+        oracle::generate_scaling_hbd(bitdepth, pts, num, scaling);
+}
+/* buf: 73 rows of 82; uv: 0 = U, 1 = V (buf_y = the luma LUT) */
+extern "C" void oracle_generate_grain_y(int16_t *buf, const Rav2dFilmGrainData *d, uint32_t seed, int bitdepth) {
+    oracle::generate_grain_y((int16_t(*)[oracle::GW])buf, *d, seed, bitdepth);
+}
+extern "C" void oracle_generate_grain_uv(int16_t *buf, const int16_t *buf_y, const Rav2dFilmGrainData *d,
+                                         uint32_t seed, int uv, int subx, int suby, int bitdepth) {
+    oracle::generate_grain_uv((int16_t(*)[oracle::GW])buf, (const int16_t(*)[oracle::GW])buf_y, *d, seed, uv, subx,
+                              suby, bitdepth);
+}
+

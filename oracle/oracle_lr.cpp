@@ -819,3 +819,29 @@ template void lr_sbrow_apply<uint8_t>(const LrCtx<uint8_t> &, Frame<uint8_t> &, 
 template void lr_sbrow_apply<uint16_t>(const LrCtx<uint16_t> &, Frame<uint16_t> &, int);
 
 }  // namespace oracle
+
+/* Known-answer hooks (looprestoration.rs:40-49, 54-109, 114-192, 273-329, 408-522). */
+extern "C" int oracle_get_qval_given_tskip(int qstep, int tskip, int i, int bdm8) {
+    return oracle::get_qval_given_tskip(qstep, tskip, i, bdm8);
+}
+/* dst: row buffer whose origin is sample 6 */
+extern "C" void oracle_backup_row_8(uint8_t *dst, const uint8_t *src, const uint8_t *left6, int w, int edge_len,
+                                    int edges) {
+    oracle::backup_row(dst, src, left6, w, edge_len, (uint8_t)edges);
+}
+extern "C" void oracle_backup_row_lpf_8(uint8_t *dst, const uint8_t *src, int w, int edge_len, int edges) {
+    oracle::backup_row_lpf(dst, src, w, edge_len, (uint8_t)edges);
+}
+extern "C" void oracle_backup_row_luma_8(uint8_t *dst, const uint8_t *src, ptrdiff_t src_stride, int w, int edges,
+                                         int ss_hor, int ss_ver, int ds) {
+    oracle::backup_row_luma(dst, src, src_stride, w, (uint8_t)edges, ss_hor, ss_ver, ds);
+}
+extern "C" void oracle_compute_gradient_row_8(uint16_t (*dst)[4], const uint8_t *const *rows, int row_off, int col,
+                                              int w, int shift) {
+    oracle::compute_gradient_row(dst, rows, row_off, col, w, shift);
+}
+extern "C" void oracle_gdf_add_8(uint8_t *p, ptrdiff_t stride, const int8_t *err, int err_stride, int w, int h,
+                                 int scale, const uint16_t (*ll)[4], int bitdepth) {
+    oracle::gdf_add(p, stride, err, err_stride, w, h, scale, ll, bitdepth);
+}
+

@@ -32,8 +32,43 @@ def load():
         lib.oracle_deblock_quant_thr.argtypes = [C.c_int, C.c_int]
         lib.oracle_deblock_side_thr.restype = C.c_uint32
         lib.oracle_deblock_side_thr.argtypes = [C.c_int, C.c_int]
+        P8 = C.POINTER(C.c_uint8)
+        sig = {
+            "oracle_filter_choice": (C.c_int, [P, C.c_int, C.c_ssize_t, C.c_ssize_t, C.c_ssize_t, C.c_int, C.c_int,
+                                               C.c_uint32, C.c_uint32]),
+            "oracle_edge_thr": (C.c_int, [C.c_int, C.c_int]),
+            "oracle_cdef_constrain": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "oracle_cdef_adjust_strength": (C.c_int, [C.c_int, C.c_uint32]),
+            "oracle_cdef_find_dir": (C.c_int, [P, C.c_int, C.c_ssize_t, C.c_int, C.POINTER(C.c_uint32)]),
+            "oracle_ccso_score": (C.c_uint32, [C.c_int, C.c_int, C.c_uint32]),
+            "oracle_ccso_prep_8": (None, [P, C.c_int, P, C.c_ssize_t, C.c_ssize_t, P, P, C.c_ssize_t, P,
+                                          C.c_ssize_t, C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
+                                          C.c_int, C.c_int, C.c_int, C.c_int]),
+            "oracle_ccso_add_8": (None, [P, C.c_ssize_t, P, C.c_int, P, P, C.c_int, C.c_int, P]),
+            "oracle_get_qval_given_tskip": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
+            "oracle_backup_row_8": (None, [P, P, P, C.c_int, C.c_int, C.c_int]),
+            "oracle_backup_row_lpf_8": (None, [P, P, C.c_int, C.c_int, C.c_int]),
+            "oracle_backup_row_luma_8": (None, [P, P, C.c_ssize_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "oracle_compute_gradient_row_8": (None, [P, P, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "oracle_gdf_add_8": (None, [P, C.c_ssize_t, P, C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_int]),
+            "oracle_grain_random": (C.c_uint32, [C.c_uint32, C.POINTER(C.c_uint32)]),
+            "oracle_grain_round2": (C.c_int, [C.c_int, C.c_uint32]),
+            "oracle_generate_scaling": (None, [C.c_int, P, C.c_int, P]),
+            "oracle_generate_grain_y": (None, [P, C.POINTER(R.FilmGrainData), C.c_uint32, C.c_int]),
+            "oracle_generate_grain_uv": (None, [P, P, C.POINTER(R.FilmGrainData), C.c_uint32, C.c_int, C.c_int,
+                                                C.c_int, C.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
         _lib = lib
     return _lib
+
+
+def ptr(a):
+    """ctypes address of a numpy array."""
+    return a.ctypes.data_as(C.c_void_p)
 
 
 def filter_frame(sf, inloop_mask=R.INLOOP_ALL, params=None):

@@ -76,3 +76,70 @@ def test_deblock_changes_blocky_frame():
     # deblocking only moves samples next to transform edges: interior of 4x4 grid untouched?
     # (edges are at multiples of 4; the filter reaches at most 8 samples across)
     assert np.abs(inp[0].astype(int) - out[0]).max() < 256
+
+
+def _fill(sf, values):
+    for pl in range(3):
+        h, w = sf.plane_shape(pl)
+        ct = C.c_uint16 if sf.bitdepth > 8 else C.c_uint8
+        arr = np.frombuffer((ct * (h * w)).from_address(sf.plane_ptr(pl)), dtype=np.uint16 if sf.bitdepth > 8 else np.uint8)
+        arr[:] = values[pl]
+
+
+def test_uniform_frame_unchanged_by_cdef_and_wiener():
+    """cdef.rs:1372-1392 (uniform block unchanged) and looprestoration.rs:
+    2778-2820, 2958-3033, 3082-3132 (identity filters): on a flat frame every
+    CDEF tap difference and every Wiener tap difference is zero, and the
+    PC-Wiener filters have unit DC gain, so CDEF and NS/PC-Wiener leave it."""
+    for bd in (8, 10):
+        sf = R.SynthFrame(192, 128, bd, 0, seed=21 + bd, tools=R.TOOL_CDEF | R.TOOL_WIENER | R.TOOL_DEBLOCK)
+        v = [300, 500, 700] if bd == 10 else [77, 120, 200]
+        _fill(sf, v)
+        out = O.filter_frame(sf)
+        for pl in range(3):
+            assert (out[pl] == v[pl]).all(), (bd, pl)
+
+
+def test_lossless_frame_unchanged_by_deblock_and_restoration():
+    """Lossless 4x4 blocks are never written by deblock (deblock.rs:2463-2490)
+    or by LR/GDF (looprestoration.rs:2823-2862, 3036-3079, 3188-3238,
+    2672-2678).  CDEF and CCSO consult the lossless masks only on
+    segmented-lossless frames (cdef.rs:594-598, 894-897), which the supported
+    parameter space (segmentation off) excludes."""
+    for bd in (8, 10):
+        sf = R.SynthFrame(256, 192, bd, 0, seed=31 + bd, tools=R.TOOL_DEBLOCK | R.TOOL_WIENER | R.TOOL_GDF,
+                          lossless_pct=100)
+        inp = sf.planes()
+        out = O.filter_frame(sf)
+        for a, b in zip(inp, out):
+            assert np.array_equal(a, b)
+
+
+def test_zero_scaling_grain_is_a_copy():
+    """filmgrain.rs:1222-1235: with a zero scaling function the noise is zero."""
+    sf = R.SynthFrame(160, 96, 10, 0, seed=41, tools=R.TOOL_GRAIN)
+    p = R.FrameParams.from_buffer_copy(sf.params)
+    g = p.grain
+    g.clip_to_restricted_range = 0
+    for pl in range(3):
+        for i in range(g.num_points[pl]):
+            g.points[pl][i][1] = 0
+    planes = sf.planes()
+    disp = O.apply_grain(sf, planes, params=p)
+    for a, b in zip(planes, disp):
+        assert np.array_equal(a, b)
+
+
+def test_grain_restricted_range_bounds():
+    """filmgrain.rs:1179-1197: restricted range clips luma to [16, 235] and
+    chroma to [16, 240] (scaled by 1 << (bitdepth - 8))."""
+    for bd in (8, 10):
+        sf = R.SynthFrame(160, 96, bd, 0, seed=43 + bd, tools=R.TOOL_GRAIN)
+        p = R.FrameParams.from_buffer_copy(sf.params)
+        p.grain.clip_to_restricted_range = 1
+        p.grain.mc_identity = 0
+        disp = O.apply_grain(sf, sf.planes(), params=p)
+        s = 1 << (bd - 8)
+        assert disp[0].min() >= 16 * s and disp[0].max() <= 235 * s
+        for pl in (1, 2):
+            assert disp[pl].min() >= 16 * s and disp[pl].max() <= 240 * s
