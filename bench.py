@@ -42,6 +42,18 @@ CONFIGS = {
 
 PASS_NAMES = ["deblock_v", "deblock_h", "cdef_ccso", "loop_restoration", "film_grain"]
 
+# HBM bytes per launch of each pass on the 4k10 workload, from two rocprofv3
+# --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0; the factors make the copy
+# probe's counters equal its known 256 MiB read / 256 MiB written); see
+# profiles/hbm_traffic_4k10_pmc.md.  Valid for the kernels profiled there.
+PMC_TRAFFIC_4K10 = {
+    "deblock_v": int(37637.2 * 2 * 1024 + 24300 * 1024),
+    "deblock_h": int(26618.5 * 2 * 1024 + 24300 * 1024),
+    "cdef_ccso": int(43555.2 * 2 * 1024 + 24300 * 1024),
+    "loop_restoration": int(38152.2 * 2 * 1024 + 24300 * 1024),
+    "film_grain": int(19947.0 * 2 * 1024 + 24300 * 1024),
+}
+
 
 def frame_samples(w, h):
     return w * h + 2 * (w // 2) * (h // 2)
@@ -306,6 +318,8 @@ def main():
         for x in sfs:
             x.close()
 
+    copy_gbs = ctx.bandwidth_probe(256 << 20, 20)
+
     cpu = None
     if not args.no_cpu:
         print("bench: CPU baseline leg", file=sys.stderr, flush=True)
@@ -332,7 +346,9 @@ def main():
             "pass_us": passes},
         "roofline": {"bound": "hbm", "kernel": PASS_NAMES[dom], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": PMC_TRAFFIC_4K10.get(PASS_NAMES[dom]) if args.config == "4k10" else None,
+                     "copy_probe_GBps": round(copy_gbs, 1),
+                     "frac_of_copy_probe": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
         "batch": batch,
         "serving": serve,

@@ -354,6 +354,12 @@ int rav2d_gpu_server_simulate(Rav2dGpuServer *s, Rav2dGpuFrame *const *frames, i
  * y << 40 | x << 20 | value).  Synchronizes the context stream. */
 int rav2d_gpu_frame_checksum(Rav2dGpuFrame *f, int which, uint64_t *out);
 
+/* Achievable HBM bandwidth: a streaming copy of `bytes` (multiple of 16,
+ * >= 4096) between two fresh buffers, iters times on the context stream;
+ * *gbps = (read + write bytes) / time.  No counterpart in the reference: a
+ * measurement utility. */
+int rav2d_gpu_bandwidth_probe(Rav2dGpuCtx *ctx, uint64_t bytes, int iters, double *gbps);
+
 /* sizeof() of the ABI structs, so bindings can check their mirrors:
  * 0 Rav2dAv2Filter, 1 Rav2dAv2Restoration, 2 Rav2dFrameParams,
  * 3 Rav2dFilmGrainData, 4 Rav2dPlanes, 5 Rav2dBatchConfig, 6 Rav2dServeConfig,

@@ -201,6 +201,7 @@ def load_gpu():
              [P, C.POINTER(P), C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, C.POINTER(ServeResult),
               C.POINTER(C.c_int32), C.POINTER(C.c_int32)])
     _declare(lib, "rav2d_gpu_frame_checksum", C.c_int, [P, C.c_int, C.POINTER(C.c_uint64)])
+    _declare(lib, "rav2d_gpu_bandwidth_probe", C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double)])
     _gpu = lib
     return lib
 
@@ -326,6 +327,12 @@ class GpuContext:
         ms = C.c_float()
         _check(self.lib.rav2d_gpu_timer_stop(self.h, C.byref(ms)), "timer_stop")
         return ms.value
+
+    def bandwidth_probe(self, nbytes=256 << 20, iters=20):
+        """GB/s of a plain streaming copy (read + write bytes)."""
+        g = C.c_double()
+        _check(self.lib.rav2d_gpu_bandwidth_probe(self.h, nbytes, iters, C.byref(g)), "bandwidth_probe")
+        return g.value
 
     def close(self):
         if self.h:
