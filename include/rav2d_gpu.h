@@ -204,9 +204,10 @@ uint64_t rav2d_gpu_ctx_bytes_in_use(const Rav2dGpuCtx *ctx);
 /* ---- device-resident frames ---------------------------------------------
  * A frame owns: the reconstructed input planes (F0), the filtered output
  * planes (in-loop output, F4), optional display planes for the grained copy,
- * intermediate planes, and device copies of the side info.  This is the
- * PicAllocator-compatible pool of crates/rav2d/src/picture.rs:38-41, 77-125:
- * w/h are padded to 128 and strides get +64 B when a multiple of 1024 B. */
+ * intermediate planes, and device copies of the side info.  It plays the part
+ * of the PicAllocator pool of crates/rav2d/src/picture.rs:38-41, 77-125, with
+ * its own layout: rows padded to 256 B (+256 B when the stride is a multiple
+ * of 1 KiB), 8 spare rows per plane. */
 int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params,
                            Rav2dGpuFrame **out);
 int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f);
