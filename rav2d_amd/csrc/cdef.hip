@@ -105,22 +105,19 @@ __device__ int find_dir(const int16_t *img, int ls, uint32_t *var, int bdm8) {
 }
 
 // One output sample of cdef_filter_block: `t` points at the sample in an LDS
-// tile with row stride `ls` whose outside-frame entries hold INT16_MIN.
-__device__ __forceinline__ int cdef_px(const int16_t *t, int ls, int pri, int sec, int dir, int damping, int bdm8) {
-    int px = t[0];
-    // direction taps as (dy, dx): the reference's offsets into a 12-wide buffer
-    auto off = [&](int d, int k) { return kCdefDyDx[d][k][0] * ls + kCdefDyDx[d][k][1]; };
+// tile whose outside-frame entries hold INT16_MIN; `op`, `os0`, `os1` are the
+// tap offsets (in that tile's stride) of the primary direction and of the two
+// secondary directions, for taps k = 0, 1.
+__device__ __forceinline__ int cdef_px(const int16_t *t, const int op[2], const int os0[2], const int os1[2], int pri,
+                                       int pri_tap, int pri_shift, int sec, int sec_shift) {
+    const int px = t[0];
     int sum = 0;
     if (pri) {
-        int pri_tap = 4 - ((pri >> bdm8) & 1);
-        int pri_shift = dmax(0, damping - ulog2((uint32_t)pri));
         if (sec) {
-            int sec_shift = damping - ulog2((uint32_t)sec);
             int mx = px, mn = px, tk = pri_tap;
 #pragma unroll
             for (int k = 0; k < 2; k++) {
-                int o1 = off(dir + 2, k);
-                int p0 = t[o1], p1 = t[-o1];
+                const int p0 = t[op[k]], p1 = t[-op[k]];
                 sum += tk * constrain(p0 - px, pri, pri_shift);
                 sum += tk * constrain(p1 - px, pri, pri_shift);
                 tk = (tk & 3) | 2;
@@ -128,9 +125,8 @@ __device__ __forceinline__ int cdef_px(const int16_t *t, int ls, int pri, int se
                 mx = dmax(p0, mx);
                 mn = dmin(p1, mn);
                 mx = dmax(p1, mx);
-                int o2 = off(dir + 4, k), o3 = off(dir, k);
-                int s0 = t[o2], s1 = t[-o2], s2 = t[o3], s3 = t[-o3];
-                int st = 2 - k;
+                const int s0 = t[os0[k]], s1 = t[-os0[k]], s2 = t[os1[k]], s3 = t[-os1[k]];
+                const int st = 2 - k;
                 sum += st * constrain(s0 - px, sec, sec_shift);
                 sum += st * constrain(s1 - px, sec, sec_shift);
                 sum += st * constrain(s2 - px, sec, sec_shift);
@@ -149,22 +145,19 @@ __device__ __forceinline__ int cdef_px(const int16_t *t, int ls, int pri, int se
         int tk = pri_tap;
 #pragma unroll
         for (int k = 0; k < 2; k++) {
-            int o1 = off(dir + 2, k);
-            sum += tk * constrain(t[o1] - px, pri, pri_shift);
-            sum += tk * constrain(t[-o1] - px, pri, pri_shift);
+            sum += tk * constrain(t[op[k]] - px, pri, pri_shift);
+            sum += tk * constrain(t[-op[k]] - px, pri, pri_shift);
             tk = (tk & 3) | 2;
         }
         return px + ((sum - (sum < 0) + 8) >> 4);
     }
-    int sec_shift = damping - ulog2((uint32_t)sec);
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-        int o1 = off(dir + 4, k), o2 = off(dir, k);
-        int st = 2 - k;
-        sum += st * constrain(t[o1] - px, sec, sec_shift);
-        sum += st * constrain(t[-o1] - px, sec, sec_shift);
-        sum += st * constrain(t[o2] - px, sec, sec_shift);
-        sum += st * constrain(t[-o2] - px, sec, sec_shift);
+        const int st = 2 - k;
+        sum += st * constrain(t[os0[k]] - px, sec, sec_shift);
+        sum += st * constrain(t[-os0[k]] - px, sec, sec_shift);
+        sum += st * constrain(t[os1[k]] - px, sec, sec_shift);
+        sum += st * constrain(t[-os1[k]] - px, sec, sec_shift);
     }
     return px + ((sum - (sum < 0) + 8) >> 4);
 }
@@ -178,7 +171,26 @@ struct CdefSmem {
     uint8_t b_state[64];   // bit0 luma filtered, bit1 chroma filtered
     uint8_t b_dir[64];
     int16_t b_ypri[64];    // adjusted luma primary strength
+    int16_t off[2][12][2]; // CDEF_DIRECTIONS as offsets in the luma / chroma tile
 };
+
+// CCSO offset of one sample: `lt` = staged luma at the collocated position
+// (lx, lyy in frame coordinates), neighbours clamped to the frame.
+__device__ __forceinline__ int ccso_offset(const CcsoDev &c, const int16_t *ly, int lx, int lyy, int x0, int y0, int fw,
+                                           int fh) {
+    const int cval = ly[(lyy - y0 + 2) * LY + lx - x0 + 2];
+    int idx = (int)((uint32_t)cval >> c.shift);
+    if (!c.bo_only) {
+        int nx0 = iclip(lx + c.dx, 0, fw - 1) - x0 + 2, ny0 = iclip(lyy + c.dy, 0, fh - 1) - y0 + 2;
+        int nx1 = iclip(lx - c.dx, 0, fw - 1) - x0 + 2, ny1 = iclip(lyy - c.dy, 0, fh - 1) - y0 + 2;
+        uint32_t c0 = ccso_score(ly[ny0 * LY + nx0] - cval, c.quant_step, c.edge_clf);
+        uint32_t c1 = ccso_score(ly[ny1 * LY + nx1] - cval, c.quant_step, c.edge_clf);
+        idx |= (int)((c0 << 5) | (c1 << 3));
+        idx &= 0xff;
+    }
+    const int packed = (idx >> 1) < 64 ? c.filter_off[idx >> 1] : 0;
+    return kCcsoOffset[c.scale_idx][7 & (packed >> (4 * (idx & 1)))];
+}
 
 template <typename P>
 __device__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
@@ -186,25 +198,46 @@ __device__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
     int16_t(*lc)[LC * LC] = sm.lc;
     uint8_t *b_state = sm.b_state, *b_dir = sm.b_dir;
     int16_t *b_ypri = sm.b_ypri;
+    const int t = threadIdx.x;
     const int x0 = sbx * 64, y0 = sby * 64;
     const int fw = a.g.bw * 4, fh = a.g.bh * 4;
     const int cw = fw >> 1, ch = fh >> 1;
     const int bdm8 = a.g.bitdepth - 8;
-    for (int i = threadIdx.x; i < LY * LY; i += kThreads) {
-        int r = i / LY, c = i - r * LY;
-        int gx = x0 - 2 + c, gy = y0 - 2 + r;
-        int16_t v = INT16_MIN;
-        if (gx >= 0 && gx < fw && gy >= 0 && gy < fh) v = (int16_t)row_ptr<const P>(a.src[0], gy)[gx];
-        ly[i] = v;
-    }
-    for (int i = threadIdx.x; i < 2 * LC * LC; i += kThreads) {
-        int pl = i >= LC * LC;
-        int j = i - pl * LC * LC;
-        int r = j / LC, c = j - r * LC;
-        int gx = (x0 >> 1) - 2 + c, gy = (y0 >> 1) - 2 + r;
-        int16_t v = INT16_MIN;
-        if (gx >= 0 && gx < cw && gy >= 0 && gy < ch) v = (int16_t)row_ptr<const P>(a.src[1 + pl], gy)[gx];
-        lc[pl][j] = v;
+    // ---- staging of F2 with INT16_MIN outside the frame; every load of a lane
+    // is issued before its first LDS store
+    {
+        constexpr int PY = (LY * LY + kThreads - 1) / kThreads, PC = (2 * LC * LC + kThreads - 1) / kThreads;
+        int16_t vy[PY], vc[PC];
+#pragma unroll
+        for (int j = 0; j < PY; j++) {
+            const int i = t + j * kThreads;
+            const int r = i / LY, c = i - r * LY;
+            const int gx = x0 - 2 + c, gy = y0 - 2 + r;
+            // unconditional load from a clamped (always valid) address, then select
+            const int16_t v = (int16_t)row_ptr<const P>(a.src[0], iclip(gy, 0, fh - 1))[iclip(gx, 0, fw - 1)];
+            vy[j] = (i < LY * LY && gx >= 0 && gx < fw && gy >= 0 && gy < fh) ? v : (int16_t)INT16_MIN;
+        }
+#pragma unroll
+        for (int j = 0; j < PC; j++) {
+            const int i = t + j * kThreads;
+            const int pl = i >= LC * LC;
+            const int k = i - pl * LC * LC;
+            const int r = k / LC, c = k - r * LC;
+            const int gx = (x0 >> 1) - 2 + c, gy = (y0 >> 1) - 2 + r;
+            const int16_t v =
+                (int16_t)row_ptr<const P>(a.src[1 + (pl & 1)], iclip(gy, 0, ch - 1))[iclip(gx, 0, cw - 1)];
+            vc[j] = (i < 2 * LC * LC && gx >= 0 && gx < cw && gy >= 0 && gy < ch) ? v : (int16_t)INT16_MIN;
+        }
+#pragma unroll
+        for (int j = 0; j < PY; j++)
+            if (t + j * kThreads < LY * LY) ly[t + j * kThreads] = vy[j];
+#pragma unroll
+        for (int j = 0; j < PC; j++)
+            if (t + j * kThreads < 2 * LC * LC) (&lc[0][0])[t + j * kThreads] = vc[j];
+        if (t < 48) {
+            const int tab = t / 24, d = (t >> 1) % 12, k = t & 1;
+            sm.off[tab][d][k] = (int16_t)(kCdefDyDx[d][k][0] * (tab ? LC : LY) + kCdefDyDx[d][k][1]);
+        }
     }
     const Rav2dAv2Filter &m = a.mask[(sby >> 2) * a.g.sb256w + (sbx >> 2)];
     const int sb64_idx = ((sby & 3) << 2) + (sbx & 3);
@@ -223,8 +256,8 @@ __device__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
     uv_sec <<= bdm8;
     const int damping = a.damping + bdm8;
     __syncthreads();
-    if (threadIdx.x < 64) {
-        int bxi = threadIdx.x & 7, byi = threadIdx.x >> 3;
+    if (t < 64) {
+        int bxi = t & 7, byi = t >> 3;
         int bx = (x0 >> 2) + bxi * 2, by = (y0 >> 2) + byi * 2;  // 4px units
         uint8_t st = 0, dir = 0;
         int16_t ypri = 0;
@@ -244,85 +277,93 @@ __device__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
                 dir = (uint8_t)d;
             }
         }
-        b_state[threadIdx.x] = st;
-        b_dir[threadIdx.x] = dir;
-        b_ypri[threadIdx.x] = ypri;
+        b_state[t] = st;
+        b_dir[t] = dir;
+        b_ypri[t] = ypri;
     }
     __syncthreads();
-    // luma
-    const bool cc0 = a.ccso_on && m.ccso[0];
-    for (int i = threadIdx.x; i < 64 * 64; i += kThreads) {
-        int yy = i >> 6, xx = i & 63;
-        int gx = x0 + xx, gy = y0 + yy;
-        if (gx >= fw || gy >= fh) continue;
-        int bi = (yy >> 3) * 8 + (xx >> 3);
-        const int16_t *t = &ly[(yy + 2) * LY + xx + 2];
-        int v = t[0];
-        if (b_state[bi] & 1) {
-            int pri = y_pri ? b_ypri[bi] : 0;
-            int dir = y_pri ? b_dir[bi] : 0;
-            // the reference stores the result through a truncating cast to the
-            // sample type (Pixel::from_i32), before CCSO reads it back
-            v = (int)(P)cdef_px(t, LY, pri, y_sec, dir, damping, bdm8);
+    // ---- luma: lane t filters rows 2(t&3), 2(t&3)+1 of 8x8 block t>>2
+    {
+        const int bi = t >> 2, r0 = (t & 3) * 2;
+        const int bx = (bi & 7) * 8, by = (bi >> 3) * 8;
+        const bool on = b_state[bi] & 1;
+        const int pri = y_pri ? b_ypri[bi] : 0;
+        const int dir = y_pri ? b_dir[bi] : 0;
+        const int pri_tap = 4 - ((pri >> bdm8) & 1);
+        const int pri_shift = pri ? dmax(0, damping - ulog2((uint32_t)pri)) : 0;
+        const int sec_shift = y_sec ? damping - ulog2((uint32_t)y_sec) : 0;
+        int op[2], os0[2], os1[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            op[k] = sm.off[0][dir + 2][k];
+            os0[k] = sm.off[0][dir + 4][k];
+            os1[k] = sm.off[0][dir][k];
         }
-        if (cc0) {
-            const CcsoDev &c = a.ccso[0];
-            int cval = t[0];
-            int idx = (int)((uint32_t)cval >> c.shift);
-            if (!c.bo_only) {
-                int nx0 = iclip(gx + c.dx, 0, fw - 1) - x0 + 2, ny0 = iclip(gy + c.dy, 0, fh - 1) - y0 + 2;
-                int nx1 = iclip(gx - c.dx, 0, fw - 1) - x0 + 2, ny1 = iclip(gy - c.dy, 0, fh - 1) - y0 + 2;
-                uint32_t c0 = ccso_score(ly[ny0 * LY + nx0] - cval, c.quant_step, c.edge_clf);
-                uint32_t c1 = ccso_score(ly[ny1 * LY + nx1] - cval, c.quant_step, c.edge_clf);
-                idx |= (int)((c0 << 5) | (c1 << 3));
-                idx &= 0xff;
+        const bool cc0 = a.ccso_on && m.ccso[0];
+#pragma unroll 1
+        for (int yy = r0; yy < r0 + 2; yy++) {
+            const int gy = y0 + by + yy;
+            if (gy >= fh) continue;
+            P *drow = row_ptr<P>(a.dst[0], gy);
+#pragma unroll 2
+            for (int xx = 0; xx < 8; xx++) {
+                const int gx = x0 + bx + xx;
+                if (gx >= fw) continue;
+                const int16_t *tp = &ly[(by + yy + 2) * LY + bx + xx + 2];
+                int v = tp[0];
+                // the reference stores the result through a truncating cast to the
+                // sample type (Pixel::from_i32), before CCSO reads it back
+                if (on) v = (int)(P)cdef_px(tp, op, os0, os1, pri, pri_tap, pri_shift, y_sec, sec_shift);
+                if (cc0) v = iclip(v + ccso_offset(a.ccso[0], ly, gx, gy, x0, y0, fw, fh), 0, a.g.bdmax);
+                drow[gx] = (P)v;
             }
-            int packed = (idx >> 1) < 64 ? c.filter_off[idx >> 1] : 0;
-            v = iclip(v + kCcsoOffset[c.scale_idx][7 & (packed >> (4 * (idx & 1)))], 0, a.g.bdmax);
         }
-        row_ptr<P>(a.dst[0], gy)[gx] = (P)v;
     }
-    // chroma
-    for (int i = threadIdx.x; i < 2 * 32 * 32; i += kThreads) {
-        int pl = i >> 10;
-        int j = i & 1023;
-        int yy = j >> 5, xx = j & 31;
-        int gx = (x0 >> 1) + xx, gy = (y0 >> 1) + yy;
-        if (gx >= cw || gy >= ch) continue;
-        int bi = (yy >> 2) * 8 + (xx >> 2);
-        const int16_t *t = &lc[pl][(yy + 2) * LC + xx + 2];
-        int v = t[0];
-        if (b_state[bi] & 2) {
-            int uvdir = uv_pri ? b_dir[bi] : 0;
-            v = (int)(P)cdef_px(t, LC, uv_pri, uv_sec, uvdir, damping - 1, bdm8);
+    // ---- chroma: lane t filters rows 2(t&1), 2(t&1)+1 of 4x4 block (t>>1)&63 of plane t>>7
+    {
+        const int pl = t >> 7, bi = (t >> 1) & 63, r0 = (t & 1) * 2;
+        const int bx = (bi & 7) * 4, by = (bi >> 3) * 4;
+        const bool on = b_state[bi] & 2;
+        const int dir = uv_pri ? b_dir[bi] : 0;
+        const int pri_tap = 4 - ((uv_pri >> bdm8) & 1);
+        const int pri_shift = uv_pri ? dmax(0, damping - 1 - ulog2((uint32_t)uv_pri)) : 0;
+        const int sec_shift = uv_sec ? damping - 1 - ulog2((uint32_t)uv_sec) : 0;
+        int op[2], os0[2], os1[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            op[k] = sm.off[1][dir + 2][k];
+            os0[k] = sm.off[1][dir + 4][k];
+            os1[k] = sm.off[1][dir][k];
         }
-        if (a.ccso_on && m.ccso[1 + pl]) {
-            const CcsoDev &c = a.ccso[1 + pl];
-            int lx = gx << 1, lyy = gy << 1;
-            int cval = ly[(lyy - y0 + 2) * LY + lx - x0 + 2];
-            int idx = (int)((uint32_t)cval >> c.shift);
-            if (!c.bo_only) {
-                int nx0 = iclip(lx + c.dx, 0, fw - 1) - x0 + 2, ny0 = iclip(lyy + c.dy, 0, fh - 1) - y0 + 2;
-                int nx1 = iclip(lx - c.dx, 0, fw - 1) - x0 + 2, ny1 = iclip(lyy - c.dy, 0, fh - 1) - y0 + 2;
-                uint32_t c0 = ccso_score(ly[ny0 * LY + nx0] - cval, c.quant_step, c.edge_clf);
-                uint32_t c1 = ccso_score(ly[ny1 * LY + nx1] - cval, c.quant_step, c.edge_clf);
-                idx |= (int)((c0 << 5) | (c1 << 3));
-                idx &= 0xff;
+        const bool cc = a.ccso_on && m.ccso[1 + pl];
+#pragma unroll 1
+        for (int yy = r0; yy < r0 + 2; yy++) {
+            const int gy = (y0 >> 1) + by + yy;
+            if (gy >= ch) continue;
+            P *drow = row_ptr<P>(a.dst[1 + pl], gy);
+#pragma unroll 2
+            for (int xx = 0; xx < 4; xx++) {
+                const int gx = (x0 >> 1) + bx + xx;
+                if (gx >= cw) continue;
+                const int16_t *tp = &lc[pl][(by + yy + 2) * LC + bx + xx + 2];
+                int v = tp[0];
+                if (on) v = (int)(P)cdef_px(tp, op, os0, os1, uv_pri, pri_tap, pri_shift, uv_sec, sec_shift);
+                if (cc) v = iclip(v + ccso_offset(a.ccso[1 + pl], ly, gx << 1, gy << 1, x0, y0, fw, fh), 0, a.g.bdmax);
+                drow[gx] = (P)v;
             }
-            int packed = (idx >> 1) < 64 ? c.filter_off[idx >> 1] : 0;
-            v = iclip(v + kCcsoOffset[c.scale_idx][7 & (packed >> (4 * (idx & 1)))], 0, a.g.bdmax);
         }
-        row_ptr<P>(a.dst[1 + pl], gy)[gx] = (P)v;
     }
 }
 
-template <typename P> __global__ __launch_bounds__(kThreads) void cdef_kernel(CdefArgs a) {
+template <typename P>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void cdef_kernel(CdefArgs a) {
     __shared__ CdefSmem sm;
     cdef_body<P>(a, blockIdx.x, blockIdx.y, sm);
 }
 
 // Batched launch: one item per 64x64 superblock of every frame.
-__global__ __launch_bounds__(kThreads) void cdef_batch(const CdefArgs *args, BatchSched s) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void cdef_batch(const CdefArgs *args,
+                                                                                        BatchSched s) {
     __shared__ CdefSmem sm;
     for_each_item(s, [&](int f, int t) {
         const CdefArgs &a = args[f];

@@ -180,6 +180,10 @@ __device__ EdgeDesc edge_desc(const DeblockArgs &a, int pl, bool vert, int x4, i
 
 constexpr int kTW = 64, kTH = 32, kHalo = 16, kThreads = 256;
 
+// Both passes: each lane issues its tile loads, then looks up its edge's
+// descriptor (masks and thresholds in global memory) while those loads are in
+// flight, then stores the tile to LDS.  Every (edge line, 4-line segment) of a
+// tile is one lane (152 for V, 176 for H).
 template <typename P>
 __device__ void deblock_v_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW + 2 * kHalo]) {
     int pl, tx, ty;
@@ -188,29 +192,34 @@ __device__ void deblock_v_body(const DeblockArgs &a, int tile_id, uint16_t (*til
     const DevPlane &dst = a.dst[pl];
     const int x0 = tx * kTW, y0 = ty * kTH;
     const int W = src.w, H = src.h;
+    const int t = threadIdx.x;
     constexpr int LW = kTW + 2 * kHalo;
-    for (int i = threadIdx.x; i < kTH * LW; i += kThreads) {
-        int r = i / LW, c = i - r * LW;
-        int gx = x0 - kHalo + c, gy = y0 + r;
-        uint16_t v = 0;
-        if (gy < H && gx >= 0 && gx < W) v = row_ptr<const P>(src, gy)[gx];
-        tile[r][c] = v;
+    constexpr int PV = kTH * LW / kThreads;
+    uint16_t v[PV];
+#pragma unroll
+    for (int j = 0; j < PV; j++) {
+        const int i = t + j * kThreads;
+        const int r = i / LW, c = i - r * LW;
+        const int gx = x0 - kHalo + c, gy = y0 + r;
+        // unconditional load from a clamped (always valid) address, then select
+        const uint16_t l = row_ptr<const P>(src, dmin(gy, H - 1))[iclip(gx, 0, W - 1)];
+        v[j] = (gy < H && gx >= 0 && gx < W) ? l : (uint16_t)0;
     }
+    constexpr int NE = kTW / 4 + 3;  // edge columns x0-4 .. x0+TW+4
+    static_assert(NE * (kTH / 4) <= kThreads, "one lane per edge segment");
+    EdgeDesc e;
+    e.on = false;
+    const int rs = t / NE, ec = t - rs * NE;
+    const int ex = x0 - 4 + ec * 4, ey = y0 + rs * 4;
+    // no filtering at the frame's left edge
+    if (a.plane_on[pl] && t < NE * (kTH / 4) && ex > 0 && ex < W && ey < H) e = edge_desc(a, pl, true, ex >> 2, ey >> 2);
+#pragma unroll
+    for (int j = 0; j < PV; j++) (&tile[0][0])[t + j * kThreads] = v[j];
     __syncthreads();
-    if (a.plane_on[pl]) {
-        constexpr int NE = kTW / 4 + 3;  // edge columns x0-4 .. x0+TW+4
-        for (int i = threadIdx.x; i < NE * (kTH / 4); i += kThreads) {
-            int rs = i / NE, ec = i - rs * NE;
-            int ex = x0 - 4 + ec * 4, ey = y0 + rs * 4;
-            if (ex <= 0 || ex >= W || ey >= H) continue;  // no filtering at the frame's left edge
-            EdgeDesc e = edge_desc(a, pl, true, ex >> 2, ey >> 2);
-            if (!e.on) continue;
-            filter_edge(&tile[rs * 4][ex - x0 + kHalo], LW, 1, a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll,
-                        e.neg_ll);
-        }
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < kTH * kTW; i += kThreads) {
+    if (e.on)
+        filter_edge(&tile[rs * 4][ex - x0 + kHalo], LW, 1, a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
+    __syncthreads();
+    for (int i = t; i < kTH * kTW; i += kThreads) {
         int r = i / kTW, c = i - r * kTW;
         int gx = x0 + c, gy = y0 + r;
         if (gy < H && gx < W) row_ptr<P>(dst, gy)[gx] = (P)tile[r][c + kHalo];
@@ -225,29 +234,33 @@ __device__ void deblock_h_body(const DeblockArgs &a, int tile_id, uint16_t (*til
     const DevPlane &dst = a.dst[pl];
     const int x0 = tx * kTW, y0 = ty * kTH;
     const int W = src.w, H = src.h;
+    const int t = threadIdx.x;
     constexpr int LH = kTH + 2 * kHalo;
-    for (int i = threadIdx.x; i < LH * kTW; i += kThreads) {
-        int r = i / kTW, c = i - r * kTW;
-        int gx = x0 + c, gy = y0 - kHalo + r;
-        uint16_t v = 0;
-        if (gy >= 0 && gy < H && gx < W) v = row_ptr<const P>(src, gy)[gx];
-        tile[r][c] = v;
+    constexpr int PV = LH * kTW / kThreads;
+    uint16_t v[PV];
+#pragma unroll
+    for (int j = 0; j < PV; j++) {
+        const int i = t + j * kThreads;
+        const int r = i / kTW, c = i - r * kTW;
+        const int gx = x0 + c, gy = y0 - kHalo + r;
+        const uint16_t l = row_ptr<const P>(src, iclip(gy, 0, H - 1))[dmin(gx, W - 1)];
+        v[j] = (gy >= 0 && gy < H && gx < W) ? l : (uint16_t)0;
     }
+    constexpr int NE = kTH / 4 + 3;  // edge rows y0-4 .. y0+TH+4
+    static_assert(NE * (kTW / 4) <= kThreads, "one lane per edge segment");
+    EdgeDesc e;
+    e.on = false;
+    const int er = t / (kTW / 4), cs = t - er * (kTW / 4);
+    const int ey = y0 - 4 + er * 4, ex = x0 + cs * 4;
+    // no filtering at the frame's top edge
+    if (a.plane_on[pl] && t < NE * (kTW / 4) && ey > 0 && ey < H && ex < W) e = edge_desc(a, pl, false, ex >> 2, ey >> 2);
+#pragma unroll
+    for (int j = 0; j < PV; j++) (&tile[0][0])[t + j * kThreads] = v[j];
     __syncthreads();
-    if (a.plane_on[pl]) {
-        constexpr int NE = kTH / 4 + 3;  // edge rows y0-4 .. y0+TH+4
-        for (int i = threadIdx.x; i < NE * (kTW / 4); i += kThreads) {
-            int er = i / (kTW / 4), cs = i - er * (kTW / 4);
-            int ey = y0 - 4 + er * 4, ex = x0 + cs * 4;
-            if (ey <= 0 || ey >= H || ex >= W) continue;  // no filtering at the frame's top edge
-            EdgeDesc e = edge_desc(a, pl, false, ex >> 2, ey >> 2);
-            if (!e.on) continue;
-            filter_edge(&tile[ey - y0 + kHalo][cs * 4], 1, kTW, a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll,
-                        e.neg_ll);
-        }
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < kTH * kTW; i += kThreads) {
+    if (e.on)
+        filter_edge(&tile[ey - y0 + kHalo][cs * 4], 1, kTW, a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
+    __syncthreads();
+    for (int i = t; i < kTH * kTW; i += kThreads) {
         int r = i / kTW, c = i - r * kTW;
         int gx = x0 + c, gy = y0 + r;
         if (gy < H && gx < W) row_ptr<P>(dst, gy)[gx] = (P)tile[r + kHalo][c];

@@ -15,6 +15,8 @@
 // repeat the edge column at the frame's left/right border.  Here that virtual
 // image is staged once per work item in LDS and every output sample reads it
 // directly.
+#include <type_traits>
+
 #include "internal.h"
 #include "tables.h"
 
@@ -46,23 +48,46 @@ __device__ __forceinline__ int qval_tskip(int qstep, int tskip, int i, int bdm8)
     return 255 * (kModeOff[i] + q);
 }
 
+// Compile-time tap tables (offsets fold into LDS immediate offsets).
+struct Tap {
+    int8_t dy, dx;
+};
+constexpr Tap kNsYT[16] = {{1, 0}, {0, 1}, {2, 0}, {0, 2}, {1, 1}, {-1, 1}, {2, 1}, {2, -1},
+                           {1, 2}, {1, -2}, {3, 0}, {0, 3}, {4, 0}, {0, 4}, {3, 3}, {3, -3}};
+constexpr Tap kPcT[12] = {{1, 0}, {0, 1}, {2, 0}, {0, 2}, {1, 1}, {-1, 1}, {2, 1}, {2, -1}, {1, 2}, {1, -2}, {3, 0}, {0, 3}};
+constexpr Tap kNsUvT[6] = {{1, 0}, {0, 1}, {1, 1}, {-1, 1}, {2, 0}, {0, 2}};
+constexpr Tap kNsUvYT[12] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, -1}, {-1, 1}, {1, -1}, {2, 0}, {-2, 0}, {0, 2}, {0, -2}};
+constexpr Tap kGdfT[18] = {{6, 0}, {5, 0}, {4, 0}, {3, 0}, {2, 1}, {2, 0}, {2, -1}, {1, 2}, {1, 1},
+                           {1, 0}, {1, -1}, {1, -2}, {0, 6}, {0, 5}, {0, 4}, {0, 3}, {0, 2}, {0, 1}};
+
+constexpr int LS = 64 + 4;  // stride of the downsampled luma rows (2-sample halo)
+
+// Narrows a value known to fit 16 bits (sums and differences of at most four
+// 8/10-bit samples, clamped GDF features) so that products with filter taps
+// compile to 24-bit multiplies.
+__device__ __forceinline__ int s16(int v) { return (int)(int16_t)v; }
+constexpr int CS = 33;      // stride of the GDF gradient cells
+
 struct LrSmem {
-    uint16_t V[VH * VW];
-    uint16_t Ld[(32 + 4) * (64 + 4)];
-    uint16_t cells[33 * 33 * 4];
-    uint8_t cls[16 * 16];
-    uint16_t s_alpha[88];
-    int16_t s_weight[264];
+    alignas(16) uint16_t V[VH * VW];  // virtual image, rows/cols [-E, 64 + E)
+    uint16_t Ld[(32 + 4) * LS];      // downsampled luma of a chroma stripe
+    uint16_t cells[33 * CS * 4];     // GDF gradient cells
+    uint8_t cls[16 * 16];            // class of each 4x4 block
+    alignas(16) uint16_t s_alpha[88];
+    alignas(16) int16_t s_weight[264];
+    alignas(16) int16_t s_pc[64 * 13];  // PC-Wiener filters of the frame's wiener_idx
+    int8_t s_mf[16 * 16];               // multi-class frame filters of the plane
+    alignas(16) int8_t s_err[4096];     // GDF error table slice of (ref, qp)
     LrItem it;
 };
 
 template <typename P>
 __device__ void lr_body(const LrArgs &a, int item, LrSmem &sm) {
-    uint16_t *V = sm.V, *Ld = sm.Ld, *cells = sm.cells, *s_alpha = sm.s_alpha;
+    uint16_t *V = sm.V, *Ld = sm.Ld, *cells = sm.cells;
     uint8_t *cls = sm.cls;
-    int16_t *s_weight = sm.s_weight;
     LrItem &it = sm.it;
-    if (threadIdx.x < sizeof(LrItem) / 4) ((uint32_t *)&it)[threadIdx.x] = ((const uint32_t *)&a.items[item])[threadIdx.x];
+    const int t = threadIdx.x;
+    if (t < sizeof(LrItem) / 4) ((uint32_t *)&it)[t] = ((const uint32_t *)&a.items[item])[t];
     __syncthreads();
     const int pl = it.plane, x0 = it.x0, w = it.w, y0 = it.y0, h = it.h;
     const uint8_t edges = it.edges;
@@ -70,110 +95,179 @@ __device__ void lr_body(const LrArgs &a, int item, LrSmem &sm) {
     const int bd = a.g.bitdepth, bdm8 = bd - 8, bdmax = a.g.bdmax;
     const int kind = it.kind;
     const bool gdf = it.gdf_on;
-    const int VS = w + 2 * E;  // row stride of V
     if (kind == LR_KIND_NONE && !gdf) {
-        for (int i = threadIdx.x; i < w * h; i += kThreads) {
-            int y = i / w, x = i - y * w;
-            row_ptr<P>(dst, y0 + y)[x0 + x] = row_ptr<const P>(f3, y0 + y)[x0 + x];
+        // plane copy, 4 samples per access (x0 and w are multiples of 4)
+        using Q = typename std::conditional<sizeof(P) == 1, uint32_t, uint2>::type;
+        constexpr int CP = 16;
+        const int n4 = w >> 2, n = h * n4;
+        for (int b = 0; b < n; b += CP * kThreads) {
+            Q v[CP];
+#pragma unroll
+            for (int j = 0; j < CP; j++) {
+                const int i = b + t + j * kThreads;
+                if (i < n) {
+                    const int r = i / n4, c = i - r * n4;
+                    v[j] = ((const Q *)(row_ptr<const P>(f3, y0 + r) + x0))[c];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < CP; j++) {
+                const int i = b + t + j * kThreads;
+                if (i < n) {
+                    const int r = i / n4, c = i - r * n4;
+                    ((Q *)(row_ptr<P>(dst, y0 + r) + x0))[c] = v[j];
+                }
+            }
         }
         return;
     }
-    // ---- stage the virtual image V: rows [-E, h+E), columns [-E, w+E)
-    for (int i = threadIdx.x; i < (h + 2 * E) * VS; i += kThreads) {
-        int r = i / VS - E, c = i % VS - E;
+    // ---- staging.  Each lane issues all of its global loads before its first
+    // LDS store, so the load latencies overlap instead of adding up.
+    // Virtual image V: rows [-E, h+E), columns [-E, w+E).
+    constexpr int PER = (VH * VW + kThreads - 1) / kThreads;
+    const int nv = (h + 2 * E) * VW;
+    uint16_t vals[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = t + j * kThreads;
+        // rows and columns beyond the item clamp to valid ones, so the load is
+        // unconditional (lanes past the image load a harmless in-frame sample)
+        const int r = dmin(i / VW, h + 2 * E - 1) - E, c = dmin(i % VW - E, w + E - 1);
         int xs = (c < 0 && !(edges & LRE_LEFT)) ? 0 : ((c >= w && !(edges & LRE_RIGHT)) ? w - 1 : c);
-        int gx = x0 + xs;
-        P v;
+        const P *row;
         if (r >= 0 && r < h)
-            v = row_ptr<const P>(f3, y0 + r)[gx];
+            row = row_ptr<const P>(f3, y0 + r);
         else if (r < 0)
-            v = (edges & LRE_TOP) ? row_ptr<const P>(f2, y0 + (r == -1 ? -1 : -2))[gx] : row_ptr<const P>(f3, y0)[gx];
+            row = (edges & LRE_TOP) ? row_ptr<const P>(f2, y0 + (r == -1 ? -1 : -2)) : row_ptr<const P>(f3, y0);
         else
-            v = (edges & LRE_BOTTOM) ? row_ptr<const P>(f2, y0 + h + (r == h ? 0 : 1))[gx]
-                                     : row_ptr<const P>(f3, y0 + h - 1)[gx];
-        V[i] = (uint16_t)v;
+            row = (edges & LRE_BOTTOM) ? row_ptr<const P>(f2, y0 + h + (r == h ? 0 : 1))
+                                       : row_ptr<const P>(f3, y0 + h - 1);
+        vals[j] = (uint16_t)row[x0 + xs];
+    }
+    // tables of this item's filter kinds, 16 (or 8) bytes per lane
+    uint4 pcv = {0, 0, 0, 0}, alv = {0, 0, 0, 0}, wv = {0, 0, 0, 0}, erv = {0, 0, 0, 0};
+    uint2 erv2 = {0, 0};
+    if (kind == LR_KIND_PC && t < 64 * 13 * 2 / 16) pcv = ((const uint4 *)(a.t.pc_filters + a.widx * 64 * 13))[t];
+    if (gdf) {
+        if (t < 88 * 2 / 16) alv = ((const uint4 *)(a.t.gdf_alpha + a.gdf_ref * 528 + a.gdf_qp * 88))[t];
+        if (t < 264 * 2 / 16) wv = ((const uint4 *)(a.t.gdf_weight + a.gdf_ref * 1584 + a.gdf_qp * 264))[t];
+        if (a.gdf_ref == 0)
+            erv = ((const uint4 *)(a.t.gdf_intra + a.gdf_qp * 4096))[t];  // 4096 B: one uint4 per lane
+        else if (t < 1000 / 8)
+            erv2 = ((const uint2 *)(a.t.gdf_inter + (a.gdf_ref - 1) * 6000 + a.gdf_qp * 1000))[t];
     }
     const bool uv_single = kind == LR_KIND_SINGLE && pl != 0;
-    const int LS = w + 4;
+    constexpr int PERL = ((32 + 4) * LS + kThreads - 1) / kThreads;
+    uint16_t lvals[PERL];
     if (uv_single) {
         // downsampled luma rows of the chroma stripe (cfl_ds_filter_index)
         const DevPlane &l3 = a.f3[0], &l2 = a.f2[0];
-        for (int i = threadIdx.x; i < (h + 4) * LS; i += kThreads) {
-            int r = i / LS - 2, c = i % LS - 2;
-            int xs = (c < 0 && !(edges & LRE_LEFT)) ? 0 : ((c >= w && !(edges & LRE_RIGHT)) ? w - 1 : c);
-            int lx = 2 * (x0 + xs);
-            const P *ra, *rb;
-            int rr = r;
-            if (rr >= h && !(edges & LRE_BOTTOM)) rr = h - 1;
-            if (rr < 0 && !(edges & LRE_TOP)) rr = 0;
-            if (rr >= 0 && rr < h) {
-                ra = row_ptr<const P>(l3, 2 * (y0 + rr));
-                rb = row_ptr<const P>(l3, 2 * (y0 + rr) + 1);
-            } else if (rr == -2) {
-                ra = rb = row_ptr<const P>(l2, 2 * y0 - 2);
-            } else if (rr == -1) {
-                ra = row_ptr<const P>(l2, 2 * y0 - 2);
-                rb = row_ptr<const P>(l2, 2 * y0 - 1);
-            } else if (rr == h) {
-                ra = row_ptr<const P>(l2, 2 * (y0 + h));
-                rb = row_ptr<const P>(l2, 2 * (y0 + h) + 1);
-            } else {
-                ra = rb = row_ptr<const P>(l2, 2 * (y0 + h) + 1);
+        const int nl = (h + 4) * LS;
+#pragma unroll
+        for (int j = 0; j < PERL; j++) {
+            const int i = t + j * kThreads;
+            lvals[j] = 0;
+            const int r = i / LS - 2, c = i % LS - 2;
+            if (i < nl && c < w + 2) {
+                int xs = (c < 0 && !(edges & LRE_LEFT)) ? 0 : ((c >= w && !(edges & LRE_RIGHT)) ? w - 1 : c);
+                int lx = 2 * (x0 + xs);
+                const P *ra, *rb;
+                int rr = r;
+                if (rr >= h && !(edges & LRE_BOTTOM)) rr = h - 1;
+                if (rr < 0 && !(edges & LRE_TOP)) rr = 0;
+                if (rr >= 0 && rr < h) {
+                    ra = row_ptr<const P>(l3, 2 * (y0 + rr));
+                    rb = row_ptr<const P>(l3, 2 * (y0 + rr) + 1);
+                } else if (rr == -2) {
+                    ra = rb = row_ptr<const P>(l2, 2 * y0 - 2);
+                } else if (rr == -1) {
+                    ra = row_ptr<const P>(l2, 2 * y0 - 2);
+                    rb = row_ptr<const P>(l2, 2 * y0 - 1);
+                } else if (rr == h) {
+                    ra = row_ptr<const P>(l2, 2 * (y0 + h));
+                    rb = row_ptr<const P>(l2, 2 * (y0 + h) + 1);
+                } else {
+                    ra = rb = row_ptr<const P>(l2, 2 * (y0 + h) + 1);
+                }
+                int v;
+                if (a.cfl_ds == 0)
+                    v = ((int)ra[lx] + rb[lx] + ra[lx + 1] + rb[lx + 1]) >> 2;
+                else if (a.cfl_ds == 1)
+                    v = ((int)ra[lx] + rb[lx]) >> 1;
+                else
+                    v = ra[lx];
+                lvals[j] = (uint16_t)v;
             }
-            int v;
-            if (a.cfl_ds == 0)
-                v = ((int)ra[lx] + rb[lx] + ra[lx + 1] + rb[lx + 1]) >> 2;
-            else if (a.cfl_ds == 1)
-                v = ((int)ra[lx] + rb[lx]) >> 1;
-            else
-                v = ra[lx];
-            Ld[i] = (uint16_t)v;
         }
     }
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = t + j * kThreads;
+        if (i < nv) V[i] = vals[j];
+    }
+    if (uv_single) {
+#pragma unroll
+        for (int j = 0; j < PERL; j++) {
+            const int i = t + j * kThreads;
+            if (i < (h + 4) * LS) Ld[i] = lvals[j];
+        }
+    }
+    if (kind == LR_KIND_PC && t < 64 * 13 * 2 / 16) ((uint4 *)sm.s_pc)[t] = pcv;
+    if (kind == LR_KIND_MULTI)
+        for (int i = t; i < 16 * 16; i += kThreads) sm.s_mf[i] = a.mfilt[pl][i >> 4][i & 15];
     if (gdf) {
-        int ab = a.gdf_ref * 528 + a.gdf_qp * 88, wb = a.gdf_ref * 1584 + a.gdf_qp * 264;
-        for (int i = threadIdx.x; i < 88; i += kThreads) s_alpha[i] = a.t.gdf_alpha[ab + i];
-        for (int i = threadIdx.x; i < 264; i += kThreads) s_weight[i] = a.t.gdf_weight[wb + i];
+        if (t < 88 * 2 / 16) ((uint4 *)sm.s_alpha)[t] = alv;
+        if (t < 264 * 2 / 16) ((uint4 *)sm.s_weight)[t] = wv;
+        if (a.gdf_ref == 0)
+            ((uint4 *)sm.s_err)[t] = erv;
+        else if (t < 1000 / 8)
+            ((uint2 *)sm.s_err)[t] = erv2;
     }
     __syncthreads();
-    auto VA = [&](int r, int c) -> int { return V[(r + E) * VS + c + E]; };
-    const int down = bd == 12 ? 2 : 0, up = bd == 8 ? 2 : 0;
+    // samples of V relative to the centre pointer
+#define VAT(p, dy, dx) ((int)(p)[(dy) * VW + (dx)])
+    constexpr int up = sizeof(P) == 1 ? 2 : 0;  // GDF works in the 10-bit domain (8 and 10 bit here)
     // ---- GDF gradient cells: cell (j,k) covers centre columns 2j-1,2j and rows 2k-1,2k
     const int ncx = w / 2 + 1, ncy = h / 2 + 1;
     if (gdf) {
-        for (int i = threadIdx.x; i < ncx * ncy; i += kThreads) {
+        for (int i = t; i < ncx * ncy; i += kThreads) {
             int k = i / ncx, j = i - k * ncx;
-            int g[4] = {0, 0, 0, 0};
+            int g0 = 0, g1 = 0, g2 = 0, g3 = 0;
 #pragma unroll
             for (int yy = 0; yy < 2; yy++)
 #pragma unroll
                 for (int xx = 0; xx < 2; xx++) {
-                    int cy = 2 * k - 1 + yy, cx = 2 * j - 1 + xx;
-                    int b = VA(cy, cx) >> down;
-                    g[0] += abs(b * 2 - (VA(cy - 1, cx) >> down) - (VA(cy + 1, cx) >> down));
-                    g[1] += abs(b * 2 - (VA(cy, cx - 1) >> down) - (VA(cy, cx + 1) >> down));
-                    g[2] += abs(b * 2 - (VA(cy - 1, cx - 1) >> down) - (VA(cy + 1, cx + 1) >> down));
-                    g[3] += abs(b * 2 - (VA(cy + 1, cx - 1) >> down) - (VA(cy - 1, cx + 1) >> down));
+                    const uint16_t *p = &V[(2 * k - 1 + yy + E) * VW + 2 * j - 1 + xx + E];
+                    int b = VAT(p, 0, 0) * 2;
+                    g0 += abs(b - VAT(p, -1, 0) - VAT(p, 1, 0));
+                    g1 += abs(b - VAT(p, 0, -1) - VAT(p, 0, 1));
+                    g2 += abs(b - VAT(p, -1, -1) - VAT(p, 1, 1));
+                    g3 += abs(b - VAT(p, 1, -1) - VAT(p, -1, 1));
                 }
-#pragma unroll
-            for (int d = 0; d < 4; d++) cells[i * 4 + d] = (uint16_t)g[d];
+            uint16_t *cp = &cells[(k * CS + j) * 4];
+            cp[0] = (uint16_t)g0;
+            cp[1] = (uint16_t)g1;
+            cp[2] = (uint16_t)g2;
+            cp[3] = (uint16_t)g3;
         }
     }
     // ---- classes of the multi-class filters (one per 4x4 block)
     const int bw4 = w >> 2, bh4 = h >> 2;
     if (kind == LR_KIND_MULTI || kind == LR_KIND_PC) {
         const uint8_t *sub = kind == LR_KIND_PC ? a.t.pc_sub + a.widx * 256 : a.t.ns_sub + (a.widx * 7 + a.ns_idx) * 256;
-        for (int i = threadIdx.x; i < bw4 * bh4; i += kThreads) {
+        for (int i = t; i < bw4 * bh4; i += kThreads) {
             int by = i / bw4, bx = i - by * bw4;
             int f0 = 0, f1 = 0, f2v = 0, s = 0;
-            for (int dy = -1; dy <= 4; dy++)
+            for (int dy = -1; dy <= 4; dy++) {
+                const uint16_t *p = &V[(by * 4 + dy + E) * VW + bx * 4 + E];
+#pragma unroll
                 for (int dx = -1; dx <= 4; dx++) {
-                    int x = bx * 4 + dx, y = by * 4 + dy;
-                    int m = VA(y, x);
-                    f0 += abs(VA(y - 1, x) - 2 * m + VA(y + 1, x));
-                    f1 += abs(VA(y - 1, x + 1) - 2 * m + VA(y + 1, x - 1));
-                    f2v += abs(VA(y - 1, x - 1) - 2 * m + VA(y + 1, x + 1));
+                    int m2 = 2 * VAT(p, 0, dx);
+                    f0 += abs(VAT(p, -1, dx) - m2 + VAT(p, 1, dx));
+                    f1 += abs(VAT(p, -1, dx + 1) - m2 + VAT(p, 1, dx - 1));
+                    f2v += abs(VAT(p, -1, dx - 1) - m2 + VAT(p, 1, dx + 1));
                 }
+            }
             const int npx[3] = {16, 4, 1};
             for (int dy = -1; dy <= 1; dy++)
                 for (int dx = -1; dx <= 1; dx++) {
@@ -199,118 +293,153 @@ __device__ void lr_body(const LrArgs &a, int item, LrSmem &sm) {
         }
     }
     __syncthreads();
-    // ---- outputs
+    // ---- outputs, one 2x2 quad per lane: a quad shares its 4x4 block (class,
+    // lossless flag) and its GDF cell sums (class c and gradient features)
     const int sh = 12 - bd, rnd_g = (1 << sh) >> 1;
-    int err_base, gscale;
-    if (a.gdf_ref == 0) {
-        err_base = a.gdf_qp * 4096;
-        gscale = 8;
-    } else {
-        err_base = (a.gdf_ref - 1) * 6000 + a.gdf_qp * 1000;
-        gscale = 5;
+    const int gscale = a.gdf_ref == 0 ? 8 : 5;
+    int filt[18];
+    if (kind == LR_KIND_SINGLE) {
+#pragma unroll
+        for (int k = 0; k < 18; k++) filt[k] = it.filt[k];
     }
-    const int bias = a.gdf_ref * 6 + a.gdf_qp;
-    for (int i = threadIdx.x; i < w * h; i += kThreads) {
-        int y = i / w, x = i - y * w;
-        bool lossless = (it.ll[y >> 2] >> (x >> 2)) & 1;
-        int m = VA(y, x);
-        int out = m;
+    const int qw = w >> 1, nq = qw * (h >> 1);
+    for (int q = t; q < nq; q += kThreads) {
+        const int qy = q / qw, qx = q - qy * qw;
+        const int x = 2 * qx, y = 2 * qy;
+        const uint16_t *c0 = &V[(y + E) * VW + x + E];
+        const uint16_t *pp[4] = {c0, c0 + 1, c0 + VW, c0 + VW + 1};
+        int out[4];
+#pragma unroll
+        for (int p = 0; p < 4; p++) out[p] = VAT(pp[p], 0, 0);
+        const bool lossless = (it.ll[y >> 2] >> (x >> 2)) & 1;
         if (!lossless) {
             if (kind == LR_KIND_SINGLE && pl == 0) {
-                int s = m << 7;
 #pragma unroll
-                for (int k = 0; k < 16; k++) {
-                    int dy = kNsY[k][0], dx = kNsY[k][1];
-                    s += (VA(y + dy, x + dx) + VA(y - dy, x - dx) - 2 * m) * it.filt[k];
+                for (int p = 0; p < 4; p++) {
+                    int m = out[p], s = m << 7;
+#pragma unroll
+                    for (int k = 0; k < 16; k++)
+                        s += s16(VAT(pp[p], kNsYT[k].dy, kNsYT[k].dx) + VAT(pp[p], -kNsYT[k].dy, -kNsYT[k].dx) - 2 * m) *
+                             filt[k];
+                    out[p] = iclip((s + 64) >> 7, 0, bdmax);
                 }
-                out = iclip((s + 64) >> 7, 0, bdmax);
             } else if (kind == LR_KIND_SINGLE) {
-                int s = m << 7;
 #pragma unroll
-                for (int k = 0; k < 6; k++) {
-                    int dy = kNsUv[k][0], dx = kNsUv[k][1];
-                    s += (VA(y + dy, x + dx) + VA(y - dy, x - dx) - 2 * m) * it.filt[k];
-                }
-                int l = Ld[(y + 2) * LS + x + 2];
+                for (int p = 0; p < 4; p++) {
+                    int m = out[p], s = m << 7;
 #pragma unroll
-                for (int k = 0; k < 12; k++) {
-                    int dy = kNsUvY[k][0], dx = kNsUvY[k][1];
-                    s += ((int)Ld[(y + 2 + dy) * LS + x + 2 + dx] - l) * it.filt[6 + k];
+                    for (int k = 0; k < 6; k++)
+                        s += s16(VAT(pp[p], kNsUvT[k].dy, kNsUvT[k].dx) + VAT(pp[p], -kNsUvT[k].dy, -kNsUvT[k].dx) -
+                                 2 * m) * s16(filt[k]);
+                    const uint16_t *lp = &Ld[(y + (p >> 1) + 2) * LS + x + (p & 1) + 2];
+                    int l = lp[0];
+#pragma unroll
+                    for (int k = 0; k < 12; k++) s += s16((int)lp[kNsUvYT[k].dy * LS + kNsUvYT[k].dx] - l) * s16(filt[6 + k]);
+                    out[p] = iclip((s + 64) >> 7, 0, bdmax);
                 }
-                out = iclip((s + 64) >> 7, 0, bdmax);
             } else if (kind == LR_KIND_MULTI) {
-                const int8_t *fl = a.mfilt[pl][cls[(y >> 2) * 16 + (x >> 2)]];
-                int s = m << 7;
+                const int8_t *fl = &sm.s_mf[cls[(y >> 2) * 16 + (x >> 2)] * 16];
+                int fk[16];
 #pragma unroll
-                for (int k = 0; k < 16; k++) {
-                    int dy = kNsY[k][0], dx = kNsY[k][1];
-                    s += (VA(y + dy, x + dx) + VA(y - dy, x - dx) - 2 * m) * fl[k];
+                for (int k = 0; k < 16; k++) fk[k] = fl[k];
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    int m = out[p], s = m << 7;
+#pragma unroll
+                    for (int k = 0; k < 16; k++)
+                        s += s16(VAT(pp[p], kNsYT[k].dy, kNsYT[k].dx) + VAT(pp[p], -kNsYT[k].dy, -kNsYT[k].dx) - 2 * m) *
+                             s16(fk[k]);
+                    out[p] = iclip((s + 64) >> 7, 0, bdmax);
                 }
-                out = iclip((s + 64) >> 7, 0, bdmax);
             } else if (kind == LR_KIND_PC) {
-                const int16_t *fl = a.t.pc_filters + (a.widx * 64 + cls[(y >> 2) * 16 + (x >> 2)]) * 13;
-                int s = m * fl[12];
+                const int16_t *fl = &sm.s_pc[cls[(y >> 2) * 16 + (x >> 2)] * 13];
+                int fk[13];
 #pragma unroll
-                for (int k = 0; k < 12; k++) {
-                    int dy = kPc[k][0], dx = kPc[k][1];
-                    s += (VA(y + dy, x + dx) + VA(y - dy, x - dx)) * fl[k];
+                for (int k = 0; k < 13; k++) fk[k] = fl[k];
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    int s = s16(out[p]) * s16(fk[12]);
+#pragma unroll
+                    for (int k = 0; k < 12; k++)
+                        s += s16(VAT(pp[p], kPcT[k].dy, kPcT[k].dx) + VAT(pp[p], -kPcT[k].dy, -kPcT[k].dx)) * s16(fk[k]);
+                    out[p] = iclip((s + 64) >> 7, 0, bdmax);
                 }
-                out = iclip((s + 64) >> 7, 0, bdmax);
             }
             if (gdf) {
-                int x1 = x & ~1, hx = x1 >> 1, ky = (y & ~1) >> 1;
+                const uint16_t *cp = &cells[((y >> 1) * CS + (x >> 1)) * 4];
                 int gs[4];
 #pragma unroll
-                for (int d = 0; d < 4; d++)
-                    gs[d] = cells[(ky * ncx + hx) * 4 + d] + cells[(ky * ncx + hx + 1) * 4 + d] +
-                            cells[((ky + 1) * ncx + hx) * 4 + d] + cells[((ky + 1) * ncx + hx + 1) * 4 + d];
-                int c = (gs[0] <= gs[1]) | ((gs[2] <= gs[3]) << 1);
-                int iv[3];
+                for (int d = 0; d < 4; d++) gs[d] = cp[d] + cp[4 + d] + cp[CS * 4 + d] + cp[CS * 4 + 4 + d];
+                const int c = (gs[0] <= gs[1]) | ((gs[2] <= gs[3]) << 1);
+                const int bias = a.gdf_ref * 6 + a.gdf_qp;
+                int base[3];
 #pragma unroll
-                for (int k = 0; k < 3; k++) iv[k] = a.t.gdf_bias[bias * 3 + k];
+                for (int k = 0; k < 3; k++) base[k] = a.t.gdf_bias[bias * 3 + k];
 #pragma unroll
                 for (int d = 0; d < 4; d++) {
-                    int kk = d + 18;
-                    int alpha = s_alpha[kk * 4 + c];
-                    int v = dmin(gs[d] >> (4 - up), alpha);
+                    const int kk = d + 18;
+                    const int v = dmin(gs[d] >> (4 - up), (int)sm.s_alpha[kk * 4 + c]);
 #pragma unroll
-                    for (int k = 0; k < 3; k++) iv[k] += v * s_weight[k * 88 + kk * 4 + c];
+                    for (int k = 0; k < 3; k++) base[k] += s16(v) * s16(sm.s_weight[k * 88 + kk * 4 + c]);
                 }
-                int mm = m >> down;
+                int iv[4][3];
 #pragma unroll
+                for (int p = 0; p < 4; p++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++) iv[p][k] = base[k];
+#pragma unroll 3
                 for (int kk = 0; kk < 18; kk++) {
-                    int alpha = s_alpha[kk * 4 + c];
-                    int dy = kGdfCoords[kk][0], dx = kGdfCoords[kk][1];
-                    int av = VA(y - dy, x - dx) >> down, bv = VA(y + dy, x + dx) >> down;
-                    int above = iclip((av - mm) * (1 << up), -alpha, alpha);
-                    int below = iclip((bv - mm) * (1 << up), -alpha, alpha);
-                    int v = iclip(above + below, -512, 511);
+                    const int alpha = sm.s_alpha[kk * 4 + c];
+                    const int w0 = sm.s_weight[kk * 4 + c], w1 = sm.s_weight[88 + kk * 4 + c],
+                              w2 = sm.s_weight[176 + kk * 4 + c];  // int16
+                    const int dy = kGdfT[kk].dy, dx = kGdfT[kk].dx;
 #pragma unroll
-                    for (int k = 0; k < 3; k++) iv[k] += v * s_weight[k * 88 + kk * 4 + c];
+                    for (int p = 0; p < 4; p++) {
+                        const int m = VAT(pp[p], 0, 0);
+                        // alpha >= 0, so min(max(.)) is the clip
+                        const int above = dmin(dmax((VAT(pp[p], -dy, -dx) - m) * (1 << up), -alpha), alpha);
+                        const int below = dmin(dmax((VAT(pp[p], dy, dx) - m) * (1 << up), -alpha), alpha);
+                        const int v = s16(dmin(dmax(above + below, -512), 511));
+                        iv[p][0] += v * s16(w0);
+                        iv[p][1] += v * s16(w1);
+                        iv[p][2] += v * s16(w2);
+                    }
                 }
-                int full = 0;
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    int sv = iv[k] * gscale;
-                    int v = apply_sign((abs(sv) + (1 << 14)) >> 15, sv);
-                    full = full * (gscale * 2) + iclip(v, -gscale, gscale - 1) + gscale;
+                for (int p = 0; p < 4; p++) {
+                    int full = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        int sv = iv[p][k] * gscale;
+                        int v = apply_sign((abs(sv) + (1 << 14)) >> 15, sv);
+                        full = full * (gscale * 2) + iclip(v, -gscale, gscale - 1) + gscale;
+                    }
+                    int diff = sm.s_err[full] * a.gdf_scale;
+                    out[p] = iclip(out[p] + apply_sign((abs(diff) + rnd_g) >> sh, diff), 0, bdmax);
                 }
-                int err = a.gdf_ref == 0 ? a.t.gdf_intra[err_base + full] : a.t.gdf_inter[err_base + full];
-                int diff = err * a.gdf_scale;
-                out = iclip(out + apply_sign((abs(diff) + rnd_g) >> sh, diff), 0, bdmax);
             }
         }
-        row_ptr<P>(dst, y0 + y)[x0 + x] = (P)out;
+        P *d0 = row_ptr<P>(dst, y0 + y) + x0 + x;
+        P *d1 = row_ptr<P>(dst, y0 + y + 1) + x0 + x;
+        d0[0] = (P)out[0];
+        d0[1] = (P)out[1];
+        d1[0] = (P)out[2];
+        d1[1] = (P)out[3];
     }
+#undef VAT
 }
 
-template <typename P> __global__ __launch_bounds__(kThreads) void lr_kernel(LrArgs a) {
+// at least 4 waves per SIMD: the body is VALU/LDS-latency bound, so the
+// register budget is capped (128 VGPRs) in exchange for occupancy
+template <typename P>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void lr_kernel(LrArgs a) {
     __shared__ LrSmem sm;
     lr_body<P>(a, blockIdx.x, sm);
 }
 
 // Batched launch: every frame's items (restoration units and plane copies).
-__global__ __launch_bounds__(kThreads) void lr_batch(const LrArgs *args, BatchSched s) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void lr_batch(const LrArgs *args,
+                                                                                      BatchSched s) {
     __shared__ LrSmem sm;
     for_each_item(s, [&](int f, int t) {
         const LrArgs &a = args[f];
