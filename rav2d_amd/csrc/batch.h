@@ -6,27 +6,31 @@ namespace r2g {
 
 constexpr int kPasses = RAV2D_NUM_PASSES;
 
-// Host-side launch arguments of a set of frames.
+// Host-side launch arguments of a set of frames.  Each pass has one launch
+// per bit depth present (group 0: 8-bit frames, group 1: 10-bit frames), so
+// every launch runs one sample-type instantiation of the kernel body.
 struct HostBatch {
-    std::vector<DeblockArgs> dbv, dbh;
-    std::vector<CdefArgs> cdef;
-    std::vector<LrArgs> lr;
-    std::vector<GrainArgs> gapply, gsetup;
-    std::vector<int32_t> prefix[kPasses];
-    int64_t total[kPasses] = {};  // work items of each pass's launch
-    int64_t bytes[kPasses] = {};  // algorithmic bytes: one read + one write of the planes
-    int64_t pixels = 0;           // luma pixels of the frames
+    std::vector<DeblockArgs> dbv[2], dbh[2];
+    std::vector<CdefArgs> cdef[2];
+    std::vector<LrArgs> lr[2];
+    std::vector<GrainArgs> gapply[2], gsetup;
+    std::vector<int32_t> prefix[kPasses][2];
+    int64_t total[kPasses][2] = {};  // work items of each launch
+    int64_t bytes[kPasses] = {};     // algorithmic bytes: one read + one write of the planes
+    int64_t pixels = 0;              // luma pixels of the frames
+    int frames[kPasses] = {};        // frames in each pass
+    int64_t items(int p) const { return total[p][0] + total[p][1]; }
 };
 
 // Device view of a packed HostBatch.
 struct DevBatch {
-    DeblockArgs *dbv = nullptr, *dbh = nullptr;
-    CdefArgs *cdef = nullptr;
-    LrArgs *lr = nullptr;
-    GrainArgs *gapply = nullptr, *gsetup = nullptr;
-    int32_t *prefix[kPasses] = {};
-    int nf[kPasses] = {};
-    int32_t total[kPasses] = {};
+    DeblockArgs *dbv[2] = {}, *dbh[2] = {};
+    CdefArgs *cdef[2] = {};
+    LrArgs *lr[2] = {};
+    GrainArgs *gapply[2] = {}, *gsetup = nullptr;
+    int32_t *prefix[kPasses][2] = {};
+    int nf[kPasses][2] = {};
+    int32_t total[kPasses][2] = {};
     int n_setup = 0;
 };
 

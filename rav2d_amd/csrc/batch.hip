@@ -1,10 +1,12 @@
 // Batched filtering: many frames of any size and bit depth, one launch per
-// pass over all of them, optionally captured once and replayed as a hipGraph.
+// pass and bit depth over all of them, optionally captured once and replayed
+// as a hipGraph.
 //
-// Each pass's launch gets an array of per-frame argument blocks and the prefix
-// sum of the frames' work items (tiles, superblocks, restoration units); a
-// workgroup finds its frame by binary search over the prefix and dispatches on
-// the frame's bit depth at run time.  The launch shape (one workgroup per item,
+// Each launch gets an array of per-frame argument blocks and the prefix sum of
+// the frames' work items (tiles, superblocks, restoration units); a workgroup
+// finds its frame by binary search over the prefix.  8-bit and 10-bit frames
+// go to separate launches of the pass (each instantiates one sample type, so
+// its register budget is the per-frame kernel's).  The launch shape (one workgroup per item,
 // or a persistent grid that loops; XCD-contiguous item ranges or round robin)
 // is per pass and is what rav2d_gpu_batch_autotune picks.
 //
@@ -30,7 +32,7 @@ namespace {
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 struct Layout {
-    size_t dbv, dbh, cdef, lr, gapply, gsetup, prefix[kPasses], total;
+    size_t dbv[2], dbh[2], cdef[2], lr[2], gapply[2], gsetup, prefix[kPasses][2], total;
 };
 
 Layout layout_of(const HostBatch &hb) {
@@ -41,19 +43,20 @@ Layout layout_of(const HostBatch &hb) {
         off = align_up(off + bytes, 256);
         return o;
     };
-    L.dbv = reserve(hb.dbv.size() * sizeof(DeblockArgs));
-    L.dbh = reserve(hb.dbh.size() * sizeof(DeblockArgs));
-    L.cdef = reserve(hb.cdef.size() * sizeof(CdefArgs));
-    L.lr = reserve(hb.lr.size() * sizeof(LrArgs));
-    L.gapply = reserve(hb.gapply.size() * sizeof(GrainArgs));
+    for (int g = 0; g < 2; g++) {
+        L.dbv[g] = reserve(hb.dbv[g].size() * sizeof(DeblockArgs));
+        L.dbh[g] = reserve(hb.dbh[g].size() * sizeof(DeblockArgs));
+        L.cdef[g] = reserve(hb.cdef[g].size() * sizeof(CdefArgs));
+        L.lr[g] = reserve(hb.lr[g].size() * sizeof(LrArgs));
+        L.gapply[g] = reserve(hb.gapply[g].size() * sizeof(GrainArgs));
+        for (int p = 0; p < kPasses; p++) L.prefix[p][g] = reserve(hb.prefix[p][g].size() * sizeof(int32_t));
+    }
     L.gsetup = reserve(hb.gsetup.size() * sizeof(GrainArgs));
-    for (int p = 0; p < kPasses; p++) L.prefix[p] = reserve(hb.prefix[p].size() * sizeof(int32_t));
     L.total = std::max<size_t>(off, 256);
     return L;
 }
 
-int grid_of(const DevBatch &db, const LaunchShape &sh, int pass) {
-    int64_t total = db.total[pass];
+int grid_of(int64_t total, const LaunchShape &sh, int pass) {
     int g = sh.grid[pass];
     int64_t n = g > 0 ? std::min<int64_t>(g, total) : total;
     if (sh.xcd_remap[pass]) n = (n + 7) & ~int64_t(7);
@@ -78,52 +81,55 @@ int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply
         ChainPlan cp;
         int r = plan_chain(ctx, f, RAV2D_PASS_LR, cp);
         if (r) return r;
+        const int g = f->g.bitdepth > 8;
         // one read and one write of the frame's 1.5 * W * H samples
         const int64_t fbytes = 3 * (int64_t)f->pw[0] * f->ph[0] * f->bps;
         hb.pixels += (int64_t)f->pw[0] * f->ph[0];
         auto add = [&](int pass, int64_t count) {
-            hb.prefix[pass].push_back((int32_t)hb.total[pass]);
-            hb.total[pass] += count;
+            hb.prefix[pass][g].push_back((int32_t)hb.total[pass][g]);
+            hb.total[pass][g] += count;
             hb.bytes[pass] += fbytes;
+            hb.frames[pass]++;
         };
         if (cp.on[RAV2D_PASS_DEBLOCK_V]) {
-            hb.dbv.push_back(cp.dbv);
+            hb.dbv[g].push_back(cp.dbv);
             add(RAV2D_PASS_DEBLOCK_V, cp.dbv.map.first[3]);
         }
         if (cp.on[RAV2D_PASS_DEBLOCK_H]) {
-            hb.dbh.push_back(cp.dbh);
+            hb.dbh[g].push_back(cp.dbh);
             add(RAV2D_PASS_DEBLOCK_H, cp.dbh.map.first[3]);
         }
         if (cp.on[RAV2D_PASS_CDEF]) {
-            hb.cdef.push_back(cp.cdef);
+            hb.cdef[g].push_back(cp.cdef);
             add(RAV2D_PASS_CDEF, cdef_items(cp.cdef));
         }
-        hb.lr.push_back(cp.lr);
+        hb.lr[g].push_back(cp.lr);
         add(RAV2D_PASS_LR, cp.n_lr);
         if (apply_grain) {
-            GrainArgs g;
+            GrainArgs ga;
             if (frame_has_grain(f->params)) {
-                r = grain_args(f, g);
+                r = grain_args(f, ga);
                 if (r) return r;
-                hb.gsetup.push_back(g);
+                hb.gsetup.push_back(ga);
             } else {
-                memset(&g, 0, sizeof(g));
-                g.bitdepth = f->g.bitdepth;
+                memset(&ga, 0, sizeof(ga));
+                ga.bitdepth = f->g.bitdepth;
                 for (int pl = 0; pl < 3; pl++) {
-                    g.src[pl] = plane_of(f, BUF_OUT, pl);
-                    g.dst[pl] = plane_of(f, BUF_DISP, pl);
+                    ga.src[pl] = plane_of(f, BUF_OUT, pl);
+                    ga.dst[pl] = plane_of(f, BUF_DISP, pl);
                 }
             }
-            grain_tile_map(g);
-            hb.gapply.push_back(g);
-            add(RAV2D_PASS_GRAIN, g.map.first[3]);
+            grain_tile_map(ga);
+            hb.gapply[g].push_back(ga);
+            add(RAV2D_PASS_GRAIN, ga.map.first[3]);
         }
     }
     for (int p = 0; p < kPasses; p++)
-        if (hb.total[p] > INT32_MAX) {
-            set_last_error("batch too large: more than 2^31 work items in one pass");
-            return RAV2D_EINVAL;
-        }
+        for (int g = 0; g < 2; g++)
+            if (hb.total[p][g] > INT32_MAX) {
+                set_last_error("batch too large: more than 2^31 work items in one launch");
+                return RAV2D_EINVAL;
+            }
     return RAV2D_OK;
 }
 
@@ -134,30 +140,33 @@ void pack_batch(const HostBatch &hb, uint8_t *host, uint8_t *dev, DevBatch &db) 
     auto put = [&](size_t o, const void *src, size_t bytes) {
         if (bytes) memcpy(host + o, src, bytes);
     };
-    put(L.dbv, hb.dbv.data(), hb.dbv.size() * sizeof(DeblockArgs));
-    put(L.dbh, hb.dbh.data(), hb.dbh.size() * sizeof(DeblockArgs));
-    put(L.cdef, hb.cdef.data(), hb.cdef.size() * sizeof(CdefArgs));
-    put(L.lr, hb.lr.data(), hb.lr.size() * sizeof(LrArgs));
-    put(L.gapply, hb.gapply.data(), hb.gapply.size() * sizeof(GrainArgs));
-    put(L.gsetup, hb.gsetup.data(), hb.gsetup.size() * sizeof(GrainArgs));
-    for (int p = 0; p < kPasses; p++) put(L.prefix[p], hb.prefix[p].data(), hb.prefix[p].size() * sizeof(int32_t));
-    db.dbv = (DeblockArgs *)(dev + L.dbv);
-    db.dbh = (DeblockArgs *)(dev + L.dbh);
-    db.cdef = (CdefArgs *)(dev + L.cdef);
-    db.lr = (LrArgs *)(dev + L.lr);
-    db.gapply = (GrainArgs *)(dev + L.gapply);
-    db.gsetup = (GrainArgs *)(dev + L.gsetup);
-    for (int p = 0; p < kPasses; p++) {
-        db.prefix[p] = (int32_t *)(dev + L.prefix[p]);
-        db.nf[p] = (int)hb.prefix[p].size();
-        db.total[p] = (int32_t)hb.total[p];
+    db = DevBatch();
+    for (int g = 0; g < 2; g++) {
+        put(L.dbv[g], hb.dbv[g].data(), hb.dbv[g].size() * sizeof(DeblockArgs));
+        put(L.dbh[g], hb.dbh[g].data(), hb.dbh[g].size() * sizeof(DeblockArgs));
+        put(L.cdef[g], hb.cdef[g].data(), hb.cdef[g].size() * sizeof(CdefArgs));
+        put(L.lr[g], hb.lr[g].data(), hb.lr[g].size() * sizeof(LrArgs));
+        put(L.gapply[g], hb.gapply[g].data(), hb.gapply[g].size() * sizeof(GrainArgs));
+        db.dbv[g] = (DeblockArgs *)(dev + L.dbv[g]);
+        db.dbh[g] = (DeblockArgs *)(dev + L.dbh[g]);
+        db.cdef[g] = (CdefArgs *)(dev + L.cdef[g]);
+        db.lr[g] = (LrArgs *)(dev + L.lr[g]);
+        db.gapply[g] = (GrainArgs *)(dev + L.gapply[g]);
+        for (int p = 0; p < kPasses; p++) {
+            put(L.prefix[p][g], hb.prefix[p][g].data(), hb.prefix[p][g].size() * sizeof(int32_t));
+            db.prefix[p][g] = (int32_t *)(dev + L.prefix[p][g]);
+            db.nf[p][g] = (int)hb.prefix[p][g].size();
+            db.total[p][g] = (int32_t)hb.total[p][g];
+        }
     }
+    put(L.gsetup, hb.gsetup.data(), hb.gsetup.size() * sizeof(GrainArgs));
+    db.gsetup = (GrainArgs *)(dev + L.gsetup);
     db.n_setup = (int)hb.gsetup.size();
 }
 
 int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain, const EnqueueCtx &ec) {
     const hipStream_t s = ec.s;
-    const bool grain = apply_grain && db.total[RAV2D_PASS_GRAIN] > 0;
+    const bool grain = apply_grain && (db.total[RAV2D_PASS_GRAIN][0] + db.total[RAV2D_PASS_GRAIN][1]) > 0;
     if (grain && db.n_setup) {
         HIP_TRY(hipEventRecord(ec.ev_fork, s));
         HIP_TRY(hipStreamWaitEvent(ec.side, ec.ev_fork, 0));
@@ -166,25 +175,29 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
         HIP_TRY(hipEventRecord(ec.ev_join, ec.side));
     }
     for (int p = 0; p < kPasses; p++) {
-        if (db.total[p] == 0) continue;
+        if (db.total[p][0] + db.total[p][1] == 0) continue;
         if (p == RAV2D_PASS_GRAIN && !grain) continue;
         if (p == RAV2D_PASS_GRAIN && db.n_setup) HIP_TRY(hipStreamWaitEvent(s, ec.ev_join, 0));
         if (p == ec.time_pass) HIP_TRY(hipEventRecord(ec.ev_a, s));
         PassTimer pt(ec.profile, p, s);
-        BatchSched sc;
-        sc.prefix = db.prefix[p];
-        sc.n_frames = db.nf[p];
-        sc.total = db.total[p];
-        sc.xcd_remap = shape.xcd_remap[p] != 0;
-        const int grid = grid_of(db, shape, p);
-        switch (p) {
-            case RAV2D_PASS_DEBLOCK_V: launch_deblock_batch(true, db.dbv, sc, grid, s); break;
-            case RAV2D_PASS_DEBLOCK_H: launch_deblock_batch(false, db.dbh, sc, grid, s); break;
-            case RAV2D_PASS_CDEF: launch_cdef_batch(db.cdef, sc, grid, s); break;
-            case RAV2D_PASS_LR: launch_lr_batch(db.lr, sc, grid, s); break;
-            default: launch_grain_apply_batch(db.gapply, sc, grid, s); break;
+        for (int g = 0; g < 2; g++) {
+            if (db.total[p][g] == 0) continue;
+            BatchSched sc;
+            sc.prefix = db.prefix[p][g];
+            sc.n_frames = db.nf[p][g];
+            sc.total = db.total[p][g];
+            sc.xcd_remap = shape.xcd_remap[p] != 0;
+            const int grid = grid_of(db.total[p][g], shape, p);
+            const int bd = g ? 10 : 8;
+            switch (p) {
+                case RAV2D_PASS_DEBLOCK_V: launch_deblock_batch(true, db.dbv[g], sc, grid, bd, s); break;
+                case RAV2D_PASS_DEBLOCK_H: launch_deblock_batch(false, db.dbh[g], sc, grid, bd, s); break;
+                case RAV2D_PASS_CDEF: launch_cdef_batch(db.cdef[g], sc, grid, bd, s); break;
+                case RAV2D_PASS_LR: launch_lr_batch(db.lr[g], sc, grid, bd, s); break;
+                default: launch_grain_apply_batch(db.gapply[g], sc, grid, bd, s); break;
+            }
+            HIP_TRY(hipGetLastError());
         }
-        HIP_TRY(hipGetLastError());
         if (p == ec.time_pass) HIP_TRY(hipEventRecord(ec.ev_b, s));
     }
     return RAV2D_OK;
@@ -417,9 +430,9 @@ int rav2d_gpu_batch_set_config(Rav2dGpuBatch *b, const Rav2dBatchConfig *cfg) {
 
 int rav2d_gpu_batch_info(const Rav2dGpuBatch *b, int pass, int64_t *items, int64_t *bytes, int *frames) {
     if (!b || pass < 0 || pass >= kPasses) return RAV2D_EINVAL;
-    if (items) *items = b->hb.total[pass];
+    if (items) *items = b->hb.items(pass);
     if (bytes) *bytes = b->hb.bytes[pass];
-    if (frames) *frames = (int)b->hb.prefix[pass].size();
+    if (frames) *frames = b->hb.frames[pass];
     return RAV2D_OK;
 }
 
@@ -433,11 +446,11 @@ int rav2d_gpu_batch_autotune(Rav2dGpuBatch *b, int iters, Rav2dBatchConfig *chos
     hipStream_t s = b->ctx->stream;
     Rav2dBatchConfig best = b->cfg;
     for (int p = 0; p < kPasses; p++) {
-        if (b->hb.total[p] == 0) continue;
+        if (b->hb.items(p) == 0) continue;
         if (p == RAV2D_PASS_GRAIN && !b->cfg.apply_grain) continue;
         std::vector<std::pair<int, int>> cands = {{0, 0}, {0, 1}};
         for (int g : {256, 512, 1024, 2048, 4096})
-            if (g < b->hb.total[p]) {
+            if (g < b->hb.items(p)) {
                 cands.push_back({g, 0});
                 cands.push_back({g, 1});
             }

@@ -282,74 +282,74 @@ __device__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
         b_ypri[t] = ypri;
     }
     __syncthreads();
-    // ---- luma: lane t filters rows 2(t&3), 2(t&3)+1 of 8x8 block t>>2
+    // ---- luma: lane x = t & 63 of wave t >> 6 filters column x of rows
+    // 16 (t >> 6) .. +15 (two 8x8 blocks); a wave reads consecutive samples
     {
-        const int bi = t >> 2, r0 = (t & 3) * 2;
-        const int bx = (bi & 7) * 8, by = (bi >> 3) * 8;
-        const bool on = b_state[bi] & 1;
-        const int pri = y_pri ? b_ypri[bi] : 0;
-        const int dir = y_pri ? b_dir[bi] : 0;
-        const int pri_tap = 4 - ((pri >> bdm8) & 1);
-        const int pri_shift = pri ? dmax(0, damping - ulog2((uint32_t)pri)) : 0;
-        const int sec_shift = y_sec ? damping - ulog2((uint32_t)y_sec) : 0;
-        int op[2], os0[2], os1[2];
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            op[k] = sm.off[0][dir + 2][k];
-            os0[k] = sm.off[0][dir + 4][k];
-            os1[k] = sm.off[0][dir][k];
-        }
+        const int x = t & 63, ry = (t >> 6) * 16;
+        const int gx = x0 + x;
         const bool cc0 = a.ccso_on && m.ccso[0];
+        const int sec_shift = y_sec ? damping - ulog2((uint32_t)y_sec) : 0;
 #pragma unroll 1
-        for (int yy = r0; yy < r0 + 2; yy++) {
-            const int gy = y0 + by + yy;
-            if (gy >= fh) continue;
-            P *drow = row_ptr<P>(a.dst[0], gy);
+        for (int half = 0; half < 2; half++) {
+            const int by = ry + half * 8;
+            const int bi = (by >> 3) * 8 + (x >> 3);
+            const bool on = b_state[bi] & 1;
+            const int pri = y_pri ? b_ypri[bi] : 0;
+            const int dir = y_pri ? b_dir[bi] : 0;
+            const int pri_tap = 4 - ((pri >> bdm8) & 1);
+            const int pri_shift = pri ? dmax(0, damping - ulog2((uint32_t)pri)) : 0;
+            int op[2], os0[2], os1[2];
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                op[k] = sm.off[0][dir + 2][k];
+                os0[k] = sm.off[0][dir + 4][k];
+                os1[k] = sm.off[0][dir][k];
+            }
 #pragma unroll 2
-            for (int xx = 0; xx < 8; xx++) {
-                const int gx = x0 + bx + xx;
-                if (gx >= fw) continue;
-                const int16_t *tp = &ly[(by + yy + 2) * LY + bx + xx + 2];
+            for (int yy = by; yy < by + 8; yy++) {
+                const int gy = y0 + yy;
+                if (gy >= fh || gx >= fw) continue;
+                const int16_t *tp = &ly[(yy + 2) * LY + x + 2];
                 int v = tp[0];
                 // the reference stores the result through a truncating cast to the
                 // sample type (Pixel::from_i32), before CCSO reads it back
                 if (on) v = (int)(P)cdef_px(tp, op, os0, os1, pri, pri_tap, pri_shift, y_sec, sec_shift);
                 if (cc0) v = iclip(v + ccso_offset(a.ccso[0], ly, gx, gy, x0, y0, fw, fh), 0, a.g.bdmax);
-                drow[gx] = (P)v;
+                row_ptr<P>(a.dst[0], gy)[gx] = (P)v;
             }
         }
     }
-    // ---- chroma: lane t filters rows 2(t&1), 2(t&1)+1 of 4x4 block (t>>1)&63 of plane t>>7
+    // ---- chroma: lane (t >> 5) & 1 = plane, t & 31 = column, wave t >> 6
+    // filters rows 8 (t >> 6) .. +7 (two 4x4 blocks)
     {
-        const int pl = t >> 7, bi = (t >> 1) & 63, r0 = (t & 1) * 2;
-        const int bx = (bi & 7) * 4, by = (bi >> 3) * 4;
-        const bool on = b_state[bi] & 2;
-        const int dir = uv_pri ? b_dir[bi] : 0;
+        const int pl = (t >> 5) & 1, x = t & 31, ry = (t >> 6) * 8;
+        const int gx = (x0 >> 1) + x;
+        const bool cc = a.ccso_on && m.ccso[1 + pl];
         const int pri_tap = 4 - ((uv_pri >> bdm8) & 1);
         const int pri_shift = uv_pri ? dmax(0, damping - 1 - ulog2((uint32_t)uv_pri)) : 0;
         const int sec_shift = uv_sec ? damping - 1 - ulog2((uint32_t)uv_sec) : 0;
-        int op[2], os0[2], os1[2];
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            op[k] = sm.off[1][dir + 2][k];
-            os0[k] = sm.off[1][dir + 4][k];
-            os1[k] = sm.off[1][dir][k];
-        }
-        const bool cc = a.ccso_on && m.ccso[1 + pl];
 #pragma unroll 1
-        for (int yy = r0; yy < r0 + 2; yy++) {
-            const int gy = (y0 >> 1) + by + yy;
-            if (gy >= ch) continue;
-            P *drow = row_ptr<P>(a.dst[1 + pl], gy);
+        for (int half = 0; half < 2; half++) {
+            const int by = ry + half * 4;
+            const int bi = (by >> 2) * 8 + (x >> 2);
+            const bool on = b_state[bi] & 2;
+            const int dir = uv_pri ? b_dir[bi] : 0;
+            int op[2], os0[2], os1[2];
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                op[k] = sm.off[1][dir + 2][k];
+                os0[k] = sm.off[1][dir + 4][k];
+                os1[k] = sm.off[1][dir][k];
+            }
 #pragma unroll 2
-            for (int xx = 0; xx < 4; xx++) {
-                const int gx = (x0 >> 1) + bx + xx;
-                if (gx >= cw) continue;
-                const int16_t *tp = &lc[pl][(by + yy + 2) * LC + bx + xx + 2];
+            for (int yy = by; yy < by + 4; yy++) {
+                const int gy = (y0 >> 1) + yy;
+                if (gy >= ch || gx >= cw) continue;
+                const int16_t *tp = &lc[pl][(yy + 2) * LC + x + 2];
                 int v = tp[0];
                 if (on) v = (int)(P)cdef_px(tp, op, os0, os1, uv_pri, pri_tap, pri_shift, uv_sec, sec_shift);
                 if (cc) v = iclip(v + ccso_offset(a.ccso[1 + pl], ly, gx << 1, gy << 1, x0, y0, fw, fh), 0, a.g.bdmax);
-                drow[gx] = (P)v;
+                row_ptr<P>(a.dst[1 + pl], gy)[gx] = (P)v;
             }
         }
     }
@@ -361,7 +361,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
     cdef_body<P>(a, blockIdx.x, blockIdx.y, sm);
 }
 
-// Batched launch: one item per 64x64 superblock of every frame.
+// Batched launch: one item per 64x64 superblock of every frame of one bit depth.
+template <typename P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void cdef_batch(const CdefArgs *args,
                                                                                         BatchSched s) {
     __shared__ CdefSmem sm;
@@ -369,10 +370,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
         const CdefArgs &a = args[f];
         const int sbw = (a.g.bw * 4 + 63) >> 6;
         const int sby = t / sbw, sbx = t - sby * sbw;
-        if (a.g.bitdepth > 8)
-            cdef_body<uint16_t>(a, sbx, sby, sm);
-        else
-            cdef_body<uint8_t>(a, sbx, sby, sm);
+        cdef_body<P>(a, sbx, sby, sm);
         __syncthreads();
     });
 }
@@ -400,10 +398,12 @@ void cdef_init_consts() {
     }
 }
 
-void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, hipStream_t st) {
+void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st) {
     if (s.total <= 0) return;
-    cdef_init_consts();
-    hipLaunchKernelGGL(cdef_batch, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+    if (bitdepth > 8)
+        hipLaunchKernelGGL(cdef_batch<uint16_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+    else
+        hipLaunchKernelGGL(cdef_batch<uint8_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
 }
 
 void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s) {

@@ -277,37 +277,37 @@ template <typename P> __global__ __launch_bounds__(kThreads) void deblock_h_kern
     deblock_h_body<P>(a, blockIdx.x, tile);
 }
 
-// Batched launches: every frame's tiles, 8- and 10-bit frames alike.
-__global__ __launch_bounds__(kThreads) void deblock_v_batch(const DeblockArgs *args, BatchSched s) {
+// Batched launches: every tile of every frame of one bit depth.
+template <typename P> __global__ __launch_bounds__(kThreads) void deblock_v_batch(const DeblockArgs *args, BatchSched s) {
     __shared__ uint16_t tile[kTH][kTW + 2 * kHalo];
     for_each_item(s, [&](int f, int t) {
-        const DeblockArgs &a = args[f];
-        if (a.g.bitdepth > 8)
-            deblock_v_body<uint16_t>(a, t, tile);
-        else
-            deblock_v_body<uint8_t>(a, t, tile);
+        deblock_v_body<P>(args[f], t, tile);
         __syncthreads();
     });
 }
 
-__global__ __launch_bounds__(kThreads) void deblock_h_batch(const DeblockArgs *args, BatchSched s) {
+template <typename P> __global__ __launch_bounds__(kThreads) void deblock_h_batch(const DeblockArgs *args, BatchSched s) {
     __shared__ uint16_t tile[kTH + 2 * kHalo][kTW];
     for_each_item(s, [&](int f, int t) {
-        const DeblockArgs &a = args[f];
-        if (a.g.bitdepth > 8)
-            deblock_h_body<uint16_t>(a, t, tile);
-        else
-            deblock_h_body<uint8_t>(a, t, tile);
+        deblock_h_body<P>(args[f], t, tile);
         __syncthreads();
     });
 }
 
-void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, hipStream_t st) {
+void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, int bitdepth,
+                          hipStream_t st) {
     if (s.total <= 0) return;
-    if (vert)
-        hipLaunchKernelGGL(deblock_v_batch, dim3(grid), dim3(kThreads), 0, st, d_args, s);
-    else
-        hipLaunchKernelGGL(deblock_h_batch, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+    if (vert) {
+        if (bitdepth > 8)
+            hipLaunchKernelGGL(deblock_v_batch<uint16_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+        else
+            hipLaunchKernelGGL(deblock_v_batch<uint8_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+    } else {
+        if (bitdepth > 8)
+            hipLaunchKernelGGL(deblock_h_batch<uint16_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+        else
+            hipLaunchKernelGGL(deblock_h_batch<uint8_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+    }
 }
 
 TileMap deblock_tile_map(const DeblockArgs &a) {

@@ -306,14 +306,8 @@ template <typename P> __global__ __launch_bounds__(256) void grain_apply_kernel(
     grain_apply_body<P>(a, blockIdx.x);
 }
 
-__global__ __launch_bounds__(256) void grain_apply_batch(const GrainArgs *args, BatchSched s) {
-    for_each_item(s, [&](int f, int t) {
-        const GrainArgs &a = args[f];
-        if (a.bitdepth > 8)
-            grain_apply_body<uint16_t>(a, t);
-        else
-            grain_apply_body<uint8_t>(a, t);
-    });
+template <typename P> __global__ __launch_bounds__(256) void grain_apply_batch(const GrainArgs *args, BatchSched s) {
+    for_each_item(s, [&](int f, int t) { grain_apply_body<P>(args[f], t); });
 }
 
 // Per device: the Gaussian sequence in device memory and the LFSR jump-ahead
@@ -371,9 +365,12 @@ void launch_grain_setup_batch(const GrainArgs *d_args, int n, hipStream_t s) {
     hipLaunchKernelGGL(grain_setup_batch, dim3(n), dim3(kSetupThreads), 0, s, d_args, gauss);
 }
 
-void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &sc, int grid, hipStream_t s) {
+void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &sc, int grid, int bitdepth, hipStream_t s) {
     if (sc.total <= 0) return;
-    hipLaunchKernelGGL(grain_apply_batch, dim3(grid), dim3(256), 0, s, d_args, sc);
+    if (bitdepth > 8)
+        hipLaunchKernelGGL(grain_apply_batch<uint16_t>, dim3(grid), dim3(256), 0, s, d_args, sc);
+    else
+        hipLaunchKernelGGL(grain_apply_batch<uint8_t>, dim3(grid), dim3(256), 0, s, d_args, sc);
 }
 
 void grain_tile_map(GrainArgs &a) {

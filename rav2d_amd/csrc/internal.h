@@ -48,7 +48,7 @@ struct CdefArgs {
 };
 
 void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s);
-void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, hipStream_t st);
+void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st);
 int cdef_items(const CdefArgs &a);  // 64x64 superblocks of the frame
 void cdef_init_consts();            // constant tables of the current device
 
@@ -92,7 +92,7 @@ struct LrArgs {
 };
 
 void launch_lr(const LrArgs &a, int n_items, int bitdepth, hipStream_t s);
-void launch_lr_batch(const LrArgs *d_args, const BatchSched &s, int grid, hipStream_t st);
+void launch_lr_batch(const LrArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st);
 int lr_tables(int device, LrTables *out);
 
 // Film grain (filmgrain.rs): LUTs, scaling and block offsets live in a
@@ -113,13 +113,14 @@ struct GrainArgs {
 void launch_grain_setup(const GrainArgs &a, hipStream_t s);
 void launch_grain_apply(GrainArgs a, int bitdepth, hipStream_t s);
 void launch_grain_setup_batch(const GrainArgs *d_args, int n, hipStream_t s);
-void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &s, int grid, hipStream_t st);
+void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st);
 void grain_tile_map(GrainArgs &a);  // 64x32 tiles of the three planes
 bool grain_init_tables();           // Gaussian table and jump matrices of the current device
 
 void launch_deblock_v(const DeblockArgs &a, int bitdepth, hipStream_t s);
 void launch_deblock_h(const DeblockArgs &a, int bitdepth, hipStream_t s);
-void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, hipStream_t st);
+void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, int bitdepth,
+                          hipStream_t st);
 TileMap deblock_tile_map(const DeblockArgs &a);  // 64x32 tiles of the three planes
 
 void set_last_error(const std::string &msg);

@@ -437,23 +437,24 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
     lr_body<P>(a, blockIdx.x, sm);
 }
 
-// Batched launch: every frame's items (restoration units and plane copies).
+// Batched launch: every item (restoration units and plane copies) of every
+// frame of one bit depth.
+template <typename P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void lr_batch(const LrArgs *args,
                                                                                       BatchSched s) {
     __shared__ LrSmem sm;
     for_each_item(s, [&](int f, int t) {
-        const LrArgs &a = args[f];
-        if (a.g.bitdepth > 8)
-            lr_body<uint16_t>(a, t, sm);
-        else
-            lr_body<uint8_t>(a, t, sm);
+        lr_body<P>(args[f], t, sm);
         __syncthreads();
     });
 }
 
-void launch_lr_batch(const LrArgs *d_args, const BatchSched &sc, int grid, hipStream_t s) {
+void launch_lr_batch(const LrArgs *d_args, const BatchSched &sc, int grid, int bitdepth, hipStream_t s) {
     if (sc.total <= 0) return;
-    hipLaunchKernelGGL(lr_batch, dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+    if (bitdepth > 8)
+        hipLaunchKernelGGL(lr_batch<uint16_t>, dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+    else
+        hipLaunchKernelGGL(lr_batch<uint8_t>, dim3(grid), dim3(kThreads), 0, s, d_args, sc);
 }
 
 void launch_lr(const LrArgs &a, int n_items, int bitdepth, hipStream_t s) {
