@@ -193,7 +193,7 @@ __device__ __forceinline__ int ccso_offset(const CcsoDev &c, const int16_t *ly, 
 }
 
 template <typename P>
-__device__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
+__device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
     int16_t *ly = sm.ly;
     int16_t(*lc)[LC * LC] = sm.lc;
     uint8_t *b_state = sm.b_state, *b_dir = sm.b_dir;
@@ -362,11 +362,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
 }
 
 // Batched launch: one item per 64x64 superblock of every frame of one bit depth.
-template <typename P>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void cdef_batch(const CdefArgs *args,
+template <typename P, bool kLoop>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void cdef_batch(const CdefArgs *__restrict__ args,
                                                                                         BatchSched s) {
     __shared__ CdefSmem sm;
-    for_each_item(s, [&](int f, int t) {
+    for_each_item<kLoop>(s, [&](int f, int t) {
         const CdefArgs &a = args[f];
         const int sbw = (a.g.bw * 4 + 63) >> 6;
         const int sby = t / sbw, sbx = t - sby * sbw;
@@ -400,10 +400,18 @@ void cdef_init_consts() {
 
 void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st) {
     if (s.total <= 0) return;
-    if (bitdepth > 8)
-        hipLaunchKernelGGL(cdef_batch<uint16_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
-    else
-        hipLaunchKernelGGL(cdef_batch<uint8_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
+    const bool loop = grid < s.total;
+    if (bitdepth > 8) {
+        if (loop)
+            hipLaunchKernelGGL((cdef_batch<uint16_t, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+        else
+            hipLaunchKernelGGL((cdef_batch<uint16_t, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+    } else {
+        if (loop)
+            hipLaunchKernelGGL((cdef_batch<uint8_t, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+        else
+            hipLaunchKernelGGL((cdef_batch<uint8_t, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+    }
 }
 
 void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s) {

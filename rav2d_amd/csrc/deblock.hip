@@ -185,7 +185,7 @@ constexpr int kTW = 64, kTH = 32, kHalo = 16, kThreads = 256;
 // flight, then stores the tile to LDS.  Every (edge line, 4-line segment) of a
 // tile is one lane (152 for V, 176 for H).
 template <typename P>
-__device__ void deblock_v_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW + 2 * kHalo]) {
+__device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW + 2 * kHalo]) {
     int pl, tx, ty;
     decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl];
@@ -227,7 +227,7 @@ __device__ void deblock_v_body(const DeblockArgs &a, int tile_id, uint16_t (*til
 }
 
 template <typename P>
-__device__ void deblock_h_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW]) {
+__device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW]) {
     int pl, tx, ty;
     decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl];
@@ -278,17 +278,19 @@ template <typename P> __global__ __launch_bounds__(kThreads) void deblock_h_kern
 }
 
 // Batched launches: every tile of every frame of one bit depth.
-template <typename P> __global__ __launch_bounds__(kThreads) void deblock_v_batch(const DeblockArgs *args, BatchSched s) {
+template <typename P, bool kLoop>
+__global__ __launch_bounds__(kThreads) void deblock_v_batch(const DeblockArgs *__restrict__ args, BatchSched s) {
     __shared__ uint16_t tile[kTH][kTW + 2 * kHalo];
-    for_each_item(s, [&](int f, int t) {
+    for_each_item<kLoop>(s, [&](int f, int t) {
         deblock_v_body<P>(args[f], t, tile);
         __syncthreads();
     });
 }
 
-template <typename P> __global__ __launch_bounds__(kThreads) void deblock_h_batch(const DeblockArgs *args, BatchSched s) {
+template <typename P, bool kLoop>
+__global__ __launch_bounds__(kThreads) void deblock_h_batch(const DeblockArgs *__restrict__ args, BatchSched s) {
     __shared__ uint16_t tile[kTH + 2 * kHalo][kTW];
-    for_each_item(s, [&](int f, int t) {
+    for_each_item<kLoop>(s, [&](int f, int t) {
         deblock_h_body<P>(args[f], t, tile);
         __syncthreads();
     });
@@ -297,17 +299,28 @@ template <typename P> __global__ __launch_bounds__(kThreads) void deblock_h_batc
 void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, int bitdepth,
                           hipStream_t st) {
     if (s.total <= 0) return;
-    if (vert) {
-        if (bitdepth > 8)
-            hipLaunchKernelGGL(deblock_v_batch<uint16_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
-        else
-            hipLaunchKernelGGL(deblock_v_batch<uint8_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
-    } else {
-        if (bitdepth > 8)
-            hipLaunchKernelGGL(deblock_h_batch<uint16_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
-        else
-            hipLaunchKernelGGL(deblock_h_batch<uint8_t>, dim3(grid), dim3(kThreads), 0, st, d_args, s);
-    }
+    const bool loop = grid < s.total;
+#define R2G_DB_LAUNCH(K, P, L) hipLaunchKernelGGL((K<P, L>), dim3(grid), dim3(kThreads), 0, st, d_args, s)
+#define R2G_DB_PICK(K)                                     \
+    do {                                                   \
+        if (bitdepth > 8) {                                \
+            if (loop)                                      \
+                R2G_DB_LAUNCH(K, uint16_t, true);          \
+            else                                           \
+                R2G_DB_LAUNCH(K, uint16_t, false);         \
+        } else {                                           \
+            if (loop)                                      \
+                R2G_DB_LAUNCH(K, uint8_t, true);           \
+            else                                           \
+                R2G_DB_LAUNCH(K, uint8_t, false);          \
+        }                                                  \
+    } while (0)
+    if (vert)
+        R2G_DB_PICK(deblock_v_batch);
+    else
+        R2G_DB_PICK(deblock_h_batch);
+#undef R2G_DB_PICK
+#undef R2G_DB_LAUNCH
 }
 
 TileMap deblock_tile_map(const DeblockArgs &a) {

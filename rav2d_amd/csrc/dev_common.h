@@ -29,8 +29,16 @@ __device__ __forceinline__ int iclip(int v, int lo, int hi) { return v < lo ? lo
 __device__ __forceinline__ int apply_sign(int v, int s) { return s < 0 ? -v : v; }
 __device__ __forceinline__ int ulog2(uint32_t v) { return 31 ^ __clz((int)v); }
 
-template <typename P> __device__ __forceinline__ P *row_ptr(const DevPlane &p, int y) {
+template <typename P, typename D> __device__ __forceinline__ P *row_ptr(const D &p, int y) {
     return (P *)(p.ptr + (int64_t)y * p.stride);
+}
+
+// Argument blocks of batched launches live in device memory that no kernel
+// writes; reading them through the constant address space lets the compiler
+// use scalar loads (and reload instead of holding fields in registers).
+template <typename T> using ConstArg = const __attribute__((address_space(4))) T;
+template <typename T> __device__ __forceinline__ ConstArg<T> &const_arg(const T *base, int i) {
+    return *(ConstArg<T> *)(base + i);
 }
 
 // Tile schedule over the three planes of one frame: tiles of plane 0 first.
@@ -71,21 +79,29 @@ __device__ __forceinline__ int find_frame(const int32_t *prefix, int n, int item
 }
 
 // Visits the items of workgroup blockIdx.x in order: fn(frame, local item).
-template <typename F> __device__ __forceinline__ void for_each_item(const BatchSched &s, F fn) {
+// kLoop = false: one item per workgroup (the grid covers every item);
+// kLoop = true: a persistent grid, each workgroup loops.  Frame and item are
+// workgroup-uniform; readfirstlane tells the compiler so.
+template <bool kLoop, typename F> __device__ __forceinline__ void for_each_item(const BatchSched &s, F fn) {
     const int G = gridDim.x, b = blockIdx.x;
+    int hi = s.total, step = G, first = b;
     if (s.xcd_remap && (G & 7) == 0) {
         const int per_xcd = G >> 3, xcd = b & 7, slot = b >> 3;
         const int chunk = (s.total + 7) >> 3;
-        const int lo = xcd * chunk, hi = min(s.total, lo + chunk);
-        for (int it = lo + slot; it < hi; it += per_xcd) {
-            int f = find_frame(s.prefix, s.n_frames, it);
-            fn(f, it - s.prefix[f]);
+        first = xcd * chunk + slot;
+        hi = min(s.total, xcd * chunk + chunk);
+        step = per_xcd;
+    }
+    if (!kLoop) {
+        if (first < hi) {
+            const int f = __builtin_amdgcn_readfirstlane(find_frame(s.prefix, s.n_frames, first));
+            fn(f, __builtin_amdgcn_readfirstlane(first - s.prefix[f]));
         }
-    } else {
-        for (int it = b; it < s.total; it += G) {
-            int f = find_frame(s.prefix, s.n_frames, it);
-            fn(f, it - s.prefix[f]);
-        }
+        return;
+    }
+    for (int it = first; it < hi; it += step) {
+        const int f = __builtin_amdgcn_readfirstlane(find_frame(s.prefix, s.n_frames, it));
+        fn(f, __builtin_amdgcn_readfirstlane(it - s.prefix[f]));
     }
 }
 

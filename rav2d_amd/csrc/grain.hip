@@ -56,7 +56,7 @@ struct GrainSmem {
     int8_t win[3][4][7];
 };
 
-__device__ void grain_setup_body(const GrainArgs &a, const int16_t *gauss, GrainSmem &sm) {
+__device__ __forceinline__ void grain_setup_body(const GrainArgs &a, const int16_t *gauss, GrainSmem &sm) {
     auto &lut = sm.lut;
     auto &g_lds = sm.g_lds;
     auto &coef = sm.coef;
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kSetupThreads) void grain_setup_batch(const GrainAr
 // One output sample: S(bxo, byo) samples the block `bxo` to the left and
 // `byo` above, as sample_lut with offsets[bxo][byo].
 template <typename P>
-__device__ void grain_apply_body(const GrainArgs &a, int tile_id) {
+__device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id) {
     int pl, tx, ty;
     decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl], &dst = a.dst[pl];
@@ -306,8 +306,9 @@ template <typename P> __global__ __launch_bounds__(256) void grain_apply_kernel(
     grain_apply_body<P>(a, blockIdx.x);
 }
 
-template <typename P> __global__ __launch_bounds__(256) void grain_apply_batch(const GrainArgs *args, BatchSched s) {
-    for_each_item(s, [&](int f, int t) { grain_apply_body<P>(args[f], t); });
+template <typename P, bool kLoop>
+__global__ __launch_bounds__(256) void grain_apply_batch(const GrainArgs *__restrict__ args, BatchSched s) {
+    for_each_item<kLoop>(s, [&](int f, int t) { grain_apply_body<P>(args[f], t); });
 }
 
 // Per device: the Gaussian sequence in device memory and the LFSR jump-ahead
@@ -367,10 +368,18 @@ void launch_grain_setup_batch(const GrainArgs *d_args, int n, hipStream_t s) {
 
 void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &sc, int grid, int bitdepth, hipStream_t s) {
     if (sc.total <= 0) return;
-    if (bitdepth > 8)
-        hipLaunchKernelGGL(grain_apply_batch<uint16_t>, dim3(grid), dim3(256), 0, s, d_args, sc);
-    else
-        hipLaunchKernelGGL(grain_apply_batch<uint8_t>, dim3(grid), dim3(256), 0, s, d_args, sc);
+    const bool loop = grid < sc.total;
+    if (bitdepth > 8) {
+        if (loop)
+            hipLaunchKernelGGL((grain_apply_batch<uint16_t, true>), dim3(grid), dim3(256), 0, s, d_args, sc);
+        else
+            hipLaunchKernelGGL((grain_apply_batch<uint16_t, false>), dim3(grid), dim3(256), 0, s, d_args, sc);
+    } else {
+        if (loop)
+            hipLaunchKernelGGL((grain_apply_batch<uint8_t, true>), dim3(grid), dim3(256), 0, s, d_args, sc);
+        else
+            hipLaunchKernelGGL((grain_apply_batch<uint8_t, false>), dim3(grid), dim3(256), 0, s, d_args, sc);
+    }
 }
 
 void grain_tile_map(GrainArgs &a) {

@@ -62,6 +62,8 @@ constexpr Tap kGdfT[18] = {{6, 0}, {5, 0}, {4, 0}, {3, 0}, {2, 1}, {2, 0}, {2, -
 
 constexpr int LS = 64 + 4;  // stride of the downsampled luma rows (2-sample halo)
 
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
 // Narrows a value known to fit 16 bits (sums and differences of at most four
 // 8/10-bit samples, clamped GDF features) so that products with filter taps
 // compile to 24-bit multiplies.
@@ -78,11 +80,13 @@ struct LrSmem {
     alignas(16) int16_t s_pc[64 * 13];  // PC-Wiener filters of the frame's wiener_idx
     int8_t s_mf[16 * 16];               // multi-class frame filters of the plane
     alignas(16) int8_t s_err[4096];     // GDF error table slice of (ref, qp)
+    uint32_t apk[4 * 9];                // GDF alpha of taps (2i, 2i+1) per class, packed 16-bit pairs
+    uint32_t wpk[4 * 27];               // GDF weights likewise: [class][weight set][pair]
     LrItem it;
 };
 
-template <typename P>
-__device__ void lr_body(const LrArgs &a, int item, LrSmem &sm) {
+template <typename P, typename Args>
+__device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     uint16_t *V = sm.V, *Ld = sm.Ld, *cells = sm.cells;
     uint8_t *cls = sm.cls;
     LrItem &it = sm.it;
@@ -91,7 +95,9 @@ __device__ void lr_body(const LrArgs &a, int item, LrSmem &sm) {
     __syncthreads();
     const int pl = it.plane, x0 = it.x0, w = it.w, y0 = it.y0, h = it.h;
     const uint8_t edges = it.edges;
-    const DevPlane &f3 = a.f3[pl], &f2 = a.f2[pl], &dst = a.dst[pl];
+    auto &f3 = a.f3[pl];
+    auto &f2 = a.f2[pl];
+    auto &dst = a.dst[pl];
     const int bd = a.g.bitdepth, bdm8 = bd - 8, bdmax = a.g.bdmax;
     const int kind = it.kind;
     const bool gdf = it.gdf_on;
@@ -161,7 +167,8 @@ __device__ void lr_body(const LrArgs &a, int item, LrSmem &sm) {
     uint16_t lvals[PERL];
     if (uv_single) {
         // downsampled luma rows of the chroma stripe (cfl_ds_filter_index)
-        const DevPlane &l3 = a.f3[0], &l2 = a.f2[0];
+        auto &l3 = a.f3[0];
+        auto &l2 = a.f2[0];
         const int nl = (h + 4) * LS;
 #pragma unroll
         for (int j = 0; j < PERL; j++) {
@@ -230,6 +237,15 @@ __device__ void lr_body(const LrArgs &a, int item, LrSmem &sm) {
     // ---- GDF gradient cells: cell (j,k) covers centre columns 2j-1,2j and rows 2k-1,2k
     const int ncx = w / 2 + 1, ncy = h / 2 + 1;
     if (gdf) {
+        // tap-pair tables for the packed feature loop
+        if (t < 4 * 9) {
+            const int c = t / 9, q2 = t - c * 9;
+            sm.apk[t] = (uint32_t)sm.s_alpha[(2 * q2) * 4 + c] | ((uint32_t)sm.s_alpha[(2 * q2 + 1) * 4 + c] << 16);
+        } else if (t < 4 * 9 + 4 * 27) {
+            const int i = t - 4 * 9, c = i / 27, r = i - c * 27, k = r / 9, q2 = r - k * 9;
+            sm.wpk[i] = (uint32_t)(uint16_t)sm.s_weight[k * 88 + (2 * q2) * 4 + c] |
+                        ((uint32_t)(uint16_t)sm.s_weight[k * 88 + (2 * q2 + 1) * 4 + c] << 16);
+        }
         for (int i = t; i < ncx * ncy; i += kThreads) {
             int k = i / ncx, j = i - k * ncx;
             int g0 = 0, g1 = 0, g2 = 0, g3 = 0;
@@ -387,22 +403,34 @@ __device__ void lr_body(const LrArgs &a, int item, LrSmem &sm) {
                 for (int p = 0; p < 4; p++)
 #pragma unroll
                     for (int k = 0; k < 3; k++) iv[p][k] = base[k];
-#pragma unroll 3
-                for (int kk = 0; kk < 18; kk++) {
-                    const int alpha = sm.s_alpha[kk * 4 + c];
-                    const int w0 = sm.s_weight[kk * 4 + c], w1 = sm.s_weight[88 + kk * 4 + c],
-                              w2 = sm.s_weight[176 + kk * 4 + c];  // int16
-                    const int dy = kGdfT[kk].dy, dx = kGdfT[kk].dx;
+                // the 18 symmetric taps two at a time in packed 16-bit lanes
+                // (|sample - m| << up <= 1023, alpha <= 511, weights in
+                // [-512, 511]), with a 16-bit dot product per weight set
+                const uint32_t *apk = &sm.apk[c * 9];
+                const uint32_t *wpk = &sm.wpk[c * 27];
+#pragma unroll
+                for (int q2 = 0; q2 < 9; q2++) {
+                    const s16x2 al = __builtin_bit_cast(s16x2, apk[q2]);
+                    const s16x2 nal = -al;
+                    const s16x2 w0 = __builtin_bit_cast(s16x2, wpk[q2]);
+                    const s16x2 w1 = __builtin_bit_cast(s16x2, wpk[9 + q2]);
+                    const s16x2 w2 = __builtin_bit_cast(s16x2, wpk[18 + q2]);
+                    constexpr s16x2 lo = {-512, -512}, hi = {511, 511};
 #pragma unroll
                     for (int p = 0; p < 4; p++) {
-                        const int m = VAT(pp[p], 0, 0);
-                        // alpha >= 0, so min(max(.)) is the clip
-                        const int above = dmin(dmax((VAT(pp[p], -dy, -dx) - m) * (1 << up), -alpha), alpha);
-                        const int below = dmin(dmax((VAT(pp[p], dy, dx) - m) * (1 << up), -alpha), alpha);
-                        const int v = s16(dmin(dmax(above + below, -512), 511));
-                        iv[p][0] += v * s16(w0);
-                        iv[p][1] += v * s16(w1);
-                        iv[p][2] += v * s16(w2);
+                        const short m = (short)VAT(pp[p], 0, 0);
+                        const s16x2 mm = {m, m};
+                        const s16x2 A = {(short)VAT(pp[p], -kGdfT[2 * q2].dy, -kGdfT[2 * q2].dx),
+                                         (short)VAT(pp[p], -kGdfT[2 * q2 + 1].dy, -kGdfT[2 * q2 + 1].dx)};
+                        const s16x2 B = {(short)VAT(pp[p], kGdfT[2 * q2].dy, kGdfT[2 * q2].dx),
+                                         (short)VAT(pp[p], kGdfT[2 * q2 + 1].dy, kGdfT[2 * q2 + 1].dx)};
+                        s16x2 da = (A - mm) << (short)up, db = (B - mm) << (short)up;
+                        da = __builtin_elementwise_min(__builtin_elementwise_max(da, nal), al);
+                        db = __builtin_elementwise_min(__builtin_elementwise_max(db, nal), al);
+                        const s16x2 v = __builtin_elementwise_min(__builtin_elementwise_max(da + db, lo), hi);
+                        iv[p][0] = __builtin_amdgcn_sdot2(v, w0, iv[p][0], false);
+                        iv[p][1] = __builtin_amdgcn_sdot2(v, w1, iv[p][1], false);
+                        iv[p][2] = __builtin_amdgcn_sdot2(v, w2, iv[p][2], false);
                     }
                 }
 #pragma unroll
@@ -439,22 +467,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
 
 // Batched launch: every item (restoration units and plane copies) of every
 // frame of one bit depth.
-template <typename P>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void lr_batch(const LrArgs *args,
+template <typename P, bool kLoop>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void lr_batch(const LrArgs *__restrict__ args,
                                                                                       BatchSched s) {
     __shared__ LrSmem sm;
-    for_each_item(s, [&](int f, int t) {
-        lr_body<P>(args[f], t, sm);
+    for_each_item<kLoop>(s, [&](int f, int t) {
+        lr_body<P>(const_arg(args, f), t, sm);
         __syncthreads();
     });
 }
 
 void launch_lr_batch(const LrArgs *d_args, const BatchSched &sc, int grid, int bitdepth, hipStream_t s) {
     if (sc.total <= 0) return;
-    if (bitdepth > 8)
-        hipLaunchKernelGGL(lr_batch<uint16_t>, dim3(grid), dim3(kThreads), 0, s, d_args, sc);
-    else
-        hipLaunchKernelGGL(lr_batch<uint8_t>, dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+    const bool loop = grid < sc.total;
+    if (bitdepth > 8) {
+        if (loop)
+            hipLaunchKernelGGL((lr_batch<uint16_t, true>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+        else
+            hipLaunchKernelGGL((lr_batch<uint16_t, false>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+    } else {
+        if (loop)
+            hipLaunchKernelGGL((lr_batch<uint8_t, true>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+        else
+            hipLaunchKernelGGL((lr_batch<uint8_t, false>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+    }
 }
 
 void launch_lr(const LrArgs &a, int n_items, int bitdepth, hipStream_t s) {

@@ -47,7 +47,7 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
 
 __global__ __launch_bounds__(256) void checksum_batch(const SumArgs *args, BatchSched s) {
     __shared__ unsigned long long part[4];
-    for_each_item(s, [&](int f, int t) {
+    for_each_item<false>(s, [&](int f, int t) {
         const SumArgs &a = args[f];
         const int per = a.first[3];
         const int which = t / per;
