@@ -59,7 +59,8 @@ __device__ int find_dir(const int16_t *img, int ls, uint32_t *var, int bdm8) {
             a3[(y >> 1) + x] += px;
         }
     }
-    auto sq = [](int v) -> uint32_t { return (uint32_t)v * (uint32_t)v; };
+    // partial sums are at most 8 x 128 in magnitude: the squares are 24-bit products
+    auto sq = [](int v) -> uint32_t { return (uint32_t)((int)(int16_t)v * (int)(int16_t)v); };
     const uint32_t DIV[7] = {840, 420, 280, 210, 168, 140, 120};
     uint32_t cost[8];
     cost[2] = 0;
@@ -104,62 +105,79 @@ __device__ int find_dir(const int16_t *img, int ls, uint32_t *var, int bdm8) {
     return best_dir;
 }
 
-// One output sample of cdef_filter_block: `t` points at the sample in an LDS
-// tile whose outside-frame entries hold INT16_MIN; `op`, `os0`, `os1` are the
-// tap offsets (in that tile's stride) of the primary direction and of the two
-// secondary directions, for taps k = 0, 1.
-__device__ __forceinline__ int cdef_px(const int16_t *t, const int op[2], const int os0[2], const int os1[2], int pri,
-                                       int pri_tap, int pri_shift, int sec, int sec_shift) {
-    const int px = t[0];
-    int sum = 0;
+// Two horizontally adjacent output samples of cdef_filter_block in packed
+// 16-bit lanes.  `t` points at the left sample in an LDS tile whose
+// outside-frame entries hold INT16_MIN; `op`, `os0`, `os1` are the tap offsets
+// (in that tile's stride) of the primary direction and of the two secondary
+// directions, for taps k = 0, 1.  16 bits hold every intermediate: tap
+// differences are formed with a saturating subtract, so a difference with an
+// INT16_MIN neighbour stays at least 32767 in magnitude and constrains to 0
+// exactly as in 32-bit arithmetic (thresholds are at most 15 << 2 and shifts
+// at most 8); |sum| <= 12 * 4 * 60 < 2^15.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s16x2 splat2(int v) { return (s16x2){(short)v, (short)v}; }
+
+__device__ __forceinline__ s16x2 constrain2(s16x2 d, s16x2 thr, s16x2 sh) {
+    const s16x2 z = {0, 0};
+    const s16x2 ad = __builtin_elementwise_max(d, __builtin_elementwise_sub_sat(z, d));
+    const s16x2 c = __builtin_elementwise_max(z, thr - (ad >> sh));
+    return __builtin_elementwise_min(__builtin_elementwise_max(d, z - c), c);
+}
+
+__device__ __forceinline__ s16x2 cdef_px2(const int16_t *t, const int op[2], const int os0[2], const int os1[2],
+                                          int pri, int pri_tap, int pri_shift, int sec, int sec_shift) {
+    // two 16-bit LDS reads per pair: odd tap offsets would make a merged 32-bit
+    // read unaligned (volatile keeps the compiler from merging them)
+    const volatile int16_t *vt = t;
+    auto ld = [&](int o) -> s16x2 { return (s16x2){vt[o], vt[o + 1]}; };
+    const s16x2 px = ld(0);
+    const s16x2 ps = splat2(pri), pss = splat2(pri_shift), ss = splat2(sec), sss = splat2(sec_shift);
+    auto dif = [&](s16x2 v) { return __builtin_elementwise_sub_sat(v, px); };
+    s16x2 sum = {0, 0};
     if (pri) {
+        s16x2 tk = splat2(pri_tap);
         if (sec) {
-            int mx = px, mn = px, tk = pri_tap;
+            s16x2 mx = px, mn = px;
 #pragma unroll
             for (int k = 0; k < 2; k++) {
-                const int p0 = t[op[k]], p1 = t[-op[k]];
-                sum += tk * constrain(p0 - px, pri, pri_shift);
-                sum += tk * constrain(p1 - px, pri, pri_shift);
-                tk = (tk & 3) | 2;
-                mn = dmin(p0, mn);
-                mx = dmax(p0, mx);
-                mn = dmin(p1, mn);
-                mx = dmax(p1, mx);
-                const int s0 = t[os0[k]], s1 = t[-os0[k]], s2 = t[os1[k]], s3 = t[-os1[k]];
-                const int st = 2 - k;
-                sum += st * constrain(s0 - px, sec, sec_shift);
-                sum += st * constrain(s1 - px, sec, sec_shift);
-                sum += st * constrain(s2 - px, sec, sec_shift);
-                sum += st * constrain(s3 - px, sec, sec_shift);
-                mn = dmin(s0, mn);
-                mx = dmax(s0, mx);
-                mn = dmin(s1, mn);
-                mx = dmax(s1, mx);
-                mn = dmin(s2, mn);
-                mx = dmax(s2, mx);
-                mn = dmin(s3, mn);
-                mx = dmax(s3, mx);
+                const s16x2 p0 = ld(op[k]), p1 = ld(-op[k]);
+                sum += tk * constrain2(dif(p0), ps, pss);
+                sum += tk * constrain2(dif(p1), ps, pss);
+                tk = (tk & (short)3) | (short)2;
+                mn = __builtin_elementwise_min(mn, __builtin_elementwise_min(p0, p1));
+                mx = __builtin_elementwise_max(mx, __builtin_elementwise_max(p0, p1));
+                const s16x2 s0 = ld(os0[k]), s1 = ld(-os0[k]), s2 = ld(os1[k]), s3 = ld(-os1[k]);
+                const s16x2 st = splat2(2 - k);
+                sum += st * constrain2(dif(s0), ss, sss);
+                sum += st * constrain2(dif(s1), ss, sss);
+                sum += st * constrain2(dif(s2), ss, sss);
+                sum += st * constrain2(dif(s3), ss, sss);
+                mn = __builtin_elementwise_min(mn, __builtin_elementwise_min(__builtin_elementwise_min(s0, s1),
+                                                                              __builtin_elementwise_min(s2, s3)));
+                mx = __builtin_elementwise_max(mx, __builtin_elementwise_max(__builtin_elementwise_max(s0, s1),
+                                                                              __builtin_elementwise_max(s2, s3)));
             }
-            return iclip(px + ((sum - (sum < 0) + 8) >> 4), mn, mx);
+            const s16x2 v = px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
+            return __builtin_elementwise_min(__builtin_elementwise_max(v, mn), mx);
         }
-        int tk = pri_tap;
 #pragma unroll
         for (int k = 0; k < 2; k++) {
-            sum += tk * constrain(t[op[k]] - px, pri, pri_shift);
-            sum += tk * constrain(t[-op[k]] - px, pri, pri_shift);
-            tk = (tk & 3) | 2;
+            sum += tk * constrain2(dif(ld(op[k])), ps, pss);
+            sum += tk * constrain2(dif(ld(-op[k])), ps, pss);
+            tk = (tk & (short)3) | (short)2;
         }
-        return px + ((sum - (sum < 0) + 8) >> 4);
+        return px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
     }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-        const int st = 2 - k;
-        sum += st * constrain(t[os0[k]] - px, sec, sec_shift);
-        sum += st * constrain(t[-os0[k]] - px, sec, sec_shift);
-        sum += st * constrain(t[os1[k]] - px, sec, sec_shift);
-        sum += st * constrain(t[-os1[k]] - px, sec, sec_shift);
+        const s16x2 st = splat2(2 - k);
+        sum += st * constrain2(dif(ld(os0[k])), ss, sss);
+        sum += st * constrain2(dif(ld(-os0[k])), ss, sss);
+        sum += st * constrain2(dif(ld(os1[k])), ss, sss);
+        sum += st * constrain2(dif(ld(-os1[k])), ss, sss);
     }
-    return px + ((sum - (sum < 0) + 8) >> 4);
+    return px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
 }
 
 constexpr int kThreads = 256;
@@ -282,74 +300,90 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
         b_ypri[t] = ypri;
     }
     __syncthreads();
-    // ---- luma: lane x = t & 63 of wave t >> 6 filters column x of rows
-    // 16 (t >> 6) .. +15 (two 8x8 blocks); a wave reads consecutive samples
+    // ---- luma: lane t filters the column pair x = 2 (t & 31), 2 (t & 31) + 1
+    // of the 8 rows of block row t >> 5 (one 8x8 block per lane)
     {
-        const int x = t & 63, ry = (t >> 6) * 16;
+        const int x = 2 * (t & 31), by = 8 * (t >> 5);
         const int gx = x0 + x;
-        const bool cc0 = a.ccso_on && m.ccso[0];
+        const int bi = (by >> 3) * 8 + (x >> 3);
+        const bool on = b_state[bi] & 1;
+        const int pri = y_pri ? b_ypri[bi] : 0;
+        const int dir = y_pri ? b_dir[bi] : 0;
+        const int pri_tap = 4 - ((pri >> bdm8) & 1);
+        const int pri_shift = pri ? dmax(0, damping - ulog2((uint32_t)pri)) : 0;
         const int sec_shift = y_sec ? damping - ulog2((uint32_t)y_sec) : 0;
-#pragma unroll 1
-        for (int half = 0; half < 2; half++) {
-            const int by = ry + half * 8;
-            const int bi = (by >> 3) * 8 + (x >> 3);
-            const bool on = b_state[bi] & 1;
-            const int pri = y_pri ? b_ypri[bi] : 0;
-            const int dir = y_pri ? b_dir[bi] : 0;
-            const int pri_tap = 4 - ((pri >> bdm8) & 1);
-            const int pri_shift = pri ? dmax(0, damping - ulog2((uint32_t)pri)) : 0;
-            int op[2], os0[2], os1[2];
+        int op[2], os0[2], os1[2];
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                op[k] = sm.off[0][dir + 2][k];
-                os0[k] = sm.off[0][dir + 4][k];
-                os1[k] = sm.off[0][dir][k];
-            }
+        for (int k = 0; k < 2; k++) {
+            op[k] = sm.off[0][dir + 2][k];
+            os0[k] = sm.off[0][dir + 4][k];
+            os1[k] = sm.off[0][dir][k];
+        }
+        const bool cc0 = a.ccso_on && m.ccso[0];
+        if (gx < fw) {
 #pragma unroll 2
             for (int yy = by; yy < by + 8; yy++) {
                 const int gy = y0 + yy;
-                if (gy >= fh || gx >= fw) continue;
+                if (gy >= fh) break;
                 const int16_t *tp = &ly[(yy + 2) * LY + x + 2];
-                int v = tp[0];
-                // the reference stores the result through a truncating cast to the
-                // sample type (Pixel::from_i32), before CCSO reads it back
-                if (on) v = (int)(P)cdef_px(tp, op, os0, os1, pri, pri_tap, pri_shift, y_sec, sec_shift);
-                if (cc0) v = iclip(v + ccso_offset(a.ccso[0], ly, gx, gy, x0, y0, fw, fh), 0, a.g.bdmax);
-                row_ptr<P>(a.dst[0], gy)[gx] = (P)v;
+                int v0 = tp[0], v1 = tp[1];
+                if (on) {
+                    // the reference stores the result through a truncating cast to
+                    // the sample type (Pixel::from_i32), before CCSO reads it back
+                    const s16x2 r = cdef_px2(tp, op, os0, os1, pri, pri_tap, pri_shift, y_sec, sec_shift);
+                    v0 = (int)(P)(uint16_t)r.x;
+                    v1 = (int)(P)(uint16_t)r.y;
+                }
+                if (cc0) {
+                    v0 = iclip(v0 + ccso_offset(a.ccso[0], ly, gx, gy, x0, y0, fw, fh), 0, a.g.bdmax);
+                    v1 = iclip(v1 + ccso_offset(a.ccso[0], ly, gx + 1, gy, x0, y0, fw, fh), 0, a.g.bdmax);
+                }
+                P *d = row_ptr<P>(a.dst[0], gy) + gx;
+                d[0] = (P)v0;
+                d[1] = (P)v1;
             }
         }
     }
-    // ---- chroma: lane (t >> 5) & 1 = plane, t & 31 = column, wave t >> 6
-    // filters rows 8 (t >> 6) .. +7 (two 4x4 blocks)
+    // ---- chroma: lane t filters the column pair 2 (t & 15), +1 of the 4 rows of
+    // block row (t >> 4) & 7 of plane t >> 7 (one 4x4 block per lane)
     {
-        const int pl = (t >> 5) & 1, x = t & 31, ry = (t >> 6) * 8;
+        const int pl = t >> 7, x = 2 * (t & 15), by = 4 * ((t >> 4) & 7);
         const int gx = (x0 >> 1) + x;
-        const bool cc = a.ccso_on && m.ccso[1 + pl];
+        const int bi = (by >> 2) * 8 + (x >> 2);
+        const bool on = b_state[bi] & 2;
+        const int dir = uv_pri ? b_dir[bi] : 0;
         const int pri_tap = 4 - ((uv_pri >> bdm8) & 1);
         const int pri_shift = uv_pri ? dmax(0, damping - 1 - ulog2((uint32_t)uv_pri)) : 0;
         const int sec_shift = uv_sec ? damping - 1 - ulog2((uint32_t)uv_sec) : 0;
-#pragma unroll 1
-        for (int half = 0; half < 2; half++) {
-            const int by = ry + half * 4;
-            const int bi = (by >> 2) * 8 + (x >> 2);
-            const bool on = b_state[bi] & 2;
-            const int dir = uv_pri ? b_dir[bi] : 0;
-            int op[2], os0[2], os1[2];
+        int op[2], os0[2], os1[2];
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                op[k] = sm.off[1][dir + 2][k];
-                os0[k] = sm.off[1][dir + 4][k];
-                os1[k] = sm.off[1][dir][k];
-            }
+        for (int k = 0; k < 2; k++) {
+            op[k] = sm.off[1][dir + 2][k];
+            os0[k] = sm.off[1][dir + 4][k];
+            os1[k] = sm.off[1][dir][k];
+        }
+        const bool cc = a.ccso_on && m.ccso[1 + pl];
+        if (gx < cw) {
 #pragma unroll 2
             for (int yy = by; yy < by + 4; yy++) {
                 const int gy = (y0 >> 1) + yy;
-                if (gy >= ch || gx >= cw) continue;
+                if (gy >= ch) break;
                 const int16_t *tp = &lc[pl][(yy + 2) * LC + x + 2];
-                int v = tp[0];
-                if (on) v = (int)(P)cdef_px(tp, op, os0, os1, uv_pri, pri_tap, pri_shift, uv_sec, sec_shift);
-                if (cc) v = iclip(v + ccso_offset(a.ccso[1 + pl], ly, gx << 1, gy << 1, x0, y0, fw, fh), 0, a.g.bdmax);
-                row_ptr<P>(a.dst[1 + pl], gy)[gx] = (P)v;
+                int v0 = tp[0], v1 = tp[1];
+                if (on) {
+                    const s16x2 r = cdef_px2(tp, op, os0, os1, uv_pri, pri_tap, pri_shift, uv_sec, sec_shift);
+                    v0 = (int)(P)(uint16_t)r.x;
+                    v1 = (int)(P)(uint16_t)r.y;
+                }
+                if (cc) {
+                    v0 = iclip(v0 + ccso_offset(a.ccso[1 + pl], ly, gx << 1, gy << 1, x0, y0, fw, fh), 0,
+                               a.g.bdmax);
+                    v1 = iclip(v1 + ccso_offset(a.ccso[1 + pl], ly, (gx + 1) << 1, gy << 1, x0, y0, fw, fh), 0,
+                               a.g.bdmax);
+                }
+                P *d = row_ptr<P>(a.dst[1 + pl], gy) + gx;
+                d[0] = (P)v0;
+                d[1] = (P)v1;
             }
         }
     }

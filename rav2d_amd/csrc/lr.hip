@@ -460,7 +460,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
 // at least 4 waves per SIMD: the body is VALU/LDS-latency bound, so the
 // register budget is capped (128 VGPRs) in exchange for occupancy
 template <typename P>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void lr_kernel(LrArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void lr_kernel(const LrArgs a) {
     __shared__ LrSmem sm;
     lr_body<P>(a, blockIdx.x, sm);
 }
