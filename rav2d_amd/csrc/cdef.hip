@@ -41,57 +41,72 @@ __device__ __forceinline__ uint32_t ccso_score(int diff, int q, uint32_t edge_cl
 }
 
 // Direction search of one 8x8 luma block staged at `img` (row stride `ls`).
+// The eight line families are accumulated in two rounds of four (re-reading
+// the 64 samples from LDS), which halves the live partial sums.
 __device__ int find_dir(const int16_t *img, int ls, uint32_t *var, int bdm8) {
-    int hv0[8] = {0}, hv1[8] = {0}, d0[15] = {0}, d1[15] = {0};
-    int a0[11] = {0}, a1[11] = {0}, a2[11] = {0}, a3[11] = {0};
-#pragma unroll
-    for (int y = 0; y < 8; y++) {
-#pragma unroll
-        for (int x = 0; x < 8; x++) {
-            int px = ((int)img[y * ls + x] >> bdm8) - 128;
-            d0[y + x] += px;
-            a0[y + (x >> 1)] += px;
-            hv0[y] += px;
-            a1[3 + y - (x >> 1)] += px;
-            d1[7 + y - x] += px;
-            a2[3 - (y >> 1) + x] += px;
-            hv1[x] += px;
-            a3[(y >> 1) + x] += px;
-        }
-    }
     // partial sums are at most 8 x 128 in magnitude: the squares are 24-bit products
     auto sq = [](int v) -> uint32_t { return (uint32_t)((int)(int16_t)v * (int)(int16_t)v); };
     const uint32_t DIV[7] = {840, 420, 280, 210, 168, 140, 120};
     uint32_t cost[8];
-    cost[2] = 0;
-    cost[6] = 0;
+    {
+        int hv0[8] = {0}, hv1[8] = {0}, d0[15] = {0}, d1[15] = {0};
 #pragma unroll
-    for (int n = 0; n < 8; n++) {
-        cost[2] += sq(hv0[n]);
-        cost[6] += sq(hv1[n]);
+        for (int y = 0; y < 8; y++) {
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                int px = ((int)img[y * ls + x] >> bdm8) - 128;
+                d0[y + x] += px;
+                hv0[y] += px;
+                d1[7 + y - x] += px;
+                hv1[x] += px;
+            }
+            __builtin_amdgcn_sched_barrier(0);  // one row of loads in flight at a time
+        }
+        cost[2] = 0;
+        cost[6] = 0;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            cost[2] += sq(hv0[n]);
+            cost[6] += sq(hv1[n]);
+        }
+        cost[2] *= 105;
+        cost[6] *= 105;
+        cost[0] = 0;
+        cost[4] = 0;
+#pragma unroll
+        for (int n = 0; n < 7; n++) {
+            cost[0] += (sq(d0[n]) + sq(d0[14 - n])) * DIV[n];
+            cost[4] += (sq(d1[n]) + sq(d1[14 - n])) * DIV[n];
+        }
+        cost[0] += sq(d0[7]) * 105;
+        cost[4] += sq(d1[7]) * 105;
     }
-    cost[2] *= 105;
-    cost[6] *= 105;
-    cost[0] = 0;
-    cost[4] = 0;
+    {
+        int a0[11] = {0}, a1[11] = {0}, a2[11] = {0}, a3[11] = {0};
 #pragma unroll
-    for (int n = 0; n < 7; n++) {
-        cost[0] += (sq(d0[n]) + sq(d0[14 - n])) * DIV[n];
-        cost[4] += (sq(d1[n]) + sq(d1[14 - n])) * DIV[n];
-    }
-    cost[0] += sq(d0[7]) * 105;
-    cost[4] += sq(d1[7]) * 105;
-    int *alts[4] = {a0, a1, a2, a3};
+        for (int y = 0; y < 8; y++) {
 #pragma unroll
-    for (int n = 0; n < 4; n++) {
-        int *a = alts[n];
-        uint32_t c = 0;
+            for (int x = 0; x < 8; x++) {
+                int px = ((int)img[y * ls + x] >> bdm8) - 128;
+                a0[y + (x >> 1)] += px;
+                a1[3 + y - (x >> 1)] += px;
+                a2[3 - (y >> 1) + x] += px;
+                a3[(y >> 1) + x] += px;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        int *alts[4] = {a0, a1, a2, a3};
 #pragma unroll
-        for (int m = 0; m < 5; m++) c += sq(a[3 + m]);
-        c *= 105;
+        for (int n = 0; n < 4; n++) {
+            int *al = alts[n];
+            uint32_t c = 0;
 #pragma unroll
-        for (int m = 0; m < 3; m++) c += (sq(a[m]) + sq(a[10 - m])) * DIV[2 * m + 1];
-        cost[n * 2 + 1] = c;
+            for (int m = 0; m < 5; m++) c += sq(al[3 + m]);
+            c *= 105;
+#pragma unroll
+            for (int m = 0; m < 3; m++) c += (sq(al[m]) + sq(al[10 - m])) * DIV[2 * m + 1];
+            cost[n * 2 + 1] = c;
+        }
     }
     int best_dir = 0;
     uint32_t best = cost[0];
@@ -390,14 +405,14 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
 }
 
 template <typename P>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void cdef_kernel(CdefArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void cdef_kernel(CdefArgs a) {
     __shared__ CdefSmem sm;
     cdef_body<P>(a, blockIdx.x, blockIdx.y, sm);
 }
 
 // Batched launch: one item per 64x64 superblock of every frame of one bit depth.
 template <typename P, bool kLoop>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void cdef_batch(const CdefArgs *__restrict__ args,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void cdef_batch(const CdefArgs *__restrict__ args,
                                                                                         BatchSched s) {
     __shared__ CdefSmem sm;
     for_each_item<kLoop>(s, [&](int f, int t) {
