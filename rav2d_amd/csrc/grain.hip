@@ -254,103 +254,51 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         minv = 16 << bdm8;
         maxv = (pl == 0 || d.mc_identity ? 235 : 240) << bdm8;
     }
-    // A lane handles column x of rows yb + 4k, k < 8.  Its loads go out in
-    // rounds (samples, block offsets, grain LUT entries, scaling entries), each
-    // round for all eight samples at once from clamped (always valid) addresses.
-    constexpr int N = 4;  // samples per round: two rounds of rows yb0 + 4k and yb0 + 16 + 4k
-    const int x = tx * 64 + (threadIdx.x & 63);
-#pragma unroll 1
-    for (int half = 0; half < 2; half++) {
-        const int yb = ty * 32 + (threadIdx.x >> 6) + 16 * half;
-        const int W_ = src.w, H_ = src.h;
-        const bool colok = x < W_;
-        const int xc = dmin(x, W_ - 1);
-        int sv[N], lsum[N];
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-            const int yc = dmin(yb + 4 * k, H_ - 1);
-            sv[k] = row_ptr<const P>(src, yc)[xc];
-            lsum[k] = 0;
-            if (pl) {
-                const P *lrow = row_ptr<const P>(a.src[0], 2 * yc);
-                lsum[k] = (int)lrow[2 * xc] + lrow[2 * xc + 1];
-            }
-        }
+    for (int i = threadIdx.x; i < 64 * 32; i += 256) {
+        int x = tx * 64 + (i & 63), y = ty * 32 + (i >> 6);
+        if (x >= src.w || y >= src.h) continue;
+        int sv = row_ptr<const P>(src, y)[x];
         if (!on) {
-#pragma unroll
-            for (int k = 0; k < N; k++)
-                if (colok && yb + 4 * k < H_) row_ptr<P>(dst, yb + 4 * k)[x] = (P)sv[k];
+            row_ptr<P>(dst, y)[x] = (P)sv;
             continue;
         }
-        const int b = xc / pbs, lx = xc - b * pbs;
-        const int bw = dmin(pbs, W_ - b * pbs);
-        const int xstart = (d.overlap_flag && b > 0) ? dmin(2 >> sub, bw) : 0;
-        const int bl = dmax(b - 1, 0);
-        // offsets of the block, its left, upper and upper-left neighbours ([0] = y, [1] = x)
-        uint32_t o00[N], o10[N], o01[N], o11[N];
-        const uint16_t *ofs = (const uint16_t *)a.offsets;
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-            const int band = dmin(yb + 4 * k, H_ - 1) / pbs, bu = dmax(band - 1, 0);
-            o00[k] = ofs[band * a.n_blocks + b];
-            o10[k] = ofs[band * a.n_blocks + bl];
-            o01[k] = ofs[bu * a.n_blocks + b];
-            o11[k] = ofs[bu * a.n_blocks + bl];
-        }
-        const int step = 2 >> sub;
-        auto S = [&](uint32_t o, int ly, int bxo, int byo) -> int {
-            const int offx = 3 + step * (3 + (int)(o >> 8)), offy = 3 + step * (3 + (int)(o & 0xff));
-            return lut[dmin(offy + ly + pbs * byo, GH - 1)][dmin(offx + lx + pbs * bxo, GW - 1)];
+        int band = y / pbs, b = x / pbs;
+        int ly = y - band * pbs, lx = x - b * pbs;
+        int bh = dmin(pbs, src.h - band * pbs), bw = dmin(pbs, src.w - b * pbs);
+        int ystart = (d.overlap_flag && band > 0) ? dmin(2 >> sub, bh) : 0;
+        int xstart = (d.overlap_flag && b > 0) ? dmin(2 >> sub, bw) : 0;
+        auto S = [&](int bxo, int byo) -> int {
+            const uint8_t *o = a.offsets + ((size_t)(band - byo) * a.n_blocks + (b - bxo)) * 2;
+            int offx = 3 + (2 >> sub) * (3 + o[1]);
+            int offy = 3 + (2 >> sub) * (3 + o[0]);
+            return lut[offy + ly + pbs * byo][offx + lx + pbs * bxo];
         };
-        int g00[N], g10[N], g01[N], g11[N];
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-            const int yc = dmin(yb + 4 * k, H_ - 1);
-            const int ly = yc - (yc / pbs) * pbs;
-            g00[k] = S(o00[k], ly, 0, 0);
-            g10[k] = S(o10[k], ly, 1, 0);
-            g01[k] = S(o01[k], ly, 0, 1);
-            g11[k] = S(o11[k], ly, 1, 1);
-        }
         // overlap weights: [27,17],[17,27] (full-size); [23,22] (subsampled)
-        auto Wt = [&](int k, int j) -> int { return sub ? (k == 0 ? (j == 0 ? 23 : 22) : 0) : ((k == j) ? 27 : 17); };
-        int g[N], val[N];
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-            const int yc = dmin(yb + 4 * k, H_ - 1);
-            const int band = yc / pbs, ly = yc - band * pbs;
-            const int bh = dmin(pbs, H_ - band * pbs);
-            const int ystart = (d.overlap_flag && band > 0) ? dmin(2 >> sub, bh) : 0;
-            int gg = g00[k];
-            if (lx < xstart) gg = iclip(round2(g10[k] * Wt(lx, 0) + gg * Wt(lx, 1), 5), gmin, gmax);
-            if (ly < ystart) {
-                int top = g01[k];
-                if (lx < xstart) top = iclip(round2(g11[k] * Wt(lx, 0) + top * Wt(lx, 1), 5), gmin, gmax);
-                gg = iclip(round2(top * Wt(ly, 0) + gg * Wt(ly, 1), 5), gmin, gmax);
-            }
-            g[k] = gg;
-            int v = sv[k];
-            if (pl) {
-                const int avg = (lsum[k] + 1) >> 1;
-                if (!d.chroma_scaling_from_luma) {
-                    const int uv = pl - 1;
-                    const int combined = avg * d.uv_luma_mult[uv] + sv[k] * d.uv_mult[uv];
-                    v = iclip((combined >> 6) + d.uv_offset[uv] * (1 << bdm8), 0, (1 << bd) - 1);
-                } else {
-                    v = avg;
-                }
-            }
-            // samples above the coded maximum (wrapped CDEF results left unrestored)
-            // would index past the scaling LUT, where the reference aborts; clamp
-            val[k] = scaling[dmin(v, (1 << bd) - 1)];
+        auto W = [&](int k, int j) -> int { return sub ? (k == 0 ? (j == 0 ? 23 : 22) : 0) : ((k == j) ? 27 : 17); };
+        int g = S(0, 0);
+        if (lx < xstart) g = iclip(round2(S(1, 0) * W(lx, 0) + g * W(lx, 1), 5), gmin, gmax);
+        if (ly < ystart) {
+            int top = S(0, 1);
+            if (lx < xstart) top = iclip(round2(S(1, 1) * W(lx, 0) + top * W(lx, 1), 5), gmin, gmax);
+            g = iclip(round2(top * W(ly, 0) + g * W(ly, 1), 5), gmin, gmax);
         }
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-            const int y = yb + 4 * k;
-            if (!colok || y >= H_) continue;
-            const int noise = round2(val[k] * g[k], d.scaling_shift);
-            row_ptr<P>(dst, y)[x] = (P)iclip(sv[k] + noise, minv, maxv);
+        int val = sv;
+        if (pl) {
+            const P *lrow = row_ptr<const P>(a.src[0], 2 * y);
+            int l = lrow[2 * x];
+            int avg = (l + lrow[2 * x + 1] + 1) >> 1;
+            if (!d.chroma_scaling_from_luma) {
+                int uv = pl - 1;
+                int combined = avg * d.uv_luma_mult[uv] + sv * d.uv_mult[uv];
+                val = iclip((combined >> 6) + d.uv_offset[uv] * (1 << bdm8), 0, (1 << bd) - 1);
+            } else {
+                val = avg;
+            }
         }
+        // samples above the coded maximum (wrapped CDEF results left unrestored)
+        // would index past the scaling LUT, where the reference aborts; clamp
+        int noise = round2(scaling[dmin(val, (1 << bd) - 1)] * g, d.scaling_shift);
+        row_ptr<P>(dst, y)[x] = (P)iclip(sv + noise, minv, maxv);
     }
 }
 
