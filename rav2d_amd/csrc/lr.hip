@@ -37,7 +37,9 @@ __constant__ int16_t kModeOff[4] = {-547, -21565, -573, -680};
 
 constexpr int kThreads = 256;
 constexpr int E = 6;               // staged halo (GDF reach)
-constexpr int VW = 64 + 2 * E;     // staged width
+constexpr int VC = 8;              // column of item sample 0 in a staged row
+constexpr int VW = 64 + 2 * VC;    // staged row: item columns [-8, 72), loaded 4 samples at a time
+constexpr int VQ = VW / 4;
 constexpr int VH = 64 + 2 * E;     // staged height
 
 __device__ __forceinline__ int qval_tskip(int qstep, int tskip, int i, int bdm8) {
@@ -71,7 +73,7 @@ __device__ __forceinline__ int s16(int v) { return (int)(int16_t)v; }
 constexpr int CS = 33;      // stride of the GDF gradient cells
 
 struct LrSmem {
-    alignas(16) uint16_t V[VH * VW];  // virtual image, rows/cols [-E, 64 + E)
+    alignas(16) uint16_t V[VH * VW];  // virtual image, rows [-E, 64 + E), columns [-VC, 64 + VC)
     uint16_t Ld[(32 + 4) * LS];      // downsampled luma of a chroma stripe
     uint16_t cells[33 * CS * 4];     // GDF gradient cells
     uint8_t cls[16 * 16];            // class of each 4x4 block
@@ -129,17 +131,19 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     }
     // ---- staging.  Each lane issues all of its global loads before its first
     // LDS store, so the load latencies overlap instead of adding up.
-    // Virtual image V: rows [-E, h+E), columns [-E, w+E).
-    constexpr int PER = (VH * VW + kThreads - 1) / kThreads;
-    const int nv = (h + 2 * E) * VW;
-    uint16_t vals[PER];
+    // Virtual image V: rows [-E, h+E), item columns [-8, 72) in groups of 4
+    // samples read at once (x0 and plane widths are multiples of 4).  Groups
+    // are read from clamped, always valid addresses; the columns the
+    // reference replicates at the frame's left/right edge are patched after.
+    using Q = typename std::conditional<sizeof(P) == 1, uint32_t, uint2>::type;
+    constexpr int PERQ = (VH * VQ + kThreads - 1) / kThreads;
+    const int nvq = (h + 2 * E) * VQ;
+    const int pw = f3.w;
+    Q qv[PERQ];
 #pragma unroll
-    for (int j = 0; j < PER; j++) {
-        const int i = t + j * kThreads;
-        // rows and columns beyond the item clamp to valid ones, so the load is
-        // unconditional (lanes past the image load a harmless in-frame sample)
-        const int r = dmin(i / VW, h + 2 * E - 1) - E, c = dmin(i % VW - E, w + E - 1);
-        int xs = (c < 0 && !(edges & LRE_LEFT)) ? 0 : ((c >= w && !(edges & LRE_RIGHT)) ? w - 1 : c);
+    for (int j = 0; j < PERQ; j++) {
+        const int i = dmin(t + j * kThreads, nvq - 1);
+        const int r = i / VQ - E, c0 = (i % VQ) * 4 - VC;
         const P *row;
         if (r >= 0 && r < h)
             row = row_ptr<const P>(f3, y0 + r);
@@ -148,7 +152,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
         else
             row = (edges & LRE_BOTTOM) ? row_ptr<const P>(f2, y0 + h + (r == h ? 0 : 1))
                                        : row_ptr<const P>(f3, y0 + h - 1);
-        vals[j] = (uint16_t)row[x0 + xs];
+        qv[j] = *(const Q *)(row + iclip(x0 + c0, 0, pw - 4));
     }
     // tables of this item's filter kinds, 16 (or 8) bytes per lane
     uint4 pcv = {0, 0, 0, 0}, alv = {0, 0, 0, 0}, wv = {0, 0, 0, 0}, erv = {0, 0, 0, 0};
@@ -208,9 +212,19 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
         }
     }
 #pragma unroll
-    for (int j = 0; j < PER; j++) {
+    for (int j = 0; j < PERQ; j++) {
         const int i = t + j * kThreads;
-        if (i < nv) V[i] = vals[j];
+        if (i < nvq) {
+            uint2 u;
+            if constexpr (sizeof(P) == 1) {
+                const uint32_t b = (uint32_t)qv[j];
+                u.x = (b & 0xff) | ((b << 8) & 0xff0000);
+                u.y = ((b >> 16) & 0xff) | ((b >> 8) & 0xff0000);
+            } else {
+                u = *(const uint2 *)&qv[j];
+            }
+            *(uint2 *)&V[i * 4] = u;
+        }
     }
     if (uv_single) {
 #pragma unroll
@@ -229,6 +243,19 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
             ((uint4 *)sm.s_err)[t] = erv;
         else if (t < 1000 / 8)
             ((uint2 *)sm.s_err)[t] = erv2;
+    }
+    if (!(edges & LRE_LEFT) || !(edges & LRE_RIGHT)) {
+        // frame edge: columns -6..-1 repeat column 0, columns w..w+5 repeat w-1
+        __syncthreads();
+        for (int i = t; i < (h + 2 * E) * 12; i += kThreads) {
+            const int r = i / 12, k = i - r * 12;
+            uint16_t *vr = &V[r * VW + VC];
+            if (k < 6) {
+                if (!(edges & LRE_LEFT)) vr[k - 6] = vr[0];
+            } else if (!(edges & LRE_RIGHT)) {
+                vr[w + k - 6] = vr[w - 1];
+            }
+        }
     }
     __syncthreads();
     // samples of V relative to the centre pointer
@@ -253,7 +280,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
             for (int yy = 0; yy < 2; yy++)
 #pragma unroll
                 for (int xx = 0; xx < 2; xx++) {
-                    const uint16_t *p = &V[(2 * k - 1 + yy + E) * VW + 2 * j - 1 + xx + E];
+                    const uint16_t *p = &V[(2 * k - 1 + yy + E) * VW + 2 * j - 1 + xx + VC];
                     int b = VAT(p, 0, 0) * 2;
                     g0 += abs(b - VAT(p, -1, 0) - VAT(p, 1, 0));
                     g1 += abs(b - VAT(p, 0, -1) - VAT(p, 0, 1));
@@ -275,7 +302,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
             int by = i / bw4, bx = i - by * bw4;
             int f0 = 0, f1 = 0, f2v = 0, s = 0;
             for (int dy = -1; dy <= 4; dy++) {
-                const uint16_t *p = &V[(by * 4 + dy + E) * VW + bx * 4 + E];
+                const uint16_t *p = &V[(by * 4 + dy + E) * VW + bx * 4 + VC];
 #pragma unroll
                 for (int dx = -1; dx <= 4; dx++) {
                     int m2 = 2 * VAT(p, 0, dx);
@@ -322,7 +349,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     for (int q = t; q < nq; q += kThreads) {
         const int qy = q / qw, qx = q - qy * qw;
         const int x = 2 * qx, y = 2 * qy;
-        const uint16_t *c0 = &V[(y + E) * VW + x + E];
+        const uint16_t *c0 = &V[(y + E) * VW + x + VC];
         const uint16_t *pp[4] = {c0, c0 + 1, c0 + VW, c0 + VW + 1};
         int out[4];
 #pragma unroll
