@@ -194,16 +194,17 @@ __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id
     const int W = src.w, H = src.h;
     const int t = threadIdx.x;
     constexpr int LW = kTW + 2 * kHalo;
-    constexpr int PV = kTH * LW / kThreads;
-    uint16_t v[PV];
+    // 4-sample groups (x0 - kHalo and plane widths are multiples of 4, so a
+    // group is wholly inside or outside the plane), loaded from clamped,
+    // always valid addresses, then selected
+    constexpr int QW = LW / 4, PV = kTH * QW / kThreads;
+    uint2 v[PV];
 #pragma unroll
     for (int j = 0; j < PV; j++) {
         const int i = t + j * kThreads;
-        const int r = i / LW, c = i - r * LW;
-        const int gx = x0 - kHalo + c, gy = y0 + r;
-        // unconditional load from a clamped (always valid) address, then select
-        const uint16_t l = row_ptr<const P>(src, dmin(gy, H - 1))[iclip(gx, 0, W - 1)];
-        v[j] = (gy < H && gx >= 0 && gx < W) ? l : (uint16_t)0;
+        const int r = i / QW, gx = x0 - kHalo + (i - r * QW) * 4, gy = y0 + r;
+        const uint2 l = load4(row_ptr<const P>(src, dmin(gy, H - 1)) + iclip(gx, 0, W - 4));
+        v[j] = (gy < H && gx >= 0 && gx < W) ? l : make_uint2(0, 0);
     }
     constexpr int NE = kTW / 4 + 3;  // edge columns x0-4 .. x0+TW+4
     static_assert(NE * (kTH / 4) <= kThreads, "one lane per edge segment");
@@ -214,15 +215,17 @@ __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id
     // no filtering at the frame's left edge
     if (a.plane_on[pl] && t < NE * (kTH / 4) && ex > 0 && ex < W && ey < H) e = edge_desc(a, pl, true, ex >> 2, ey >> 2);
 #pragma unroll
-    for (int j = 0; j < PV; j++) (&tile[0][0])[t + j * kThreads] = v[j];
+    for (int j = 0; j < PV; j++) ((uint2 *)&tile[0][0])[t + j * kThreads] = v[j];
     __syncthreads();
     if (e.on)
         filter_edge(&tile[rs * 4][ex - x0 + kHalo], LW, 1, a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
     __syncthreads();
-    for (int i = t; i < kTH * kTW; i += kThreads) {
-        int r = i / kTW, c = i - r * kTW;
-        int gx = x0 + c, gy = y0 + r;
-        if (gy < H && gx < W) row_ptr<P>(dst, gy)[gx] = (P)tile[r][c + kHalo];
+#pragma unroll
+    for (int j = 0; j < kTH * kTW / 4 / kThreads; j++) {
+        const int i = t + j * kThreads;
+        const int r = i / (kTW / 4), c = (i - r * (kTW / 4)) * 4;
+        const int gx = x0 + c, gy = y0 + r;
+        if (gy < H && gx < W) store4(row_ptr<P>(dst, gy) + gx, *(const uint2 *)&tile[r][c + kHalo]);
     }
 }
 
@@ -236,15 +239,14 @@ __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id
     const int W = src.w, H = src.h;
     const int t = threadIdx.x;
     constexpr int LH = kTH + 2 * kHalo;
-    constexpr int PV = LH * kTW / kThreads;
-    uint16_t v[PV];
+    constexpr int QW = kTW / 4, PV = LH * QW / kThreads;
+    uint2 v[PV];
 #pragma unroll
     for (int j = 0; j < PV; j++) {
         const int i = t + j * kThreads;
-        const int r = i / kTW, c = i - r * kTW;
-        const int gx = x0 + c, gy = y0 - kHalo + r;
-        const uint16_t l = row_ptr<const P>(src, iclip(gy, 0, H - 1))[dmin(gx, W - 1)];
-        v[j] = (gy >= 0 && gy < H && gx < W) ? l : (uint16_t)0;
+        const int r = i / QW, gx = x0 + (i - r * QW) * 4, gy = y0 - kHalo + r;
+        const uint2 l = load4(row_ptr<const P>(src, iclip(gy, 0, H - 1)) + dmin(gx, W - 4));
+        v[j] = (gy >= 0 && gy < H && gx < W) ? l : make_uint2(0, 0);
     }
     constexpr int NE = kTH / 4 + 3;  // edge rows y0-4 .. y0+TH+4
     static_assert(NE * (kTW / 4) <= kThreads, "one lane per edge segment");
@@ -255,32 +257,34 @@ __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id
     // no filtering at the frame's top edge
     if (a.plane_on[pl] && t < NE * (kTW / 4) && ey > 0 && ey < H && ex < W) e = edge_desc(a, pl, false, ex >> 2, ey >> 2);
 #pragma unroll
-    for (int j = 0; j < PV; j++) (&tile[0][0])[t + j * kThreads] = v[j];
+    for (int j = 0; j < PV; j++) ((uint2 *)&tile[0][0])[t + j * kThreads] = v[j];
     __syncthreads();
     if (e.on)
         filter_edge(&tile[ey - y0 + kHalo][cs * 4], 1, kTW, a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
     __syncthreads();
-    for (int i = t; i < kTH * kTW; i += kThreads) {
-        int r = i / kTW, c = i - r * kTW;
-        int gx = x0 + c, gy = y0 + r;
-        if (gy < H && gx < W) row_ptr<P>(dst, gy)[gx] = (P)tile[r + kHalo][c];
+#pragma unroll
+    for (int j = 0; j < kTH * kTW / 4 / kThreads; j++) {
+        const int i = t + j * kThreads;
+        const int r = i / (kTW / 4), c = (i - r * (kTW / 4)) * 4;
+        const int gx = x0 + c, gy = y0 + r;
+        if (gy < H && gx < W) store4(row_ptr<P>(dst, gy) + gx, *(const uint2 *)&tile[r + kHalo][c]);
     }
 }
 
 template <typename P> __global__ __launch_bounds__(kThreads) void deblock_v_kernel(DeblockArgs a) {
-    __shared__ uint16_t tile[kTH][kTW + 2 * kHalo];
+    __shared__ __align__(16) uint16_t tile[kTH][kTW + 2 * kHalo];
     deblock_v_body<P>(a, blockIdx.x, tile);
 }
 
 template <typename P> __global__ __launch_bounds__(kThreads) void deblock_h_kernel(DeblockArgs a) {
-    __shared__ uint16_t tile[kTH + 2 * kHalo][kTW];
+    __shared__ __align__(16) uint16_t tile[kTH + 2 * kHalo][kTW];
     deblock_h_body<P>(a, blockIdx.x, tile);
 }
 
 // Batched launches: every tile of every frame of one bit depth.
 template <typename P, bool kLoop>
 __global__ __launch_bounds__(kThreads) void deblock_v_batch(const DeblockArgs *__restrict__ args, BatchSched s) {
-    __shared__ uint16_t tile[kTH][kTW + 2 * kHalo];
+    __shared__ __align__(16) uint16_t tile[kTH][kTW + 2 * kHalo];
     for_each_item<kLoop>(s, [&](int f, int t) {
         deblock_v_body<P>(args[f], t, tile);
         __syncthreads();
@@ -289,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void deblock_v_batch(const DeblockArgs *_
 
 template <typename P, bool kLoop>
 __global__ __launch_bounds__(kThreads) void deblock_h_batch(const DeblockArgs *__restrict__ args, BatchSched s) {
-    __shared__ uint16_t tile[kTH + 2 * kHalo][kTW];
+    __shared__ __align__(16) uint16_t tile[kTH + 2 * kHalo][kTW];
     for_each_item<kLoop>(s, [&](int f, int t) {
         deblock_h_body<P>(args[f], t, tile);
         __syncthreads();

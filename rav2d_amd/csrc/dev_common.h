@@ -33,6 +33,23 @@ template <typename P, typename D> __device__ __forceinline__ P *row_ptr(const D 
     return (P *)(p.ptr + (int64_t)y * p.stride);
 }
 
+// Four consecutive samples (4-sample aligned) <-> four u16 packed in a uint2,
+// the LDS tile layout.
+template <typename P> __device__ __forceinline__ uint2 load4(const P *p) {
+    if constexpr (sizeof(P) == 1) {
+        const uint32_t b = *(const uint32_t *)p;
+        return make_uint2((b & 0xff) | ((b << 8) & 0xff0000), ((b >> 16) & 0xff) | ((b >> 8) & 0xff0000));
+    } else {
+        return *(const uint2 *)p;
+    }
+}
+template <typename P> __device__ __forceinline__ void store4(P *p, uint2 v) {
+    if constexpr (sizeof(P) == 1)
+        *(uint32_t *)p = (v.x & 0xff) | ((v.x >> 8) & 0xff00) | ((v.y & 0xff) << 16) | ((v.y << 8) & 0xff000000);
+    else
+        *(uint2 *)p = v;
+}
+
 // Argument blocks of batched launches live in device memory that no kernel
 // writes; reading them through the constant address space lets the compiler
 // use scalar loads (and reload instead of holding fields in registers).
