@@ -196,11 +196,15 @@ __device__ __forceinline__ s16x2 cdef_px2(const int16_t *t, const int op[2], con
 }
 
 constexpr int kThreads = 256;
-constexpr int LY = 64 + 4, LC = 32 + 4;  // staged widths: 2-sample halo each side
+// Staged tiles: rows with a 2-sample halo; columns with a 4-sample halo so
+// that rows are loaded 4 samples per access (the filters read 2 beyond).
+constexpr int LYR = 64 + 4, LCR = 32 + 4;  // staged rows
+constexpr int LY = 64 + 8, LC = 32 + 8;    // staged row strides
+constexpr int CX = 4;                      // column of tile sample 0
 
 struct CdefSmem {
-    int16_t ly[LY * LY];
-    int16_t lc[2][LC * LC];
+    alignas(16) int16_t ly[LYR * LY];
+    alignas(16) int16_t lc[2][LCR * LC];
     uint8_t b_state[64];   // bit0 luma filtered, bit1 chroma filtered
     uint8_t b_dir[64];
     int16_t b_ypri[64];    // adjusted luma primary strength
@@ -211,11 +215,11 @@ struct CdefSmem {
 // (lx, lyy in frame coordinates), neighbours clamped to the frame.
 __device__ __forceinline__ int ccso_offset(const CcsoDev &c, const int16_t *ly, int lx, int lyy, int x0, int y0, int fw,
                                            int fh) {
-    const int cval = ly[(lyy - y0 + 2) * LY + lx - x0 + 2];
+    const int cval = ly[(lyy - y0 + 2) * LY + lx - x0 + CX];
     int idx = (int)((uint32_t)cval >> c.shift);
     if (!c.bo_only) {
-        int nx0 = iclip(lx + c.dx, 0, fw - 1) - x0 + 2, ny0 = iclip(lyy + c.dy, 0, fh - 1) - y0 + 2;
-        int nx1 = iclip(lx - c.dx, 0, fw - 1) - x0 + 2, ny1 = iclip(lyy - c.dy, 0, fh - 1) - y0 + 2;
+        int nx0 = iclip(lx + c.dx, 0, fw - 1) - x0 + CX, ny0 = iclip(lyy + c.dy, 0, fh - 1) - y0 + 2;
+        int nx1 = iclip(lx - c.dx, 0, fw - 1) - x0 + CX, ny1 = iclip(lyy - c.dy, 0, fh - 1) - y0 + 2;
         uint32_t c0 = ccso_score(ly[ny0 * LY + nx0] - cval, c.quant_step, c.edge_clf);
         uint32_t c1 = ccso_score(ly[ny1 * LY + nx1] - cval, c.quant_step, c.edge_clf);
         idx |= (int)((c0 << 5) | (c1 << 3));
@@ -228,7 +232,7 @@ __device__ __forceinline__ int ccso_offset(const CcsoDev &c, const int16_t *ly, 
 template <typename P>
 __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
     int16_t *ly = sm.ly;
-    int16_t(*lc)[LC * LC] = sm.lc;
+    int16_t(*lc)[LCR * LC] = sm.lc;
     uint8_t *b_state = sm.b_state, *b_dir = sm.b_dir;
     int16_t *b_ypri = sm.b_ypri;
     const int t = threadIdx.x;
@@ -239,34 +243,37 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
     // ---- staging of F2 with INT16_MIN outside the frame; every load of a lane
     // is issued before its first LDS store
     {
-        constexpr int PY = (LY * LY + kThreads - 1) / kThreads, PC = (2 * LC * LC + kThreads - 1) / kThreads;
-        int16_t vy[PY], vc[PC];
+        // 4-sample groups (tile origins and plane widths are multiples of 4, so a
+        // group is wholly inside or outside the frame), loaded from clamped,
+        // always valid addresses, then selected
+        constexpr int QY = LY / 4, QC = LC / 4;
+        constexpr int NY = LYR * QY, NC = 2 * LCR * QC;
+        constexpr int PY = (NY + kThreads - 1) / kThreads, PC = (NC + kThreads - 1) / kThreads;
+        const uint2 outside = make_uint2(0x80008000u, 0x80008000u);  // INT16_MIN x 4
+        uint2 vy[PY], vc[PC];
 #pragma unroll
         for (int j = 0; j < PY; j++) {
             const int i = t + j * kThreads;
-            const int r = i / LY, c = i - r * LY;
-            const int gx = x0 - 2 + c, gy = y0 - 2 + r;
-            // unconditional load from a clamped (always valid) address, then select
-            const int16_t v = (int16_t)row_ptr<const P>(a.src[0], iclip(gy, 0, fh - 1))[iclip(gx, 0, fw - 1)];
-            vy[j] = (i < LY * LY && gx >= 0 && gx < fw && gy >= 0 && gy < fh) ? v : (int16_t)INT16_MIN;
+            const int r = i / QY, gx = x0 - CX + (i - r * QY) * 4, gy = y0 - 2 + r;
+            const uint2 v = load4(row_ptr<const P>(a.src[0], iclip(gy, 0, fh - 1)) + iclip(gx, 0, fw - 4));
+            vy[j] = (gx >= 0 && gx < fw && gy >= 0 && gy < fh) ? v : outside;
         }
 #pragma unroll
         for (int j = 0; j < PC; j++) {
             const int i = t + j * kThreads;
-            const int pl = i >= LC * LC;
-            const int k = i - pl * LC * LC;
-            const int r = k / LC, c = k - r * LC;
-            const int gx = (x0 >> 1) - 2 + c, gy = (y0 >> 1) - 2 + r;
-            const int16_t v =
-                (int16_t)row_ptr<const P>(a.src[1 + (pl & 1)], iclip(gy, 0, ch - 1))[iclip(gx, 0, cw - 1)];
-            vc[j] = (i < 2 * LC * LC && gx >= 0 && gx < cw && gy >= 0 && gy < ch) ? v : (int16_t)INT16_MIN;
+            const int pl = i >= LCR * QC;
+            const int k = i - pl * LCR * QC;
+            const int r = k / QC, gx = (x0 >> 1) - CX + (k - r * QC) * 4, gy = (y0 >> 1) - 2 + r;
+            const uint2 v =
+                load4(row_ptr<const P>(a.src[1 + (pl & 1)], iclip(gy, 0, ch - 1)) + iclip(gx, 0, cw - 4));
+            vc[j] = (gx >= 0 && gx < cw && gy >= 0 && gy < ch) ? v : outside;
         }
 #pragma unroll
         for (int j = 0; j < PY; j++)
-            if (t + j * kThreads < LY * LY) ly[t + j * kThreads] = vy[j];
+            if (t + j * kThreads < NY) ((uint2 *)ly)[t + j * kThreads] = vy[j];
 #pragma unroll
         for (int j = 0; j < PC; j++)
-            if (t + j * kThreads < 2 * LC * LC) (&lc[0][0])[t + j * kThreads] = vc[j];
+            if (t + j * kThreads < NC) ((uint2 *)&lc[0][0])[t + j * kThreads] = vc[j];
         if (t < 48) {
             const int tab = t / 24, d = (t >> 1) % 12, k = t & 1;
             sm.off[tab][d][k] = (int16_t)(kCdefDyDx[d][k][0] * (tab ? LC : LY) + kCdefDyDx[d][k][1]);
@@ -298,7 +305,7 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
             uint16_t noskip = a.on_skiptx ? 0xffff : m.noskip_mask[(by & 0x3e) >> 1][sbx & 3];
             if (noskip & (3u << (bx & 14))) {
                 uint32_t var = 0;
-                int d = (y_pri || uv_pri) ? find_dir(&ly[(byi * 8 + 2) * LY + bxi * 8 + 2], LY, &var, bdm8) : 0;
+                int d = (y_pri || uv_pri) ? find_dir(&ly[(byi * 8 + 2) * LY + bxi * 8 + CX], LY, &var, bdm8) : 0;
                 if (y_pri) {
                     int adj = adjust_strength(y_pri, var);
                     if (adj || y_sec) st |= 1;
@@ -340,7 +347,7 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
             for (int yy = by; yy < by + 8; yy++) {
                 const int gy = y0 + yy;
                 if (gy >= fh) break;
-                const int16_t *tp = &ly[(yy + 2) * LY + x + 2];
+                const int16_t *tp = &ly[(yy + 2) * LY + x + CX];
                 int v0 = tp[0], v1 = tp[1];
                 if (on) {
                     // the reference stores the result through a truncating cast to
@@ -383,7 +390,7 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
             for (int yy = by; yy < by + 4; yy++) {
                 const int gy = (y0 >> 1) + yy;
                 if (gy >= ch) break;
-                const int16_t *tp = &lc[pl][(yy + 2) * LC + x + 2];
+                const int16_t *tp = &lc[pl][(yy + 2) * LC + x + CX];
                 int v0 = tp[0], v1 = tp[1];
                 if (on) {
                     const s16x2 r = cdef_px2(tp, op, os0, os1, uv_pri, pri_tap, pri_shift, uv_sec, sec_shift);
