@@ -254,51 +254,85 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         minv = 16 << bdm8;
         maxv = (pl == 0 || d.mc_identity ? 235 : 240) << bdm8;
     }
-    for (int i = threadIdx.x; i < 64 * 32; i += 256) {
-        int x = tx * 64 + (i & 63), y = ty * 32 + (i >> 6);
+    // one lane = four horizontally adjacent samples (a group never straddles a
+    // grain block: block sizes are multiples of 4), two groups per lane
+    auto Wt = [&](int k, int j) -> int { return sub ? (k == 0 ? (j == 0 ? 23 : 22) : 0) : ((k == j) ? 27 : 17); };
+#pragma unroll 1
+    for (int qi = threadIdx.x; qi < 16 * 32; qi += 256) {
+        const int x = tx * 64 + (qi & 15) * 4, y = ty * 32 + (qi >> 4);
         if (x >= src.w || y >= src.h) continue;
-        int sv = row_ptr<const P>(src, y)[x];
+        const uint2 sq = load4(row_ptr<const P>(src, y) + x);
+        int sv[4] = {(int)(sq.x & 0xffff), (int)(sq.x >> 16), (int)(sq.y & 0xffff), (int)(sq.y >> 16)};
         if (!on) {
-            row_ptr<P>(dst, y)[x] = (P)sv;
+            store4(row_ptr<P>(dst, y) + x, sq);
             continue;
         }
-        int band = y / pbs, b = x / pbs;
-        int ly = y - band * pbs, lx = x - b * pbs;
-        int bh = dmin(pbs, src.h - band * pbs), bw = dmin(pbs, src.w - b * pbs);
-        int ystart = (d.overlap_flag && band > 0) ? dmin(2 >> sub, bh) : 0;
-        int xstart = (d.overlap_flag && b > 0) ? dmin(2 >> sub, bw) : 0;
-        auto S = [&](int bxo, int byo) -> int {
-            const uint8_t *o = a.offsets + ((size_t)(band - byo) * a.n_blocks + (b - bxo)) * 2;
-            int offx = 3 + (2 >> sub) * (3 + o[1]);
-            int offy = 3 + (2 >> sub) * (3 + o[0]);
-            return lut[offy + ly + pbs * byo][offx + lx + pbs * bxo];
-        };
-        // overlap weights: [27,17],[17,27] (full-size); [23,22] (subsampled)
-        auto W = [&](int k, int j) -> int { return sub ? (k == 0 ? (j == 0 ? 23 : 22) : 0) : ((k == j) ? 27 : 17); };
-        int g = S(0, 0);
-        if (lx < xstart) g = iclip(round2(S(1, 0) * W(lx, 0) + g * W(lx, 1), 5), gmin, gmax);
-        if (ly < ystart) {
-            int top = S(0, 1);
-            if (lx < xstart) top = iclip(round2(S(1, 1) * W(lx, 0) + top * W(lx, 1), 5), gmin, gmax);
-            g = iclip(round2(top * W(ly, 0) + g * W(ly, 1), 5), gmin, gmax);
-        }
-        int val = sv;
+        int val[4];
         if (pl) {
-            const P *lrow = row_ptr<const P>(a.src[0], 2 * y);
-            int l = lrow[2 * x];
-            int avg = (l + lrow[2 * x + 1] + 1) >> 1;
-            if (!d.chroma_scaling_from_luma) {
-                int uv = pl - 1;
-                int combined = avg * d.uv_luma_mult[uv] + sv * d.uv_mult[uv];
-                val = iclip((combined >> 6) + d.uv_offset[uv] * (1 << bdm8), 0, (1 << bd) - 1);
-            } else {
-                val = avg;
+            const P *lrow = row_ptr<const P>(a.src[0], 2 * y) + 2 * x;
+            const uint2 l0 = load4(lrow), l1 = load4(lrow + 4);
+            const int ls[8] = {(int)(l0.x & 0xffff), (int)(l0.x >> 16), (int)(l0.y & 0xffff), (int)(l0.y >> 16),
+                               (int)(l1.x & 0xffff), (int)(l1.x >> 16), (int)(l1.y & 0xffff), (int)(l1.y >> 16)};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int avg = (ls[2 * k] + ls[2 * k + 1] + 1) >> 1;
+                if (!d.chroma_scaling_from_luma) {
+                    const int uv = pl - 1;
+                    const int combined = avg * d.uv_luma_mult[uv] + sv[k] * d.uv_mult[uv];
+                    val[k] = iclip((combined >> 6) + d.uv_offset[uv] * (1 << bdm8), 0, (1 << bd) - 1);
+                } else {
+                    val[k] = avg;
+                }
             }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) val[k] = sv[k];
         }
         // samples above the coded maximum (wrapped CDEF results left unrestored)
         // would index past the scaling LUT, where the reference aborts; clamp
-        int noise = round2(scaling[dmin(val, (1 << bd) - 1)] * g, d.scaling_shift);
-        row_ptr<P>(dst, y)[x] = (P)iclip(sv + noise, minv, maxv);
+        int sc[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) sc[k] = scaling[dmin(val[k], (1 << bd) - 1)];
+        const int band = y / pbs, b = x / pbs;
+        const int ly = y - band * pbs, lx0 = x - b * pbs;
+        const int bh = dmin(pbs, src.h - band * pbs), bw = dmin(pbs, src.w - b * pbs);
+        const int ystart = (d.overlap_flag && band > 0) ? dmin(2 >> sub, bh) : 0;
+        const int xstart = (d.overlap_flag && b > 0) ? dmin(2 >> sub, bw) : 0;
+        auto off = [&](int bxo, int byo) -> const uint8_t * {
+            return a.offsets + ((size_t)(band - byo) * a.n_blocks + (b - bxo)) * 2;
+        };
+        const uint8_t *o00 = off(0, 0);
+        const int offx = 3 + (2 >> sub) * (3 + o00[1]), offy = 3 + (2 >> sub) * (3 + o00[0]);
+        const int16_t *lr0 = &lut[offy + ly][offx + lx0];
+        int g[4] = {lr0[0], lr0[1], lr0[2], lr0[3]};
+        if (lx0 < xstart || ly < ystart) {
+            // overlap with the left and/or upper neighbour block (first 1-2 columns/rows)
+            auto S = [&](int bxo, int byo, int lx) -> int {
+                const uint8_t *o = off(bxo, byo);
+                const int ox = 3 + (2 >> sub) * (3 + o[1]), oy = 3 + (2 >> sub) * (3 + o[0]);
+                return lut[oy + ly + pbs * byo][ox + lx + pbs * bxo];
+            };
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int lx = lx0 + k;
+                int gg = g[k];
+                if (lx < xstart) gg = iclip(round2(S(1, 0, lx) * Wt(lx, 0) + gg * Wt(lx, 1), 5), gmin, gmax);
+                if (ly < ystart) {
+                    int top = S(0, 1, lx);
+                    if (lx < xstart) top = iclip(round2(S(1, 1, lx) * Wt(lx, 0) + top * Wt(lx, 1), 5), gmin, gmax);
+                    gg = iclip(round2(top * Wt(ly, 0) + gg * Wt(ly, 1), 5), gmin, gmax);
+                }
+                g[k] = gg;
+            }
+        }
+        int out[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int noise = round2(sc[k] * g[k], d.scaling_shift);
+            out[k] = iclip(sv[k] + noise, minv, maxv);
+        }
+        store4(row_ptr<P>(dst, y) + x, make_uint2((uint32_t)out[0] | ((uint32_t)out[1] << 16),
+                                                   (uint32_t)out[2] | ((uint32_t)out[3] << 16)));
     }
 }
 
