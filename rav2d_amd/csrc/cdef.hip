@@ -360,9 +360,7 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
                     v0 = iclip(v0 + ccso_offset(a.ccso[0], ly, gx, gy, x0, y0, fw, fh), 0, a.g.bdmax);
                     v1 = iclip(v1 + ccso_offset(a.ccso[0], ly, gx + 1, gy, x0, y0, fw, fh), 0, a.g.bdmax);
                 }
-                P *d = row_ptr<P>(a.dst[0], gy) + gx;
-                d[0] = (P)v0;
-                d[1] = (P)v1;
+                store2(row_ptr<P>(a.dst[0], gy) + gx, v0, v1);
             }
         }
     }
@@ -403,9 +401,7 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
                     v1 = iclip(v1 + ccso_offset(a.ccso[1 + pl], ly, (gx + 1) << 1, gy << 1, x0, y0, fw, fh), 0,
                                a.g.bdmax);
                 }
-                P *d = row_ptr<P>(a.dst[1 + pl], gy) + gx;
-                d[0] = (P)v0;
-                d[1] = (P)v1;
+                store2(row_ptr<P>(a.dst[1 + pl], gy) + gx, v0, v1);
             }
         }
     }

@@ -50,6 +50,14 @@ template <typename P> __device__ __forceinline__ void store4(P *p, uint2 v) {
         *(uint2 *)p = v;
 }
 
+// Two adjacent samples (2-sample aligned) with one store.
+template <typename P> __device__ __forceinline__ void store2(P *p, int a, int b) {
+    if constexpr (sizeof(P) == 1)
+        *(uint16_t *)p = (uint16_t)((a & 0xff) | ((b & 0xff) << 8));
+    else
+        *(uint32_t *)p = (uint32_t)(a & 0xffff) | ((uint32_t)b << 16);
+}
+
 // Argument blocks of batched launches live in device memory that no kernel
 // writes; reading them through the constant address space lets the compiler
 // use scalar loads (and reload instead of holding fields in registers).
