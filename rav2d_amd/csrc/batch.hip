@@ -66,7 +66,6 @@ int grid_of(int64_t total, const LaunchShape &sh, int pass) {
 }  // namespace
 
 int batch_init_device() {
-    cdef_init_consts();
     if (!grain_init_tables()) {
         set_last_error("film grain tables: device allocation failed");
         return RAV2D_ENOMEM;
@@ -366,8 +365,8 @@ int rav2d_gpu_batch_create(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n
         rav2d_gpu_batch_default_config(&b->cfg);
     if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_fork, kStreamSyncEventFlags) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join, kStreamSyncEventFlags) != hipSuccess ||
         hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess) {
         set_last_error("stream/event creation failed");
         rav2d_gpu_batch_destroy(b);

@@ -20,8 +20,6 @@
 
 namespace r2g {
 
-__constant__ int8_t kCdefDyDx[12][2][2];  // CDEF_DIRECTIONS as (dy, dx)
-__constant__ int8_t kCcsoOffset[4][8];
 
 __device__ __forceinline__ int constrain(int diff, int threshold, int shift) {
     int adiff = abs(diff);
@@ -40,84 +38,87 @@ __device__ __forceinline__ uint32_t ccso_score(int diff, int q, uint32_t edge_cl
     return 1;
 }
 
-// Direction search of one 8x8 luma block staged at `img` (row stride `ls`).
-// The eight line families are accumulated in two rounds of four (re-reading
-// the 64 samples from LDS), which halves the live partial sums.
-__device__ int find_dir(const int16_t *img, int ls, uint32_t *var, int bdm8) {
-    // partial sums are at most 8 x 128 in magnitude: the squares are 24-bit products
-    auto sq = [](int v) -> uint32_t { return (uint32_t)((int)(int16_t)v * (int)(int16_t)v); };
-    const uint32_t DIV[7] = {840, 420, 280, 210, 168, 140, 120};
-    uint32_t cost[8];
-    {
-        int hv0[8] = {0}, hv1[8] = {0}, d0[15] = {0}, d1[15] = {0};
+// Direction search of the 8x8 luma blocks, spread over the four waves of a
+// workgroup: wave `role` computes, for the block of each lane, the costs of
+// two orthogonal directions (d, d + 4) -- role 0: 0/4 (diagonal line sums),
+// 1: 2/6 (rows, columns), 2: 1/5, 3: 3/7 (the half-slope families).  The line
+// sums accumulate (px >> bdm8) as read and subtract 128 x (samples in the
+// line) afterwards, which equals summing (px >> bdm8) - 128 per sample.
+__device__ __forceinline__ uint32_t dir_sq(int v) {  // |v| <= 8 x 128: a 24-bit product
+    return (uint32_t)((int)(int16_t)v * (int)(int16_t)v);
+}
+
+// cost of the half-slope family with 11 line sums `al` (each biased by 128 x count)
+template <int N> __device__ __forceinline__ uint32_t alt_cost(const int (&al)[N], const int (&cnt)[N]) {
+    constexpr uint32_t DIV[7] = {840, 420, 280, 210, 168, 140, 120};
+    uint32_t c = 0;
 #pragma unroll
-        for (int y = 0; y < 8; y++) {
+    for (int m = 0; m < 5; m++) c += dir_sq(al[3 + m] - 128 * cnt[3 + m]);
+    c *= 105;
 #pragma unroll
-            for (int x = 0; x < 8; x++) {
-                int px = ((int)img[y * ls + x] >> bdm8) - 128;
-                d0[y + x] += px;
-                hv0[y] += px;
-                d1[7 + y - x] += px;
-                hv1[x] += px;
+    for (int m = 0; m < 3; m++)
+        c += (dir_sq(al[m] - 128 * cnt[m]) + dir_sq(al[10 - m] - 128 * cnt[10 - m])) * DIV[2 * m + 1];
+    return c;
+}
+
+template <int kRole>
+__device__ __forceinline__ void dir_costs(const int16_t *img, int ls, int bdm8, uint32_t &c_lo, uint32_t &c_hi) {
+    constexpr uint32_t DIV[7] = {840, 420, 280, 210, 168, 140, 120};
+    int s0[15] = {0}, s1[15] = {0};
+#pragma unroll
+    for (int y = 0; y < 8; y++) {
+        // one row: 8 samples in two 8-byte LDS reads (rows are 8-byte aligned)
+        const uint2 lo = *(const uint2 *)&img[y * ls], hi = *(const uint2 *)&img[y * ls + 4];
+        const uint32_t wv[4] = {lo.x, lo.y, hi.x, hi.y};
+#pragma unroll
+        for (int x = 0; x < 8; x++) {
+            const int px = (int)((wv[x >> 1] >> (16 * (x & 1))) & 0xffff) >> bdm8;
+            if constexpr (kRole == 0) {
+                s0[y + x] += px;
+                s1[7 + y - x] += px;
+            } else if constexpr (kRole == 1) {
+                s0[y] += px;
+                s1[x] += px;
+            } else if constexpr (kRole == 2) {
+                s0[y + (x >> 1)] += px;
+                s1[3 - (y >> 1) + x] += px;
+            } else {
+                s0[3 + y - (x >> 1)] += px;
+                s1[(y >> 1) + x] += px;
             }
-            __builtin_amdgcn_sched_barrier(0);  // one row of loads in flight at a time
         }
-        cost[2] = 0;
-        cost[6] = 0;
-#pragma unroll
-        for (int n = 0; n < 8; n++) {
-            cost[2] += sq(hv0[n]);
-            cost[6] += sq(hv1[n]);
-        }
-        cost[2] *= 105;
-        cost[6] *= 105;
-        cost[0] = 0;
-        cost[4] = 0;
+    }
+    if constexpr (kRole == 0) {
+        uint32_t c0 = 0, c1 = 0;
 #pragma unroll
         for (int n = 0; n < 7; n++) {
-            cost[0] += (sq(d0[n]) + sq(d0[14 - n])) * DIV[n];
-            cost[4] += (sq(d1[n]) + sq(d1[14 - n])) * DIV[n];
+            const int cn = n + 1;  // samples on the diagonals n and 14 - n
+            c0 += (dir_sq(s0[n] - 128 * cn) + dir_sq(s0[14 - n] - 128 * cn)) * DIV[n];
+            c1 += (dir_sq(s1[n] - 128 * cn) + dir_sq(s1[14 - n] - 128 * cn)) * DIV[n];
         }
-        cost[0] += sq(d0[7]) * 105;
-        cost[4] += sq(d1[7]) * 105;
+        c_lo = c0 + dir_sq(s0[7] - 1024) * 105;
+        c_hi = c1 + dir_sq(s1[7] - 1024) * 105;
+    } else if constexpr (kRole == 1) {
+        uint32_t c0 = 0, c1 = 0;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            c0 += dir_sq(s0[n] - 1024);
+            c1 += dir_sq(s1[n] - 1024);
+        }
+        c_lo = c0 * 105;
+        c_hi = c1 * 105;
+    } else {
+        // samples per line of a half-slope family: 2 x |{j in 0..3 : 0 <= n - j <= 7}|
+        constexpr int cnt[11] = {2, 4, 6, 8, 8, 8, 8, 8, 6, 4, 2};
+        int a0[11], a1[11];
+#pragma unroll
+        for (int n = 0; n < 11; n++) {
+            a0[n] = s0[n];
+            a1[n] = s1[n];
+        }
+        c_lo = alt_cost(a0, cnt);
+        c_hi = alt_cost(a1, cnt);
     }
-    {
-        int a0[11] = {0}, a1[11] = {0}, a2[11] = {0}, a3[11] = {0};
-#pragma unroll
-        for (int y = 0; y < 8; y++) {
-#pragma unroll
-            for (int x = 0; x < 8; x++) {
-                int px = ((int)img[y * ls + x] >> bdm8) - 128;
-                a0[y + (x >> 1)] += px;
-                a1[3 + y - (x >> 1)] += px;
-                a2[3 - (y >> 1) + x] += px;
-                a3[(y >> 1) + x] += px;
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        int *alts[4] = {a0, a1, a2, a3};
-#pragma unroll
-        for (int n = 0; n < 4; n++) {
-            int *al = alts[n];
-            uint32_t c = 0;
-#pragma unroll
-            for (int m = 0; m < 5; m++) c += sq(al[3 + m]);
-            c *= 105;
-#pragma unroll
-            for (int m = 0; m < 3; m++) c += (sq(al[m]) + sq(al[10 - m])) * DIV[2 * m + 1];
-            cost[n * 2 + 1] = c;
-        }
-    }
-    int best_dir = 0;
-    uint32_t best = cost[0];
-#pragma unroll
-    for (int n = 1; n < 8; n++)
-        if (cost[n] > best) {
-            best = cost[n];
-            best_dir = n;
-        }
-    *var = (best - cost[best_dir ^ 4]) >> 10;
-    return best_dir;
 }
 
 // Two horizontally adjacent output samples of cdef_filter_block in packed
@@ -143,54 +144,77 @@ __device__ __forceinline__ s16x2 constrain2(s16x2 d, s16x2 thr, s16x2 sh) {
 __device__ __forceinline__ s16x2 cdef_px2(const int16_t *t, const int op[2], const int os0[2], const int os1[2],
                                           int pri, int pri_tap, int pri_shift, int sec, int sec_shift) {
     // two 16-bit LDS reads per pair: odd tap offsets would make a merged 32-bit
-    // read unaligned (volatile keeps the compiler from merging them)
-    const volatile int16_t *vt = t;
-    auto ld = [&](int o) -> s16x2 { return (s16x2){vt[o], vt[o + 1]}; };
-    const s16x2 px = ld(0);
+    // read unaligned, so the reads are relaxed atomics, which are not merged.
+    // They keep their source order, so every tap is read before the first use.
+    auto ld = [&](int o) -> s16x2 { return (s16x2){lds_load_i16(t + o), lds_load_i16(t + o + 1)}; };
     const s16x2 ps = splat2(pri), pss = splat2(pri_shift), ss = splat2(sec), sss = splat2(sec_shift);
-    auto dif = [&](s16x2 v) { return __builtin_elementwise_sub_sat(v, px); };
     s16x2 sum = {0, 0};
-    if (pri) {
-        s16x2 tk = splat2(pri_tap);
-        if (sec) {
-            s16x2 mx = px, mn = px;
+    if (pri && sec) {
+        const s16x2 px = ld(0);
+        s16x2 pv[4], sv[8];
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const s16x2 p0 = ld(op[k]), p1 = ld(-op[k]);
-                sum += tk * constrain2(dif(p0), ps, pss);
-                sum += tk * constrain2(dif(p1), ps, pss);
-                tk = (tk & (short)3) | (short)2;
-                mn = __builtin_elementwise_min(mn, __builtin_elementwise_min(p0, p1));
-                mx = __builtin_elementwise_max(mx, __builtin_elementwise_max(p0, p1));
-                const s16x2 s0 = ld(os0[k]), s1 = ld(-os0[k]), s2 = ld(os1[k]), s3 = ld(-os1[k]);
-                const s16x2 st = splat2(2 - k);
-                sum += st * constrain2(dif(s0), ss, sss);
-                sum += st * constrain2(dif(s1), ss, sss);
-                sum += st * constrain2(dif(s2), ss, sss);
-                sum += st * constrain2(dif(s3), ss, sss);
-                mn = __builtin_elementwise_min(mn, __builtin_elementwise_min(__builtin_elementwise_min(s0, s1),
-                                                                              __builtin_elementwise_min(s2, s3)));
-                mx = __builtin_elementwise_max(mx, __builtin_elementwise_max(__builtin_elementwise_max(s0, s1),
-                                                                              __builtin_elementwise_max(s2, s3)));
-            }
-            const s16x2 v = px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
-            return __builtin_elementwise_min(__builtin_elementwise_max(v, mn), mx);
+        for (int k = 0; k < 2; k++) {
+            pv[2 * k] = ld(op[k]);
+            pv[2 * k + 1] = ld(-op[k]);
         }
 #pragma unroll
         for (int k = 0; k < 2; k++) {
-            sum += tk * constrain2(dif(ld(op[k])), ps, pss);
-            sum += tk * constrain2(dif(ld(-op[k])), ps, pss);
+            sv[4 * k] = ld(os0[k]);
+            sv[4 * k + 1] = ld(-os0[k]);
+            sv[4 * k + 2] = ld(os1[k]);
+            sv[4 * k + 3] = ld(-os1[k]);
+        }
+        auto dif = [&](s16x2 v) { return __builtin_elementwise_sub_sat(v, px); };
+        s16x2 tk = splat2(pri_tap), mx = px, mn = px;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            sum += tk * constrain2(dif(pv[2 * k]), ps, pss);
+            sum += tk * constrain2(dif(pv[2 * k + 1]), ps, pss);
+            tk = (tk & (short)3) | (short)2;
+            mn = __builtin_elementwise_min(mn, __builtin_elementwise_min(pv[2 * k], pv[2 * k + 1]));
+            mx = __builtin_elementwise_max(mx, __builtin_elementwise_max(pv[2 * k], pv[2 * k + 1]));
+            const s16x2 st = splat2(2 - k);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                sum += st * constrain2(dif(sv[4 * k + j]), ss, sss);
+                mn = __builtin_elementwise_min(mn, sv[4 * k + j]);
+                mx = __builtin_elementwise_max(mx, sv[4 * k + j]);
+            }
+        }
+        const s16x2 v = px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
+        return __builtin_elementwise_min(__builtin_elementwise_max(v, mn), mx);
+    }
+    if (pri) {
+        const s16x2 px = ld(0);
+        s16x2 pv[4];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            pv[2 * k] = ld(op[k]);
+            pv[2 * k + 1] = ld(-op[k]);
+        }
+        s16x2 tk = splat2(pri_tap);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            sum += tk * constrain2(__builtin_elementwise_sub_sat(pv[2 * k], px), ps, pss);
+            sum += tk * constrain2(__builtin_elementwise_sub_sat(pv[2 * k + 1], px), ps, pss);
             tk = (tk & (short)3) | (short)2;
         }
         return px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
     }
+    const s16x2 px = ld(0);
+    s16x2 sv[8];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        sv[4 * k] = ld(os0[k]);
+        sv[4 * k + 1] = ld(-os0[k]);
+        sv[4 * k + 2] = ld(os1[k]);
+        sv[4 * k + 3] = ld(-os1[k]);
+    }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         const s16x2 st = splat2(2 - k);
-        sum += st * constrain2(dif(ld(os0[k])), ss, sss);
-        sum += st * constrain2(dif(ld(-os0[k])), ss, sss);
-        sum += st * constrain2(dif(ld(os1[k])), ss, sss);
-        sum += st * constrain2(dif(ld(-os1[k])), ss, sss);
+#pragma unroll
+        for (int j = 0; j < 4; j++) sum += st * constrain2(__builtin_elementwise_sub_sat(sv[4 * k + j], px), ss, sss);
     }
     return px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
 }
@@ -208,13 +232,16 @@ struct CdefSmem {
     uint8_t b_state[64];   // bit0 luma filtered, bit1 chroma filtered
     uint8_t b_dir[64];
     int16_t b_ypri[64];    // adjusted luma primary strength
-    int16_t off[2][12][2]; // CDEF_DIRECTIONS as offsets in the luma / chroma tile
+    alignas(4) int16_t off[2][12][2];    // CDEF_DIRECTIONS as offsets in the luma / chroma tile
+    alignas(4) int8_t ccso_lut[3][256];  // CCSO offset per band/edge index, per plane
+    uint32_t dcost[8][64]; // direction-search cost of each direction per block
 };
 
 // CCSO offset of one sample: `lt` = staged luma at the collocated position
-// (lx, lyy in frame coordinates), neighbours clamped to the frame.
-__device__ __forceinline__ int ccso_offset(const CcsoDev &c, const int16_t *ly, int lx, int lyy, int x0, int y0, int fw,
-                                           int fh) {
+// (lx, lyy in frame coordinates), neighbours clamped to the frame; `lut` is
+// the plane's offset for each of the 256 band/edge indices, staged in LDS.
+__device__ __forceinline__ int ccso_offset(const CcsoDev &c, const int8_t *lut, const int16_t *ly, int lx, int lyy,
+                                           int x0, int y0, int fw, int fh) {
     const int cval = ly[(lyy - y0 + 2) * LY + lx - x0 + CX];
     int idx = (int)((uint32_t)cval >> c.shift);
     if (!c.bo_only) {
@@ -223,10 +250,8 @@ __device__ __forceinline__ int ccso_offset(const CcsoDev &c, const int16_t *ly, 
         uint32_t c0 = ccso_score(ly[ny0 * LY + nx0] - cval, c.quant_step, c.edge_clf);
         uint32_t c1 = ccso_score(ly[ny1 * LY + nx1] - cval, c.quant_step, c.edge_clf);
         idx |= (int)((c0 << 5) | (c1 << 3));
-        idx &= 0xff;
     }
-    const int packed = (idx >> 1) < 64 ? c.filter_off[idx >> 1] : 0;
-    return kCcsoOffset[c.scale_idx][7 & (packed >> (4 * (idx & 1)))];
+    return lut[idx & 0xff];
 }
 
 template <typename P>
@@ -240,6 +265,14 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
     const int fw = a.g.bw * 4, fh = a.g.bh * 4;
     const int cw = fw >> 1, ch = fh >> 1;
     const int bdm8 = a.g.bitdepth - 8;
+    // side info of the superblock first, so that its loads overlap the staging
+    const Rav2dAv2Filter &m = a.mask[(sby >> 2) * a.g.sb256w + (sbx >> 2)];
+    const int sb64_idx = ((sby & 3) << 2) + (sbx & 3);
+    const int cdef_idx_v = m.cdef_idx[sb64_idx];
+    const int ccso_pl_v = (m.ccso[0] != 0) | ((m.ccso[1] != 0) << 1) | ((m.ccso[2] != 0) << 2);
+    const uint64_t y_str = *(const uint64_t *)a.y_strength, uv_str = *(const uint64_t *)a.uv_strength;
+    // noskip row of the 8x8 block of lane t & 63 (block row (t >> 3) & 7)
+    const uint16_t noskip_b = a.on_skiptx ? 0xffff : m.noskip_mask[(((y0 >> 3) + ((t >> 3) & 7)) & 31)][sbx & 3];
     // ---- staging of F2 with INT16_MIN outside the frame; every load of a lane
     // is issued before its first LDS store
     {
@@ -249,14 +282,16 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
         constexpr int QY = LY / 4, QC = LC / 4;
         constexpr int NY = LYR * QY, NC = 2 * LCR * QC;
         constexpr int PY = (NY + kThreads - 1) / kThreads, PC = (NC + kThreads - 1) / kThreads;
-        const uint2 outside = make_uint2(0x80008000u, 0x80008000u);  // INT16_MIN x 4
+        constexpr uint32_t kOutside = 0x80008000u;  // INT16_MIN x 2
         uint2 vy[PY], vc[PC];
 #pragma unroll
         for (int j = 0; j < PY; j++) {
             const int i = t + j * kThreads;
             const int r = i / QY, gx = x0 - CX + (i - r * QY) * 4, gy = y0 - 2 + r;
             const uint2 v = load4(row_ptr<const P>(a.src[0], iclip(gy, 0, fh - 1)) + iclip(gx, 0, fw - 4));
-            vy[j] = (gx >= 0 && gx < fw && gy >= 0 && gy < fh) ? v : outside;
+            const bool in = gx >= 0 && gx < fw && gy >= 0 && gy < fh;
+            vy[j].x = in ? v.x : kOutside;  // per word: a vector select would go through scratch
+            vy[j].y = in ? v.y : kOutside;
         }
 #pragma unroll
         for (int j = 0; j < PC; j++) {
@@ -266,26 +301,31 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
             const int r = k / QC, gx = (x0 >> 1) - CX + (k - r * QC) * 4, gy = (y0 >> 1) - 2 + r;
             const uint2 v =
                 load4(row_ptr<const P>(a.src[1 + (pl & 1)], iclip(gy, 0, ch - 1)) + iclip(gx, 0, cw - 4));
-            vc[j] = (gx >= 0 && gx < cw && gy >= 0 && gy < ch) ? v : outside;
+            const bool in = gx >= 0 && gx < cw && gy >= 0 && gy < ch;
+            vc[j].x = in ? v.x : kOutside;
+            vc[j].y = in ? v.y : kOutside;
         }
+        // the CCSO offset tables (3 x 256 B) and the tap offsets (96 B) of the argument block
+        uint32_t lutw = 0, offw = 0;
+        if (t < 3 * 64) lutw = ((const uint32_t *)a.ccso[t >> 6].lut)[t & 63];
+        if (t < 24) offw = ((const uint32_t *)&a.off[0][0][0])[t];
 #pragma unroll
         for (int j = 0; j < PY; j++)
             if (t + j * kThreads < NY) ((uint2 *)ly)[t + j * kThreads] = vy[j];
 #pragma unroll
         for (int j = 0; j < PC; j++)
             if (t + j * kThreads < NC) ((uint2 *)&lc[0][0])[t + j * kThreads] = vc[j];
-        if (t < 48) {
-            const int tab = t / 24, d = (t >> 1) % 12, k = t & 1;
-            sm.off[tab][d][k] = (int16_t)(kCdefDyDx[d][k][0] * (tab ? LC : LY) + kCdefDyDx[d][k][1]);
-        }
+        if (t < 3 * 64) ((uint32_t *)&sm.ccso_lut[0][0])[t] = lutw;
+        if (t < 24) ((uint32_t *)&sm.off[0][0][0])[t] = offw;
     }
-    const Rav2dAv2Filter &m = a.mask[(sby >> 2) * a.g.sb256w + (sbx >> 2)];
-    const int sb64_idx = ((sby & 3) << 2) + (sbx & 3);
-    const int cdef_idx = m.cdef_idx[sb64_idx];
-    const bool active = a.cdef_on && cdef_idx >= 0 &&
-                        (a.y_strength[cdef_idx] != 0 || a.uv_strength[cdef_idx] != 0);
-    const int y_lvl = active ? a.y_strength[cdef_idx] : 0;
-    const int uv_lvl = active ? a.uv_strength[cdef_idx] : 0;
+    // uniform side info (read after the staging loads are in flight)
+    const int cdef_idx = __builtin_amdgcn_readfirstlane(cdef_idx_v);
+    const int ccso_pl = __builtin_amdgcn_readfirstlane(ccso_pl_v);
+    const int y_s = cdef_idx >= 0 ? (int)((y_str >> (8 * (cdef_idx & 7))) & 0xff) : 0;
+    const int uv_s = cdef_idx >= 0 ? (int)((uv_str >> (8 * (cdef_idx & 7))) & 0xff) : 0;
+    const bool active = a.cdef_on && cdef_idx >= 0 && (y_s != 0 || uv_s != 0);
+    const int y_lvl = active ? y_s : 0;
+    const int uv_lvl = active ? uv_s : 0;
     const int y_pri = (y_lvl >> 2) << bdm8;
     int y_sec = y_lvl & 3;
     y_sec += y_sec == 3;
@@ -296,16 +336,52 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
     uv_sec <<= bdm8;
     const int damping = a.damping + bdm8;
     __syncthreads();
+    // ---- direction search: lane t & 63 owns 8x8 block (t & 7, (t >> 3) & 7)
+    auto block_searched = [&](int b) -> bool {
+        const int bx = (x0 >> 2) + (b & 7) * 2, by = (y0 >> 2) + (b >> 3) * 2;  // 4px units
+        if (!active || !(y_pri || uv_pri) || bx >= a.g.bw || by >= a.g.bh) return false;
+        return (noskip_b & (3u << (bx & 14))) != 0;
+    };
+    {
+        const int b = t & 63, role = __builtin_amdgcn_readfirstlane(t >> 6);
+        uint32_t c_lo = 0, c_hi = 0;
+        if (block_searched(b)) {
+            const int16_t *img = &ly[((b >> 3) * 8 + 2) * LY + (b & 7) * 8 + CX];
+            if (role == 0)
+                dir_costs<0>(img, LY, bdm8, c_lo, c_hi);
+            else if (role == 1)
+                dir_costs<1>(img, LY, bdm8, c_lo, c_hi);
+            else if (role == 2)
+                dir_costs<2>(img, LY, bdm8, c_lo, c_hi);
+            else
+                dir_costs<3>(img, LY, bdm8, c_lo, c_hi);
+        }
+        const int dlo = role == 0 ? 0 : role == 1 ? 2 : role == 2 ? 1 : 3;
+        sm.dcost[dlo][b] = c_lo;
+        sm.dcost[dlo + 4][b] = c_hi;
+    }
+    __syncthreads();
     if (t < 64) {
-        int bxi = t & 7, byi = t >> 3;
-        int bx = (x0 >> 2) + bxi * 2, by = (y0 >> 2) + byi * 2;  // 4px units
+        int bx = (x0 >> 2) + (t & 7) * 2, by = (y0 >> 2) + (t >> 3) * 2;  // 4px units
         uint8_t st = 0, dir = 0;
         int16_t ypri = 0;
         if (active && bx < a.g.bw && by < a.g.bh) {
-            uint16_t noskip = a.on_skiptx ? 0xffff : m.noskip_mask[(by & 0x3e) >> 1][sbx & 3];
-            if (noskip & (3u << (bx & 14))) {
+            if (noskip_b & (3u << (bx & 14))) {
                 uint32_t var = 0;
-                int d = (y_pri || uv_pri) ? find_dir(&ly[(byi * 8 + 2) * LY + bxi * 8 + CX], LY, &var, bdm8) : 0;
+                int d = 0;
+                if (y_pri || uv_pri) {
+                    // first maximum in direction order, as cdef_find_dir_bd
+                    uint32_t best = sm.dcost[0][t];
+#pragma unroll
+                    for (int n = 1; n < 8; n++) {
+                        const uint32_t c = sm.dcost[n][t];
+                        if (c > best) {
+                            best = c;
+                            d = n;
+                        }
+                    }
+                    var = (best - sm.dcost[d ^ 4][t]) >> 10;
+                }
                 if (y_pri) {
                     int adj = adjust_strength(y_pri, var);
                     if (adj || y_sec) st |= 1;
@@ -341,7 +417,7 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
             os0[k] = sm.off[0][dir + 4][k];
             os1[k] = sm.off[0][dir][k];
         }
-        const bool cc0 = a.ccso_on && m.ccso[0];
+        const bool cc0 = a.ccso_on && (ccso_pl & 1);
         if (gx < fw) {
 #pragma unroll 2
             for (int yy = by; yy < by + 8; yy++) {
@@ -357,8 +433,9 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
                     v1 = (int)(P)(uint16_t)r.y;
                 }
                 if (cc0) {
-                    v0 = iclip(v0 + ccso_offset(a.ccso[0], ly, gx, gy, x0, y0, fw, fh), 0, a.g.bdmax);
-                    v1 = iclip(v1 + ccso_offset(a.ccso[0], ly, gx + 1, gy, x0, y0, fw, fh), 0, a.g.bdmax);
+                    const int8_t *lut = sm.ccso_lut[0];
+                    v0 = iclip(v0 + ccso_offset(a.ccso[0], lut, ly, gx, gy, x0, y0, fw, fh), 0, a.g.bdmax);
+                    v1 = iclip(v1 + ccso_offset(a.ccso[0], lut, ly, gx + 1, gy, x0, y0, fw, fh), 0, a.g.bdmax);
                 }
                 store2(row_ptr<P>(a.dst[0], gy) + gx, v0, v1);
             }
@@ -367,7 +444,7 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
     // ---- chroma: lane t filters the column pair 2 (t & 15), +1 of the 4 rows of
     // block row (t >> 4) & 7 of plane t >> 7 (one 4x4 block per lane)
     {
-        const int pl = t >> 7, x = 2 * (t & 15), by = 4 * ((t >> 4) & 7);
+        const int pl = __builtin_amdgcn_readfirstlane(t >> 7), x = 2 * (t & 15), by = 4 * ((t >> 4) & 7);
         const int gx = (x0 >> 1) + x;
         const int bi = (by >> 2) * 8 + (x >> 2);
         const bool on = b_state[bi] & 2;
@@ -382,7 +459,7 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
             os0[k] = sm.off[1][dir + 4][k];
             os1[k] = sm.off[1][dir][k];
         }
-        const bool cc = a.ccso_on && m.ccso[1 + pl];
+        const bool cc = a.ccso_on && ((ccso_pl >> (1 + pl)) & 1);
         if (gx < cw) {
 #pragma unroll 2
             for (int yy = by; yy < by + 4; yy++) {
@@ -396,10 +473,10 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
                     v1 = (int)(P)(uint16_t)r.y;
                 }
                 if (cc) {
-                    v0 = iclip(v0 + ccso_offset(a.ccso[1 + pl], ly, gx << 1, gy << 1, x0, y0, fw, fh), 0,
-                               a.g.bdmax);
-                    v1 = iclip(v1 + ccso_offset(a.ccso[1 + pl], ly, (gx + 1) << 1, gy << 1, x0, y0, fw, fh), 0,
-                               a.g.bdmax);
+                    const CcsoDev &c = a.ccso[1 + pl];
+                    const int8_t *lut = sm.ccso_lut[1 + pl];
+                    v0 = iclip(v0 + ccso_offset(c, lut, ly, gx << 1, gy << 1, x0, y0, fw, fh), 0, a.g.bdmax);
+                    v1 = iclip(v1 + ccso_offset(c, lut, ly, (gx + 1) << 1, gy << 1, x0, y0, fw, fh), 0, a.g.bdmax);
                 }
                 store2(row_ptr<P>(a.dst[1 + pl], gy) + gx, v0, v1);
             }
@@ -430,24 +507,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
 int cdef_items(const CdefArgs &a) { return ((a.g.bw * 4 + 63) >> 6) * ((a.g.bh * 4 + 63) >> 6); }
 
 // Per device: the direction offsets as (dy, dx) and the CCSO offset table.
-void cdef_init_consts() {
-    static bool done[64] = {};
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    if (dev < 0 || dev >= 64 || done[dev]) return;
-    {
-        int8_t dydx[12][2][2];
-        for (int d = 0; d < 12; d++)
-            for (int k = 0; k < 2; k++) {
-                int o = R2T_CDEF_DIRECTIONS[d][k];  // dy * 12 + dx with |dx| <= 2
-                int dx = ((o + 2) % 12 + 12) % 12 - 2;
-                dydx[d][k][0] = (int8_t)((o - dx) / 12);
-                dydx[d][k][1] = (int8_t)dx;
-            }
-        (void)hipMemcpyToSymbol(HIP_SYMBOL(kCdefDyDx), dydx, sizeof(dydx));
-        (void)hipMemcpyToSymbol(HIP_SYMBOL(kCcsoOffset), R2T_CCSO_OFFSET, sizeof(R2T_CCSO_OFFSET));
-        done[dev] = true;
+void cdef_ccso_lut(CcsoDev &d, const uint8_t filter_off[64], int scale_idx) {
+    // ccso_add (ccso.rs:215-254): index -> packed nibble of filter_off -> CCSO_OFFSET
+    for (int idx = 0; idx < 256; idx++) {
+        const int packed = (idx >> 1) < 64 ? filter_off[idx >> 1] : 0;
+        d.lut[idx] = R2T_CCSO_OFFSET[scale_idx & 3][7 & (packed >> (4 * (idx & 1)))];
     }
+}
+
+void cdef_tap_offsets(CdefArgs &a) {
+    for (int d = 0; d < 12; d++)
+        for (int k = 0; k < 2; k++) {
+            const int o = R2T_CDEF_DIRECTIONS[d][k];  // dy * 12 + dx with |dx| <= 2
+            const int dx = ((o + 2) % 12 + 12) % 12 - 2, dy = (o - dx) / 12;
+            a.off[0][d][k] = (int16_t)(dy * LY + dx);
+            a.off[1][d][k] = (int16_t)(dy * LC + dx);
+        }
 }
 
 void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st) {
@@ -467,7 +542,6 @@ void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, in
 }
 
 void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s) {
-    cdef_init_consts();
     dim3 grid((a.g.bw * 4 + 63) / 64, (a.g.bh * 4 + 63) / 64);
     if (bitdepth > 8)
         hipLaunchKernelGGL(cdef_kernel<uint16_t>, grid, dim3(kThreads), 0, s, a);

@@ -50,6 +50,15 @@ template <typename P> __device__ __forceinline__ void store4(P *p, uint2 v) {
         *(uint2 *)p = v;
 }
 
+// One 16-bit LDS read that the compiler keeps separate from its neighbours (a
+// relaxed workgroup-scope atomic load: not merged into a possibly unaligned
+// wider access, and still a plain ds_read with no waits of its own).  `p`
+// must point into LDS.
+__device__ __forceinline__ int16_t lds_load_i16(const int16_t *p) {
+    return __hip_atomic_load((__attribute__((address_space(3))) int16_t *)p, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Two adjacent samples (2-sample aligned) with one store.
 template <typename P> __device__ __forceinline__ void store2(P *p, int a, int b) {
     if constexpr (sizeof(P) == 1)

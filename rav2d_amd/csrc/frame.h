@@ -54,6 +54,11 @@ inline hipEvent_t take_event(Rav2dGpuCtx *c) {
 }
 
 // Brackets one pass launch with events when profiling is on.
+// Events that only order work between streams of one device: no timing and no
+// system-scope fence (which would write back and invalidate the L2 caches at
+// every fork and join); the kernels' own device-scope fences order the data.
+constexpr unsigned kStreamSyncEventFlags = hipEventDisableTiming | hipEventDisableSystemFence;
+
 struct PassTimer {
     Rav2dGpuCtx *c;
     int pass;

@@ -34,8 +34,7 @@ struct CcsoDev {
     int32_t quant_step;  // CCSO_QUANT_SZ[scale_idx][quant_idx]
     uint32_t edge_clf;
     int32_t bo_only;
-    int32_t scale_idx;
-    uint8_t filter_off[64];
+    alignas(4) int8_t lut[256];  // offset per band/edge index: CCSO_OFFSET[scale_idx] of filter_off's nibble
 };
 
 struct CdefArgs {
@@ -43,14 +42,19 @@ struct CdefArgs {
     DevGeom g;
     const Rav2dAv2Filter *mask;
     int32_t cdef_on, ccso_on, on_skiptx, damping;
-    uint8_t y_strength[8], uv_strength[8];
+    alignas(8) uint8_t y_strength[8];
+    alignas(8) uint8_t uv_strength[8];
     CcsoDev ccso[3];
+    alignas(4) int16_t off[2][12][2];  // CDEF_DIRECTIONS as offsets in the kernel's luma / chroma tiles
 };
 
 void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s);
 void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st);
 int cdef_items(const CdefArgs &a);  // 64x64 superblocks of the frame
-void cdef_init_consts();            // constant tables of the current device
+// Host side of the argument block: CCSO's offset table of one plane (from the
+// plane's packed filter_off nibbles and scale_idx), and the direction offsets.
+void cdef_ccso_lut(CcsoDev &d, const uint8_t filter_off[64], int scale_idx);
+void cdef_tap_offsets(CdefArgs &a);
 
 // Loop restoration: one work item per (plane, stripe, 64-column unit), built
 // on the host from the stripe/unit walk of lr_sbrow_apply / lr_sbrow_plane /

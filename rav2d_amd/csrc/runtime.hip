@@ -539,9 +539,9 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
             d.quant_step = R2T_CCSO_QUANT_SZ[c.scale_idx & 3][c.quant_idx & 3];
             d.edge_clf = c.edge_clf;
             d.bo_only = c.bo_only;
-            d.scale_idx = c.scale_idx & 3;
-            memcpy(d.filter_off, c.filter_off, 64);
+            cdef_ccso_lut(d, c.filter_off, c.scale_idx & 3);
         }
+        cdef_tap_offsets(a);
         for (int pl = 0; pl < 3; pl++) {
             int out = next_tmp(cur[pl]);
             a.src[pl] = plane_of(f, cur[pl], pl);
@@ -774,8 +774,8 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
         set_last_error("hipMalloc failed");
         return RAV2D_ENOMEM;
     }
-    if (hipEventCreateWithFlags(&f->ev_setup, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&f->ev_free, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&f->ev_setup, kStreamSyncEventFlags) != hipSuccess ||
+        hipEventCreateWithFlags(&f->ev_free, kStreamSyncEventFlags) != hipSuccess) {
         set_last_error("event creation failed");
         return RAV2D_EDEVICE;
     }

@@ -354,8 +354,8 @@ int rav2d_gpu_server_create(Rav2dGpuCtx *ctx, const Rav2dServeConfig *cfg, Rav2d
     for (Slot &sl : sv->slots) {
         ok = ok && hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking) == hipSuccess;
         ok = ok && hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming) == hipSuccess;
-        ok = ok && hipEventCreateWithFlags(&sl.ev_fork, hipEventDisableTiming) == hipSuccess;
-        ok = ok && hipEventCreateWithFlags(&sl.ev_join, hipEventDisableTiming) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&sl.ev_fork, kStreamSyncEventFlags) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&sl.ev_join, kStreamSyncEventFlags) == hipSuccess;
     }
     if (!ok) {
         set_last_error("server: stream/event creation failed");
