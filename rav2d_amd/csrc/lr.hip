@@ -95,35 +95,42 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     const int t = threadIdx.x;
     if (t < sizeof(LrItem) / 4) ((uint32_t *)&it)[t] = ((const uint32_t *)&a.items[item])[t];
     __syncthreads();
-    const int pl = it.plane, x0 = it.x0, w = it.w, y0 = it.y0, h = it.h;
-    const uint8_t edges = it.edges;
+    // item fields are uniform over the workgroup
+    const int pl = __builtin_amdgcn_readfirstlane(it.plane), x0 = __builtin_amdgcn_readfirstlane(it.x0),
+              w = __builtin_amdgcn_readfirstlane(it.w), y0 = __builtin_amdgcn_readfirstlane(it.y0),
+              h = __builtin_amdgcn_readfirstlane(it.h);
+    const int edges = __builtin_amdgcn_readfirstlane(it.edges);
+    const int kind = __builtin_amdgcn_readfirstlane(it.kind);
+    const bool gdf = __builtin_amdgcn_readfirstlane(it.gdf_on) != 0;
     auto &f3 = a.f3[pl];
     auto &f2 = a.f2[pl];
     auto &dst = a.dst[pl];
     const int bd = a.g.bitdepth, bdm8 = bd - 8, bdmax = a.g.bdmax;
-    const int kind = it.kind;
-    const bool gdf = it.gdf_on;
+    // every buffer of a plane has the same row stride (< 4 GiB)
+    const uint32_t stride = (uint32_t)f3.stride;
+    using Q = typename std::conditional<sizeof(P) == 1, uint32_t, uint2>::type;  // 4 samples
     if (kind == LR_KIND_NONE && !gdf) {
-        // plane copy, 4 samples per access (x0 and w are multiples of 4)
-        using Q = typename std::conditional<sizeof(P) == 1, uint32_t, uint2>::type;
-        constexpr int CP = 16;
+        // plane copy, 4 samples per access (x0 and w are multiples of 4): units of
+        // planes LR leaves alone, or blocks of up to 256 x 64 of planes it does
+        // not touch; per round, loads from clamped indices first, then the stores
+        constexpr int CP = 8;
         const int n4 = w >> 2, n = h * n4;
+        const uint8_t *src = f3.ptr + (uint64_t)(uint32_t)y0 * stride + x0 * sizeof(P);
+        uint8_t *dp = dst.ptr + (uint64_t)(uint32_t)y0 * stride + x0 * sizeof(P);
         for (int b = 0; b < n; b += CP * kThreads) {
             Q v[CP];
 #pragma unroll
             for (int j = 0; j < CP; j++) {
-                const int i = b + t + j * kThreads;
-                if (i < n) {
-                    const int r = i / n4, c = i - r * n4;
-                    v[j] = ((const Q *)(row_ptr<const P>(f3, y0 + r) + x0))[c];
-                }
+                const int i = dmin(b + t + j * kThreads, n - 1);
+                const int r = i / n4, c = i - r * n4;
+                v[j] = *(const Q *)(src + (uint64_t)(uint32_t)r * stride + c * sizeof(Q));
             }
 #pragma unroll
             for (int j = 0; j < CP; j++) {
                 const int i = b + t + j * kThreads;
                 if (i < n) {
                     const int r = i / n4, c = i - r * n4;
-                    ((Q *)(row_ptr<P>(dst, y0 + r) + x0))[c] = v[j];
+                    *(Q *)(dp + (uint64_t)(uint32_t)r * stride + c * sizeof(Q)) = v[j];
                 }
             }
         }
@@ -135,24 +142,22 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     // samples read at once (x0 and plane widths are multiples of 4).  Groups
     // are read from clamped, always valid addresses; the columns the
     // reference replicates at the frame's left/right edge are patched after.
-    using Q = typename std::conditional<sizeof(P) == 1, uint32_t, uint2>::type;
+    // Rows: inside the stripe from F3; across a stripe boundary the two
+    // deblocked rows beyond it (F2), further rows repeating the second; at the
+    // frame's top/bottom the stripe's first/last row repeats.
     constexpr int PERQ = (VH * VQ + kThreads - 1) / kThreads;
     const int nvq = (h + 2 * E) * VQ;
     const int pw = f3.w;
+    const bool top = edges & LRE_TOP, bottom = edges & LRE_BOTTOM;
     Q qv[PERQ];
 #pragma unroll
     for (int j = 0; j < PERQ; j++) {
         const int i = dmin(t + j * kThreads, nvq - 1);
         const int r = i / VQ - E, c0 = (i % VQ) * 4 - VC;
-        const P *row;
-        if (r >= 0 && r < h)
-            row = row_ptr<const P>(f3, y0 + r);
-        else if (r < 0)
-            row = (edges & LRE_TOP) ? row_ptr<const P>(f2, y0 + (r == -1 ? -1 : -2)) : row_ptr<const P>(f3, y0);
-        else
-            row = (edges & LRE_BOTTOM) ? row_ptr<const P>(f2, y0 + h + (r == h ? 0 : 1))
-                                       : row_ptr<const P>(f3, y0 + h - 1);
-        qv[j] = *(const Q *)(row + iclip(x0 + c0, 0, pw - 4));
+        const bool from2 = (r < 0 && top) || (r >= h && bottom);
+        const int yy = y0 + iclip(r, from2 ? -2 : 0, from2 ? h + 1 : h - 1);
+        const uint8_t *row = (from2 ? f2.ptr : f3.ptr) + (uint64_t)(uint32_t)yy * stride;
+        qv[j] = *(const Q *)(row + iclip(x0 + c0, 0, pw - 4) * sizeof(P));
     }
     // tables of this item's filter kinds, 16 (or 8) bytes per lane
     uint4 pcv = {0, 0, 0, 0}, alv = {0, 0, 0, 0}, wv = {0, 0, 0, 0}, erv = {0, 0, 0, 0};
@@ -170,45 +175,34 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     constexpr int PERL = ((32 + 4) * LS + kThreads - 1) / kThreads;
     uint16_t lvals[PERL];
     if (uv_single) {
-        // downsampled luma rows of the chroma stripe (cfl_ds_filter_index)
+        // downsampled luma rows of the chroma stripe (cfl_ds_filter_index):
+        // luma rows 2y, 2y + 1 inside the stripe (F3); across a stripe boundary
+        // the deblocked rows (F2): above -2 -> (2y0-2, 2y0-2), -1 -> (2y0-2, 2y0-1),
+        // below h -> (2(y0+h), +1), h+1 -> (2(y0+h)+1, 2(y0+h)+1)
         auto &l3 = a.f3[0];
         auto &l2 = a.f2[0];
+        const uint32_t lstride = (uint32_t)l3.stride;
         const int nl = (h + 4) * LS;
+        // cfl_ds 0: 2x2 average, 1: vertical pair average, 2: top-left sample
+        const int ds = a.cfl_ds, mb = ds <= 1 ? -1 : 0, m1 = ds == 0 ? -1 : 0, sh = ds == 0 ? 2 : (ds == 1 ? 1 : 0);
+        using Q2 = typename std::conditional<sizeof(P) == 1, uint16_t, uint32_t>::type;  // 2 samples
 #pragma unroll
         for (int j = 0; j < PERL; j++) {
-            const int i = t + j * kThreads;
-            lvals[j] = 0;
+            const int i0 = t + j * kThreads, i = dmin(i0, nl - 1);
             const int r = i / LS - 2, c = i % LS - 2;
-            if (i < nl && c < w + 2) {
-                int xs = (c < 0 && !(edges & LRE_LEFT)) ? 0 : ((c >= w && !(edges & LRE_RIGHT)) ? w - 1 : c);
-                int lx = 2 * (x0 + xs);
-                const P *ra, *rb;
-                int rr = r;
-                if (rr >= h && !(edges & LRE_BOTTOM)) rr = h - 1;
-                if (rr < 0 && !(edges & LRE_TOP)) rr = 0;
-                if (rr >= 0 && rr < h) {
-                    ra = row_ptr<const P>(l3, 2 * (y0 + rr));
-                    rb = row_ptr<const P>(l3, 2 * (y0 + rr) + 1);
-                } else if (rr == -2) {
-                    ra = rb = row_ptr<const P>(l2, 2 * y0 - 2);
-                } else if (rr == -1) {
-                    ra = row_ptr<const P>(l2, 2 * y0 - 2);
-                    rb = row_ptr<const P>(l2, 2 * y0 - 1);
-                } else if (rr == h) {
-                    ra = row_ptr<const P>(l2, 2 * (y0 + h));
-                    rb = row_ptr<const P>(l2, 2 * (y0 + h) + 1);
-                } else {
-                    ra = rb = row_ptr<const P>(l2, 2 * (y0 + h) + 1);
-                }
-                int v;
-                if (a.cfl_ds == 0)
-                    v = ((int)ra[lx] + rb[lx] + ra[lx + 1] + rb[lx + 1]) >> 2;
-                else if (a.cfl_ds == 1)
-                    v = ((int)ra[lx] + rb[lx]) >> 1;
-                else
-                    v = ra[lx];
-                lvals[j] = (uint16_t)v;
-            }
+            const int xs = (c < 0 && !(edges & LRE_LEFT)) ? 0 : ((c >= w && !(edges & LRE_RIGHT)) ? w - 1 : dmin(c, w + 1));
+            const int rr = iclip(r, top ? -2 : 0, bottom ? h + 1 : h - 1);
+            const bool from2 = rr < 0 || rr >= h, single = rr == -2 || rr == h + 1;
+            const int ya = 2 * (y0 + rr) + (rr == -2 ? 2 : (rr == h + 1 ? -1 : 0));
+            const uint8_t *ra = (from2 ? l2.ptr : l3.ptr) + (uint64_t)(uint32_t)ya * lstride;
+            const uint8_t *rb = single ? ra : ra + lstride;
+            const int lx = 2 * (x0 + xs);
+            const Q2 qa = *(const Q2 *)(ra + lx * sizeof(P)), qb = *(const Q2 *)(rb + lx * sizeof(P));
+            constexpr int SB = 8 * sizeof(P);
+            const int a0 = (int)(qa & ((1u << SB) - 1)), a1 = (int)(qa >> SB);
+            const int b0 = (int)(qb & ((1u << SB) - 1)), b1 = (int)(qb >> SB);
+            const int v = (a0 + (b0 & mb) + ((a1 + b1) & m1)) >> sh;
+            lvals[j] = (i0 < nl && c < w + 2) ? (uint16_t)v : 0;
         }
     }
 #pragma unroll
