@@ -74,14 +74,18 @@ constexpr int CS = 33;      // stride of the GDF gradient cells
 
 struct LrSmem {
     alignas(16) uint16_t V[VH * VW];  // virtual image, rows [-E, 64 + E), columns [-VC, 64 + VC)
-    uint16_t Ld[(32 + 4) * LS];      // downsampled luma of a chroma stripe
-    uint16_t cells[33 * CS * 4];     // GDF gradient cells
+    union {
+        uint16_t Ld[(32 + 4) * LS];  // downsampled luma of a chroma stripe (chroma items)
+        struct {
+            uint16_t cells[33 * CS * 4];     // GDF gradient cells (luma items)
+            alignas(16) int8_t s_err[4096];  // GDF error table slice of (ref, qp)
+        };
+    };
     uint8_t cls[16 * 16];            // class of each 4x4 block
     alignas(16) uint16_t s_alpha[88];
     alignas(16) int16_t s_weight[264];
     alignas(16) int16_t s_pc[64 * 13];  // PC-Wiener filters of the frame's wiener_idx
     int8_t s_mf[16 * 16];               // multi-class frame filters of the plane
-    alignas(16) int8_t s_err[4096];     // GDF error table slice of (ref, qp)
     uint32_t apk[4 * 9];                // GDF alpha of taps (2i, 2i+1) per class, packed 16-bit pairs
     uint32_t wpk[4 * 27];               // GDF weights likewise: [class][weight set][pair]
     LrItem it;
@@ -101,7 +105,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
               h = __builtin_amdgcn_readfirstlane(it.h);
     const int edges = __builtin_amdgcn_readfirstlane(it.edges);
     const int kind = __builtin_amdgcn_readfirstlane(it.kind);
-    const bool gdf = __builtin_amdgcn_readfirstlane(it.gdf_on) != 0;
+    const bool gdf = __builtin_amdgcn_readfirstlane(it.gdf_on) != 0 && pl == 0;  // GDF is luma-only
     auto &f3 = a.f3[pl];
     auto &f2 = a.f2[pl];
     auto &dst = a.dst[pl];
@@ -478,10 +482,11 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
 #undef VAT
 }
 
-// at least 4 waves per SIMD: the body is VALU/LDS-latency bound, so the
-// register budget is capped (128 VGPRs) in exchange for occupancy
+// at least 5 waves per SIMD: the body is VALU/LDS-latency bound, so the
+// register budget is capped (96 VGPRs) in exchange for occupancy (LDS allows
+// 5 workgroups per CU)
 template <typename P>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void lr_kernel(const LrArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 8))) void lr_kernel(const LrArgs a) {
     __shared__ LrSmem sm;
     lr_body<P>(a, blockIdx.x, sm);
 }
@@ -489,7 +494,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
 // Batched launch: every item (restoration units and plane copies) of every
 // frame of one bit depth.
 template <typename P, bool kLoop>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void lr_batch(const LrArgs *__restrict__ args,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 8))) void lr_batch(const LrArgs *__restrict__ args,
                                                                                       BatchSched s) {
     __shared__ LrSmem sm;
     for_each_item<kLoop>(s, [&](int f, int t) {
