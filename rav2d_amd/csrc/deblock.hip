@@ -23,16 +23,15 @@ __constant__ int8_t kQFirst[3] = {45, 40, 32};
 __constant__ int8_t kQThreshMults[8] = {32, 25, 19, 19, 0, 18, 0, 17};
 __constant__ int8_t kWMult[8] = {85, 51, 37, 28, 0, 20, 0, 15};
 
-// filter_choice_bd: `s` = line 0, `t` = line 3 of the segment, `st` steps
-// across the edge (positive side = +st).
-__device__ __forceinline__ int filter_choice(const uint16_t *s, const uint16_t *t, int st, int mwn, int mwp,
-                                             uint32_t q_thr, uint32_t side_thr) {
-    auto A = [&](const uint16_t *b, int k) -> int { return (int)b[k * st]; };
+// filter_choice_bd on two lines of a segment held in registers: s[k], t[k] =
+// line 0 / line 3 sample at offset k - 8 across the edge (positive side k >= 8).
+__device__ __forceinline__ int filter_choice(const int (&s)[16], const int (&t)[16], int mwn, int mwp, uint32_t q_thr,
+                                             uint32_t side_thr) {
     uint32_t sd[4];
 #pragma unroll
     for (int d = -2; d < 2; d++) {
-        uint32_t ds = (uint32_t)abs(A(s, d - 1) - 2 * A(s, d) + A(s, d + 1));
-        uint32_t dt = (uint32_t)abs(A(t, d - 1) - 2 * A(t, d) + A(t, d + 1));
+        uint32_t ds = (uint32_t)abs(s[d + 7] - 2 * s[d + 8] + s[d + 9]);
+        uint32_t dt = (uint32_t)abs(t[d + 7] - 2 * t[d + 8] + t[d + 9]);
         sd[d + 2] = (ds + dt + 1) >> 1;
     }
     uint32_t high = dmax(sd[0], sd[3]);
@@ -45,30 +44,32 @@ __device__ __forceinline__ int filter_choice(const uint16_t *s, const uint16_t *
     if (transition > q_thr * 3) return 2;
     uint32_t end_thr = (side_thr * 3) >> 4;
     if (mwn >= 3) {
-        uint32_t ds = (uint32_t)abs(A(s, -1) - A(s, -4) - 3 * (A(s, -1) - A(s, -2)));
-        uint32_t dt = (uint32_t)abs(A(t, -1) - A(t, -4) - 3 * (A(t, -1) - A(t, -2)));
+        uint32_t ds = (uint32_t)abs(s[7] - s[4] - 3 * (s[7] - s[6]));
+        uint32_t dt = (uint32_t)abs(t[7] - t[4] - 3 * (t[7] - t[6]));
         if (((ds + dt + 1) >> 1) > end_thr) return 2;
     }
     {
-        uint32_t ds = (uint32_t)abs(A(s, 0) - A(s, 3) - 3 * (A(s, 0) - A(s, 1)));
-        uint32_t dt = (uint32_t)abs(A(t, 0) - A(t, 3) - 3 * (A(t, 0) - A(t, 1)));
+        uint32_t ds = (uint32_t)abs(s[8] - s[11] - 3 * (s[8] - s[9]));
+        uint32_t dt = (uint32_t)abs(t[8] - t[11] - 3 * (t[8] - t[9]));
         if (((ds + dt + 1) >> 1) > end_thr) return 2;
     }
     if (mwp == 3) return 3;
     transition <<= 4;
     int prev = 3;
-    for (int dist = 4; dist <= mwp; dist += 2) {
+#pragma unroll
+    for (int dist = 4; dist <= 8; dist += 2) {  // widths above 3 are at most 8
+        if (dist > mwp) break;
         uint32_t q4 = q_thr * (uint32_t)kQFirst[(dist - 4) >> 1];
         uint32_t e4 = (side_thr * (uint32_t)dist) >> 4;
         if (transition > q4) return prev;
-        int d2 = dmin(7, dist);
+        const int d2 = dist < 7 ? dist : 7;
         if (mwn >= d2) {
-            uint32_t ds = (uint32_t)abs(A(s, -1) - A(s, -d2 - 1) - d2 * (A(s, -1) - A(s, -2)));
-            uint32_t dt = (uint32_t)abs(A(t, -1) - A(t, -d2 - 1) - d2 * (A(t, -1) - A(t, -2)));
+            uint32_t ds = (uint32_t)abs(s[7] - s[7 - d2] - d2 * (s[7] - s[6]));
+            uint32_t dt = (uint32_t)abs(t[7] - t[7 - d2] - d2 * (t[7] - t[6]));
             if (((ds + dt + 1) >> 1) > e4) return prev;
         }
-        uint32_t ds = (uint32_t)abs(A(s, 0) - A(s, d2) - d2 * (A(s, 0) - A(s, 1)));
-        uint32_t dt = (uint32_t)abs(A(t, 0) - A(t, d2) - d2 * (A(t, 0) - A(t, 1)));
+        uint32_t ds = (uint32_t)abs(s[8] - s[8 + d2] - d2 * (s[8] - s[9]));
+        uint32_t dt = (uint32_t)abs(t[8] - t[8 + d2] - d2 * (t[8] - t[9]));
         if (((ds + dt + 1) >> 1) > e4) return prev;
         prev = dist;
     }
@@ -76,31 +77,42 @@ __device__ __forceinline__ int filter_choice(const uint16_t *s, const uint16_t *
 }
 
 // deblock_bd on an LDS segment: `p` = first sample on the positive side of
-// line 0, `sa` steps along the edge, `sb` across it.
-__device__ __forceinline__ void filter_edge(uint16_t *p, int sa, int sb, int bdmax, uint32_t q_thr,
-                                            uint32_t side_thr, int mwp, int mwn, bool pos_ll, bool neg_ll) {
-    int width = filter_choice(p, p + 3 * sa, sb, mwn, mwp, q_thr, side_thr);
-    int wn = dmin(width, mwn), wp = width;
+// line 0, SA steps along the edge, SB across it.  Each line's 16 samples
+// (-8..7 across the edge; the filters read and write at most 8 per side) are
+// read before any is written: the lines of a segment, and the segments of
+// one pass, touch disjoint samples.
+template <int SA, int SB>
+__device__ __forceinline__ void filter_edge(uint16_t *p, int bdmax, uint32_t q_thr, uint32_t side_thr, int mwp,
+                                            int mwn, bool pos_ll, bool neg_ll) {
+    int l0[16], l3[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        l0[k] = p[(k - 8) * SB];
+        l3[k] = p[3 * SA + (k - 8) * SB];
+    }
+    const int width = filter_choice(l0, l3, mwn, mwp, q_thr, side_thr);
+    const int wn = dmin(width, mwn), wp = width;
     if (wp < 1) return;
-    int clampv = (int)q_thr * kQThreshMults[width - 1];
-    int mneg = kWMult[wn - 1], mpos = kWMult[wp - 1];
+    const int clampv = (int)q_thr * kQThreshMults[width - 1];
+    const int mneg = kWMult[wn - 1], mpos = kWMult[wp - 1];
+#pragma unroll
     for (int i = 0; i < 4; i++) {
-        uint16_t *d = p + i * sa;
-        int d0 = d[0], dm1 = d[-sb], dp1 = d[sb], dm2 = d[-2 * sb];
-        int delta = iclip(4 * (3 * (d0 - dm1) - (dp1 - dm2)), -clampv, clampv);
+        int L[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) L[k] = i == 0 ? l0[k] : (i == 3 ? l3[k] : (int)p[i * SA + (k - 8) * SB]);
+        uint16_t *d = p + i * SA;
+        const int delta = iclip(4 * (3 * (L[8] - L[7]) - (L[9] - L[6])), -clampv, clampv);
         if (!neg_ll) {
-            int dn = delta * mneg;
-            for (int j = 0; j < wn; j++) {
-                int v = d[(-j - 1) * sb] + ((dn * (wn - j) + 1024) >> 11);
-                d[(-j - 1) * sb] = (uint16_t)iclip(v, 0, bdmax);
-            }
+            const int dn = delta * mneg;
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (j < wn) d[(-j - 1) * SB] = (uint16_t)iclip(L[7 - j] + ((dn * (wn - j) + 1024) >> 11), 0, bdmax);
         }
         if (!pos_ll) {
-            int dp = delta * mpos;
-            for (int j = 0; j < wp; j++) {
-                int v = d[j * sb] - ((dp * (wp - j) + 1024) >> 11);
-                d[j * sb] = (uint16_t)iclip(v, 0, bdmax);
-            }
+            const int dp = delta * mpos;
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (j < wp) d[j * SB] = (uint16_t)iclip(L[8 + j] - ((dp * (wp - j) + 1024) >> 11), 0, bdmax);
         }
     }
 }
@@ -218,7 +230,7 @@ __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id
     for (int j = 0; j < PV; j++) ((uint2 *)&tile[0][0])[t + j * kThreads] = v[j];
     __syncthreads();
     if (e.on)
-        filter_edge(&tile[rs * 4][ex - x0 + kHalo], LW, 1, a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
+        filter_edge<LW, 1>(&tile[rs * 4][ex - x0 + kHalo], a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kTH * kTW / 4 / kThreads; j++) {
@@ -260,7 +272,7 @@ __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id
     for (int j = 0; j < PV; j++) ((uint2 *)&tile[0][0])[t + j * kThreads] = v[j];
     __syncthreads();
     if (e.on)
-        filter_edge(&tile[ey - y0 + kHalo][cs * 4], 1, kTW, a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
+        filter_edge<1, kTW>(&tile[ey - y0 + kHalo][cs * 4], a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kTH * kTW / 4 / kThreads; j++) {
