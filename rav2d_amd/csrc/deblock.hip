@@ -17,11 +17,15 @@
 
 namespace r2g {
 
-__constant__ int8_t kMaxWidthY[4] = {1, 3, 6, 8};
-__constant__ int8_t kMaxWidthUV[3] = {1, 3, 4};
-__constant__ int8_t kQFirst[3] = {45, 40, 32};
-__constant__ int8_t kQThreshMults[8] = {32, 25, 19, 19, 0, 18, 0, 17};
-__constant__ int8_t kWMult[8] = {85, 51, 37, 28, 0, 20, 0, 15};
+// Small tables indexed by per-lane values, held as packed byte constants (a
+// table in memory would cost a dependent load per lookup):
+// MAX_WIDTH_Y {1, 3, 6, 8}, MAX_WIDTH_UV {1, 3, 4}, Q_THRESH_MULTS
+// {32, 25, 19, 19, 0, 18, 0, 17}, W_MULT {85, 51, 37, 28, 0, 20, 0, 15}.
+constexpr uint32_t kMaxWidthY = 0x08060301u, kMaxWidthUV = 0x040301u;
+constexpr uint64_t kQThreshMults = 0x1100120013131920ull, kWMult = 0x0f0014001c253355ull;
+constexpr int kQFirst[3] = {45, 40, 32};
+
+__device__ __forceinline__ int byte_at(uint64_t packed, int i) { return (int)((packed >> (8 * i)) & 0xff); }
 
 // filter_choice_bd on two lines of a segment held in registers: s[k], t[k] =
 // line 0 / line 3 sample at offset k - 8 across the edge (positive side k >= 8).
@@ -93,8 +97,8 @@ __device__ __forceinline__ void filter_edge(uint16_t *p, int bdmax, uint32_t q_t
     const int width = filter_choice(l0, l3, mwn, mwp, q_thr, side_thr);
     const int wn = dmin(width, mwn), wp = width;
     if (wp < 1) return;
-    const int clampv = (int)q_thr * kQThreshMults[width - 1];
-    const int mneg = kWMult[wn - 1], mpos = kWMult[wp - 1];
+    const int clampv = (int)q_thr * byte_at(kQThreshMults, width - 1);
+    const int mneg = byte_at(kWMult, wn - 1), mpos = byte_at(kWMult, wp - 1);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         int L[16];
@@ -134,50 +138,33 @@ __device__ __forceinline__ int qidx_at(const Rav2dAv2Filter *m, int sb256w, int 
 struct EdgeDesc {
     bool on;
     int mwp, mwn;
-    uint32_t q, side;
+    int subpu, qc, qp;  // threshold shift and the qidx of this block and of its neighbour
     bool pos_ll, neg_ll;
 };
 
 // Edge of the given pass (vert = vertical edges, cols pass) at the plane's 4px
-// unit (x4, y4); pl selects luma or chroma indexing.
-__device__ EdgeDesc edge_desc(const DeblockArgs &a, int pl, bool vert, int x4, int y4) {
+// unit (x4, y4); pl selects luma or chroma indexing.  Every side-info load is
+// issued unconditionally (one round of loads); the thresholds come later from
+// the LUT rows staged in LDS.
+__device__ __forceinline__ EdgeDesc edge_desc(const DeblockArgs &a, int pl, bool vert, int x4, int y4) {
     EdgeDesc e;
-    e.on = false;
     const int sbw = a.g.sb256w;
     const bool ch = pl != 0;
     // luma 4px units of this sample (4:2:0 chroma)
     const int lx4 = ch ? x4 * 2 : x4, ly4 = ch ? y4 * 2 : y4;
     const Rav2dAv2Filter &f = sb_at(a.mask, sbw, lx4, ly4);
     const uint16_t(*fm)[5][4] = ch ? f.filter_uv[vert ? 0 : 1] : f.filter_y[vert ? 0 : 1];
-    int along = vert ? x4 : y4;   // index of the edge line within the SB256
-    int across = vert ? y4 : x4;  // position along the edge
-    int line = ch ? (along & 31) : (along & 63);
-    int sidx = ch ? ((across >> 4) & 1) : ((across >> 4) & 3);
-    uint32_t bit = 1u << (across & 15);
-    int idx;
-    if (ch) {
-        uint32_t m0 = fm[line][0][sidx], m1 = fm[line][1][sidx], m2 = fm[line][2][sidx];
-        if (!((m0 | m1 | m2) & bit)) return e;
-        idx = (m2 & bit) ? 2 : ((m1 & bit) ? 1 : 0);
-        e.mwp = kMaxWidthUV[idx];
-    } else {
-        uint32_t m0 = fm[line][0][sidx], m1 = fm[line][1][sidx], m2 = fm[line][2][sidx], m3 = fm[line][3][sidx];
-        if (!((m0 | m1 | m2 | m3) & bit)) return e;
-        idx = (m3 & bit) ? 3 : (m2 & bit) ? 2 : ((m1 & bit) ? 1 : 0);
-        e.mwp = kMaxWidthY[idx];
-    }
-    int subpu = 3 * ((fm[line][4][sidx] & bit) ? 1 : 0);
-    // the rows pass clamps the negative side at the top of every 64-luma-row band
-    bool band_top = !vert && ((ch ? (y4 & 7) : (y4 & 15)) == 0);
-    e.mwn = band_top ? dmin(ch ? 2 : 6, e.mwp) : e.mwp;
-    // thresholds: this block's and the neighbour's (left or above) LUT values
-    int kind = ch ? 1 + pl : (vert ? 0 : 1);
-    int nx4 = vert ? x4 - 1 : x4, ny4 = vert ? y4 : y4 - 1;
-    int lnx4 = ch ? nx4 * 2 : nx4, lny4 = ch ? ny4 * 2 : ny4;
-    int qc = qidx_at(a.mask, sbw, lx4, ly4), qp = qidx_at(a.mask, sbw, lnx4, lny4);
-    e.q = (uint32_t)(edge_thr(a.lut->q[kind][qc], a.lut->q[kind][qp]) >> subpu);
-    e.side = (uint32_t)(edge_thr(a.lut->side[kind][qc], a.lut->side[kind][qp]) >> subpu);
-    // lossless sides
+    const int along = vert ? x4 : y4;   // index of the edge line within the SB256
+    const int across = vert ? y4 : x4;  // position along the edge
+    const int line = ch ? (along & 31) : (along & 63);
+    const int sidx = ch ? ((across >> 4) & 1) : ((across >> 4) & 3);
+    const uint32_t bit = 1u << (across & 15);
+    const uint32_t m0 = fm[line][0][sidx], m1 = fm[line][1][sidx], m2 = fm[line][2][sidx];
+    const uint32_t m3 = ch ? 0u : (uint32_t)fm[line][3][sidx], m4 = fm[line][4][sidx];
+    const int nx4 = vert ? x4 - 1 : x4, ny4 = vert ? y4 : y4 - 1;
+    const int lnx4 = ch ? nx4 * 2 : nx4, lny4 = ch ? ny4 * 2 : ny4;
+    e.qc = qidx_at(a.mask, sbw, lx4, ly4);
+    e.qp = qidx_at(a.mask, sbw, lnx4, lny4);
     const Rav2dAv2Filter &fn = sb_at(a.mask, sbw, lnx4, lny4);
     if (ch) {
         e.pos_ll = (f.lossless_mask_uv[y4 & 31][(x4 >> 3) & 3] >> (x4 & 7)) & 1;
@@ -186,18 +173,25 @@ __device__ EdgeDesc edge_desc(const DeblockArgs &a, int pl, bool vert, int x4, i
         e.pos_ll = (f.lossless_mask_y[y4 & 63][(x4 >> 4) & 3] >> (x4 & 15)) & 1;
         e.neg_ll = (fn.lossless_mask_y[ny4 & 63][(nx4 >> 4) & 3] >> (nx4 & 15)) & 1;
     }
-    e.on = true;
+    e.on = ((m0 | m1 | m2 | m3) & bit) != 0;
+    const int idx = (m3 & bit) ? 3 : (m2 & bit) ? 2 : ((m1 & bit) ? 1 : 0);
+    e.mwp = byte_at(ch ? kMaxWidthUV : kMaxWidthY, idx);
+    e.subpu = (m4 & bit) ? 3 : 0;
+    // the rows pass clamps the negative side at the top of every 64-luma-row band
+    const bool band_top = !vert && ((ch ? (y4 & 7) : (y4 & 15)) == 0);
+    e.mwn = band_top ? dmin(ch ? 2 : 6, e.mwp) : e.mwp;
     return e;
 }
 
 constexpr int kTW = 64, kTH = 32, kHalo = 16, kThreads = 256;
 
-// Both passes: each lane issues its tile loads, then looks up its edge's
-// descriptor (masks and thresholds in global memory) while those loads are in
-// flight, then stores the tile to LDS.  Every (edge line, 4-line segment) of a
+// Both passes: each lane issues its tile loads, its share of the threshold LUT
+// rows and its edge's side-info loads in one round, then stores the tile and
+// the LUT rows to LDS; thresholds are looked up there.  Every (edge line, 4-line segment) of a
 // tile is one lane (152 for V, 176 for H).
 template <typename P>
-__device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW + 2 * kHalo]) {
+__device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW + 2 * kHalo],
+                                               uint16_t (*lut)[512]) {
     int pl, tx, ty;
     decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl];
@@ -220,6 +214,9 @@ __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id
     }
     constexpr int NE = kTW / 4 + 3;  // edge columns x0-4 .. x0+TW+4
     static_assert(NE * (kTH / 4) <= kThreads, "one lane per edge segment");
+    // the threshold LUT rows of this plane's kind (2 x 512 entries)
+    const int kind = pl ? 1 + pl : 0;
+    const uint2 lutv = ((const uint2 *)(t < 128 ? a.lut->q[kind] : a.lut->side[kind]))[t & 127];
     EdgeDesc e;
     e.on = false;
     const int rs = t / NE, ec = t - rs * NE;
@@ -228,9 +225,13 @@ __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id
     if (a.plane_on[pl] && t < NE * (kTH / 4) && ex > 0 && ex < W && ey < H) e = edge_desc(a, pl, true, ex >> 2, ey >> 2);
 #pragma unroll
     for (int j = 0; j < PV; j++) ((uint2 *)&tile[0][0])[t + j * kThreads] = v[j];
+    ((uint2 *)&lut[0][0])[t] = lutv;
     __syncthreads();
-    if (e.on)
-        filter_edge<LW, 1>(&tile[rs * 4][ex - x0 + kHalo], a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
+    if (e.on) {
+        const uint32_t q = (uint32_t)(edge_thr(lut[0][e.qc], lut[0][e.qp]) >> e.subpu);
+        const uint32_t side = (uint32_t)(edge_thr(lut[1][e.qc], lut[1][e.qp]) >> e.subpu);
+        filter_edge<LW, 1>(&tile[rs * 4][ex - x0 + kHalo], a.g.bdmax, q, side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
+    }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kTH * kTW / 4 / kThreads; j++) {
@@ -242,7 +243,8 @@ __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id
 }
 
 template <typename P>
-__device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW]) {
+__device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW],
+                                               uint16_t (*lut)[512]) {
     int pl, tx, ty;
     decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl];
@@ -262,6 +264,8 @@ __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id
     }
     constexpr int NE = kTH / 4 + 3;  // edge rows y0-4 .. y0+TH+4
     static_assert(NE * (kTW / 4) <= kThreads, "one lane per edge segment");
+    const int kind = pl ? 1 + pl : 1;
+    const uint2 lutv = ((const uint2 *)(t < 128 ? a.lut->q[kind] : a.lut->side[kind]))[t & 127];
     EdgeDesc e;
     e.on = false;
     const int er = t / (kTW / 4), cs = t - er * (kTW / 4);
@@ -270,9 +274,13 @@ __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id
     if (a.plane_on[pl] && t < NE * (kTW / 4) && ey > 0 && ey < H && ex < W) e = edge_desc(a, pl, false, ex >> 2, ey >> 2);
 #pragma unroll
     for (int j = 0; j < PV; j++) ((uint2 *)&tile[0][0])[t + j * kThreads] = v[j];
+    ((uint2 *)&lut[0][0])[t] = lutv;
     __syncthreads();
-    if (e.on)
-        filter_edge<1, kTW>(&tile[ey - y0 + kHalo][cs * 4], a.g.bdmax, e.q, e.side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
+    if (e.on) {
+        const uint32_t q = (uint32_t)(edge_thr(lut[0][e.qc], lut[0][e.qp]) >> e.subpu);
+        const uint32_t side = (uint32_t)(edge_thr(lut[1][e.qc], lut[1][e.qp]) >> e.subpu);
+        filter_edge<1, kTW>(&tile[ey - y0 + kHalo][cs * 4], a.g.bdmax, q, side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
+    }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kTH * kTW / 4 / kThreads; j++) {
@@ -285,20 +293,23 @@ __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id
 
 template <typename P> __global__ __launch_bounds__(kThreads) void deblock_v_kernel(DeblockArgs a) {
     __shared__ __align__(16) uint16_t tile[kTH][kTW + 2 * kHalo];
-    deblock_v_body<P>(a, blockIdx.x, tile);
+    __shared__ __align__(16) uint16_t lut[2][512];
+    deblock_v_body<P>(a, blockIdx.x, tile, lut);
 }
 
 template <typename P> __global__ __launch_bounds__(kThreads) void deblock_h_kernel(DeblockArgs a) {
     __shared__ __align__(16) uint16_t tile[kTH + 2 * kHalo][kTW];
-    deblock_h_body<P>(a, blockIdx.x, tile);
+    __shared__ __align__(16) uint16_t lut[2][512];
+    deblock_h_body<P>(a, blockIdx.x, tile, lut);
 }
 
 // Batched launches: every tile of every frame of one bit depth.
 template <typename P, bool kLoop>
 __global__ __launch_bounds__(kThreads) void deblock_v_batch(const DeblockArgs *__restrict__ args, BatchSched s) {
     __shared__ __align__(16) uint16_t tile[kTH][kTW + 2 * kHalo];
+    __shared__ __align__(16) uint16_t lut[2][512];
     for_each_item<kLoop>(s, [&](int f, int t) {
-        deblock_v_body<P>(args[f], t, tile);
+        deblock_v_body<P>(args[f], t, tile, lut);
         __syncthreads();
     });
 }
@@ -306,8 +317,9 @@ __global__ __launch_bounds__(kThreads) void deblock_v_batch(const DeblockArgs *_
 template <typename P, bool kLoop>
 __global__ __launch_bounds__(kThreads) void deblock_h_batch(const DeblockArgs *__restrict__ args, BatchSched s) {
     __shared__ __align__(16) uint16_t tile[kTH + 2 * kHalo][kTW];
+    __shared__ __align__(16) uint16_t lut[2][512];
     for_each_item<kLoop>(s, [&](int f, int t) {
-        deblock_h_body<P>(args[f], t, tile);
+        deblock_h_body<P>(args[f], t, tile, lut);
         __syncthreads();
     });
 }
