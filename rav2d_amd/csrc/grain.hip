@@ -56,6 +56,84 @@ struct GrainSmem {
     int8_t win[3][4][7];
 };
 
+// Causal AR filter of one grain plane run by one wave as a wavefront: lane k
+// owns rows 3 + k (and 67 + k: the luma plane has 73 rows); sample (y, x) runs
+// at step x + 4y, after its three left neighbours and, on the three rows
+// above, the columns up to x + 3 (row y - 1 finished column x + 3 one step
+// earlier).  A row's seven taps are read as four aligned 16-bit pairs and
+// summed with packed dot products; x has the parity of the step for every
+// lane, so the pair alignment (and the packed tap set) is uniform per step.
+// LDS operations of one wave complete in order; the fence after each step
+// makes its sample visible to the next step and keeps the compiler from
+// moving accesses across steps, so no workgroup barrier is needed.  Taps:
+// win[r][c] for dy = r - 3, dx = c - 3 (zero outside the lag).
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+template <bool kChroma>
+__device__ __forceinline__ void ar_wave(int16_t (*L)[GW], int h, int w, const int8_t (*win)[7], int ar_shift, int gmin,
+                                        int gmax, const int16_t (*luma)[GW], int wl) {
+    const int lane = threadIdx.x & 63;
+    auto pk = [](int lo, int hi) -> s16x2 { return (s16x2){(short)lo, (short)hi}; };
+    // tap pairs for an odd x (pairs start at column x - 3) and an even x (x - 4);
+    // row 3 is the own row (taps at dx = -3..-1, the pair holding x has weight 0)
+    s16x2 po[4][4], pe[4][4];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        po[r][0] = pk(win[r][0], win[r][1]);
+        po[r][1] = pk(win[r][2], win[r][3]);
+        po[r][2] = pk(win[r][4], win[r][5]);
+        po[r][3] = pk(win[r][6], 0);
+        pe[r][0] = pk(0, win[r][0]);
+        pe[r][1] = pk(win[r][1], win[r][2]);
+        pe[r][2] = pk(win[r][3], win[r][4]);
+        pe[r][3] = pk(win[r][5], win[r][6]);
+    }
+    po[3][0] = pk(win[3][0], win[3][1]);
+    po[3][1] = pk(win[3][2], 0);
+    pe[3][0] = pk(0, win[3][0]);
+    pe[3][1] = pk(win[3][1], win[3][2]);
+    const int last = (w - 4) + 4 * (h - 1);
+    auto step = [&](int s, const s16x2(&P)[4][4], int odd) {
+        int y = 3 + lane, x = s - 4 * y;
+        if (x > w - 4) {  // first row done: the lane's second row
+            y += 64;
+            x = s - 4 * y;
+        }
+        if (y < h && x >= 3 && x <= w - 4) {
+            const int c0 = x - 4 + odd;  // even: every pair is 4-byte aligned
+            int acc[4];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const uint32_t *row = (const uint32_t *)&L[y - 3 + r][c0];
+                int a = 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) a = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, row[q]), P[r][q], a, false);
+                acc[r] = a;
+            }
+            {
+                const uint32_t *row = (const uint32_t *)&L[y][c0];
+                acc[3] = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, row[0]), P[3][0], 0, false);
+                acc[3] = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, row[1]), P[3][1], acc[3], false);
+            }
+            int sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if constexpr (kChroma) {
+                const int lx = ((x - 3) << 1) + 3, ly = ((y - 3) << 1) + 3;
+                const int l = luma[ly][lx] + luma[ly][lx + 1] + luma[ly + 1][lx] + luma[ly + 1][lx + 1];
+                sum += round2(l, 2) * wl;
+            }
+            L[y][x] = (int16_t)iclip(L[y][x] + round2(sum, ar_shift), gmin, gmax);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    };
+    // the first step (x = 3 on row 3) is odd
+    for (int s = 3 + 4 * 3; s <= last; s += 2) {
+        step(s, po, 1);
+        if (s + 1 <= last) step(s + 1, pe, 0);
+    }
+}
+
 __device__ __forceinline__ void grain_setup_body(const GrainArgs &a, const int16_t *gauss, GrainSmem &sm) {
     auto &lut = sm.lut;
     auto &g_lds = sm.g_lds;
@@ -122,100 +200,60 @@ __device__ __forceinline__ void grain_setup_body(const GrainArgs &a, const int16
         win[pl][r][c] = (int8_t)v;
     }
     __syncthreads();
-    int wy[4][7];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 7; c++) wy[r][c] = win[0][r][c];
-    for (int step = 3 + 4 * 3; step <= (GW - 4) + 4 * (GH - 1); step++) {
-        int y = t + 3;
-        if (y < GH) {
-            int x = step - 4 * y;
-            if (x >= 3 && x < GW - 3) {
-                int sum = 0;
-#pragma unroll
-                for (int r = 0; r < 3; r++)
-#pragma unroll
-                    for (int c = 0; c < 7; c++) sum += wy[r][c] * lut[0][y - 3 + r][x - 3 + c];
-#pragma unroll
-                for (int c = 0; c < 3; c++) sum += wy[3][c] * lut[0][y][x - 3 + c];
-                lut[0][y][x] = (int16_t)iclip(lut[0][y][x] + round2(sum, (int)d.ar_coeff_shift), gmin, gmax);
+    // luma AR on wave 0 while wave 1 builds the scaling LUTs
+    if (t < 64) {
+        ar_wave<false>(lut[0], GH, GW, win[0], (int)d.ar_coeff_shift, gmin, gmax, nullptr, 0);
+    } else {
+        // scaling LUTs (generate_scaling_8bpc/hbd, filmgrain.rs:22-113), every
+        // entry in closed form: the first pass writes entries at multiples of
+        // 1 << (bd-8) (later segments overwrite earlier ones, then the tail fill),
+        // the second pass interpolates between them inside each segment.
+        for (int pl = 0; pl < 3; pl++) {
+            const int num = d.num_points[pl];
+            const uint8_t(*pts)[2] = d.points[pl];
+            const int sx = bd - 8, size = 1 << bd, pad = 1 << sx;
+            auto first_pass = [&](int c) -> int {  // value at coarse position c
+                int v = 0;
+                if (c < pts[0][0]) v = pts[0][1];
+                for (int k = 0; k < num - 1; k++) {
+                    int bx = pts[k][0], dx = pts[k + 1][0] - bx;
+                    if (dx > 0 && c >= bx && c < bx + dx) {
+                        int delta = (pts[k + 1][1] - pts[k][1]) * ((0x10000 + (dx >> 1)) / dx);
+                        v = (uint8_t)(pts[k][1] + ((0x8000 + (c - bx) * delta) >> 16));
+                    }
+                }
+                if (c >= pts[num - 1][0]) v = pts[num - 1][1];
+                return v;
+            };
+            for (int e = t - 64; e < size; e += 64) {
+                int v = 0;
+                if (num > 0) {
+                    int c = e >> sx, n = e & (pad - 1);
+                    v = first_pass(c);
+                    if (n && e >= (pts[0][0] << sx) && e < (pts[num - 1][0] << sx)) {
+                        // inside some segment: interpolate like the second pass
+                        bool in_seg = false;
+                        for (int k = 0; k < num - 1; k++) {
+                            int bx = pts[k][0] << sx, ex = pts[k + 1][0] << sx;
+                            if (e >= bx && e < ex) in_seg = true;
+                        }
+                        if (in_seg) {
+                            int base = v, range = first_pass(c + 1) - base;
+                            v = (uint8_t)(base + (((pad >> 1) + n * range) >> sx));
+                        }
+                    }
+                }
+                scl[pl][e] = (uint8_t)v;
             }
         }
-        __syncthreads();
     }
-    // chroma (4:2:0 sub-grid), with the luma correlation term
+    __syncthreads();
+    // chroma (4:2:0 sub-grid), with the luma correlation term: U on wave 0, V on wave 1
     {
-        const int uv = t >> 6, y = (t & 63) + 3;
+        const int uv = t >> 6;
         const bool on = uv == 0 ? want_u : want_v;
-        int wc[4][7];
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-            for (int c = 0; c < 7; c++) wc[r][c] = win[1 + uv][r][c];
         const int wl = d.num_points[0] > 0 ? coef[1 + uv][2 * lag * (lag + 1)] : 0;
-        for (int step = 3 + 4 * 3; step <= (SGW - 4) + 4 * (SGH - 1); step++) {
-            if (on && y < SGH) {
-                int x = step - 4 * y;
-                if (x >= 3 && x < SGW - 3) {
-                    int16_t(*L)[GW] = lut[1 + uv];
-                    int sum = 0;
-#pragma unroll
-                    for (int r = 0; r < 3; r++)
-#pragma unroll
-                        for (int c = 0; c < 7; c++) sum += wc[r][c] * L[y - 3 + r][x - 3 + c];
-#pragma unroll
-                    for (int c = 0; c < 3; c++) sum += wc[3][c] * L[y][x - 3 + c];
-                    int lx = ((x - 3) << 1) + 3, ly = ((y - 3) << 1) + 3;
-                    int l = lut[0][ly][lx] + lut[0][ly][lx + 1] + lut[0][ly + 1][lx] + lut[0][ly + 1][lx + 1];
-                    sum += round2(l, 2) * wl;
-                    L[y][x] = (int16_t)iclip(L[y][x] + round2(sum, (int)d.ar_coeff_shift), gmin, gmax);
-                }
-            }
-            __syncthreads();
-        }
-    }
-    // scaling LUTs (generate_scaling_8bpc/hbd, filmgrain.rs:22-113), every
-    // entry in closed form: the first pass writes entries at multiples of
-    // 1 << (bd-8) (later segments overwrite earlier ones, then the tail fill),
-    // the second pass interpolates between them inside each segment.
-    for (int pl = 0; pl < 3; pl++) {
-        const int num = d.num_points[pl];
-        const uint8_t(*pts)[2] = d.points[pl];
-        const int sx = bd - 8, size = 1 << bd, pad = 1 << sx;
-        auto first_pass = [&](int c) -> int {  // value at coarse position c
-            int v = 0;
-            if (c < pts[0][0]) v = pts[0][1];
-            for (int k = 0; k < num - 1; k++) {
-                int bx = pts[k][0], dx = pts[k + 1][0] - bx;
-                if (dx > 0 && c >= bx && c < bx + dx) {
-                    int delta = (pts[k + 1][1] - pts[k][1]) * ((0x10000 + (dx >> 1)) / dx);
-                    v = (uint8_t)(pts[k][1] + ((0x8000 + (c - bx) * delta) >> 16));
-                }
-            }
-            if (c >= pts[num - 1][0]) v = pts[num - 1][1];
-            return v;
-        };
-        for (int e = t; e < size; e += kSetupThreads) {
-            int v = 0;
-            if (num > 0) {
-                int c = e >> sx, n = e & (pad - 1);
-                v = first_pass(c);
-                if (n && e >= (pts[0][0] << sx) && e < (pts[num - 1][0] << sx)) {
-                    // inside some segment: interpolate like the second pass
-                    bool in_seg = false;
-                    for (int k = 0; k < num - 1; k++) {
-                        int bx = pts[k][0] << sx, ex = pts[k + 1][0] << sx;
-                        if (e >= bx && e < ex) in_seg = true;
-                    }
-                    if (in_seg) {
-                        int base = v, range = first_pass(c + 1) - base;
-                        v = (uint8_t)(base + (((pad >> 1) + n * range) >> sx));
-                    }
-                }
-            }
-            scl[pl][e] = (uint8_t)v;
-        }
+        if (on) ar_wave<true>(lut[1 + uv], SGH, SGW, win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl);
     }
     __syncthreads();
     for (int i = t; i < 3 * 1024; i += kSetupThreads) a.scaling[i] = (&scl[0][0])[i];
