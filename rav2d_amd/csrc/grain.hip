@@ -271,8 +271,16 @@ __global__ __launch_bounds__(kSetupThreads) void grain_setup_batch(const GrainAr
     grain_setup_body(args[blockIdx.x], gauss, sm);
 }
 
-// One output sample: S(bxo, byo) samples the block `bxo` to the left and
-// `byo` above, as sample_lut with offsets[bxo][byo].
+// Display samples of one 64x32 tile: one lane = four horizontally adjacent
+// samples (a group never straddles a grain block: block sizes are multiples
+// of 4), two groups per lane (rows r and r + 16 of the tile).  Every load of
+// both groups that does not depend on another load is issued first (samples,
+// the collocated luma of chroma, the block offsets of the block and of its
+// left / upper neighbours), then the scaling and grain-template reads.  The
+// grain sample of block offsets o at (lx, ly) is sample_lut's
+// lut[3 + s (3 + o[0]) + ly][3 + s (3 + o[1]) + lx] (s = 2 luma, 1 chroma);
+// the overlap rows and columns of a block blend it with the left / upper
+// neighbour's (filmgrain.rs:296-380).
 template <typename P>
 __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id) {
     int pl, tx, ty;
@@ -281,10 +289,10 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
     const Rav2dFilmGrainData &d = a.fg;
     const int bd = a.bitdepth, bdm8 = bd - 8;
     const int ctr = 128 << bdm8, gmin = -ctr, gmax = ctr - 1;
-    const int bs = 16 << d.block_size;
     const int sub = pl ? 1 : 0;
-    const int pbs = bs >> sub;  // block size in this plane
-    const int16_t(*lut)[GW] = (const int16_t(*)[GW])(a.lut + pl * GH * GW);
+    const int lg = 4 + d.block_size - sub;  // log2 of the block size in this plane (16 << block_size luma)
+    const int pbs = 1 << lg;
+    const int16_t *lut = a.lut + pl * GH * GW;
     const uint8_t *scaling = a.scaling + (pl && d.chroma_scaling_from_luma ? 0 : pl) * 1024;
     const bool on = a.plane_on[pl];
     int minv = 0, maxv = (1 << bd) - 1;
@@ -292,31 +300,71 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         minv = 16 << bdm8;
         maxv = (pl == 0 || d.mc_identity ? 235 : 240) << bdm8;
     }
-    // one lane = four horizontally adjacent samples (a group never straddles a
-    // grain block: block sizes are multiples of 4), two groups per lane
-    auto Wt = [&](int k, int j) -> int { return sub ? (k == 0 ? (j == 0 ? 23 : 22) : 0) : ((k == j) ? 27 : 17); };
-#pragma unroll 1
-    for (int qi = threadIdx.x; qi < 16 * 32; qi += 256) {
-        const int x = tx * 64 + (qi & 15) * 4, y = ty * 32 + (qi >> 4);
-        if (x >= src.w || y >= src.h) continue;
-        const uint2 sq = load4(row_ptr<const P>(src, y) + x);
-        int sv[4] = {(int)(sq.x & 0xffff), (int)(sq.x >> 16), (int)(sq.y & 0xffff), (int)(sq.y >> 16)};
-        if (!on) {
-            store4(row_ptr<P>(dst, y) + x, sq);
-            continue;
+    const int W = src.w, H = src.h;
+    const uint32_t sstride = (uint32_t)src.stride, dstride = (uint32_t)dst.stride;
+    const int x0 = tx * 64, y0 = ty * 32;
+    // tile-uniform bases; per-lane offsets are 24-bit products (rows < 2^24 bytes)
+    const uint8_t *sbase = src.ptr + (uint64_t)(uint32_t)y0 * sstride + x0 * sizeof(P);
+    uint8_t *dbase = dst.ptr + (uint64_t)(uint32_t)y0 * dstride + x0 * sizeof(P);
+    const uint32_t lstride = (uint32_t)a.src[0].stride;
+    const uint8_t *lbase = a.src[0].ptr + (uint64_t)(uint32_t)(2 * y0) * lstride + 2 * x0 * sizeof(P);
+    const int t = threadIdx.x;
+    const int c4 = (t & 15) * 4;  // column of the group in the tile
+    constexpr int NG = 2;
+    int gr[NG];
+    bool gv[NG];
+    uint2 sq[NG], lq0[NG], lq1[NG];
+    uint32_t o00[NG], o10[NG], o01[NG], o11[NG];  // packed (y, x) offset bytes of the block, left, upper, upper-left
+#pragma unroll
+    for (int u = 0; u < NG; u++) {
+        const int r = (t >> 4) + 16 * u;
+        gr[u] = r;
+        gv[u] = x0 + c4 < W && y0 + r < H;
+        const int rc = dmin(r, H - 1 - y0), cc = dmin(c4, W - 4 - x0);  // clamped: always valid addresses
+        sq[u] = load4((const P *)(sbase + __umul24(rc, sstride)) + cc);
+        if (pl) {
+            const P *lrow = (const P *)(lbase + __umul24(2 * rc, lstride)) + 2 * cc;
+            lq0[u] = load4(lrow);
+            lq1[u] = load4(lrow + 4);
         }
+        if (on) {
+            const int band = (y0 + rc) >> lg, b = (x0 + cc) >> lg;
+            const int bl = dmax(b - 1, 0), bu = dmax(band - 1, 0);
+            const uint16_t *ob = (const uint16_t *)a.offsets;
+            const int rb = __umul24(band, a.n_blocks), ru = __umul24(bu, a.n_blocks);
+            o00[u] = ob[rb + b];
+            o10[u] = ob[rb + bl];
+            o01[u] = ob[ru + b];
+            o11[u] = ob[ru + bl];
+        }
+    }
+    if (!on) {
+#pragma unroll
+        for (int u = 0; u < NG; u++)
+            if (gv[u]) store4((P *)(dbase + __umul24(gr[u], dstride)) + c4, sq[u]);
+        return;
+    }
+    auto Wt = [&](int k, int j) -> int { return sub ? (k == 0 ? (j == 0 ? 23 : 22) : 0) : ((k == j) ? 27 : 17); };
+    const int os = 2 >> sub;
+    // template index of block offsets `o` (x byte high, y byte low) at (lx, ly)
+    auto tix = [&](uint32_t o, int lx, int ly) -> int {
+        return __umul24(3 + os * (3 + (int)(o & 0xff)) + ly, GW) + 3 + os * (3 + (int)(o >> 8)) + lx;
+    };
+#pragma unroll
+    for (int u = 0; u < NG; u++) {
+        const int x = x0 + c4, y = y0 + gr[u];
+        int sv[4] = {(int)(sq[u].x & 0xffff), (int)(sq[u].x >> 16), (int)(sq[u].y & 0xffff), (int)(sq[u].y >> 16)};
         int val[4];
         if (pl) {
-            const P *lrow = row_ptr<const P>(a.src[0], 2 * y) + 2 * x;
-            const uint2 l0 = load4(lrow), l1 = load4(lrow + 4);
-            const int ls[8] = {(int)(l0.x & 0xffff), (int)(l0.x >> 16), (int)(l0.y & 0xffff), (int)(l0.y >> 16),
-                               (int)(l1.x & 0xffff), (int)(l1.x >> 16), (int)(l1.y & 0xffff), (int)(l1.y >> 16)};
+            const int ls[8] = {(int)(lq0[u].x & 0xffff), (int)(lq0[u].x >> 16), (int)(lq0[u].y & 0xffff),
+                               (int)(lq0[u].y >> 16),    (int)(lq1[u].x & 0xffff), (int)(lq1[u].x >> 16),
+                               (int)(lq1[u].y & 0xffff), (int)(lq1[u].y >> 16)};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int avg = (ls[2 * k] + ls[2 * k + 1] + 1) >> 1;
                 if (!d.chroma_scaling_from_luma) {
                     const int uv = pl - 1;
-                    const int combined = avg * d.uv_luma_mult[uv] + sv[k] * d.uv_mult[uv];
+                    const int combined = __mul24(avg, d.uv_luma_mult[uv]) + __mul24(sv[k], d.uv_mult[uv]);
                     val[k] = iclip((combined >> 6) + d.uv_offset[uv] * (1 << bdm8), 0, (1 << bd) - 1);
                 } else {
                     val[k] = avg;
@@ -331,34 +379,29 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         int sc[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) sc[k] = scaling[dmin(val[k], (1 << bd) - 1)];
-        const int band = y / pbs, b = x / pbs;
-        const int ly = y - band * pbs, lx0 = x - b * pbs;
-        const int bh = dmin(pbs, src.h - band * pbs), bw = dmin(pbs, src.w - b * pbs);
-        const int ystart = (d.overlap_flag && band > 0) ? dmin(2 >> sub, bh) : 0;
-        const int xstart = (d.overlap_flag && b > 0) ? dmin(2 >> sub, bw) : 0;
-        auto off = [&](int bxo, int byo) -> const uint8_t * {
-            return a.offsets + ((size_t)(band - byo) * a.n_blocks + (b - bxo)) * 2;
-        };
-        const uint8_t *o00 = off(0, 0);
-        const int offx = 3 + (2 >> sub) * (3 + o00[1]), offy = 3 + (2 >> sub) * (3 + o00[0]);
-        const int16_t *lr0 = &lut[offy + ly][offx + lx0];
-        int g[4] = {lr0[0], lr0[1], lr0[2], lr0[3]};
+        const int band = y >> lg, b = x >> lg;
+        const int ly = y & (pbs - 1), lx0 = x & (pbs - 1);
+        const int bh = dmin(pbs, H - (band << lg)), bw = dmin(pbs, W - (b << lg));
+        const int ystart = (d.overlap_flag && band > 0) ? dmin(os, bh) : 0;
+        const int xstart = (d.overlap_flag && b > 0) ? dmin(os, bw) : 0;
+        const int16_t *g0 = lut + tix(o00[u], lx0, ly);
+        int g[4] = {g0[0], g0[1], g0[2], g0[3]};
         if (lx0 < xstart || ly < ystart) {
             // overlap with the left and/or upper neighbour block (first 1-2 columns/rows)
-            auto S = [&](int bxo, int byo, int lx) -> int {
-                const uint8_t *o = off(bxo, byo);
-                const int ox = 3 + (2 >> sub) * (3 + o[1]), oy = 3 + (2 >> sub) * (3 + o[0]);
-                return lut[oy + ly + pbs * byo][ox + lx + pbs * bxo];
-            };
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int lx = lx0 + k;
                 int gg = g[k];
-                if (lx < xstart) gg = iclip(round2(S(1, 0, lx) * Wt(lx, 0) + gg * Wt(lx, 1), 5), gmin, gmax);
+                if (lx < xstart)
+                    gg = iclip(round2(__mul24(lut[tix(o10[u], lx + pbs, ly)], Wt(lx, 0)) + __mul24(gg, Wt(lx, 1)), 5), gmin,
+                               gmax);
                 if (ly < ystart) {
-                    int top = S(0, 1, lx);
-                    if (lx < xstart) top = iclip(round2(S(1, 1, lx) * Wt(lx, 0) + top * Wt(lx, 1), 5), gmin, gmax);
-                    gg = iclip(round2(top * Wt(ly, 0) + gg * Wt(ly, 1), 5), gmin, gmax);
+                    int top = lut[tix(o01[u], lx, ly + pbs)];
+                    if (lx < xstart)
+                        top = iclip(round2(__mul24(lut[tix(o11[u], lx + pbs, ly + pbs)], Wt(lx, 0)) + __mul24(top, Wt(lx, 1)),
+                                           5),
+                                    gmin, gmax);
+                    gg = iclip(round2(__mul24(top, Wt(ly, 0)) + __mul24(gg, Wt(ly, 1)), 5), gmin, gmax);
                 }
                 g[k] = gg;
             }
@@ -366,11 +409,12 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         int out[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const int noise = round2(sc[k] * g[k], d.scaling_shift);
+            const int noise = round2(__mul24(sc[k], g[k]), d.scaling_shift);  // 8-bit x 12-bit
             out[k] = iclip(sv[k] + noise, minv, maxv);
         }
-        store4(row_ptr<P>(dst, y) + x, make_uint2((uint32_t)out[0] | ((uint32_t)out[1] << 16),
-                                                   (uint32_t)out[2] | ((uint32_t)out[3] << 16)));
+        if (gv[u])
+            store4((P *)(dbase + __umul24(gr[u], dstride)) + c4,
+                   make_uint2((uint32_t)out[0] | ((uint32_t)out[1] << 16), (uint32_t)out[2] | ((uint32_t)out[3] << 16)));
     }
 }
 
