@@ -91,6 +91,21 @@ struct LrSmem {
     LrItem it;
 };
 
+// Row and column of element t + j * kThreads of a row-major grid with COLS
+// columns, stepped across j without divisions.
+template <int COLS> struct GridPos {
+    int r, c;
+    __device__ __forceinline__ explicit GridPos(int t) : r(t / COLS), c(t % COLS) {}
+    __device__ __forceinline__ void next() {
+        r += kThreads / COLS;
+        c += kThreads % COLS;
+        if (c >= COLS) {
+            c -= COLS;
+            r++;
+        }
+    }
+};
+
 template <typename P, typename Args>
 __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     uint16_t *V = sm.V, *Ld = sm.Ld, *cells = sm.cells;
@@ -110,8 +125,10 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     auto &f2 = a.f2[pl];
     auto &dst = a.dst[pl];
     const int bd = a.g.bitdepth, bdm8 = bd - 8, bdmax = a.g.bdmax;
-    // every buffer of a plane has the same row stride (< 4 GiB)
+    // every buffer of a plane has the same row stride; rows and strides are
+    // below 2^24, so row offsets are 24-bit products
     const uint32_t stride = (uint32_t)f3.stride;
+    uint8_t *const dbase = dst.ptr + (uint64_t)(uint32_t)y0 * stride + x0 * sizeof(P);
     using Q = typename std::conditional<sizeof(P) == 1, uint32_t, uint2>::type;  // 4 samples
     if (kind == LR_KIND_NONE && !gdf) {
         // plane copy, 4 samples per access (x0 and w are multiples of 4): units of
@@ -121,22 +138,27 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
         const int n4 = w >> 2, n = h * n4;
         const uint8_t *src = f3.ptr + (uint64_t)(uint32_t)y0 * stride + x0 * sizeof(P);
         uint8_t *dp = dst.ptr + (uint64_t)(uint32_t)y0 * stride + x0 * sizeof(P);
+        // element t + j * kThreads of the h x n4 grid, stepped without divisions
+        const int dq = kThreads / n4, dm = kThreads - dq * n4;
+        int r = t / n4, c = t - (t / n4) * n4;
         for (int b = 0; b < n; b += CP * kThreads) {
             Q v[CP];
+            int rs[CP], cs[CP];
 #pragma unroll
             for (int j = 0; j < CP; j++) {
-                const int i = dmin(b + t + j * kThreads, n - 1);
-                const int r = i / n4, c = i - r * n4;
-                v[j] = *(const Q *)(src + (uint64_t)(uint32_t)r * stride + c * sizeof(Q));
-            }
-#pragma unroll
-            for (int j = 0; j < CP; j++) {
-                const int i = b + t + j * kThreads;
-                if (i < n) {
-                    const int r = i / n4, c = i - r * n4;
-                    *(Q *)(dp + (uint64_t)(uint32_t)r * stride + c * sizeof(Q)) = v[j];
+                rs[j] = r;
+                cs[j] = c;
+                v[j] = *(const Q *)(src + __umul24(dmin(r, h - 1), stride) + c * sizeof(Q));
+                r += dq;
+                c += dm;
+                if (c >= n4) {
+                    c -= n4;
+                    r++;
                 }
             }
+#pragma unroll
+            for (int j = 0; j < CP; j++)
+                if (rs[j] < h) *(Q *)(dp + __umul24(rs[j], stride) + cs[j] * sizeof(Q)) = v[j];
         }
         return;
     }
@@ -154,13 +176,13 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     const int pw = f3.w;
     const bool top = edges & LRE_TOP, bottom = edges & LRE_BOTTOM;
     Q qv[PERQ];
+    GridPos<VQ> gq(t);
 #pragma unroll
-    for (int j = 0; j < PERQ; j++) {
-        const int i = dmin(t + j * kThreads, nvq - 1);
-        const int r = i / VQ - E, c0 = (i % VQ) * 4 - VC;
+    for (int j = 0; j < PERQ; j++, gq.next()) {
+        const int r = dmin(gq.r, h + 2 * E - 1) - E, c0 = gq.c * 4 - VC;
         const bool from2 = (r < 0 && top) || (r >= h && bottom);
         const int yy = y0 + iclip(r, from2 ? -2 : 0, from2 ? h + 1 : h - 1);
-        const uint8_t *row = (from2 ? f2.ptr : f3.ptr) + (uint64_t)(uint32_t)yy * stride;
+        const uint8_t *row = (from2 ? f2.ptr : f3.ptr) + __umul24(yy, stride);
         qv[j] = *(const Q *)(row + iclip(x0 + c0, 0, pw - 4) * sizeof(P));
     }
     // tables of this item's filter kinds, 16 (or 8) bytes per lane
@@ -186,19 +208,18 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
         auto &l3 = a.f3[0];
         auto &l2 = a.f2[0];
         const uint32_t lstride = (uint32_t)l3.stride;
-        const int nl = (h + 4) * LS;
         // cfl_ds 0: 2x2 average, 1: vertical pair average, 2: top-left sample
         const int ds = a.cfl_ds, mb = ds <= 1 ? -1 : 0, m1 = ds == 0 ? -1 : 0, sh = ds == 0 ? 2 : (ds == 1 ? 1 : 0);
         using Q2 = typename std::conditional<sizeof(P) == 1, uint16_t, uint32_t>::type;  // 2 samples
+        GridPos<LS> gl(t);
 #pragma unroll
-        for (int j = 0; j < PERL; j++) {
-            const int i0 = t + j * kThreads, i = dmin(i0, nl - 1);
-            const int r = i / LS - 2, c = i % LS - 2;
+        for (int j = 0; j < PERL; j++, gl.next()) {
+            const int r = dmin(gl.r, h + 3) - 2, c = gl.c - 2;
             const int xs = (c < 0 && !(edges & LRE_LEFT)) ? 0 : ((c >= w && !(edges & LRE_RIGHT)) ? w - 1 : dmin(c, w + 1));
             const int rr = iclip(r, top ? -2 : 0, bottom ? h + 1 : h - 1);
             const bool from2 = rr < 0 || rr >= h, single = rr == -2 || rr == h + 1;
             const int ya = 2 * (y0 + rr) + (rr == -2 ? 2 : (rr == h + 1 ? -1 : 0));
-            const uint8_t *ra = (from2 ? l2.ptr : l3.ptr) + (uint64_t)(uint32_t)ya * lstride;
+            const uint8_t *ra = (from2 ? l2.ptr : l3.ptr) + __umul24(ya, lstride);
             const uint8_t *rb = single ? ra : ra + lstride;
             const int lx = 2 * (x0 + xs);
             const Q2 qa = *(const Q2 *)(ra + lx * sizeof(P)), qb = *(const Q2 *)(rb + lx * sizeof(P));
@@ -206,7 +227,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
             const int a0 = (int)(qa & ((1u << SB) - 1)), a1 = (int)(qa >> SB);
             const int b0 = (int)(qb & ((1u << SB) - 1)), b1 = (int)(qb >> SB);
             const int v = (a0 + (b0 & mb) + ((a1 + b1) & m1)) >> sh;
-            lvals[j] = (i0 < nl && c < w + 2) ? (uint16_t)v : 0;
+            lvals[j] = (gl.r < h + 4 && c < w + 2) ? (uint16_t)v : 0;
         }
     }
 #pragma unroll
@@ -361,7 +382,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
 #pragma unroll
                     for (int k = 0; k < 16; k++)
                         s += s16(VAT(pp[p], kNsYT[k].dy, kNsYT[k].dx) + VAT(pp[p], -kNsYT[k].dy, -kNsYT[k].dx) - 2 * m) *
-                             filt[k];
+                             s16(filt[k]);
                     out[p] = iclip((s + 64) >> 7, 0, bdmax);
                 }
             } else if (kind == LR_KIND_SINGLE) {
@@ -463,17 +484,18 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
                     int full = 0;
 #pragma unroll
                     for (int k = 0; k < 3; k++) {
-                        int sv = iv[p][k] * gscale;
+                        // gscale is 8 (intra) or 5 (inter): shifts instead of a 32-bit multiply
+                        const int sv = (iv[p][k] << 2) + (gscale == 8 ? iv[p][k] << 2 : iv[p][k]);
                         int v = apply_sign((abs(sv) + (1 << 14)) >> 15, sv);
-                        full = full * (gscale * 2) + iclip(v, -gscale, gscale - 1) + gscale;
+                        full = (full << 3) + (gscale == 8 ? full << 3 : full << 1) + iclip(v, -gscale, gscale - 1) + gscale;
                     }
                     int diff = sm.s_err[full] * a.gdf_scale;
                     out[p] = iclip(out[p] + apply_sign((abs(diff) + rnd_g) >> sh, diff), 0, bdmax);
                 }
             }
         }
-        P *d0 = row_ptr<P>(dst, y0 + y) + x0 + x;
-        P *d1 = row_ptr<P>(dst, y0 + y + 1) + x0 + x;
+        P *d0 = (P *)(dbase + __umul24(y, stride)) + x;
+        P *d1 = (P *)(dbase + __umul24(y + 1, stride)) + x;
         d0[0] = (P)out[0];
         d0[1] = (P)out[1];
         d1[0] = (P)out[2];
