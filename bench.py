@@ -47,11 +47,11 @@ PASS_NAMES = ["deblock_v", "deblock_h", "cdef_ccso", "loop_restoration", "film_g
 # probe's counters equal its known 256 MiB read / 256 MiB written); see
 # profiles/hbm_traffic_4k10_pmc.md.  Valid for the kernels profiled there.
 PMC_TRAFFIC_4K10 = {
-    "deblock_v": int(37634.4 * 2 * 1024 + 24300 * 1024),
-    "deblock_h": int(26602.5 * 2 * 1024 + 24300 * 1024),
-    "cdef_ccso": int(46632.8 * 2 * 1024 + 37067.4 * 1024),
-    "loop_restoration": int(38261.9 * 2 * 1024 + 24300 * 1024),
-    "film_grain": int(19924.5 * 2 * 1024 + 24300 * 1024),
+    "deblock_v": int(37638.3 * 2 * 1024 + 24300 * 1024),
+    "deblock_h": int(26610.1 * 2 * 1024 + 24300 * 1024),
+    "cdef_ccso": int(46181.3 * 2 * 1024 + 24300 * 1024),
+    "loop_restoration": int(38181.4 * 2 * 1024 + 24300 * 1024),
+    "film_grain": int(19941.4 * 2 * 1024 + 24300 * 1024),
 }
 
 
