@@ -63,9 +63,10 @@ struct GrainSmem {
 // earlier).  A row's seven taps are read as four aligned 16-bit pairs and
 // summed with packed dot products; x has the parity of the step for every
 // lane, so the pair alignment (and the packed tap set) is uniform per step.
-// LDS operations of one wave complete in order; the fence after each step
-// makes its sample visible to the next step and keeps the compiler from
-// moving accesses across steps, so no workgroup barrier is needed.  Taps:
+// The LDS executes one wave's operations in issue order, so a read issued
+// after another lane's write sees it; the wavefront-scope fence after each
+// step keeps the compiler from moving accesses across steps (it waits for
+// nothing), so no workgroup barrier or counter wait is needed.  Taps:
 // win[r][c] for dy = r - 3, dx = c - 3 (zero outside the lag).
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
@@ -123,9 +124,9 @@ __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], int h, int w, const in
             }
             L[y][x] = (int16_t)iclip(L[y][x] + round2(sum, ar_shift), gmin, gmax);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
     // the first step (x = 3 on row 3) is odd
     for (int s = 3 + 4 * 3; s <= last; s += 2) {
