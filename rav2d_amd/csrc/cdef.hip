@@ -487,7 +487,10 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
 template <typename P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void cdef_kernel(CdefArgs a) {
     __shared__ CdefSmem sm;
-    cdef_body<P>(a, blockIdx.x, blockIdx.y, sm);
+    // XCD-contiguous superblock runs: neighbours share halo rows and columns in L2
+    const int sbw = (a.g.bw * 4 + 63) / 64;
+    const int sb = xcd_item(blockIdx.x, gridDim.x);
+    cdef_body<P>(a, sb % sbw, sb / sbw, sm);
 }
 
 // Batched launch: one item per 64x64 superblock of every frame of one bit depth.
@@ -542,7 +545,7 @@ void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, in
 }
 
 void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s) {
-    dim3 grid((a.g.bw * 4 + 63) / 64, (a.g.bh * 4 + 63) / 64);
+    dim3 grid(((a.g.bw * 4 + 63) / 64) * ((a.g.bh * 4 + 63) / 64));
     if (bitdepth > 8)
         hipLaunchKernelGGL(cdef_kernel<uint16_t>, grid, dim3(kThreads), 0, s, a);
     else

@@ -59,6 +59,17 @@ __device__ __forceinline__ int16_t lds_load_i16(const int16_t *p) {
                              __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Workgroup b of a one-dimensional launch runs on XCD b mod 8: the dispatcher
+// deals workgroups round-robin over the 8 XCDs, each with its own L2.  This
+// maps b to a work item so that each XCD takes a contiguous run of items and
+// neighbouring tiles, whose halos overlap, share an L2: the first 8 * (n / 8)
+// workgroups as XCD-major runs, the remaining n mod 8 unchanged (a bijection
+// of [0, n)).
+__device__ __forceinline__ int xcd_item(int b, int n) {
+    const int q = n >> 3;
+    return b < (q << 3) ? (b & 7) * q + (b >> 3) : b;
+}
+
 // Two adjacent samples (2-sample aligned) with one store.
 template <typename P> __device__ __forceinline__ void store2(P *p, int a, int b) {
     if constexpr (sizeof(P) == 1)
