@@ -134,11 +134,31 @@ typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ s16x2 splat2(int v) { return (s16x2){(short)v, (short)v}; }
 
-__device__ __forceinline__ s16x2 constrain2(s16x2 d, s16x2 thr, s16x2 sh) {
+// constrain() of N taps' differences d, in place, stage by stage across the
+// taps: consecutive packed operations are then independent (a dependent pair
+// of packed operations costs a wait state).
+template <int N>
+__device__ __forceinline__ void constrain_n(s16x2 (&d)[N], s16x2 thr, s16x2 sh) {
     const s16x2 z = {0, 0};
-    const s16x2 ad = __builtin_elementwise_max(d, __builtin_elementwise_sub_sat(z, d));
-    const s16x2 c = __builtin_elementwise_max(z, thr - (ad >> sh));
-    return __builtin_elementwise_min(__builtin_elementwise_max(d, z - c), c);
+    s16x2 c[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) c[k] = __builtin_elementwise_sub_sat(z, d[k]);
+#pragma unroll
+    for (int k = 0; k < N; k++) c[k] = __builtin_elementwise_max(d[k], c[k]);  // |d|
+#pragma unroll
+    for (int k = 0; k < N; k++) c[k] = c[k] >> sh;
+#pragma unroll
+    for (int k = 0; k < N; k++) c[k] = thr - c[k];
+#pragma unroll
+    for (int k = 0; k < N; k++) c[k] = __builtin_elementwise_max(z, c[k]);  // max(0, thr - (|d| >> sh))
+#pragma unroll
+    for (int k = 0; k < N; k++) d[k] = __builtin_elementwise_max(d[k], z - c[k]);
+#pragma unroll
+    for (int k = 0; k < N; k++) d[k] = __builtin_elementwise_min(d[k], c[k]);
+}
+
+__device__ __forceinline__ s16x2 cdef_round(s16x2 px, s16x2 sum) {
+    return px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
 }
 
 __device__ __forceinline__ s16x2 cdef_px2(const int16_t *t, const int op[2], const int os0[2], const int os1[2],
@@ -148,15 +168,18 @@ __device__ __forceinline__ s16x2 cdef_px2(const int16_t *t, const int op[2], con
     // They keep their source order, so every tap is read before the first use.
     auto ld = [&](int o) -> s16x2 { return (s16x2){lds_load_i16(t + o), lds_load_i16(t + o + 1)}; };
     const s16x2 ps = splat2(pri), pss = splat2(pri_shift), ss = splat2(sec), sss = splat2(sec_shift);
-    s16x2 sum = {0, 0};
-    if (pri && sec) {
-        const s16x2 px = ld(0);
-        s16x2 pv[4], sv[8];
+    // primary taps: k = 0 (weight pri_tap), k = 1 (weight (pri_tap & 3) | 2);
+    // secondary taps: k = 0 (weight 2), k = 1 (weight 1).  16-bit sums wrap
+    // like the reference's (|sum| < 2^15), so their order does not matter.
+    const s16x2 tk0 = splat2(pri_tap), tk1 = splat2((pri_tap & 3) | 2);
+    auto load_pri = [&](s16x2(&pv)[4]) {
 #pragma unroll
         for (int k = 0; k < 2; k++) {
             pv[2 * k] = ld(op[k]);
             pv[2 * k + 1] = ld(-op[k]);
         }
+    };
+    auto load_sec = [&](s16x2(&sv)[8]) {
 #pragma unroll
         for (int k = 0; k < 2; k++) {
             sv[4 * k] = ld(os0[k]);
@@ -164,59 +187,56 @@ __device__ __forceinline__ s16x2 cdef_px2(const int16_t *t, const int op[2], con
             sv[4 * k + 2] = ld(os1[k]);
             sv[4 * k + 3] = ld(-os1[k]);
         }
-        auto dif = [&](s16x2 v) { return __builtin_elementwise_sub_sat(v, px); };
-        s16x2 tk = splat2(pri_tap), mx = px, mn = px;
+    };
+    auto pri_sum = [&](const s16x2(&c)[4]) { return tk0 * (c[0] + c[1]) + tk1 * (c[2] + c[3]); };
+    auto sec_sum = [&](const s16x2(&c)[8]) {
+        return (((c[0] + c[1]) + (c[2] + c[3])) << (short)1) + ((c[4] + c[5]) + (c[6] + c[7]));
+    };
+    if (pri && sec) {
+        const s16x2 px = ld(0);
+        s16x2 pv[4], sv[8], dp[4], dsv[8];
+        load_pri(pv);
+        load_sec(sv);
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-            sum += tk * constrain2(dif(pv[2 * k]), ps, pss);
-            sum += tk * constrain2(dif(pv[2 * k + 1]), ps, pss);
-            tk = (tk & (short)3) | (short)2;
-            mn = __builtin_elementwise_min(mn, __builtin_elementwise_min(pv[2 * k], pv[2 * k + 1]));
-            mx = __builtin_elementwise_max(mx, __builtin_elementwise_max(pv[2 * k], pv[2 * k + 1]));
-            const s16x2 st = splat2(2 - k);
+        for (int k = 0; k < 4; k++) dp[k] = __builtin_elementwise_sub_sat(pv[k], px);
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                sum += st * constrain2(dif(sv[4 * k + j]), ss, sss);
-                mn = __builtin_elementwise_min(mn, sv[4 * k + j]);
-                mx = __builtin_elementwise_max(mx, sv[4 * k + j]);
-            }
+        for (int k = 0; k < 8; k++) dsv[k] = __builtin_elementwise_sub_sat(sv[k], px);
+        constrain_n(dp, ps, pss);
+        constrain_n(dsv, ss, sss);
+        // clamp to the range of the taps read (and the sample itself)
+        s16x2 mn[6], mx[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const s16x2 a = k < 2 ? pv[2 * k] : sv[2 * k - 4], b = k < 2 ? pv[2 * k + 1] : sv[2 * k - 3];
+            mn[k] = __builtin_elementwise_min(a, b);
+            mx[k] = __builtin_elementwise_max(a, b);
         }
-        const s16x2 v = px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
-        return __builtin_elementwise_min(__builtin_elementwise_max(v, mn), mx);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            mn[k] = __builtin_elementwise_min(mn[k], mn[k + 3]);
+            mx[k] = __builtin_elementwise_max(mx[k], mx[k + 3]);
+        }
+        const s16x2 lo = __builtin_elementwise_min(__builtin_elementwise_min(mn[0], mn[1]), __builtin_elementwise_min(mn[2], px));
+        const s16x2 hi = __builtin_elementwise_max(__builtin_elementwise_max(mx[0], mx[1]), __builtin_elementwise_max(mx[2], px));
+        const s16x2 v = cdef_round(px, pri_sum(dp) + sec_sum(dsv));
+        return __builtin_elementwise_min(__builtin_elementwise_max(v, lo), hi);
     }
     if (pri) {
         const s16x2 px = ld(0);
         s16x2 pv[4];
+        load_pri(pv);
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-            pv[2 * k] = ld(op[k]);
-            pv[2 * k + 1] = ld(-op[k]);
-        }
-        s16x2 tk = splat2(pri_tap);
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            sum += tk * constrain2(__builtin_elementwise_sub_sat(pv[2 * k], px), ps, pss);
-            sum += tk * constrain2(__builtin_elementwise_sub_sat(pv[2 * k + 1], px), ps, pss);
-            tk = (tk & (short)3) | (short)2;
-        }
-        return px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
+        for (int k = 0; k < 4; k++) pv[k] = __builtin_elementwise_sub_sat(pv[k], px);
+        constrain_n(pv, ps, pss);
+        return cdef_round(px, pri_sum(pv));
     }
     const s16x2 px = ld(0);
     s16x2 sv[8];
+    load_sec(sv);
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-        sv[4 * k] = ld(os0[k]);
-        sv[4 * k + 1] = ld(-os0[k]);
-        sv[4 * k + 2] = ld(os1[k]);
-        sv[4 * k + 3] = ld(-os1[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        const s16x2 st = splat2(2 - k);
-#pragma unroll
-        for (int j = 0; j < 4; j++) sum += st * constrain2(__builtin_elementwise_sub_sat(sv[4 * k + j], px), ss, sss);
-    }
-    return px + ((sum + (sum >> (short)15) + (short)8) >> (short)4);
+    for (int k = 0; k < 8; k++) sv[k] = __builtin_elementwise_sub_sat(sv[k], px);
+    constrain_n(sv, ss, sss);
+    return cdef_round(px, sec_sum(sv));
 }
 
 constexpr int kThreads = 256;

@@ -462,6 +462,9 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
                     const s16x2 w1 = __builtin_bit_cast(s16x2, wpk[9 + q2]);
                     const s16x2 w2 = __builtin_bit_cast(s16x2, wpk[18 + q2]);
                     constexpr s16x2 lo = {-512, -512}, hi = {511, 511};
+                    // stage by stage over the four pixels: consecutive packed
+                    // operations are independent (a dependent pair would need a wait state)
+                    s16x2 da[4], db[4], v[4];
 #pragma unroll
                     for (int p = 0; p < 4; p++) {
                         const short m = (short)VAT(pp[p], 0, 0);
@@ -470,13 +473,26 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
                                          (short)VAT(pp[p], -kGdfT[2 * q2 + 1].dy, -kGdfT[2 * q2 + 1].dx)};
                         const s16x2 B = {(short)VAT(pp[p], kGdfT[2 * q2].dy, kGdfT[2 * q2].dx),
                                          (short)VAT(pp[p], kGdfT[2 * q2 + 1].dy, kGdfT[2 * q2 + 1].dx)};
-                        s16x2 da = (A - mm) << (short)up, db = (B - mm) << (short)up;
-                        da = __builtin_elementwise_min(__builtin_elementwise_max(da, nal), al);
-                        db = __builtin_elementwise_min(__builtin_elementwise_max(db, nal), al);
-                        const s16x2 v = __builtin_elementwise_min(__builtin_elementwise_max(da + db, lo), hi);
-                        iv[p][0] = __builtin_amdgcn_sdot2(v, w0, iv[p][0], false);
-                        iv[p][1] = __builtin_amdgcn_sdot2(v, w1, iv[p][1], false);
-                        iv[p][2] = __builtin_amdgcn_sdot2(v, w2, iv[p][2], false);
+                        da[p] = (A - mm) << (short)up;
+                        db[p] = (B - mm) << (short)up;
+                    }
+#pragma unroll
+                    for (int p = 0; p < 4; p++) da[p] = __builtin_elementwise_max(da[p], nal);
+#pragma unroll
+                    for (int p = 0; p < 4; p++) db[p] = __builtin_elementwise_max(db[p], nal);
+#pragma unroll
+                    for (int p = 0; p < 4; p++) da[p] = __builtin_elementwise_min(da[p], al);
+#pragma unroll
+                    for (int p = 0; p < 4; p++) db[p] = __builtin_elementwise_min(db[p], al);
+#pragma unroll
+                    for (int p = 0; p < 4; p++) v[p] = __builtin_elementwise_max(da[p] + db[p], lo);
+#pragma unroll
+                    for (int p = 0; p < 4; p++) v[p] = __builtin_elementwise_min(v[p], hi);
+#pragma unroll
+                    for (int p = 0; p < 4; p++) {
+                        iv[p][0] = __builtin_amdgcn_sdot2(v[p], w0, iv[p][0], false);
+                        iv[p][1] = __builtin_amdgcn_sdot2(v[p], w1, iv[p][1], false);
+                        iv[p][2] = __builtin_amdgcn_sdot2(v[p], w2, iv[p][2], false);
                     }
                 }
 #pragma unroll
