@@ -56,6 +56,10 @@ int batch_init_device();
 struct EnqueueCtx {
     hipStream_t s, side;
     hipEvent_t ev_fork, ev_join;
+    // optional: a second stream that runs the 8-bit group's passes beside the
+    // 10-bit group's (untimed, unprofiled runs only)
+    hipStream_t side2 = nullptr;
+    hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
     int time_pass = -1;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     Rav2dGpuCtx *profile = nullptr;  // context whose profiling events bracket each pass

@@ -166,6 +166,15 @@ void pack_batch(const HostBatch &hb, uint8_t *host, uint8_t *dev, DevBatch &db) 
 int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain, const EnqueueCtx &ec) {
     const hipStream_t s = ec.s;
     const bool grain = apply_grain && (db.total[RAV2D_PASS_GRAIN][0] + db.total[RAV2D_PASS_GRAIN][1]) > 0;
+    // the two bit-depth groups are independent chains: with a second stream they
+    // run side by side and fill each other's launch gaps and tails
+    bool both = false;
+    for (int p = 0; p < kPasses; p++) both = both || (db.total[p][0] > 0 && db.total[p][1] > 0);
+    const bool par = both && ec.side2 && ec.time_pass < 0 && !ec.profile;
+    if (par) {
+        HIP_TRY(hipEventRecord(ec.ev_fork2, s));
+        HIP_TRY(hipStreamWaitEvent(ec.side2, ec.ev_fork2, 0));
+    }
     if (grain && db.n_setup) {
         HIP_TRY(hipEventRecord(ec.ev_fork, s));
         HIP_TRY(hipStreamWaitEvent(ec.side, ec.ev_fork, 0));
@@ -176,11 +185,15 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
     for (int p = 0; p < kPasses; p++) {
         if (db.total[p][0] + db.total[p][1] == 0) continue;
         if (p == RAV2D_PASS_GRAIN && !grain) continue;
-        if (p == RAV2D_PASS_GRAIN && db.n_setup) HIP_TRY(hipStreamWaitEvent(s, ec.ev_join, 0));
+        if (p == RAV2D_PASS_GRAIN && db.n_setup) {
+            HIP_TRY(hipStreamWaitEvent(s, ec.ev_join, 0));
+            if (par) HIP_TRY(hipStreamWaitEvent(ec.side2, ec.ev_join, 0));
+        }
         if (p == ec.time_pass) HIP_TRY(hipEventRecord(ec.ev_a, s));
         PassTimer pt(ec.profile, p, s);
         for (int g = 0; g < 2; g++) {
             if (db.total[p][g] == 0) continue;
+            const hipStream_t sg = (par && g == 0) ? ec.side2 : s;
             BatchSched sc;
             sc.prefix = db.prefix[p][g];
             sc.n_frames = db.nf[p][g];
@@ -189,15 +202,19 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
             const int grid = grid_of(db.total[p][g], shape, p);
             const int bd = g ? 10 : 8;
             switch (p) {
-                case RAV2D_PASS_DEBLOCK_V: launch_deblock_batch(true, db.dbv[g], sc, grid, bd, s); break;
-                case RAV2D_PASS_DEBLOCK_H: launch_deblock_batch(false, db.dbh[g], sc, grid, bd, s); break;
-                case RAV2D_PASS_CDEF: launch_cdef_batch(db.cdef[g], sc, grid, bd, s); break;
-                case RAV2D_PASS_LR: launch_lr_batch(db.lr[g], sc, grid, bd, s); break;
-                default: launch_grain_apply_batch(db.gapply[g], sc, grid, bd, s); break;
+                case RAV2D_PASS_DEBLOCK_V: launch_deblock_batch(true, db.dbv[g], sc, grid, bd, sg); break;
+                case RAV2D_PASS_DEBLOCK_H: launch_deblock_batch(false, db.dbh[g], sc, grid, bd, sg); break;
+                case RAV2D_PASS_CDEF: launch_cdef_batch(db.cdef[g], sc, grid, bd, sg); break;
+                case RAV2D_PASS_LR: launch_lr_batch(db.lr[g], sc, grid, bd, sg); break;
+                default: launch_grain_apply_batch(db.gapply[g], sc, grid, bd, sg); break;
             }
             HIP_TRY(hipGetLastError());
         }
         if (p == ec.time_pass) HIP_TRY(hipEventRecord(ec.ev_b, s));
+    }
+    if (par) {
+        HIP_TRY(hipEventRecord(ec.ev_join2, ec.side2));
+        HIP_TRY(hipStreamWaitEvent(s, ec.ev_join2, 0));
     }
     return RAV2D_OK;
 }
@@ -215,8 +232,9 @@ struct Rav2dGpuBatch {
     DevBatch db;
     uint8_t *dmem = nullptr;  // packed argument block
     size_t dbytes = 0;
-    hipStream_t side = nullptr, cap = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_a = nullptr, ev_b = nullptr;
+    hipStream_t side = nullptr, side2 = nullptr, cap = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr, ev_a = nullptr,
+               ev_b = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -287,6 +305,9 @@ int enqueue(Rav2dGpuBatch *b, hipStream_t s, int time_pass, bool timers) {
     ec.side = b->side;
     ec.ev_fork = b->ev_fork;
     ec.ev_join = b->ev_join;
+    ec.side2 = b->side2;
+    ec.ev_fork2 = b->ev_fork2;
+    ec.ev_join2 = b->ev_join2;
     ec.time_pass = time_pass;
     ec.ev_a = b->ev_a;
     ec.ev_b = b->ev_b;
@@ -364,9 +385,12 @@ int rav2d_gpu_batch_create(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n
     else
         rav2d_gpu_batch_default_config(&b->cfg);
     if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side2, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev_fork, kStreamSyncEventFlags) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev_join, kStreamSyncEventFlags) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_fork2, kStreamSyncEventFlags) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join2, kStreamSyncEventFlags) != hipSuccess ||
         hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess) {
         set_last_error("stream/event creation failed");
         rav2d_gpu_batch_destroy(b);
@@ -388,8 +412,9 @@ int rav2d_gpu_batch_destroy(Rav2dGpuBatch *b) {
     drop_graph(b);
     drop_args(b);
     if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->side2) (void)hipStreamDestroy(b->side2);
     if (b->cap) (void)hipStreamDestroy(b->cap);
-    for (hipEvent_t e : {b->ev_fork, b->ev_join, b->ev_a, b->ev_b})
+    for (hipEvent_t e : {b->ev_fork, b->ev_join, b->ev_fork2, b->ev_join2, b->ev_a, b->ev_b})
         if (e) (void)hipEventDestroy(e);
     delete b;
     return RAV2D_OK;

@@ -274,8 +274,76 @@ __device__ __forceinline__ int ccso_offset(const CcsoDev &c, const int8_t *lut, 
     return lut[idx & 0xff];
 }
 
+// Staging of one superblock, split so that a workgroup can have the next
+// superblock's loads in flight while it filters the current one.
+constexpr int kQY = LY / 4, kQC = LC / 4;
+constexpr int kNY = LYR * kQY, kNC = 2 * LCR * kQC;
+constexpr int kPY = (kNY + kThreads - 1) / kThreads, kPC = (kNC + kThreads - 1) / kThreads;
+
+struct CdefLoads {
+    uint2 vy[kPY], vc[kPC];  // F2 groups of the luma / chroma tiles (INT16_MIN outside the frame)
+    uint32_t lutw, offw;     // words of the CCSO offset tables and of the tap offsets
+    int cdef_idx, ccso_pl;   // side info of the superblock
+    uint32_t noskip_b;       // noskip row of the 8x8 block of lane t & 63 (block row (t >> 3) & 7)
+};
+
+// Every global load of a lane for superblock (sbx, sby), issued together.
+// F2 is read in 4-sample groups (tile origins and plane widths are multiples
+// of 4, so a group is wholly inside or outside the frame) from clamped,
+// always valid addresses, then selected.
 template <typename P>
-__device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
+__device__ __forceinline__ void cdef_load(const CdefArgs &a, int sbx, int sby, CdefLoads &L) {
+    const int t = threadIdx.x;
+    const int x0 = sbx * 64, y0 = sby * 64;
+    const int fw = a.g.bw * 4, fh = a.g.bh * 4;
+    const int cw = fw >> 1, ch = fh >> 1;
+    const Rav2dAv2Filter &m = a.mask[(sby >> 2) * a.g.sb256w + (sbx >> 2)];
+    L.cdef_idx = m.cdef_idx[((sby & 3) << 2) + (sbx & 3)];
+    L.ccso_pl = (m.ccso[0] != 0) | ((m.ccso[1] != 0) << 1) | ((m.ccso[2] != 0) << 2);
+    L.noskip_b = a.on_skiptx ? 0xffffu : m.noskip_mask[(((y0 >> 3) + ((t >> 3) & 7)) & 31)][sbx & 3];
+    constexpr uint32_t kOutside = 0x80008000u;  // INT16_MIN x 2
+#pragma unroll
+    for (int j = 0; j < kPY; j++) {
+        const int i = t + j * kThreads;
+        const int r = i / kQY, gx = x0 - CX + (i - r * kQY) * 4, gy = y0 - 2 + r;
+        const uint2 v = load4(row_ptr<const P>(a.src[0], iclip(gy, 0, fh - 1)) + iclip(gx, 0, fw - 4));
+        const bool in = gx >= 0 && gx < fw && gy >= 0 && gy < fh;
+        L.vy[j].x = in ? v.x : kOutside;  // per word: a vector select would go through scratch
+        L.vy[j].y = in ? v.y : kOutside;
+    }
+#pragma unroll
+    for (int j = 0; j < kPC; j++) {
+        const int i = t + j * kThreads;
+        const int pl = i >= LCR * kQC;
+        const int k = i - pl * LCR * kQC;
+        const int r = k / kQC, gx = (x0 >> 1) - CX + (k - r * kQC) * 4, gy = (y0 >> 1) - 2 + r;
+        const uint2 v = load4(row_ptr<const P>(a.src[1 + (pl & 1)], iclip(gy, 0, ch - 1)) + iclip(gx, 0, cw - 4));
+        const bool in = gx >= 0 && gx < cw && gy >= 0 && gy < ch;
+        L.vc[j].x = in ? v.x : kOutside;
+        L.vc[j].y = in ? v.y : kOutside;
+    }
+    // the CCSO offset tables (3 x 256 B) and the tap offsets (96 B) of the argument block
+    L.lutw = t < 3 * 64 ? ((const uint32_t *)a.ccso[t >> 6].lut)[t & 63] : 0u;
+    L.offw = t < 24 ? ((const uint32_t *)&a.off[0][0][0])[t] : 0u;
+}
+
+// The LDS stores of a superblock's staging (after every lane is done with the
+// previous superblock's LDS contents).
+__device__ __forceinline__ void cdef_store(const CdefLoads &L, CdefSmem &sm) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kPY; j++)
+        if (t + j * kThreads < kNY) ((uint2 *)sm.ly)[t + j * kThreads] = L.vy[j];
+#pragma unroll
+    for (int j = 0; j < kPC; j++)
+        if (t + j * kThreads < kNC) ((uint2 *)&sm.lc[0][0])[t + j * kThreads] = L.vc[j];
+    if (t < 3 * 64) ((uint32_t *)&sm.ccso_lut[0][0])[t] = L.lutw;
+    if (t < 24) ((uint32_t *)&sm.off[0][0][0])[t] = L.offw;
+}
+
+// Direction search, filtering and CCSO of a superblock staged by cdef_store.
+template <typename P>
+__device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby, const CdefLoads &L, CdefSmem &sm) {
     int16_t *ly = sm.ly;
     int16_t(*lc)[LCR * LC] = sm.lc;
     uint8_t *b_state = sm.b_state, *b_dir = sm.b_dir;
@@ -285,59 +353,9 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
     const int fw = a.g.bw * 4, fh = a.g.bh * 4;
     const int cw = fw >> 1, ch = fh >> 1;
     const int bdm8 = a.g.bitdepth - 8;
-    // side info of the superblock first, so that its loads overlap the staging
-    const Rav2dAv2Filter &m = a.mask[(sby >> 2) * a.g.sb256w + (sbx >> 2)];
-    const int sb64_idx = ((sby & 3) << 2) + (sbx & 3);
-    const int cdef_idx_v = m.cdef_idx[sb64_idx];
-    const int ccso_pl_v = (m.ccso[0] != 0) | ((m.ccso[1] != 0) << 1) | ((m.ccso[2] != 0) << 2);
     const uint64_t y_str = *(const uint64_t *)a.y_strength, uv_str = *(const uint64_t *)a.uv_strength;
-    // noskip row of the 8x8 block of lane t & 63 (block row (t >> 3) & 7)
-    const uint16_t noskip_b = a.on_skiptx ? 0xffff : m.noskip_mask[(((y0 >> 3) + ((t >> 3) & 7)) & 31)][sbx & 3];
-    // ---- staging of F2 with INT16_MIN outside the frame; every load of a lane
-    // is issued before its first LDS store
-    {
-        // 4-sample groups (tile origins and plane widths are multiples of 4, so a
-        // group is wholly inside or outside the frame), loaded from clamped,
-        // always valid addresses, then selected
-        constexpr int QY = LY / 4, QC = LC / 4;
-        constexpr int NY = LYR * QY, NC = 2 * LCR * QC;
-        constexpr int PY = (NY + kThreads - 1) / kThreads, PC = (NC + kThreads - 1) / kThreads;
-        constexpr uint32_t kOutside = 0x80008000u;  // INT16_MIN x 2
-        uint2 vy[PY], vc[PC];
-#pragma unroll
-        for (int j = 0; j < PY; j++) {
-            const int i = t + j * kThreads;
-            const int r = i / QY, gx = x0 - CX + (i - r * QY) * 4, gy = y0 - 2 + r;
-            const uint2 v = load4(row_ptr<const P>(a.src[0], iclip(gy, 0, fh - 1)) + iclip(gx, 0, fw - 4));
-            const bool in = gx >= 0 && gx < fw && gy >= 0 && gy < fh;
-            vy[j].x = in ? v.x : kOutside;  // per word: a vector select would go through scratch
-            vy[j].y = in ? v.y : kOutside;
-        }
-#pragma unroll
-        for (int j = 0; j < PC; j++) {
-            const int i = t + j * kThreads;
-            const int pl = i >= LCR * QC;
-            const int k = i - pl * LCR * QC;
-            const int r = k / QC, gx = (x0 >> 1) - CX + (k - r * QC) * 4, gy = (y0 >> 1) - 2 + r;
-            const uint2 v =
-                load4(row_ptr<const P>(a.src[1 + (pl & 1)], iclip(gy, 0, ch - 1)) + iclip(gx, 0, cw - 4));
-            const bool in = gx >= 0 && gx < cw && gy >= 0 && gy < ch;
-            vc[j].x = in ? v.x : kOutside;
-            vc[j].y = in ? v.y : kOutside;
-        }
-        // the CCSO offset tables (3 x 256 B) and the tap offsets (96 B) of the argument block
-        uint32_t lutw = 0, offw = 0;
-        if (t < 3 * 64) lutw = ((const uint32_t *)a.ccso[t >> 6].lut)[t & 63];
-        if (t < 24) offw = ((const uint32_t *)&a.off[0][0][0])[t];
-#pragma unroll
-        for (int j = 0; j < PY; j++)
-            if (t + j * kThreads < NY) ((uint2 *)ly)[t + j * kThreads] = vy[j];
-#pragma unroll
-        for (int j = 0; j < PC; j++)
-            if (t + j * kThreads < NC) ((uint2 *)&lc[0][0])[t + j * kThreads] = vc[j];
-        if (t < 3 * 64) ((uint32_t *)&sm.ccso_lut[0][0])[t] = lutw;
-        if (t < 24) ((uint32_t *)&sm.off[0][0][0])[t] = offw;
-    }
+    const uint32_t noskip_b = L.noskip_b;
+    const int cdef_idx_v = L.cdef_idx, ccso_pl_v = L.ccso_pl;
     // uniform side info (read after the staging loads are in flight)
     const int cdef_idx = __builtin_amdgcn_readfirstlane(cdef_idx_v);
     const int ccso_pl = __builtin_amdgcn_readfirstlane(ccso_pl_v);
@@ -502,6 +520,14 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
             }
         }
     }
+}
+
+template <typename P>
+__device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
+    CdefLoads L;
+    cdef_load<P>(a, sbx, sby, L);
+    cdef_store(L, sm);
+    cdef_filter<P>(a, sbx, sby, L, sm);
 }
 
 template <typename P>
