@@ -111,6 +111,7 @@ struct Job {
 
 struct Slot {
     hipStream_t s = nullptr;
+    hipStream_t side = nullptr;  // the slot's grain set-ups (slots do not queue behind each other's)
     hipEvent_t ev_done = nullptr, ev_fork = nullptr, ev_join = nullptr;
     bool busy = false;
     std::vector<Job> jobs;
@@ -135,7 +136,6 @@ struct Rav2dGpuServer {
     std::deque<Job> pending;
     std::deque<Rav2dServeResult> done;
     std::vector<Slot> slots;
-    hipStream_t side = nullptr;
     int64_t submitted = 0, completed = 0;
     bool stop = false;
 
@@ -202,7 +202,7 @@ int launch(Rav2dGpuServer *sv, Slot &sl) {
     HIP_TRY(hipMemcpyAsync(sl.d_args, sl.h_args, bytes, hipMemcpyHostToDevice, sl.s));
     EnqueueCtx ec;
     ec.s = sl.s;
-    ec.side = sv->side;
+    ec.side = sl.side;
     ec.ev_fork = sl.ev_fork;
     ec.ev_join = sl.ev_join;
     LaunchShape shape;
@@ -350,9 +350,10 @@ int rav2d_gpu_server_create(Rav2dGpuCtx *ctx, const Rav2dServeConfig *cfg, Rav2d
     sv->cfg = c;
     sv->t0 = Clock::now();
     sv->slots.resize(c.max_inflight);
-    bool ok = hipStreamCreateWithFlags(&sv->side, hipStreamNonBlocking) == hipSuccess;
+    bool ok = true;
     for (Slot &sl : sv->slots) {
         ok = ok && hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking) == hipSuccess;
+        ok = ok && hipStreamCreateWithFlags(&sl.side, hipStreamNonBlocking) == hipSuccess;
         ok = ok && hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming) == hipSuccess;
         ok = ok && hipEventCreateWithFlags(&sl.ev_fork, kStreamSyncEventFlags) == hipSuccess;
         ok = ok && hipEventCreateWithFlags(&sl.ev_join, kStreamSyncEventFlags) == hipSuccess;
@@ -379,6 +380,7 @@ int rav2d_gpu_server_destroy(Rav2dGpuServer *sv) {
     (void)hipDeviceSynchronize();
     for (Slot &sl : sv->slots) {
         if (sl.s) (void)hipStreamDestroy(sl.s);
+        if (sl.side) (void)hipStreamDestroy(sl.side);
         for (hipEvent_t e : {sl.ev_done, sl.ev_fork, sl.ev_join})
             if (e) (void)hipEventDestroy(e);
         for (void *p : {(void *)sl.d_args, (void *)sl.d_sum, (void *)sl.d_out, (void *)sl.d_prefix})
@@ -386,7 +388,6 @@ int rav2d_gpu_server_destroy(Rav2dGpuServer *sv) {
         for (void *p : {(void *)sl.h_args, (void *)sl.h_sum, (void *)sl.h_out, (void *)sl.h_prefix})
             if (p) (void)hipHostFree(p);
     }
-    if (sv->side) (void)hipStreamDestroy(sv->side);
     delete sv;
     return RAV2D_OK;
 }
