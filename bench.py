@@ -3,8 +3,10 @@
 
 One step = the whole in-loop filter chain (plus film grain when the frame has
 grain) over one synthetic 10-bit 4:2:0 3840x2160 frame whose input planes are
-already resident in HBM.  Prints one JSON line (see DESIGN.md for the metric,
-the algorithmic byte counts and the CPU baseline).
+already resident in HBM.  Prints one JSON line: Mpix/s of displayed samples,
+the roofline of the dominant pass (algorithmic bytes = each pass reads and
+writes its planes once) and the CPU baseline (the C restatement under oracle/
+on a bounded sample of the same frame).
 
 A second leg times a mixed batch of 64 frames (sizes uniform over 360p..2160p,
 8/10-bit 50/50, grain on the 10-bit frames) as 64 sequential per-frame

@@ -31,7 +31,7 @@ def build_gpu(jobs=8):
     objs = []
     flags = [
         "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
-        # no packed-fp32 instruction selection on this target
+        # keep the compiler from selecting packed-fp32 instructions
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wno-unused-command-line-argument", "-Wno-missing-braces",
     ]

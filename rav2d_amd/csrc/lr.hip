@@ -72,6 +72,33 @@ typedef short s16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ int s16(int v) { return (int)(int16_t)v; }
 constexpr int CS = 33;      // stride of the GDF gradient cells
 
+// GDF error-table index of each pixel's three weighted sums (gdf_prep,
+// looprestoration.rs:1620-1630): sum * G (G = 8 intra, 5 inter) rounded half
+// away from zero at 2^15, clipped to [-G, G-1], offset by G and combined in
+// base 2G.  The rounding is taken as a floor, (s + 2^14 - [s < 0]) >> 15,
+// with the factor 8 folded into the shift; the constant part of the offsets,
+// G (4G^2 + 2G + 1), moves into the table base (kGdfErrBase*).
+template <int G> __device__ __forceinline__ void gdf_index(const int (&iv)[4][3], int (&idx)[4]) {
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        int v[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int x = iv[p][k];
+            int q;
+            if constexpr (G == 8) {
+                q = (x + (1 << 11) + (x >> 31)) >> 12;
+            } else {
+                const int s = (x << 2) + x;
+                q = (s + (1 << 14) + (s >> 31)) >> 15;
+            }
+            v[k] = dmin(dmax(q, -G), G - 1);
+        }
+        idx[p] = __mul24(v[0], 4 * G * G) + __mul24(v[1], 2 * G) + v[2];
+    }
+}
+constexpr int kGdfErrBase8 = 8 * (4 * 64 + 2 * 8 + 1), kGdfErrBase5 = 5 * (4 * 25 + 2 * 5 + 1);
+
 struct LrSmem {
     alignas(16) uint16_t V[VH * VW];  // virtual image, rows [-E, 64 + E), columns [-VC, 64 + VC)
     union {
@@ -88,6 +115,7 @@ struct LrSmem {
     int8_t s_mf[16 * 16];               // multi-class frame filters of the plane
     uint32_t apk[4 * 9];                // GDF alpha of taps (2i, 2i+1) per class, packed 16-bit pairs
     uint32_t wpk[4 * 27];               // GDF weights likewise: [class][weight set][pair]
+    int16_t s_bias[4];                  // GDF bias of the three weight sets for (ref, qp)
     LrItem it;
 };
 
@@ -124,7 +152,9 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     auto &f3 = a.f3[pl];
     auto &f2 = a.f2[pl];
     auto &dst = a.dst[pl];
-    const int bd = a.g.bitdepth, bdm8 = bd - 8, bdmax = a.g.bdmax;
+    // 8-bit frames run with P = uint8_t, 10-bit frames with P = uint16_t (the
+    // only depths the runtime accepts), so clip bounds and shifts are constants
+    constexpr int bd = sizeof(P) == 1 ? 8 : 10, bdm8 = bd - 8, bdmax = (1 << bd) - 1;
     // every buffer of a plane has the same row stride; rows and strides are
     // below 2^24, so row offsets are 24-bit products
     const uint32_t stride = (uint32_t)f3.stride;
@@ -188,10 +218,12 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     // tables of this item's filter kinds, 16 (or 8) bytes per lane
     uint4 pcv = {0, 0, 0, 0}, alv = {0, 0, 0, 0}, wv = {0, 0, 0, 0}, erv = {0, 0, 0, 0};
     uint2 erv2 = {0, 0};
+    int biasv = 0;
     if (kind == LR_KIND_PC && t < 64 * 13 * 2 / 16) pcv = ((const uint4 *)(a.t.pc_filters + a.widx * 64 * 13))[t];
     if (gdf) {
         if (t < 88 * 2 / 16) alv = ((const uint4 *)(a.t.gdf_alpha + a.gdf_ref * 528 + a.gdf_qp * 88))[t];
         if (t < 264 * 2 / 16) wv = ((const uint4 *)(a.t.gdf_weight + a.gdf_ref * 1584 + a.gdf_qp * 264))[t];
+        if (t < 3) biasv = a.t.gdf_bias[(a.gdf_ref * 6 + a.gdf_qp) * 3 + t];
         if (a.gdf_ref == 0)
             erv = ((const uint4 *)(a.t.gdf_intra + a.gdf_qp * 4096))[t];  // 4096 B: one uint4 per lane
         else if (t < 1000 / 8)
@@ -258,6 +290,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     if (gdf) {
         if (t < 88 * 2 / 16) ((uint4 *)sm.s_alpha)[t] = alv;
         if (t < 264 * 2 / 16) ((uint4 *)sm.s_weight)[t] = wv;
+        if (t < 3) sm.s_bias[t] = (int16_t)biasv;
         if (a.gdf_ref == 0)
             ((uint4 *)sm.s_err)[t] = erv;
         else if (t < 1000 / 8)
@@ -357,8 +390,12 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     __syncthreads();
     // ---- outputs, one 2x2 quad per lane: a quad shares its 4x4 block (class,
     // lossless flag) and its GDF cell sums (class c and gradient features)
-    const int sh = 12 - bd, rnd_g = (1 << sh) >> 1;
-    const int gscale = a.gdf_ref == 0 ? 8 : 5;
+    const bool intra = a.gdf_ref == 0;
+    int gbias[3] = {0, 0, 0};
+    if (gdf) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) gbias[k] = __builtin_amdgcn_readfirstlane(sm.s_bias[k]);
+    }
     int filt[18];
     if (kind == LR_KIND_SINGLE) {
 #pragma unroll
@@ -433,10 +470,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
 #pragma unroll
                 for (int d = 0; d < 4; d++) gs[d] = cp[d] + cp[4 + d] + cp[CS * 4 + d] + cp[CS * 4 + 4 + d];
                 const int c = (gs[0] <= gs[1]) | ((gs[2] <= gs[3]) << 1);
-                const int bias = a.gdf_ref * 6 + a.gdf_qp;
-                int base[3];
-#pragma unroll
-                for (int k = 0; k < 3; k++) base[k] = a.t.gdf_bias[bias * 3 + k];
+                int base[3] = {gbias[0], gbias[1], gbias[2]};
 #pragma unroll
                 for (int d = 0; d < 4; d++) {
                     const int kk = d + 18;
@@ -495,18 +529,18 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
                         iv[p][2] = __builtin_amdgcn_sdot2(v[p], w2, iv[p][2], false);
                     }
                 }
+                int idx[4];
+                if (intra)
+                    gdf_index<8>(iv, idx);
+                else
+                    gdf_index<5>(iv, idx);
+                const int8_t *err = sm.s_err + (intra ? kGdfErrBase8 : kGdfErrBase5);
 #pragma unroll
                 for (int p = 0; p < 4; p++) {
-                    int full = 0;
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        // gscale is 8 (intra) or 5 (inter): shifts instead of a 32-bit multiply
-                        const int sv = (iv[p][k] << 2) + (gscale == 8 ? iv[p][k] << 2 : iv[p][k]);
-                        int v = apply_sign((abs(sv) + (1 << 14)) >> 15, sv);
-                        full = (full << 3) + (gscale == 8 ? full << 3 : full << 1) + iclip(v, -gscale, gscale - 1) + gscale;
-                    }
-                    int diff = sm.s_err[full] * a.gdf_scale;
-                    out[p] = iclip(out[p] + apply_sign((abs(diff) + rnd_g) >> sh, diff), 0, bdmax);
+                    // (|diff| + rnd) >> sh with diff's sign, as a floor (sh = 12 - bd >= 2)
+                    constexpr int sh = 12 - bd;
+                    const int diff = __mul24((int)err[idx[p]], a.gdf_scale);
+                    out[p] = iclip(out[p] + ((diff + (1 << (sh - 1)) + (diff >> 31)) >> sh), 0, bdmax);
                 }
             }
         }
