@@ -74,6 +74,7 @@ struct LrItem {
 struct LrTables {
     const uint8_t *lut_to_class;    // PC_WIENER_LUT_TO_CLASS[4096]
     const int16_t *pc_filters;      // PC_WIENER_FILTERS[4][64][13]
+    const uint32_t *pc_pairs;       // the same as [4][64][8] packed 16-bit pairs (f0|f1, .., f10|f11, f12|0, 0)
     const uint8_t *pc_sub;          // PC_WIENER_SUB_CLASSIFY[4][256]
     const uint8_t *ns_sub;          // PC_WIENER_SUB_CLASSIFY_NS[4][7][256]
     const uint16_t *gdf_alpha;      // GDF_ALPHA[3168]

@@ -70,6 +70,42 @@ typedef short s16x2 __attribute__((ext_vector_type(2)));
 // 8/10-bit samples, clamped GDF features) so that products with filter taps
 // compile to 24-bit multiplies.
 __device__ __forceinline__ int s16(int v) { return (int)(int16_t)v; }
+__device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)(uint16_t)lo | ((uint32_t)(uint16_t)hi << 16); }
+
+// Wiener taps of a 2x2 quad two at a time: the pair of samples of taps 2i and
+// 2i+1 (sums of two samples for symmetric taps, at most 2046) in packed
+// 16-bit lanes, dotted with the packed 16-bit filter pair f[i] into each
+// pixel's 32-bit sum.  Stage by stage across the four pixels, so that no
+// packed operation waits on the one before it.  pp[p] points at pixel p in a
+// sample array of row stride S.
+template <const Tap *T, int NP, int S>
+__device__ __forceinline__ void sym_pairs(const uint16_t *const (&pp)[4], const uint32_t *f, int (&s)[4]) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+        s16x2 x[4];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const s16x2 A = {(short)pp[p][-T[2 * i].dy * S - T[2 * i].dx], (short)pp[p][-T[2 * i + 1].dy * S - T[2 * i + 1].dx]};
+            const s16x2 B = {(short)pp[p][T[2 * i].dy * S + T[2 * i].dx], (short)pp[p][T[2 * i + 1].dy * S + T[2 * i + 1].dx]};
+            x[p] = A + B;
+        }
+#pragma unroll
+        for (int p = 0; p < 4; p++) s[p] = __builtin_amdgcn_sdot2(x[p], __builtin_bit_cast(s16x2, f[i]), s[p], false);
+    }
+}
+// the same for one-sided taps (the cross-component luma taps of chroma)
+template <const Tap *T, int NP, int S>
+__device__ __forceinline__ void tap_pairs(const uint16_t *const (&pp)[4], const uint32_t *f, int (&s)[4]) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+        s16x2 x[4];
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+            x[p] = s16x2{(short)pp[p][T[2 * i].dy * S + T[2 * i].dx], (short)pp[p][T[2 * i + 1].dy * S + T[2 * i + 1].dx]};
+#pragma unroll
+        for (int p = 0; p < 4; p++) s[p] = __builtin_amdgcn_sdot2(x[p], __builtin_bit_cast(s16x2, f[i]), s[p], false);
+    }
+}
 constexpr int CS = 33;      // stride of the GDF gradient cells
 
 // GDF error-table index of each pixel's three weighted sums (gdf_prep,
@@ -111,8 +147,9 @@ struct LrSmem {
     uint8_t cls[16 * 16];            // class of each 4x4 block
     alignas(16) uint16_t s_alpha[88];
     alignas(16) int16_t s_weight[264];
-    alignas(16) int16_t s_pc[64 * 13];  // PC-Wiener filters of the frame's wiener_idx
-    int8_t s_mf[16 * 16];               // multi-class frame filters of the plane
+    alignas(16) uint32_t s_pcp[64 * 8];  // PC-Wiener filters of the frame's wiener_idx, packed tap pairs
+    alignas(16) uint32_t s_mfp[16 * 8];  // multi-class frame filters of the plane, packed tap pairs
+    int32_t s_mF[16];                    // their centre weights 128 - 2 sum(f)
     uint32_t apk[4 * 9];                // GDF alpha of taps (2i, 2i+1) per class, packed 16-bit pairs
     uint32_t wpk[4 * 27];               // GDF weights likewise: [class][weight set][pair]
     int16_t s_bias[4];                  // GDF bias of the three weight sets for (ref, qp)
@@ -219,7 +256,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     uint4 pcv = {0, 0, 0, 0}, alv = {0, 0, 0, 0}, wv = {0, 0, 0, 0}, erv = {0, 0, 0, 0};
     uint2 erv2 = {0, 0};
     int biasv = 0;
-    if (kind == LR_KIND_PC && t < 64 * 13 * 2 / 16) pcv = ((const uint4 *)(a.t.pc_filters + a.widx * 64 * 13))[t];
+    if (kind == LR_KIND_PC && t < 64 * 8 * 4 / 16) pcv = ((const uint4 *)(a.t.pc_pairs + a.widx * 64 * 8))[t];
     if (gdf) {
         if (t < 88 * 2 / 16) alv = ((const uint4 *)(a.t.gdf_alpha + a.gdf_ref * 528 + a.gdf_qp * 88))[t];
         if (t < 264 * 2 / 16) wv = ((const uint4 *)(a.t.gdf_weight + a.gdf_ref * 1584 + a.gdf_qp * 264))[t];
@@ -284,9 +321,19 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
             if (i < (h + 4) * LS) Ld[i] = lvals[j];
         }
     }
-    if (kind == LR_KIND_PC && t < 64 * 13 * 2 / 16) ((uint4 *)sm.s_pc)[t] = pcv;
-    if (kind == LR_KIND_MULTI)
-        for (int i = t; i < 16 * 16; i += kThreads) sm.s_mf[i] = a.mfilt[pl][i >> 4][i & 15];
+    if (kind == LR_KIND_PC && t < 64 * 8 * 4 / 16) ((uint4 *)sm.s_pcp)[t] = pcv;
+    if (kind == LR_KIND_MULTI && t < 16 * 8) {
+        // tap pairs of class t / 8; lane 0 of each class also sums its taps
+        const auto &f = a.mfilt[pl][t >> 3];
+        const int i = t & 7;
+        sm.s_mfp[t] = pack16(f[2 * i], f[2 * i + 1]);
+        if (i == 0) {
+            int sum = 0;
+#pragma unroll
+            for (int k = 0; k < 16; k++) sum += f[k];
+            sm.s_mF[t >> 3] = 128 - 2 * sum;
+        }
+    }
     if (gdf) {
         if (t < 88 * 2 / 16) ((uint4 *)sm.s_alpha)[t] = alv;
         if (t < 264 * 2 / 16) ((uint4 *)sm.s_weight)[t] = wv;
@@ -396,10 +443,26 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
 #pragma unroll
         for (int k = 0; k < 3; k++) gbias[k] = __builtin_amdgcn_readfirstlane(sm.s_bias[k]);
     }
-    int filt[18];
+    // The NS-Wiener sum m << 7 + sum_k f_k (A_k + B_k - 2m) is evaluated as
+    // m (128 - 2 sum_k f_k) + sum_k f_k (A_k + B_k): the same integer.  The
+    // single filter's packed tap pairs and centre weights are uniform.
+    uint32_t fs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int fc = 128, fl = 0;  // centre weight; minus the weight of the co-located luma sample (chroma)
     if (kind == LR_KIND_SINGLE) {
+        const int nc = pl == 0 ? 16 : 6;
+        int sc = 0, sl = 0;
 #pragma unroll
-        for (int k = 0; k < 18; k++) filt[k] = it.filt[k];
+        for (int k = 0; k < 18; k++) {
+            const int f = it.filt[k];
+            if (k < nc)
+                sc += f;
+            else
+                sl += f;
+        }
+        fc = __builtin_amdgcn_readfirstlane(128 - 2 * sc);
+        fl = __builtin_amdgcn_readfirstlane(sl);
+#pragma unroll
+        for (int i = 0; i < 9; i++) fs[i] = __builtin_amdgcn_readfirstlane(pack16(it.filt[2 * i], it.filt[2 * i + 1]));
     }
     const int qw = w >> 1, nq = qw * (h >> 1);
     for (int q = t; q < nq; q += kThreads) {
@@ -412,57 +475,39 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
         for (int p = 0; p < 4; p++) out[p] = VAT(pp[p], 0, 0);
         const bool lossless = (it.ll[y >> 2] >> (x >> 2)) & 1;
         if (!lossless) {
-            if (kind == LR_KIND_SINGLE && pl == 0) {
+            if (kind != LR_KIND_NONE) {
+                int s[4];
+                if (kind == LR_KIND_SINGLE && pl == 0) {
 #pragma unroll
-                for (int p = 0; p < 4; p++) {
-                    int m = out[p], s = m << 7;
+                    for (int p = 0; p < 4; p++) s[p] = __mul24(out[p], fc);
+                    sym_pairs<kNsYT, 8, VW>(pp, fs, s);
+                } else if (kind == LR_KIND_SINGLE) {
+                    const uint16_t *l0 = &Ld[(y + 2) * LS + x + 2];
+                    const uint16_t *lq[4] = {l0, l0 + 1, l0 + LS, l0 + LS + 1};
 #pragma unroll
-                    for (int k = 0; k < 16; k++)
-                        s += s16(VAT(pp[p], kNsYT[k].dy, kNsYT[k].dx) + VAT(pp[p], -kNsYT[k].dy, -kNsYT[k].dx) - 2 * m) *
-                             s16(filt[k]);
-                    out[p] = iclip((s + 64) >> 7, 0, bdmax);
+                    for (int p = 0; p < 4; p++) s[p] = __mul24(out[p], fc) - __mul24((int)lq[p][0], fl);
+                    sym_pairs<kNsUvT, 3, VW>(pp, fs, s);
+                    tap_pairs<kNsUvYT, 6, LS>(lq, fs + 3, s);
+                } else {
+                    // multi-class NS-Wiener or PC-Wiener: the filter of the quad's 4x4 block
+                    const int c = cls[(y >> 2) * 16 + (x >> 2)];
+                    const uint4 *fp = (const uint4 *)(kind == LR_KIND_MULTI ? &sm.s_mfp[c * 8] : &sm.s_pcp[c * 8]);
+                    const uint4 f0 = fp[0], f1 = fp[1];
+                    const uint32_t fm[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
+                    if (kind == LR_KIND_MULTI) {
+                        const int fcm = sm.s_mF[c];
+#pragma unroll
+                        for (int p = 0; p < 4; p++) s[p] = __mul24(out[p], fcm);
+                        sym_pairs<kNsYT, 8, VW>(pp, fm, s);
+                    } else {
+                        const int f12 = s16((int)fm[6]);  // centre tap
+#pragma unroll
+                        for (int p = 0; p < 4; p++) s[p] = __mul24(out[p], f12);
+                        sym_pairs<kPcT, 6, VW>(pp, fm, s);
+                    }
                 }
-            } else if (kind == LR_KIND_SINGLE) {
 #pragma unroll
-                for (int p = 0; p < 4; p++) {
-                    int m = out[p], s = m << 7;
-#pragma unroll
-                    for (int k = 0; k < 6; k++)
-                        s += s16(VAT(pp[p], kNsUvT[k].dy, kNsUvT[k].dx) + VAT(pp[p], -kNsUvT[k].dy, -kNsUvT[k].dx) -
-                                 2 * m) * s16(filt[k]);
-                    const uint16_t *lp = &Ld[(y + (p >> 1) + 2) * LS + x + (p & 1) + 2];
-                    int l = lp[0];
-#pragma unroll
-                    for (int k = 0; k < 12; k++) s += s16((int)lp[kNsUvYT[k].dy * LS + kNsUvYT[k].dx] - l) * s16(filt[6 + k]);
-                    out[p] = iclip((s + 64) >> 7, 0, bdmax);
-                }
-            } else if (kind == LR_KIND_MULTI) {
-                const int8_t *fl = &sm.s_mf[cls[(y >> 2) * 16 + (x >> 2)] * 16];
-                int fk[16];
-#pragma unroll
-                for (int k = 0; k < 16; k++) fk[k] = fl[k];
-#pragma unroll
-                for (int p = 0; p < 4; p++) {
-                    int m = out[p], s = m << 7;
-#pragma unroll
-                    for (int k = 0; k < 16; k++)
-                        s += s16(VAT(pp[p], kNsYT[k].dy, kNsYT[k].dx) + VAT(pp[p], -kNsYT[k].dy, -kNsYT[k].dx) - 2 * m) *
-                             s16(fk[k]);
-                    out[p] = iclip((s + 64) >> 7, 0, bdmax);
-                }
-            } else if (kind == LR_KIND_PC) {
-                const int16_t *fl = &sm.s_pc[cls[(y >> 2) * 16 + (x >> 2)] * 13];
-                int fk[13];
-#pragma unroll
-                for (int k = 0; k < 13; k++) fk[k] = fl[k];
-#pragma unroll
-                for (int p = 0; p < 4; p++) {
-                    int s = s16(out[p]) * s16(fk[12]);
-#pragma unroll
-                    for (int k = 0; k < 12; k++)
-                        s += s16(VAT(pp[p], kPcT[k].dy, kPcT[k].dx) + VAT(pp[p], -kPcT[k].dy, -kPcT[k].dx)) * s16(fk[k]);
-                    out[p] = iclip((s + 64) >> 7, 0, bdmax);
-                }
+                for (int p = 0; p < 4; p++) out[p] = iclip((s[p] + 64) >> 7, 0, bdmax);
             }
             if (gdf) {
                 const uint16_t *cp = &cells[((y >> 1) * CS + (x >> 1)) * 4];
@@ -605,6 +650,16 @@ int lr_tables(int device, LrTables *out) {
     static bool ready[16] = {false};
     if (device < 0 || device >= 16) return RAV2D_EINVAL;
     if (!ready[device]) {
+        // PC-Wiener filters as packed tap pairs for the 16-bit dot products
+        static uint32_t pc_pairs[4][64][8];
+        for (int wi = 0; wi < 4; wi++)
+            for (int c = 0; c < 64; c++) {
+                const int16_t *f = R2T_PC_WIENER_FILTERS[wi][c];
+                for (int i = 0; i < 8; i++) {
+                    const int lo = 2 * i < 13 ? f[2 * i] : 0, hi = 2 * i + 1 < 13 ? f[2 * i + 1] : 0;
+                    pc_pairs[wi][c][i] = (uint32_t)(uint16_t)lo | ((uint32_t)(uint16_t)hi << 16);
+                }
+            }
         struct Src {
             const void *p;
             size_t n;
@@ -618,15 +673,17 @@ int lr_tables(int device, LrTables *out) {
             {R2T_GDF_BIAS, sizeof(R2T_GDF_BIAS)},
             {R2T_GDF_INTRA_ERROR, sizeof(R2T_GDF_INTRA_ERROR)},
             {R2T_GDF_INTER_ERROR, sizeof(R2T_GDF_INTER_ERROR)},
+            {pc_pairs, sizeof(pc_pairs)},
         };
-        size_t total = 0, off[9];
-        for (int i = 0; i < 9; i++) {
+        constexpr int kN = sizeof(src) / sizeof(src[0]);
+        size_t total = 0, off[kN];
+        for (int i = 0; i < kN; i++) {
             off[i] = total;
             total += (src[i].n + 255) & ~(size_t)255;
         }
         uint8_t *d = nullptr;
         if (hipMalloc((void **)&d, total) != hipSuccess) return RAV2D_ENOMEM;
-        for (int i = 0; i < 9; i++)
+        for (int i = 0; i < kN; i++)
             if (hipMemcpy(d + off[i], src[i].p, src[i].n, hipMemcpyHostToDevice) != hipSuccess) return RAV2D_EDEVICE;
         LrTables &t = cache[device];
         t.lut_to_class = d + off[0];
@@ -638,6 +695,7 @@ int lr_tables(int device, LrTables *out) {
         t.gdf_bias = (const int16_t *)(d + off[6]);
         t.gdf_intra = (const int8_t *)(d + off[7]);
         t.gdf_inter = (const int8_t *)(d + off[8]);
+        t.pc_pairs = (const uint32_t *)(d + off[9]);
         ready[device] = true;
     }
     *out = cache[device];
