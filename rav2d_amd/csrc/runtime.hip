@@ -292,6 +292,20 @@ void build_lr_items(Rav2dGpuFrame *f) {
     const Rav2dFrameParams &P = f->params;
     const bool lr_stage = rp && (P.inloop_filters & (RAV2D_INLOOPFILTER_WIENER | RAV2D_INLOOPFILTER_GDF));
     if (!lr_stage) f->lr_items.clear();
+    // Items write disjoint outputs, so their order is free.  Workgroups start
+    // in item order: the longest items go first and the short ones fill the
+    // tail of the launch (cost = area x vector instructions per 2x2 quad of
+    // each tool on gfx950, plus the staging every item pays).
+    auto cost = [](const LrItem &it) {
+        int c = 50 + (it.gdf_on ? 720 : 0);
+        if (it.kind == LR_KIND_SINGLE)
+            c += it.plane ? 350 : 140;
+        else if (it.kind == LR_KIND_MULTI || it.kind == LR_KIND_PC)
+            c += 300;
+        return c * it.w * it.h;
+    };
+    std::stable_sort(f->lr_items.begin(), f->lr_items.end(),
+                     [&](const LrItem &a, const LrItem &b) { return cost(a) > cost(b); });
     f->lr_n_restore = (int)f->lr_items.size();
     for (int pl = 0; pl < 3; pl++)
         if (!lr_stage || !(rp & (1 << pl))) copy_items(f, pl, f->lr_items);
