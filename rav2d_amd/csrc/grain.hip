@@ -56,82 +56,81 @@ struct GrainSmem {
     int8_t win[3][4][7];
 };
 
-// Causal AR filter of one grain plane run by one wave as a wavefront: lane k
-// owns rows 3 + k (and 67 + k: the luma plane has 73 rows); sample (y, x) runs
-// at step x + 4y, after its three left neighbours and, on the three rows
-// above, the columns up to x + 3 (row y - 1 finished column x + 3 one step
-// earlier).  A row's seven taps are read as four aligned 16-bit pairs and
-// summed with packed dot products; x has the parity of the step for every
-// lane, so the pair alignment (and the packed tap set) is uniform per step.
-// The LDS executes one wave's operations in issue order, so a read issued
-// after another lane's write sees it; the wavefront-scope fence after each
-// step keeps the compiler from moving accesses across steps (it waits for
-// nothing), so no workgroup barrier or counter wait is needed.  Taps:
-// win[r][c] for dy = r - 3, dx = c - 3 (zero outside the lag).
+// Causal AR filter of one grain plane by one wave, as a wavefront whose
+// recent samples stay in registers.  Lane k owns row k (for the luma plane
+// also row 64 + k, k < 9, from step 128: rows k end by step 4k + 81 < 128 and
+// rows 64 + k start at step 256 + 4k).  Sample (y, x) is final at step x + 4y:
+// at step s every lane emits the value of (y, s - 4y), filtered inside the AR
+// area and the Gaussian draw outside it, and keeps four 16-entry rings indexed
+// by step: E its own emissions, A lane k-1's (row y - 1, one step late), B
+// row y - 2's (two steps late, relayed by lane k-1), C row y - 3's (three
+// steps late).  Columns x-3..x+3 of row y - 1 were emitted at steps s-7..s-1,
+// of row y - 2 at s-11..s-5, of row y - 3 at s-15..s-9, and the own row's
+// x-3..x-1 at s-3..s-1, so each step needs only its three lane rotations (DPP
+// wave_ror:1, lane k reads lane k-1) and no LDS round trip on the dependency
+// chain.  Taps: win[r][c] for dy = r - 3, dx = c - 3 (zero outside the lag).
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
+__device__ __forceinline__ int lane_up(int v) { return __builtin_amdgcn_mov_dpp(v, 0x13C, 0xF, 0xF, true); }
+
 template <bool kChroma>
-__device__ __forceinline__ void ar_wave(int16_t (*L)[GW], int h, int w, const int8_t (*win)[7], int ar_shift, int gmin,
-                                        int gmax, const int16_t (*luma)[GW], int wl) {
+__device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7], int ar_shift, int gmin, int gmax,
+                                        const int16_t (*luma)[GW], int wl) {
+    constexpr int W = kChroma ? SGW : GW, H = kChroma ? SGH : GH;
+    constexpr int kSteps = (W - 1) + 4 * (H - 1) + 1;
+    static_assert(kChroma || (4 * 8 + GW - 1 < 128 && H - 64 <= 9), "luma rows 64.. start after the switch");
     const int lane = threadIdx.x & 63;
-    auto pk = [](int lo, int hi) -> s16x2 { return (s16x2){(short)lo, (short)hi}; };
-    // tap pairs for an odd x (pairs start at column x - 3) and an even x (x - 4);
-    // row 3 is the own row (taps at dx = -3..-1, the pair holding x has weight 0)
-    s16x2 po[4][4], pe[4][4];
+    int cf[4][7];
 #pragma unroll
-    for (int r = 0; r < 3; r++) {
-        po[r][0] = pk(win[r][0], win[r][1]);
-        po[r][1] = pk(win[r][2], win[r][3]);
-        po[r][2] = pk(win[r][4], win[r][5]);
-        po[r][3] = pk(win[r][6], 0);
-        pe[r][0] = pk(0, win[r][0]);
-        pe[r][1] = pk(win[r][1], win[r][2]);
-        pe[r][2] = pk(win[r][3], win[r][4]);
-        pe[r][3] = pk(win[r][5], win[r][6]);
-    }
-    po[3][0] = pk(win[3][0], win[3][1]);
-    po[3][1] = pk(win[3][2], 0);
-    pe[3][0] = pk(0, win[3][0]);
-    pe[3][1] = pk(win[3][1], win[3][2]);
-    const int last = (w - 4) + 4 * (h - 1);
-    auto step = [&](int s, const s16x2(&P)[4][4], int odd) {
-        int y = 3 + lane, x = s - 4 * y;
-        if (x > w - 4) {  // first row done: the lane's second row
-            y += 64;
-            x = s - 4 * y;
-        }
-        if (y < h && x >= 3 && x <= w - 4) {
-            const int c0 = x - 4 + odd;  // even: every pair is 4-byte aligned
-            int acc[4];
+    for (int r = 0; r < 4; r++)
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const uint32_t *row = (const uint32_t *)&L[y - 3 + r][c0];
-                int a = 0;
+        for (int c = 0; c < 7; c++) cf[r][c] = __builtin_amdgcn_readfirstlane((int)win[r][c]);
+    int E[16], A[16], B[16], C[16];
 #pragma unroll
-                for (int q = 0; q < 4; q++) a = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, row[q]), P[r][q], a, false);
-                acc[r] = a;
-            }
-            {
-                const uint32_t *row = (const uint32_t *)&L[y][c0];
-                acc[3] = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, row[0]), P[3][0], 0, false);
-                acc[3] = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, row[1]), P[3][1], acc[3], false);
-            }
-            int sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    for (int i = 0; i < 16; i++) E[i] = A[i] = B[i] = C[i] = 0;
+    const int16_t *Lf = &L[0][0];
+    const int16_t *lf = kChroma ? &luma[0][0] : nullptr;
+    const int rnd = (1 << ar_shift) >> 1;
+    int y = lane;
+    for (int b = 0; b < (kSteps + 15) / 16; b++) {
+        if (!kChroma && b == 8 && lane < H - 64) y = lane + 64;
+        const bool rows_ok = y >= 3 && y < H;
+        // (y, s - 4y) is element y (GW - 4) + s of the plane's rows; lanes
+        // without a sample at step s read (and never write) other LDS entries
+        const int base = y * (GW - 4), xm = -4 * y - 3;
+        // co-located luma of chroma sample (y, x): rows 2y-3, 2y-2, columns 2x-3, 2x-2
+        const int lbase = kChroma ? dmax(y * (2 * GW - 8) - 3 * GW - 3, 0) : 0;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int s = b * 16 + u;
+            A[u] = lane_up(E[(u - 1) & 15]);
+            B[u] = lane_up(A[(u - 1) & 15]);
+            C[u] = lane_up(B[(u - 1) & 15]);
+            const int raw = Lf[base + s];
+            int lt = 0;
             if constexpr (kChroma) {
-                const int lx = ((x - 3) << 1) + 3, ly = ((y - 3) << 1) + 3;
-                const int l = luma[ly][lx] + luma[ly][lx + 1] + luma[ly + 1][lx] + luma[ly + 1][lx + 1];
-                sum += round2(l, 2) * wl;
+                const int i = lbase + 2 * s;
+                lt = __mul24((lf[i] + lf[i + 1] + lf[i + GW] + lf[i + GW + 1] + 2) >> 2, wl);
             }
-            L[y][x] = (int16_t)iclip(L[y][x] + round2(sum, ar_shift), gmin, gmax);
+            // four partial sums: rows y-3, y-2, y-1 and the own row
+            int a0 = 0, a1 = 0, a2 = 0, a3 = lt;
+#pragma unroll
+            for (int c = 0; c < 7; c++) {
+                a0 += __mul24(C[(u - 12 + c) & 15], cf[0][c]);
+                a1 += __mul24(B[(u - 9 + c) & 15], cf[1][c]);
+                a2 += __mul24(A[(u - 6 + c) & 15], cf[2][c]);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) a3 += __mul24(E[(u - 3 + c) & 15], cf[3][c]);
+            const int sum = (a0 + a1) + (a2 + a3);
+            const bool act = rows_ok && (unsigned)(s + xm) <= (unsigned)(W - 7);
+            int v = raw;
+            if (act) {
+                v = dmin(dmax(raw + ((sum + rnd) >> ar_shift), gmin), gmax);
+                L[0][base + s] = (int16_t)v;
+            }
+            E[u] = v;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
-    // the first step (x = 3 on row 3) is odd
-    for (int s = 3 + 4 * 3; s <= last; s += 2) {
-        step(s, po, 1);
-        if (s + 1 <= last) step(s + 1, pe, 0);
     }
 }
 
@@ -203,7 +202,7 @@ __device__ __forceinline__ void grain_setup_body(const GrainArgs &a, const int16
     __syncthreads();
     // luma AR on wave 0 while wave 1 builds the scaling LUTs
     if (t < 64) {
-        ar_wave<false>(lut[0], GH, GW, win[0], (int)d.ar_coeff_shift, gmin, gmax, nullptr, 0);
+        ar_wave<false>(lut[0], win[0], (int)d.ar_coeff_shift, gmin, gmax, nullptr, 0);
     } else {
         // scaling LUTs (generate_scaling_8bpc/hbd, filmgrain.rs:22-113), every
         // entry in closed form: the first pass writes entries at multiples of
@@ -254,7 +253,7 @@ __device__ __forceinline__ void grain_setup_body(const GrainArgs &a, const int16
         const int uv = t >> 6;
         const bool on = uv == 0 ? want_u : want_v;
         const int wl = d.num_points[0] > 0 ? coef[1 + uv][2 * lag * (lag + 1)] : 0;
-        if (on) ar_wave<true>(lut[1 + uv], SGH, SGW, win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl);
+        if (on) ar_wave<true>(lut[1 + uv], win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl);
     }
     __syncthreads();
     for (int i = t; i < 3 * 1024; i += kSetupThreads) a.scaling[i] = (&scl[0][0])[i];
