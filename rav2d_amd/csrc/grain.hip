@@ -32,28 +32,33 @@ __device__ __forceinline__ uint32_t lfsr(uint32_t bits, uint32_t &s) {
 
 __device__ __forceinline__ int round2(int x, int shift) { return (x + ((1 << shift) >> 1)) >> shift; }
 
-// LFSR jump-ahead: the 16-bit LFSR is linear over GF(2), so the state before
-// draw k*kChunk is M^(k*kChunk) applied to the seed.  kJump[k] holds that
-// matrix as 16 row masks (output bit i = parity(row_i & state)); the host
-// builds it once.
-constexpr int kChunk = 94;
-__constant__ uint16_t kJump[64][16];
+// LFSR jump-ahead: the 16-bit LFSR is linear over GF(2), so its state n
+// draws on is M^n applied to the state.  Jump tables hold such matrices as 16
+// row masks (output bit i = parity(row_i & state)); the host builds them once
+// per device.  kJump[k] = M^(k kChunk) starts lane k's share of a plane's
+// Gaussian draws; kJumpSeg[j] = M^(8 kBandSeg j) starts segment j of a row
+// band's block offsets (each block takes 8 draws).
+constexpr int kChunk = 47, kBandSeg = 16, kSegs = 128;
+__constant__ uint16_t kJump[128][16];
+__constant__ uint16_t kJumpSeg[kSegs][16];
 
-__device__ __forceinline__ uint32_t jump(int k, uint32_t s) {
+__device__ __forceinline__ uint32_t jump(const uint16_t (&m)[16], uint32_t s) {
     uint32_t r = 0;
 #pragma unroll
-    for (int i = 0; i < 16; i++) r |= (uint32_t)(__popc(kJump[k][i] & s) & 1) << i;
+    for (int i = 0; i < 16; i++) r |= (uint32_t)(__popc(m[i] & s) & 1) << i;
     return r;
 }
 
-constexpr int kSetupThreads = 128;
+// four waves: set-up phases run the luma, U and V AR filters and the scaling
+// LUTs / block offsets side by side
+constexpr int kSetupThreads = 256;
 
 struct GrainSmem {
     int16_t lut[3][GH][GW];
     int16_t g_lds[2048];
-    int8_t coef[3][28];  // AR coefficients staged out of the argument block
     uint8_t scl[3][1024];
     int8_t win[3][4][7];
+    int luma_done;  // luma AR steps completed, published per 16-step block
 };
 
 // Causal AR filter of one grain plane by one wave, as a wavefront whose
@@ -69,15 +74,25 @@ struct GrainSmem {
 // x-3..x-1 at s-3..s-1, so each step needs only its three lane rotations (DPP
 // wave_ror:1, lane k reads lane k-1) and no LDS round trip on the dependency
 // chain.  Taps: win[r][c] for dy = r - 3, dx = c - 3 (zero outside the lag).
+//
+// The luma wave publishes its completed step count in *done after every
+// block of 16 steps (after its LDS writes).  A chroma sample (y, x) at step
+// c = x + 4y reads luma rows 2y-3, 2y-2 and columns 2x-3, 2x-2, final by luma
+// step 2c - 10, so a chroma wave runs alongside the luma one and waits, before
+// its block b, for 32b + 21 luma steps (when it uses the luma term).
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ int lane_up(int v) { return __builtin_amdgcn_mov_dpp(v, 0x13C, 0xF, 0xF, true); }
 
+constexpr int kLumaSteps = (GW - 1) + 4 * (GH - 1) + 1;
+
 template <bool kChroma>
 __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7], int ar_shift, int gmin, int gmax,
-                                        const int16_t (*luma)[GW], int wl) {
+                                        const int16_t (*luma)[GW], int wl, int *done) {
     constexpr int W = kChroma ? SGW : GW, H = kChroma ? SGH : GH;
     constexpr int kSteps = (W - 1) + 4 * (H - 1) + 1;
+    static_assert(!kChroma || 32 * ((kSteps + 15) / 16 - 1) + 21 <= 16 * ((kLumaSteps + 15) / 16),
+                  "every chroma block's luma steps get published");
     static_assert(kChroma || (4 * 8 + GW - 1 < 128 && H - 64 <= 9), "luma rows 64.. start after the switch");
     const int lane = threadIdx.x & 63;
     int cf[4][7];
@@ -94,6 +109,15 @@ __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7]
     int y = lane;
     for (int b = 0; b < (kSteps + 15) / 16; b++) {
         if (!kChroma && b == 8 && lane < H - 64) y = lane + 64;
+        if (kChroma && wl != 0) {
+            // bounded wait (the luma wave never waits, so the count always arrives)
+            const int need = 32 * b + 21;
+            for (int spin = 0; spin < (1 << 20); spin++) {
+                if (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= need) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
         const bool rows_ok = y >= 3 && y < H;
         // (y, s - 4y) is element y (GW - 4) + s of the plane's rows; lanes
         // without a sample at step s read (and never write) other LDS entries
@@ -131,133 +155,158 @@ __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7]
             }
             E[u] = v;
         }
+        if constexpr (!kChroma) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (lane == 0) __hip_atomic_store(done, 16 * (b + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+}
+
+// Scaling LUTs (generate_scaling_8bpc/hbd, filmgrain.rs:22-113) by one wave,
+// every entry in closed form: the first pass writes entries at multiples of
+// 1 << (bd-8) (later segments overwrite earlier ones, then the tail fill), the
+// second pass interpolates between them inside each segment.
+__device__ __forceinline__ void scaling_luts(const Rav2dFilmGrainData &d, int bd, uint8_t (*scl)[1024], int lane) {
+    for (int pl = 0; pl < 3; pl++) {
+        const int num = d.num_points[pl];
+        const uint8_t(*pts)[2] = d.points[pl];
+        const int sx = bd - 8, size = 1 << bd, pad = 1 << sx;
+        auto first_pass = [&](int c) -> int {  // value at coarse position c
+            int v = 0;
+            if (c < pts[0][0]) v = pts[0][1];
+            for (int k = 0; k < num - 1; k++) {
+                int bx = pts[k][0], dx = pts[k + 1][0] - bx;
+                if (dx > 0 && c >= bx && c < bx + dx) {
+                    int delta = (pts[k + 1][1] - pts[k][1]) * ((0x10000 + (dx >> 1)) / dx);
+                    v = (uint8_t)(pts[k][1] + ((0x8000 + (c - bx) * delta) >> 16));
+                }
+            }
+            if (c >= pts[num - 1][0]) v = pts[num - 1][1];
+            return v;
+        };
+        for (int e = lane; e < size; e += 64) {
+            int v = 0;
+            if (num > 0) {
+                int c = e >> sx, n = e & (pad - 1);
+                v = first_pass(c);
+                if (n && e >= (pts[0][0] << sx) && e < (pts[num - 1][0] << sx)) {
+                    // inside some segment: interpolate like the second pass
+                    bool in_seg = false;
+                    for (int k = 0; k < num - 1; k++) {
+                        int bx = pts[k][0] << sx, ex = pts[k + 1][0] << sx;
+                        if (e >= bx && e < ex) in_seg = true;
+                    }
+                    if (in_seg) {
+                        int base = v, range = first_pass(c + 1) - base;
+                        v = (uint8_t)(base + (((pad >> 1) + n * range) >> sx));
+                    }
+                }
+            }
+            scl[pl][e] = (uint8_t)v;
+        }
+    }
+}
+
+// LUT offsets of every (row band, block): each band's LFSR walk (fgy_32x32xn,
+// filmgrain.rs:276-304: the band seed, then per block and direction one 9-bit
+// draw and three 16-bit ones) split into segments of kBandSeg blocks, one lane
+// per (band, segment), each starting from the band seed moved on by kJumpSeg.
+__device__ __forceinline__ void band_offsets(const GrainArgs &a, int lane) {
+    const int nseg = (a.n_blocks + kBandSeg - 1) / kBandSeg;
+    const int bs_mul = 3 - a.fg.block_size;
+    for (int task = lane; task < a.n_bands * nseg; task += 64) {
+        const int band = task / nseg, seg = task - band * nseg;
+        uint32_t s = a.seed;
+        s ^= (uint32_t)(((band * 37 + 178) & 0xFF)) << 8;
+        s ^= (uint32_t)((band * 173 + 105) & 0xFF);
+        s = jump(kJumpSeg[dmin(seg, kSegs - 1)], s & 0xffff);
+        for (int k = kSegs - 1; k < seg; k++)  // frames wider than kSegs segments
+            for (int n = 0; n < 8 * kBandSeg; n++) lfsr(16, s);
+        const int b0 = seg * kBandSeg, b1 = dmin(b0 + kBandSeg, a.n_blocks);
+        uint8_t *o = a.offsets + (size_t)band * a.n_blocks * 2;
+        for (int b = b0; b < b1; b++)
+            for (int n = 0; n < 2; n++) {
+                o[b * 2 + n] = (uint8_t)(((uint32_t)bs_mul * lfsr(9, s)) >> 6);
+                lfsr(16, s);
+                lfsr(16, s);
+                lfsr(16, s);
+            }
     }
 }
 
 __device__ __forceinline__ void grain_setup_body(const GrainArgs &a, const int16_t *gauss, GrainSmem &sm) {
     auto &lut = sm.lut;
     auto &g_lds = sm.g_lds;
-    auto &coef = sm.coef;
-    auto &scl = sm.scl;
     auto &win = sm.win;
     const Rav2dFilmGrainData &d = a.fg;
     const int bd = a.bitdepth, bdm8 = bd - 8;
     const int shift = 4 - bdm8 + d.grain_scale_shift;
     const int ctr = 128 << bdm8, gmin = -ctr, gmax = ctr - 1;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const bool want_u = d.num_points[1] > 0 || d.chroma_scaling_from_luma;
     const bool want_v = d.num_points[2] > 0 || d.chroma_scaling_from_luma;
-    for (int i = t; i < 2048; i += kSetupThreads) g_lds[i] = gauss[i];
-    if (t < 3 * 28) coef[t / 28][t % 28] = d.ar_coeffs[t / 28][t % 28];
-    for (int i = t; i < 3 * GH * GW; i += kSetupThreads) (&lut[0][0][0])[i] = 0;
-    __syncthreads();
-    // Gaussian draws: lanes 0..63 walk chunks of the luma sequence, lanes
-    // 64..81 and 96..113 the U and V sequences (44x38 draws each)
-    {
-        int pl = t < 64 ? 0 : (t < 96 ? 1 : 2);
-        int k = pl == 0 ? t : (pl == 1 ? t - 64 : t - 96);
-        int w = pl ? SGW : GW, h = pl ? SGH : GH;
-        bool on = pl == 0 || (pl == 1 ? want_u : want_v);
-        int first = k * kChunk, last = dmin(first + kChunk, w * h);
-        if (on && first < w * h) {
-            uint32_t s = a.seed;
-            if (pl) s ^= pl == 2 ? 0x49d8 : 0xb524;
-            s = jump(k, s & 0xffff);
-            for (int i = first; i < last; i++) {
-                int y = i / w, x = i - y * w;
-                lut[pl][y][x] = (int16_t)round2(g_lds[lfsr(11, s)], shift);
-            }
-        }
-    }
-    // block offsets: one lane per row band
-    for (int band = t; band < a.n_bands; band += kSetupThreads) {
-        uint32_t s = a.seed;
-        s ^= (uint32_t)(((band * 37 + 178) & 0xFF)) << 8;
-        s ^= (uint32_t)((band * 173 + 105) & 0xFF);
-        uint8_t *o = a.offsets + (size_t)band * a.n_blocks * 2;
-        for (int b = 0; b < a.n_blocks; b++)
-            for (int n = 0; n < 2; n++) {
-                o[b * 2 + n] = (uint8_t)(((uint32_t)(3 - d.block_size) * lfsr(9, s)) >> 6);
-                lfsr(16, s);
-                lfsr(16, s);
-                lfsr(16, s);
-            }
-    }
-    __syncthreads();
-    // causal AR filter as a wavefront: sample (y, x) runs at step x + 4y, after
-    // its three left neighbours and the rows above up to x + 3.  The taps are
-    // laid out on a fixed lag-3 window (zero where the frame's lag is smaller):
-    // window row r = dy + 3 (dy = -3..0), column c = dx + 3 (dx = -3..3).
     const int lag = d.ar_coeff_lag;
-    // tap window: ci = (dy + lag) * (2 lag + 1) + (dx + lag), as the reference
-    // enumerates rows dy = -lag..0 and columns dx = -lag..lag (-lag..-1 on row 0)
+    for (int i = t; i < 2048; i += kSetupThreads) g_lds[i] = gauss[i];
+    // AR taps on a fixed lag-3 window (zero where the frame's lag is smaller):
+    // window row r = dy + 3 (dy = -3..0), column c = dx + 3 (dx = -3..3); the
+    // reference enumerates rows dy = -lag..0 and columns dx = -lag..lag
+    // (-lag..-1 on row 0): coefficient (dy + lag) (2 lag + 1) + (dx + lag)
     if (t < 3 * 28) {
         int pl = t / 28, r = (t % 28) / 7, c = t % 7;
         int dy = r - 3, dx = c - 3;
         int v = 0;
         bool in = dy >= -lag && dx >= -lag && (dy < 0 ? dx <= lag : dx < 0);
-        if (in) v = coef[pl][(dy + lag) * (2 * lag + 1) + (dx + lag)];
+        if (in) v = d.ar_coeffs[pl][(dy + lag) * (2 * lag + 1) + (dx + lag)];
         win[pl][r][c] = (int8_t)v;
     }
+    if (t == 0) sm.luma_done = 0;
     __syncthreads();
-    // luma AR on wave 0 while wave 1 builds the scaling LUTs
-    if (t < 64) {
-        ar_wave<false>(lut[0], win[0], (int)d.ar_coeff_shift, gmin, gmax, nullptr, 0);
-    } else {
-        // scaling LUTs (generate_scaling_8bpc/hbd, filmgrain.rs:22-113), every
-        // entry in closed form: the first pass writes entries at multiples of
-        // 1 << (bd-8) (later segments overwrite earlier ones, then the tail fill),
-        // the second pass interpolates between them inside each segment.
-        for (int pl = 0; pl < 3; pl++) {
-            const int num = d.num_points[pl];
-            const uint8_t(*pts)[2] = d.points[pl];
-            const int sx = bd - 8, size = 1 << bd, pad = 1 << sx;
-            auto first_pass = [&](int c) -> int {  // value at coarse position c
-                int v = 0;
-                if (c < pts[0][0]) v = pts[0][1];
-                for (int k = 0; k < num - 1; k++) {
-                    int bx = pts[k][0], dx = pts[k + 1][0] - bx;
-                    if (dx > 0 && c >= bx && c < bx + dx) {
-                        int delta = (pts[k + 1][1] - pts[k][1]) * ((0x10000 + (dx >> 1)) / dx);
-                        v = (uint8_t)(pts[k][1] + ((0x8000 + (c - bx) * delta) >> 16));
-                    }
+    // Gaussian draws, kChunk per lane: lanes 0..127 the luma sequence (73x82),
+    // 128..163 and 164..199 the U and V sequences (38x44 each)
+    {
+        const int pl = t < 128 ? 0 : (t < 164 ? 1 : 2);
+        const int k = pl == 0 ? t : (pl == 1 ? t - 128 : t - 164);
+        const int w = pl ? SGW : GW, n = pl ? SGW * SGH : GW * GH;
+        const bool on = t < 200 && (pl == 0 || (pl == 1 ? want_u : want_v));
+        const int first = k * kChunk, last = dmin(first + kChunk, n);
+        if (on && first < n) {
+            uint32_t s = a.seed;
+            if (pl) s ^= pl == 2 ? 0x49d8 : 0xb524;
+            s = jump(kJump[k], s & 0xffff);
+            int y = first / w, x = first - y * w;
+            for (int i = first; i < last; i++) {
+                lut[pl][y][x] = (int16_t)round2(g_lds[lfsr(11, s)], shift);
+                if (++x == w) {
+                    x = 0;
+                    y++;
                 }
-                if (c >= pts[num - 1][0]) v = pts[num - 1][1];
-                return v;
-            };
-            for (int e = t - 64; e < size; e += 64) {
-                int v = 0;
-                if (num > 0) {
-                    int c = e >> sx, n = e & (pad - 1);
-                    v = first_pass(c);
-                    if (n && e >= (pts[0][0] << sx) && e < (pts[num - 1][0] << sx)) {
-                        // inside some segment: interpolate like the second pass
-                        bool in_seg = false;
-                        for (int k = 0; k < num - 1; k++) {
-                            int bx = pts[k][0] << sx, ex = pts[k + 1][0] << sx;
-                            if (e >= bx && e < ex) in_seg = true;
-                        }
-                        if (in_seg) {
-                            int base = v, range = first_pass(c + 1) - base;
-                            v = (uint8_t)(base + (((pad >> 1) + n * range) >> sx));
-                        }
-                    }
-                }
-                scl[pl][e] = (uint8_t)v;
             }
         }
     }
     __syncthreads();
-    // chroma (4:2:0 sub-grid), with the luma correlation term: U on wave 0, V on wave 1
-    {
-        const int uv = t >> 6;
-        const bool on = uv == 0 ? want_u : want_v;
-        const int wl = d.num_points[0] > 0 ? coef[1 + uv][2 * lag * (lag + 1)] : 0;
-        if (on) ar_wave<true>(lut[1 + uv], win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl);
+    // the three causal AR filters side by side (chroma after the luma samples
+    // it reads), while the fourth wave builds the scaling LUTs and the offsets
+    if (wave == 0) {
+        ar_wave<false>(lut[0], win[0], (int)d.ar_coeff_shift, gmin, gmax, nullptr, 0, &sm.luma_done);
+    } else if (wave < 3) {
+        const int uv = wave - 1;
+        const int wl = d.num_points[0] > 0 ? d.ar_coeffs[1 + uv][2 * lag * (lag + 1)] : 0;
+        if (uv == 0 ? want_u : want_v)
+            ar_wave<true>(lut[1 + uv], win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl, &sm.luma_done);
+    } else {
+        scaling_luts(d, bd, sm.scl, lane);
+        band_offsets(a, lane);
     }
     __syncthreads();
-    for (int i = t; i < 3 * 1024; i += kSetupThreads) a.scaling[i] = (&scl[0][0])[i];
-    for (int i = t; i < 3 * GH * GW; i += kSetupThreads) a.lut[i] = (&lut[0][0][0])[i];
+    // out: the scaling LUTs and the grain LUTs' sample areas (luma rows 0..72,
+    // chroma rows 0..37 at the same row stride), four bytes at a time
+    for (int i = t; i < 3 * 1024 / 4; i += kSetupThreads) ((uint32_t *)a.scaling)[i] = ((const uint32_t *)sm.scl)[i];
+    constexpr int kLw = GH * GW / 2, kCw = SGH * GW / 2;
+    for (int i = t; i < kLw + 2 * kCw; i += kSetupThreads) {
+        const int pl = i < kLw ? 0 : (i < kLw + kCw ? 1 : 2);
+        const int j = pl * kLw + (pl ? (i - kLw - (pl - 1) * kCw) : i);
+        ((uint32_t *)a.lut)[j] = ((const uint32_t *)&lut[0][0][0])[j];
+    }
 }
 
 __global__ __launch_bounds__(kSetupThreads) void grain_setup_kernel(GrainArgs a, const int16_t *gauss) {
@@ -439,7 +488,7 @@ static const int16_t *grain_tables() {
         int16_t *g = nullptr;
         if (hipMalloc((void **)&g, sizeof(R2T_GAUSSIAN_SEQUENCE)) != hipSuccess) return nullptr;
         (void)hipMemcpy(g, R2T_GAUSSIAN_SEQUENCE, sizeof(R2T_GAUSSIAN_SEQUENCE), hipMemcpyHostToDevice);
-        // jump-ahead matrices M^(k*kChunk), k = 0..63
+        // jump-ahead matrices: M^(k kChunk), k < 128, and M^(8 kBandSeg j), j < kSegs
         auto mul = [](const uint16_t *A, const uint16_t *B, uint16_t *C) {
             for (int i = 0; i < 16; i++) {
                 uint16_t r = 0;
@@ -448,21 +497,27 @@ static const int16_t *grain_tables() {
                 C[i] = r;
             }
         };
-        uint16_t step[16], stepC[16], cur[16], tmp[16], jumps[64][16];
+        uint16_t step[16];
         for (int i = 0; i < 15; i++) step[i] = (uint16_t)(1u << (i + 1));
         step[15] = (uint16_t)((1u << 0) | (1u << 1) | (1u << 3) | (1u << 12));
-        for (int i = 0; i < 16; i++) stepC[i] = (uint16_t)(1u << i);
-        for (int n = 0; n < kChunk; n++) {  // stepC = M^kChunk
-            mul(step, stepC, tmp);
-            ::memcpy(stepC, tmp, sizeof(tmp));
-        }
-        for (int i = 0; i < 16; i++) cur[i] = (uint16_t)(1u << i);
-        for (int k = 0; k < 64; k++) {
-            ::memcpy(jumps[k], cur, sizeof(cur));
-            mul(stepC, cur, tmp);
-            ::memcpy(cur, tmp, sizeof(tmp));
-        }
+        auto powers = [&](int n, int count, uint16_t (*out)[16]) {  // out[k] = M^(k n)
+            uint16_t stepN[16], cur[16], tmp[16];
+            for (int i = 0; i < 16; i++) stepN[i] = cur[i] = (uint16_t)(1u << i);
+            for (int k = 0; k < n; k++) {
+                mul(step, stepN, tmp);
+                ::memcpy(stepN, tmp, sizeof(tmp));
+            }
+            for (int k = 0; k < count; k++) {
+                ::memcpy(out[k], cur, sizeof(cur));
+                mul(stepN, cur, tmp);
+                ::memcpy(cur, tmp, sizeof(tmp));
+            }
+        };
+        static uint16_t jumps[128][16], segs[kSegs][16];
+        powers(kChunk, 128, jumps);
+        powers(8 * kBandSeg, kSegs, segs);
         (void)hipMemcpyToSymbol(HIP_SYMBOL(kJump), jumps, sizeof(jumps));
+        (void)hipMemcpyToSymbol(HIP_SYMBOL(kJumpSeg), segs, sizeof(segs));
         gauss = g;
     }
     return gauss;
