@@ -137,6 +137,8 @@ __device__ __forceinline__ s16x2 splat2(int v) { return (s16x2){(short)v, (short
 // constrain() of N taps' differences d, in place, stage by stage across the
 // taps: consecutive packed operations are then independent (a dependent pair
 // of packed operations costs a wait state).
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
 template <int N>
 __device__ __forceinline__ void constrain_n(s16x2 (&d)[N], s16x2 thr, s16x2 sh) {
     const s16x2 z = {0, 0};
@@ -147,10 +149,11 @@ __device__ __forceinline__ void constrain_n(s16x2 (&d)[N], s16x2 thr, s16x2 sh) 
     for (int k = 0; k < N; k++) c[k] = __builtin_elementwise_max(d[k], c[k]);  // |d|
 #pragma unroll
     for (int k = 0; k < N; k++) c[k] = c[k] >> sh;
+    // max(0, thr - (|d| >> sh)) as one unsigned saturating subtract (both >= 0)
 #pragma unroll
-    for (int k = 0; k < N; k++) c[k] = thr - c[k];
-#pragma unroll
-    for (int k = 0; k < N; k++) c[k] = __builtin_elementwise_max(z, c[k]);  // max(0, thr - (|d| >> sh))
+    for (int k = 0; k < N; k++)
+        c[k] = __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, thr),
+                                                                       __builtin_bit_cast(u16x2, c[k])));
 #pragma unroll
     for (int k = 0; k < N; k++) d[k] = __builtin_elementwise_max(d[k], z - c[k]);
 #pragma unroll
@@ -274,6 +277,35 @@ __device__ __forceinline__ int ccso_offset(const CcsoDev &c, const int8_t *lut, 
     return lut[idx & 0xff];
 }
 
+// CCSO offsets of two horizontally adjacent samples whose collocated luma and
+// its neighbours at +-(dx, dy) all lie inside the frame (no clamping): `cp`
+// points at the collocated luma of the left sample in the staged tile, the
+// right sample's is kStep further (1 luma, 2 chroma), `o` = dy * LY + dx.
+// ccso_score from the sign bits of packed 16-bit differences (|d| <= 1023):
+// 1 - [d < -q] + [d > q] (the last only without edge_clf), as
+// 1 + ((d + q) >> 15) - ((q - d) >> 15).
+template <int kStep>
+__device__ __forceinline__ void ccso_pair_inner(const CcsoDev &c, const int8_t *lut, const int16_t *cp, int o,
+                                                int &off0, int &off1) {
+    const int c0 = cp[0], c1 = cp[kStep];
+    uint32_t idx = ((uint32_t)c0 >> c.shift) | (((uint32_t)c1 >> c.shift) << 16);
+    if (!c.bo_only) {
+        const s16x2 cv = {(short)c0, (short)c1};
+        const s16x2 na = {(short)cp[o], (short)cp[o + kStep]}, nb = {(short)cp[-o], (short)cp[kStep - o]};
+        const s16x2 q = splat2(c.quant_step), one = {1, 1};
+        const s16x2 da = na - cv, db = nb - cv;
+        s16x2 sa = one + ((da + q) >> (short)15), sb = one + ((db + q) >> (short)15);
+        if (c.edge_clf == 0) {
+            sa -= (q - da) >> (short)15;
+            sb -= (q - db) >> (short)15;
+        }
+        // (cls0 << 5) | (cls1 << 3) | band, the band in bits 0..15 of each half
+        idx |= __builtin_bit_cast(uint32_t, (s16x2)((sa << (short)5) + (sb << (short)3)));
+    }
+    off0 = lut[idx & 0xff];
+    off1 = lut[(idx >> 16) & 0xff];
+}
+
 // Staging of one superblock, split so that a workgroup can have the next
 // superblock's loads in flight while it filters the current one.
 constexpr int kQY = LY / 4, kQC = LC / 4;
@@ -373,6 +405,10 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
     uv_sec += uv_sec == 3;
     uv_sec <<= bdm8;
     const int damping = a.damping + bdm8;
+    constexpr int bdmax = sizeof(P) == 1 ? 255 : 1023;  // 8-bit frames run with P = uint8_t, 10-bit with uint16_t
+    // every CCSO neighbour (|dx| <= 2, |dy| <= 1) of the superblock's luma is
+    // inside the frame: no clamping
+    const bool cc_inner = x0 >= 2 && y0 >= 1 && x0 + 66 <= fw && y0 + 65 <= fh;
     __syncthreads();
     // ---- direction search: lane t & 63 owns 8x8 block (t & 7, (t >> 3) & 7)
     auto block_searched = [&](int b) -> bool {
@@ -472,8 +508,15 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
                 }
                 if (cc0) {
                     const int8_t *lut = sm.ccso_lut[0];
-                    v0 = iclip(v0 + ccso_offset(a.ccso[0], lut, ly, gx, gy, x0, y0, fw, fh), 0, a.g.bdmax);
-                    v1 = iclip(v1 + ccso_offset(a.ccso[0], lut, ly, gx + 1, gy, x0, y0, fw, fh), 0, a.g.bdmax);
+                    int o0, o1;
+                    if (cc_inner) {
+                        ccso_pair_inner<1>(a.ccso[0], lut, tp, a.ccso[0].dy * LY + a.ccso[0].dx, o0, o1);
+                    } else {
+                        o0 = ccso_offset(a.ccso[0], lut, ly, gx, gy, x0, y0, fw, fh);
+                        o1 = ccso_offset(a.ccso[0], lut, ly, gx + 1, gy, x0, y0, fw, fh);
+                    }
+                    v0 = iclip(v0 + o0, 0, bdmax);
+                    v1 = iclip(v1 + o1, 0, bdmax);
                 }
                 store2(row_ptr<P>(a.dst[0], gy) + gx, v0, v1);
             }
@@ -513,8 +556,15 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
                 if (cc) {
                     const CcsoDev &c = a.ccso[1 + pl];
                     const int8_t *lut = sm.ccso_lut[1 + pl];
-                    v0 = iclip(v0 + ccso_offset(c, lut, ly, gx << 1, gy << 1, x0, y0, fw, fh), 0, a.g.bdmax);
-                    v1 = iclip(v1 + ccso_offset(c, lut, ly, (gx + 1) << 1, gy << 1, x0, y0, fw, fh), 0, a.g.bdmax);
+                    int o0, o1;
+                    if (cc_inner) {
+                        ccso_pair_inner<2>(c, lut, &ly[(2 * yy + 2) * LY + 2 * x + CX], c.dy * LY + c.dx, o0, o1);
+                    } else {
+                        o0 = ccso_offset(c, lut, ly, gx << 1, gy << 1, x0, y0, fw, fh);
+                        o1 = ccso_offset(c, lut, ly, (gx + 1) << 1, gy << 1, x0, y0, fw, fh);
+                    }
+                    v0 = iclip(v0 + o0, 0, bdmax);
+                    v1 = iclip(v1 + o1, 0, bdmax);
                 }
                 store2(row_ptr<P>(a.dst[1 + pl], gy) + gx, v0, v1);
             }
@@ -533,9 +583,11 @@ __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, C
 template <typename P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void cdef_kernel(CdefArgs a) {
     __shared__ CdefSmem sm;
-    // XCD-contiguous superblock runs: neighbours share halo rows and columns in L2
+    // XCD-contiguous superblock runs (neighbours share halo rows and columns in
+    // L2), costliest first inside each run
     const int sbw = (a.g.bw * 4 + 63) / 64;
-    const int sb = xcd_item(blockIdx.x, gridDim.x);
+    const int slot = xcd_item(blockIdx.x, gridDim.x);
+    const int sb = a.order ? a.order[slot] : slot;
     cdef_body<P>(a, sb % sbw, sb / sbw, sm);
 }
 

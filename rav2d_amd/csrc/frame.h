@@ -106,6 +106,7 @@ struct Rav2dGpuFrame {
     std::vector<r2g::LrItem> lr_items;
     r2g::LrItem *d_lr_items = nullptr;
     size_t d_lr_cap = 0;
+    int32_t *d_cdef_order = nullptr;  // launch order of the 64x64 CDEF superblocks
     int lr_restore_planes = 0, lr_base_q = 0, lr_widx = 0, lr_ns_idx = 0;
     int lr_n_restore = 0;  // lr_items: restoration items first, then plane copies
     bool lr_dirty = true;

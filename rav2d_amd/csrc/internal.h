@@ -41,6 +41,7 @@ struct CdefArgs {
     DevPlane src[3], dst[3];
     DevGeom g;
     const Rav2dAv2Filter *mask;
+    const int32_t *order;  // superblock of each launch slot (null: raster order)
     int32_t cdef_on, ccso_on, on_skiptx, damping;
     alignas(8) uint8_t y_strength[8];
     alignas(8) uint8_t uv_strength[8];

@@ -311,6 +311,38 @@ void build_lr_items(Rav2dGpuFrame *f) {
         if (!lr_stage || !(rp & (1 << pl))) copy_items(f, pl, f->lr_items);
 }
 
+// Launch order of the 64x64 CDEF superblocks.  The kernel gives each XCD a
+// contiguous run of launch slots (xcd_item); inside a run the costliest
+// superblocks go first (CDEF on: direction search and filtering; CCSO planes),
+// so that the cheap ones fill the tail of the launch.
+int upload_cdef_order(Rav2dGpuFrame *f) {
+    const DevGeom &g = f->g;
+    const Rav2dFrameParams &P = f->params;
+    const int sbw = (g.bw * 4 + 63) / 64, sbh = (g.bh * 4 + 63) / 64, n = sbw * sbh;
+    std::vector<int32_t> ord(n), cost(n, 0);
+    const bool cdef_on = P.inloop_filters & RAV2D_INLOOPFILTER_CDEF;
+    const bool ccso_on = P.inloop_filters & RAV2D_INLOOPFILTER_CCSO;
+    for (int sby = 0; sby < sbh; sby++)
+        for (int sbx = 0; sbx < sbw; sbx++) {
+            const size_t mi = (size_t)(sby >> 2) * g.sb256w + (sbx >> 2);
+            if (mi >= f->h_mask.size()) continue;
+            const Rav2dAv2Filter &m = f->h_mask[mi];
+            const int idx = m.cdef_idx[((sby & 3) << 2) + (sbx & 3)];
+            const bool active = cdef_on && idx >= 0 && (P.cdef_y_strength[idx & 7] || P.cdef_uv_strength[idx & 7]);
+            int c = active ? 8 : 0;
+            for (int pl = 0; pl < 3; pl++)
+                if (ccso_on && P.ccso[pl].enabled && m.ccso[pl]) c += pl ? 1 : 2;
+            cost[sby * sbw + sbx] = c;
+        }
+    for (int i = 0; i < n; i++) ord[i] = i;
+    const int q = n >> 3;
+    for (int x = 0; x < 8 && q > 0; x++)
+        std::stable_sort(ord.begin() + (size_t)x * q, ord.begin() + (size_t)(x + 1) * q,
+                         [&](int a, int b) { return cost[a] > cost[b]; });
+    HIP_TRY(hipMemcpy(f->d_cdef_order, ord.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+    return RAV2D_OK;
+}
+
 int upload_lr_items(Rav2dGpuFrame *f) {
     size_t n = f->lr_items.size();
     if (n > f->d_lr_cap) {
@@ -535,6 +567,7 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
         CdefArgs &a = cp.cdef;
         a.g = f->g;
         a.mask = f->d_mask;
+        a.order = f->d_cdef_order;
         a.cdef_on = (P.inloop_filters & RAV2D_INLOOPFILTER_CDEF) != 0;
         a.ccso_on = (P.inloop_filters & RAV2D_INLOOPFILTER_CCSO) &&
                     (P.ccso[0].enabled || P.ccso[1].enabled || P.ccso[2].enabled);
@@ -566,8 +599,10 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
     }
     // loop restoration (decode.rs:3999-4057), written straight into OUT
     if (f->lr_dirty) {
+        int r = upload_cdef_order(f);
+        if (r) return r;
         build_lr_items(f);
-        int r = upload_lr_items(f);
+        r = upload_lr_items(f);
         if (r) return r;
     }
     {
@@ -760,7 +795,9 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
         lr_cap += (size_t)((f->pw[pl] + 63) / 64) * sbrows * 3 +
                   (size_t)((f->pw[pl] + 255) / 256) * ((f->ph[pl] + 63) / 64);
     const size_t side_bytes = n_sb * (sizeof(Rav2dAv2Filter) + sizeof(Rav2dAv2Restoration));
-    size_t need = total * NBUF + sizeof(DbLut) + grain_bytes + side_bytes + lr_cap * sizeof(LrItem);
+    const size_t n_cdef = (size_t)((params->width + 63) / 64) * ((params->height + 63) / 64);
+    size_t need = total * NBUF + sizeof(DbLut) + grain_bytes + side_bytes + lr_cap * sizeof(LrItem) +
+                  n_cdef * sizeof(int32_t);
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         if (ctx->budget && ctx->in_use + need > ctx->budget) {
@@ -776,9 +813,10 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
         hipMalloc((void **)&f->d_grain, grain_bytes) != hipSuccess ||
         hipMalloc((void **)&f->d_mask, n_sb * sizeof(Rav2dAv2Filter)) != hipSuccess ||
         hipMalloc((void **)&f->d_lr, n_sb * sizeof(Rav2dAv2Restoration)) != hipSuccess ||
-        hipMalloc((void **)&f->d_lr_items, lr_cap * sizeof(LrItem)) != hipSuccess) {
+        hipMalloc((void **)&f->d_lr_items, lr_cap * sizeof(LrItem)) != hipSuccess ||
+        hipMalloc((void **)&f->d_cdef_order, n_cdef * sizeof(int32_t)) != hipSuccess) {
         for (void *p : {(void *)f->block, (void *)f->d_dblut, (void *)f->d_grain, (void *)f->d_mask, (void *)f->d_lr,
-                        (void *)f->d_lr_items})
+                        (void *)f->d_lr_items, (void *)f->d_cdef_order})
             if (p) (void)hipFree(p);
         {
             std::lock_guard<std::mutex> lk(ctx->mu);
@@ -821,6 +859,7 @@ int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f) {
     if (f->d_mask) (void)hipFree(f->d_mask);
     if (f->d_lr) (void)hipFree(f->d_lr);
     if (f->d_lr_items) (void)hipFree(f->d_lr_items);
+    if (f->d_cdef_order) (void)hipFree(f->d_cdef_order);
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->in_use -= f->bytes;
