@@ -76,6 +76,8 @@ struct LrTables {
     const uint8_t *lut_to_class;    // PC_WIENER_LUT_TO_CLASS[4096]
     const int16_t *pc_filters;      // PC_WIENER_FILTERS[4][64][13]
     const uint32_t *pc_pairs;       // the same as [4][64][8] packed 16-bit pairs (f0|f1, .., f10|f11, f12|0, 0)
+    const uint8_t *cls_pc;          // [4][4096]: PC_WIENER_SUB_CLASSIFY[w][LUT_TO_CLASS[li]]
+    const uint8_t *cls_ns;          // [4][7][4096]: PC_WIENER_SUB_CLASSIFY_NS[w][n][LUT_TO_CLASS[li]]
     const uint8_t *pc_sub;          // PC_WIENER_SUB_CLASSIFY[4][256]
     const uint8_t *ns_sub;          // PC_WIENER_SUB_CLASSIFY_NS[4][7][256]
     const uint16_t *gdf_alpha;      // GDF_ALPHA[3168]

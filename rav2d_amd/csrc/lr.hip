@@ -372,44 +372,66 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
             sm.wpk[i] = (uint32_t)(uint16_t)sm.s_weight[k * 88 + (2 * q2) * 4 + c] |
                         ((uint32_t)(uint16_t)sm.s_weight[k * 88 + (2 * q2 + 1) * 4 + c] << 16);
         }
-        for (int i = t; i < ncx * ncy; i += kThreads) {
-            int k = i / ncx, j = i - k * ncx;
-            int g0 = 0, g1 = 0, g2 = 0, g3 = 0;
+        // lane t: cell column t % 33, rows t / 33, + 7, ... (ncx, ncy <= 33)
+        const int j = t % 33;
+        if (t < 7 * 33 && j < ncx) {
+            for (int k = t / 33; k < ncy; k += 7) {
+                // rows 2k-2 .. 2k+1, columns 2j-2 .. 2j+1 as aligned 16-bit pairs
+                uint32_t wv[4][2];
+                const uint32_t *w0 = (const uint32_t *)&V[(2 * k - 2 + E) * VW + 2 * j - 2 + VC];
 #pragma unroll
-            for (int yy = 0; yy < 2; yy++)
-#pragma unroll
-                for (int xx = 0; xx < 2; xx++) {
-                    const uint16_t *p = &V[(2 * k - 1 + yy + E) * VW + 2 * j - 1 + xx + VC];
-                    int b = VAT(p, 0, 0) * 2;
-                    g0 += abs(b - VAT(p, -1, 0) - VAT(p, 1, 0));
-                    g1 += abs(b - VAT(p, 0, -1) - VAT(p, 0, 1));
-                    g2 += abs(b - VAT(p, -1, -1) - VAT(p, 1, 1));
-                    g3 += abs(b - VAT(p, 1, -1) - VAT(p, -1, 1));
+                for (int r = 0; r < 4; r++) {
+                    wv[r][0] = w0[r * (VW / 2)];
+                    wv[r][1] = w0[r * (VW / 2) + 1];
                 }
-            uint16_t *cp = &cells[(k * CS + j) * 4];
-            cp[0] = (uint16_t)g0;
-            cp[1] = (uint16_t)g1;
-            cp[2] = (uint16_t)g2;
-            cp[3] = (uint16_t)g3;
+                auto S = [&](int r, int c) -> uint32_t { return (wv[r][c >> 1] >> (16 * (c & 1))) & 0xffff; };
+                auto sad = [](uint32_t x, uint32_t y, uint32_t acc) { return (x > y ? x - y : y - x) + acc; };
+                uint32_t g0 = 0, g1 = 0, g2 = 0, g3 = 0;
+#pragma unroll
+                for (int r = 1; r <= 2; r++)
+#pragma unroll
+                    for (int c = 1; c <= 2; c++) {
+                        const uint32_t b2 = 2 * S(r, c);
+                        g0 = sad(S(r - 1, c) + S(r + 1, c), b2, g0);
+                        g1 = sad(S(r, c - 1) + S(r, c + 1), b2, g1);
+                        g2 = sad(S(r - 1, c - 1) + S(r + 1, c + 1), b2, g2);
+                        g3 = sad(S(r + 1, c - 1) + S(r - 1, c + 1), b2, g3);
+                    }
+                uint32_t *cp = (uint32_t *)&cells[(k * CS + j) * 4];
+                cp[0] = g0 | (g1 << 16);
+                cp[1] = g2 | (g3 << 16);
+            }
         }
     }
     // ---- classes of the multi-class filters (one per 4x4 block)
     const int bw4 = w >> 2, bh4 = h >> 2;
     if (kind == LR_KIND_MULTI || kind == LR_KIND_PC) {
-        const uint8_t *sub = kind == LR_KIND_PC ? a.t.pc_sub + a.widx * 256 : a.t.ns_sub + (a.widx * 7 + a.ns_idx) * 256;
-        for (int i = t; i < bw4 * bh4; i += kThreads) {
-            int by = i / bw4, bx = i - by * bw4;
-            int f0 = 0, f1 = 0, f2v = 0, s = 0;
-            for (int dy = -1; dy <= 4; dy++) {
-                const uint16_t *p = &V[(by * 4 + dy + E) * VW + bx * 4 + VC];
+        const uint8_t *cmap = kind == LR_KIND_PC ? a.t.cls_pc + a.widx * 4096 : a.t.cls_ns + (a.widx * 7 + a.ns_idx) * 4096;
+        // a unit has at most 16 x 16 blocks: one per lane
+        if (t < bw4 * bh4) {
+            const int by = t / bw4, bx = t - by * bw4;
+            // the block's window, rows by*4-2 .. by*4+5 and columns bx*4-2 ..
+            // bx*4+5, as four aligned 16-bit pairs per row, all read before use
+            uint32_t wv[8][4];
+            const uint32_t *w0 = (const uint32_t *)&V[(by * 4 - 2 + E) * VW + bx * 4 - 2 + VC];
 #pragma unroll
-                for (int dx = -1; dx <= 4; dx++) {
-                    int m2 = 2 * VAT(p, 0, dx);
-                    f0 += abs(VAT(p, -1, dx) - m2 + VAT(p, 1, dx));
-                    f1 += abs(VAT(p, -1, dx + 1) - m2 + VAT(p, 1, dx - 1));
-                    f2v += abs(VAT(p, -1, dx - 1) - m2 + VAT(p, 1, dx + 1));
+            for (int r = 0; r < 8; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) wv[r][c] = w0[r * (VW / 2) + c];
+            auto S = [&](int r, int c) -> uint32_t { return (wv[r][c >> 1] >> (16 * (c & 1))) & 0xffff; };
+            // |up + down - 2 m| summed with v_sad_u32 (sums of two samples <= 2046)
+            auto sad = [](uint32_t x, uint32_t y, uint32_t acc) { return (x > y ? x - y : y - x) + acc; };
+            uint32_t f0 = 0, f1 = 0, f2v = 0;
+#pragma unroll
+            for (int r = 1; r <= 6; r++)
+#pragma unroll
+                for (int c = 1; c <= 6; c++) {
+                    const uint32_t m2 = 2 * S(r, c);
+                    f0 = sad(S(r - 1, c) + S(r + 1, c), m2, f0);
+                    f1 = sad(S(r - 1, c + 1) + S(r + 1, c - 1), m2, f1);
+                    f2v = sad(S(r - 1, c - 1) + S(r + 1, c + 1), m2, f2v);
                 }
-            }
+            int s = 0;
             const int npx[3] = {16, 4, 1};
             for (int dy = -1; dy <= 1; dy++)
                 for (int dx = -1; dx <= 1; dx++) {
@@ -418,7 +440,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
                     s += npx[e] * (((it.noskip[fy] >> fx) & 1) == 0);
                 }
             int rnd = (1 << bdm8) >> 1;
-            int f[3] = {f0, f1, f2v};
+            int f[3] = {(int)f0, (int)f1, (int)f2v};
 #pragma unroll
             for (int k = 0; k < 3; k++) f[k] = (int)((uint32_t)f[k] * kPcNorm[k] + (uint32_t)rnd) >> bdm8;
             s *= kPcNorm[3];
@@ -431,7 +453,7 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
                 q = dmin(q, 255) >> 5;
                 li |= q << (3 * (2 - k));
             }
-            cls[by * 16 + bx] = sub[a.t.lut_to_class[li]];
+            cls[by * 16 + bx] = cmap[li];
         }
     }
     __syncthreads();
@@ -660,6 +682,14 @@ int lr_tables(int device, LrTables *out) {
                     pc_pairs[wi][c][i] = (uint32_t)(uint16_t)lo | ((uint32_t)(uint16_t)hi << 16);
                 }
             }
+        // the class of every feature index per sub-classification table
+        static uint8_t cls_pc[4][4096], cls_ns[4][7][4096];
+        for (int wi = 0; wi < 4; wi++)
+            for (int li = 0; li < 4096; li++) {
+                const int c = R2T_PC_WIENER_LUT_TO_CLASS[li];
+                cls_pc[wi][li] = R2T_PC_WIENER_SUB_CLASSIFY[wi][c];
+                for (int n = 0; n < 7; n++) cls_ns[wi][n][li] = R2T_PC_WIENER_SUB_CLASSIFY_NS[wi][n][c];
+            }
         struct Src {
             const void *p;
             size_t n;
@@ -674,6 +704,8 @@ int lr_tables(int device, LrTables *out) {
             {R2T_GDF_INTRA_ERROR, sizeof(R2T_GDF_INTRA_ERROR)},
             {R2T_GDF_INTER_ERROR, sizeof(R2T_GDF_INTER_ERROR)},
             {pc_pairs, sizeof(pc_pairs)},
+            {cls_pc, sizeof(cls_pc)},
+            {cls_ns, sizeof(cls_ns)},
         };
         constexpr int kN = sizeof(src) / sizeof(src[0]);
         size_t total = 0, off[kN];
@@ -696,6 +728,8 @@ int lr_tables(int device, LrTables *out) {
         t.gdf_intra = (const int8_t *)(d + off[7]);
         t.gdf_inter = (const int8_t *)(d + off[8]);
         t.pc_pairs = (const uint32_t *)(d + off[9]);
+        t.cls_pc = d + off[10];
+        t.cls_ns = d + off[11];
         ready[device] = true;
     }
     *out = cache[device];
