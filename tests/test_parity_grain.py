@@ -46,7 +46,7 @@ def test_grain_chain(ctx, case):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("variant", ["no_overlap_bs16", "overlap_bs32", "restricted_mcid", "uv_points", "lag0",
-                                     "no_luma_points"])
+                                     "lag1", "lag2", "no_luma_points"])
 def test_grain_parameter_variants(ctx, variant):
     sf = R.SynthFrame(512, 272, 10, 0, seed=77, tools=R.TOOL_GRAIN)
     g = sf.params.grain
@@ -67,6 +67,9 @@ def test_grain_parameter_variants(ctx, variant):
         g.uv_mult[1], g.uv_luma_mult[1], g.uv_offset[1] = -100, 120, -200
     elif variant == "lag0":
         g.ar_coeff_lag = 0
+    elif variant in ("lag1", "lag2"):
+        # the AR window's taps beyond the lag are zero; chroma keeps its luma term
+        g.ar_coeff_lag = int(variant[-1])
     elif variant == "no_luma_points":
         g.num_points[0] = 0
         g.clip_to_restricted_range = 1
