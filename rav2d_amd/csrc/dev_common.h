@@ -111,16 +111,20 @@ struct BatchSched {
     int32_t xcd_remap;
 };
 
+// Frame of a batch item: prefix is increasing with prefix[0] = 0, so the frame
+// is the number of entries <= item, minus one.  Each lane of the wave reads one
+// entry (one round of loads for up to 64 frames, instead of a binary search's
+// chain of dependent loads) and a ballot counts them.  Called by whole waves
+// with a wave-uniform item.
 __device__ __forceinline__ int find_frame(const int32_t *prefix, int n, int item) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= item)
-            lo = mid;
-        else
-            hi = mid - 1;
+    const int lane = threadIdx.x & 63;
+    int count = 0;
+    for (int base = 0; base < n; base += 64) {
+        const int i = base + lane;
+        const bool le = i < n && prefix[i] <= item;
+        count += __popcll(__ballot(le));
     }
-    return lo;
+    return count - 1;
 }
 
 // Visits the items of workgroup blockIdx.x in order: fn(frame, local item).
