@@ -156,21 +156,6 @@ struct LrSmem {
     LrItem it;
 };
 
-// Row and column of element t + j * kThreads of a row-major grid with COLS
-// columns, stepped across j without divisions.
-template <int COLS> struct GridPos {
-    int r, c;
-    __device__ __forceinline__ explicit GridPos(int t) : r(t / COLS), c(t % COLS) {}
-    __device__ __forceinline__ void next() {
-        r += kThreads / COLS;
-        c += kThreads % COLS;
-        if (c >= COLS) {
-            c -= COLS;
-            r++;
-        }
-    }
-};
-
 template <typename P, typename Args>
 __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     uint16_t *V = sm.V, *Ld = sm.Ld, *cells = sm.cells;
@@ -238,19 +223,22 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     // Rows: inside the stripe from F3; across a stripe boundary the two
     // deblocked rows beyond it (F2), further rows repeating the second; at the
     // frame's top/bottom the stripe's first/last row repeats.
-    constexpr int PERQ = (VH * VQ + kThreads - 1) / kThreads;
-    const int nvq = (h + 2 * E) * VQ;
+    // lane t < RPP * VQ owns column group t % VQ of staged rows t / VQ + RPP j:
+    // the column clamp is computed once, rows step by a constant
+    constexpr int RPP = kThreads / VQ, PERQ = (VH + RPP - 1) / RPP;
     const int pw = f3.w;
     const bool top = edges & LRE_TOP, bottom = edges & LRE_BOTTOM;
+    const int cq = t % VQ, r0 = t / VQ;
+    const bool lane_on = t < RPP * VQ;
+    const int col = iclip(x0 + cq * 4 - VC, 0, pw - 4) * (int)sizeof(P);
     Q qv[PERQ];
-    GridPos<VQ> gq(t);
 #pragma unroll
-    for (int j = 0; j < PERQ; j++, gq.next()) {
-        const int r = dmin(gq.r, h + 2 * E - 1) - E, c0 = gq.c * 4 - VC;
+    for (int j = 0; j < PERQ; j++) {
+        const int r = dmin(r0 + RPP * j, h + 2 * E - 1) - E;
         const bool from2 = (r < 0 && top) || (r >= h && bottom);
         const int yy = y0 + iclip(r, from2 ? -2 : 0, from2 ? h + 1 : h - 1);
         const uint8_t *row = (from2 ? f2.ptr : f3.ptr) + __umul24(yy, stride);
-        qv[j] = *(const Q *)(row + iclip(x0 + c0, 0, pw - 4) * sizeof(P));
+        qv[j] = *(const Q *)(row + col);
     }
     // tables of this item's filter kinds, 16 (or 8) bytes per lane
     uint4 pcv = {0, 0, 0, 0}, alv = {0, 0, 0, 0}, wv = {0, 0, 0, 0}, erv = {0, 0, 0, 0};
@@ -267,7 +255,10 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
             erv2 = ((const uint2 *)(a.t.gdf_inter + (a.gdf_ref - 1) * 6000 + a.gdf_qp * 1000))[t];
     }
     const bool uv_single = kind == LR_KIND_SINGLE && pl != 0;
-    constexpr int PERL = ((32 + 4) * LS + kThreads - 1) / kThreads;
+    // lane t < LRP * LS owns column t % LS of the rows t / LS + LRP j
+    constexpr int LRP = kThreads / LS, PERL = (32 + 4 + LRP - 1) / LRP;
+    const int lc = t % LS, lr0 = t / LS;
+    const bool llane = t < LRP * LS;
     uint16_t lvals[PERL];
     if (uv_single) {
         // downsampled luma rows of the chroma stripe (cfl_ds_filter_index):
@@ -280,29 +271,31 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
         // cfl_ds 0: 2x2 average, 1: vertical pair average, 2: top-left sample
         const int ds = a.cfl_ds, mb = ds <= 1 ? -1 : 0, m1 = ds == 0 ? -1 : 0, sh = ds == 0 ? 2 : (ds == 1 ? 1 : 0);
         using Q2 = typename std::conditional<sizeof(P) == 1, uint16_t, uint32_t>::type;  // 2 samples
-        GridPos<LS> gl(t);
+        const int c = lc - 2;
+        const int xs = (c < 0 && !(edges & LRE_LEFT)) ? 0 : ((c >= w && !(edges & LRE_RIGHT)) ? w - 1 : dmin(c, w + 1));
+        const int lx = 2 * (x0 + xs) * (int)sizeof(P);
 #pragma unroll
-        for (int j = 0; j < PERL; j++, gl.next()) {
-            const int r = dmin(gl.r, h + 3) - 2, c = gl.c - 2;
-            const int xs = (c < 0 && !(edges & LRE_LEFT)) ? 0 : ((c >= w && !(edges & LRE_RIGHT)) ? w - 1 : dmin(c, w + 1));
+        for (int j = 0; j < PERL; j++) {
+            const int gr = lr0 + LRP * j;
+            const int r = dmin(gr, h + 3) - 2;
             const int rr = iclip(r, top ? -2 : 0, bottom ? h + 1 : h - 1);
             const bool from2 = rr < 0 || rr >= h, single = rr == -2 || rr == h + 1;
             const int ya = 2 * (y0 + rr) + (rr == -2 ? 2 : (rr == h + 1 ? -1 : 0));
             const uint8_t *ra = (from2 ? l2.ptr : l3.ptr) + __umul24(ya, lstride);
             const uint8_t *rb = single ? ra : ra + lstride;
-            const int lx = 2 * (x0 + xs);
-            const Q2 qa = *(const Q2 *)(ra + lx * sizeof(P)), qb = *(const Q2 *)(rb + lx * sizeof(P));
+            const Q2 qa = *(const Q2 *)(ra + lx), qb = *(const Q2 *)(rb + lx);
             constexpr int SB = 8 * sizeof(P);
             const int a0 = (int)(qa & ((1u << SB) - 1)), a1 = (int)(qa >> SB);
             const int b0 = (int)(qb & ((1u << SB) - 1)), b1 = (int)(qb >> SB);
             const int v = (a0 + (b0 & mb) + ((a1 + b1) & m1)) >> sh;
-            lvals[j] = (gl.r < h + 4 && c < w + 2) ? (uint16_t)v : 0;
+            lvals[j] = (gr < h + 4 && c < w + 2) ? (uint16_t)v : 0;
         }
     }
 #pragma unroll
     for (int j = 0; j < PERQ; j++) {
-        const int i = t + j * kThreads;
-        if (i < nvq) {
+        const int rr = r0 + RPP * j;
+        if (lane_on && rr < h + 2 * E) {
+            const int i = rr * VQ + cq;
             uint2 u;
             if constexpr (sizeof(P) == 1) {
                 const uint32_t b = (uint32_t)qv[j];
@@ -317,8 +310,8 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     if (uv_single) {
 #pragma unroll
         for (int j = 0; j < PERL; j++) {
-            const int i = t + j * kThreads;
-            if (i < (h + 4) * LS) Ld[i] = lvals[j];
+            const int gr = lr0 + LRP * j;
+            if (llane && gr < h + 4) Ld[gr * LS + lc] = lvals[j];
         }
     }
     if (kind == LR_KIND_PC && t < 64 * 8 * 4 / 16) ((uint4 *)sm.s_pcp)[t] = pcv;
