@@ -336,15 +336,16 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
     decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl], &dst = a.dst[pl];
     const Rav2dFilmGrainData &d = a.fg;
-    const int bd = a.bitdepth, bdm8 = bd - 8;
-    const int ctr = 128 << bdm8, gmin = -ctr, gmax = ctr - 1;
+    // 8-bit frames run with P = uint8_t, 10-bit frames with uint16_t
+    constexpr int bd = sizeof(P) == 1 ? 8 : 10, bdm8 = bd - 8, bdmax = (1 << bd) - 1;
+    constexpr int ctr = 128 << bdm8, gmin = -ctr, gmax = ctr - 1;
     const int sub = pl ? 1 : 0;
     const int lg = 4 + d.block_size - sub;  // log2 of the block size in this plane (16 << block_size luma)
     const int pbs = 1 << lg;
     const int16_t *lut = a.lut + pl * GH * GW;
     const uint8_t *scaling = a.scaling + (pl && d.chroma_scaling_from_luma ? 0 : pl) * 1024;
     const bool on = a.plane_on[pl];
-    int minv = 0, maxv = (1 << bd) - 1;
+    int minv = 0, maxv = bdmax;
     if (d.clip_to_restricted_range) {
         minv = 16 << bdm8;
         maxv = (pl == 0 || d.mc_identity ? 235 : 240) << bdm8;
@@ -414,7 +415,7 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
                 if (!d.chroma_scaling_from_luma) {
                     const int uv = pl - 1;
                     const int combined = __mul24(avg, d.uv_luma_mult[uv]) + __mul24(sv[k], d.uv_mult[uv]);
-                    val[k] = iclip((combined >> 6) + d.uv_offset[uv] * (1 << bdm8), 0, (1 << bd) - 1);
+                    val[k] = iclip((combined >> 6) + d.uv_offset[uv] * (1 << bdm8), 0, bdmax);
                 } else {
                     val[k] = avg;
                 }
@@ -427,7 +428,7 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         // would index past the scaling LUT, where the reference aborts; clamp
         int sc[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) sc[k] = scaling[dmin(val[k], (1 << bd) - 1)];
+        for (int k = 0; k < 4; k++) sc[k] = scaling[dmin(val[k], bdmax)];
         const int band = y >> lg, b = x >> lg;
         const int ly = y & (pbs - 1), lx0 = x & (pbs - 1);
         const int bh = dmin(pbs, H - (band << lg)), bw = dmin(pbs, W - (b << lg));
@@ -435,31 +436,38 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         const int xstart = (d.overlap_flag && b > 0) ? dmin(os, bw) : 0;
         const int16_t *g0 = lut + tix(o00[u], lx0, ly);
         int g[4] = {g0[0], g0[1], g0[2], g0[3]};
-        if (lx0 < xstart || ly < ystart) {
-            // overlap with the left and/or upper neighbour block (first 1-2 columns/rows)
+        // overlap with the left neighbour block (its first xstart <= 2 columns)
+        // and/or the upper one (first ystart <= 2 rows).  lx0 is a multiple of
+        // 4, so only a group that starts a block blends horizontally, and only
+        // its samples 0 and 1.
+        const bool hov = lx0 < xstart, vov = ly < ystart;
+        if (hov || vov) {
+            auto blend = [&](int n, int c, int wn, int wc) {
+                return iclip(round2(__mul24(n, wn) + __mul24(c, wc), 5), gmin, gmax);
+            };
+            // hov implies xstart >= 1: sample 0 always blends, sample 1 when
+            // xstart = 2 (both template entries are read either way)
+            auto hblend = [&](const int16_t *n, int (&c)[4]) {
+                const int n0 = n[0], n1 = n[1];
+                const int b1 = blend(n1, c[1], Wt(1, 0), Wt(1, 1));
+                c[0] = blend(n0, c[0], Wt(0, 0), Wt(0, 1));
+                c[1] = xstart > 1 ? b1 : c[1];
+            };
+            if (hov) hblend(lut + tix(o10[u], pbs, ly), g);
+            if (vov) {
+                const int16_t *gt = lut + tix(o01[u], lx0, ly + pbs);
+                int top[4] = {gt[0], gt[1], gt[2], gt[3]};
+                if (hov) hblend(lut + tix(o11[u], pbs, ly + pbs), top);
+                const int w0 = Wt(ly, 0), w1 = Wt(ly, 1);
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int lx = lx0 + k;
-                int gg = g[k];
-                if (lx < xstart)
-                    gg = iclip(round2(__mul24(lut[tix(o10[u], lx + pbs, ly)], Wt(lx, 0)) + __mul24(gg, Wt(lx, 1)), 5), gmin,
-                               gmax);
-                if (ly < ystart) {
-                    int top = lut[tix(o01[u], lx, ly + pbs)];
-                    if (lx < xstart)
-                        top = iclip(round2(__mul24(lut[tix(o11[u], lx + pbs, ly + pbs)], Wt(lx, 0)) + __mul24(top, Wt(lx, 1)),
-                                           5),
-                                    gmin, gmax);
-                    gg = iclip(round2(__mul24(top, Wt(ly, 0)) + __mul24(gg, Wt(ly, 1)), 5), gmin, gmax);
-                }
-                g[k] = gg;
+                for (int k = 0; k < 4; k++) g[k] = blend(top[k], g[k], w0, w1);
             }
         }
         int out[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int noise = round2(__mul24(sc[k], g[k]), d.scaling_shift);  // 8-bit x 12-bit
-            out[k] = iclip(sv[k] + noise, minv, maxv);
+            out[k] = dmin(dmax(sv[k] + noise, minv), maxv);
         }
         if (gv[u])
             store4((P *)(dbase + __umul24(gr[u], dstride)) + c4,
