@@ -329,9 +329,11 @@ __global__ __launch_bounds__(kSetupThreads) void grain_setup_batch(const GrainAr
 // grain sample of block offsets o at (lx, ly) is sample_lut's
 // lut[3 + s (3 + o[0]) + ly][3 + s (3 + o[1]) + lx] (s = 2 luma, 1 chroma);
 // the overlap rows and columns of a block blend it with the left / upper
-// neighbour's (filmgrain.rs:296-380).
+// neighbour's (filmgrain.rs:296-380).  The plane's scaling LUT (1 KiB) is
+// staged in `scl` (LDS) by one word per lane, so a lookup addresses LDS by the
+// sample value itself.
 template <typename P>
-__device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id) {
+__device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id, uint8_t *scl) {
     int pl, tx, ty;
     decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl], &dst = a.dst[pl];
@@ -360,6 +362,8 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
     const uint8_t *lbase = a.src[0].ptr + (uint64_t)(uint32_t)(2 * y0) * lstride + 2 * x0 * sizeof(P);
     const int t = threadIdx.x;
     const int c4 = (t & 15) * 4;  // column of the group in the tile
+    // first load: the scaling LUT word of this lane
+    const uint32_t sclw = on ? ((const uint32_t *)scaling)[t] : 0u;
     constexpr int NG = 2;
     int gr[NG];
     bool gv[NG];
@@ -394,6 +398,8 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
             if (gv[u]) store4((P *)(dbase + __umul24(gr[u], dstride)) + c4, sq[u]);
         return;
     }
+    ((uint32_t *)scl)[t] = sclw;  // on is tile-uniform: every lane reaches the barrier
+    __syncthreads();
     auto Wt = [&](int k, int j) -> int { return sub ? (k == 0 ? (j == 0 ? 23 : 22) : 0) : ((k == j) ? 27 : 17); };
     const int os = 2 >> sub;
     // template index of block offsets `o` (x byte high, y byte low) at (lx, ly)
@@ -428,7 +434,7 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         // would index past the scaling LUT, where the reference aborts; clamp
         int sc[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) sc[k] = scaling[dmin(val[k], bdmax)];
+        for (int k = 0; k < 4; k++) sc[k] = scl[dmin(val[k], bdmax)];
         const int band = y >> lg, b = x >> lg;
         const int ly = y & (pbs - 1), lx0 = x & (pbs - 1);
         const int bh = dmin(pbs, H - (band << lg)), bw = dmin(pbs, W - (b << lg));
@@ -476,12 +482,17 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
 }
 
 template <typename P> __global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
-    grain_apply_body<P>(a, blockIdx.x);
+    __shared__ __align__(4) uint8_t scl[1024];
+    grain_apply_body<P>(a, blockIdx.x, scl);
 }
 
 template <typename P, bool kLoop>
 __global__ __launch_bounds__(256) void grain_apply_batch(const GrainArgs *__restrict__ args, BatchSched s) {
-    for_each_item<kLoop>(s, [&](int f, int t) { grain_apply_body<P>(args[f], t); });
+    __shared__ __align__(4) uint8_t scl[1024];
+    for_each_item<kLoop>(s, [&](int f, int t) {
+        grain_apply_body<P>(args[f], t, scl);
+        if (kLoop) __syncthreads();  // the next item restages scl
+    });
 }
 
 // Per device: the Gaussian sequence in device memory and the LFSR jump-ahead
