@@ -49,11 +49,11 @@ PASS_NAMES = ["deblock_v", "deblock_h", "cdef_ccso", "loop_restoration", "film_g
 # probe's counters equal its known 256 MiB read / 256 MiB written); see
 # profiles/hbm_traffic_4k10_pmc.md.  Valid for the kernels profiled there.
 PMC_TRAFFIC_4K10 = {
-    "deblock_v": int(37638.1 * 2 * 1024 + 24300.0 * 1024),
-    "deblock_h": int(26610.4 * 2 * 1024 + 24300.0 * 1024),
-    "cdef_ccso": int(14821.2 * 2 * 1024 + 24804.9 * 1024),
-    "loop_restoration": int(46008.2 * 2 * 1024 + 24300.0 * 1024),
-    "film_grain": int(19942.0 * 2 * 1024 + 24300.1 * 1024),
+    "deblock_v": int(37637.2 * 2 * 1024 + 24300.0 * 1024),
+    "deblock_h": int(26609.6 * 2 * 1024 + 24300.0 * 1024),
+    "cdef_ccso": int(14787.1 * 2 * 1024 + 24801.0 * 1024),
+    "loop_restoration": int(46017.7 * 2 * 1024 + 24300.0 * 1024),
+    "film_grain": int(19939.8 * 2 * 1024 + 24301.5 * 1024),
 }
 
 

@@ -323,9 +323,10 @@ __global__ __launch_bounds__(kSetupThreads) void grain_setup_batch(const GrainAr
 // Display samples of one 64x32 tile: one lane = four horizontally adjacent
 // samples (a group never straddles a grain block: block sizes are multiples
 // of 4), two groups per lane (rows r and r + 16 of the tile).  Every load of
-// both groups that does not depend on another load is issued first (samples,
-// the collocated luma of chroma, the block offsets of the block and of its
-// left / upper neighbours), then the scaling and grain-template reads.  The
+// both groups that does not depend on another load is issued first (the
+// lane's word of the scaling LUT, samples, the collocated luma of chroma, the
+// block offsets of the block and of its left / upper neighbours), then the
+// grain-template reads.  The
 // grain sample of block offsets o at (lx, ly) is sample_lut's
 // lut[3 + s (3 + o[0]) + ly][3 + s (3 + o[1]) + lx] (s = 2 luma, 1 chroma);
 // the overlap rows and columns of a block blend it with the left / upper
