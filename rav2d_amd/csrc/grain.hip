@@ -438,27 +438,24 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         for (int k = 0; k < 4; k++) sc[k] = scl[dmin(val[k], bdmax)];
         const int band = y >> lg, b = x >> lg;
         const int ly = y & (pbs - 1), lx0 = x & (pbs - 1);
-        const int bh = dmin(pbs, H - (band << lg)), bw = dmin(pbs, W - (b << lg));
-        const int ystart = (d.overlap_flag && band > 0) ? dmin(os, bh) : 0;
-        const int xstart = (d.overlap_flag && b > 0) ? dmin(os, bw) : 0;
         const int16_t *g0 = lut + tix(o00[u], lx0, ly);
         int g[4] = {g0[0], g0[1], g0[2], g0[3]};
-        // overlap with the left neighbour block (its first xstart <= 2 columns)
-        // and/or the upper one (first ystart <= 2 rows).  lx0 is a multiple of
-        // 4, so only a group that starts a block blends horizontally, and only
-        // its samples 0 and 1.
-        const bool hov = lx0 < xstart, vov = ly < ystart;
+        // overlap with the left neighbour block (its first os <= 2 columns)
+        // and/or the upper one (first os rows).  The reference clips the
+        // overlap to the block's size at the plane's edge (min(os, bw)); plane
+        // sizes are multiples of 4 (frames are multiples of 8), so a clipped
+        // block still spans at least 4 > os samples.  lx0 is a multiple of 4:
+        // only a group that starts a block blends horizontally, and only its
+        // samples 0 and 1 (sample 1 for luma only, os = 2).
+        const bool hov = d.overlap_flag && b > 0 && lx0 == 0;
+        const bool vov = d.overlap_flag && band > 0 && ly < os;
         if (hov || vov) {
             auto blend = [&](int n, int c, int wn, int wc) {
                 return iclip(round2(__mul24(n, wn) + __mul24(c, wc), 5), gmin, gmax);
             };
-            // hov implies xstart >= 1: sample 0 always blends, sample 1 when
-            // xstart = 2 (both template entries are read either way)
             auto hblend = [&](const int16_t *n, int (&c)[4]) {
-                const int n0 = n[0], n1 = n[1];
-                const int b1 = blend(n1, c[1], Wt(1, 0), Wt(1, 1));
-                c[0] = blend(n0, c[0], Wt(0, 0), Wt(0, 1));
-                c[1] = xstart > 1 ? b1 : c[1];
+                c[0] = blend(n[0], c[0], Wt(0, 0), Wt(0, 1));
+                if (!sub) c[1] = blend(n[1], c[1], Wt(1, 0), Wt(1, 1));
             };
             if (hov) hblend(lut + tix(o10[u], pbs, ly), g);
             if (vov) {
