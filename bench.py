@@ -4,22 +4,29 @@
 One step = the whole in-loop filter chain (plus film grain when the frame has
 grain) over one synthetic 10-bit 4:2:0 3840x2160 frame whose input planes are
 already resident in HBM.  Prints one JSON line: Mpix/s of displayed samples,
-the roofline of the dominant pass (algorithmic bytes = each pass reads and
-writes its planes once) and the CPU baseline (the C restatement under oracle/
-on a bounded sample of the same frame).
+the roofline of the dominant kernel (by kernel time: HIP events the runtime
+stamps at each kernel's own start and end) and the CPU baseline.
 
 A second leg times a mixed batch of 64 frames (sizes uniform over 360p..2160p,
 8/10-bit 50/50, grain on the 10-bit frames) as 64 sequential per-frame
-launches and as one launch per pass (autotuned, hipGraph replay), and checks
-both give the same planes.  A third leg serves 10,000 jobs from 32 streams
-with Poisson arrivals (60 fps each) under an 80 GB HBM budget through the
-continuous-batching server and reports p50/p99 latency; every job's planes
-are hashed on the device and compared with the same frame filtered alone.
+launches and as one launch per pass (autotuned, hipGraph replay).  A third leg
+serves 10,000 jobs from 32 streams with Poisson arrivals (60 fps each) under an
+80 GB HBM budget through the continuous-batching server and reports p50/p99
+latency; every job's planes are hashed on the device and compared with the
+same frame filtered alone.
+
+The CPU-baseline leg runs the C restatement under oracle/ on the same seeded
+inputs: the 4K frame (one thread, whole frames for ~12 s) and the 64 batch
+frames (frame-parallel, one frame per thread).  It is also the checker: the
+first 4K oracle frame is compared with the GPU's output and display planes,
+and every batch frame's oracle planes with the batched GPU planes
+("bit_exact_vs_oracle").
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 4k10|1080p10|cif8]
                     [--batch 64] [--batch-steps 5] [--serve-jobs 10000] [--serve-streams 32]
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -31,78 +38,122 @@ sys.path.insert(0, ROOT)
 import rav2d_amd as R  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-
-# tools whose GPU passes exist; the CPU baseline runs the same chain
-BUILT = R.TOOL_ALL
+# integer VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD
+# every 4 cycles at 2.4 GHz
+VALU_PEAK_WINST_S = 1024 * 2.4e9 / 4
 
 CONFIGS = {
     # name: (w, h, bitdepth, sb128, tools)
-    "4k10": (3840, 2160, 10, 1, R.TOOL_ALL & BUILT),
-    "1080p10": (1920, 1080, 10, 1, R.TOOL_ALL & BUILT),
-    "cif8": (352, 288, 8, 0, R.TOOL_ALL & ~R.TOOL_GRAIN & BUILT),
+    "4k10": (3840, 2160, 10, 1, R.TOOL_ALL),
+    "1080p10": (1920, 1080, 10, 1, R.TOOL_ALL),
+    "cif8": (352, 288, 8, 0, R.TOOL_ALL & ~R.TOOL_GRAIN),
 }
 
 PASS_NAMES = ["deblock_v", "deblock_h", "cdef_ccso", "loop_restoration", "film_grain"]
+KERNEL_NAMES = ["deblock_v_kernel<u16>", "deblock_h_kernel<u16>", "cdef_kernel<u16>", "lr_kernel<u16>",
+                "grain_apply_kernel<u16>"]
 
-# HBM bytes per launch of each pass on the 4k10 workload, from two rocprofv3
-# --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0; the factors make the copy
-# probe's counters equal its known 256 MiB read / 256 MiB written); see
-# profiles/hbm_traffic_4k10_pmc.md.  Valid for the kernels profiled there.
-PMC_TRAFFIC_4K10 = {
-    "deblock_v": int(37637.2 * 2 * 1024 + 24300.0 * 1024),
-    "deblock_h": int(26609.6 * 2 * 1024 + 24300.0 * 1024),
-    "cdef_ccso": int(14787.1 * 2 * 1024 + 24801.0 * 1024),
-    "loop_restoration": int(46017.7 * 2 * 1024 + 24300.0 * 1024),
-    "film_grain": int(19939.8 * 2 * 1024 + 24301.5 * 1024),
+# Per-launch counters of the 4k10 workload's kernels from rocprofv3 --pmc runs
+# of `bench.py --steps 3 --warmup 1 --no-cpu --batch 0 --serve-jobs 0`
+# (profiles/hbm_traffic_4k10_pmc.md): FETCH_SIZE and WRITE_SIZE in KiB per
+# dispatch and SQ_INSTS_VALU in wave instructions.  HBM bytes = FETCH x 2.0 +
+# WRITE x 1.0 KiB: the factors make the copy probe's counters equal its known
+# 256 MiB read and 256 MiB written.  Valid for the kernels profiled there.
+PMC_4K10 = {
+    "deblock_v": (37637.2, 24300.0, 8.84e6),
+    "deblock_h": (26609.6, 24300.0, 8.37e6),
+    "cdef_ccso": (14787.1, 24801.0, 19.27e6),
+    "loop_restoration": (46017.7, 24300.0, 23.47e6),
+    "film_grain": (19939.8, 24301.5, 7.24e6),
 }
+
+
+def pmc_traffic(name):
+    f, w, _ = PMC_4K10[name]
+    return int(f * 2 * 1024 + w * 1024)
 
 
 def frame_samples(w, h):
     return w * h + 2 * (w // 2) * (h // 2)
 
 
-def cpu_baseline(w, h, bd, sb128, tools, seed, budget_s=12.0):
-    """The CPU oracle (scalar restatement of rav2d's chain), one thread, timed on
-    a bounded sample of the same workload: whole frames until ~budget_s."""
+def digest(planes):
+    h = hashlib.sha256()
+    for a in planes:
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def cpu_cores():
+    """Threads for the frame-parallel CPU leg: the cores this process may run
+    on, at most 16 (the host share that goes with one GPU here)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16)), n
+
+
+def cpu_baseline(w, h, bd, sb128, tools, seed, gpu_digests, batch_spec, batch_digests, budget_s=12.0):
+    """The CPU oracle (scalar restatement of rav2d's chain) as baseline and
+    checker.  Single thread: whole frames of the bench workload for ~budget_s;
+    the first one is compared with the GPU planes.  Multi-core: the batch
+    frames, one frame per thread, each compared with the batched GPU planes."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O  # the oracle is only the CPU-baseline leg here
+    from concurrent.futures import ThreadPoolExecutor
 
     sf = R.SynthFrame(w, h, bd, sb128, seed, tools=tools)
     grain = bool(sf.params.grain_present)
 
-    def step():
-        out = O.filter_frame(sf)
-        if grain:
-            O.apply_grain(sf, out)
+    def chain(src):
+        out = O.filter_frame(src)
+        disp = O.apply_grain(src, out) if src.params.grain_present else None
+        return out, disp
 
-    step()  # warm
+    out, disp = chain(sf)  # warm, and the checked frame
+    exact_4k = {"output": digest(out) == gpu_digests[0]}
+    if grain:
+        exact_4k["display"] = digest(disp) == gpu_digests[1]
     n, t0 = 0, time.perf_counter()
     while True:
-        step()
+        chain(sf)
         n += 1
         dt = time.perf_counter() - t0
         if dt >= budget_s or n >= 200:
             break
-    # frame-parallel: one frame per thread (the oracle runs outside the GIL)
-    from concurrent.futures import ThreadPoolExecutor
-    cores = max(1, min(os.cpu_count() or 1, 16))
-    per = max(1, int(round(budget_s / max(dt / n, 1e-3) * 0.5)))  # frames per thread, ~half the budget
-
-    def worker(_):
-        for _ in range(per):
-            step()
-        return per
-
-    t1 = time.perf_counter()
-    with ThreadPoolExecutor(cores) as ex:
-        nm = sum(ex.map(worker, range(cores)))
-    dtm = time.perf_counter() - t1
     sf.close()
-    return {"value": round(n * w * h / 1e6 / dt, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": "%d whole %dx%d %d-bit frames through the oracle chain%s in %.1f s" % (
-                n, w, h, bd, " + film grain" if grain else "", dt),
-            "multi_core": {"value": round(nm * w * h / 1e6 / dtm, 3), "cores": cores,
-                           "sample": "%d frames, one per thread, in %.1f s" % (nm, dtm)}}
+    res = {"value": round(n * w * h / 1e6 / dt, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+           "sample": "%d whole %dx%d %d-bit frames through the oracle chain%s in %.1f s (one thread)" % (
+               n, w, h, bd, " + film grain" if grain else "", dt)}
+    exact = {"4k_frame": exact_4k}
+    if batch_spec:
+        cores, affinity = cpu_cores()
+
+        def one(i):
+            c = batch_spec[i]
+            src = R.SynthFrame(c[0], c[1], c[2], c[3], c[4], tools=c[5])
+            try:
+                t = time.perf_counter()
+                o, d = chain(src)
+                el = time.perf_counter() - t
+                ok = digest(o) == batch_digests[i][0] and (d is None or digest(d) == batch_digests[i][1])
+                return ok, el
+            finally:
+                src.close()
+
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(cores) as ex:
+            rs = list(ex.map(one, range(len(batch_spec))))
+        dtm = time.perf_counter() - t1
+        pix = sum(c[0] * c[1] for c in batch_spec)
+        exact["batch_frames"] = {"frames": len(rs), "equal": sum(1 for r in rs if r[0])}
+        res["multi_core"] = {"value": round(pix / 1e6 / dtm, 3), "unit": "Mpix/s", "cores": cores,
+                             "cores_in_affinity": affinity,
+                             "sample": "the %d mixed batch frames (%.1f Mpix), one frame per thread, in %.1f s "
+                                       "(wall, including each frame's synthetic input generation)" % (
+                                           len(rs), pix / 1e6, dtm)}
+    return res, exact
 
 
 BATCH_SIZES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
@@ -130,7 +181,6 @@ def make_sources(spec):
 
 def bench_batch(ctx, spec, sfs, steps, warmup):
     """Mixed batch: per-frame launches vs one launch per pass (graph)."""
-    import hashlib
     n = len(spec)
     pix = sum(c[0] * c[1] for c in spec)
     A = [R.GpuFrame.from_synth(ctx, sf) for sf in sfs]  # per-frame form
@@ -138,7 +188,7 @@ def bench_batch(ctx, spec, sfs, steps, warmup):
     grain = [bool(sf.params.grain_present) for sf in sfs]
 
     def per_frame():
-        for f, g in zip(A, grain):
+        for f in A:
             f.filter()
             f.apply_grain()
 
@@ -161,7 +211,7 @@ def bench_batch(ctx, spec, sfs, steps, warmup):
     for _ in range(steps):
         b.run()
     bt_ms = ctx.timer_stop() / steps
-    # per-pass durations of the batched launches (eager run of the same shapes)
+    # kernel time of each pass's batched launches (eager run of the same shapes)
     ecfg = b.config()
     ecfg.use_graph = 0
     b.set_config(ecfg)
@@ -171,22 +221,14 @@ def bench_batch(ctx, spec, sfs, steps, warmup):
         b.run()
     times = ctx.pass_times(reset=True)
     ctx.set_profiling(False)
-    pass_us, pass_bytes = {}, {}
+    pass_us = {}
     for i, name in enumerate(PASS_NAMES):
         tot, cnt = times[i]
         if cnt:
-            pass_us[name] = round(tot / cnt * 1e3, 1)
-            pass_bytes[name] = b.info(i)[1]
+            pass_us[name] = round(tot / steps * 1e3, 1)  # kernel time per batch run
 
-    def digest(frames):
-        h = hashlib.sha256()
-        for f in frames:
-            for which in (1, 2):
-                for a in f.download(which):
-                    h.update(a.tobytes())
-        return h.hexdigest()
-
-    exact = digest(A) == digest(B)
+    dig_a = [(digest(f.download(1)), digest(f.download(2)) if g else None) for f, g in zip(A, grain)]
+    dig_b = [(digest(f.download(1)), digest(f.download(2)) if g else None) for f, g in zip(B, grain)]
     b.close()
     for f in A + B:
         f.close()
@@ -201,12 +243,12 @@ def bench_batch(ctx, spec, sfs, steps, warmup):
         "per_frame_mpix_s": round(pix / 1e6 / (pf_ms / 1e3), 1),
         "batched_mpix_s": round(pix / 1e6 / (bt_ms / 1e3), 1),
         "speedup": round(pf_ms / bt_ms, 3),
-        "batched_bit_exact_vs_per_frame": exact,
+        "batched_bit_exact_vs_per_frame": dig_a == dig_b,
         "autotuned": cfg.as_dict(),
         "autotune_s": round(tune_s, 2),
-        "pass_us": pass_us,
+        "kernel_us_per_batch": pass_us,
     }
-    return res, times, pass_bytes
+    return res, dig_b
 
 
 def bench_serve(spec, sfs, n_streams, slots, fps, n_jobs, seed, budget):
@@ -228,6 +270,12 @@ def bench_serve(spec, sfs, n_streams, slots, fps, n_jobs, seed, budget):
         cfg.checksum = 1
         srv = R.GpuServer(ctx, cfg)
         try:
+            # warm-up: every stream's first frame once through the server, so the
+            # batched kernels of both bit depths have run before the clock starts
+            for i, fs in enumerate(frames):
+                srv.submit(fs[0], 1 << 40 | i)
+            srv.drain()
+            srv.poll(max_results=len(frames))
             res, js, jl = srv.simulate(frames, slots, fps, n_jobs, seed)
         finally:
             srv.close()
@@ -263,14 +311,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="4k10", choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline (and oracle check) leg")
     ap.add_argument("--batch", type=int, default=64, help="frames of the mixed batch leg (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-jobs", type=int, default=10000, help="jobs of the serving leg (0 = skip)")
     ap.add_argument("--serve-streams", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
+        print("bench.py: this build measures one GPU only (the filter stage has no multi-GPU path); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     w, h, bd, sb128, tools = CONFIGS[args.config]
     print("bench: building %s workload %dx%d %d-bit" % (args.config, w, h, bd), file=sys.stderr, flush=True)
@@ -296,8 +345,11 @@ def main():
     wall = (time.perf_counter() - wall0) * 1e3
     ms_per_step = ms / args.steps
     mpix = w * h * args.steps / (ms / 1e3) / 1e6
+    gpu_digests = (digest(f.download(1)), digest(f.download(2)) if grain else None)
 
-    # per-pass kernel durations over a second run of the same K steps
+    # kernel time of every pass over a second run of the same K steps: the
+    # runtime hands each pass's kernel a start/stop event pair stamped at the
+    # kernel's own start and end (on the stream it runs on)
     ctx.set_profiling(True)
     ctx.pass_times(reset=True)
     for _ in range(args.steps):
@@ -306,21 +358,27 @@ def main():
     ctx.set_profiling(False)
     bps = 2 if bd > 8 else 1
     S = frame_samples(w, h)
+    alg_bytes = 2 * S * bps  # one read and one write of the frame's planes per pass
     passes = {}
     for i, name in enumerate(PASS_NAMES):
         tot, cnt = times[i]
         if cnt:
-            passes[name] = round(tot / cnt * 1e3, 2)  # us per launch
-    # dominant kernel: the pass with the largest total time; its algorithmic
-    # bytes are one read of its input planes and one write of its output planes
-    dom = max(range(5), key=lambda i: times[i][0])
-    dom_us = times[dom][0] / max(times[dom][1], 1) * 1e3
-    alg_bytes = 2 * S * bps
+            us = tot / cnt * 1e3
+            passes[name] = {"kernel": KERNEL_NAMES[i] if bd > 8 else KERNEL_NAMES[i].replace("u16", "u8"),
+                            "avg_us": round(us, 2), "launches": cnt,
+                            "hbm_frac": round(alg_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+            if args.config == "4k10":
+                valu = PMC_4K10[name][2]
+                passes[name]["valu_frac"] = round(valu / VALU_PEAK_WINST_S / (us * 1e-6), 4)
+    # dominant kernel: the one with the longest kernel time
+    dom = max(passes, key=lambda k: passes[k]["avg_us"])
+    dom_us = passes[dom]["avg_us"]
     achieved = alg_bytes / (dom_us * 1e-6) / 1e9
     frame_bytes = S * bps * (2 + (1 if grain else 0))
     chain_gbs = frame_bytes / (ms_per_step * 1e-3) / 1e9
 
     batch = serve = None
+    batch_spec, batch_digests = None, None
     n_src = max(args.batch, args.serve_streams if args.serve_jobs > 0 else 0)
     if n_src > 0:
         spec = batch_workload(n_src, args.seed)
@@ -328,7 +386,8 @@ def main():
         sfs = make_sources(spec)
         if args.batch > 0:
             print("bench: mixed batch of %d frames" % args.batch, file=sys.stderr, flush=True)
-            batch, _, _ = bench_batch(ctx, spec[:args.batch], sfs[:args.batch], args.batch_steps, 1)
+            batch, batch_digests = bench_batch(ctx, spec[:args.batch], sfs[:args.batch], args.batch_steps, 1)
+            batch_spec = spec[:args.batch]
         if args.serve_jobs > 0:
             print("bench: serving %d jobs from %d streams" % (args.serve_jobs, args.serve_streams),
                   file=sys.stderr, flush=True)
@@ -338,11 +397,23 @@ def main():
 
     copy_gbs = ctx.bandwidth_probe(256 << 20, 20)
 
-    cpu = None
+    cpu = exact = None
     if not args.no_cpu:
-        print("bench: CPU baseline leg", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(w, h, bd, sb128, tools, args.seed)
+        print("bench: CPU baseline and oracle check leg", file=sys.stderr, flush=True)
+        cpu, exact = cpu_baseline(w, h, bd, sb128, tools, args.seed, gpu_digests, batch_spec, batch_digests)
+        if batch is not None:
+            batch["bit_exact_vs_oracle"] = exact["batch_frames"]["equal"] == exact["batch_frames"]["frames"]
 
+    roof = {"bound": "hbm", "kernel": passes[dom]["kernel"], "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": pmc_traffic(dom) if args.config == "4k10" else None,
+            "algorithmic_bytes": alg_bytes, "kernel_us": dom_us,
+            "chain_frac": round(chain_gbs / HBM_PEAK_GBS, 4),
+            "copy_probe_GBps": round(copy_gbs, 1),
+            "frac_of_copy_probe": round(achieved / copy_gbs, 4)}
+    if args.config == "4k10":
+        roof["valu"] = {"insts_valu": PMC_4K10[dom][2], "peak_winst_per_s": VALU_PEAK_WINST_S,
+                        "frac": passes[dom]["valu_frac"]}
     line = {
         "metric": "bit-exact filtered Mpix/s, 10-bit 4K, batched vs per-frame; p99 ms/frame",
         "value": round(mpix, 2),
@@ -357,17 +428,15 @@ def main():
         "vs_baseline": None,
         "dtype": "u16" if bd > 8 else "u8",
         "data": "synthetic",
+        "bit_exact_vs_oracle": None if exact is None else all(exact["4k_frame"].values()) and (
+            batch is None or batch["bit_exact_vs_oracle"]),
+        "oracle_check": exact,
         "config": {"workload": "per-frame launch, %dx%d %d-bit 4:2:0, sb%d, in-loop chain%s" % (
             w, h, bd, 64 << sb128, " + film grain" if grain else ""),
-            "passes_built": [n for n in PASS_NAMES if n in passes],
             "wall_ms_timed_region": round(wall, 2),
             "chain_GBps_algorithmic": round(chain_gbs, 1),
-            "pass_us": passes},
-        "roofline": {"bound": "hbm", "kernel": PASS_NAMES[dom], "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": PMC_TRAFFIC_4K10.get(PASS_NAMES[dom]) if args.config == "4k10" else None,
-                     "copy_probe_GBps": round(copy_gbs, 1),
-                     "frac_of_copy_probe": round(achieved / copy_gbs, 4)},
+            "passes": passes},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "batch": batch,
         "serving": serve,

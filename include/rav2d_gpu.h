@@ -35,6 +35,9 @@ extern "C" {
 #define RAV2D_EAGAIN (-11)  /* admission rejected (memory budget / slots)  */
 #define RAV2D_ENOENT (-2)   /* unknown ticket / stream                     */
 
+/* Largest frame width and height the library accepts (luma samples). */
+#define RAV2D_MAX_FRAME_DIM 16384
+
 /* ---- enums with the reference's values --------------------------------- */
 /* PixelLayout, crates/rav2d/src/headers.rs:112-118 */
 #define RAV2D_PIXEL_LAYOUT_I400 0
@@ -217,7 +220,12 @@ int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f);
 int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask,
                                   size_t n_mask, const Rav2dAv2Restoration *lr,
                                   size_t n_lr, void *stream);
-/* Replace the frame parameters (same size, bit depth and layout). */
+/* Replace the frame parameters (same size, bit depth and layout).  This and
+ * rav2d_gpu_frame_set_side_info first wait for the frame's earlier
+ * rav2d_gpu_filter_frame / apply_grain / run_pass launches and its grain
+ * set-up, whose device tables they rewrite; a grain set-up built for the old
+ * parameters is dropped.  A frame held by a batch run or a server job in
+ * flight must not be changed (the batch and server sections say so). */
 int rav2d_gpu_frame_set_params(Rav2dGpuFrame *f, const Rav2dFrameParams *params);
 /* Device planes of the frame: which = 0 input F0, 1 output, 2 display. */
 int rav2d_gpu_frame_planes(const Rav2dGpuFrame *f, int which, Rav2dPlanes *out);
@@ -327,7 +335,10 @@ typedef struct Rav2dServeResult {
     int32_t status;        /* RAV2D_OK or the error of its batch */
     int32_t batch_frames;  /* jobs in its batch */
 } Rav2dServeResult;
-/* max_batch 16, max_inflight 3, apply_grain 1, checksum 0. */
+/* max_batch 16, max_inflight 2, apply_grain 1, checksum 0.  Each slot owns
+ * one HIP stream; the slots share the context's auxiliary stream for grain
+ * set-ups, and the argument buffers of a max_batch batch are allocated per
+ * slot by rav2d_gpu_server_create (the dispatcher allocates nothing). */
 void rav2d_gpu_server_default_config(Rav2dServeConfig *cfg);
 int rav2d_gpu_server_create(Rav2dGpuCtx *ctx, const Rav2dServeConfig *cfg, Rav2dGpuServer **out);
 /* Finishes every submitted job, then stops the dispatcher. */

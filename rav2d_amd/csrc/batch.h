@@ -45,6 +45,8 @@ struct LaunchShape {
 int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply_grain, HostBatch &hb);
 // Bytes of the packed argument block.
 size_t packed_size(const HostBatch &hb);
+// An upper bound of packed_size for any batch of at most n frames.
+size_t packed_bound(int n);
 // Packs hb into `host` (packed_size bytes) laid out for device address `dev`.
 void pack_batch(const HostBatch &hb, uint8_t *host, uint8_t *dev, DevBatch &db);
 // Makes the constant tables of the current device ready (outside any capture).

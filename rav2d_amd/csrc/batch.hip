@@ -134,6 +134,14 @@ int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply
 
 size_t packed_size(const HostBatch &hb) { return layout_of(hb).total; }
 
+size_t packed_bound(int n) {
+    // per frame at most: both deblock passes, CDEF, LR, grain apply and set-up,
+    // one prefix entry per pass; each of the 21 arrays pads to 256 bytes
+    const size_t per = 2 * sizeof(DeblockArgs) + sizeof(CdefArgs) + sizeof(LrArgs) + 2 * sizeof(GrainArgs) +
+                       kPasses * sizeof(int32_t);
+    return (size_t)std::max(n, 1) * per + (4 * kPasses + 1) * 256 + 256;
+}
+
 void pack_batch(const HostBatch &hb, uint8_t *host, uint8_t *dev, DevBatch &db) {
     const Layout L = layout_of(hb);
     auto put = [&](size_t o, const void *src, size_t bytes) {
@@ -190,10 +198,10 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
             if (par) HIP_TRY(hipStreamWaitEvent(ec.side2, ec.ev_join, 0));
         }
         if (p == ec.time_pass) HIP_TRY(hipEventRecord(ec.ev_a, s));
-        PassTimer pt(ec.profile, p, s);
         for (int g = 0; g < 2; g++) {
             if (db.total[p][g] == 0) continue;
             const hipStream_t sg = (par && g == 0) ? ec.side2 : s;
+            PassTimer pt(ec.profile, p, sg);
             BatchSched sc;
             sc.prefix = db.prefix[p][g];
             sc.n_frames = db.nf[p][g];

@@ -631,23 +631,23 @@ void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, in
     const bool loop = grid < s.total;
     if (bitdepth > 8) {
         if (loop)
-            hipLaunchKernelGGL((cdef_batch<uint16_t, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+            R2G_LAUNCH((cdef_batch<uint16_t, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
         else
-            hipLaunchKernelGGL((cdef_batch<uint16_t, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+            R2G_LAUNCH((cdef_batch<uint16_t, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
     } else {
         if (loop)
-            hipLaunchKernelGGL((cdef_batch<uint8_t, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+            R2G_LAUNCH((cdef_batch<uint8_t, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
         else
-            hipLaunchKernelGGL((cdef_batch<uint8_t, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+            R2G_LAUNCH((cdef_batch<uint8_t, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
     }
 }
 
 void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s) {
     dim3 grid(((a.g.bw * 4 + 63) / 64) * ((a.g.bh * 4 + 63) / 64));
     if (bitdepth > 8)
-        hipLaunchKernelGGL(cdef_kernel<uint16_t>, grid, dim3(kThreads), 0, s, a);
+        R2G_LAUNCH(cdef_kernel<uint16_t>, grid, dim3(kThreads), 0, s, a);
     else
-        hipLaunchKernelGGL(cdef_kernel<uint8_t>, grid, dim3(kThreads), 0, s, a);
+        R2G_LAUNCH(cdef_kernel<uint8_t>, grid, dim3(kThreads), 0, s, a);
 }
 
 }  // namespace r2g

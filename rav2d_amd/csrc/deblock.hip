@@ -328,7 +328,7 @@ void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched
                           hipStream_t st) {
     if (s.total <= 0) return;
     const bool loop = grid < s.total;
-#define R2G_DB_LAUNCH(K, P, L) hipLaunchKernelGGL((K<P, L>), dim3(grid), dim3(kThreads), 0, st, d_args, s)
+#define R2G_DB_LAUNCH(K, P, L) R2G_LAUNCH((K<P, L>), dim3(grid), dim3(kThreads), 0, st, d_args, s)
 #define R2G_DB_PICK(K)                                     \
     do {                                                   \
         if (bitdepth > 8) {                                \
@@ -367,9 +367,9 @@ void launch_deblock_v(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
     a.map = deblock_tile_map(a);
     if (a.map.first[3] == 0) return;
     if (bitdepth > 8)
-        hipLaunchKernelGGL(deblock_v_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+        R2G_LAUNCH(deblock_v_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
     else
-        hipLaunchKernelGGL(deblock_v_kernel<uint8_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+        R2G_LAUNCH(deblock_v_kernel<uint8_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
 }
 
 void launch_deblock_h(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
@@ -377,9 +377,9 @@ void launch_deblock_h(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
     a.map = deblock_tile_map(a);
     if (a.map.first[3] == 0) return;
     if (bitdepth > 8)
-        hipLaunchKernelGGL(deblock_h_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+        R2G_LAUNCH(deblock_h_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
     else
-        hipLaunchKernelGGL(deblock_h_kernel<uint8_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+        R2G_LAUNCH(deblock_h_kernel<uint8_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
 }
 
 }  // namespace r2g

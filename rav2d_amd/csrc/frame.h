@@ -53,29 +53,44 @@ inline hipEvent_t take_event(Rav2dGpuCtx *c) {
     return e;
 }
 
-// Brackets one pass launch with events when profiling is on.
 // Events that only order work between streams of one device: no timing and no
 // system-scope fence (which would write back and invalidate the L2 caches at
 // every fork and join); the kernels' own device-scope fences order the data.
 constexpr unsigned kStreamSyncEventFlags = hipEventDisableTiming | hipEventDisableSystemFence;
 
+// Times one pass launch when profiling is on: arm() hands two events to the
+// next kernel launched on this thread (its own start and end, launch gaps
+// excluded); the constructor arms unless told not to (a pass whose timed
+// kernel follows another launch arms just before it).
 struct PassTimer {
     Rav2dGpuCtx *c;
     int pass;
-    hipStream_t s;
-    hipEvent_t a = nullptr;
-    PassTimer(Rav2dGpuCtx *c_, int pass_, hipStream_t s_) : c(c_), pass(pass_), s(s_) {
+    hipEvent_t a = nullptr, b = nullptr;
+    PassTimer(Rav2dGpuCtx *c_, int pass_, hipStream_t s, bool arm_now = true) : c(c_), pass(pass_) {
         if (c && c->profiling && s == c->stream) {
             a = take_event(c);
-            if (a) (void)hipEventRecord(a, s);
+            b = take_event(c);
+            if (arm_now) arm();
         }
     }
+    void arm() {
+        if (!a || !b) return;
+        LaunchTiming &lt = launch_timing();
+        lt.start = a;
+        lt.stop = b;
+        lt.used = false;
+    }
     ~PassTimer() {
-        if (!a) return;
-        hipEvent_t b = take_event(c);
-        if (!b) return;
-        (void)hipEventRecord(b, s);
-        c->recs.push_back({pass, a, b});
+        LaunchTiming &lt = launch_timing();
+        const bool used = a && b && lt.used && !lt.start;
+        lt.start = lt.stop = nullptr;
+        lt.used = false;
+        if (used) {
+            c->recs.push_back({pass, a, b});
+        } else {
+            if (a) c->free_events.push_back(a);
+            if (b) c->free_events.push_back(b);
+        }
     }
 };
 
@@ -111,7 +126,12 @@ struct Rav2dGpuFrame {
     int lr_n_restore = 0;  // lr_items: restoration items first, then plane copies
     bool lr_dirty = true;
     uint8_t *d_grain = nullptr;  // grain LUTs + scaling + offsets scratch
+    uint32_t *d_grain_err = nullptr;  // error flags of the grain set-up (last word of d_grain)
     hipEvent_t ev_setup = nullptr, ev_free = nullptr;
+    // recorded after the frame's last per-frame launch (filter, grain, pass):
+    // parameter and side-info changes wait for it before rewriting device tables
+    hipEvent_t ev_done = nullptr;
+    bool done_pending = false;
     bool grain_ready = false, grain_used = false;
     uint64_t gen = 0;  // bumped when parameters or side info change
 };
@@ -132,6 +152,7 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp);
 DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl);
 bool frame_has_grain(const Rav2dFrameParams &P);
 int grain_args(Rav2dGpuFrame *f, GrainArgs &a);
+int frame_errors(Rav2dGpuFrame *f);  // after a synchronization: RAV2D_EDEVICE if a kernel flagged an error
 hipStream_t pick_stream(Rav2dGpuCtx *ctx, void *stream);
 
 }  // namespace r2g

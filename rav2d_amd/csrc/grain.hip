@@ -88,7 +88,7 @@ constexpr int kLumaSteps = (GW - 1) + 4 * (GH - 1) + 1;
 
 template <bool kChroma>
 __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7], int ar_shift, int gmin, int gmax,
-                                        const int16_t (*luma)[GW], int wl, int *done) {
+                                        const int16_t (*luma)[GW], int wl, int *done, uint32_t *err) {
     constexpr int W = kChroma ? SGW : GW, H = kChroma ? SGH : GH;
     constexpr int kSteps = (W - 1) + 4 * (H - 1) + 1;
     static_assert(!kChroma || 32 * ((kSteps + 15) / 16 - 1) + 21 <= 16 * ((kLumaSteps + 15) / 16),
@@ -110,12 +110,20 @@ __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7]
     for (int b = 0; b < (kSteps + 15) / 16; b++) {
         if (!kChroma && b == 8 && lane < H - 64) y = lane + 64;
         if (kChroma && wl != 0) {
-            // bounded wait (the luma wave never waits, so the count always arrives)
+            // bounded wait (the luma wave never waits, so the count always
+            // arrives); should it not, the error word tells the host, which then
+            // fails the frame's next synchronizing call instead of returning
+            // grain built from unfinished luma rows
             const int need = 32 * b + 21;
+            bool arrived = false;
             for (int spin = 0; spin < (1 << 20); spin++) {
-                if (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= need) break;
+                if (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= need) {
+                    arrived = true;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
+            if (!arrived && lane == 0 && err) atomicOr(err, kGrainErrLumaWait);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
         const bool rows_ok = y >= 3 && y < H;
@@ -189,8 +197,12 @@ __device__ __forceinline__ void scaling_luts(const Rav2dFilmGrainData &d, int bd
             if (num > 0) {
                 int c = e >> sx, n = e & (pad - 1);
                 v = first_pass(c);
-                if (n && e >= (pts[0][0] << sx) && e < (pts[num - 1][0] << sx)) {
-                    // inside some segment: interpolate like the second pass
+                if (n) {
+                    // inside a segment (dx > 0): the second pass interpolates
+                    // between the coarse entries around it, the same values
+                    // whichever segment writes last; elsewhere the first
+                    // pass's fills hold (points that repeat or go back, which
+                    // the reference tolerates by skipping dx <= 0 segments)
                     bool in_seg = false;
                     for (int k = 0; k < num - 1; k++) {
                         int bx = pts[k][0] << sx, ex = pts[k + 1][0] << sx;
@@ -287,12 +299,13 @@ __device__ __forceinline__ void grain_setup_body(const GrainArgs &a, const int16
     // the three causal AR filters side by side (chroma after the luma samples
     // it reads), while the fourth wave builds the scaling LUTs and the offsets
     if (wave == 0) {
-        ar_wave<false>(lut[0], win[0], (int)d.ar_coeff_shift, gmin, gmax, nullptr, 0, &sm.luma_done);
+        ar_wave<false>(lut[0], win[0], (int)d.ar_coeff_shift, gmin, gmax, nullptr, 0, &sm.luma_done, a.err);
     } else if (wave < 3) {
         const int uv = wave - 1;
         const int wl = d.num_points[0] > 0 ? d.ar_coeffs[1 + uv][2 * lag * (lag + 1)] : 0;
         if (uv == 0 ? want_u : want_v)
-            ar_wave<true>(lut[1 + uv], win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl, &sm.luma_done);
+            ar_wave<true>(lut[1 + uv], win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl, &sm.luma_done,
+                          a.err);
     } else {
         scaling_luts(d, bd, sm.scl, lane);
         band_offsets(a, lane);
@@ -543,7 +556,7 @@ static const int16_t *grain_tables() {
 void launch_grain_setup(const GrainArgs &a, hipStream_t s) {
     const int16_t *gauss = grain_tables();
     if (!gauss) return;
-    hipLaunchKernelGGL(grain_setup_kernel, dim3(1), dim3(kSetupThreads), 0, s, a, gauss);
+    R2G_LAUNCH(grain_setup_kernel, dim3(1), dim3(kSetupThreads), 0, s, a, gauss);
 }
 
 bool grain_init_tables() { return grain_tables() != nullptr; }
@@ -551,7 +564,7 @@ bool grain_init_tables() { return grain_tables() != nullptr; }
 void launch_grain_setup_batch(const GrainArgs *d_args, int n, hipStream_t s) {
     const int16_t *gauss = grain_tables();
     if (!gauss || n <= 0) return;
-    hipLaunchKernelGGL(grain_setup_batch, dim3(n), dim3(kSetupThreads), 0, s, d_args, gauss);
+    R2G_LAUNCH(grain_setup_batch, dim3(n), dim3(kSetupThreads), 0, s, d_args, gauss);
 }
 
 void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &sc, int grid, int bitdepth, hipStream_t s) {
@@ -559,14 +572,14 @@ void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &sc, int
     const bool loop = grid < sc.total;
     if (bitdepth > 8) {
         if (loop)
-            hipLaunchKernelGGL((grain_apply_batch<uint16_t, true>), dim3(grid), dim3(256), 0, s, d_args, sc);
+            R2G_LAUNCH((grain_apply_batch<uint16_t, true>), dim3(grid), dim3(256), 0, s, d_args, sc);
         else
-            hipLaunchKernelGGL((grain_apply_batch<uint16_t, false>), dim3(grid), dim3(256), 0, s, d_args, sc);
+            R2G_LAUNCH((grain_apply_batch<uint16_t, false>), dim3(grid), dim3(256), 0, s, d_args, sc);
     } else {
         if (loop)
-            hipLaunchKernelGGL((grain_apply_batch<uint8_t, true>), dim3(grid), dim3(256), 0, s, d_args, sc);
+            R2G_LAUNCH((grain_apply_batch<uint8_t, true>), dim3(grid), dim3(256), 0, s, d_args, sc);
         else
-            hipLaunchKernelGGL((grain_apply_batch<uint8_t, false>), dim3(grid), dim3(256), 0, s, d_args, sc);
+            R2G_LAUNCH((grain_apply_batch<uint8_t, false>), dim3(grid), dim3(256), 0, s, d_args, sc);
     }
 }
 
@@ -582,9 +595,9 @@ void grain_tile_map(GrainArgs &a) {
 void launch_grain_apply(GrainArgs a, int bitdepth, hipStream_t s) {
     grain_tile_map(a);
     if (bitdepth > 8)
-        hipLaunchKernelGGL(grain_apply_kernel<uint16_t>, dim3(a.map.first[3]), dim3(256), 0, s, a);
+        R2G_LAUNCH(grain_apply_kernel<uint16_t>, dim3(a.map.first[3]), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(grain_apply_kernel<uint8_t>, dim3(a.map.first[3]), dim3(256), 0, s, a);
+        R2G_LAUNCH(grain_apply_kernel<uint8_t>, dim3(a.map.first[3]), dim3(256), 0, s, a);
 }
 
 }  // namespace r2g

@@ -1,5 +1,6 @@
 // Host-side internals shared by the runtime and the kernel launchers.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -105,6 +106,9 @@ int lr_tables(int device, LrTables *out);
 
 // Film grain (filmgrain.rs): LUTs, scaling and block offsets live in a
 // per-frame device scratch area written by the setup kernel.
+// Error flag of the set-up: a chroma AR wave's bounded wait for the luma
+// wave's progress ran out (the luma wave never waits, so this is not expected).
+constexpr uint32_t kGrainErrLumaWait = 1u;
 struct GrainArgs {
     DevPlane src[3], dst[3];
     Rav2dFilmGrainData fg;
@@ -113,6 +117,7 @@ struct GrainArgs {
     int16_t *lut;        // [3][73][82]
     uint8_t *scaling;    // [3][1024]
     uint8_t *offsets;    // [n_bands][n_blocks][2]
+    uint32_t *err;       // error flags the set-up raises (kGrainErr*); may be null
     int32_t n_bands, n_blocks;
     int32_t plane_on[3];
     TileMap map;
@@ -132,5 +137,28 @@ void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched
 TileMap deblock_tile_map(const DeblockArgs &a);  // 64x32 tiles of the three planes
 
 void set_last_error(const std::string &msg);
+
+// Kernel-accurate timing of one launch.  While armed, the next kernel launched
+// through R2G_LAUNCH on this thread carries the two events, which the runtime
+// stamps at that kernel's own start and end (hipExtLaunchKernelGGL), so the
+// interval excludes launch gaps; the slot is then disarmed.
+struct LaunchTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool used = false;
+};
+LaunchTiming &launch_timing();
+
+#define R2G_LAUNCH(K, G, B, SH, S, ...)                                                    \
+    do {                                                                                   \
+        ::r2g::LaunchTiming &lt_ = ::r2g::launch_timing();                                 \
+        if (lt_.start) {                                                                   \
+            hipEvent_t a_ = lt_.start, b_ = lt_.stop;                                      \
+            lt_.start = lt_.stop = nullptr;                                                \
+            lt_.used = true;                                                               \
+            hipExtLaunchKernelGGL(K, G, B, SH, S, a_, b_, 0u, __VA_ARGS__);                \
+        } else {                                                                           \
+            hipLaunchKernelGGL(K, G, B, SH, S, __VA_ARGS__);                               \
+        }                                                                                  \
+    } while (0)
 
 }  // namespace r2g

@@ -640,23 +640,23 @@ void launch_lr_batch(const LrArgs *d_args, const BatchSched &sc, int grid, int b
     const bool loop = grid < sc.total;
     if (bitdepth > 8) {
         if (loop)
-            hipLaunchKernelGGL((lr_batch<uint16_t, true>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+            R2G_LAUNCH((lr_batch<uint16_t, true>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
         else
-            hipLaunchKernelGGL((lr_batch<uint16_t, false>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+            R2G_LAUNCH((lr_batch<uint16_t, false>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
     } else {
         if (loop)
-            hipLaunchKernelGGL((lr_batch<uint8_t, true>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+            R2G_LAUNCH((lr_batch<uint8_t, true>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
         else
-            hipLaunchKernelGGL((lr_batch<uint8_t, false>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
+            R2G_LAUNCH((lr_batch<uint8_t, false>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
     }
 }
 
 void launch_lr(const LrArgs &a, int n_items, int bitdepth, hipStream_t s) {
     if (n_items <= 0) return;
     if (bitdepth > 8)
-        hipLaunchKernelGGL(lr_kernel<uint16_t>, dim3(n_items), dim3(kThreads), 0, s, a);
+        R2G_LAUNCH(lr_kernel<uint16_t>, dim3(n_items), dim3(kThreads), 0, s, a);
     else
-        hipLaunchKernelGGL(lr_kernel<uint8_t>, dim3(n_items), dim3(kThreads), 0, s, a);
+        R2G_LAUNCH(lr_kernel<uint8_t>, dim3(n_items), dim3(kThreads), 0, s, a);
 }
 
 // Device copies of the tables, made once per device.

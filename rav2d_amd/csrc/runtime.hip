@@ -15,6 +15,11 @@ namespace r2g {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
+LaunchTiming &launch_timing() {
+    static thread_local LaunchTiming t;
+    return t;
+}
+
 }  // namespace r2g
 
 using namespace r2g;
@@ -398,11 +403,8 @@ int grain_args(Rav2dGpuFrame *f, GrainArgs &a) {
             set_last_error("film grain point count out of range");
             return RAV2D_EINVAL;
         }
-        for (int i = 1; i < d.num_points[pl]; i++)
-            if (d.points[pl][i][0] < d.points[pl][i - 1][0]) {
-                set_last_error("film grain scaling points must not decrease");
-                return RAV2D_EINVAL;
-            }
+        // points whose x repeats or goes back are accepted: like the reference
+        // (filmgrain.rs:40-46), the scaling LUT skips such segments
     }
     memset(&a, 0, sizeof(a));
     a.fg = d;
@@ -411,6 +413,7 @@ int grain_args(Rav2dGpuFrame *f, GrainArgs &a) {
     a.lut = (int16_t *)f->d_grain;
     a.scaling = f->d_grain + 3 * 73 * 82 * 2;
     a.offsets = a.scaling + 3 * 1024;
+    a.err = f->d_grain_err;
     int bs = 16 << d.block_size;
     a.n_bands = (P.height + bs - 1) / bs;
     a.n_blocks = (P.width + bs - 1) / bs;
@@ -445,10 +448,54 @@ int start_grain_setup(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, hipStream_t s) {
     return RAV2D_OK;
 }
 
+// Waits until the frame's earlier per-frame launches (and an in-flight grain
+// set-up) have finished, so that its device tables may be rewritten.
+int frame_wait_idle(Rav2dGpuFrame *f) {
+    if (f->done_pending) {
+        HIP_TRY(hipEventSynchronize(f->ev_done));
+        f->done_pending = false;
+    }
+    if (f->grain_ready) HIP_TRY(hipEventSynchronize(f->ev_setup));
+    return RAV2D_OK;
+}
+
+int mark_done(Rav2dGpuFrame *f, hipStream_t s) {
+    HIP_TRY(hipEventRecord(f->ev_done, s));
+    f->done_pending = true;
+    return RAV2D_OK;
+}
+
+}  // namespace
+
+namespace r2g {
+
+// Error flags the frame's kernels raised (the grain set-up's bounded wait):
+// read after the caller's synchronization, cleared once reported.
+int frame_errors(Rav2dGpuFrame *f) {
+    uint32_t v = 0;
+    HIP_TRY(hipMemcpy(&v, f->d_grain_err, sizeof(v), hipMemcpyDeviceToHost));
+    if (v) {
+        HIP_TRY(hipMemset(f->d_grain_err, 0, sizeof(v)));
+        set_last_error("film grain set-up: the chroma AR filter's wait for luma timed out; display planes invalid");
+        return RAV2D_EDEVICE;
+    }
+    return RAV2D_OK;
+}
+
+}  // namespace r2g
+
+namespace {
+
 int validate(const Rav2dFrameParams *p) {
     if (!p) return RAV2D_EINVAL;
     if (p->width <= 0 || p->height <= 0 || (p->width & 7) || (p->height & 7)) {
         set_last_error("frame size must be a positive multiple of 8");
+        return RAV2D_EINVAL;
+    }
+    // the loop-restoration work items hold plane coordinates as int16_t and the
+    // kernels form row offsets as 32-bit products of rows and byte strides
+    if (p->width > RAV2D_MAX_FRAME_DIM || p->height > RAV2D_MAX_FRAME_DIM) {
+        set_last_error("frame width and height must be at most 16384");
         return RAV2D_EINVAL;
     }
     if (p->bitdepth != 8 && p->bitdepth != 10) {
@@ -827,8 +874,17 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
         return RAV2D_ENOMEM;
     }
     if (hipEventCreateWithFlags(&f->ev_setup, kStreamSyncEventFlags) != hipSuccess ||
-        hipEventCreateWithFlags(&f->ev_free, kStreamSyncEventFlags) != hipSuccess) {
+        hipEventCreateWithFlags(&f->ev_free, kStreamSyncEventFlags) != hipSuccess ||
+        hipEventCreateWithFlags(&f->ev_done, hipEventDisableTiming) != hipSuccess) {
         set_last_error("event creation failed");
+        rav2d_gpu_frame_destroy(f);
+        return RAV2D_EDEVICE;
+    }
+    // the last word of the grain scratch: error flags the set-up kernel raises
+    f->d_grain_err = (uint32_t *)(f->d_grain + grain_bytes - 256);
+    if (hipMemset(f->d_grain_err, 0, sizeof(uint32_t)) != hipSuccess) {
+        set_last_error("hipMemset failed");
+        rav2d_gpu_frame_destroy(f);
         return RAV2D_EDEVICE;
     }
     f->d_lr_cap = lr_cap;
@@ -855,6 +911,7 @@ int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f) {
     (void)hipFree(f->d_dblut);
     if (f->ev_setup) (void)hipEventDestroy(f->ev_setup);
     if (f->ev_free) (void)hipEventDestroy(f->ev_free);
+    if (f->ev_done) (void)hipEventDestroy(f->ev_done);
     if (f->d_grain) (void)hipFree(f->d_grain);
     if (f->d_mask) (void)hipFree(f->d_mask);
     if (f->d_lr) (void)hipFree(f->d_lr);
@@ -878,12 +935,18 @@ int rav2d_gpu_frame_set_params(Rav2dGpuFrame *f, const Rav2dFrameParams *params)
         set_last_error("frame geometry differs");
         return RAV2D_EINVAL;
     }
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    // earlier launches of this frame read the tables rewritten here, and a
+    // grain set-up built for the old parameters must not be applied with the
+    // new ones: wait for both, then drop the set-up
+    r = frame_wait_idle(f);
+    if (r) return r;
+    f->grain_ready = false;
     f->params = *params;
     f->lr_dirty = true;
     f->gen++;
     DbLut lut;
     build_db_lut(*params, f->g.hbd, lut);
-    HIP_TRY(hipSetDevice(f->ctx->device));
     HIP_TRY(hipMemcpy(f->d_dblut, &lut, sizeof(lut), hipMemcpyHostToDevice));
     return RAV2D_OK;
 }
@@ -898,6 +961,8 @@ int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, 
     }
     HIP_TRY(hipSetDevice(f->ctx->device));
     hipStream_t s = pick_stream(f->ctx, stream);
+    int r = frame_wait_idle(f);
+    if (r) return r;
     std::vector<Rav2dAv2Filter> copy(mask, mask + n_mask);
     crop_bottom(copy, f->g);
     HIP_TRY(hipMemcpyAsync(f->d_mask, copy.data(), need * sizeof(Rav2dAv2Filter), hipMemcpyHostToDevice, s));
@@ -947,7 +1012,7 @@ int rav2d_gpu_frame_download(Rav2dGpuFrame *f, int which, void *y, void *u, void
         HIP_TRY(hipMemcpy2DAsync(dst[pl], (size_t)f->pw[pl] * f->bps, f->buf[b][pl], f->stride[pl],
                                  (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    return RAV2D_OK;
+    return which == 2 ? frame_errors(f) : RAV2D_OK;
 }
 
 int rav2d_gpu_filter_frame(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream) {
@@ -958,7 +1023,9 @@ int rav2d_gpu_filter_frame(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream) {
         int r = start_grain_setup(ctx, f, s);
         if (r) return r;
     }
-    return run_chain(ctx, f, RAV2D_PASS_GRAIN, s);
+    int r = run_chain(ctx, f, RAV2D_PASS_GRAIN, s);
+    if (r) return r;
+    return mark_done(f, s);
 }
 
 int rav2d_gpu_apply_grain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream) {
@@ -970,28 +1037,34 @@ int rav2d_gpu_apply_grain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream) {
             int r = copy_plane(f, BUF_OUT, BUF_DISP, pl, s);
             if (r) return r;
         }
-        return RAV2D_OK;
+        return mark_done(f, s);
     }
     GrainArgs a;
     int r = grain_args(f, a);
     if (r) return r;
-    PassTimer pt(ctx, RAV2D_PASS_GRAIN, s);
-    if (f->grain_ready) {
-        HIP_TRY(hipStreamWaitEvent(s, f->ev_setup, 0));
-    } else {
-        launch_grain_setup(a, s);
+    {
+        PassTimer pt(ctx, RAV2D_PASS_GRAIN, s, false);
+        if (f->grain_ready) {
+            HIP_TRY(hipStreamWaitEvent(s, f->ev_setup, 0));
+        } else {
+            launch_grain_setup(a, s);
+            HIP_TRY(hipGetLastError());
+        }
+        f->grain_ready = false;
+        pt.arm();
+        launch_grain_apply(a, f->g.bitdepth, s);
         HIP_TRY(hipGetLastError());
     }
-    f->grain_ready = false;
-    launch_grain_apply(a, f->g.bitdepth, s);
-    HIP_TRY(hipGetLastError());
-    return RAV2D_OK;
+    return mark_done(f, s);
 }
 
 int rav2d_gpu_run_pass(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int pass, void *stream) {
     if (!ctx || !f || pass < 0 || pass > RAV2D_PASS_GRAIN) return RAV2D_EINVAL;
     HIP_TRY(hipSetDevice(ctx->device));
-    return run_chain(ctx, f, pass, pick_stream(ctx, stream));
+    hipStream_t s = pick_stream(ctx, stream);
+    int r = run_chain(ctx, f, pass, s);
+    if (r) return r;
+    return mark_done(f, s);
 }
 
 int rav2d_gpu_set_profiling(Rav2dGpuCtx *ctx, int on) {

@@ -111,7 +111,7 @@ struct Job {
 
 struct Slot {
     hipStream_t s = nullptr;
-    hipStream_t side = nullptr;  // the slot's grain set-ups (slots do not queue behind each other's)
+    hipStream_t side = nullptr;  // grain set-ups: the context's auxiliary stream, shared by the slots
     hipEvent_t ev_done = nullptr, ev_fork = nullptr, ev_join = nullptr;
     bool busy = false;
     std::vector<Job> jobs;
@@ -158,7 +158,7 @@ int grow(Slot &sl, size_t bytes, int frames) {
         if (sl.h_args) (void)hipHostFree(sl.h_args);
         sl.d_args = sl.h_args = nullptr;
         sl.cap = 0;
-        size_t cap = std::max<size_t>(bytes * 2, 64 << 10);
+        size_t cap = std::max<size_t>(bytes, 64 << 10);
         HIP_TRY(hipMalloc((void **)&sl.d_args, cap));
         HIP_TRY(hipHostMalloc((void **)&sl.h_args, cap, 0));
         sl.cap = cap;
@@ -174,7 +174,7 @@ int grow(Slot &sl, size_t bytes, int frames) {
         sl.d_out = sl.h_out = nullptr;
         sl.d_prefix = sl.h_prefix = nullptr;
         sl.sum_cap = 0;
-        int cap = std::max(frames * 2, 16);
+        int cap = std::max(frames, 16);
         HIP_TRY(hipMalloc((void **)&sl.d_sum, cap * sizeof(SumArgs)));
         HIP_TRY(hipHostMalloc((void **)&sl.h_sum, cap * sizeof(SumArgs), 0));
         HIP_TRY(hipMalloc((void **)&sl.d_out, 2 * cap * sizeof(unsigned long long)));
@@ -325,7 +325,7 @@ void rav2d_gpu_server_default_config(Rav2dServeConfig *cfg) {
     if (!cfg) return;
     memset(cfg, 0, sizeof(*cfg));
     cfg->max_batch = 16;
-    cfg->max_inflight = 3;
+    cfg->max_inflight = 2;
     cfg->apply_grain = 1;
     cfg->checksum = 0;
 }
@@ -351,9 +351,12 @@ int rav2d_gpu_server_create(Rav2dGpuCtx *ctx, const Rav2dServeConfig *cfg, Rav2d
     sv->t0 = Clock::now();
     sv->slots.resize(c.max_inflight);
     bool ok = true;
+    // one stream per slot plus the context's two: the process stays within the
+    // device's hardware queues (4 by default) at the default max_inflight, so
+    // no slot's launches queue behind another stream's on a shared queue
     for (Slot &sl : sv->slots) {
         ok = ok && hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking) == hipSuccess;
-        ok = ok && hipStreamCreateWithFlags(&sl.side, hipStreamNonBlocking) == hipSuccess;
+        sl.side = ctx->aux;
         ok = ok && hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming) == hipSuccess;
         ok = ok && hipEventCreateWithFlags(&sl.ev_fork, kStreamSyncEventFlags) == hipSuccess;
         ok = ok && hipEventCreateWithFlags(&sl.ev_join, kStreamSyncEventFlags) == hipSuccess;
@@ -362,6 +365,15 @@ int rav2d_gpu_server_create(Rav2dGpuCtx *ctx, const Rav2dServeConfig *cfg, Rav2d
         set_last_error("server: stream/event creation failed");
         rav2d_gpu_server_destroy(sv);
         return RAV2D_EDEVICE;
+    }
+    // argument and checksum buffers for the largest batch, allocated here so
+    // the dispatcher never allocates while jobs run
+    for (Slot &sl : sv->slots) {
+        r = grow(sl, packed_bound(c.max_batch), c.max_batch);
+        if (r) {
+            rav2d_gpu_server_destroy(sv);
+            return r;
+        }
     }
     sv->th = std::thread(dispatcher, sv);
     *out = sv;
@@ -380,7 +392,6 @@ int rav2d_gpu_server_destroy(Rav2dGpuServer *sv) {
     (void)hipDeviceSynchronize();
     for (Slot &sl : sv->slots) {
         if (sl.s) (void)hipStreamDestroy(sl.s);
-        if (sl.side) (void)hipStreamDestroy(sl.side);
         for (hipEvent_t e : {sl.ev_done, sl.ev_fork, sl.ev_join})
             if (e) (void)hipEventDestroy(e);
         for (void *p : {(void *)sl.d_args, (void *)sl.d_sum, (void *)sl.d_out, (void *)sl.d_prefix})
@@ -524,7 +535,7 @@ int rav2d_gpu_frame_checksum(Rav2dGpuFrame *f, int which, uint64_t *out) {
         return rc;
     }
     *out = h;
-    return RAV2D_OK;
+    return which == 2 ? frame_errors(f) : RAV2D_OK;
 }
 
 }  // extern "C"

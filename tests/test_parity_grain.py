@@ -7,12 +7,22 @@ import oracle_lib as O
 from test_parity_deblock import _compare
 
 CASES = [
-    # (w, h, bitdepth, seed)
-    (352, 288, 8, 31),
-    (352, 288, 10, 32),
-    (1920, 1080, 10, 33),
-    (640, 360, 8, 34),
-    (136, 72, 10, 35),
+    # (w, h, bitdepth, sb128, seed)
+    (352, 288, 8, 0, 31),
+    (352, 288, 10, 0, 32),
+    (1920, 1080, 10, 0, 33),
+    (640, 360, 8, 0, 34),
+    (136, 72, 10, 0, 35),
+    # the benchmark's roofline workload: every tool, 128x128 superblocks
+    (3840, 2160, 10, 1, 36),
+    (2560, 1440, 10, 0, 37),
+]
+
+# sizes of the mixed batch the grain-free (8-bit) frames take
+CASES_NO_GRAIN = [
+    # (w, h, bitdepth, sb128, seed)
+    (2560, 1440, 8, 1, 38),
+    (3840, 2160, 8, 0, 39),
 ]
 
 
@@ -37,11 +47,21 @@ def _run(ctx, sf, tag):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", CASES, ids=lambda c: "%dx%d_%db_s%d" % c)
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%dx%d_%db_sb%d_s%d" % (c[0], c[1], c[2], 64 << c[3], c[4]))
 def test_grain_chain(ctx, case):
-    w, h, bd, seed = case
-    sf = R.SynthFrame(w, h, bd, 0, seed, tools=R.TOOL_ALL)
+    w, h, bd, sb128, seed = case
+    sf = R.SynthFrame(w, h, bd, sb128, seed, tools=R.TOOL_ALL)
     _run(ctx, sf, "grain")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES_NO_GRAIN,
+                         ids=lambda c: "%dx%d_%db_sb%d_s%d" % (c[0], c[1], c[2], 64 << c[3], c[4]))
+def test_chain_without_grain(ctx, case):
+    # the display copy of a frame without grain equals its in-loop output
+    w, h, bd, sb128, seed = case
+    sf = R.SynthFrame(w, h, bd, sb128, seed, tools=R.TOOL_ALL & ~R.TOOL_GRAIN)
+    _run(ctx, sf, "no grain")
 
 
 @pytest.mark.gpu
