@@ -525,3 +525,12 @@ extern "C" void oracle_ccso_add_8(uint8_t *dst, ptrdiff_t ds, const uint8_t *idx
     oracle::ccso_add(dst, ds, idx, idx_stride, offs, lut, w, h, ll, 255);
 }
 
+
+/* Block-level hook for the pinning tests (16-bit storage): cdef_filter_block
+ * on dst (stride ds samples); left[y] = the 2 samples left of row y; top and
+ * bottom = 2 rows each (stride ds) whose column 0 is the block's column -2. */
+extern "C" void oracle_cdef_filter_block_16(uint16_t *dst, ptrdiff_t ds, const uint16_t (*left)[2],
+                                            const uint16_t *top, const uint16_t *bottom, int pri, int sec, int dir,
+                                            int damping, int w, int h, int edges, int bdm8) {
+    oracle::cdef_filter_block(dst, ds, 0, left, top, 2, bottom, 2, pri, sec, dir, damping, w, h, edges, bdm8);
+}

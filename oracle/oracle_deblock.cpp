@@ -511,3 +511,13 @@ extern "C" int oracle_filter_choice(const void *buf, int hbd, ptrdiff_t s, ptrdi
 }
 extern "C" int oracle_edge_thr(int cur, int prev) { return oracle::edge_thr(cur, prev); }
 
+
+/* Segment-level hook for the pinning tests (16-bit storage): deblock_bd on the
+ * 4-line segment whose first positive-side sample is dst[off]; stridea steps
+ * along the edge, strideb across it. */
+extern "C" void oracle_deblock_edge_16(uint16_t *dst, ptrdiff_t off, int bdmax, uint32_t q_thr, uint32_t side_thr,
+                                       ptrdiff_t stridea, ptrdiff_t strideb, int max_width_pos, int max_width_neg,
+                                       int pos_lossless, int neg_lossless) {
+    oracle::deblock_edge(dst, off, bdmax, q_thr, side_thr, stridea, strideb, max_width_pos, max_width_neg,
+                         pos_lossless != 0, neg_lossless != 0);
+}

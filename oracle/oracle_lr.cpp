@@ -845,3 +845,55 @@ extern "C" void oracle_gdf_add_8(uint8_t *p, ptrdiff_t stride, const int8_t *err
     oracle::gdf_add(p, stride, err, err_stride, w, h, scale, ll, bitdepth);
 }
 
+
+/* Unit-level hooks for the pinning tests: the restated filters on one stripe
+ * of one restoration unit, 16-bit storage (8-bit samples are stored widened;
+ * every computation depends on `bitdepth`, not on the storage type).  p points
+ * at the stripe's first sample; left[y] holds the 6 samples left of row y;
+ * lpf / lpf_bottom the two rows above / below the stripe (same stride). */
+extern "C" void oracle_lr_ns_wiener_y(uint16_t *p, ptrdiff_t stride, const uint16_t (*left)[6], const uint16_t *lpf,
+                                      const uint16_t *lpf_bottom, int w, int h, const int8_t *filter, int edges,
+                                      const uint16_t (*ll)[4], int bitdepth) {
+    oracle::ns_wiener_single_y(p, stride, left, lpf, lpf_bottom, w, h, filter, (uint8_t)edges, ll, bitdepth);
+}
+/* filters_user: NS multi-class filters [16][18] (NULL = PC-Wiener with filters_pre [64][13]) */
+extern "C" void oracle_lr_wiener_multi(uint16_t *p, ptrdiff_t stride, const uint16_t (*left)[6], const uint16_t *lpf,
+                                       const uint16_t *lpf_bottom, int w, int h, const int8_t (*filters_user)[18],
+                                       const int16_t (*filters_pre)[13], const uint8_t *subclass,
+                                       const uint16_t *noskip, int base_q, int edges, const uint16_t (*ll)[4],
+                                       int bitdepth) {
+    oracle::wiener_multi(p, stride, left, lpf, lpf_bottom, w, h, filters_user, filters_pre, subclass, noskip, base_q,
+                         (uint8_t)edges, ll, bitdepth);
+}
+extern "C" void oracle_lr_ns_wiener_uv(uint16_t *p, ptrdiff_t stride, const uint16_t (*left)[6], const uint16_t *lpf,
+                                       const uint16_t *lpf_bottom, int w, int h, const int8_t *filter,
+                                       const uint16_t *luma, ptrdiff_t lstride, const uint16_t *luma_top,
+                                       const uint16_t *luma_bottom, int ss_hor, int ss_ver, int ds, int edges,
+                                       const uint16_t (*ll)[4], int bitdepth) {
+    oracle::ns_wiener_single_uv(p, stride, left, lpf, lpf_bottom, w, h, filter, luma, lstride, luma_top, luma_bottom,
+                                ss_hor, ss_ver, ds, (uint8_t)edges, ll, bitdepth);
+}
+/* dst: one int8 error per sample, dst_stride per row */
+extern "C" void oracle_lr_gdf_prep(int8_t *dst, int dst_stride, const uint16_t *p, ptrdiff_t stride,
+                                   const uint16_t (*left)[6], const uint16_t *lpf, const uint16_t *lpf_bottom, int w,
+                                   int h, int ref_dst_idx, int qp_idx, int edges, int bitdepth) {
+    oracle::gdf_prep(dst, dst_stride, p, stride, left, lpf, lpf_bottom, w, h, ref_dst_idx, qp_idx, (uint8_t)edges,
+                     bitdepth);
+}
+/* The tap-offset tables the restatement uses ([n][2] = (dy, dx)): 0
+ * WIENER_NS_CONFIG_Y, 1 WIENER_NS_CONFIG_UV, 2 WIENER_NS_CONFIG_UV_FROM_Y,
+ * 3 PC_WIENER_CONFIG, 4 GDF_COORDS.  Returns the entry count. */
+extern "C" int oracle_lr_taps(int which, int8_t *out) {
+    const int8_t(*t)[2] = nullptr;
+    int n = 0;
+    switch (which) {
+        case 0: t = oracle::WIENER_NS_CONFIG_Y; n = 16; break;
+        case 1: t = oracle::WIENER_NS_CONFIG_UV; n = 6; break;
+        case 2: t = oracle::WIENER_NS_CONFIG_UV_FROM_Y; n = 12; break;
+        case 3: t = oracle::PC_WIENER_CONFIG; n = 12; break;
+        case 4: t = oracle::GDF_COORDS; n = 18; break;
+        default: return 0;
+    }
+    memcpy(out, t, (size_t)n * 2);
+    return n;
+}
