@@ -204,6 +204,16 @@ int rav2d_gpu_ctx_synchronize(Rav2dGpuCtx *ctx);
 /* Bytes currently held by frames of this context. */
 uint64_t rav2d_gpu_ctx_bytes_in_use(const Rav2dGpuCtx *ctx);
 
+/* ---- device memory ---------------------------------------------------------
+ * Device buffers counted against the context's HBM budget (RAV2D_EAGAIN when
+ * it cannot hold them): for the per-family entries' buffers and for pictures
+ * a decoder keeps in HBM.  rav2d_gpu_copy copies between host and device
+ * memory in either direction (hipMemcpyDefault), enqueued on `stream`; with
+ * stream NULL it runs on the context stream and returns once done. */
+int rav2d_gpu_alloc(Rav2dGpuCtx *ctx, size_t bytes, void **out);
+int rav2d_gpu_free(Rav2dGpuCtx *ctx, void *p);
+int rav2d_gpu_copy(Rav2dGpuCtx *ctx, void *dst, const void *src, size_t bytes, void *stream);
+
 /* ---- device-resident frames ---------------------------------------------
  * A frame owns: the reconstructed input planes (F0), the filtered output
  * planes (in-loop output, F4), optional display planes for the grained copy,
@@ -243,6 +253,32 @@ int rav2d_gpu_frame_download(Rav2dGpuFrame *f, int which, void *y, void *u, void
  * a no-op copy when the frame carries no grain (picture_has_grain). */
 int rav2d_gpu_filter_frame(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream);
 int rav2d_gpu_apply_grain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream);
+
+/* ---- zero-copy pictures ---------------------------------------------------
+ * A PicAllocator backed by device memory (picture.rs:38-41, 77-125): pictures
+ * in the library's plane layout (rav2d_gpu_frame_planes' strides for the same
+ * size and depth: rows padded to 256 B, +256 B when the stride is a multiple
+ * of 1 KiB, 8 spare rows), one allocation against the HBM budget.  With them
+ * the decoder's reconstruction, the in-loop filters and film grain share
+ * device pictures and the copies at decode.rs:4549 and :4681 go away. */
+int rav2d_gpu_picture_alloc(Rav2dGpuCtx *ctx, int width, int height, int bitdepth, int layout, Rav2dPlanes *out);
+int rav2d_gpu_picture_free(Rav2dGpuCtx *ctx, Rav2dPlanes *p);
+/* The in-loop chain of rav2d_gpu_filter_frame on caller-owned device planes:
+ * reads `in`, writes `out` (out == in filters in place), with the frame's
+ * parameters, side info and intermediate planes.  Both plane sets need the
+ * frame's strides and 8-byte-aligned plane pointers (RAV2D_EINVAL otherwise);
+ * rav2d_gpu_picture_alloc pictures of the frame's size and depth qualify. */
+int rav2d_gpu_filter_planes(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, const Rav2dPlanes *in, const Rav2dPlanes *out,
+                            void *stream);
+/* apply_grain_to_picture_mt (decode.rs:4672) on caller-owned device pictures:
+ * dst = src with film grain `data` / `seed` (a copy when the data carries no
+ * grain, picture_has_grain decode.rs:4641-4648).  Strides are multiples of 8
+ * covering a row; 4:2:0.  The grain templates, scaling LUTs and block offsets
+ * live in a scratch area this call allocates and frees, stream-ordered
+ * (hipMallocAsync / hipFreeAsync on `stream`). */
+int rav2d_gpu_apply_grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *dst, int width,
+                                 int height, int bitdepth, int layout, const Rav2dFilmGrainData *data, uint32_t seed,
+                                 void *stream);
 
 /* ---- per-pass entry points (unit parity and per-kernel profiling) -------- */
 #define RAV2D_PASS_DEBLOCK_V 0
@@ -372,10 +408,150 @@ int rav2d_gpu_frame_checksum(Rav2dGpuFrame *f, int which, uint64_t *out);
  * measurement utility. */
 int rav2d_gpu_bandwidth_probe(Rav2dGpuCtx *ctx, uint64_t bytes, int iters, double *gbps);
 
+/* ---- per-family DSP entries -------------------------------------------------
+ * The function-table entries of rav2d's DSP contexts for the filter stage
+ * (DeblockDSPContext, CdefDSPContext, CcsoDSPContext,
+ * LoopRestorationDSPContext, FilmGrainDSPContext; crates/rav2d/src/dsp.rs:
+ * 9-14, 86-109), one GPU launch over one call's unit each, for table-level
+ * drop-in and unit parity; the frame-level entries above are the fast path.
+ * Arguments follow the reference's kernels one for one.  Pixel buffers
+ * (dst/src, the left/top/bottom line buffers, luma rows, grain templates,
+ * scaling LUTs, index and error maps) are DEVICE memory: samples are uint8_t
+ * at 8-bit and uint16_t at 10-bit and strides are in bytes.  Small side arrays
+ * (edge masks, thresholds, lossless masks, filters, class LUTs, film-grain
+ * data) are HOST memory, copied into the launch.  A call is enqueued on
+ * `stream` (NULL = the context stream) and returns at once.  In-place calls
+ * read their whole input before writing (as the reference's line backups
+ * do).  RAV2D_EINVAL for a bit depth other than 8/10 or a unit larger than
+ * the reference's (stated per entry). */
+
+/* CDEF_HAVE_* / LR_HAVE_* edge bits (cdef.rs, looprestoration.rs) */
+#define RAV2D_EDGE_LEFT 1
+#define RAV2D_EDGE_RIGHT 2
+#define RAV2D_EDGE_TOP 4
+#define RAV2D_EDGE_BOTTOM 8
+
+/* DeblockDSPContext.deblock_sb[dir][uv]: deblock_{h,v}_sb64{y,uv}_bd
+ * (deblock.rs:303-551).  The 4-line segments of one edge line: segment i
+ * (bit i of the level masks) starts 4 i samples along the edge from dst, which
+ * is the first sample on the edge's positive side.  dir 0 = vertical edges
+ * (the "h" walkers: segments go down the rows), 1 = horizontal edges (the "v"
+ * walkers).  vmask: the level masks (4 luma, 3 chroma); ll_mask[0] / [1]:
+ * lossless negative / positive side; q_thr, side_thr: one per segment (16);
+ * edge: clamp the negative side's width (6 luma, 2 chroma). */
+int rav2d_gpu_deblock_sb(Rav2dGpuCtx *ctx, int dir, int uv, void *dst, ptrdiff_t stride, const uint16_t *vmask,
+                         const uint16_t ll_mask[2], const uint16_t q_thr[16], const uint16_t side_thr[16], int edge,
+                         int bitdepth, void *stream);
+
+/* CdefDSPContext.dir: cdef_find_dir_bd (cdef.rs:134-216) of the 8x8 block at
+ * img; the direction (0..7) and variance go to dir_out / var_out (device). */
+int rav2d_gpu_cdef_dir(Rav2dGpuCtx *ctx, const void *img, ptrdiff_t stride, int bitdepth, int32_t *dir_out,
+                       uint32_t *var_out, void *stream);
+
+/* CdefDSPContext.fb[]: cdef_filter_block (cdef.rs:320-460), the w x h block
+ * at dst (w, h in {4, 8}) filtered in place.  left: h rows of 2 samples (the
+ * columns left of the block); top / bottom: 2 rows each at dst's stride,
+ * pointing at the block's column 0 (columns -2 .. w + 1 readable); edges:
+ * RAV2D_EDGE_* of the neighbours that exist. */
+int rav2d_gpu_cdef_filter(Rav2dGpuCtx *ctx, void *dst, ptrdiff_t stride, const void *left, const void *top,
+                          const void *bottom, int pri_strength, int sec_strength, int dir, int damping, int w, int h,
+                          int edges, int bitdepth, void *stream);
+
+/* CcsoDSPContext.prep[plane]: ccso_prep (ccso.rs:26-186), the band / edge
+ * class byte of each of the w x h samples of a plane block (its luma, w << ss_hor
+ * by h << ss_ver, at most 64 x 64: one superblock column, as the reference's)
+ * from its collocated luma: src (src_stride), left (rows of 2 luma samples),
+ * top / bottom (2 luma rows each at src_stride, at the block's column 0);
+ * ext_filter indexes CCSO_POS (0..6); dst: bytes at dst_stride. */
+int rav2d_gpu_ccso_prep(Rav2dGpuCtx *ctx, uint8_t *dst, ptrdiff_t dst_stride, const void *src, ptrdiff_t src_stride,
+                        const void *left, const void *top, const void *bottom, int max_band_log2, int ext_filter,
+                        int quant_step, uint32_t edge_clf, int bo_only, int edges, int w, int h, int ss_hor,
+                        int ss_ver, int bitdepth, void *stream);
+
+/* CcsoDSPContext.add: ccso_add (ccso.rs:215-254), dst += offset_lut[the
+ * nibble of offset_idxs the class byte selects], clipped, except lossless 4x4
+ * blocks (ll_mask: h / 4 rows, word 0 = bit per 4x4 column; host).
+ * offset_idxs: 64 bytes, offset_lut: 8 (host); w <= 256, h <= 64. */
+int rav2d_gpu_ccso_add(Rav2dGpuCtx *ctx, void *dst, ptrdiff_t dst_stride, const uint8_t *idx, ptrdiff_t idx_stride,
+                       const uint8_t offset_idxs[64], const int8_t offset_lut[8], int w, int h,
+                       const uint16_t (*ll_mask)[4], int bitdepth, void *stream);
+
+/* WienerParamsSingle / WienerParamsMulti / WienerParamsC
+ * (looprestoration.rs:1716-1740).  Single: filter = 16 (luma) or 18 (chroma)
+ * host taps; luma / luma_top / luma_bottom (device) and stride, ss_ver,
+ * ss_hor, ds_flt are the chroma kernel's collocated luma.  Multi: filters =
+ * host [16][18] int8 class filters (NS multi) or [64][13] int16 pretrained
+ * filters (PC-Wiener); subclass_lut: host 256 bytes; noskip_mask: host, one
+ * 16-bit row per 4 sample rows of the stripe plus one. */
+typedef struct Rav2dWienerParamsSingle {
+    const int8_t *filter;
+    const void *luma;
+    const void *luma_top;
+    const void *luma_bottom;
+    ptrdiff_t stride;
+    int32_t ss_ver;
+    int32_t ss_hor;
+    int32_t ds_flt;
+} Rav2dWienerParamsSingle;
+typedef struct Rav2dWienerParamsMulti {
+    const uint8_t *filters;
+    const uint8_t *subclass_lut;
+    const uint16_t *noskip_mask;
+    int32_t base_q;
+} Rav2dWienerParamsMulti;
+typedef union Rav2dWienerParams {
+    Rav2dWienerParamsSingle single;
+    Rav2dWienerParamsMulti multi;
+} Rav2dWienerParams;
+
+/* LoopRestorationDSPContext entries through WienerFilterFn
+ * (looprestoration.rs:1742-1753): kind 0 ns_wiener_single (luma, :524-713),
+ * 1 ns_wiener_single (chroma with its cross-component luma taps, 4:2:0,
+ * :1145-1430), 2 ns_wiener_multi (:715-1100), 3 pc_wiener (:1102-1141).  The
+ * stripe of one restoration unit at dst (w <= 64, h <= 64 samples, multiples
+ * of 4); left: h rows of 6 samples; top / bottom: the 2 rows above / below at
+ * dst's stride; ll_mask: h / 4 + 1 host rows of 4 words (word 0 = lossless bit
+ * per 4x4 column). */
+int rav2d_gpu_lr_wiener(Rav2dGpuCtx *ctx, int kind, void *dst, ptrdiff_t dst_stride, const void *left,
+                        const void *top, const void *bottom, int w, int h, const Rav2dWienerParams *params,
+                        uint8_t edges, const uint16_t (*ll_mask)[4], int bitdepth, void *stream);
+/* GdfPrepFn (looprestoration.rs:1755-1768): gdf_prep (:1432-1643), the int8
+ * error of each sample of the w x h stripe at p (w <= 64, h <= 64, even); left:
+ * h rows of 6 samples, top / bottom: the 2 rows above / below at p's stride. */
+int rav2d_gpu_gdf_prep(Rav2dGpuCtx *ctx, int8_t *dst, ptrdiff_t dst_stride, const void *p, ptrdiff_t stride,
+                       const void *left, const void *top, const void *bottom, int w, int h, int ref_dst_idx,
+                       int qp_idx, uint8_t edges, int bitdepth, void *stream);
+/* GdfAddFn (looprestoration.rs:1770-1779): gdf_add (:273-329), p += the
+ * scaled error, clipped, except lossless 4x4 blocks (w, h <= 64). */
+int rav2d_gpu_gdf_add(Rav2dGpuCtx *ctx, void *p, ptrdiff_t stride, const int8_t *err, ptrdiff_t err_stride, int w,
+                      int h, int scale, const uint16_t (*ll_mask)[4], int bitdepth, void *stream);
+
+/* FilmGrainDSPContext.generate_grain_y / generate_grain_uv[I420]
+ * (filmgrain.rs:115-229): the 73 x 82 luma template, or a 38 x 44 chroma
+ * template (rows of 82 entries, subx = suby = 1 only) built from the luma
+ * template buf_y; buffers int16_t, device. */
+int rav2d_gpu_generate_grain_y(Rav2dGpuCtx *ctx, int16_t *buf, const Rav2dFilmGrainData *data, uint32_t seed,
+                               int bitdepth, void *stream);
+int rav2d_gpu_generate_grain_uv(Rav2dGpuCtx *ctx, int16_t *buf, const int16_t *buf_y,
+                                const Rav2dFilmGrainData *data, uint32_t seed, int uv, int subx, int suby,
+                                int bitdepth, void *stream);
+/* FilmGrainDSPContext.fgy_32x32xn / fguv_32x32xn[I420] (filmgrain.rs:248-592):
+ * one row band (bh rows, band row_num) of a plane, pw samples wide; dst, src
+ * at stride; scaling: the plane's 1 << bitdepth LUT (device); grain_lut: the
+ * plane's template (device, rows of 82); fguv also reads the band's luma rows
+ * (luma_row at luma_stride), is_id = mc_identity, sx / sy = 1 (4:2:0 only). */
+int rav2d_gpu_fgy_32x32xn(Rav2dGpuCtx *ctx, void *dst, const void *src, ptrdiff_t stride,
+                          const Rav2dFilmGrainData *data, uint32_t seed, int pw, const uint8_t *scaling,
+                          const int16_t *grain_lut, int bh, int row_num, int bitdepth, void *stream);
+int rav2d_gpu_fguv_32x32xn(Rav2dGpuCtx *ctx, void *dst, const void *src, ptrdiff_t stride,
+                           const Rav2dFilmGrainData *data, uint32_t seed, int pw, const uint8_t *scaling,
+                           const int16_t *grain_lut, int bh, int row_num, const void *luma_row,
+                           ptrdiff_t luma_stride, int uv, int is_id, int sx, int sy, int bitdepth, void *stream);
+
 /* sizeof() of the ABI structs, so bindings can check their mirrors:
  * 0 Rav2dAv2Filter, 1 Rav2dAv2Restoration, 2 Rav2dFrameParams,
  * 3 Rav2dFilmGrainData, 4 Rav2dPlanes, 5 Rav2dBatchConfig, 6 Rav2dServeConfig,
- * 7 Rav2dServeResult; 0 for an unknown index. */
+ * 7 Rav2dServeResult, 8 Rav2dWienerParams; 0 for an unknown index. */
 size_t rav2d_gpu_struct_size(int which);
 
 const char *rav2d_gpu_version(void);

@@ -534,3 +534,17 @@ extern "C" void oracle_cdef_filter_block_16(uint16_t *dst, ptrdiff_t ds, const u
                                             int damping, int w, int h, int edges, int bdm8) {
     oracle::cdef_filter_block(dst, ds, 0, left, top, 2, bottom, 2, pri, sec, dir, damping, w, h, edges, bdm8);
 }
+/* ccso_prep / ccso_add (16-bit storage, strides in samples); top / bottom at
+ * the block's column 0 with src's stride. */
+extern "C" void oracle_ccso_prep_16(uint8_t *dst, int dst_stride, const uint16_t *src, ptrdiff_t ss,
+                                    const uint16_t (*left)[2], const uint16_t *top, const uint16_t *bottom,
+                                    uint32_t max_band_log2, int ext, int quant_step, uint32_t edge_clf, int bo_only,
+                                    int edges, int w, int h, int ss_hor, int ss_ver, int bitdepth) {
+    oracle::ccso_prep(dst, dst_stride, src, ss, 0, left, top, 0, bottom, 0, max_band_log2, ext, quant_step, edge_clf,
+                      bo_only != 0, edges, w, h, ss_hor, ss_ver, bitdepth);
+}
+extern "C" void oracle_ccso_add_16(uint16_t *dst, ptrdiff_t ds, const uint8_t *idx, int idx_stride,
+                                   const uint8_t *offs, const int8_t *lut, int w, int h, const uint16_t (*ll)[4],
+                                   int bdmax) {
+    oracle::ccso_add(dst, ds, idx, idx_stride, offs, lut, w, h, ll, bdmax);
+}

@@ -521,3 +521,11 @@ extern "C" void oracle_deblock_edge_16(uint16_t *dst, ptrdiff_t off, int bdmax, 
     oracle::deblock_edge(dst, off, bdmax, q_thr, side_thr, stridea, strideb, max_width_pos, max_width_neg,
                          pos_lossless != 0, neg_lossless != 0);
 }
+/* deblock_{h,v}_sb64{y,uv}_bd (16-bit storage, stride in samples): dir_h = the
+ * vertical-edge ("h") walker. */
+extern "C" void oracle_deblock_sb64_16(uint16_t *dst, ptrdiff_t stride, int bdmax, const uint16_t *vmask,
+                                       const uint16_t *ll_mask, const uint16_t *q_thr, const uint16_t *side_thr,
+                                       int edge, int dir_h, int chroma) {
+    oracle::deblock_sb64(dst, 0, stride, bdmax, vmask, chroma ? 3 : 4, ll_mask, q_thr, side_thr, edge != 0,
+                         dir_h != 0, chroma != 0);
+}

@@ -412,3 +412,18 @@ extern "C" void oracle_generate_grain_uv(int16_t *buf, const int16_t *buf_y, con
                               suby, bitdepth);
 }
 
+
+/* Band-level hooks (16-bit storage; strides in samples) for the per-family
+ * entry tests: fgy_32x32xn / fguv_32x32xn on one row band. */
+extern "C" void oracle_fgy_16(uint16_t *dst, const uint16_t *src, ptrdiff_t stride, const Rav2dFilmGrainData *d,
+                              uint32_t seed, int pw, const uint8_t *scaling, const int16_t *lut, int bh, int row_num,
+                              int bitdepth) {
+    oracle::fgy(dst, src, stride, *d, seed, pw, scaling, (const int16_t(*)[oracle::GW])lut, bh, row_num, bitdepth);
+}
+extern "C" void oracle_fguv_16(uint16_t *dst, const uint16_t *src, ptrdiff_t stride, const Rav2dFilmGrainData *d,
+                               uint32_t seed, int pw, const uint8_t *scaling, const int16_t *lut, int bh, int row_num,
+                               const uint16_t *luma, ptrdiff_t lstride, int uv, int is_id, int sx, int sy,
+                               int bitdepth) {
+    oracle::fguv(dst, src, stride, *d, seed, pw, scaling, (const int16_t(*)[oracle::GW])lut, bh, row_num, luma,
+                 lstride, uv, is_id != 0, sx, sy, bitdepth);
+}

@@ -897,3 +897,7 @@ extern "C" int oracle_lr_taps(int which, int8_t *out) {
     memcpy(out, t, (size_t)n * 2);
     return n;
 }
+extern "C" void oracle_lr_gdf_add(uint16_t *p, ptrdiff_t stride, const int8_t *err, int err_stride, int w, int h,
+                                  int scale, const uint16_t (*ll)[4], int bitdepth) {
+    oracle::gdf_add(p, stride, err, err_stride, w, h, scale, ll, bitdepth);
+}

@@ -202,6 +202,30 @@ def load_gpu():
               C.POINTER(C.c_int32), C.POINTER(C.c_int32)])
     _declare(lib, "rav2d_gpu_frame_checksum", C.c_int, [P, C.c_int, C.POINTER(C.c_uint64)])
     _declare(lib, "rav2d_gpu_bandwidth_probe", C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double)])
+    _declare(lib, "rav2d_gpu_picture_alloc", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Planes)])
+    _declare(lib, "rav2d_gpu_picture_free", C.c_int, [P, C.POINTER(Planes)])
+    _declare(lib, "rav2d_gpu_filter_planes", C.c_int, [P, P, C.POINTER(Planes), C.POINTER(Planes), P])
+    _declare(lib, "rav2d_gpu_apply_grain_planes", C.c_int,
+             [P, C.POINTER(Planes), C.POINTER(Planes), C.c_int, C.c_int, C.c_int, C.c_int,
+              C.POINTER(FilmGrainData), C.c_uint32, P])
+    _declare(lib, "rav2d_gpu_alloc", C.c_int, [P, C.c_size_t, C.POINTER(P)])
+    _declare(lib, "rav2d_gpu_free", C.c_int, [P, P])
+    _declare(lib, "rav2d_gpu_copy", C.c_int, [P, P, P, C.c_size_t, P])
+    # per-family DSP entries
+    I, U32, SS = C.c_int, C.c_uint32, C.c_ssize_t
+    _declare(lib, "rav2d_gpu_deblock_sb", I, [P, I, I, P, SS, P, P, P, P, I, I, P])
+    _declare(lib, "rav2d_gpu_cdef_dir", I, [P, P, SS, I, P, P, P])
+    _declare(lib, "rav2d_gpu_cdef_filter", I, [P, P, SS, P, P, P, I, I, I, I, I, I, I, I, P])
+    _declare(lib, "rav2d_gpu_ccso_prep", I, [P, P, SS, P, SS, P, P, P, I, I, I, U32, I, I, I, I, I, I, I, P])
+    _declare(lib, "rav2d_gpu_ccso_add", I, [P, P, SS, P, SS, P, P, I, I, P, I, P])
+    _declare(lib, "rav2d_gpu_lr_wiener", I, [P, I, P, SS, P, P, P, I, I, P, C.c_uint8, P, I, P])
+    _declare(lib, "rav2d_gpu_gdf_prep", I, [P, P, SS, P, SS, P, P, P, I, I, I, I, C.c_uint8, I, P])
+    _declare(lib, "rav2d_gpu_gdf_add", I, [P, P, SS, P, SS, I, I, I, P, I, P])
+    _declare(lib, "rav2d_gpu_generate_grain_y", I, [P, P, C.POINTER(FilmGrainData), U32, I, P])
+    _declare(lib, "rav2d_gpu_generate_grain_uv", I, [P, P, P, C.POINTER(FilmGrainData), U32, I, I, I, I, P])
+    _declare(lib, "rav2d_gpu_fgy_32x32xn", I, [P, P, P, SS, C.POINTER(FilmGrainData), U32, I, P, P, I, I, I, P])
+    _declare(lib, "rav2d_gpu_fguv_32x32xn", I, [P, P, P, SS, C.POINTER(FilmGrainData), U32, I, P, P, I, I, P, SS,
+                                                I, I, I, I, I, P])
     _gpu = lib
     return lib
 
@@ -328,6 +352,10 @@ class GpuContext:
         _check(self.lib.rav2d_gpu_timer_stop(self.h, C.byref(ms)), "timer_stop")
         return ms.value
 
+    def device_array(self, host):
+        """A DeviceArray holding a copy of the numpy array `host`."""
+        return DeviceArray(self, host)
+
     def bandwidth_probe(self, nbytes=256 << 20, iters=20):
         """GB/s of a plain streaming copy (read + write bytes)."""
         g = C.c_double()
@@ -338,6 +366,99 @@ class GpuContext:
         if self.h:
             self.lib.rav2d_gpu_ctx_destroy(self.h)
             self.h = None
+
+
+class DeviceArray:
+    """Device memory (rav2d_gpu_alloc) shaped like a numpy array: `ptr` is the
+    device address of element 0; `read()` copies it back."""
+
+    def __init__(self, ctx, host):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.shape, self.dtype = host.shape, host.dtype
+        self.nbytes = max(int(host.nbytes), 1)
+        p = C.c_void_p()
+        _check(self.lib.rav2d_gpu_alloc(ctx.h, self.nbytes, C.byref(p)), "alloc")
+        self.ptr = p.value
+        self.write(host)
+
+    def write(self, host):
+        h = np.ascontiguousarray(host, dtype=self.dtype)
+        _check(self.lib.rav2d_gpu_copy(self.ctx.h, self.ptr, h.ctypes.data, h.nbytes, None), "copy")
+
+    def read(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(self.lib.rav2d_gpu_copy(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes, None), "copy")
+        return out
+
+    def at(self, *index):
+        """Device address of element `index`."""
+        off = int(np.ravel_multi_index(index, self.shape)) * self.dtype.itemsize if index else 0
+        return self.ptr + off
+
+    def close(self):
+        if self.ptr:
+            self.lib.rav2d_gpu_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Picture:
+    """A device picture from rav2d_gpu_picture_alloc (the library's plane
+    layout: the strides rav2d_gpu_frame_planes reports for that size)."""
+
+    def __init__(self, ctx, width, height, bitdepth, layout=1):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.width, self.height, self.bitdepth = width, height, bitdepth
+        self.planes = Planes()
+        _check(self.lib.rav2d_gpu_picture_alloc(ctx.h, width, height, bitdepth, layout, C.byref(self.planes)),
+               "picture_alloc")
+
+    @property
+    def dtype(self):
+        return np.uint16 if self.bitdepth > 8 else np.uint8
+
+    def _shape(self, pl):
+        return (self.height, self.width) if pl == 0 else ((self.height + 1) // 2, (self.width + 1) // 2)
+
+    def write(self, planes):
+        """Upload three numpy planes (rows of any stride are packed here)."""
+        for pl, a in enumerate(planes):
+            h, w = self._shape(pl)
+            s = self.planes.stride[1 if pl else 0]
+            buf = np.zeros((h, s // np.dtype(self.dtype).itemsize), dtype=self.dtype)
+            buf[:, :w] = a
+            _check(self.lib.rav2d_gpu_copy(self.ctx.h, self.planes.data[pl], buf.ctypes.data, buf.nbytes, None),
+                   "copy")
+
+    def read(self):
+        out = []
+        for pl in range(3):
+            h, w = self._shape(pl)
+            s = self.planes.stride[1 if pl else 0]
+            buf = np.empty((h, s // np.dtype(self.dtype).itemsize), dtype=self.dtype)
+            _check(self.lib.rav2d_gpu_copy(self.ctx.h, buf.ctypes.data, self.planes.data[pl], buf.nbytes, None),
+                   "copy")
+            out.append(buf[:, :w].copy())
+        return out
+
+    def close(self):
+        if self.planes.data[0]:
+            self.lib.rav2d_gpu_picture_free(self.ctx.h, C.byref(self.planes))
+
+
+class WienerParamsSingle(C.Structure):
+    _fields_ = [("filter", C.c_void_p), ("luma", C.c_void_p), ("luma_top", C.c_void_p),
+                ("luma_bottom", C.c_void_p), ("stride", C.c_ssize_t), ("ss_ver", C.c_int32),
+                ("ss_hor", C.c_int32), ("ds_flt", C.c_int32)]
+
+
+class WienerParamsMulti(C.Structure):
+    _fields_ = [("filters", C.c_void_p), ("subclass_lut", C.c_void_p), ("noskip_mask", C.c_void_p),
+                ("base_q", C.c_int32)]
+
+
+class WienerParams(C.Union):
+    _fields_ = [("single", WienerParamsSingle), ("multi", WienerParamsMulti)]
 
 
 class GpuFrame:

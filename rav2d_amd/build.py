@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 
-HIP_SOURCES = ["runtime.hip", "deblock.hip", "cdef.hip", "lr.hip", "grain.hip", "batch.hip", "serve.hip", "probe.hip"]
+HIP_SOURCES = ["runtime.hip", "deblock.hip", "cdef.hip", "lr.hip", "grain.hip", "batch.hip", "serve.hip", "probe.hip", "dsp.hip"]
 ARCH = "gfx950"
 
 

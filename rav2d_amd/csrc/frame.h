@@ -2,6 +2,7 @@
 #pragma once
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "internal.h"
@@ -38,6 +39,8 @@ struct Rav2dGpuCtx {
     std::vector<hipEvent_t> free_events;
     double pass_ms[RAV2D_NUM_PASSES] = {0};
     int64_t pass_count[RAV2D_NUM_PASSES] = {0};
+    // device buffers handed out by rav2d_gpu_alloc / rav2d_gpu_picture_alloc
+    std::unordered_map<void *, size_t> allocs;
 };
 
 namespace r2g {
@@ -108,6 +111,9 @@ struct Rav2dGpuFrame {
     size_t plane_bytes[3];
     uint8_t *block = nullptr;     // one allocation: all plane buffers
     uint8_t *buf[NBUF][3];
+    // caller-owned planes standing in for buf[b] during one call
+    // (rav2d_gpu_filter_planes); null = the frame's own
+    uint8_t *ext[NBUF][3] = {};
     Rav2dAv2Filter *d_mask = nullptr;
     size_t n_mask = 0;
     Rav2dAv2Restoration *d_lr = nullptr;
@@ -152,6 +158,13 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp);
 DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl);
 bool frame_has_grain(const Rav2dFrameParams &P);
 int grain_args(Rav2dGpuFrame *f, GrainArgs &a);
+bool data_has_grain(const Rav2dFilmGrainData &d);
+// Bytes of a grain scratch area (templates, scaling LUTs, block offsets, error
+// word) for a width x height picture, and the arguments of the grain kernels
+// for one with its scratch at `scratch` (src / dst planes left to the caller).
+size_t grain_scratch_bytes(int width, int height);
+int grain_args_for(const Rav2dFilmGrainData &d, uint32_t seed, int bitdepth, int width, int height, uint8_t *scratch,
+                   uint32_t *err, GrainArgs &a);
 int frame_errors(Rav2dGpuFrame *f);  // after a synchronization: RAV2D_EDEVICE if a kernel flagged an error
 hipStream_t pick_stream(Rav2dGpuCtx *ctx, void *stream);
 

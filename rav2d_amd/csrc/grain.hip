@@ -16,6 +16,8 @@
 // computes every output sample independently.
 #include <string.h>
 
+#include <mutex>
+
 #include "internal.h"
 #include "tables.h"
 
@@ -508,16 +510,25 @@ __global__ __launch_bounds__(256) void grain_apply_batch(const GrainArgs *__rest
 
 // Per device: the Gaussian sequence in device memory and the LFSR jump-ahead
 // matrices in constant memory.
+// The Gaussian sequence lives in a static device array (each device's code
+// object holds one), filled once per device: building the tables allocates
+// nothing.
+__device__ __align__(16) int16_t g_gauss[2048];
+
 static const int16_t *grain_tables() {
+    static std::mutex mu;
     static const int16_t *per_dev[64] = {};
     int dev = 0;
     (void)hipGetDevice(&dev);
     if (dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
     const int16_t *&gauss = per_dev[dev];
     if (!gauss) {
         int16_t *g = nullptr;
-        if (hipMalloc((void **)&g, sizeof(R2T_GAUSSIAN_SEQUENCE)) != hipSuccess) return nullptr;
-        (void)hipMemcpy(g, R2T_GAUSSIAN_SEQUENCE, sizeof(R2T_GAUSSIAN_SEQUENCE), hipMemcpyHostToDevice);
+        static_assert(sizeof(R2T_GAUSSIAN_SEQUENCE) == sizeof(int16_t) * 2048, "Gaussian sequence size");
+        if (hipGetSymbolAddress((void **)&g, HIP_SYMBOL(g_gauss)) != hipSuccess) return nullptr;
+        if (hipMemcpy(g, R2T_GAUSSIAN_SEQUENCE, sizeof(R2T_GAUSSIAN_SEQUENCE), hipMemcpyHostToDevice) != hipSuccess)
+            return nullptr;
         // jump-ahead matrices: M^(k kChunk), k < 128, and M^(8 kBandSeg j), j < kSegs
         auto mul = [](const uint16_t *A, const uint16_t *B, uint16_t *C) {
             for (int i = 0; i < 16; i++) {
@@ -551,6 +562,85 @@ static const int16_t *grain_tables() {
         gauss = g;
     }
     return gauss;
+}
+
+// generate_grain_y / generate_grain_uv (filmgrain.rs:115-229) of one plane
+// into a caller's template buffer (rows of 82): the set-up's Gaussian draws
+// and wavefront AR filter for that plane alone; a chroma plane reads the
+// caller's luma template, complete before the filter starts.
+struct GrainGenArgs {
+    Rav2dFilmGrainData fg;
+    uint32_t seed;
+    int32_t bitdepth, plane;
+    int16_t *out;
+    const int16_t *luma;
+};
+
+__global__ __launch_bounds__(kSetupThreads) void grain_generate_kernel(GrainGenArgs a, const int16_t *gauss) {
+    __shared__ GrainSmem sm;
+    const Rav2dFilmGrainData &d = a.fg;
+    const int bdm8 = a.bitdepth - 8;
+    const int shift = 4 - bdm8 + d.grain_scale_shift;
+    const int ctr = 128 << bdm8, gmin = -ctr, gmax = ctr - 1;
+    const int t = threadIdx.x, wave = t >> 6, lag = d.ar_coeff_lag, pl = a.plane;
+    for (int i = t; i < 2048; i += kSetupThreads) sm.g_lds[i] = gauss[i];
+    if (t < 28) {
+        const int r = t / 7, c = t % 7, dy = r - 3, dx = c - 3;
+        const bool in = dy >= -lag && dx >= -lag && (dy < 0 ? dx <= lag : dx < 0);
+        sm.win[pl][r][c] = (int8_t)(in ? d.ar_coeffs[pl][(dy + lag) * (2 * lag + 1) + (dx + lag)] : 0);
+    }
+    if (pl)
+        for (int i = t; i < GH * GW; i += kSetupThreads) sm.lut[0][i / GW][i % GW] = a.luma[i];
+    if (t == 0) sm.luma_done = 1 << 30;  // the luma template is complete
+    __syncthreads();
+    {
+        const int w = pl ? SGW : GW, n = pl ? SGW * SGH : GW * GH, lanes = (n + kChunk - 1) / kChunk;
+        const int first = t * kChunk, last = dmin(first + kChunk, n);
+        if (t < lanes) {
+            uint32_t s = a.seed;
+            if (pl) s ^= pl == 2 ? 0x49d8 : 0xb524;
+            s = jump(kJump[t], s & 0xffff);
+            int y = first / w, x = first - y * w;
+            for (int i = first; i < last; i++) {
+                sm.lut[pl][y][x] = (int16_t)round2(sm.g_lds[lfsr(11, s)], shift);
+                if (++x == w) {
+                    x = 0;
+                    y++;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        if (pl == 0) {
+            ar_wave<false>(sm.lut[0], sm.win[0], (int)d.ar_coeff_shift, gmin, gmax, nullptr, 0, &sm.luma_done, nullptr);
+        } else {
+            const int wl = d.num_points[0] > 0 ? d.ar_coeffs[pl][2 * lag * (lag + 1)] : 0;
+            ar_wave<true>(sm.lut[pl], sm.win[pl], (int)d.ar_coeff_shift, gmin, gmax, sm.lut[0], wl, &sm.luma_done,
+                          nullptr);
+        }
+    }
+    __syncthreads();
+    const int rows = pl ? SGH : GH, cols = pl ? SGW : GW;
+    for (int i = t; i < rows * cols; i += kSetupThreads) {
+        const int y = i / cols, x = i - y * cols;
+        a.out[y * GW + x] = sm.lut[pl][y][x];
+    }
+}
+
+int launch_grain_generate(const Rav2dFilmGrainData &fg, uint32_t seed, int bitdepth, int plane, int16_t *out,
+                          const int16_t *luma, hipStream_t s) {
+    const int16_t *gauss = grain_tables();
+    if (!gauss) return RAV2D_ENOMEM;
+    GrainGenArgs a;
+    a.fg = fg;
+    a.seed = seed;
+    a.bitdepth = bitdepth;
+    a.plane = plane;
+    a.out = out;
+    a.luma = luma;
+    hipLaunchKernelGGL(grain_generate_kernel, dim3(1), dim3(kSetupThreads), 0, s, a, gauss);
+    return hipGetLastError() == hipSuccess ? RAV2D_OK : RAV2D_EDEVICE;
 }
 
 void launch_grain_setup(const GrainArgs &a, hipStream_t s) {

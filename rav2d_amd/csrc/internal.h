@@ -129,6 +129,10 @@ void launch_grain_setup_batch(const GrainArgs *d_args, int n, hipStream_t s);
 void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st);
 void grain_tile_map(GrainArgs &a);  // 64x32 tiles of the three planes
 bool grain_init_tables();           // Gaussian table and jump matrices of the current device
+// generate_grain_y (plane 0) / generate_grain_uv (plane 1 U, 2 V; 4:2:0, from
+// the luma template `luma`) of one plane into `out` (rows of 82, device).
+int launch_grain_generate(const Rav2dFilmGrainData &fg, uint32_t seed, int bitdepth, int plane, int16_t *out,
+                          const int16_t *luma, hipStream_t s);
 
 void launch_deblock_v(const DeblockArgs &a, int bitdepth, hipStream_t s);
 void launch_deblock_h(const DeblockArgs &a, int bitdepth, hipStream_t s);

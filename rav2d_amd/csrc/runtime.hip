@@ -375,11 +375,13 @@ int upload_lr_items(Rav2dGpuFrame *f) {
 
 namespace r2g {
 
-bool frame_has_grain(const Rav2dFrameParams &P) {
-    const Rav2dFilmGrainData &d = P.grain;
-    return P.grain_present && (d.num_points[0] || d.num_points[1] || d.num_points[2] ||
-                               (d.clip_to_restricted_range && d.chroma_scaling_from_luma));
+// picture_has_grain (decode.rs:4641-4648)
+bool data_has_grain(const Rav2dFilmGrainData &d) {
+    return d.num_points[0] || d.num_points[1] || d.num_points[2] ||
+           (d.clip_to_restricted_range && d.chroma_scaling_from_luma);
 }
+
+bool frame_has_grain(const Rav2dFrameParams &P) { return P.grain_present && data_has_grain(P.grain); }
 
 }  // namespace r2g
 
@@ -389,9 +391,14 @@ namespace {
 
 namespace r2g {
 
-int grain_args(Rav2dGpuFrame *f, GrainArgs &a) {
-    const Rav2dFrameParams &P = f->params;
-    const Rav2dFilmGrainData &d = P.grain;
+size_t grain_scratch_bytes(int width, int height) {
+    // templates, scaling LUTs, block offsets at the smallest block size, and a
+    // last 256-byte line for the error word
+    return 3 * 73 * 82 * 2 + 3 * 1024 + (size_t)((height + 15) / 16) * ((width + 15) / 16) * 2 + 256;
+}
+
+int grain_args_for(const Rav2dFilmGrainData &d, uint32_t seed, int bitdepth, int width, int height, uint8_t *scratch,
+                   uint32_t *err, GrainArgs &a) {
     if (d.ar_coeff_lag < 0 || d.ar_coeff_lag > 3 || d.block_size < 0 || d.block_size > 1 || d.scaling_shift < 8 ||
         d.scaling_shift > 11 || d.ar_coeff_shift < 6 || d.ar_coeff_shift > 9 || d.grain_scale_shift < 0 ||
         d.grain_scale_shift > 3) {
@@ -408,20 +415,27 @@ int grain_args(Rav2dGpuFrame *f, GrainArgs &a) {
     }
     memset(&a, 0, sizeof(a));
     a.fg = d;
-    a.seed = P.grain_seed;
-    a.bitdepth = f->g.bitdepth;
-    a.lut = (int16_t *)f->d_grain;
-    a.scaling = f->d_grain + 3 * 73 * 82 * 2;
+    a.seed = seed;
+    a.bitdepth = bitdepth;
+    a.lut = (int16_t *)scratch;
+    a.scaling = scratch + 3 * 73 * 82 * 2;
     a.offsets = a.scaling + 3 * 1024;
-    a.err = f->d_grain_err;
+    a.err = err;
     int bs = 16 << d.block_size;
-    a.n_bands = (P.height + bs - 1) / bs;
-    a.n_blocks = (P.width + bs - 1) / bs;
+    a.n_bands = (height + bs - 1) / bs;
+    a.n_blocks = (width + bs - 1) / bs;
     // apply_grain_row (filmgrain.rs:713, 744-747): a plane is grained only when
     // its scaling LUT exists; chroma-from-luma uses the luma LUT
     a.plane_on[0] = d.num_points[0] > 0;
     for (int uv = 1; uv < 3; uv++)
         a.plane_on[uv] = d.chroma_scaling_from_luma ? d.num_points[0] > 0 : d.num_points[uv] > 0;
+    return RAV2D_OK;
+}
+
+int grain_args(Rav2dGpuFrame *f, GrainArgs &a) {
+    const Rav2dFrameParams &P = f->params;
+    int r = grain_args_for(P.grain, P.grain_seed, f->g.bitdepth, P.width, P.height, f->d_grain, f->d_grain_err, a);
+    if (r) return r;
     for (int pl = 0; pl < 3; pl++) {
         a.src[pl] = plane_of(f, BUF_OUT, pl);
         a.dst[pl] = plane_of(f, BUF_DISP, pl);
@@ -523,7 +537,7 @@ namespace r2g {
 
 DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl) {
     DevPlane d;
-    d.ptr = f->buf[b][pl];
+    d.ptr = f->ext[b][pl] ? f->ext[b][pl] : f->buf[b][pl];
     d.stride = f->stride[pl];
     d.w = f->pw[pl];
     d.h = f->ph[pl];
@@ -535,9 +549,34 @@ DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl) {
 namespace {
 
 int copy_plane(Rav2dGpuFrame *f, int from, int to, int pl, hipStream_t s) {
-    HIP_TRY(hipMemcpy2DAsync(f->buf[to][pl], f->stride[pl], f->buf[from][pl], f->stride[pl],
+    HIP_TRY(hipMemcpy2DAsync(plane_of(f, to, pl).ptr, f->stride[pl], plane_of(f, from, pl).ptr, f->stride[pl],
                              (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyDeviceToDevice, s));
     return RAV2D_OK;
+}
+
+// Plane layout of the library's pictures: rows padded to 256 B (+256 B when
+// the stride is a multiple of 1 KiB, keeping rows off one channel), 8 spare
+// rows per plane.
+void picture_layout(int w, int h, int bps, int64_t stride[3], size_t bytes[3], int pw[3], int ph[3]) {
+    pw[0] = w;
+    ph[0] = h;
+    pw[1] = pw[2] = (w + 1) >> 1;
+    ph[1] = ph[2] = (h + 1) >> 1;
+    for (int pl = 0; pl < 3; pl++) {
+        int64_t s = ((int64_t)pw[pl] * bps + 255) & ~(int64_t)255;
+        if ((s & 1023) == 0) s += 256;
+        stride[pl] = s;
+        bytes[pl] = (size_t)s * (ph[pl] + 8);
+    }
+}
+
+// Caller planes must carry the frame's strides (the kernels address every
+// buffer of a plane with one stride) and 8-byte alignment (4-sample groups).
+bool planes_match(const Rav2dGpuFrame *f, const Rav2dPlanes *p) {
+    if (!p || p->stride[0] != f->stride[0] || p->stride[1] != f->stride[1]) return false;
+    for (int pl = 0; pl < 3; pl++)
+        if (!p->data[pl] || ((uintptr_t)p->data[pl] & 7)) return false;
+    return true;
 }
 
 }  // namespace
@@ -733,6 +772,7 @@ size_t rav2d_gpu_struct_size(int which) {
         case 5: return sizeof(Rav2dBatchConfig);
         case 6: return sizeof(Rav2dServeConfig);
         case 7: return sizeof(Rav2dServeResult);
+        case 8: return sizeof(Rav2dWienerParams);
         default: return 0;
     }
 }
@@ -780,6 +820,7 @@ int rav2d_gpu_ctx_destroy(Rav2dGpuCtx *ctx) {
         (void)hipEventDestroy(r.b);
     }
     for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+    for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamSynchronize(ctx->aux);
@@ -797,6 +838,62 @@ int rav2d_gpu_ctx_synchronize(Rav2dGpuCtx *ctx) {
 }
 
 uint64_t rav2d_gpu_ctx_bytes_in_use(const Rav2dGpuCtx *ctx) { return ctx ? ctx->in_use : 0; }
+
+int rav2d_gpu_alloc(Rav2dGpuCtx *ctx, size_t bytes, void **out) {
+    if (!ctx || !out || bytes == 0) return RAV2D_EINVAL;
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(ctx->device));
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (ctx->budget && ctx->in_use + bytes > ctx->budget) {
+            set_last_error("HBM budget exceeded");
+            return RAV2D_EAGAIN;
+        }
+        ctx->in_use += bytes;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->in_use -= bytes;
+        set_last_error("hipMalloc failed");
+        return RAV2D_ENOMEM;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->allocs[p] = bytes;
+    *out = p;
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_free(Rav2dGpuCtx *ctx, void *p) {
+    if (!ctx || !p) return RAV2D_EINVAL;
+    size_t bytes = 0;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->allocs.find(p);
+        if (it == ctx->allocs.end()) {
+            set_last_error("not a buffer of this context");
+            return RAV2D_ENOENT;
+        }
+        bytes = it->second;
+        ctx->allocs.erase(it);
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    // work already enqueued on the device may still use the buffer
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipFree(p));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->in_use -= bytes;
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_copy(Rav2dGpuCtx *ctx, void *dst, const void *src, size_t bytes, void *stream) {
+    if (!ctx || !dst || !src) return RAV2D_EINVAL;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = pick_stream(ctx, stream);
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
+    if (!stream) HIP_TRY(hipStreamSynchronize(s));
+    return RAV2D_OK;
+}
 
 int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav2dGpuFrame **out) {
     if (!ctx || !out) return RAV2D_EINVAL;
@@ -819,20 +916,10 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
     g.ss_hor = 1;
     g.ss_ver = 1;
     f->bps = params->bitdepth > 8 ? 2 : 1;
-    f->pw[0] = params->width;
-    f->ph[0] = params->height;
-    f->pw[1] = f->pw[2] = (params->width + 1) >> 1;
-    f->ph[1] = f->ph[2] = (params->height + 1) >> 1;
     size_t total = 0;
-    for (int pl = 0; pl < 3; pl++) {
-        int64_t s = ((int64_t)f->pw[pl] * f->bps + 255) & ~(int64_t)255;
-        if ((s & 1023) == 0) s += 256;  // keep rows off the same channel stride
-        f->stride[pl] = s;
-        f->plane_bytes[pl] = (size_t)s * (f->ph[pl] + 8);
-        total += f->plane_bytes[pl];
-    }
-    const size_t grain_bytes = 3 * 73 * 82 * 2 + 3 * 1024 +
-                               (size_t)((params->height + 15) / 16) * ((params->width + 15) / 16) * 2 + 256;
+    picture_layout(params->width, params->height, f->bps, f->stride, f->plane_bytes, f->pw, f->ph);
+    for (int pl = 0; pl < 3; pl++) total += f->plane_bytes[pl];
+    const size_t grain_bytes = grain_scratch_bytes(params->width, params->height);
     // side info and the LR work list are reserved here too, so that the budget
     // check covers everything the frame will hold
     const size_t n_sb = (size_t)g.sb256w * g.sb256h;
@@ -1065,6 +1152,157 @@ int rav2d_gpu_run_pass(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int pass, void *strea
     int r = run_chain(ctx, f, pass, s);
     if (r) return r;
     return mark_done(f, s);
+}
+
+int rav2d_gpu_picture_alloc(Rav2dGpuCtx *ctx, int width, int height, int bitdepth, int layout, Rav2dPlanes *out) {
+    if (!ctx || !out) return RAV2D_EINVAL;
+    Rav2dFrameParams p;
+    memset(&p, 0, sizeof(p));
+    p.width = width;
+    p.height = height;
+    p.bitdepth = bitdepth;
+    p.layout = layout;
+    int r = validate(&p);
+    if (r) return r;
+    int64_t stride[3];
+    size_t bytes[3];
+    int pw[3], ph[3];
+    picture_layout(width, height, bitdepth > 8 ? 2 : 1, stride, bytes, pw, ph);
+    void *mem = nullptr;
+    r = rav2d_gpu_alloc(ctx, bytes[0] + bytes[1] + bytes[2], &mem);
+    if (r) return r;
+    out->data[0] = mem;
+    out->data[1] = (uint8_t *)mem + bytes[0];
+    out->data[2] = (uint8_t *)mem + bytes[0] + bytes[1];
+    out->stride[0] = stride[0];
+    out->stride[1] = stride[1];
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_picture_free(Rav2dGpuCtx *ctx, Rav2dPlanes *p) {
+    if (!ctx || !p || !p->data[0]) return RAV2D_EINVAL;
+    int r = rav2d_gpu_free(ctx, p->data[0]);
+    if (r) return r;
+    memset(p, 0, sizeof(*p));
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_filter_planes(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, const Rav2dPlanes *in, const Rav2dPlanes *out,
+                            void *stream) {
+    if (!ctx || !f || f->ctx != ctx) return RAV2D_EINVAL;
+    if (!planes_match(f, in) || !planes_match(f, out)) {
+        set_last_error("filter_planes: planes need the frame's strides (rav2d_gpu_frame_planes) and 8-byte alignment");
+        return RAV2D_EINVAL;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = pick_stream(ctx, stream);
+    if (frame_has_grain(f->params) && !f->grain_ready) {
+        int r = start_grain_setup(ctx, f, s);
+        if (r) return r;
+    }
+    // in place (out = in): the passes that read the input after others have
+    // written the output (loop restoration's stripe-edge rows when a plane is
+    // not deblocked) take it from the frame's own input copy
+    bool alias = false;
+    for (int pl = 0; pl < 3; pl++) alias = alias || in->data[pl] == out->data[pl];
+    for (int pl = 0; pl < 3; pl++) {
+        if (alias)
+            HIP_TRY(hipMemcpy2DAsync(f->buf[BUF_IN][pl], f->stride[pl], in->data[pl], f->stride[pl],
+                                     (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyDeviceToDevice, s));
+        else
+            f->ext[BUF_IN][pl] = (uint8_t *)in->data[pl];
+        f->ext[BUF_OUT][pl] = (uint8_t *)out->data[pl];
+    }
+    int r = run_chain(ctx, f, RAV2D_PASS_GRAIN, s);
+    memset(f->ext, 0, sizeof(f->ext));
+    if (r) return r;
+    return mark_done(f, s);
+}
+
+}  // extern "C"
+
+namespace {
+
+// The grain kernels on caller planes with the scratch at `scratch`.
+int grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *dst, int width, int height,
+                 int bitdepth, int layout, const Rav2dFilmGrainData *data, uint32_t seed, uint8_t *scratch,
+                 hipStream_t s) {
+    const int bps = bitdepth > 8 ? 2 : 1;
+    const int pw[3] = {width, (width + 1) >> 1, (width + 1) >> 1};
+    const int ph[3] = {height, (height + 1) >> 1, (height + 1) >> 1};
+    if (!data_has_grain(*data)) {
+        // the display copy of a picture without grain (whatever the other fields)
+        for (int pl = 0; pl < 3; pl++)
+            HIP_TRY(hipMemcpy2DAsync(dst->data[pl], dst->stride[pl ? 1 : 0], src->data[pl], src->stride[pl ? 1 : 0],
+                                     (size_t)pw[pl] * bps, ph[pl], hipMemcpyDeviceToDevice, s));
+        return RAV2D_OK;
+    }
+    GrainArgs a;
+    int r = grain_args_for(*data, seed, bitdepth, width, height, scratch, nullptr, a);
+    if (r) return r;
+    for (int pl = 0; pl < 3; pl++) {
+        a.src[pl] = DevPlane{(uint8_t *)src->data[pl], src->stride[pl ? 1 : 0], pw[pl], ph[pl]};
+        a.dst[pl] = DevPlane{(uint8_t *)dst->data[pl], dst->stride[pl ? 1 : 0], pw[pl], ph[pl]};
+    }
+    if (!grain_init_tables()) {
+        set_last_error("film grain tables: device set-up failed");
+        return RAV2D_EDEVICE;
+    }
+    launch_grain_setup(a, s);
+    HIP_TRY(hipGetLastError());
+    launch_grain_apply(a, bitdepth, s);
+    HIP_TRY(hipGetLastError());
+    (void)layout;
+    return RAV2D_OK;
+}
+
+int check_grain_planes(const Rav2dPlanes *src, const Rav2dPlanes *dst, int width, int height, int bitdepth,
+                       int layout, const Rav2dFilmGrainData *data) {
+    if (!src || !dst || !data) return RAV2D_EINVAL;
+    Rav2dFrameParams p;
+    memset(&p, 0, sizeof(p));
+    p.width = width;
+    p.height = height;
+    p.bitdepth = bitdepth;
+    p.layout = layout;
+    int r = validate(&p);
+    if (r) return r;
+    const int bps = bitdepth > 8 ? 2 : 1;
+    for (const Rav2dPlanes *q : {src, dst}) {
+        for (int pl = 0; pl < 3; pl++)
+            if (!q->data[pl] || ((uintptr_t)q->data[pl] & 7)) {
+                set_last_error("grain planes: plane pointers must be 8-byte aligned");
+                return RAV2D_EINVAL;
+            }
+        if ((q->stride[0] & 7) || (q->stride[1] & 7) || q->stride[0] < (ptrdiff_t)width * bps ||
+            q->stride[1] < (ptrdiff_t)((width + 1) >> 1) * bps || q->stride[0] >= (1 << 24) ||
+            q->stride[1] >= (1 << 24)) {
+            set_last_error("grain planes: strides must be multiples of 8 covering a row, below 16 MiB");
+            return RAV2D_EINVAL;
+        }
+    }
+    return RAV2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rav2d_gpu_apply_grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *dst, int width,
+                                 int height, int bitdepth, int layout, const Rav2dFilmGrainData *data, uint32_t seed,
+                                 void *stream) {
+    if (!ctx) return RAV2D_EINVAL;
+    int r = check_grain_planes(src, dst, width, height, bitdepth, layout, data);
+    if (r) return r;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = pick_stream(ctx, stream);
+    if (!data_has_grain(*data)) return grain_planes(ctx, src, dst, width, height, bitdepth, layout, data, seed, nullptr, s);
+    // a scratch area of its own per call, stream-ordered
+    uint8_t *scratch = nullptr;
+    HIP_TRY(hipMallocAsync((void **)&scratch, grain_scratch_bytes(width, height), s));
+    r = grain_planes(ctx, src, dst, width, height, bitdepth, layout, data, seed, scratch, s);
+    HIP_TRY(hipFreeAsync(scratch, s));
+    return r;
 }
 
 int rav2d_gpu_set_profiling(Rav2dGpuCtx *ctx, int on) {

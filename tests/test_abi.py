@@ -38,6 +38,7 @@ def test_struct_layouts_match_mirrors():
     assert lib.rav2d_gpu_struct_size(5) == C.sizeof(R.BatchConfig)
     assert lib.rav2d_gpu_struct_size(6) == C.sizeof(R.ServeConfig)
     assert lib.rav2d_gpu_struct_size(7) == C.sizeof(R.ServeResult)
+    assert lib.rav2d_gpu_struct_size(8) == C.sizeof(R.WienerParams)
     assert lib.rav2d_gpu_struct_size(99) == 0
 
 
