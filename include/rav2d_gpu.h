@@ -279,6 +279,16 @@ int rav2d_gpu_filter_planes(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, const Rav2dPlane
 int rav2d_gpu_apply_grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *dst, int width,
                                  int height, int bitdepth, int layout, const Rav2dFilmGrainData *data, uint32_t seed,
                                  void *stream);
+/* No counterpart in the reference: rav2d_gpu_apply_grain_planes for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and output; the scratch
+ * area is a static device array per GPU (563,532 bytes, held until the process
+ * ends), so the call allocates nothing.  Pictures up to 8192 x 8192 luma
+ * samples (RAV2D_EINVAL beyond).  Every call on one GPU shares that array:
+ * calls that overlap on a GPU (other streams, other threads) race. */
+int rav2d_gpu_apply_grain_planes_noalloc(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *dst,
+                                         int width, int height, int bitdepth, int layout,
+                                         const Rav2dFilmGrainData *data, uint32_t seed, void *stream);
 
 /* ---- per-pass entry points (unit parity and per-kernel profiling) -------- */
 #define RAV2D_PASS_DEBLOCK_V 0

@@ -208,6 +208,9 @@ def load_gpu():
     _declare(lib, "rav2d_gpu_apply_grain_planes", C.c_int,
              [P, C.POINTER(Planes), C.POINTER(Planes), C.c_int, C.c_int, C.c_int, C.c_int,
               C.POINTER(FilmGrainData), C.c_uint32, P])
+    _declare(lib, "rav2d_gpu_apply_grain_planes_noalloc", C.c_int,
+             [P, C.POINTER(Planes), C.POINTER(Planes), C.c_int, C.c_int, C.c_int, C.c_int,
+              C.POINTER(FilmGrainData), C.c_uint32, P])
     _declare(lib, "rav2d_gpu_alloc", C.c_int, [P, C.c_size_t, C.POINTER(P)])
     _declare(lib, "rav2d_gpu_free", C.c_int, [P, P])
     _declare(lib, "rav2d_gpu_copy", C.c_int, [P, P, P, C.c_size_t, P])

@@ -1305,6 +1305,65 @@ int rav2d_gpu_apply_grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const
     return r;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Scratch of rav2d_gpu_apply_grain_planes_noalloc: one static device array per
+// GPU (held until the process ends), large enough for pictures up to
+// kNoAllocMaxDim x kNoAllocMaxDim.  Host code never uses the array's name as
+// an address: each device's address comes from hipGetSymbolAddress, taken
+// once per device with that device current.
+constexpr int kNoAllocMaxDim = 8192;
+constexpr size_t kNoAllocScratchBytes = 3 * 73 * 82 * 2 + 3 * 1024 +
+                                        (size_t)(kNoAllocMaxDim / 16) * (kNoAllocMaxDim / 16) * 2 + 256;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[64] = {};
+
+int noalloc_scratch(int device, uint8_t **out) {
+    if (device < 0 || device >= 64) {
+        set_last_error("device index out of range");
+        return RAV2D_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_last_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return RAV2D_EDEVICE;
+        }
+        g_scratch_ptr[device] = (uint8_t *)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return RAV2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rav2d_gpu_apply_grain_planes_noalloc(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *dst,
+                                         int width, int height, int bitdepth, int layout,
+                                         const Rav2dFilmGrainData *data, uint32_t seed, void *stream) {
+    if (!ctx) return RAV2D_EINVAL;
+    int r = check_grain_planes(src, dst, width, height, bitdepth, layout, data);
+    if (r) return r;
+    if (width > kNoAllocMaxDim || height > kNoAllocMaxDim ||
+        grain_scratch_bytes(width, height) > kNoAllocScratchBytes) {
+        set_last_error("apply_grain_planes_noalloc: pictures up to 8192 x 8192 only (static scratch)");
+        return RAV2D_EINVAL;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint8_t *ptr = nullptr;
+    r = noalloc_scratch(ctx->device, &ptr);
+    if (r) return r;
+    // the set-up writes every scratch entry the apply kernel reads: nothing to clear
+    return grain_planes(ctx, src, dst, width, height, bitdepth, layout, data, seed, ptr, pick_stream(ctx, stream));
+}
+
 int rav2d_gpu_set_profiling(Rav2dGpuCtx *ctx, int on) {
     if (!ctx) return RAV2D_EINVAL;
     ctx->profiling = on != 0;
