@@ -34,6 +34,7 @@ extern "C" {
 #define RAV2D_EDEVICE (-5)  /* a HIP call failed                           */
 #define RAV2D_EAGAIN (-11)  /* admission rejected (memory budget / slots)  */
 #define RAV2D_ENOENT (-2)   /* unknown ticket / stream                     */
+#define RAV2D_ECANCELED (-125) /* the job was cancelled before it launched */
 
 /* Largest frame width and height the library accepts (luma samples). */
 #define RAV2D_MAX_FRAME_DIM 16384
@@ -398,6 +399,13 @@ int rav2d_gpu_server_submit(Rav2dGpuServer *s, Rav2dGpuFrame *f, uint64_t tag, d
 int rav2d_gpu_server_poll(Rav2dGpuServer *s, Rav2dServeResult *out, int max, int wait_ms);
 /* Blocks until every submitted job has completed. */
 int rav2d_gpu_server_drain(Rav2dGpuServer *s);
+/* paused != 0: the dispatcher launches no new batch (batches already on the
+ * device finish; queued jobs wait) until the server is resumed with 0. */
+int rav2d_gpu_server_pause(Rav2dGpuServer *s, int paused);
+/* Cancel a job still queued (not yet in a batch): its result comes back from
+ * rav2d_gpu_server_poll with status RAV2D_ECANCELED.  RAV2D_ENOENT when no
+ * queued job has the tag (already launched, done, or never submitted). */
+int rav2d_gpu_server_cancel(Rav2dGpuServer *s, uint64_t tag);
 /* Simulated streams: stream i owns frames[i*slots .. i*slots + slots-1],
  * set up by the caller.  Each stream's frames arrive as a Poisson process of
  * rate fps (seeded); job j filters its stream's next slot in turn.  Submits
@@ -567,6 +575,16 @@ size_t rav2d_gpu_struct_size(int which);
 const char *rav2d_gpu_version(void);
 /* Text of the last error recorded on this thread. */
 const char *rav2d_gpu_last_error(void);
+
+/* Fault injection for tests of the error paths (no counterpart in the
+ * reference): the next `count` device allocations of the library (frames,
+ * rav2d_gpu_alloc, batch argument blocks; what = RAV2D_FAIL_ALLOC) or the
+ * next `count` filter launches (the per-frame chain and batch / server
+ * enqueues; what = RAV2D_FAIL_LAUNCH) fail on the host as if the HIP call had
+ * failed, before anything reaches the device.  count 0 clears it. */
+#define RAV2D_FAIL_ALLOC 0
+#define RAV2D_FAIL_LAUNCH 1
+int rav2d_gpu_debug_inject_failure(int what, int count);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,14 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 RAV2D_OK = 0
+RAV2D_EINVAL, RAV2D_ENOMEM, RAV2D_EDEVICE, RAV2D_EAGAIN, RAV2D_ENOENT, RAV2D_ECANCELED = -22, -12, -5, -11, -2, -125
+FAIL_ALLOC, FAIL_LAUNCH = 0, 1
+
+
+def inject_failure(what, count):
+    """The next `count` library allocations (FAIL_ALLOC) or filter launches
+    (FAIL_LAUNCH) fail on the host (rav2d_gpu_debug_inject_failure)."""
+    _check(load_gpu().rav2d_gpu_debug_inject_failure(what, count), "inject_failure")
 INLOOP_DEBLOCK, INLOOP_CDEF, INLOOP_CCSO, INLOOP_WIENER, INLOOP_GDF = 1, 2, 4, 8, 16
 INLOOP_ALL = 31
 PASS_DEBLOCK_V, PASS_DEBLOCK_H, PASS_CDEF, PASS_LR, PASS_GRAIN = range(5)
@@ -197,6 +205,9 @@ def load_gpu():
     _declare(lib, "rav2d_gpu_server_submit", C.c_int, [P, P, C.c_uint64, C.c_double])
     _declare(lib, "rav2d_gpu_server_poll", C.c_int, [P, C.POINTER(ServeResult), C.c_int, C.c_int])
     _declare(lib, "rav2d_gpu_server_drain", C.c_int, [P])
+    _declare(lib, "rav2d_gpu_server_pause", C.c_int, [P, C.c_int])
+    _declare(lib, "rav2d_gpu_server_cancel", C.c_int, [P, C.c_uint64])
+    _declare(lib, "rav2d_gpu_debug_inject_failure", C.c_int, [C.c_int, C.c_int])
     _declare(lib, "rav2d_gpu_server_simulate", C.c_int,
              [P, C.POINTER(P), C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, C.POINTER(ServeResult),
               C.POINTER(C.c_int32), C.POINTER(C.c_int32)])
@@ -620,6 +631,17 @@ class GpuServer:
 
     def drain(self):
         _check(self.lib.rav2d_gpu_server_drain(self.h), "server_drain")
+
+    def pause(self, on=True):
+        _check(self.lib.rav2d_gpu_server_pause(self.h, 1 if on else 0), "server_pause")
+
+    def cancel(self, tag):
+        """True if the queued job was cancelled, False if no queued job has the tag."""
+        rc = self.lib.rav2d_gpu_server_cancel(self.h, tag)
+        if rc == RAV2D_ENOENT:
+            return False
+        _check(rc, "server_cancel")
+        return True
 
     def simulate(self, stream_frames, slots, fps, n_jobs, seed):
         """stream_frames: list (per stream) of `slots` GpuFrames.  Returns

@@ -172,6 +172,10 @@ void pack_batch(const HostBatch &hb, uint8_t *host, uint8_t *dev, DevBatch &db) 
 }
 
 int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain, const EnqueueCtx &ec) {
+    if (injected_failure(RAV2D_FAIL_LAUNCH)) {
+        set_last_error("batch launch failed (injected)");
+        return RAV2D_EDEVICE;
+    }
     const hipStream_t s = ec.s;
     const bool grain = apply_grain && (db.total[RAV2D_PASS_GRAIN][0] + db.total[RAV2D_PASS_GRAIN][1]) > 0;
     // the two bit-depth groups are independent chains: with a second stream they
@@ -292,7 +296,7 @@ int build(Rav2dGpuBatch *b) {
         }
         b->ctx->in_use += bytes;
     }
-    if (hipMalloc((void **)&b->dmem, bytes) != hipSuccess) {
+    if (injected_failure(RAV2D_FAIL_ALLOC) || hipMalloc((void **)&b->dmem, bytes) != hipSuccess) {
         std::lock_guard<std::mutex> lk(b->ctx->mu);
         b->ctx->in_use -= bytes;
         b->dmem = nullptr;

@@ -141,6 +141,9 @@ void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched
 TileMap deblock_tile_map(const DeblockArgs &a);  // 64x32 tiles of the three planes
 
 void set_last_error(const std::string &msg);
+// rav2d_gpu_debug_inject_failure: true when the next operation of kind `what`
+// (RAV2D_FAIL_ALLOC / RAV2D_FAIL_LAUNCH) is to fail
+bool injected_failure(int what);
 
 // Kernel-accurate timing of one launch.  While armed, the next kernel launched
 // through R2G_LAUNCH on this thread carries the two events, which the runtime

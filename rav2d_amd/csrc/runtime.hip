@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -18,6 +19,16 @@ void set_last_error(const std::string &msg) { g_last_error = msg; }
 LaunchTiming &launch_timing() {
     static thread_local LaunchTiming t;
     return t;
+}
+
+static std::atomic<int> g_fail[2];
+
+bool injected_failure(int what) {
+    if (what < 0 || what > 1) return false;
+    int n = g_fail[what].load();
+    while (n > 0)
+        if (g_fail[what].compare_exchange_weak(n, n - 1)) return true;
+    return false;
 }
 
 }  // namespace r2g
@@ -729,6 +740,10 @@ int run_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, hipStream_t s) {
     ChainPlan cp;
     int r = plan_chain(ctx, f, upto, cp);
     if (r) return r;
+    if (injected_failure(RAV2D_FAIL_LAUNCH)) {
+        set_last_error("filter launch failed (injected)");
+        return RAV2D_EDEVICE;
+    }
     const int bd = f->g.bitdepth;
     if (cp.on[RAV2D_PASS_DEBLOCK_V]) {
         PassTimer pt(ctx, RAV2D_PASS_DEBLOCK_V, s);
@@ -778,6 +793,12 @@ size_t rav2d_gpu_struct_size(int which) {
 }
 
 const char *rav2d_gpu_version(void) { return "rav2d-gpu 0.1 (gfx950)"; }
+
+int rav2d_gpu_debug_inject_failure(int what, int count) {
+    if (what < 0 || what > 1 || count < 0) return RAV2D_EINVAL;
+    g_fail[what].store(count);
+    return RAV2D_OK;
+}
 const char *rav2d_gpu_last_error(void) { return g_last_error.c_str(); }
 
 int rav2d_gpu_device_count(void) {
@@ -852,7 +873,7 @@ int rav2d_gpu_alloc(Rav2dGpuCtx *ctx, size_t bytes, void **out) {
         ctx->in_use += bytes;
     }
     void *p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
+    if (injected_failure(RAV2D_FAIL_ALLOC) || hipMalloc(&p, bytes) != hipSuccess) {
         std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->in_use -= bytes;
         set_last_error("hipMalloc failed");
@@ -942,7 +963,7 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
         ctx->in_use += need;
     }
     f->bytes = need;
-    if (hipMalloc((void **)&f->block, total * NBUF) != hipSuccess ||
+    if (injected_failure(RAV2D_FAIL_ALLOC) || hipMalloc((void **)&f->block, total * NBUF) != hipSuccess ||
         hipMalloc((void **)&f->d_dblut, sizeof(DbLut)) != hipSuccess ||
         hipMalloc((void **)&f->d_grain, grain_bytes) != hipSuccess ||
         hipMalloc((void **)&f->d_mask, n_sb * sizeof(Rav2dAv2Filter)) != hipSuccess ||

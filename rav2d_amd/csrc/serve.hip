@@ -138,6 +138,7 @@ struct Rav2dGpuServer {
     std::vector<Slot> slots;
     int64_t submitted = 0, completed = 0;
     bool stop = false;
+    bool paused = false;  // no new batches launch while set (queued jobs wait)
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -284,7 +285,7 @@ void dispatcher(Rav2dGpuServer *sv) {
         if (sv->stop && sv->pending.empty() && !any_busy) break;
         // jobs whose frame is idle, oldest first; a frame's later jobs wait
         std::vector<Job> take;
-        if (free_slot && !sv->pending.empty()) {
+        if (free_slot && !sv->pending.empty() && !(sv->paused && !sv->stop)) {
             std::vector<const Rav2dGpuFrame *> seen;
             for (auto it = sv->pending.begin(); it != sv->pending.end() && (int)take.size() < sv->cfg.max_batch;) {
                 const Rav2dGpuFrame *f = it->f;
@@ -437,6 +438,39 @@ int rav2d_gpu_server_poll(Rav2dGpuServer *sv, Rav2dServeResult *out, int max, in
         sv->done.pop_front();
     }
     return n;
+}
+
+int rav2d_gpu_server_pause(Rav2dGpuServer *sv, int paused) {
+    if (!sv) return RAV2D_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->paused = paused != 0;
+    }
+    sv->cv_work.notify_all();
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_server_cancel(Rav2dGpuServer *sv, uint64_t tag) {
+    if (!sv) return RAV2D_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        auto it = std::find_if(sv->pending.begin(), sv->pending.end(), [&](const Job &j) { return j.tag == tag; });
+        if (it == sv->pending.end()) {
+            set_last_error("server_cancel: no queued job with this tag (launched, done or unknown)");
+            return RAV2D_ENOENT;
+        }
+        Rav2dServeResult res;
+        memset(&res, 0, sizeof(res));
+        res.tag = it->tag;
+        res.t_submit = it->t_submit;
+        res.t_done = sv->now();
+        res.status = RAV2D_ECANCELED;
+        sv->pending.erase(it);
+        sv->done.push_back(res);
+        sv->completed++;
+    }
+    sv->cv_done.notify_all();
+    return RAV2D_OK;
 }
 
 int rav2d_gpu_server_drain(Rav2dGpuServer *sv) {

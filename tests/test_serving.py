@@ -148,3 +148,103 @@ def test_admission_follows_hbm_budget(sources):
         for f in frames:
             f.close()
         c.close()
+
+
+@pytest.mark.gpu
+def test_injected_allocation_failure_is_reported_and_recovers(ctx, sources):
+    # frame creation whose device allocation fails returns ENOMEM, holds no
+    # memory, and the next creation succeeds
+    sfs, _ = sources
+    before = ctx.bytes_in_use()
+    R.inject_failure(R.FAIL_ALLOC, 1)
+    try:
+        with pytest.raises(R.Rav2dError, match="-12"):
+            R.GpuFrame(ctx, sfs[0].params)
+    finally:
+        R.inject_failure(R.FAIL_ALLOC, 0)
+    assert ctx.bytes_in_use() == before
+    f = R.GpuFrame.from_synth(ctx, sfs[0])
+    f.close()
+
+
+@pytest.mark.gpu
+def test_injected_launch_failure_fails_its_batch_only(ctx, sources):
+    # a batch whose launch fails reports the error in every job of it; the
+    # frames stay usable and later jobs give the planes the frame gives alone
+    sfs, expect = sources
+    frames = [R.GpuFrame.from_synth(ctx, sfs[i]) for i in range(3)]
+    cfg = R.default_serve_config()
+    cfg.checksum = 1
+    srv = R.GpuServer(ctx, cfg)
+    try:
+        srv.pause()
+        for i, f in enumerate(frames):
+            srv.submit(f, i)
+        R.inject_failure(R.FAIL_LAUNCH, 1)
+        srv.pause(False)
+        srv.drain()
+        res = sorted(srv.poll(16), key=lambda r: r.tag)
+        R.inject_failure(R.FAIL_LAUNCH, 0)
+        assert [r.tag for r in res] == [0, 1, 2]
+        assert all(r.status == R.RAV2D_EDEVICE for r in res if r.batch_frames == res[0].batch_frames)
+        assert res[0].status == R.RAV2D_EDEVICE
+        for i, f in enumerate(frames):
+            srv.submit(f, 10 + i)
+        srv.drain()
+        res = sorted(srv.poll(16), key=lambda r: r.tag)
+        assert [r.status for r in res] == [0, 0, 0]
+        assert [(r.checksum[0], r.checksum[1]) for r in res] == [expect[i] for i in range(3)]
+    finally:
+        R.inject_failure(R.FAIL_LAUNCH, 0)
+        srv.close()
+        for f in frames:
+            f.close()
+
+
+@pytest.mark.gpu
+def test_cancel_queued_jobs(ctx, sources):
+    # jobs cancelled while queued come back with ECANCELED; the others finish
+    # with the planes their frames give alone; a launched or unknown job
+    # cannot be cancelled
+    sfs, expect = sources
+    frames = [R.GpuFrame.from_synth(ctx, sfs[i]) for i in range(5)]
+    cfg = R.default_serve_config()
+    cfg.checksum = 1
+    srv = R.GpuServer(ctx, cfg)
+    try:
+        srv.pause()
+        for i, f in enumerate(frames):
+            srv.submit(f, 100 + i)
+        assert srv.cancel(101) and srv.cancel(103)
+        assert not srv.cancel(101) and not srv.cancel(999)
+        srv.pause(False)
+        srv.drain()
+        res = {r.tag: r for r in srv.poll(16)}
+        assert sorted(res) == [100, 101, 102, 103, 104]
+        for i in range(5):
+            r = res[100 + i]
+            if i in (1, 3):
+                assert r.status == R.RAV2D_ECANCELED
+            else:
+                assert r.status == 0 and (r.checksum[0], r.checksum[1]) == expect[i], i
+        assert not srv.cancel(100)  # done
+    finally:
+        srv.close()
+        for f in frames:
+            f.close()
+
+
+@pytest.mark.gpu
+def test_injected_launch_failure_of_a_frame_leaves_it_usable(ctx, sources):
+    sfs, expect = sources
+    f = R.GpuFrame.from_synth(ctx, sfs[0])
+    try:
+        R.inject_failure(R.FAIL_LAUNCH, 1)
+        with pytest.raises(R.Rav2dError, match="injected"):
+            f.filter()
+        f.filter()
+        f.apply_grain()
+        assert (f.checksum(1), f.checksum(2)) == expect[0]
+    finally:
+        R.inject_failure(R.FAIL_LAUNCH, 0)
+        f.close()
