@@ -149,8 +149,15 @@ typedef struct Rav2dFrameParams {
     int32_t yac;
     int32_t uac_delta;
     int32_t vac_delta;
-    /* segmentation must be disabled (segmap input not supported) */
+    /* FhSegmentation (headers.rs, parsed at obu.rs:1263-1280): with seg_enabled
+     * the deblock thresholds of a 4x4 unit use its segment's qidx
+     * clip(qidx + seg_delta_q[id], 0, 255 + 48 * hbd) (init_deblock_thr_lut_y/uv,
+     * deblock.rs:13-57), from the segment maps of rav2d_gpu_frame_set_segmap;
+     * CDEF and CCSO skip the lossless 4x4 units of the masks when a segment is
+     * lossless (decode.rs:3868-3869, cdef.rs:894-926, 1117-1150). */
     int32_t seg_enabled;
+    int16_t seg_delta_q[8];
+    uint8_t seg_lossless[8];
     /* FhDeblock */
     uint8_t db_level_y[2];
     uint8_t db_level_u;
@@ -231,6 +238,18 @@ int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f);
 int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask,
                                   size_t n_mask, const Rav2dAv2Restoration *lr,
                                   size_t n_lr, void *stream);
+/* Segment maps of a segmented frame: FilterFrameParams' cur_segmap and
+ * segmap_uv (decode.rs:285-310, 2355-2367), as deblock64_cols/rows read them
+ * (deblock.rs:1623-2140).  segmap: one segment id (0..7) per luma 4x4 unit,
+ * `stride` bytes per row, for the frame's ((W+7)>>3)*2 x ((H+7)>>3)*2 units;
+ * segmap_uv: one id per chroma 4x4 unit (4:2:0: half as many each way),
+ * uv_stride bytes per row.  Read by the deblocking thresholds when
+ * params.seg_enabled is set: a segmented frame with deblocking on must have
+ * its maps set before it is filtered (RAV2D_EINVAL otherwise).  Ids above 7
+ * are rejected.  segmap NULL drops the maps.  Waits for the frame's earlier
+ * launches like rav2d_gpu_frame_set_side_info. */
+int rav2d_gpu_frame_set_segmap(Rav2dGpuFrame *f, const uint8_t *segmap, ptrdiff_t stride,
+                               const uint8_t *segmap_uv, ptrdiff_t uv_stride, void *stream);
 /* Replace the frame parameters (same size, bit depth and layout).  This and
  * rav2d_gpu_frame_set_side_info first wait for the frame's earlier
  * rav2d_gpu_filter_frame / apply_grain / run_pass launches and its grain

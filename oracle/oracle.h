@@ -18,6 +18,12 @@ extern "C" {
 int oracle_filter_frame(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask, size_t n_mask,
                         const Rav2dAv2Restoration *lr, size_t n_lr, void *y, void *u, void *v,
                         ptrdiff_t y_stride, ptrdiff_t uv_stride, uint32_t inloop_mask);
+/* The same for a segmented frame: seg / seg_uv are the segment maps of the
+ * luma / chroma 4x4 units (strides in bytes), read when p->seg_enabled. */
+int oracle_filter_frame_seg(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask, size_t n_mask,
+                            const Rav2dAv2Restoration *lr, size_t n_lr, void *y, void *u, void *v,
+                            ptrdiff_t y_stride, ptrdiff_t uv_stride, uint32_t inloop_mask, const uint8_t *seg,
+                            ptrdiff_t seg_stride, const uint8_t *seg_uv, ptrdiff_t uv_seg_stride);
 
 /* Film grain on the output copy: dst = src with grain (planes of the display
  * copy), as apply_grain_to_picture_mt; a plain copy when the frame has none. */

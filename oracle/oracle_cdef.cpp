@@ -392,8 +392,16 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
                 int uv_sec = uv_lvl & 3;
                 uv_sec += uv_sec == 3;
                 uv_sec <<= bdm8;
-                // lossless masks are read only on segmented-lossless frames (any_lossless)
+                // lossless masks are read only on segmented-lossless frames (cdef.rs:894-911)
                 uint16_t y_ll0 = 0, y_ll1 = 0, uv_ll0 = 0, uv_ll1 = 0;
+                if (p.any_lossless && m) {
+                    int yr = imin(2 * by_idx, 63);
+                    y_ll0 = m->lossless_mask_y[yr][sb64x_idx];
+                    y_ll1 = m->lossless_mask_y[imin(yr + 1, 63)][sb64x_idx];
+                    int uvr = imin((2 * by_idx) >> ss_ver, 63);
+                    uv_ll0 = m->lossless_mask_uv[uvr][sb64x_idx];
+                    uv_ll1 = m->lossless_mask_uv[imin(uvr + (1 - ss_ver), 63)][sb64x_idx];
+                }
                 ptrdiff_t b_y = sb_y, b_uv = sb_uv;
                 int sb_bx_end = imin((sbx + 1) * sbsz, p.bw);
                 for (int bx = sbx * sbsz; bx < sb_bx_end; bx += 2) {
@@ -480,7 +488,14 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
                     P *dst = pl == 0 ? f.y : pl == 1 ? f.u : f.v;
                     ptrdiff_t off = pl == 0 ? sb_y : sb_uv;
                     ptrdiff_t ls = pl == 0 ? y_ls : uv_ls;
+                    // rows of the lossless masks for the add (cdef.rs:1117-1150)
                     uint16_t ll[2][4] = {{0}};
+                    if (p.any_lossless && m) {
+                        const uint16_t (*src)[4] = pl == 0 ? m->lossless_mask_y : m->lossless_mask_uv;
+                        int base = (2 * by_idx) >> pver;
+                        for (int r = 0; r < 2; r++)
+                            if (base + r < 64) ll[r][0] = src[base + r][sb64x_idx];
+                    }
                     ccso_add(dst + off, ls, lut_idx[pl], 64 >> phor, cfg.filter_off, R2T_CCSO_OFFSET[cfg.scale_idx],
                              w_full >> phor, 8 >> pver, ll, bdmax);
                 }

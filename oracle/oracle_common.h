@@ -37,6 +37,10 @@ int dq_lookup(int qidx);
 struct Geom {
     int bw, bh, sb256w, sb256h, sb128, sb_shift, sb_step, sbh;
     int ss_hor, ss_ver, hbd, bitdepth, bdmax, layout;
+    /* segment maps (cur_segmap / segmap_uv of decode.rs:285-310, 2355-2367);
+     * null on frames without segmentation (the PLACEHOLDER_SEGMAP path) */
+    const uint8_t *seg = nullptr, *seg_uv = nullptr;
+    ptrdiff_t seg_stride = 0, uv_seg_stride = 0;
     Geom(const Rav2dFrameParams *p) {
         bw = ((p->width + 7) >> 3) << 1;
         bh = ((p->height + 7) >> 3) << 1;

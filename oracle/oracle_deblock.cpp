@@ -1,9 +1,8 @@
 /*
  * Oracle: AV2 deblocking, restated from crates/rav2d/src/deblock.rs and
  * crates/rav2d/src/lf_mask.rs.  TEST INFRASTRUCTURE ONLY (see oracle_common.h).
- * Segmentation is disabled in every frame this oracle accepts, so the segment
- * id of every 4x4 block is 0 (the reference's PLACEHOLDER_SEGMAP path,
- * deblock.rs:1463, 1675-1679).
+ * On frames without segmentation the segment id of every 4x4 block is 0 (the
+ * reference's PLACEHOLDER_SEGMAP path, deblock.rs:1463, 1675-1679).
  */
 #include "oracle_common.h"
 #include "../rav2d_amd/csrc/tables.h"
@@ -40,25 +39,33 @@ uint32_t deblock_side_thr(int hbd, int qidx) {
     return (uint32_t)imax(side + ((1 << 4) >> bdm8), 0) >> (5 - bdm8);
 }
 
-/* deblock.rs:13-32 (segmentation disabled: one entry) */
+/* deblock.rs:13-32: one entry per segment (one without segmentation) */
 static void init_lut_y(const Rav2dFrameParams *p, int hbd, int dir, int qidx, uint32_t lut[2][16]) {
     memset(lut, 0, sizeof(uint32_t) * 32);
-    int yac = qidx;
-    int dir_yac = yac + 8 * p->db_delta_q_y[dir];
-    lut[0][0] = deblock_quant_thr(hbd, dir_yac);
-    lut[1][0] = deblock_side_thr(hbd, dir_yac);
+    int qmax = 255 + 48 * hbd;
+    int n = p->seg_enabled ? 8 : 1;
+    for (int i = 0; i < n; i++) {
+        int yac = p->seg_enabled ? iclip(qidx + p->seg_delta_q[i], 0, qmax) : qidx;
+        int dir_yac = yac + 8 * p->db_delta_q_y[dir];
+        lut[0][i] = deblock_quant_thr(hbd, dir_yac);
+        lut[1][i] = deblock_side_thr(hbd, dir_yac);
+    }
 }
 
 /* deblock.rs:34-57 */
 static void init_lut_uv(const Rav2dFrameParams *p, int hbd, int qidx, uint32_t lut[2][2][16]) {
     memset(lut, 0, sizeof(uint32_t) * 64);
-    int yac = qidx;
-    int uac = yac + p->uac_delta + 8 * p->db_delta_q_u;
-    lut[0][0][0] = deblock_quant_thr(hbd, uac);
-    lut[0][1][0] = deblock_side_thr(hbd, uac);
-    int vac = yac + p->vac_delta + 8 * p->db_delta_q_v;
-    lut[1][0][0] = deblock_quant_thr(hbd, vac);
-    lut[1][1][0] = deblock_side_thr(hbd, vac);
+    int qmax = 255 + 48 * hbd;
+    int n = p->seg_enabled ? 8 : 1;
+    for (int i = 0; i < n; i++) {
+        int yac = p->seg_enabled ? iclip(qidx + p->seg_delta_q[i], 0, qmax) : qidx;
+        int uac = yac + p->uac_delta + 8 * p->db_delta_q_u;
+        lut[0][0][i] = deblock_quant_thr(hbd, uac);
+        lut[0][1][i] = deblock_side_thr(hbd, uac);
+        int vac = yac + p->vac_delta + 8 * p->db_delta_q_v;
+        lut[1][0][i] = deblock_quant_thr(hbd, vac);
+        lut[1][1][i] = deblock_side_thr(hbd, vac);
+    }
 }
 
 /* deblock.rs:81-186 */
@@ -185,8 +192,12 @@ static void deblock_sb64(P *dst, ptrdiff_t dst_off, ptrdiff_t stride, int bdmax,
 
 static inline int edge_thr(int cur, int prev) { return (cur && prev) ? (cur + prev + 1) >> 1 : (cur | prev); }
 
-/* deblock.rs:1477-1516 (segment id 0 everywhere) */
-static void setup_thr_cols(uint16_t q_dst[256], uint16_t s_dst[256], const uint16_t (*mask)[5][4], int bx4_base,
+/* segment id at seg[off] (0 without a segment map) */
+static inline int seg_at(const uint8_t *seg, ptrdiff_t off) { return seg ? seg[off] : 0; }
+
+/* deblock.rs:1477-1516 */
+static void setup_thr_cols(uint16_t q_dst[256], uint16_t s_dst[256], const uint8_t *seg, ptrdiff_t seg_off,
+                           ptrdiff_t seg_stride, const uint16_t (*mask)[5][4], int bx4_base,
                            const uint32_t lut[2][16], uint16_t left_q[16], uint16_t left_s[16], int y64,
                            int ss_ver, int w4, int h4) {
     int mask_idx = y64 >> ss_ver;
@@ -194,7 +205,8 @@ static void setup_thr_cols(uint16_t q_dst[256], uint16_t s_dst[256], const uint1
     for (int y4 = 0; y4 < h4; y4++) {
         int prev_q = left_q[y4], prev_s = left_s[y4];
         for (int x4 = 0; x4 < w4; x4++) {
-            int cur_q = (int)lut[0][0], cur_s = (int)lut[1][0];
+            int seg_id = seg_at(seg, seg_off + x4 + y4 * seg_stride);
+            int cur_q = (int)lut[0][seg_id], cur_s = (int)lut[1][seg_id];
             int subpu = 3 * ((mask[bx4_base + x4][4][mask_idx] >> (mask_shift + y4)) & 1);
             q_dst[x4 * 16 + y4] = (uint16_t)(edge_thr(cur_q, prev_q) >> subpu);
             s_dst[x4 * 16 + y4] = (uint16_t)(edge_thr(cur_s, prev_s) >> subpu);
@@ -207,22 +219,25 @@ static void setup_thr_cols(uint16_t q_dst[256], uint16_t s_dst[256], const uint1
 }
 
 /* deblock.rs:1520-1567 */
-static void setup_thr_rows(uint16_t q_dst[256], uint16_t s_dst[256], const uint16_t (*mask)[5][4], int starty4,
-                           const uint32_t lut[2][16], const uint32_t (*above_lut)[16], int sb64x, int ss_hor,
-                           int w4, int h4) {
+static void setup_thr_rows(uint16_t q_dst[256], uint16_t s_dst[256], const uint8_t *seg, ptrdiff_t seg_off,
+                           ptrdiff_t seg_stride, const uint16_t (*mask)[5][4], int starty4,
+                           const uint32_t lut[2][16], const uint32_t (*above_lut)[16], ptrdiff_t above_off,
+                           int sb64x, int ss_hor, int w4, int h4) {
     int mask_idx = sb64x >> ss_hor;
     int mask_shift = (sb64x & ss_hor) ? 8 : 0;
     uint16_t aq[16] = {0}, as_[16] = {0};
     if (above_lut) {
         for (int x4 = 0; x4 < w4; x4++) {
-            aq[x4] = (uint16_t)above_lut[0][0];
-            as_[x4] = (uint16_t)above_lut[1][0];
+            int seg_id = seg_at(seg, above_off + x4);
+            aq[x4] = (uint16_t)above_lut[0][seg_id];
+            as_[x4] = (uint16_t)above_lut[1][seg_id];
         }
     }
     for (int x4 = 0; x4 < w4; x4++) {
         int prev_q = aq[x4], prev_s = as_[x4];
         for (int y4 = 0; y4 < h4; y4++) {
-            int cur_q = (int)lut[0][0], cur_s = (int)lut[1][0];
+            int seg_id = seg_at(seg, seg_off + x4 + y4 * seg_stride);
+            int cur_q = (int)lut[0][seg_id], cur_s = (int)lut[1][seg_id];
             int subpu = 3 * ((mask[starty4 + y4][4][mask_idx] >> (mask_shift + x4)) & 1);
             q_dst[x4 + y4 * 16] = (uint16_t)(edge_thr(cur_q, prev_q) >> subpu);
             s_dst[x4 + y4 * 16] = (uint16_t)(edge_thr(cur_s, prev_s) >> subpu);
@@ -291,7 +306,8 @@ static void deblock64_cols(const Rav2dFrameParams *p, const Geom &g, const Rav2d
             int bx4_base = (x64 & 3) * 16;
             int w4 = imin(g.bw - x64 * 16, 16);
             uint16_t q[256] = {0}, s[256] = {0};
-            setup_thr_cols(q, s, cl.filter_y[0], bx4_base, lut, left_q, left_s, y64 & 3, 0, w4, h4);
+            setup_thr_cols(q, s, g.seg, (ptrdiff_t)y64 * 16 * g.seg_stride + x64 * 16, g.seg_stride,
+                           cl.filter_y[0], bx4_base, lut, left_q, left_s, y64 & 3, 0, w4, h4);
             transpose_ll(ll, &cl.lossless_mask_y[starty4], x64 & 3, 0, 0);
             ptrdiff_t cur = y_off + (ptrdiff_t)x64 * 64;
             for (int x = 0; x < w4; x++) {
@@ -324,7 +340,9 @@ static void deblock64_cols(const Rav2dFrameParams *p, const Geom &g, const Rav2d
         int uv_w4 = imin(g.bw - x64 * 16, 16) >> g.ss_hor;
         uint16_t q[2][256] = {{0}}, s[2][256] = {{0}};
         for (int pl = 0; pl < 2; pl++)
-            setup_thr_cols(q[pl], s[pl], cl.filter_uv[0], bx4_base, lut[pl], left_q[pl], left_s[pl], y64 & 3,
+            setup_thr_cols(q[pl], s[pl], g.seg_uv,
+                           (ptrdiff_t)y64 * (16 >> g.ss_ver) * g.uv_seg_stride + x64 * (16 >> g.ss_hor),
+                           g.uv_seg_stride, cl.filter_uv[0], bx4_base, lut[pl], left_q[pl], left_s[pl], y64 & 3,
                            g.ss_ver, uv_w4, uv_h4);
         transpose_ll(ll, &cl.lossless_mask_uv[starty4 >> g.ss_ver], x64 & 3, g.ss_hor, g.ss_ver);
         ptrdiff_t cur = uv_off + (ptrdiff_t)x64 * (64 >> g.ss_hor);
@@ -397,7 +415,9 @@ static void deblock64_rows(const Rav2dFrameParams *p, const Geom &g, const Rav2d
             }
             int w4 = imin(g.bw - x64 * 16, 16);
             uint16_t q[256] = {0}, s[256] = {0};
-            setup_thr_rows(q, s, cl.filter_y[1], starty4, lut, above, x64 & 3, 0, w4, h4);
+            ptrdiff_t seg_off = (ptrdiff_t)y64 * 16 * g.seg_stride + x64 * 16;
+            setup_thr_rows(q, s, g.seg, seg_off, g.seg_stride, cl.filter_y[1], starty4, lut, above,
+                           seg_off - g.seg_stride, x64 & 3, 0, w4, h4);
             ptrdiff_t cur = y_off + (ptrdiff_t)x64 * 64;
             for (int y = 0; y < h4; y++) {
                 if (!have_top && y == 0) continue;
@@ -440,9 +460,11 @@ static void deblock64_rows(const Rav2dFrameParams *p, const Geom &g, const Rav2d
         }
         int uv_w4 = imin(g.bw - x64 * 16, 16) >> g.ss_hor;
         uint16_t q[2][256] = {{0}}, s[2][256] = {{0}};
+        ptrdiff_t uv_seg_off = (ptrdiff_t)y64 * (16 >> g.ss_ver) * g.uv_seg_stride + x64 * (16 >> g.ss_hor);
         for (int pl = 0; pl < 2; pl++)
-            setup_thr_rows(q[pl], s[pl], cl.filter_uv[1], starty4 >> g.ss_ver, lut[pl],
-                           above_present ? a_lut[pl] : nullptr, x64 & 3, g.ss_hor, uv_w4, uv_h4);
+            setup_thr_rows(q[pl], s[pl], g.seg_uv, uv_seg_off, g.uv_seg_stride, cl.filter_uv[1],
+                           starty4 >> g.ss_ver, lut[pl], above_present ? a_lut[pl] : nullptr,
+                           uv_seg_off - g.uv_seg_stride, x64 & 3, g.ss_hor, uv_w4, uv_h4);
         ptrdiff_t cur = uv_off + (ptrdiff_t)x64 * (64 >> g.ss_hor);
         int mask_idx = (x64 & 3) >> g.ss_hor;
         int mask_shift = ((x64 & 3) & g.ss_hor) ? 8 : 0;

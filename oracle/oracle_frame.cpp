@@ -20,10 +20,17 @@ using namespace oracle;
 template <typename P>
 static int run_frame(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask_in, size_t n_mask,
                      const Rav2dAv2Restoration *lr, size_t n_lr, P *y, P *u, P *v, ptrdiff_t ys, ptrdiff_t uvs,
-                     uint32_t stop_mask) {
+                     uint32_t stop_mask, const uint8_t *seg, ptrdiff_t seg_stride, const uint8_t *seg_uv,
+                     ptrdiff_t uv_seg_stride) {
     (void)lr;
     (void)n_lr;
     Geom g(p);
+    if (p->seg_enabled) {
+        g.seg = seg;
+        g.seg_uv = seg_uv;
+        g.seg_stride = seg_stride;
+        g.uv_seg_stride = uv_seg_stride;
+    }
     /* the bottom-edge crop mutates the mask: work on a copy (deblock.rs:1569-1606) */
     std::vector<Rav2dAv2Filter> mask(mask_in, mask_in + n_mask);
     Frame<P> f{y, u, v, ys, uvs};
@@ -38,6 +45,9 @@ static int run_frame(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask_in, s
     cp.damping = p->cdef_damping;
     cp.on_skip_tx = p->cdef_on_skiptx != 0;
     cp.cdef_on = (inloop & RAV2D_INLOOPFILTER_CDEF) != 0;
+    /* decode.rs:3868-3869 */
+    cp.any_lossless = false;
+    for (int i = 0; i < 8; i++) cp.any_lossless = cp.any_lossless || (p->seg_enabled && p->seg_lossless[i]);
     cp.ccso_on = (inloop & RAV2D_INLOOPFILTER_CCSO) && (p->ccso[0].enabled || p->ccso[1].enabled || p->ccso[2].enabled);
     cp.y_strength = p->cdef_y_strength;
     cp.uv_strength = p->cdef_uv_strength;
@@ -105,15 +115,25 @@ static int run_frame(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask_in, s
     return 0;
 }
 
+extern "C" int oracle_filter_frame_seg(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask, size_t n_mask,
+                                       const Rav2dAv2Restoration *lr, size_t n_lr, void *y, void *u, void *v,
+                                       ptrdiff_t y_stride, ptrdiff_t uv_stride, uint32_t inloop_mask,
+                                       const uint8_t *seg, ptrdiff_t seg_stride, const uint8_t *seg_uv,
+                                       ptrdiff_t uv_seg_stride) {
+    if (!p || !mask || p->layout != RAV2D_PIXEL_LAYOUT_I420) return RAV2D_EINVAL;
+    if (p->seg_enabled && (!seg || !seg_uv)) return RAV2D_EINVAL;
+    if (p->bitdepth == 8)
+        return run_frame<uint8_t>(p, mask, n_mask, lr, n_lr, (uint8_t *)y, (uint8_t *)u, (uint8_t *)v, y_stride,
+                                  uv_stride, inloop_mask, seg, seg_stride, seg_uv, uv_seg_stride);
+    if (p->bitdepth == 10)
+        return run_frame<uint16_t>(p, mask, n_mask, lr, n_lr, (uint16_t *)y, (uint16_t *)u, (uint16_t *)v,
+                                   y_stride / 2, uv_stride / 2, inloop_mask, seg, seg_stride, seg_uv, uv_seg_stride);
+    return RAV2D_EINVAL;
+}
+
 extern "C" int oracle_filter_frame(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask, size_t n_mask,
                                    const Rav2dAv2Restoration *lr, size_t n_lr, void *y, void *u, void *v,
                                    ptrdiff_t y_stride, ptrdiff_t uv_stride, uint32_t inloop_mask) {
-    if (!p || !mask || p->layout != RAV2D_PIXEL_LAYOUT_I420 || p->seg_enabled) return RAV2D_EINVAL;
-    if (p->bitdepth == 8)
-        return run_frame<uint8_t>(p, mask, n_mask, lr, n_lr, (uint8_t *)y, (uint8_t *)u, (uint8_t *)v, y_stride,
-                                  uv_stride, inloop_mask);
-    if (p->bitdepth == 10)
-        return run_frame<uint16_t>(p, mask, n_mask, lr, n_lr, (uint16_t *)y, (uint16_t *)u, (uint16_t *)v,
-                                   y_stride / 2, uv_stride / 2, inloop_mask);
-    return RAV2D_EINVAL;
+    return oracle_filter_frame_seg(p, mask, n_mask, lr, n_lr, y, u, v, y_stride, uv_stride, inloop_mask, nullptr, 0,
+                                   nullptr, 0);
 }

@@ -27,6 +27,8 @@ PASS_DEBLOCK_V, PASS_DEBLOCK_H, PASS_CDEF, PASS_LR, PASS_GRAIN = range(5)
 
 TOOL_DEBLOCK, TOOL_CDEF, TOOL_CCSO, TOOL_WIENER, TOOL_GDF, TOOL_GRAIN = 1, 2, 4, 8, 16, 32
 TOOL_ALL = 63
+# synthetic mask shapes: transform partitions (incl. H5/V5), sub-PU / TIP edges, segmentation
+MASK_TXPART, MASK_SUBPU, MASK_SEG = 1, 2, 4
 
 
 class FilmGrainData(C.Structure):
@@ -71,7 +73,7 @@ class FrameParams(C.Structure):
         ("layout", C.c_int32), ("sb128", C.c_int32), ("inloop_filters", C.c_uint32),
         ("seq_cdef", C.c_int32), ("cfl_ds_filter_index", C.c_int32),
         ("yac", C.c_int32), ("uac_delta", C.c_int32), ("vac_delta", C.c_int32),
-        ("seg_enabled", C.c_int32),
+        ("seg_enabled", C.c_int32), ("seg_delta_q", C.c_int16 * 8), ("seg_lossless", C.c_uint8 * 8),
         ("db_level_y", C.c_uint8 * 2), ("db_level_u", C.c_uint8), ("db_level_v", C.c_uint8),
         ("db_delta_q_y", C.c_int8 * 2), ("db_delta_q_u", C.c_int8), ("db_delta_q_v", C.c_int8),
         ("cdef_damping", C.c_uint8), ("cdef_n_strengths", C.c_uint8), ("cdef_on_skiptx", C.c_uint8),
@@ -133,7 +135,7 @@ class SynthConfig(C.Structure):
     _fields_ = [
         ("width", C.c_int32), ("height", C.c_int32), ("bitdepth", C.c_int32), ("sb128", C.c_int32),
         ("seed", C.c_uint64), ("tools", C.c_uint32), ("lossless_pct", C.c_int32),
-        ("skip_pct", C.c_int32), ("pc_gdf_pct", C.c_int32),
+        ("skip_pct", C.c_int32), ("pc_gdf_pct", C.c_int32), ("mask_flags", C.c_uint32),
     ]
 
 
@@ -174,6 +176,7 @@ def load_gpu():
     _declare(lib, "rav2d_gpu_frame_create", C.c_int, [P, C.POINTER(FrameParams), C.POINTER(P)])
     _declare(lib, "rav2d_gpu_frame_destroy", C.c_int, [P])
     _declare(lib, "rav2d_gpu_frame_set_side_info", C.c_int, [P, P, C.c_size_t, P, C.c_size_t, P])
+    _declare(lib, "rav2d_gpu_frame_set_segmap", C.c_int, [P, P, C.c_ssize_t, P, C.c_ssize_t, P])
     _declare(lib, "rav2d_gpu_frame_set_params", C.c_int, [P, C.POINTER(FrameParams)])
     _declare(lib, "rav2d_gpu_frame_planes", C.c_int, [P, C.c_int, C.POINTER(Planes)])
     _declare(lib, "rav2d_gpu_frame_upload", C.c_int, [P, P, P, P, P])
@@ -262,6 +265,7 @@ def load_synth():
     _declare(lib, "r2s_plane_bytes", C.c_size_t, [P, C.c_int])
     _declare(lib, "r2s_mask", P, [P, C.POINTER(C.c_size_t)])
     _declare(lib, "r2s_lr", P, [P, C.POINTER(C.c_size_t)])
+    _declare(lib, "r2s_segmap", P, [P, C.c_int, C.POINTER(C.c_ssize_t)])
     _synth = lib
     return lib
 
@@ -285,9 +289,10 @@ class SynthFrame:
     """A seeded synthetic reconstructed frame plus its filter side info."""
 
     def __init__(self, width, height, bitdepth=10, sb128=0, seed=1, tools=TOOL_ALL,
-                 lossless_pct=0, skip_pct=30, pc_gdf_pct=50):
+                 lossless_pct=0, skip_pct=30, pc_gdf_pct=50, mask_flags=0):
         lib = load_synth()
-        cfg = SynthConfig(width, height, bitdepth, sb128, seed, tools, lossless_pct, skip_pct, pc_gdf_pct)
+        cfg = SynthConfig(width, height, bitdepth, sb128, seed, tools, lossless_pct, skip_pct, pc_gdf_pct,
+                          mask_flags)
         self._lib = lib
         self._h = lib.r2s_frame_create(C.byref(cfg))
         if not self._h:
@@ -319,6 +324,13 @@ class SynthFrame:
             buf = (C.c_uint8 * n).from_address(self.plane_ptr(pl))
             out.append(np.frombuffer(buf, dtype=self.dtype).reshape(h, w).copy())
         return out
+
+    def segmaps(self):
+        """(luma map, stride, chroma map, stride) of a segmented frame, else Nones."""
+        st, uvst = C.c_ssize_t(), C.c_ssize_t()
+        y = self._lib.r2s_segmap(self._h, 0, C.byref(st))
+        uv = self._lib.r2s_segmap(self._h, 1, C.byref(uvst))
+        return y, st.value, uv, uvst.value
 
     def mask_bytes(self):
         return (C.c_uint8 * (self.n_mask * AV2FILTER_SIZE)).from_address(self.mask_ptr)
@@ -490,6 +502,8 @@ class GpuFrame:
     def from_synth(cls, ctx, sf):
         f = cls(ctx, sf.params)
         f.set_side_info(sf.mask_ptr, sf.n_mask, sf.lr_ptr, sf.n_lr)
+        if sf.params.seg_enabled:
+            f.set_segmap(*sf.segmaps())
         _check(f.lib.rav2d_gpu_frame_upload(f.h, sf.plane_ptr(0), sf.plane_ptr(1), sf.plane_ptr(2), None),
                "upload")
         return f
@@ -500,6 +514,11 @@ class GpuFrame:
 
     def set_params(self, params):
         _check(self.lib.rav2d_gpu_frame_set_params(self.h, C.byref(params)), "set_params")
+
+    def set_segmap(self, segmap, stride, segmap_uv, uv_stride):
+        """Segment maps (pointers or None, strides in bytes) of a segmented frame."""
+        _check(self.lib.rav2d_gpu_frame_set_segmap(self.h, segmap, stride, segmap_uv, uv_stride, None),
+               "set_segmap")
 
     def filter(self):
         _check(self.lib.rav2d_gpu_filter_frame(self.ctx.h, self.h, None), "filter_frame")

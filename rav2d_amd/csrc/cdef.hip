@@ -258,6 +258,7 @@ struct CdefSmem {
     alignas(4) int16_t off[2][12][2];    // CDEF_DIRECTIONS as offsets in the luma / chroma tile
     alignas(4) int8_t ccso_lut[3][256];  // CCSO offset per band/edge index, per plane
     uint32_t dcost[8][64]; // direction-search cost of each direction per block
+    uint16_t ll[24];       // lossless 4x4 units of the superblock: 16 luma rows, 8 chroma rows
 };
 
 // CCSO offset of one sample: `lt` = staged luma at the collocated position
@@ -317,6 +318,7 @@ struct CdefLoads {
     uint32_t lutw, offw;     // words of the CCSO offset tables and of the tap offsets
     int cdef_idx, ccso_pl;   // side info of the superblock
     uint32_t noskip_b;       // noskip row of the 8x8 block of lane t & 63 (block row (t >> 3) & 7)
+    uint32_t llw;            // lanes 0..23: one row of the superblock's lossless masks
 };
 
 // Every global load of a lane for superblock (sbx, sby), issued together.
@@ -333,6 +335,11 @@ __device__ __forceinline__ void cdef_load(const CdefArgs &a, int sbx, int sby, C
     L.cdef_idx = m.cdef_idx[((sby & 3) << 2) + (sbx & 3)];
     L.ccso_pl = (m.ccso[0] != 0) | ((m.ccso[1] != 0) << 1) | ((m.ccso[2] != 0) << 2);
     L.noskip_b = a.on_skiptx ? 0xffffu : m.noskip_mask[(((y0 >> 3) + ((t >> 3) & 7)) & 31)][sbx & 3];
+    // the lossless rows are read only when a segment is lossless (cdef.rs:894-911)
+    L.llw = 0;
+    if (a.any_ll && t < 24)
+        L.llw = t < 16 ? m.lossless_mask_y[((sby & 3) << 4) + t][sbx & 3]
+                       : m.lossless_mask_uv[((sby & 3) << 3) + t - 16][sbx & 3];
     constexpr uint32_t kOutside = 0x80008000u;  // INT16_MIN x 2
 #pragma unroll
     for (int j = 0; j < kPY; j++) {
@@ -370,7 +377,10 @@ __device__ __forceinline__ void cdef_store(const CdefLoads &L, CdefSmem &sm) {
     for (int j = 0; j < kPC; j++)
         if (t + j * kThreads < kNC) ((uint2 *)&sm.lc[0][0])[t + j * kThreads] = L.vc[j];
     if (t < 3 * 64) ((uint32_t *)&sm.ccso_lut[0][0])[t] = L.lutw;
-    if (t < 24) ((uint32_t *)&sm.off[0][0][0])[t] = L.offw;
+    if (t < 24) {
+        ((uint32_t *)&sm.off[0][0][0])[t] = L.offw;
+        sm.ll[t] = (uint16_t)L.llw;
+    }
 }
 
 // Direction search, filtering and CCSO of a superblock staged by cdef_store.
@@ -456,14 +466,19 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
                     }
                     var = (best - sm.dcost[d ^ 4][t]) >> 10;
                 }
+                // a lossless 4x4 in the block leaves that plane unfiltered
+                // (cdef.rs:922-926, 984-1025)
+                const int r = (t >> 3) * 2, sh = (t & 7) * 2;
+                const bool y_ll = ((sm.ll[r] | sm.ll[r + 1]) >> sh) & 3;
+                const bool uv_ll = (sm.ll[16 + (t >> 3)] >> (t & 7)) & 1;
                 if (y_pri) {
                     int adj = adjust_strength(y_pri, var);
-                    if (adj || y_sec) st |= 1;
+                    if ((adj || y_sec) && !y_ll) st |= 1;
                     ypri = (int16_t)adj;
-                } else if (y_sec) {
+                } else if (y_sec && !y_ll) {
                     st |= 1;
                 }
-                if (uv_lvl) st |= 2;
+                if (uv_lvl && !uv_ll) st |= 2;
                 dir = (uint8_t)d;
             }
         }
@@ -506,7 +521,7 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
                     v0 = (int)(P)(uint16_t)r.x;
                     v1 = (int)(P)(uint16_t)r.y;
                 }
-                if (cc0) {
+                if (cc0 && !((sm.ll[yy >> 2] >> (x >> 2)) & 1)) {  // ccso_add skips lossless 4x4 units
                     const int8_t *lut = sm.ccso_lut[0];
                     int o0, o1;
                     if (cc_inner) {
@@ -553,7 +568,7 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
                     v0 = (int)(P)(uint16_t)r.x;
                     v1 = (int)(P)(uint16_t)r.y;
                 }
-                if (cc) {
+                if (cc && !((sm.ll[16 + (yy >> 2)] >> (x >> 2)) & 1)) {
                     const CcsoDev &c = a.ccso[1 + pl];
                     const int8_t *lut = sm.ccso_lut[1 + pl];
                     int o0, o1;

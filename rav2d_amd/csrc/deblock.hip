@@ -103,6 +103,14 @@ __device__ __forceinline__ EdgeDesc edge_desc(const DeblockArgs &a, int pl, bool
     const int lnx4 = ch ? nx4 * 2 : nx4, lny4 = ch ? ny4 * 2 : ny4;
     e.qc = qidx_at(a.mask, sbw, lx4, ly4);
     e.qp = qidx_at(a.mask, sbw, lnx4, lny4);
+    if (a.seg[0]) {
+        // per-segment qidx of this unit and of its neighbour (setup_thr_cols/rows
+        // read the segment map at both, deblock.rs:1477-1567)
+        const uint8_t *sm = a.seg[ch ? 1 : 0];
+        const int st = a.seg_stride[ch ? 1 : 0];
+        e.qc = iclip(e.qc + a.seg_dq[sm[y4 * st + x4] & 7], 0, a.qmax);
+        e.qp = iclip(e.qp + a.seg_dq[sm[ny4 * st + nx4] & 7], 0, a.qmax);
+    }
     const Rav2dAv2Filter &fn = sb_at(a.mask, sbw, lnx4, lny4);
     if (ch) {
         e.pos_ll = (f.lossless_mask_uv[y4 & 31][(x4 >> 3) & 3] >> (x4 & 7)) & 1;

@@ -119,6 +119,10 @@ struct Rav2dGpuFrame {
     Rav2dAv2Restoration *d_lr = nullptr;
     size_t n_lr = 0;
     r2g::DbLut *d_dblut = nullptr;
+    // segment maps (rav2d_gpu_frame_set_segmap): luma ids bw x bh, then chroma
+    // ids (bw / 2) x (bh / 2), packed; allocated at the first set
+    uint8_t *d_seg = nullptr;
+    bool seg_ok = false;
     size_t bytes = 0;
     bool side_ok = false;
     // loop-restoration work list (rebuilt when params or side info change)

@@ -12,8 +12,8 @@ namespace r2g {
 
 // Per-frame deblock threshold LUT: [kind][qidx] -> (q_thr, side_thr); kinds
 // 0 = luma vertical edges, 1 = luma horizontal edges, 2 = U, 3 = V.
-// init_deblock_thr_lut_y/uv (crates/rav2d/src/deblock.rs:13-57) with
-// segmentation disabled, evaluated for every qidx a 64x64 block can carry.
+// init_deblock_thr_lut_y/uv (crates/rav2d/src/deblock.rs:13-57) evaluated for
+// every qidx a 64x64 block (or, segmented, a segment of it) can carry.
 struct DbLut {
     uint16_t q[4][512];
     uint16_t side[4][512];
@@ -26,6 +26,13 @@ struct DeblockArgs {
     const DbLut *lut;
     int32_t plane_on[3];  // plane filtered in this pass
     TileMap map;
+    // segmentation (null seg[0]: off): segment id per 4x4 unit of luma / chroma;
+    // a unit's LUT row is clip(qidx + seg_dq[id], 0, qmax), as the per-segment
+    // LUT of init_deblock_thr_lut_y/uv (deblock.rs:13-57) gives
+    const uint8_t *seg[2];
+    int32_t seg_stride[2];
+    int32_t qmax;
+    int16_t seg_dq[8];
 };
 
 // CCSO configuration of one plane (build_ccso_plane_cfg, decode.rs:4117-4133).
@@ -44,6 +51,7 @@ struct CdefArgs {
     const Rav2dAv2Filter *mask;
     const int32_t *order;  // superblock of each launch slot (null: raster order)
     int32_t cdef_on, ccso_on, on_skiptx, damping;
+    int32_t any_ll;  // a segment is lossless: CDEF and CCSO skip the masks' lossless 4x4 units
     alignas(8) uint8_t y_strength[8];
     alignas(8) uint8_t uv_strength[8];
     CcsoDev ccso[3];

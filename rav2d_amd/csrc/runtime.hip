@@ -531,10 +531,6 @@ int validate(const Rav2dFrameParams *p) {
         set_last_error("only 4:2:0 is supported");
         return RAV2D_EINVAL;
     }
-    if (p->seg_enabled) {
-        set_last_error("segmentation is not supported");
-        return RAV2D_EINVAL;
-    }
     if (p->sb128 != 0 && p->sb128 != 1) {
         set_last_error("sb128 must be 0 or 1");
         return RAV2D_EINVAL;
@@ -647,7 +643,21 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
             cur[pl] = out;
         }
         a.map = deblock_tile_map(a);
+        if (f->params.seg_enabled) {
+            const int bw = f->g.bw, bh = f->g.bh;
+            a.seg[0] = f->d_seg;
+            a.seg[1] = f->d_seg + (size_t)bw * bh;
+            a.seg_stride[0] = bw;
+            a.seg_stride[1] = bw >> 1;
+            a.qmax = 255 + 48 * f->g.hbd;
+            memcpy(a.seg_dq, f->params.seg_delta_q, sizeof(a.seg_dq));
+        }
     };
+    if (f->params.seg_enabled && !f->seg_ok && upto >= RAV2D_PASS_DEBLOCK_V &&
+        (plan.dbv[0] || plan.dbv[1] || plan.dbv[2] || plan.dbh[0] || plan.dbh[1] || plan.dbh[2])) {
+        set_last_error("segmentation is enabled but no segment map was set");
+        return RAV2D_EINVAL;
+    }
     if (upto >= RAV2D_PASS_DEBLOCK_V && (plan.dbv[0] || plan.dbv[1] || plan.dbv[2])) {
         deblock(cp.dbv, plan.dbv);
         cp.on[RAV2D_PASS_DEBLOCK_V] = true;
@@ -670,6 +680,8 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
                     (P.ccso[0].enabled || P.ccso[1].enabled || P.ccso[2].enabled);
         a.on_skiptx = P.cdef_on_skiptx != 0;
         a.damping = P.cdef_damping;
+        a.any_ll = 0;
+        for (int i = 0; i < 8; i++) a.any_ll |= P.seg_enabled && P.seg_lossless[i];
         memcpy(a.y_strength, P.cdef_y_strength, 8);
         memcpy(a.uv_strength, P.cdef_uv_strength, 8);
         static const int8_t pos[7][2] = {{-1, 0}, {0, -1}, {-1, -1}, {-1, 1}, {-1, -2}, {1, -2}, {0, 2}};  // ccso.rs:5-13
@@ -1025,6 +1037,7 @@ int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f) {
     if (f->d_lr) (void)hipFree(f->d_lr);
     if (f->d_lr_items) (void)hipFree(f->d_lr_items);
     if (f->d_cdef_order) (void)hipFree(f->d_cdef_order);
+    if (f->d_seg) (void)hipFree(f->d_seg);
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->in_use -= f->bytes;
@@ -1085,6 +1098,55 @@ int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, 
     }
     HIP_TRY(hipStreamSynchronize(s));  // host copies are pageable
     f->side_ok = true;
+    f->gen++;
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_frame_set_segmap(Rav2dGpuFrame *f, const uint8_t *segmap, ptrdiff_t stride,
+                               const uint8_t *segmap_uv, ptrdiff_t uv_stride, void *stream) {
+    if (!f) return RAV2D_EINVAL;
+    const int bw = f->g.bw, bh = f->g.bh, cw = bw >> 1, ch = bh >> 1;
+    if (!segmap) {
+        f->seg_ok = false;
+        return RAV2D_OK;
+    }
+    if (!segmap_uv || stride < bw || uv_stride < cw) {
+        set_last_error("segment maps need both planes and strides of at least the frame's 4x4 units");
+        return RAV2D_EINVAL;
+    }
+    // packed copy, ids checked: the kernels index an 8-entry table with them
+    std::vector<uint8_t> h((size_t)bw * bh + (size_t)cw * ch);
+    for (int y = 0; y < bh; y++) memcpy(&h[(size_t)y * bw], segmap + (size_t)y * stride, bw);
+    for (int y = 0; y < ch; y++) memcpy(&h[(size_t)bw * bh + (size_t)y * cw], segmap_uv + (size_t)y * uv_stride, cw);
+    for (uint8_t id : h)
+        if (id > 7) {
+            set_last_error("segment ids must be 0..7");
+            return RAV2D_EINVAL;
+        }
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    hipStream_t s = pick_stream(f->ctx, stream);
+    int r = frame_wait_idle(f);
+    if (r) return r;
+    if (!f->d_seg) {
+        {
+            std::lock_guard<std::mutex> lk(f->ctx->mu);
+            if (f->ctx->budget && f->ctx->in_use + h.size() > f->ctx->budget) {
+                set_last_error("HBM budget exceeded");
+                return RAV2D_EAGAIN;
+            }
+        }
+        if (injected_failure(RAV2D_FAIL_ALLOC) || hipMalloc((void **)&f->d_seg, h.size()) != hipSuccess) {
+            f->d_seg = nullptr;
+            set_last_error("segment map allocation failed");
+            return RAV2D_ENOMEM;
+        }
+        std::lock_guard<std::mutex> lk(f->ctx->mu);
+        f->ctx->in_use += h.size();
+        f->bytes += h.size();
+    }
+    HIP_TRY(hipMemcpyAsync(f->d_seg, h.data(), h.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));  // the host copy is pageable
+    f->seg_ok = true;
     f->gen++;
     return RAV2D_OK;
 }

@@ -35,6 +35,11 @@ struct Block {
     int bx, by, bs4, tx4;  // luma 4px units
     bool skip, lossless;
     int dc;                // DC offset (luma units of the coded depth)
+    // with R2SConfig.mask_flags: transform partition (TxPartition 0..7), its
+    // luma transform size, sub-PU level (subpu_flt_lvl; 3 = none), inter
+    int part = 0, tw4 = 0, th4 = 0, subpu = 3;
+    bool inter = false;
+    int seg = 0;  // segment id (R2S_MASK_SEG)
 };
 
 }  // namespace
@@ -46,6 +51,8 @@ struct R2SFrame {
     std::vector<uint8_t> planes[3];
     std::vector<Rav2dAv2Filter> mask;
     std::vector<Rav2dAv2Restoration> lr;
+    std::vector<uint8_t> seg[2];  // segment maps: luma / chroma 4x4 units
+    int seg_stride[2] = {0, 0};
 };
 
 namespace {
@@ -76,6 +83,116 @@ void block_edges(uint16_t (*dir0)[5][4], uint16_t (*dir1)[5][4], int bx4, int by
         for (int x = 0; x < w4; x++) dir1[by4 + y][lw][(bx4 + x) >> 4] |= (uint16_t)(1u << ((bx4 + x) & 15));
 }
 
+// Luma transform size (4px units) of TxPartition `part` (None, Split, H, V, H4,
+// V4, H5, V5) on a square block of s4 units: the square-block rows of
+// TX_PART_TBL (crates/rav2d/src/tables.rs:819-850); 8x8 blocks have None..V.
+void part_dims(int s4, int part, int &tw4, int &th4) {
+    static const int div_w[8] = {1, 2, 1, 2, 1, 4, 2, 4}, div_h[8] = {1, 2, 2, 1, 4, 1, 4, 2};
+    tw4 = std::max(1, s4 / div_w[part]);
+    th4 = std::max(1, s4 / div_h[part]);
+}
+
+// Rows (or columns) [from, from + n) of one edge line, as mask_inner_edges_v/h
+// (lf_mask.rs:88-154) OR them into the four 16-bit words of a level.
+void or_span(uint16_t *m, int from, int n) {
+    for (int i = 0; i < n; i++) m[(from + i) >> 4] |= (uint16_t)(1u << ((from + i) & 15));
+}
+
+enum { kPartH5 = 6, kPartV5 = 7 };
+
+// mask_edges_part (lf_mask.rs:156-254): outer and inner transform edges of one
+// block; a / l are the above / left level contexts at the block.
+void edges_part(uint16_t (*dir0)[5][4], uint16_t (*dir1)[5][4], int by4, int bx4, int w4, int h4, int part,
+                int tw4, int th4, int hlim, int vlim, uint8_t *a, uint8_t *l) {
+    const int twl4c = std::min(hlim, ilog2(tw4)), thl4c = std::min(vlim, ilog2(th4));
+    if (part < kPartH5) {
+        outer_edge(dir0[bx4], by4, h4, twl4c, l);
+        outer_edge(dir1[by4], bx4, w4, thl4c, a);
+        for (int x = tw4; x < w4; x += tw4) or_span(dir0[bx4 + x][twl4c], by4, h4);
+        for (int y = th4; y < h4; y += th4) or_span(dir1[by4 + y][thl4c], bx4, w4);
+    } else if (part == kPartH5) {
+        // quarter-height rows split in two, the middle half one transform twice as wide
+        outer_edge(dir1[by4], bx4, w4, thl4c, a);
+        outer_edge(dir0[bx4], by4, std::min(th4, h4), twl4c, l);
+        if (h4 > th4) {
+            outer_edge(dir0[bx4], by4 + th4, std::min(2 * th4, h4 - th4), std::min(twl4c + 1, hlim), l + th4);
+            if (h4 > th4 * 3) outer_edge(dir0[bx4], by4 + th4 * 3, std::min(th4, h4 - 3 * th4), twl4c, l + th4 * 3);
+        }
+        for (int y = th4; y < h4; y += th4 * 2) or_span(dir1[by4 + y][thl4c], bx4, w4);
+        for (int x = tw4; x < w4; x += tw4) {
+            or_span(dir0[bx4 + x][twl4c], by4, std::min(th4, h4));
+            if (h4 > th4 * 3) or_span(dir0[bx4 + x][twl4c], by4 + th4 * 3, h4 - th4 * 3);
+        }
+    } else {
+        outer_edge(dir0[bx4], by4, h4, twl4c, l);
+        outer_edge(dir1[by4], bx4, std::min(tw4, w4), thl4c, a);
+        if (w4 > tw4) {
+            outer_edge(dir1[by4], bx4 + tw4, std::min(2 * tw4, w4 - tw4), std::min(thl4c + 1, vlim), a + tw4);
+            if (w4 > tw4 * 3) outer_edge(dir1[by4], bx4 + tw4 * 3, std::min(tw4, w4 - 3 * tw4), thl4c, a + tw4 * 3);
+        }
+        for (int x = tw4; x < w4; x += tw4 * 2) or_span(dir0[bx4 + x][twl4c], by4, h4);
+        for (int y = th4; y < h4; y += th4) {
+            or_span(dir1[by4 + y][thl4c], bx4, std::min(tw4, w4));
+            if (w4 > tw4 * 3) or_span(dir1[by4 + y][thl4c], bx4 + tw4 * 3, w4 - tw4 * 3);
+        }
+    }
+}
+
+// mask_subpu_edges (lf_mask.rs:256-330): edges every hsz / vsz units inside
+// the block at level twl4c / thl4c; bits new at that level also get the
+// level-4 (sub-PU threshold) bit where x (y) & ds_mask is non-zero.
+void subpu_edges(uint16_t (*dir0)[5][4], uint16_t (*dir1)[5][4], int by4, int bx4, int w4, int h4, int twl4c,
+                 int thl4c, int hsz, int vsz, int ds_mask) {
+    auto put = [](uint16_t(*line)[4], int lvl, int from, int n, bool mark) {
+        uint16_t add[4] = {0, 0, 0, 0};
+        or_span(add, from, n);
+        for (int e = 0; e < 4; e++) {
+            if (!add[e]) continue;
+            const uint16_t had = line[lvl][e];
+            line[lvl][e] |= add[e];
+            if (mark) line[4][e] |= (uint16_t)(add[e] & ~had);
+        }
+    };
+    if (hsz)
+        for (int x = hsz; x < w4; x += hsz) put(dir0[bx4 + x], twl4c, by4, h4, (x & ds_mask) != 0);
+    if (vsz)
+        for (int y = vsz; y < h4; y += vsz) put(dir1[by4 + y], thl4c, bx4, w4, (y & ds_mask) != 0);
+}
+
+// create_db_mask (lf_mask.rs:399-502) for one plane of a block; bx4/by4/w4/h4
+// are in the plane's 4px units within the SB256, `ss` the plane's subsampling.
+void db_mask(uint16_t (*dir0)[5][4], uint16_t (*dir1)[5][4], const Block &b, int ss, int bx4, int by4, int w4,
+             int h4, int ds_mask, uint8_t *a, uint8_t *l) {
+    const int chroma = ss;
+    const int s = b.subpu, bl2 = ilog2(b.bs4);
+    int twl4c, thl4c;
+    if (!b.inter || !b.skip) {
+        int part = 0, tw4, th4;
+        if (b.lossless) {
+            tw4 = th4 = 1;
+        } else if (chroma) {
+            tw4 = th4 = std::max(1, std::min(b.bs4 >> 1, 8));  // largest chroma transform of the block
+        } else {
+            part = b.part;
+            tw4 = b.tw4;
+            th4 = b.th4;
+        }
+        const int lim = std::min(std::max(s - ss, 0), 3 - chroma);
+        edges_part(dir0, dir1, by4, bx4, w4, h4, part, tw4, th4, lim, lim, a, l);
+        twl4c = std::min(s, ilog2(tw4));
+        thl4c = std::min(s, ilog2(th4));
+    } else {
+        const int lim = b.lossless ? 0 : std::min(std::max(std::min(s, bl2) - ss, 0), 3 - chroma);
+        outer_edge(dir0[bx4], by4, h4, lim, l);
+        outer_edge(dir1[by4], bx4, w4, lim, a);
+        twl4c = thl4c = s;
+    }
+    if (s != 3) {
+        const int hl = twl4c - (ss && twl4c), vl = thl4c - (ss && thl4c);
+        subpu_edges(dir0, dir1, by4, bx4, w4, h4, hl, vl, (1 << s) >> ss, (1 << s) >> ss, ds_mask);
+    }
+}
+
 void partition(Rng &r, int x, int y, int s4, int bw, int bh, std::vector<Block> &out, const R2SConfig &c) {
     if (x >= bw || y >= bh) return;
     bool split = s4 > 16 || (s4 > 2 && r.pct(s4 == 16 ? 60 : s4 == 8 ? 50 : 40));
@@ -95,6 +212,19 @@ void partition(Rng &r, int x, int y, int s4, int bw, int bh, std::vector<Block> 
     b.tx4 = b.lossless ? 1 : (1 << r.range(0, ilog2(s4)));
     b.skip = r.pct(c.skip_pct);
     b.dc = r.range(-24, 24);
+    b.tw4 = b.th4 = b.tx4;
+    if (c.mask_flags & R2S_MASK_SUBPU) {
+        b.inter = r.pct(60);
+        if (b.inter && !b.lossless && r.pct(70)) b.subpu = r.range(0, 2);
+    }
+    if ((c.mask_flags & R2S_MASK_TXPART) && !b.lossless) {
+        b.part = r.range(0, s4 == 2 ? 3 : 7);
+        part_dims(s4, b.part, b.tw4, b.th4);
+    } else if (c.mask_flags) {
+        b.part = 0;  // the block's own transform (lossless: 4x4)
+        b.tw4 = b.th4 = b.lossless ? 1 : s4;
+    }
+    if (c.mask_flags & R2S_MASK_SEG) b.seg = b.lossless ? 7 : r.range(0, 6);
     out.push_back(b);
 }
 
@@ -108,14 +238,13 @@ void fill_plane(Rng &r, P *pl, int w, int h, int ss, const std::vector<Block> &b
     int scale = 1 << (bd - 8);
     for (const Block &b : blocks) {
         // DC offset per transform block
-        int t = (b.tx4 * 4) >> ss, bs = (b.bs4 * 4) >> ss;
-        if (t < 4) t = 4;
+        int tw = std::max(4, (b.tw4 * 4) >> ss), th = std::max(4, (b.th4 * 4) >> ss), bs = (b.bs4 * 4) >> ss;
         int x0 = (b.bx * 4) >> ss, y0 = (b.by * 4) >> ss;
-        for (int ty = 0; ty < bs; ty += t)
-            for (int tx = 0; tx < bs; tx += t) {
+        for (int ty = 0; ty < bs; ty += th)
+            for (int tx = 0; tx < bs; tx += tw) {
                 int off = (b.dc + r.range(-6, 6)) * scale;
-                for (int yy = y0 + ty; yy < std::min(h, y0 + ty + t); yy++)
-                    for (int xx = x0 + tx; xx < std::min(w, x0 + tx + t); xx++) dc[(size_t)yy * w + xx] = off;
+                for (int yy = y0 + ty; yy < std::min(h, y0 + ty + th); yy++)
+                    for (int xx = x0 + tx; xx < std::min(w, x0 + tx + tw); xx++) dc[(size_t)yy * w + xx] = off;
             }
     }
     for (int y = 0; y < h; y++)
@@ -271,19 +400,27 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
     for (int y = 0; y < bh; y += sb4)
         for (int x = 0; x < bw; x += sb4) partition(r, x, y, sb4, bw, bh, blocks, *cfg);
 
-    // edge masks, noskip and lossless masks
+    // edge masks, noskip and lossless masks; a TIP frame of frame_mode 2 sets
+    // no sub-PU threshold bits (create_db_mask's ds_subpu_mask = 0)
+    const int ds_mask = (cfg->mask_flags & R2S_MASK_SUBPU) && r.pct(25) ? 0 : 15;
     std::vector<uint8_t> actx_y(bw + 64, 3), lctx_y(bh + 64, 3), actx_uv(bw + 64, 2), lctx_uv(bh + 64, 2);
     for (const Block &b : blocks) {
         int sbx = b.bx >> 6, sby = b.by >> 6;
         Rav2dAv2Filter &m = f->mask[(size_t)sby * sb256w + sbx];
         int bx4 = b.bx & 63, by4 = b.by & 63;
         int w4 = std::min(bw - b.bx, b.bs4), h4 = std::min(bh - b.by, b.bs4);
-        block_edges(m.filter_y[0], m.filter_y[1], bx4, by4, w4, h4, b.tx4, 3, &actx_y[b.bx], &lctx_y[b.by]);
-        // chroma 4:2:0: chroma block is half size; transform = chroma block, at most 32
-        int ctx4 = b.lossless ? 1 : std::min(b.bs4 >> 1, 8);
-        if (ctx4 < 1) ctx4 = 1;
-        block_edges(m.filter_uv[0], m.filter_uv[1], bx4 >> 1, by4 >> 1, w4 >> 1, h4 >> 1, ctx4, 2,
+        if (cfg->mask_flags) {
+            db_mask(m.filter_y[0], m.filter_y[1], b, 0, bx4, by4, w4, h4, ds_mask, &actx_y[b.bx], &lctx_y[b.by]);
+            db_mask(m.filter_uv[0], m.filter_uv[1], b, 1, bx4 >> 1, by4 >> 1, w4 >> 1, h4 >> 1, ds_mask,
                     &actx_uv[b.bx >> 1], &lctx_uv[b.by >> 1]);
+        } else {
+            block_edges(m.filter_y[0], m.filter_y[1], bx4, by4, w4, h4, b.tx4, 3, &actx_y[b.bx], &lctx_y[b.by]);
+            // chroma 4:2:0: chroma block is half size; transform = chroma block, at most 32
+            int ctx4 = b.lossless ? 1 : std::min(b.bs4 >> 1, 8);
+            if (ctx4 < 1) ctx4 = 1;
+            block_edges(m.filter_uv[0], m.filter_uv[1], bx4 >> 1, by4 >> 1, w4 >> 1, h4 >> 1, ctx4, 2,
+                        &actx_uv[b.bx >> 1], &lctx_uv[b.by >> 1]);
+        }
         for (int y = 0; y < h4; y++)
             for (int x = 0; x < w4; x++) {
                 int yy = by4 + y, xx = bx4 + x;
@@ -299,6 +436,28 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
                     int yy = (by4 >> 1) + y, xx = (bx4 >> 1) + x;  // chroma 4px units in SB256
                     m.lossless_mask_uv[yy][xx >> 3] |= (uint16_t)(1u << (xx & 7));
                 }
+    }
+
+    // segmentation: per-segment qidx deltas, segment 7 lossless (its qidx is 0,
+    // as obu.rs:1270-1280 derives segmentation.lossless) when it is used
+    if (cfg->mask_flags & R2S_MASK_SEG) {
+        Rav2dFrameParams &sp = f->params;
+        sp.seg_enabled = 1;
+        for (int i = 0; i < 7; i++) sp.seg_delta_q[i] = (int16_t)r.range(-48, 48);
+        sp.seg_delta_q[7] = (int16_t)-sp.yac;
+        sp.seg_lossless[7] = cfg->lossless_pct > 0;
+        f->seg_stride[0] = bw;
+        f->seg_stride[1] = bw >> 1;
+        f->seg[0].assign((size_t)bw * bh, 0);
+        f->seg[1].assign((size_t)(bw >> 1) * (bh >> 1), 0);
+        for (const Block &b : blocks) {
+            int w4 = std::min(bw - b.bx, b.bs4), h4 = std::min(bh - b.by, b.bs4);
+            for (int y = 0; y < h4; y++)
+                for (int x = 0; x < w4; x++) f->seg[0][(size_t)(b.by + y) * bw + b.bx + x] = (uint8_t)b.seg;
+            for (int y = 0; y < (h4 >> 1); y++)
+                for (int x = 0; x < (w4 >> 1); x++)
+                    f->seg[1][(size_t)((b.by >> 1) + y) * (bw >> 1) + (b.bx >> 1) + x] = (uint8_t)b.seg;
+        }
     }
 
     // per-64x64 / per-SB256 side info
@@ -348,6 +507,16 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
     return f;
 }
 
+void r2s_mask_edges_part(uint16_t (*masks)[64][5][4], int by4, int bx4, int w4, int h4, int part, int tw4, int th4,
+                         int hlim, int vlim, uint8_t *a, uint8_t *l) {
+    edges_part(masks[0], masks[1], by4, bx4, w4, h4, part, tw4, th4, hlim, vlim, a, l);
+}
+
+void r2s_mask_subpu_edges(uint16_t (*masks)[64][5][4], int by4, int bx4, int w4, int h4, int twl4c, int thl4c,
+                          int hsz, int vsz, int ds_mask) {
+    subpu_edges(masks[0], masks[1], by4, bx4, w4, h4, twl4c, thl4c, hsz, vsz, ds_mask);
+}
+
 void r2s_frame_destroy(R2SFrame *f) { delete f; }
 Rav2dFrameParams *r2s_params(R2SFrame *f) { return &f->params; }
 void *r2s_plane(R2SFrame *f, int pl) { return f->planes[pl].data(); }
@@ -358,6 +527,15 @@ Rav2dAv2Filter *r2s_mask(R2SFrame *f, size_t *n) {
     if (n) *n = f->mask.size();
     return f->mask.data();
 }
+uint8_t *r2s_segmap(R2SFrame *f, int which, ptrdiff_t *stride) {
+    if (which < 0 || which > 1 || f->seg[which].empty()) {
+        if (stride) *stride = 0;
+        return nullptr;
+    }
+    if (stride) *stride = f->seg_stride[which];
+    return f->seg[which].data();
+}
+
 Rav2dAv2Restoration *r2s_lr(R2SFrame *f, size_t *n) {
     if (n) *n = f->lr.size();
     return f->lr.data();

@@ -22,6 +22,10 @@ def load():
         lib.oracle_filter_frame.restype = C.c_int
         lib.oracle_filter_frame.argtypes = [C.POINTER(R.FrameParams), P, C.c_size_t, P, C.c_size_t,
                                             P, P, P, C.c_ssize_t, C.c_ssize_t, C.c_uint32]
+        lib.oracle_filter_frame_seg.restype = C.c_int
+        lib.oracle_filter_frame_seg.argtypes = [C.POINTER(R.FrameParams), P, C.c_size_t, P, C.c_size_t,
+                                                P, P, P, C.c_ssize_t, C.c_ssize_t, C.c_uint32,
+                                                P, C.c_ssize_t, P, C.c_ssize_t]
         lib.oracle_apply_grain.restype = C.c_int
         lib.oracle_apply_grain.argtypes = [C.POINTER(R.FrameParams), P, P, P, P, P, P, C.c_ssize_t, C.c_ssize_t]
         lib.oracle_picture_has_grain.restype = C.c_int
@@ -77,9 +81,11 @@ def filter_frame(sf, inloop_mask=R.INLOOP_ALL, params=None):
     planes = sf.planes()
     bps = planes[0].itemsize
     p = params if params is not None else sf.params
-    rc = lib.oracle_filter_frame(C.byref(p), sf.mask_ptr, sf.n_mask, sf.lr_ptr, sf.n_lr,
-                                 planes[0].ctypes.data, planes[1].ctypes.data, planes[2].ctypes.data,
-                                 planes[0].shape[1] * bps, planes[1].shape[1] * bps, inloop_mask)
+    seg, seg_stride, seg_uv, uv_seg_stride = sf.segmaps()
+    rc = lib.oracle_filter_frame_seg(C.byref(p), sf.mask_ptr, sf.n_mask, sf.lr_ptr, sf.n_lr,
+                                     planes[0].ctypes.data, planes[1].ctypes.data, planes[2].ctypes.data,
+                                     planes[0].shape[1] * bps, planes[1].shape[1] * bps, inloop_mask,
+                                     seg, seg_stride, seg_uv, uv_seg_stride)
     if rc != 0:
         raise RuntimeError("oracle_filter_frame failed: %d" % rc)
     return planes

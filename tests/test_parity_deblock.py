@@ -18,6 +18,15 @@ CASES = [
     (72, 40, 10, 0, 7, 0),   # smaller than one superblock, ragged edges
 ]
 
+# masks beyond square intra transforms: (w, h, bitdepth, sb128, seed, lossless_pct, mask_flags)
+MASK_CASES = [
+    (352, 288, 8, 0, 11, 0, R.MASK_TXPART),
+    (352, 288, 10, 0, 12, 5, R.MASK_SUBPU),
+    (640, 360, 10, 1, 13, 0, R.MASK_TXPART | R.MASK_SUBPU),
+    (264, 136, 8, 1, 15, 10, R.MASK_TXPART | R.MASK_SUBPU),
+    (1920, 1080, 10, 0, 14, 3, R.MASK_TXPART | R.MASK_SUBPU),
+]
+
 
 @pytest.fixture(scope="module")
 def ctx():
@@ -38,8 +47,20 @@ def _compare(ref, got, tag):
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "%dx%d_%db_sb%d_s%d_ll%d" % (c[0], c[1], c[2], 64 << c[3], c[4], c[5]))
 def test_deblock_cols_then_rows(ctx, case):
-    w, h, bd, sb128, seed, ll = case
-    sf = R.SynthFrame(w, h, bd, sb128, seed, tools=R.TOOL_DEBLOCK, lossless_pct=ll)
+    _cols_then_rows(ctx, *case)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", MASK_CASES,
+                         ids=lambda c: "%dx%d_%db_sb%d_s%d_ll%d_m%d" % (c[0], c[1], c[2], 64 << c[3], c[4], c[5], c[6]))
+def test_deblock_partitions_and_subpu(ctx, case):
+    """Transform partitions (H5/V5 included), sub-PU edges with their >> 3
+    thresholds and TIP-frame masks, as create_db_mask produces them."""
+    _cols_then_rows(ctx, *case)
+
+
+def _cols_then_rows(ctx, w, h, bd, sb128, seed, ll, flags=0):
+    sf = R.SynthFrame(w, h, bd, sb128, seed, tools=R.TOOL_DEBLOCK, lossless_pct=ll, mask_flags=flags)
     f = R.GpuFrame.from_synth(ctx, sf)
     try:
         # vertical edges only
