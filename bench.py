@@ -137,8 +137,9 @@ def cpu_baseline(w, h, bd, sb128, tools, seed, gpu_digests, batch_spec, batch_di
                 t = time.perf_counter()
                 o, d = chain(src)
                 el = time.perf_counter() - t
-                ok = digest(o) == batch_digests[i][0] and (d is None or digest(d) == batch_digests[i][1])
-                return ok, el
+                ok_o = digest(o) == batch_digests[i][0]
+                ok_d = d is None or digest(d) == batch_digests[i][1]
+                return ok_o and ok_d, el, ("" if ok_o else "output") + ("" if ok_d else " display")
             finally:
                 src.close()
 
@@ -147,7 +148,8 @@ def cpu_baseline(w, h, bd, sb128, tools, seed, gpu_digests, batch_spec, batch_di
             rs = list(ex.map(one, range(len(batch_spec))))
         dtm = time.perf_counter() - t1
         pix = sum(c[0] * c[1] for c in batch_spec)
-        exact["batch_frames"] = {"frames": len(rs), "equal": sum(1 for r in rs if r[0])}
+        exact["batch_frames"] = {"frames": len(rs), "equal": sum(1 for r in rs if r[0]),
+                                 "differing": {i: r[2].strip() for i, r in enumerate(rs) if not r[0]}}
         res["multi_core"] = {"value": round(pix / 1e6 / dtm, 3), "unit": "Mpix/s", "cores": cores,
                              "cores_in_affinity": affinity,
                              "sample": "the %d mixed batch frames (%.1f Mpix), one frame per thread, in %.1f s "

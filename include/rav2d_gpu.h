@@ -81,6 +81,30 @@ typedef struct Rav2dAv2Filter {
     uint16_t lossless_mask_uv[64][4];
 } Rav2dAv2Filter;
 
+/* Device form of one Av2Filter record for 4:2:0 frames (8004 B against the
+ * record's 12100 B): the same fields with the chroma masks cut to the SB256's
+ * 32 x 32 chroma 4x4 units (filter_uv lines 0..31, words 0..1; lossless_mask_uv
+ * rows 0..31), which is every unit a 4:2:0 producer writes
+ * (create_db_mask, lf_mask.rs:399-502: chroma by4/bx4 are luma units >> 1).
+ * The library keeps and uploads side info in this form; a decoder may write it
+ * directly from its mask producers and hand it to
+ * rav2d_gpu_frame_set_side_info_packed, or convert Av2Filter records with
+ * rav2d_gpu_pack_filter.  The Av2Restoration records stay on the host (the
+ * library builds its loop-restoration work list from them). */
+typedef struct Rav2dPackedFilter {
+    uint16_t filter_y[2][64][5][4];
+    uint16_t filter_uv[2][32][5][2];
+    uint16_t lossless_mask_y[64][4];
+    uint16_t lossless_mask_uv[32][4];
+    uint16_t noskip_mask[32][4];
+    uint16_t lr_noskip_mask[64][4];
+    uint16_t qidx[16];
+    int8_t cdef_idx[16];
+    uint8_t ccso[3];
+    uint8_t _pad0;
+    uint8_t gdf[16];
+} Rav2dPackedFilter;
+
 /* Av2RestorationUnit / Av2Restoration, crates/rav2d/src/lf_mask.rs:1-5, 38-41 */
 typedef struct Rav2dRestorationUnit {
     uint8_t restoration_type;
@@ -234,10 +258,18 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params,
 int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f);
 /* Upload side info: n_mask = sb256 rows * sb256 cols Av2Filter records,
  * n_lr = the same number of Av2Restoration records (lr may be NULL when no
- * plane uses Wiener restoration). */
+ * plane uses Wiener restoration).  The records are packed on the host
+ * (rav2d_gpu_pack_filter) and only the packed form goes to the device. */
 int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask,
                                   size_t n_mask, const Rav2dAv2Restoration *lr,
                                   size_t n_lr, void *stream);
+/* The same from records already in the device form (no repacking). */
+int rav2d_gpu_frame_set_side_info_packed(Rav2dGpuFrame *f, const Rav2dPackedFilter *mask,
+                                         size_t n_mask, const Rav2dAv2Restoration *lr,
+                                         size_t n_lr, void *stream);
+/* Av2Filter records of a 4:2:0 frame -> their device form (host only, any
+ * thread).  Chroma mask entries outside the 4:2:0 extent are dropped. */
+int rav2d_gpu_pack_filter(const Rav2dAv2Filter *in, size_t n, Rav2dPackedFilter *out);
 /* Segment maps of a segmented frame: FilterFrameParams' cur_segmap and
  * segmap_uv (decode.rs:285-310, 2355-2367), as deblock64_cols/rows read them
  * (deblock.rs:1623-2140).  segmap: one segment id (0..7) per luma 4x4 unit,
@@ -259,9 +291,27 @@ int rav2d_gpu_frame_set_segmap(Rav2dGpuFrame *f, const uint8_t *segmap, ptrdiff_
 int rav2d_gpu_frame_set_params(Rav2dGpuFrame *f, const Rav2dFrameParams *params);
 /* Device planes of the frame: which = 0 input F0, 1 output, 2 display. */
 int rav2d_gpu_frame_planes(const Rav2dGpuFrame *f, int which, Rav2dPlanes *out);
-/* Host <-> device copies of tightly packed planes (stride = width samples). */
+/* Host <-> device copies of tightly packed planes (stride = width samples).
+ * The upload writes the frame's visible samples only; the picture padding
+ * (below) keeps what it holds: zero after rav2d_gpu_frame_create. */
 int rav2d_gpu_frame_upload(Rav2dGpuFrame *f, const void *y, const void *u,
                            const void *v, void *stream);
+/* Picture padding.  Like the reference's picture buffers (DefaultPicAllocator,
+ * picture.rs:77-125: heights and widths aligned to 128), every plane of a
+ * frame or picture of this library holds RAV2D_PICTURE_PAD rows below the
+ * frame and, when the width is not a multiple of 128, RAV2D_PICTURE_PAD
+ * samples right of it.  Deblocking reads them where a transform overhangs the
+ * frame edge (filter_choice_bd looks up to 8 samples across an edge,
+ * deblock.rs:81-186; the bottom-edge crop, :1572-1606, covers only luma
+ * 32-long edges): a decoder's reconstruction of the overhanging blocks lives
+ * there.  Planes handed to rav2d_gpu_filter_planes must carry the same
+ * padding (rav2d_gpu_picture_alloc's do).  This upload copies the padding
+ * too: src holds the frame plus pad_rows (<= 8) rows below and pad_cols
+ * (<= 8) samples right of each plane, at src->stride bytes per row; columns
+ * beyond the frame's own padding are skipped. */
+#define RAV2D_PICTURE_PAD 8
+int rav2d_gpu_frame_upload_padded(Rav2dGpuFrame *f, const Rav2dPlanes *src, int pad_rows, int pad_cols,
+                                  void *stream);
 int rav2d_gpu_frame_download(Rav2dGpuFrame *f, int which, void *y, void *u, void *v,
                              void *stream);
 
@@ -588,7 +638,8 @@ int rav2d_gpu_fguv_32x32xn(Rav2dGpuCtx *ctx, void *dst, const void *src, ptrdiff
 /* sizeof() of the ABI structs, so bindings can check their mirrors:
  * 0 Rav2dAv2Filter, 1 Rav2dAv2Restoration, 2 Rav2dFrameParams,
  * 3 Rav2dFilmGrainData, 4 Rav2dPlanes, 5 Rav2dBatchConfig, 6 Rav2dServeConfig,
- * 7 Rav2dServeResult, 8 Rav2dWienerParams; 0 for an unknown index. */
+ * 7 Rav2dServeResult, 8 Rav2dWienerParams, 9 Rav2dPackedFilter; 0 for an
+ * unknown index. */
 size_t rav2d_gpu_struct_size(int which);
 
 const char *rav2d_gpu_version(void);

@@ -141,6 +141,8 @@ class SynthConfig(C.Structure):
 
 AV2FILTER_SIZE = 12100
 AV2RESTORATION_SIZE = 24624
+PACKED_FILTER_SIZE = 8004  # Rav2dPackedFilter, the device form of an Av2Filter record
+PICTURE_PAD = 8  # RAV2D_PICTURE_PAD: rows / columns of picture padding the deblocking reads
 
 _gpu = None
 _synth = None
@@ -177,9 +179,12 @@ def load_gpu():
     _declare(lib, "rav2d_gpu_frame_destroy", C.c_int, [P])
     _declare(lib, "rav2d_gpu_frame_set_side_info", C.c_int, [P, P, C.c_size_t, P, C.c_size_t, P])
     _declare(lib, "rav2d_gpu_frame_set_segmap", C.c_int, [P, P, C.c_ssize_t, P, C.c_ssize_t, P])
+    _declare(lib, "rav2d_gpu_frame_set_side_info_packed", C.c_int, [P, P, C.c_size_t, P, C.c_size_t, P])
+    _declare(lib, "rav2d_gpu_pack_filter", C.c_int, [P, C.c_size_t, P])
     _declare(lib, "rav2d_gpu_frame_set_params", C.c_int, [P, C.POINTER(FrameParams)])
     _declare(lib, "rav2d_gpu_frame_planes", C.c_int, [P, C.c_int, C.POINTER(Planes)])
     _declare(lib, "rav2d_gpu_frame_upload", C.c_int, [P, P, P, P, P])
+    _declare(lib, "rav2d_gpu_frame_upload_padded", C.c_int, [P, C.POINTER(Planes), C.c_int, C.c_int, P])
     _declare(lib, "rav2d_gpu_frame_download", C.c_int, [P, C.c_int, P, P, P, P])
     _declare(lib, "rav2d_gpu_filter_frame", C.c_int, [P, P, P])
     _declare(lib, "rav2d_gpu_apply_grain", C.c_int, [P, P, P])
@@ -266,6 +271,7 @@ def load_synth():
     _declare(lib, "r2s_mask", P, [P, C.POINTER(C.c_size_t)])
     _declare(lib, "r2s_lr", P, [P, C.POINTER(C.c_size_t)])
     _declare(lib, "r2s_segmap", P, [P, C.c_int, C.POINTER(C.c_ssize_t)])
+    _declare(lib, "r2s_plane_padded", P, [P, C.c_int, C.POINTER(C.c_ssize_t)])
     _synth = lib
     return lib
 
@@ -283,6 +289,14 @@ def _check(rc, what):
     if rc != RAV2D_OK:
         msg = load_gpu().rav2d_gpu_last_error()
         raise Rav2dError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def pack_filter(mask_ptr, n):
+    """Av2Filter records at mask_ptr -> their device form (rav2d_gpu_pack_filter),
+    as an (n, PACKED_FILTER_SIZE) uint8 array."""
+    out = np.zeros((n, PACKED_FILTER_SIZE), np.uint8)
+    _check(load_gpu().rav2d_gpu_pack_filter(mask_ptr, n, out.ctypes.data), "pack_filter")
+    return out
 
 
 class SynthFrame:
@@ -331,6 +345,19 @@ class SynthFrame:
         y = self._lib.r2s_segmap(self._h, 0, C.byref(st))
         uv = self._lib.r2s_segmap(self._h, 1, C.byref(uvst))
         return y, st.value, uv, uvst.value
+
+    def padded_planes(self):
+        """Copies of the planes with 8 rows below / 8 columns right of picture
+        padding (a decoder's reconstruction of the blocks overhanging the frame)."""
+        out = []
+        for pl in range(3):
+            h, w = self.plane_shape(pl)
+            st = C.c_ssize_t()
+            ptr = self._lib.r2s_plane_padded(self._h, pl, C.byref(st))
+            isz = np.dtype(self.dtype).itemsize
+            buf = (C.c_uint8 * (st.value * (h + PICTURE_PAD))).from_address(ptr)
+            out.append(np.frombuffer(buf, dtype=self.dtype).reshape(h + PICTURE_PAD, st.value // isz).copy())
+        return out
 
     def mask_bytes(self):
         return (C.c_uint8 * (self.n_mask * AV2FILTER_SIZE)).from_address(self.mask_ptr)
@@ -447,12 +474,16 @@ class Picture:
         return (self.height, self.width) if pl == 0 else ((self.height + 1) // 2, (self.width + 1) // 2)
 
     def write(self, planes):
-        """Upload three numpy planes (rows of any stride are packed here)."""
+        """Upload three numpy planes (rows of any stride are packed here); planes
+        with picture padding (up to PICTURE_PAD more rows and columns) fill the
+        picture's padding too."""
         for pl, a in enumerate(planes):
             h, w = self._shape(pl)
             s = self.planes.stride[1 if pl else 0]
-            buf = np.zeros((h, s // np.dtype(self.dtype).itemsize), dtype=self.dtype)
-            buf[:, :w] = a
+            cols = s // np.dtype(self.dtype).itemsize
+            buf = np.zeros((h + PICTURE_PAD, cols), dtype=self.dtype)
+            ah, aw = min(a.shape[0], h + PICTURE_PAD), min(a.shape[1], cols)
+            buf[:ah, :aw] = a[:ah, :aw]
             _check(self.lib.rav2d_gpu_copy(self.ctx.h, self.planes.data[pl], buf.ctypes.data, buf.nbytes, None),
                    "copy")
 
@@ -504,9 +535,18 @@ class GpuFrame:
         f.set_side_info(sf.mask_ptr, sf.n_mask, sf.lr_ptr, sf.n_lr)
         if sf.params.seg_enabled:
             f.set_segmap(*sf.segmaps())
-        _check(f.lib.rav2d_gpu_frame_upload(f.h, sf.plane_ptr(0), sf.plane_ptr(1), sf.plane_ptr(2), None),
-               "upload")
+        f.upload_padded(sf)
         return f
+
+    def upload_padded(self, sf):
+        """The SynthFrame's planes with their picture padding (8 rows and columns)."""
+        src = Planes()
+        for pl in range(3):
+            st = C.c_ssize_t()
+            src.data[pl] = sf._lib.r2s_plane_padded(sf._h, pl, C.byref(st))
+            src.stride[1 if pl else 0] = st.value
+        _check(self.lib.rav2d_gpu_frame_upload_padded(self.h, C.byref(src), PICTURE_PAD, PICTURE_PAD, None),
+               "upload_padded")
 
     def set_side_info(self, mask_ptr, n_mask, lr_ptr, n_lr):
         _check(self.lib.rav2d_gpu_frame_set_side_info(self.h, mask_ptr, n_mask, lr_ptr, n_lr, None),

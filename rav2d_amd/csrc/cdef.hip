@@ -331,7 +331,7 @@ __device__ __forceinline__ void cdef_load(const CdefArgs &a, int sbx, int sby, C
     const int x0 = sbx * 64, y0 = sby * 64;
     const int fw = a.g.bw * 4, fh = a.g.bh * 4;
     const int cw = fw >> 1, ch = fh >> 1;
-    const Rav2dAv2Filter &m = a.mask[(sby >> 2) * a.g.sb256w + (sbx >> 2)];
+    const Rav2dPackedFilter &m = a.mask[(sby >> 2) * a.g.sb256w + (sbx >> 2)];
     L.cdef_idx = m.cdef_idx[((sby & 3) << 2) + (sbx & 3)];
     L.ccso_pl = (m.ccso[0] != 0) | ((m.ccso[1] != 0) << 1) | ((m.ccso[2] != 0) << 2);
     L.noskip_b = a.on_skiptx ? 0xffffu : m.noskip_mask[(((y0 >> 3) + ((t >> 3) & 7)) & 31)][sbx & 3];

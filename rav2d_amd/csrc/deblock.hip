@@ -63,13 +63,14 @@ __device__ __forceinline__ int edge_thr(int cur, int prev) {
     return (cur && prev) ? (cur + prev + 1) >> 1 : (cur | prev);
 }
 
-__device__ __forceinline__ const Rav2dAv2Filter &sb_at(const Rav2dAv2Filter *m, int sb256w, int lx4, int ly4) {
+__device__ __forceinline__ const Rav2dPackedFilter &sb_at(const Rav2dPackedFilter *m, int sb256w, int lx4,
+                                                          int ly4) {
     return m[(ly4 >> 6) * sb256w + (lx4 >> 6)];
 }
 
 // qidx of the 64x64 luma block holding luma 4px unit (lx4, ly4).
-__device__ __forceinline__ int qidx_at(const Rav2dAv2Filter *m, int sb256w, int lx4, int ly4) {
-    const Rav2dAv2Filter &f = sb_at(m, sb256w, lx4, ly4);
+__device__ __forceinline__ int qidx_at(const Rav2dPackedFilter *m, int sb256w, int lx4, int ly4) {
+    const Rav2dPackedFilter &f = sb_at(m, sb256w, lx4, ly4);
     return dmin((int)f.qidx[((ly4 >> 4) & 3) * 4 + ((lx4 >> 4) & 3)], 511);
 }
 
@@ -90,15 +91,17 @@ __device__ __forceinline__ EdgeDesc edge_desc(const DeblockArgs &a, int pl, bool
     const bool ch = pl != 0;
     // luma 4px units of this sample (4:2:0 chroma)
     const int lx4 = ch ? x4 * 2 : x4, ly4 = ch ? y4 * 2 : y4;
-    const Rav2dAv2Filter &f = sb_at(a.mask, sbw, lx4, ly4);
-    const uint16_t(*fm)[5][4] = ch ? f.filter_uv[vert ? 0 : 1] : f.filter_y[vert ? 0 : 1];
+    const Rav2dPackedFilter &f = sb_at(a.mask, sbw, lx4, ly4);
     const int along = vert ? x4 : y4;   // index of the edge line within the SB256
     const int across = vert ? y4 : x4;  // position along the edge
     const int line = ch ? (along & 31) : (along & 63);
     const int sidx = ch ? ((across >> 4) & 1) : ((across >> 4) & 3);
     const uint32_t bit = 1u << (across & 15);
-    const uint32_t m0 = fm[line][0][sidx], m1 = fm[line][1][sidx], m2 = fm[line][2][sidx];
-    const uint32_t m3 = ch ? 0u : (uint32_t)fm[line][3][sidx], m4 = fm[line][4][sidx];
+    // level words 0..4 of the edge line (chroma lines hold two words per level)
+    const uint16_t *lw = ch ? &f.filter_uv[vert ? 0 : 1][line][0][sidx] : &f.filter_y[vert ? 0 : 1][line][0][sidx];
+    const int ls = ch ? 2 : 4;
+    const uint32_t m0 = lw[0], m1 = lw[ls], m2 = lw[2 * ls];
+    const uint32_t m3 = ch ? 0u : (uint32_t)lw[3 * ls], m4 = lw[4 * ls];
     const int nx4 = vert ? x4 - 1 : x4, ny4 = vert ? y4 : y4 - 1;
     const int lnx4 = ch ? nx4 * 2 : nx4, lny4 = ch ? ny4 * 2 : ny4;
     e.qc = qidx_at(a.mask, sbw, lx4, ly4);
@@ -111,7 +114,7 @@ __device__ __forceinline__ EdgeDesc edge_desc(const DeblockArgs &a, int pl, bool
         e.qc = iclip(e.qc + a.seg_dq[sm[y4 * st + x4] & 7], 0, a.qmax);
         e.qp = iclip(e.qp + a.seg_dq[sm[ny4 * st + nx4] & 7], 0, a.qmax);
     }
-    const Rav2dAv2Filter &fn = sb_at(a.mask, sbw, lnx4, lny4);
+    const Rav2dPackedFilter &fn = sb_at(a.mask, sbw, lnx4, lny4);
     if (ch) {
         e.pos_ll = (f.lossless_mask_uv[y4 & 31][(x4 >> 3) & 3] >> (x4 & 7)) & 1;
         e.neg_ll = (fn.lossless_mask_uv[ny4 & 31][(nx4 >> 3) & 3] >> (nx4 & 7)) & 1;
@@ -144,19 +147,23 @@ __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id
     const DevPlane &dst = a.dst[pl];
     const int x0 = tx * kTW, y0 = ty * kTH;
     const int W = src.w, H = src.h;
+    // the picture padding right of and below the plane is read too (edges of
+    // transforms that overhang the frame look into it) and the rows below are
+    // carried into the output for the horizontal-edge pass
+    const int WP = W + a.pad_cols, HP = H + a.pad_rows;
     const int t = threadIdx.x;
     constexpr int LW = kTW + 2 * kHalo;
-    // 4-sample groups (x0 - kHalo and plane widths are multiples of 4, so a
-    // group is wholly inside or outside the plane), loaded from clamped,
-    // always valid addresses, then selected
+    // 4-sample groups (x0 - kHalo, plane widths and padding are multiples of
+    // 4, so a group is wholly inside or outside), loaded from clamped, always
+    // valid addresses, then selected
     constexpr int QW = LW / 4, PV = kTH * QW / kThreads;
     uint2 v[PV];
 #pragma unroll
     for (int j = 0; j < PV; j++) {
         const int i = t + j * kThreads;
         const int r = i / QW, gx = x0 - kHalo + (i - r * QW) * 4, gy = y0 + r;
-        const uint2 l = load4(row_ptr<const P>(src, dmin(gy, H - 1)) + iclip(gx, 0, W - 4));
-        v[j] = (gy < H && gx >= 0 && gx < W) ? l : make_uint2(0, 0);
+        const uint2 l = load4(row_ptr<const P>(src, dmin(gy, HP - 1)) + iclip(gx, 0, WP - 4));
+        v[j] = (gy < HP && gx >= 0 && gx < WP) ? l : make_uint2(0, 0);
     }
     constexpr int NE = kTW / 4 + 3;  // edge columns x0-4 .. x0+TW+4
     static_assert(NE * (kTH / 4) <= kThreads, "one lane per edge segment");
@@ -184,7 +191,7 @@ __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id
         const int i = t + j * kThreads;
         const int r = i / (kTW / 4), c = (i - r * (kTW / 4)) * 4;
         const int gx = x0 + c, gy = y0 + r;
-        if (gy < H && gx < W) store4(row_ptr<P>(dst, gy) + gx, *(const uint2 *)&tile[r][c + kHalo]);
+        if (gy < HP && gx < W) store4(row_ptr<P>(dst, gy) + gx, *(const uint2 *)&tile[r][c + kHalo]);
     }
 }
 
@@ -197,6 +204,7 @@ __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id
     const DevPlane &dst = a.dst[pl];
     const int x0 = tx * kTW, y0 = ty * kTH;
     const int W = src.w, H = src.h;
+    const int HP = H + a.pad_rows;  // the padding rows below the plane are read too
     const int t = threadIdx.x;
     constexpr int LH = kTH + 2 * kHalo;
     constexpr int QW = kTW / 4, PV = LH * QW / kThreads;
@@ -205,8 +213,8 @@ __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id
     for (int j = 0; j < PV; j++) {
         const int i = t + j * kThreads;
         const int r = i / QW, gx = x0 + (i - r * QW) * 4, gy = y0 - kHalo + r;
-        const uint2 l = load4(row_ptr<const P>(src, iclip(gy, 0, H - 1)) + dmin(gx, W - 4));
-        v[j] = (gy >= 0 && gy < H && gx < W) ? l : make_uint2(0, 0);
+        const uint2 l = load4(row_ptr<const P>(src, iclip(gy, 0, HP - 1)) + dmin(gx, W - 4));
+        v[j] = (gy >= 0 && gy < HP && gx < W) ? l : make_uint2(0, 0);
     }
     constexpr int NE = kTH / 4 + 3;  // edge rows y0-4 .. y0+TH+4
     static_assert(NE * (kTW / 4) <= kThreads, "one lane per edge segment");
@@ -297,12 +305,12 @@ void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched
 #undef R2G_DB_LAUNCH
 }
 
-TileMap deblock_tile_map(const DeblockArgs &a) {
+TileMap deblock_tile_map(const DeblockArgs &a, bool vert) {
     TileMap m;
     m.first[0] = 0;
     for (int p = 0; p < 3; p++) {
         m.tiles_x[p] = (a.src[p].w + kTW - 1) / kTW;
-        m.tiles_y[p] = (a.src[p].h + kTH - 1) / kTH;
+        m.tiles_y[p] = (a.src[p].h + (vert ? a.pad_rows : 0) + kTH - 1) / kTH;
         m.first[p + 1] = m.first[p] + m.tiles_x[p] * m.tiles_y[p];
     }
     return m;
@@ -310,7 +318,7 @@ TileMap deblock_tile_map(const DeblockArgs &a) {
 
 void launch_deblock_v(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
     DeblockArgs a = a0;
-    a.map = deblock_tile_map(a);
+    a.map = deblock_tile_map(a, true);
     if (a.map.first[3] == 0) return;
     if (bitdepth > 8)
         R2G_LAUNCH(deblock_v_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
@@ -320,7 +328,7 @@ void launch_deblock_v(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
 
 void launch_deblock_h(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
     DeblockArgs a = a0;
-    a.map = deblock_tile_map(a);
+    a.map = deblock_tile_map(a, false);
     if (a.map.first[3] == 0) return;
     if (bitdepth > 8)
         R2G_LAUNCH(deblock_h_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);

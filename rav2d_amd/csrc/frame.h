@@ -114,9 +114,8 @@ struct Rav2dGpuFrame {
     // caller-owned planes standing in for buf[b] during one call
     // (rav2d_gpu_filter_planes); null = the frame's own
     uint8_t *ext[NBUF][3] = {};
-    Rav2dAv2Filter *d_mask = nullptr;
+    Rav2dPackedFilter *d_mask = nullptr;
     size_t n_mask = 0;
-    Rav2dAv2Restoration *d_lr = nullptr;
     size_t n_lr = 0;
     r2g::DbLut *d_dblut = nullptr;
     // segment maps (rav2d_gpu_frame_set_segmap): luma ids bw x bh, then chroma
@@ -126,7 +125,7 @@ struct Rav2dGpuFrame {
     size_t bytes = 0;
     bool side_ok = false;
     // loop-restoration work list (rebuilt when params or side info change)
-    std::vector<Rav2dAv2Filter> h_mask;  // host copy (after the bottom crop)
+    std::vector<Rav2dPackedFilter> h_mask;  // host copy (after the bottom crop)
     std::vector<Rav2dAv2Restoration> h_lr;
     std::vector<r2g::LrItem> lr_items;
     r2g::LrItem *d_lr_items = nullptr;
@@ -160,6 +159,10 @@ struct ChainPlan {
 
 int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp);
 DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl);
+// Samples of picture padding right of each plane of a width-w picture: blocks
+// can overhang the right edge only when w is not a multiple of 128 (the
+// largest superblock).
+inline int pad_columns(int w) { return (w & 127) ? RAV2D_PICTURE_PAD : 0; }
 bool frame_has_grain(const Rav2dFrameParams &P);
 int grain_args(Rav2dGpuFrame *f, GrainArgs &a);
 bool data_has_grain(const Rav2dFilmGrainData &d);

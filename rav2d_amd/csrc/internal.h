@@ -22,10 +22,14 @@ struct DbLut {
 struct DeblockArgs {
     DevPlane src[3], dst[3];
     DevGeom g;
-    const Rav2dAv2Filter *mask;
+    const Rav2dPackedFilter *mask;
     const DbLut *lut;
     int32_t plane_on[3];  // plane filtered in this pass
     TileMap map;
+    // picture padding read below / right of every plane (rows, samples); the
+    // vertical-edge pass also carries the rows below into its output, where
+    // the horizontal-edge pass reads them
+    int32_t pad_rows, pad_cols;
     // segmentation (null seg[0]: off): segment id per 4x4 unit of luma / chroma;
     // a unit's LUT row is clip(qidx + seg_dq[id], 0, qmax), as the per-segment
     // LUT of init_deblock_thr_lut_y/uv (deblock.rs:13-57) gives
@@ -48,7 +52,7 @@ struct CcsoDev {
 struct CdefArgs {
     DevPlane src[3], dst[3];
     DevGeom g;
-    const Rav2dAv2Filter *mask;
+    const Rav2dPackedFilter *mask;
     const int32_t *order;  // superblock of each launch slot (null: raster order)
     int32_t cdef_on, ccso_on, on_skiptx, damping;
     int32_t any_ll;  // a segment is lossless: CDEF and CCSO skip the masks' lossless 4x4 units
@@ -146,7 +150,8 @@ void launch_deblock_v(const DeblockArgs &a, int bitdepth, hipStream_t s);
 void launch_deblock_h(const DeblockArgs &a, int bitdepth, hipStream_t s);
 void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, int bitdepth,
                           hipStream_t st);
-TileMap deblock_tile_map(const DeblockArgs &a);  // 64x32 tiles of the three planes
+// 64x32 tiles of the three planes (the vertical-edge pass: down to the padding rows)
+TileMap deblock_tile_map(const DeblockArgs &a, bool vert);
 
 void set_last_error(const std::string &msg);
 // rav2d_gpu_debug_inject_failure: true when the next operation of kind `what`

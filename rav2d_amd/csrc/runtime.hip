@@ -71,7 +71,7 @@ void build_db_lut(const Rav2dFrameParams &p, int hbd, DbLut &lut) {
 
 // Bottom-of-frame crop of 32-long transform edges (deblock.rs:1572-1606),
 // applied once to the library's copy for every superblock row.
-void crop_bottom(std::vector<Rav2dAv2Filter> &mask, const DevGeom &g) {
+void crop_bottom(std::vector<Rav2dPackedFilter> &mask, const DevGeom &g) {
     int sbh = (g.bh + (16 << g.sb128) - 1) >> (4 + g.sb128);
     for (int sby = 0; sby < sbh; sby++) {
         size_t mask_row = (size_t)((sby >> (2 - g.sb128)) * g.sb256w);
@@ -180,7 +180,7 @@ void lr_column_items(const Rav2dGpuFrame *f, int pl, int x, int y, int w, int ro
         int y4 = ((y << ss_ver) & 255) >> 2;
         int ll_rows = std::max(stripe_h >> 2, 1) + 1;
         if (sbi < f->h_mask.size()) {
-            const Rav2dAv2Filter &m = f->h_mask[sbi];
+            const Rav2dPackedFilter &m = f->h_mask[sbi];
             if (pl == 0 || ss_hor == 0) {
                 const uint16_t(*src)[4] = pl == 0 ? m.lossless_mask_y : m.lossless_mask_uv;
                 for (int r = 0; r < ll_rows && r < 17; r++)
@@ -189,7 +189,8 @@ void lr_column_items(const Rav2dGpuFrame *f, int pl, int x, int y, int w, int ro
                 int init_y = y >> 2, end_y = (y + stripe_h) >> 2, base = y4 >> ss_ver;
                 for (int yy = init_y, r = 0; yy < end_y; yy++, r++) {
                     int sy = base + (yy - init_y);
-                    if (sy < 64 && r < ll_rows && r < 17) {
+                    // rows past the 4:2:0 extent (32) hold no chroma unit: zero
+                    if (sy < 32 && r < ll_rows && r < 17) {
                         uint16_t lo = m.lossless_mask_uv[sy][sb64x_idx];
                         uint16_t hi = sb64x_idx + 1 < 4 ? m.lossless_mask_uv[sy][sb64x_idx + 1] : 0;
                         it.ll[r] = (uint16_t)((lo & 0xff) | ((hi & 0xff) << 8));
@@ -342,7 +343,7 @@ int upload_cdef_order(Rav2dGpuFrame *f) {
         for (int sbx = 0; sbx < sbw; sbx++) {
             const size_t mi = (size_t)(sby >> 2) * g.sb256w + (sbx >> 2);
             if (mi >= f->h_mask.size()) continue;
-            const Rav2dAv2Filter &m = f->h_mask[mi];
+            const Rav2dPackedFilter &m = f->h_mask[mi];
             const int idx = m.cdef_idx[((sby & 3) << 2) + (sbx & 3)];
             const bool active = cdef_on && idx >= 0 && (P.cdef_y_strength[idx & 7] || P.cdef_uv_strength[idx & 7]);
             int c = active ? 8 : 0;
@@ -561,19 +562,22 @@ int copy_plane(Rav2dGpuFrame *f, int from, int to, int pl, hipStream_t s) {
     return RAV2D_OK;
 }
 
-// Plane layout of the library's pictures: rows padded to 256 B (+256 B when
-// the stride is a multiple of 1 KiB, keeping rows off one channel), 8 spare
-// rows per plane.
+// Plane layout of the library's pictures: RAV2D_PICTURE_PAD rows below each
+// plane and, for widths that are not a multiple of 128, RAV2D_PICTURE_PAD
+// samples right of it (the picture padding the deblocking reads), rows padded
+// to 256 B (+256 B when the stride is a multiple of 1 KiB, keeping rows off
+// one channel).
 void picture_layout(int w, int h, int bps, int64_t stride[3], size_t bytes[3], int pw[3], int ph[3]) {
     pw[0] = w;
     ph[0] = h;
     pw[1] = pw[2] = (w + 1) >> 1;
     ph[1] = ph[2] = (h + 1) >> 1;
+    const int pad_cols = pad_columns(w);
     for (int pl = 0; pl < 3; pl++) {
-        int64_t s = ((int64_t)pw[pl] * bps + 255) & ~(int64_t)255;
+        int64_t s = ((int64_t)(pw[pl] + pad_cols) * bps + 255) & ~(int64_t)255;
         if ((s & 1023) == 0) s += 256;
         stride[pl] = s;
-        bytes[pl] = (size_t)s * (ph[pl] + 8);
+        bytes[pl] = (size_t)s * (ph[pl] + RAV2D_PICTURE_PAD);
     }
 }
 
@@ -631,7 +635,7 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
     Plan plan = make_plan(f->params);
     int cur[3] = {BUF_IN, BUF_IN, BUF_IN};
     auto next_tmp = [&](int c) { return c == BUF_T1 ? BUF_T2 : BUF_T1; };
-    auto deblock = [&](DeblockArgs &a, const bool *on) {
+    auto deblock = [&](DeblockArgs &a, const bool *on, bool vert) {
         a.g = f->g;
         a.mask = f->d_mask;
         a.lut = f->d_dblut;
@@ -642,7 +646,9 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
             a.dst[pl] = plane_of(f, out, pl);
             cur[pl] = out;
         }
-        a.map = deblock_tile_map(a);
+        a.pad_rows = RAV2D_PICTURE_PAD;
+        a.pad_cols = pad_columns(f->pw[0]);
+        a.map = deblock_tile_map(a, vert);
         if (f->params.seg_enabled) {
             const int bw = f->g.bw, bh = f->g.bh;
             a.seg[0] = f->d_seg;
@@ -659,11 +665,11 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
         return RAV2D_EINVAL;
     }
     if (upto >= RAV2D_PASS_DEBLOCK_V && (plan.dbv[0] || plan.dbv[1] || plan.dbv[2])) {
-        deblock(cp.dbv, plan.dbv);
+        deblock(cp.dbv, plan.dbv, true);
         cp.on[RAV2D_PASS_DEBLOCK_V] = true;
     }
     if (upto >= RAV2D_PASS_DEBLOCK_H && (plan.dbh[0] || plan.dbh[1] || plan.dbh[2])) {
-        deblock(cp.dbh, plan.dbh);
+        deblock(cp.dbh, plan.dbh, false);
         cp.on[RAV2D_PASS_DEBLOCK_H] = true;
     }
     int f2buf[3] = {cur[0], cur[1], cur[2]};  // deblocked planes, read by LR at stripe edges
@@ -792,6 +798,7 @@ extern "C" {
 size_t rav2d_gpu_struct_size(int which) {
     switch (which) {
         case 0: return sizeof(Rav2dAv2Filter);
+        case 9: return sizeof(Rav2dPackedFilter);
         case 1: return sizeof(Rav2dAv2Restoration);
         case 2: return sizeof(Rav2dFrameParams);
         case 3: return sizeof(Rav2dFilmGrainData);
@@ -961,7 +968,7 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
     for (int pl = 0; pl < 3; pl++)
         lr_cap += (size_t)((f->pw[pl] + 63) / 64) * sbrows * 3 +
                   (size_t)((f->pw[pl] + 255) / 256) * ((f->ph[pl] + 63) / 64);
-    const size_t side_bytes = n_sb * (sizeof(Rav2dAv2Filter) + sizeof(Rav2dAv2Restoration));
+    const size_t side_bytes = n_sb * sizeof(Rav2dPackedFilter);
     const size_t n_cdef = (size_t)((params->width + 63) / 64) * ((params->height + 63) / 64);
     size_t need = total * NBUF + sizeof(DbLut) + grain_bytes + side_bytes + lr_cap * sizeof(LrItem) +
                   n_cdef * sizeof(int32_t);
@@ -978,11 +985,10 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
     if (injected_failure(RAV2D_FAIL_ALLOC) || hipMalloc((void **)&f->block, total * NBUF) != hipSuccess ||
         hipMalloc((void **)&f->d_dblut, sizeof(DbLut)) != hipSuccess ||
         hipMalloc((void **)&f->d_grain, grain_bytes) != hipSuccess ||
-        hipMalloc((void **)&f->d_mask, n_sb * sizeof(Rav2dAv2Filter)) != hipSuccess ||
-        hipMalloc((void **)&f->d_lr, n_sb * sizeof(Rav2dAv2Restoration)) != hipSuccess ||
+        hipMalloc((void **)&f->d_mask, n_sb * sizeof(Rav2dPackedFilter)) != hipSuccess ||
         hipMalloc((void **)&f->d_lr_items, lr_cap * sizeof(LrItem)) != hipSuccess ||
         hipMalloc((void **)&f->d_cdef_order, n_cdef * sizeof(int32_t)) != hipSuccess) {
-        for (void *p : {(void *)f->block, (void *)f->d_dblut, (void *)f->d_grain, (void *)f->d_mask, (void *)f->d_lr,
+        for (void *p : {(void *)f->block, (void *)f->d_dblut, (void *)f->d_grain, (void *)f->d_mask,
                         (void *)f->d_lr_items, (void *)f->d_cdef_order})
             if (p) (void)hipFree(p);
         {
@@ -1014,6 +1020,12 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
             f->buf[b][pl] = p;
             p += f->plane_bytes[pl];
         }
+    // the input planes (first in the block) start at zero, picture padding included
+    if (hipMemset(f->buf[BUF_IN][0], 0, total) != hipSuccess) {
+        set_last_error("hipMemset failed");
+        rav2d_gpu_frame_destroy(f);
+        return RAV2D_EDEVICE;
+    }
     DbLut lut;
     build_db_lut(*params, g.hbd, lut);
     HIP_TRY(hipMemcpy(f->d_dblut, &lut, sizeof(lut), hipMemcpyHostToDevice));
@@ -1034,7 +1046,6 @@ int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f) {
     if (f->ev_done) (void)hipEventDestroy(f->ev_done);
     if (f->d_grain) (void)hipFree(f->d_grain);
     if (f->d_mask) (void)hipFree(f->d_mask);
-    if (f->d_lr) (void)hipFree(f->d_lr);
     if (f->d_lr_items) (void)hipFree(f->d_lr_items);
     if (f->d_cdef_order) (void)hipFree(f->d_cdef_order);
     if (f->d_seg) (void)hipFree(f->d_seg);
@@ -1072,11 +1083,38 @@ int rav2d_gpu_frame_set_params(Rav2dGpuFrame *f, const Rav2dFrameParams *params)
     return RAV2D_OK;
 }
 
-int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, size_t n_mask,
-                                  const Rav2dAv2Restoration *lr, size_t n_lr, void *stream) {
-    if (!f || !mask) return RAV2D_EINVAL;
-    size_t need = (size_t)f->g.sb256w * f->g.sb256h;
-    if (n_mask != need || (lr && n_lr != need)) {
+int rav2d_gpu_pack_filter(const Rav2dAv2Filter *in, size_t n, Rav2dPackedFilter *out) {
+    if (n && (!in || !out)) return RAV2D_EINVAL;
+    for (size_t i = 0; i < n; i++) {
+        const Rav2dAv2Filter &a = in[i];
+        Rav2dPackedFilter &b = out[i];
+        memcpy(b.filter_y, a.filter_y, sizeof(b.filter_y));
+        for (int d = 0; d < 2; d++)
+            for (int l = 0; l < 32; l++)
+                for (int k = 0; k < 5; k++)
+                    for (int s = 0; s < 2; s++) b.filter_uv[d][l][k][s] = a.filter_uv[d][l][k][s];
+        memcpy(b.lossless_mask_y, a.lossless_mask_y, sizeof(b.lossless_mask_y));
+        memcpy(b.lossless_mask_uv, a.lossless_mask_uv, sizeof(b.lossless_mask_uv));  // rows 0..31
+        memcpy(b.noskip_mask, a.noskip_mask, sizeof(b.noskip_mask));
+        memcpy(b.lr_noskip_mask, a.lr_noskip_mask, sizeof(b.lr_noskip_mask));
+        memcpy(b.qidx, a.qidx, sizeof(b.qidx));
+        memcpy(b.cdef_idx, a.cdef_idx, sizeof(b.cdef_idx));
+        memcpy(b.ccso, a.ccso, sizeof(b.ccso));
+        b._pad0 = 0;
+        memcpy(b.gdf, a.gdf, sizeof(b.gdf));
+    }
+    return RAV2D_OK;
+}
+
+namespace {
+
+// The frame keeps `mask` (packed, bottom crop applied to this copy) on the
+// host for its loop-restoration planner and CDEF order, and on the device for
+// the kernels; the restoration records stay on the host.
+int set_side_info_common(Rav2dGpuFrame *f, std::vector<Rav2dPackedFilter> &mask, const Rav2dAv2Restoration *lr,
+                         size_t n_lr, void *stream) {
+    const size_t need = (size_t)f->g.sb256w * f->g.sb256h;
+    if (mask.size() != need || (lr && n_lr != need)) {
         set_last_error("side info count must equal the SB256 grid size");
         return RAV2D_EINVAL;
     }
@@ -1084,22 +1122,43 @@ int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, 
     hipStream_t s = pick_stream(f->ctx, stream);
     int r = frame_wait_idle(f);
     if (r) return r;
-    std::vector<Rav2dAv2Filter> copy(mask, mask + n_mask);
-    crop_bottom(copy, f->g);
-    HIP_TRY(hipMemcpyAsync(f->d_mask, copy.data(), need * sizeof(Rav2dAv2Filter), hipMemcpyHostToDevice, s));
-    f->h_mask.swap(copy);
+    crop_bottom(mask, f->g);
+    HIP_TRY(hipMemcpyAsync(f->d_mask, mask.data(), need * sizeof(Rav2dPackedFilter), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));  // the host copy is pageable
+    f->h_mask.swap(mask);
     if (lr)
         f->h_lr.assign(lr, lr + n_lr);
     else
         f->h_lr.clear();
     f->lr_dirty = true;
-    if (lr) {
-        HIP_TRY(hipMemcpyAsync(f->d_lr, lr, need * sizeof(Rav2dAv2Restoration), hipMemcpyHostToDevice, s));
-    }
-    HIP_TRY(hipStreamSynchronize(s));  // host copies are pageable
     f->side_ok = true;
     f->gen++;
     return RAV2D_OK;
+}
+
+}  // namespace
+
+int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, size_t n_mask,
+                                  const Rav2dAv2Restoration *lr, size_t n_lr, void *stream) {
+    if (!f || !mask) return RAV2D_EINVAL;
+    if (n_mask != (size_t)f->g.sb256w * f->g.sb256h) {
+        set_last_error("side info count must equal the SB256 grid size");
+        return RAV2D_EINVAL;
+    }
+    std::vector<Rav2dPackedFilter> packed(n_mask);
+    rav2d_gpu_pack_filter(mask, n_mask, packed.data());
+    return set_side_info_common(f, packed, lr, n_lr, stream);
+}
+
+int rav2d_gpu_frame_set_side_info_packed(Rav2dGpuFrame *f, const Rav2dPackedFilter *mask, size_t n_mask,
+                                         const Rav2dAv2Restoration *lr, size_t n_lr, void *stream) {
+    if (!f || !mask) return RAV2D_EINVAL;
+    if (n_mask != (size_t)f->g.sb256w * f->g.sb256h) {
+        set_last_error("side info count must equal the SB256 grid size");
+        return RAV2D_EINVAL;
+    }
+    std::vector<Rav2dPackedFilter> copy(mask, mask + n_mask);
+    return set_side_info_common(f, copy, lr, n_lr, stream);
 }
 
 int rav2d_gpu_frame_set_segmap(Rav2dGpuFrame *f, const uint8_t *segmap, ptrdiff_t stride,
@@ -1168,6 +1227,25 @@ int rav2d_gpu_frame_upload(Rav2dGpuFrame *f, const void *y, const void *u, const
     for (int pl = 0; pl < 3; pl++)
         HIP_TRY(hipMemcpy2DAsync(f->buf[BUF_IN][pl], f->stride[pl], src[pl], (size_t)f->pw[pl] * f->bps,
                                  (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return RAV2D_OK;
+}
+
+int rav2d_gpu_frame_upload_padded(Rav2dGpuFrame *f, const Rav2dPlanes *src, int pad_rows, int pad_cols,
+                                  void *stream) {
+    if (!f || !src || pad_rows < 0 || pad_rows > RAV2D_PICTURE_PAD || pad_cols < 0 || pad_cols > RAV2D_PICTURE_PAD)
+        return RAV2D_EINVAL;
+    for (int pl = 0; pl < 3; pl++)
+        if (!src->data[pl] || src->stride[pl ? 1 : 0] < (ptrdiff_t)(f->pw[pl] + pad_cols) * f->bps) {
+            set_last_error("upload_padded: null plane or stride below (width + pad_cols) samples");
+            return RAV2D_EINVAL;
+        }
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    hipStream_t s = pick_stream(f->ctx, stream);
+    const int cols = std::min(pad_cols, pad_columns(f->pw[0]));
+    for (int pl = 0; pl < 3; pl++)
+        HIP_TRY(hipMemcpy2DAsync(f->buf[BUF_IN][pl], f->stride[pl], src->data[pl], (size_t)src->stride[pl ? 1 : 0],
+                                 (size_t)(f->pw[pl] + cols) * f->bps, f->ph[pl] + pad_rows, hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     return RAV2D_OK;
 }
@@ -1254,6 +1332,12 @@ int rav2d_gpu_picture_alloc(Rav2dGpuCtx *ctx, int width, int height, int bitdept
     void *mem = nullptr;
     r = rav2d_gpu_alloc(ctx, bytes[0] + bytes[1] + bytes[2], &mem);
     if (r) return r;
+    // zero, picture padding included, until the caller writes it
+    if (hipSetDevice(ctx->device) != hipSuccess || hipMemset(mem, 0, bytes[0] + bytes[1] + bytes[2]) != hipSuccess) {
+        rav2d_gpu_free(ctx, mem);
+        set_last_error("picture_alloc: hipMemset failed");
+        return RAV2D_EDEVICE;
+    }
     out->data[0] = mem;
     out->data[1] = (uint8_t *)mem + bytes[0];
     out->data[2] = (uint8_t *)mem + bytes[0] + bytes[1];

@@ -49,6 +49,7 @@ struct R2SFrame {
     Rav2dFrameParams params;
     int pw[3], ph[3];
     std::vector<uint8_t> planes[3];
+    std::vector<uint8_t> padded[3];  // the planes with 8 rows / columns of picture padding
     std::vector<Rav2dAv2Filter> mask;
     std::vector<Rav2dAv2Restoration> lr;
     std::vector<uint8_t> seg[2];  // segment maps: luma / chroma 4x4 units
@@ -228,8 +229,17 @@ void partition(Rng &r, int x, int y, int s4, int bw, int bh, std::vector<Block> 
     out.push_back(b);
 }
 
+// Rows below and columns right of each plane that a decoder's picture holds
+// (reconstruction of the blocks that overhang the frame edge, in the padding
+// of its picture allocation, picture.rs:77-125) and the deblocking reads.
+constexpr int kPad = 8;
+
+// `pl`: the w x h plane; `pad`: the same plane with kPad rows and columns of
+// padding ((w + kPad) samples per row), whose padding continues the plane's
+// content (its blocks' DC offsets and noise from the padding's own stream `rp`).
 template <typename P>
-void fill_plane(Rng &r, P *pl, int w, int h, int ss, const std::vector<Block> &blocks, int bd, double sigma) {
+void fill_plane(Rng &r, Rng &rp, P *pl, P *pad, int w, int h, int ss, const std::vector<Block> &blocks, int bd,
+                double sigma) {
     int bdmax = (1 << bd) - 1;
     double gx = r.uniform() * 2.0 - 1.0, gy = r.uniform() * 2.0 - 1.0;
     double base = (0.3 + 0.4 * r.uniform()) * bdmax;
@@ -247,13 +257,18 @@ void fill_plane(Rng &r, P *pl, int w, int h, int ss, const std::vector<Block> &b
                     for (int xx = x0 + tx; xx < std::min(w, x0 + tx + tw); xx++) dc[(size_t)yy * w + xx] = off;
             }
     }
+    auto sample = [&](int x, int y, double noise) -> P {
+        double v = base + amp * (gx * (x / (double)w - 0.5) + gy * (y / (double)h - 0.5)) +
+                   amp * 0.25 * sin(x * 0.013 + y * 0.021) + sigma * noise +
+                   dc[(size_t)std::min(y, h - 1) * w + std::min(x, w - 1)];
+        int iv = (int)lrint(v);
+        return (P)(iv < 0 ? 0 : iv > bdmax ? bdmax : iv);
+    };
+    const int ps = w + kPad;
     for (int y = 0; y < h; y++)
-        for (int x = 0; x < w; x++) {
-            double v = base + amp * (gx * (x / (double)w - 0.5) + gy * (y / (double)h - 0.5)) +
-                       amp * 0.25 * sin(x * 0.013 + y * 0.021) + sigma * r.gauss() + dc[(size_t)y * w + x];
-            int iv = (int)lrint(v);
-            pl[(size_t)y * w + x] = (P)(iv < 0 ? 0 : iv > bdmax ? bdmax : iv);
-        }
+        for (int x = 0; x < w; x++) pad[(size_t)y * ps + x] = pl[(size_t)y * w + x] = sample(x, y, r.gauss());
+    for (int y = 0; y < h + kPad; y++)
+        for (int x = y < h ? w : 0; x < ps; x++) pad[(size_t)y * ps + x] = sample(x, y, rp.gauss());
 }
 
 void gen_params(Rng &r, R2SFrame *f) {
@@ -496,15 +511,25 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
     // planes
     int bd = cfg->bitdepth;
     double sigma = bd == 8 ? 4.0 : 16.0;
+    Rng rp(cfg->seed ^ 0x5ad5ad5ad5ad5adull);  // the padding's own stream: the planes do not depend on it
+    const int bps = bd > 8 ? 2 : 1;
     for (int pl = 0; pl < 3; pl++) {
         size_t n = (size_t)f->pw[pl] * f->ph[pl];
-        f->planes[pl].assign(n * (bd > 8 ? 2 : 1), 0);
+        f->planes[pl].assign(n * bps, 0);
+        f->padded[pl].assign((size_t)(f->pw[pl] + kPad) * (f->ph[pl] + kPad) * bps, 0);
         if (bd > 8)
-            fill_plane(r, (uint16_t *)f->planes[pl].data(), f->pw[pl], f->ph[pl], pl ? 1 : 0, blocks, bd, sigma);
+            fill_plane(r, rp, (uint16_t *)f->planes[pl].data(), (uint16_t *)f->padded[pl].data(), f->pw[pl],
+                       f->ph[pl], pl ? 1 : 0, blocks, bd, sigma);
         else
-            fill_plane(r, f->planes[pl].data(), f->pw[pl], f->ph[pl], pl ? 1 : 0, blocks, bd, sigma);
+            fill_plane(r, rp, f->planes[pl].data(), f->padded[pl].data(), f->pw[pl], f->ph[pl], pl ? 1 : 0, blocks,
+                       bd, sigma);
     }
     return f;
+}
+
+void *r2s_plane_padded(R2SFrame *f, int pl, ptrdiff_t *stride) {
+    if (stride) *stride = (ptrdiff_t)(f->pw[pl] + kPad) * (f->cfg.bitdepth > 8 ? 2 : 1);
+    return f->padded[pl].data();
 }
 
 void r2s_mask_edges_part(uint16_t (*masks)[64][5][4], int by4, int bx4, int w4, int h4, int part, int tw4, int th4,

@@ -61,6 +61,11 @@ void r2s_frame_destroy(R2SFrame *f);
 Rav2dFrameParams *r2s_params(R2SFrame *f);
 /* tightly packed planes: stride = plane width in samples */
 void *r2s_plane(R2SFrame *f, int pl);
+/* the same plane inside a picture buffer with 8 rows below and 8 columns
+ * right of it (the padding of a decoder's picture, which holds the
+ * reconstruction of blocks that overhang the frame and which the deblocking
+ * reads); *stride = (width + 8) samples in bytes */
+void *r2s_plane_padded(R2SFrame *f, int pl, ptrdiff_t *stride);
 int32_t r2s_plane_width(const R2SFrame *f, int pl);
 int32_t r2s_plane_height(const R2SFrame *f, int pl);
 size_t r2s_plane_bytes(const R2SFrame *f, int pl);

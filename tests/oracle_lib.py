@@ -78,17 +78,19 @@ def ptr(a):
 def filter_frame(sf, inloop_mask=R.INLOOP_ALL, params=None):
     """Run the oracle chain on copies of a SynthFrame's planes; returns [y, u, v]."""
     lib = load()
-    planes = sf.planes()
-    bps = planes[0].itemsize
+    # the planes inside their picture padding (8 rows below, 8 columns right),
+    # which the deblocking reads where transforms overhang the frame edge
+    padded = sf.padded_planes()
+    bps = padded[0].itemsize
     p = params if params is not None else sf.params
     seg, seg_stride, seg_uv, uv_seg_stride = sf.segmaps()
     rc = lib.oracle_filter_frame_seg(C.byref(p), sf.mask_ptr, sf.n_mask, sf.lr_ptr, sf.n_lr,
-                                     planes[0].ctypes.data, planes[1].ctypes.data, planes[2].ctypes.data,
-                                     planes[0].shape[1] * bps, planes[1].shape[1] * bps, inloop_mask,
+                                     padded[0].ctypes.data, padded[1].ctypes.data, padded[2].ctypes.data,
+                                     padded[0].shape[1] * bps, padded[1].shape[1] * bps, inloop_mask,
                                      seg, seg_stride, seg_uv, uv_seg_stride)
     if rc != 0:
         raise RuntimeError("oracle_filter_frame failed: %d" % rc)
-    return planes
+    return [a[:sf.plane_shape(pl)[0], :sf.plane_shape(pl)[1]].copy() for pl, a in enumerate(padded)]
 
 
 def apply_grain(sf, planes, params=None):

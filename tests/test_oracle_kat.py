@@ -54,6 +54,10 @@ def test_uniform_frame_unchanged_by_deblock():
         h, w = sf.plane_shape(pl)
         arr = np.frombuffer((C.c_uint16 * (h * w)).from_address(sf.plane_ptr(pl)), dtype=np.uint16)
         arr[:] = 512
+        st = C.c_ssize_t()  # and the plane inside its picture padding
+        ptr = sf._lib.r2s_plane_padded(sf._h, pl, C.byref(st))
+        pad = np.frombuffer((C.c_uint8 * (st.value * (h + R.PICTURE_PAD))).from_address(ptr), dtype=np.uint16)
+        pad[:] = 512
     out = O.filter_frame(sf, R.INLOOP_DEBLOCK)
     for pl in range(3):
         assert (out[pl] == 512).all()
@@ -79,11 +83,16 @@ def test_deblock_changes_blocky_frame():
 
 
 def _fill(sf, values):
+    """Set every sample of a plane, its picture padding included."""
     for pl in range(3):
         h, w = sf.plane_shape(pl)
         ct = C.c_uint16 if sf.bitdepth > 8 else C.c_uint8
         arr = np.frombuffer((ct * (h * w)).from_address(sf.plane_ptr(pl)), dtype=np.uint16 if sf.bitdepth > 8 else np.uint8)
         arr[:] = values[pl]
+        st = C.c_ssize_t()
+        ptr = sf._lib.r2s_plane_padded(sf._h, pl, C.byref(st))
+        pad = np.frombuffer((C.c_uint8 * (st.value * (h + R.PICTURE_PAD))).from_address(ptr), dtype=arr.dtype)
+        pad[:] = values[pl]
 
 
 def test_uniform_frame_unchanged_by_cdef_and_wiener():
@@ -104,8 +113,8 @@ def test_lossless_frame_unchanged_by_deblock_and_restoration():
     """Lossless 4x4 blocks are never written by deblock (deblock.rs:2463-2490)
     or by LR/GDF (looprestoration.rs:2823-2862, 3036-3079, 3188-3238,
     2672-2678).  CDEF and CCSO consult the lossless masks only on
-    segmented-lossless frames (cdef.rs:594-598, 894-897), which the supported
-    parameter space (segmentation off) excludes."""
+    segmented-lossless frames (cdef.rs:594-598, 894-897): the segmented case
+    is test_segmentation's."""
     for bd in (8, 10):
         sf = R.SynthFrame(256, 192, bd, 0, seed=31 + bd, tools=R.TOOL_DEBLOCK | R.TOOL_WIENER | R.TOOL_GDF,
                           lossless_pct=100)

@@ -29,7 +29,7 @@ def _params_copy(p):
 def _oracle(sf, params, inloop, mask=None, seg=None):
     """Oracle chain with explicit params, optional replacement masks / maps."""
     lib = O.load()
-    planes = sf.planes()
+    planes = sf.padded_planes()  # with the picture padding the deblocking reads
     bps = planes[0].itemsize
     mask_ptr = C.cast(mask, C.c_void_p) if mask is not None else sf.mask_ptr
     s, st, suv, uvst = seg if seg is not None else sf.segmaps()
@@ -38,7 +38,7 @@ def _oracle(sf, params, inloop, mask=None, seg=None):
                                      planes[0].shape[1] * bps, planes[1].shape[1] * bps, inloop,
                                      s, st, suv, uvst)
     assert rc == 0
-    return planes
+    return [a[:sf.plane_shape(pl)[0], :sf.plane_shape(pl)[1]] for pl, a in enumerate(planes)]
 
 
 def _same(a, b):

@@ -45,7 +45,7 @@ def test_filter_and_grain_on_device_pictures(ctx, case, in_place):
         fp = R.Planes()
         R._check(ctx.lib.rav2d_gpu_frame_planes(f.h, 0, C.byref(fp)), "frame_planes")
         assert list(fp.stride) == list(pic_in.planes.stride)
-        pic_in.write(sf.planes())
+        pic_in.write(sf.padded_planes())  # with its picture padding
         R._check(ctx.lib.rav2d_gpu_filter_planes(ctx.h, f.h, C.byref(pic_in.planes), C.byref(pic_out.planes), None),
                  "filter_planes")
         ref = O.filter_frame(sf)
