@@ -22,15 +22,6 @@
 
 namespace r2g {
 
-__constant__ int8_t kNsY[16][2] = {{1, 0}, {0, 1}, {2, 0}, {0, 2}, {1, 1}, {-1, 1}, {2, 1}, {2, -1},
-                                   {1, 2}, {1, -2}, {3, 0}, {0, 3}, {4, 0}, {0, 4}, {3, 3}, {3, -3}};
-__constant__ int8_t kPc[12][2] = {{1, 0}, {0, 1}, {2, 0}, {0, 2}, {1, 1}, {-1, 1},
-                                  {2, 1}, {2, -1}, {1, 2}, {1, -2}, {3, 0}, {0, 3}};
-__constant__ int8_t kNsUv[6][2] = {{1, 0}, {0, 1}, {1, 1}, {-1, 1}, {2, 0}, {0, 2}};
-__constant__ int8_t kNsUvY[12][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, -1},
-                                     {-1, 1}, {1, -1}, {2, 0}, {-2, 0}, {0, 2}, {0, -2}};
-__constant__ int8_t kGdfCoords[18][2] = {{6, 0}, {5, 0}, {4, 0}, {3, 0}, {2, 1}, {2, 0}, {2, -1}, {1, 2}, {1, 1},
-                                         {1, 0}, {1, -1}, {1, -2}, {0, 6}, {0, 5}, {0, 4}, {0, 3}, {0, 2}, {0, 1}};
 __constant__ uint16_t kPcNorm[4] = {3739, 3273, 3074, 7};
 __constant__ int16_t kModeW[4][3] = {{-527, 15325, 321}, {26436, -17705, 17905}, {366, -147, -194}, {202, -267, -179}};
 __constant__ int16_t kModeOff[4] = {-547, -21565, -573, -680};
