@@ -352,7 +352,7 @@ int rav2d_gpu_apply_grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const
 /* No counterpart in the reference: rav2d_gpu_apply_grain_planes for callers
  * that cannot allocate (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same arguments and output; the scratch
- * area is a static device array per GPU (563,532 bytes, held until the process
+ * area is a static device array per GPU (572,748 bytes, held until the process
  * ends), so the call allocates nothing.  Pictures up to 8192 x 8192 luma
  * samples (RAV2D_EINVAL beyond).  Every call on one GPU shares that array:
  * calls that overlap on a GPU (other streams, other threads) race. */

@@ -54,7 +54,7 @@ struct Geom {
         ss_ver = p->layout == RAV2D_PIXEL_LAYOUT_I420;
         ss_hor = p->layout == RAV2D_PIXEL_LAYOUT_I420 || p->layout == RAV2D_PIXEL_LAYOUT_I422;
         bitdepth = p->bitdepth;
-        hbd = p->bitdepth > 8 ? 1 : 0;
+        hbd = (p->bitdepth - 8) >> 1; /* SequenceHeader.hbd: 0, 1, 2 for 8, 10, 12 bit */
         bdmax = (1 << p->bitdepth) - 1;
     }
 };

@@ -125,7 +125,7 @@ extern "C" int oracle_filter_frame_seg(const Rav2dFrameParams *p, const Rav2dAv2
     if (p->bitdepth == 8)
         return run_frame<uint8_t>(p, mask, n_mask, lr, n_lr, (uint8_t *)y, (uint8_t *)u, (uint8_t *)v, y_stride,
                                   uv_stride, inloop_mask, seg, seg_stride, seg_uv, uv_seg_stride);
-    if (p->bitdepth == 10)
+    if (p->bitdepth == 10 || p->bitdepth == 12)
         return run_frame<uint16_t>(p, mask, n_mask, lr, n_lr, (uint16_t *)y, (uint16_t *)u, (uint16_t *)v,
                                    y_stride / 2, uv_stride / 2, inloop_mask, seg, seg_stride, seg_uv, uv_seg_stride);
     return RAV2D_EINVAL;

@@ -5,33 +5,46 @@
 namespace r2g {
 
 constexpr int kPasses = RAV2D_NUM_PASSES;
+// bit-depth groups: 8-, 10- and 12-bit frames
+constexpr int kDepths = 3;
+inline int depth_group(int bitdepth) { return (bitdepth - 8) >> 1; }
+inline int group_depth(int g) { return 8 + 2 * g; }
 
 // Host-side launch arguments of a set of frames.  Each pass has one launch
-// per bit depth present (group 0: 8-bit frames, group 1: 10-bit frames), so
-// every launch runs one sample-type instantiation of the kernel body.
+// per bit depth present, so every launch runs one instantiation of the kernel
+// body.
 struct HostBatch {
-    std::vector<DeblockArgs> dbv[2], dbh[2];
-    std::vector<CdefArgs> cdef[2];
-    std::vector<LrArgs> lr[2];
-    std::vector<GrainArgs> gapply[2], gsetup;
-    std::vector<int32_t> prefix[kPasses][2];
-    int64_t total[kPasses][2] = {};  // work items of each launch
+    std::vector<DeblockArgs> dbv[kDepths], dbh[kDepths];
+    std::vector<CdefArgs> cdef[kDepths];
+    std::vector<LrArgs> lr[kDepths];
+    std::vector<GrainArgs> gapply[kDepths], gsetup;
+    std::vector<int32_t> prefix[kPasses][kDepths];
+    int64_t total[kPasses][kDepths] = {};  // work items of each launch
     int64_t bytes[kPasses] = {};     // algorithmic bytes: one read + one write of the planes
     int64_t pixels = 0;              // luma pixels of the frames
     int frames[kPasses] = {};        // frames in each pass
-    int64_t items(int p) const { return total[p][0] + total[p][1]; }
+    int64_t items(int p) const {
+        int64_t n = 0;
+        for (int g = 0; g < kDepths; g++) n += total[p][g];
+        return n;
+    }
 };
 
 // Device view of a packed HostBatch.
 struct DevBatch {
-    DeblockArgs *dbv[2] = {}, *dbh[2] = {};
-    CdefArgs *cdef[2] = {};
-    LrArgs *lr[2] = {};
-    GrainArgs *gapply[2] = {}, *gsetup = nullptr;
-    int32_t *prefix[kPasses][2] = {};
-    int nf[kPasses][2] = {};
-    int32_t total[kPasses][2] = {};
+    DeblockArgs *dbv[kDepths] = {}, *dbh[kDepths] = {};
+    CdefArgs *cdef[kDepths] = {};
+    LrArgs *lr[kDepths] = {};
+    GrainArgs *gapply[kDepths] = {}, *gsetup = nullptr;
+    int32_t *prefix[kPasses][kDepths] = {};
+    int nf[kPasses][kDepths] = {};
+    int32_t total[kPasses][kDepths] = {};
     int n_setup = 0;
+    int64_t items(int p) const {
+        int64_t n = 0;
+        for (int g = 0; g < kDepths; g++) n += total[p][g];
+        return n;
+    }
 };
 
 // Launch shape of each pass: workgroups (0 = one per item) and XCD remap.
@@ -59,7 +72,7 @@ struct EnqueueCtx {
     hipStream_t s, side;
     hipEvent_t ev_fork, ev_join;
     // optional: a second stream that runs the 8-bit group's passes beside the
-    // 10-bit group's (untimed, unprofiled runs only)
+    // high-bit-depth groups' (untimed, unprofiled runs only)
     hipStream_t side2 = nullptr;
     hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
     int time_pass = -1;

@@ -32,7 +32,8 @@ namespace {
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 struct Layout {
-    size_t dbv[2], dbh[2], cdef[2], lr[2], gapply[2], gsetup, prefix[kPasses][2], total;
+    size_t dbv[kDepths], dbh[kDepths], cdef[kDepths], lr[kDepths], gapply[kDepths], gsetup, prefix[kPasses][kDepths],
+        total;
 };
 
 Layout layout_of(const HostBatch &hb) {
@@ -43,7 +44,7 @@ Layout layout_of(const HostBatch &hb) {
         off = align_up(off + bytes, 256);
         return o;
     };
-    for (int g = 0; g < 2; g++) {
+    for (int g = 0; g < kDepths; g++) {
         L.dbv[g] = reserve(hb.dbv[g].size() * sizeof(DeblockArgs));
         L.dbh[g] = reserve(hb.dbh[g].size() * sizeof(DeblockArgs));
         L.cdef[g] = reserve(hb.cdef[g].size() * sizeof(CdefArgs));
@@ -80,7 +81,7 @@ int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply
         ChainPlan cp;
         int r = plan_chain(ctx, f, RAV2D_PASS_LR, cp);
         if (r) return r;
-        const int g = f->g.bitdepth > 8;
+        const int g = depth_group(f->g.bitdepth);
         // one read and one write of the frame's 1.5 * W * H samples
         const int64_t fbytes = 3 * (int64_t)f->pw[0] * f->ph[0] * f->bps;
         hb.pixels += (int64_t)f->pw[0] * f->ph[0];
@@ -124,7 +125,7 @@ int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply
         }
     }
     for (int p = 0; p < kPasses; p++)
-        for (int g = 0; g < 2; g++)
+        for (int g = 0; g < kDepths; g++)
             if (hb.total[p][g] > INT32_MAX) {
                 set_last_error("batch too large: more than 2^31 work items in one launch");
                 return RAV2D_EINVAL;
@@ -136,10 +137,11 @@ size_t packed_size(const HostBatch &hb) { return layout_of(hb).total; }
 
 size_t packed_bound(int n) {
     // per frame at most: both deblock passes, CDEF, LR, grain apply and set-up,
-    // one prefix entry per pass; each of the 21 arrays pads to 256 bytes
+    // one prefix entry per pass; each array (5 + kPasses per bit-depth group,
+    // and the set-ups) pads to 256 bytes
     const size_t per = 2 * sizeof(DeblockArgs) + sizeof(CdefArgs) + sizeof(LrArgs) + 2 * sizeof(GrainArgs) +
                        kPasses * sizeof(int32_t);
-    return (size_t)std::max(n, 1) * per + (4 * kPasses + 1) * 256 + 256;
+    return (size_t)std::max(n, 1) * per + (kDepths * (5 + kPasses) + 1) * 256 + 256;
 }
 
 void pack_batch(const HostBatch &hb, uint8_t *host, uint8_t *dev, DevBatch &db) {
@@ -148,7 +150,7 @@ void pack_batch(const HostBatch &hb, uint8_t *host, uint8_t *dev, DevBatch &db) 
         if (bytes) memcpy(host + o, src, bytes);
     };
     db = DevBatch();
-    for (int g = 0; g < 2; g++) {
+    for (int g = 0; g < kDepths; g++) {
         put(L.dbv[g], hb.dbv[g].data(), hb.dbv[g].size() * sizeof(DeblockArgs));
         put(L.dbh[g], hb.dbh[g].data(), hb.dbh[g].size() * sizeof(DeblockArgs));
         put(L.cdef[g], hb.cdef[g].data(), hb.cdef[g].size() * sizeof(CdefArgs));
@@ -177,11 +179,11 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
         return RAV2D_EDEVICE;
     }
     const hipStream_t s = ec.s;
-    const bool grain = apply_grain && (db.total[RAV2D_PASS_GRAIN][0] + db.total[RAV2D_PASS_GRAIN][1]) > 0;
-    // the two bit-depth groups are independent chains: with a second stream they
-    // run side by side and fill each other's launch gaps and tails
+    const bool grain = apply_grain && db.items(RAV2D_PASS_GRAIN) > 0;
+    // the 8-bit group and the high-bit-depth groups are independent chains: with
+    // a second stream they run side by side and fill each other's launch gaps and tails
     bool both = false;
-    for (int p = 0; p < kPasses; p++) both = both || (db.total[p][0] > 0 && db.total[p][1] > 0);
+    for (int p = 0; p < kPasses; p++) both = both || (db.total[p][0] > 0 && db.items(p) > db.total[p][0]);
     const bool par = both && ec.side2 && ec.time_pass < 0 && !ec.profile;
     if (par) {
         HIP_TRY(hipEventRecord(ec.ev_fork2, s));
@@ -195,14 +197,14 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
         HIP_TRY(hipEventRecord(ec.ev_join, ec.side));
     }
     for (int p = 0; p < kPasses; p++) {
-        if (db.total[p][0] + db.total[p][1] == 0) continue;
+        if (db.items(p) == 0) continue;
         if (p == RAV2D_PASS_GRAIN && !grain) continue;
         if (p == RAV2D_PASS_GRAIN && db.n_setup) {
             HIP_TRY(hipStreamWaitEvent(s, ec.ev_join, 0));
             if (par) HIP_TRY(hipStreamWaitEvent(ec.side2, ec.ev_join, 0));
         }
         if (p == ec.time_pass) HIP_TRY(hipEventRecord(ec.ev_a, s));
-        for (int g = 0; g < 2; g++) {
+        for (int g = 0; g < kDepths; g++) {
             if (db.total[p][g] == 0) continue;
             const hipStream_t sg = (par && g == 0) ? ec.side2 : s;
             PassTimer pt(ec.profile, p, sg);
@@ -212,7 +214,7 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
             sc.total = db.total[p][g];
             sc.xcd_remap = shape.xcd_remap[p] != 0;
             const int grid = grid_of(db.total[p][g], shape, p);
-            const int bd = g ? 10 : 8;
+            const int bd = group_depth(g);
             switch (p) {
                 case RAV2D_PASS_DEBLOCK_V: launch_deblock_batch(true, db.dbv[g], sc, grid, bd, sg); break;
                 case RAV2D_PASS_DEBLOCK_H: launch_deblock_batch(false, db.dbh[g], sc, grid, bd, sg); break;

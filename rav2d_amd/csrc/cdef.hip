@@ -128,8 +128,9 @@ __device__ __forceinline__ void dir_costs(const int16_t *img, int ls, int bdm8, 
 // directions, for taps k = 0, 1.  16 bits hold every intermediate: tap
 // differences are formed with a saturating subtract, so a difference with an
 // INT16_MIN neighbour stays at least 32767 in magnitude and constrains to 0
-// exactly as in 32-bit arithmetic (thresholds are at most 15 << 2 and shifts
-// at most 8); |sum| <= 12 * 4 * 60 < 2^15.
+// exactly as in 32-bit arithmetic (samples below 2^12, primary strengths at
+// most 15 << 4 and secondary 4 << 4 at 12 bit, shifts at most 10);
+// |sum| <= 2 * 7 * 240 + 12 * 64 < 2^15.
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ s16x2 splat2(int v) { return (s16x2){(short)v, (short)v}; }
@@ -415,7 +416,7 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
     uv_sec += uv_sec == 3;
     uv_sec <<= bdm8;
     const int damping = a.damping + bdm8;
-    constexpr int bdmax = sizeof(P) == 1 ? 255 : 1023;  // 8-bit frames run with P = uint8_t, 10-bit with uint16_t
+    const int bdmax = a.g.bdmax;  // 8-bit frames run with P = uint8_t, 10- and 12-bit with uint16_t
     // every CCSO neighbour (|dx| <= 2, |dy| <= 1) of the superblock's luma is
     // inside the frame: no clamping
     const bool cc_inner = x0 >= 2 && y0 >= 1 && x0 + 66 <= fw && y0 + 65 <= fh;

@@ -500,7 +500,7 @@ struct DspGdfArgs {
     Stripe s;
     int8_t *dst;
     int64_t dst_stride;
-    int32_t up, alpha_base, weight_base, bias_idx, intra, err_base, scale;
+    int32_t up, down, alpha_base, weight_base, bias_idx, intra, err_base, scale;
     const uint16_t *alpha;
     const int16_t *weight, *bias;
     const int8_t *err_intra, *err_inter;
@@ -513,6 +513,8 @@ struct DspGdfArgs {
 template <typename P> __global__ __launch_bounds__(256) void dsp_gdf_prep(DspGdfArgs a) {
     const int w = a.s.w, h = a.s.h;
     constexpr int kOff[4][2] = {{1, 0}, {0, 1}, {1, 1}, {-1, 1}};
+    // samples in the 10-bit domain of the error tables (12-bit ones shifted down)
+    auto V = [&](int r, int c) { return vimg<P>(a.s, r, c) >> a.down; };
     for (int i = threadIdx.x; i < w * h; i += blockDim.x) {
         const int y = i / w, x = i - y * w;
         const int y0 = y & ~1, x0 = x & ~1;
@@ -522,7 +524,7 @@ template <typename P> __global__ __launch_bounds__(256) void dsp_gdf_prep(DspGdf
             for (int r = y0 - 1; r <= y0 + 2; r++)
                 for (int c = x0 - 1; c <= x0 + 2; c++) {
                     const int dy = kOff[d][0], dx = kOff[d][1];
-                    g += abs(2 * vimg<P>(a.s, r, c) - vimg<P>(a.s, r - dy, c - dx) - vimg<P>(a.s, r + dy, c + dx));
+                    g += abs(2 * V(r, c) - V(r - dy, c - dx) - V(r + dy, c + dx));
                 }
             gs[d] = g;
         }
@@ -534,12 +536,12 @@ template <typename P> __global__ __launch_bounds__(256) void dsp_gdf_prep(DspGdf
             const int v = dmin(gs[d] >> (4 - a.up), (int)a.alpha[a.alpha_base + k * 4 + cl]);
             for (int j = 0; j < 3; j++) iv[j] += v * a.weight[a.weight_base + j * 88 + k * 4 + cl];
         }
-        const int m = vimg<P>(a.s, y, x);
+        const int m = V(y, x);
         for (int k = 0; k < 18; k++) {
             const int al = a.alpha[a.alpha_base + k * 4 + cl];
             const int dy = kGdfCoords[k][0], dx = kGdfCoords[k][1];
-            const int above = iclip((vimg<P>(a.s, y - dy, x - dx) - m) * (1 << a.up), -al, al);
-            const int below = iclip((vimg<P>(a.s, y + dy, x + dx) - m) * (1 << a.up), -al, al);
+            const int above = iclip((V(y - dy, x - dx) - m) * (1 << a.up), -al, al);
+            const int below = iclip((V(y + dy, x + dx) - m) * (1 << a.up), -al, al);
             const int v = iclip(above + below, -512, 511);
             for (int j = 0; j < 3; j++) iv[j] += v * a.weight[a.weight_base + j * 88 + k * 4 + cl];
         }
@@ -671,8 +673,8 @@ using namespace r2g;
 namespace {
 
 bool bad_depth(int bitdepth) {
-    if (bitdepth != 8 && bitdepth != 10) {
-        set_last_error("bit depth must be 8 or 10");
+    if (bitdepth != 8 && bitdepth != 10 && bitdepth != 12) {
+        set_last_error("bit depth must be 8, 10 or 12");
         return true;
     }
     return false;
@@ -921,6 +923,7 @@ int rav2d_gpu_gdf_prep(Rav2dGpuCtx *ctx, int8_t *dst, ptrdiff_t dst_stride, cons
     a.dst = dst;
     a.dst_stride = dst_stride;
     a.up = bitdepth == 8 ? 2 : 0;
+    a.down = bitdepth == 12 ? 2 : 0;
     a.alpha_base = ref_dst_idx * 528 + qp_idx * 88;
     a.weight_base = ref_dst_idx * 1584 + qp_idx * 264;
     a.bias_idx = ref_dst_idx * 6 + qp_idx;

@@ -58,7 +58,7 @@ constexpr int kSetupThreads = 256;
 struct GrainSmem {
     int16_t lut[3][GH][GW];
     int16_t g_lds[2048];
-    uint8_t scl[3][1024];
+    uint8_t scl[3][4096];  // 1 << bitdepth entries per plane
     int8_t win[3][4][7];
     int luma_done;  // luma AR steps completed, published per 16-step block
 };
@@ -176,7 +176,7 @@ __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7]
 // every entry in closed form: the first pass writes entries at multiples of
 // 1 << (bd-8) (later segments overwrite earlier ones, then the tail fill), the
 // second pass interpolates between them inside each segment.
-__device__ __forceinline__ void scaling_luts(const Rav2dFilmGrainData &d, int bd, uint8_t (*scl)[1024], int lane) {
+__device__ __forceinline__ void scaling_luts(const Rav2dFilmGrainData &d, int bd, uint8_t (*scl)[4096], int lane) {
     for (int pl = 0; pl < 3; pl++) {
         const int num = d.num_points[pl];
         const uint8_t(*pts)[2] = d.points[pl];
@@ -315,7 +315,7 @@ __device__ __forceinline__ void grain_setup_body(const GrainArgs &a, const int16
     __syncthreads();
     // out: the scaling LUTs and the grain LUTs' sample areas (luma rows 0..72,
     // chroma rows 0..37 at the same row stride), four bytes at a time
-    for (int i = t; i < 3 * 1024 / 4; i += kSetupThreads) ((uint32_t *)a.scaling)[i] = ((const uint32_t *)sm.scl)[i];
+    for (int i = t; i < 3 * 4096 / 4; i += kSetupThreads) ((uint32_t *)a.scaling)[i] = ((const uint32_t *)sm.scl)[i];
     constexpr int kLw = GH * GW / 2, kCw = SGH * GW / 2;
     for (int i = t; i < kLw + 2 * kCw; i += kSetupThreads) {
         const int pl = i < kLw ? 0 : (i < kLw + kCw ? 1 : 2);
@@ -348,20 +348,26 @@ __global__ __launch_bounds__(kSetupThreads) void grain_setup_batch(const GrainAr
 // neighbour's (filmgrain.rs:296-380).  The plane's scaling LUT (1 KiB) is
 // staged in `scl` (LDS) by one word per lane, so a lookup addresses LDS by the
 // sample value itself.
-template <typename P>
+// Words of a plane's scaling LUT each lane stages (1 << BD entries, at least
+// one word per lane).
+template <int BD> constexpr int scl_words() { return (1 << BD) > 1024 ? (1 << BD) / 1024 : 1; }
+
+template <typename P, int BD>
 __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id, uint8_t *scl) {
     int pl, tx, ty;
     decode_tile(a.map, tile_id, pl, tx, ty);
     const DevPlane &src = a.src[pl], &dst = a.dst[pl];
     const Rav2dFilmGrainData &d = a.fg;
-    // 8-bit frames run with P = uint8_t, 10-bit frames with uint16_t
-    constexpr int bd = sizeof(P) == 1 ? 8 : 10, bdm8 = bd - 8, bdmax = (1 << bd) - 1;
+    // one instantiation per bit depth (8-bit frames with P = uint8_t, 10- and
+    // 12-bit with uint16_t)
+    constexpr int bd = BD, bdm8 = bd - 8, bdmax = (1 << bd) - 1;
     constexpr int ctr = 128 << bdm8, gmin = -ctr, gmax = ctr - 1;
+    constexpr int NW = scl_words<BD>();
     const int sub = pl ? 1 : 0;
     const int lg = 4 + d.block_size - sub;  // log2 of the block size in this plane (16 << block_size luma)
     const int pbs = 1 << lg;
     const int16_t *lut = a.lut + pl * GH * GW;
-    const uint8_t *scaling = a.scaling + (pl && d.chroma_scaling_from_luma ? 0 : pl) * 1024;
+    const uint8_t *scaling = a.scaling + (pl && d.chroma_scaling_from_luma ? 0 : pl) * 4096;
     const bool on = a.plane_on[pl];
     int minv = 0, maxv = bdmax;
     if (d.clip_to_restricted_range) {
@@ -378,8 +384,10 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
     const uint8_t *lbase = a.src[0].ptr + (uint64_t)(uint32_t)(2 * y0) * lstride + 2 * x0 * sizeof(P);
     const int t = threadIdx.x;
     const int c4 = (t & 15) * 4;  // column of the group in the tile
-    // first load: the scaling LUT word of this lane
-    const uint32_t sclw = on ? ((const uint32_t *)scaling)[t] : 0u;
+    // first loads: the scaling LUT words of this lane
+    uint32_t sclw[NW];
+#pragma unroll
+    for (int k = 0; k < NW; k++) sclw[k] = on ? ((const uint32_t *)scaling)[t + 256 * k] : 0u;
     constexpr int NG = 2;
     int gr[NG];
     bool gv[NG];
@@ -414,7 +422,8 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
             if (gv[u]) store4((P *)(dbase + __umul24(gr[u], dstride)) + c4, sq[u]);
         return;
     }
-    ((uint32_t *)scl)[t] = sclw;  // on is tile-uniform: every lane reaches the barrier
+#pragma unroll
+    for (int k = 0; k < NW; k++) ((uint32_t *)scl)[t + 256 * k] = sclw[k];  // on is tile-uniform: every lane reaches the barrier
     __syncthreads();
     auto Wt = [&](int k, int j) -> int { return sub ? (k == 0 ? (j == 0 ? 23 : 22) : 0) : ((k == j) ? 27 : 17); };
     const int os = 2 >> sub;
@@ -494,16 +503,16 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
     }
 }
 
-template <typename P> __global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
-    __shared__ __align__(4) uint8_t scl[1024];
-    grain_apply_body<P>(a, blockIdx.x, scl);
+template <typename P, int BD> __global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
+    __shared__ __align__(4) uint8_t scl[1024 * scl_words<BD>()];
+    grain_apply_body<P, BD>(a, blockIdx.x, scl);
 }
 
-template <typename P, bool kLoop>
+template <typename P, int BD, bool kLoop>
 __global__ __launch_bounds__(256) void grain_apply_batch(const GrainArgs *__restrict__ args, BatchSched s) {
-    __shared__ __align__(4) uint8_t scl[1024];
+    __shared__ __align__(4) uint8_t scl[1024 * scl_words<BD>()];
     for_each_item<kLoop>(s, [&](int f, int t) {
-        grain_apply_body<P>(args[f], t, scl);
+        grain_apply_body<P, BD>(args[f], t, scl);
         if (kLoop) __syncthreads();  // the next item restages scl
     });
 }
@@ -660,17 +669,20 @@ void launch_grain_setup_batch(const GrainArgs *d_args, int n, hipStream_t s) {
 void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &sc, int grid, int bitdepth, hipStream_t s) {
     if (sc.total <= 0) return;
     const bool loop = grid < sc.total;
-    if (bitdepth > 8) {
-        if (loop)
-            R2G_LAUNCH((grain_apply_batch<uint16_t, true>), dim3(grid), dim3(256), 0, s, d_args, sc);
-        else
-            R2G_LAUNCH((grain_apply_batch<uint16_t, false>), dim3(grid), dim3(256), 0, s, d_args, sc);
-    } else {
-        if (loop)
-            R2G_LAUNCH((grain_apply_batch<uint8_t, true>), dim3(grid), dim3(256), 0, s, d_args, sc);
-        else
-            R2G_LAUNCH((grain_apply_batch<uint8_t, false>), dim3(grid), dim3(256), 0, s, d_args, sc);
-    }
+#define R2G_GRAIN_BATCH(P, BD)                                                                  \
+    do {                                                                                        \
+        if (loop)                                                                               \
+            R2G_LAUNCH((grain_apply_batch<P, BD, true>), dim3(grid), dim3(256), 0, s, d_args, sc);  \
+        else                                                                                    \
+            R2G_LAUNCH((grain_apply_batch<P, BD, false>), dim3(grid), dim3(256), 0, s, d_args, sc); \
+    } while (0)
+    if (bitdepth == 12)
+        R2G_GRAIN_BATCH(uint16_t, 12);
+    else if (bitdepth == 10)
+        R2G_GRAIN_BATCH(uint16_t, 10);
+    else
+        R2G_GRAIN_BATCH(uint8_t, 8);
+#undef R2G_GRAIN_BATCH
 }
 
 void grain_tile_map(GrainArgs &a) {
@@ -684,10 +696,12 @@ void grain_tile_map(GrainArgs &a) {
 
 void launch_grain_apply(GrainArgs a, int bitdepth, hipStream_t s) {
     grain_tile_map(a);
-    if (bitdepth > 8)
-        R2G_LAUNCH(grain_apply_kernel<uint16_t>, dim3(a.map.first[3]), dim3(256), 0, s, a);
+    if (bitdepth == 12)
+        R2G_LAUNCH((grain_apply_kernel<uint16_t, 12>), dim3(a.map.first[3]), dim3(256), 0, s, a);
+    else if (bitdepth == 10)
+        R2G_LAUNCH((grain_apply_kernel<uint16_t, 10>), dim3(a.map.first[3]), dim3(256), 0, s, a);
     else
-        R2G_LAUNCH(grain_apply_kernel<uint8_t>, dim3(a.map.first[3]), dim3(256), 0, s, a);
+        R2G_LAUNCH((grain_apply_kernel<uint8_t, 8>), dim3(a.map.first[3]), dim3(256), 0, s, a);
 }
 
 }  // namespace r2g

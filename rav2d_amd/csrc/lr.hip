@@ -147,7 +147,7 @@ struct LrSmem {
     LrItem it;
 };
 
-template <typename P, typename Args>
+template <typename P, int BD, typename Args>
 __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     uint16_t *V = sm.V, *Ld = sm.Ld, *cells = sm.cells;
     uint8_t *cls = sm.cls;
@@ -165,9 +165,10 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     auto &f3 = a.f3[pl];
     auto &f2 = a.f2[pl];
     auto &dst = a.dst[pl];
-    // 8-bit frames run with P = uint8_t, 10-bit frames with P = uint16_t (the
-    // only depths the runtime accepts), so clip bounds and shifts are constants
-    constexpr int bd = sizeof(P) == 1 ? 8 : 10, bdm8 = bd - 8, bdmax = (1 << bd) - 1;
+    // one instantiation per bit depth (8-bit frames with P = uint8_t, 10- and
+    // 12-bit with uint16_t), so clip bounds and shifts are constants
+    constexpr int bd = BD, bdm8 = bd - 8, bdmax = (1 << bd) - 1;
+    static_assert(sizeof(P) == (BD > 8 ? 2 : 1), "sample type of the bit depth");
     // every buffer of a plane has the same row stride; rows and strides are
     // below 2^24, so row offsets are 24-bit products
     const uint32_t stride = (uint32_t)f3.stride;
@@ -343,7 +344,10 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
     __syncthreads();
     // samples of V relative to the centre pointer
 #define VAT(p, dy, dx) ((int)(p)[(dy) * VW + (dx)])
-    constexpr int up = sizeof(P) == 1 ? 2 : 0;  // GDF works in the 10-bit domain (8 and 10 bit here)
+    // GDF works in the 10-bit domain: 8-bit differences are shifted up by 2,
+    // 12-bit samples down by 2 before any gradient or difference (gdf_prep,
+    // looprestoration.rs:1450-1453)
+    constexpr int up = BD == 8 ? 2 : 0, down = BD == 12 ? 2 : 0;
     // ---- GDF gradient cells: cell (j,k) covers centre columns 2j-1,2j and rows 2k-1,2k
     const int ncx = w / 2 + 1, ncy = h / 2 + 1;
     if (gdf) {
@@ -368,7 +372,9 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
                     wv[r][0] = w0[r * (VW / 2)];
                     wv[r][1] = w0[r * (VW / 2) + 1];
                 }
-                auto S = [&](int r, int c) -> uint32_t { return (wv[r][c >> 1] >> (16 * (c & 1))) & 0xffff; };
+                auto S = [&](int r, int c) -> uint32_t {
+                    return ((wv[r][c >> 1] >> (16 * (c & 1))) & 0xffff) >> down;
+                };
                 auto sad = [](uint32_t x, uint32_t y, uint32_t acc) { return (x > y ? x - y : y - x) + acc; };
                 uint32_t g0 = 0, g1 = 0, g2 = 0, g3 = 0;
 #pragma unroll
@@ -558,8 +564,14 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
                                          (short)VAT(pp[p], -kGdfT[2 * q2 + 1].dy, -kGdfT[2 * q2 + 1].dx)};
                         const s16x2 B = {(short)VAT(pp[p], kGdfT[2 * q2].dy, kGdfT[2 * q2].dx),
                                          (short)VAT(pp[p], kGdfT[2 * q2 + 1].dy, kGdfT[2 * q2 + 1].dx)};
-                        da[p] = (A - mm) << (short)up;
-                        db[p] = (B - mm) << (short)up;
+                        if constexpr (down != 0) {
+                            // samples (not differences) go to the 10-bit domain
+                            da[p] = (A >> (short)down) - (mm >> (short)down);
+                            db[p] = (B >> (short)down) - (mm >> (short)down);
+                        } else {
+                            da[p] = (A - mm) << (short)up;
+                            db[p] = (B - mm) << (short)up;
+                        }
                     }
 #pragma unroll
                     for (int p = 0; p < 4; p++) da[p] = __builtin_elementwise_max(da[p], nal);
@@ -591,7 +603,10 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
                     // (|diff| + rnd) >> sh with diff's sign, as a floor (sh = 12 - bd >= 2)
                     constexpr int sh = 12 - bd;
                     const int diff = __mul24((int)err[idx[p]], a.gdf_scale);
-                    out[p] = iclip(out[p] + ((diff + (1 << (sh - 1)) + (diff >> 31)) >> sh), 0, bdmax);
+                    if constexpr (sh == 0)
+                        out[p] = iclip(out[p] + diff, 0, bdmax);  // 12 bit: no rounding shift
+                    else
+                        out[p] = iclip(out[p] + ((diff + (1 << (sh - 1)) + (diff >> 31)) >> sh), 0, bdmax);
                 }
             }
         }
@@ -608,20 +623,20 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
 // at least 5 waves per SIMD: the body is VALU/LDS-latency bound, so the
 // register budget is capped (96 VGPRs) in exchange for occupancy (LDS allows
 // 5 workgroups per CU)
-template <typename P>
+template <typename P, int BD>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 8))) void lr_kernel(const LrArgs a) {
     __shared__ LrSmem sm;
-    lr_body<P>(a, blockIdx.x, sm);
+    lr_body<P, BD>(a, blockIdx.x, sm);
 }
 
 // Batched launch: every item (restoration units and plane copies) of every
 // frame of one bit depth.
-template <typename P, bool kLoop>
+template <typename P, int BD, bool kLoop>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 8))) void lr_batch(const LrArgs *__restrict__ args,
                                                                                       BatchSched s) {
     __shared__ LrSmem sm;
     for_each_item<kLoop>(s, [&](int f, int t) {
-        lr_body<P>(const_arg(args, f), t, sm);
+        lr_body<P, BD>(const_arg(args, f), t, sm);
         __syncthreads();
     });
 }
@@ -629,25 +644,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 8))
 void launch_lr_batch(const LrArgs *d_args, const BatchSched &sc, int grid, int bitdepth, hipStream_t s) {
     if (sc.total <= 0) return;
     const bool loop = grid < sc.total;
-    if (bitdepth > 8) {
-        if (loop)
-            R2G_LAUNCH((lr_batch<uint16_t, true>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
-        else
-            R2G_LAUNCH((lr_batch<uint16_t, false>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
-    } else {
-        if (loop)
-            R2G_LAUNCH((lr_batch<uint8_t, true>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
-        else
-            R2G_LAUNCH((lr_batch<uint8_t, false>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);
-    }
+#define R2G_LR_BATCH(P, BD)                                                                      \
+    do {                                                                                         \
+        if (loop)                                                                                \
+            R2G_LAUNCH((lr_batch<P, BD, true>), dim3(grid), dim3(kThreads), 0, s, d_args, sc);  \
+        else                                                                                     \
+            R2G_LAUNCH((lr_batch<P, BD, false>), dim3(grid), dim3(kThreads), 0, s, d_args, sc); \
+    } while (0)
+    if (bitdepth == 12)
+        R2G_LR_BATCH(uint16_t, 12);
+    else if (bitdepth == 10)
+        R2G_LR_BATCH(uint16_t, 10);
+    else
+        R2G_LR_BATCH(uint8_t, 8);
+#undef R2G_LR_BATCH
 }
 
 void launch_lr(const LrArgs &a, int n_items, int bitdepth, hipStream_t s) {
     if (n_items <= 0) return;
-    if (bitdepth > 8)
-        R2G_LAUNCH(lr_kernel<uint16_t>, dim3(n_items), dim3(kThreads), 0, s, a);
+    if (bitdepth == 12)
+        R2G_LAUNCH((lr_kernel<uint16_t, 12>), dim3(n_items), dim3(kThreads), 0, s, a);
+    else if (bitdepth == 10)
+        R2G_LAUNCH((lr_kernel<uint16_t, 10>), dim3(n_items), dim3(kThreads), 0, s, a);
     else
-        R2G_LAUNCH(lr_kernel<uint8_t>, dim3(n_items), dim3(kThreads), 0, s, a);
+        R2G_LAUNCH((lr_kernel<uint8_t, 8>), dim3(n_items), dim3(kThreads), 0, s, a);
 }
 
 // Device copies of the tables, made once per device.

@@ -404,9 +404,9 @@ namespace {
 namespace r2g {
 
 size_t grain_scratch_bytes(int width, int height) {
-    // templates, scaling LUTs, block offsets at the smallest block size, and a
-    // last 256-byte line for the error word
-    return 3 * 73 * 82 * 2 + 3 * 1024 + (size_t)((height + 15) / 16) * ((width + 15) / 16) * 2 + 256;
+    // templates, scaling LUTs (4096 entries per plane: 12-bit size), block
+    // offsets at the smallest block size, and a last 256-byte line for the error word
+    return 3 * 73 * 82 * 2 + 3 * 4096 + (size_t)((height + 15) / 16) * ((width + 15) / 16) * 2 + 256;
 }
 
 int grain_args_for(const Rav2dFilmGrainData &d, uint32_t seed, int bitdepth, int width, int height, uint8_t *scratch,
@@ -431,7 +431,7 @@ int grain_args_for(const Rav2dFilmGrainData &d, uint32_t seed, int bitdepth, int
     a.bitdepth = bitdepth;
     a.lut = (int16_t *)scratch;
     a.scaling = scratch + 3 * 73 * 82 * 2;
-    a.offsets = a.scaling + 3 * 1024;
+    a.offsets = a.scaling + 3 * 4096;
     a.err = err;
     int bs = 16 << d.block_size;
     a.n_bands = (height + bs - 1) / bs;
@@ -524,8 +524,8 @@ int validate(const Rav2dFrameParams *p) {
         set_last_error("frame width and height must be at most 16384");
         return RAV2D_EINVAL;
     }
-    if (p->bitdepth != 8 && p->bitdepth != 10) {
-        set_last_error("bit depth must be 8 or 10");
+    if (p->bitdepth != 8 && p->bitdepth != 10 && p->bitdepth != 12) {
+        set_last_error("bit depth must be 8, 10 or 12");
         return RAV2D_EINVAL;
     }
     if (p->layout != RAV2D_PIXEL_LAYOUT_I420) {
@@ -952,7 +952,7 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
     g.sb128 = params->sb128;
     g.bitdepth = params->bitdepth;
     g.bdmax = (1 << params->bitdepth) - 1;
-    g.hbd = params->bitdepth > 8;
+    g.hbd = (params->bitdepth - 8) >> 1;  // SequenceHeader.hbd: 0, 1, 2 for 8, 10, 12 bit
     g.ss_hor = 1;
     g.ss_ver = 1;
     f->bps = params->bitdepth > 8 ? 2 : 1;
@@ -1482,7 +1482,7 @@ namespace {
 // an address: each device's address comes from hipGetSymbolAddress, taken
 // once per device with that device current.
 constexpr int kNoAllocMaxDim = 8192;
-constexpr size_t kNoAllocScratchBytes = 3 * 73 * 82 * 2 + 3 * 1024 +
+constexpr size_t kNoAllocScratchBytes = 3 * 73 * 82 * 2 + 3 * 4096 +
                                         (size_t)(kNoAllocMaxDim / 16) * (kNoAllocMaxDim / 16) * 2 + 256;
 static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
 

@@ -391,7 +391,7 @@ extern "C" {
 
 R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
     if (!cfg || cfg->width <= 0 || cfg->height <= 0 || (cfg->width & 7) || (cfg->height & 7) ||
-        (cfg->bitdepth != 8 && cfg->bitdepth != 10))
+        (cfg->bitdepth != 8 && cfg->bitdepth != 10 && cfg->bitdepth != 12))
         return nullptr;
     R2SFrame *f = new R2SFrame();
     f->cfg = *cfg;
@@ -510,7 +510,7 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
 
     // planes
     int bd = cfg->bitdepth;
-    double sigma = bd == 8 ? 4.0 : 16.0;
+    double sigma = 4.0 * (1 << (bd - 8));  // 4 at 8 bit, 16 at 10, 64 at 12
     Rng rp(cfg->seed ^ 0x5ad5ad5ad5ad5adull);  // the padding's own stream: the planes do not depend on it
     const int bps = bd > 8 ? 2 : 1;
     for (int pl = 0; pl < 3; pl++) {

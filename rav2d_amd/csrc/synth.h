@@ -4,7 +4,7 @@
  * records and frame parameters).  Used by the tests and the benchmark; not
  * part of the drop-in ABI in rav2d_gpu.h.
  *
- * Planes: smooth 2-D gradient + Gaussian noise (sigma 4 at 8 bit, 16 at 10 bit)
+ * Planes: smooth 2-D gradient + Gaussian noise (sigma 4 at 8 bit, 16 at 10, 64 at 12 bit)
  * + per-transform-block DC offsets on a random quadtree of 8x8..64x64 blocks with
  * square transforms 4..64.  Edge masks are produced the way create_db_mask
  * (crates/rav2d/src/lf_mask.rs:399-502) does: by default for intra blocks

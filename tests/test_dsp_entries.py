@@ -77,7 +77,7 @@ def _blocky(rng, shape, bd, step=24):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 @pytest.mark.parametrize("dir_,uv", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_deblock_sb(ctx, bd, dir_, uv):
     rng = np.random.default_rng(10 * bd + 2 * dir_ + uv)
@@ -105,7 +105,7 @@ def test_deblock_sb(ctx, bd, dir_, uv):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 def test_cdef_dir(ctx, bd):
     rng = np.random.default_rng(50 + bd)
     L = O.load()
@@ -129,7 +129,7 @@ def test_cdef_dir(ctx, bd):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 @pytest.mark.parametrize("size", [(8, 8), (4, 8), (4, 4)])
 def test_cdef_filter(ctx, bd, size):
     w, h = size
@@ -161,7 +161,7 @@ def test_cdef_filter(ctx, bd, size):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 def test_ccso_prep_and_add(ctx, bd):
     rng = np.random.default_rng(90 + bd)
     lib = _olib()
@@ -222,7 +222,7 @@ def _ll_rows(rng, h, w):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 @pytest.mark.parametrize("kind", [0, 1, 2, 3])
 def test_lr_wiener(ctx, bd, kind):
     rng = np.random.default_rng(200 + bd * 4 + kind)
@@ -287,7 +287,7 @@ def test_lr_wiener(ctx, bd, kind):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 def test_gdf_prep_and_add(ctx, bd):
     rng = np.random.default_rng(300 + bd)
     lib = _olib()
@@ -332,7 +332,7 @@ def _grain_data(rng, lag=3):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 @pytest.mark.parametrize("lag", [0, 1, 3])
 def test_generate_grain(ctx, bd, lag):
     rng = np.random.default_rng(400 + bd + lag)
@@ -358,7 +358,7 @@ def test_generate_grain(ctx, bd, lag):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 @pytest.mark.parametrize("bs,ov", [(0, 1), (1, 1), (0, 0), (1, 0)])
 def test_fgy_fguv(ctx, bd, bs, ov):
     rng = np.random.default_rng(500 + bd * 8 + bs * 2 + ov)

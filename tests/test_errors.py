@@ -33,7 +33,8 @@ def _params(sf):
     ("width", 60, "multiple of 8"),
     ("height", 36, "multiple of 8"),
     ("width", 0, "multiple of 8"),
-    ("bitdepth", 12, "bit depth"),
+    ("bitdepth", 9, "bit depth"),
+    ("bitdepth", 16, "bit depth"),
     ("layout", 3, "4:2:0"),
     ("sb128", 2, "sb128"),
     ("width", 16392, "at most 16384"),
@@ -80,7 +81,7 @@ def test_grain_data_rejected_and_frame_stays_usable(ctx, case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bitdepth", [8, 10])
+@pytest.mark.parametrize("bitdepth", [8, 10, 12])
 @pytest.mark.parametrize("xs", [(10, 200, 100), (60, 60, 180), (200, 120, 40), (90, 30, 150, 240)])
 def test_grain_points_that_go_back_match_the_oracle(ctx, bitdepth, xs):
     # the reference decodes such streams: segments with dx <= 0 are skipped

@@ -371,6 +371,25 @@ def _gdf(lib, st, p_rows, lpf, lpfb, left, ref, qp):
     return err
 
 
+@pytest.mark.parametrize("edges", [0, 15, TOP | LEFT])
+@pytest.mark.parametrize("ref,qp", [(0, 3), (2, 0), (4, 5)])
+def test_gdf_error_12bit_equals_10bit_of_samples_shifted_down(edges, ref, qp):
+    """12-bit samples enter GDF's 10-bit domain one by one (>> 2 of each
+    sample, gdf_prep looprestoration.rs:1450-1453): any low two bits leave the
+    error map unchanged, which a shift of the differences would not."""
+    lib = _lib()
+    rng = np.random.default_rng(7000 + edges * 31 + ref * 7 + qp)
+    for w, h in [(64, 64), (32, 20)]:
+        st = Stripe(rng, w, h, 10, edges, hi=1024)
+        base = [st.P, st.lpf, st.lpfb, st.left]
+        e10 = _gdf(lib, st, *base, ref, qp)
+        st.bd = 12
+        low = [rng.integers(0, 4, b.shape).astype(np.uint16) for b in base]
+        e12 = _gdf(lib, st, *[b.astype(np.uint16) * 4 + l for b, l in zip(base, low)], ref, qp)
+        assert np.array_equal(e10, e12), (w, h)
+        assert np.any(e10 != 0)
+
+
 @pytest.mark.parametrize("edges", [0, 15, TOP | LEFT, BOTTOM | RIGHT])
 @pytest.mark.parametrize("ref,qp", [(0, 0), (0, 5), (1, 2), (3, 4), (5, 1)])
 def test_gdf_error_ignores_a_dc_shift_and_8bit_equals_10bit_times_4(edges, ref, qp):
