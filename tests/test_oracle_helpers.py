@@ -134,10 +134,11 @@ def test_find_dir_reference_cases():
     assert var > 0 and d == 2  # horizontal stripes: rows
 
 
-@pytest.mark.parametrize("hbd", [0, 1])
-def test_find_dir_equals_av1_definition(hbd):
-    rng = np.random.default_rng(5 + hbd)
-    bdm8 = 2 if hbd else 0
+@pytest.mark.parametrize("bd", [8, 10, 12])
+def test_find_dir_equals_av1_definition(bd):
+    rng = np.random.default_rng(5 + (bd - 8) // 2)
+    bdm8 = bd - 8
+    hbd = int(bd > 8)
     for k in range(300):
         amp = [4, 40, 120][k % 3] << bdm8
         base = rng.integers(0, (256 << bdm8) - amp)
@@ -333,22 +334,22 @@ def _scaling(bd, pts):
     return out
 
 
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 def test_generate_scaling_reference_values(bd):
     # filmgrain.rs:978-1008, 1298-1341
     sh = bd - 8
     assert np.all(_scaling(bd, []) == 0)
     assert np.all(_scaling(bd, [(128, 200)]) == 200)
     s = _scaling(bd, [(0, 0), (255, 255)])
-    assert s[0] == 0 and s[-1] == 255 and 100 < s[128 << sh] and (bd == 8 or 120 < s[512] < 140)
+    assert s[0] == 0 and s[-1] == 255 and 120 < s[128 << sh] < 140
     s = _scaling(bd, [(0, 0), (100, 100)])
     assert np.all(np.diff(s[:400].astype(int)) >= 0)
-    if bd == 10:
+    if bd > 8:
         s = _scaling(bd, [(0, 10), (128, 200)])
         assert s[1] > 0 and s[2] > 0 and s[3] > 0
 
 
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 def test_generate_scaling_interpolates_through_points(bd):
     # piecewise-linear through every point, flat outside, within one rounding
     # (8-bit) or two stacked roundings (high bit depth: the 8-bit grid, then the
@@ -454,7 +455,8 @@ def _grain_params(rng, lag, gss):
     return d
 
 
-@pytest.mark.parametrize("lag,bd,gss", [(0, 8, 0), (1, 8, 1), (2, 10, 0), (3, 10, 2), (3, 8, 3)])
+@pytest.mark.parametrize("lag,bd,gss", [(0, 8, 0), (1, 8, 1), (2, 10, 0), (3, 10, 2), (3, 8, 3), (1, 12, 0),
+                                        (3, 12, 2)])
 def test_grain_luts_equal_av1_definition(lag, bd, gss):
     rng = np.random.default_rng(lag * 10 + bd + gss)
     d = _grain_params(rng, lag, gss)

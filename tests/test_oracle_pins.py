@@ -197,7 +197,7 @@ LR_GEOMS = [(64, 64), (64, 56), (40, 16), (20, 8), (64, 60)]
 
 
 @pytest.mark.parametrize("edges", range(16))
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 def test_ns_wiener_luma_is_the_closed_form(edges, bd):
     lib = _lib()
     rng = np.random.default_rng(1000 + edges * 7 + bd)
@@ -216,7 +216,7 @@ def test_ns_wiener_luma_is_the_closed_form(edges, bd):
 
 
 @pytest.mark.parametrize("edges", [0, 15, TOP | LEFT, BOTTOM | RIGHT, TOP | BOTTOM, LEFT | RIGHT])
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 def test_ns_multi_class_filtering_is_the_closed_form(edges, bd):
     # every class carries the same filter, so the output does not depend on
     # the class assignment: this pins the multi-class path's FIR and rows
@@ -241,7 +241,7 @@ def test_ns_multi_class_filtering_is_the_closed_form(edges, bd):
 
 
 @pytest.mark.parametrize("edges", [0, 15, TOP | LEFT, BOTTOM | RIGHT, TOP | BOTTOM, LEFT | RIGHT])
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 def test_pc_wiener_filtering_is_the_closed_form(edges, bd):
     lib = _lib()
     rng = np.random.default_rng(3000 + edges * 5 + bd)
@@ -445,7 +445,7 @@ def _av1_cdef(img, y0, x0, w, h, pri, sec, dirn, damping, coeff_shift):
     return out
 
 
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 @pytest.mark.parametrize("size", [8, 4])
 def test_cdef_block_filter_is_the_av1_filter_process(bd, size):
     lib = _lib()
