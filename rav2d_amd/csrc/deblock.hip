@@ -19,42 +19,95 @@
 namespace r2g {
 
 // deblock_bd on an LDS segment: `p` = first sample on the positive side of
-// line 0, SA steps along the edge, SB across it.  Each line's 16 samples
-// (-8..7 across the edge; the filters read and write at most 8 per side) are
-// read before any is written: the lines of a segment, and the segments of
-// one pass, touch disjoint samples.
+// line 0, SA steps along the edge, SB across it.  The segment's four lines of
+// 16 samples (-8..7 across the edge; the filters read and write at most 8 per
+// side) are read into registers before any is written: the lines of a
+// segment, and the segments of one pass, touch disjoint samples.  `p` is
+// 4-sample aligned, so with SA = 1 (horizontal edges) the four lines' samples
+// at one offset are one 8-byte word, and with SB = 1 (vertical edges) a line
+// is four 8-byte words.
 template <int SA, int SB>
 __device__ __forceinline__ void filter_edge(uint16_t *p, int bdmax, uint32_t q_thr, uint32_t side_thr, int mwp,
                                             int mwn, bool pos_ll, bool neg_ll) {
-    int l0[16], l3[16];
+    static_assert(SA == 1 || SB == 1, "one of the steps is a sample");
+    int L[4][16];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        l0[k] = p[(k - 8) * SB];
-        l3[k] = p[3 * SA + (k - 8) * SB];
+        if constexpr (SA == 1) {
+            const uint2 w = *(const uint2 *)&p[(k - 8) * SB];
+            L[0][k] = (int)(w.x & 0xffff);
+            L[1][k] = (int)(w.x >> 16);
+            L[2][k] = (int)(w.y & 0xffff);
+            L[3][k] = (int)(w.y >> 16);
+        } else if ((k & 3) == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint2 w = *(const uint2 *)&p[i * SA + k - 8];
+                L[i][k] = (int)(w.x & 0xffff);
+                L[i][k + 1] = (int)(w.x >> 16);
+                L[i][k + 2] = (int)(w.y & 0xffff);
+                L[i][k + 3] = (int)(w.y >> 16);
+            }
+        }
     }
-    const int width = filter_choice(l0, l3, mwn, mwp, q_thr, side_thr);
+    const int width = filter_choice(L[0], L[3], mwn, mwp, q_thr, side_thr);
     const int wn = dmin(width, mwn), wp = width;
     if (wp < 1) return;
-    const int clampv = (int)q_thr * byte_at(kQThreshMults, width - 1);
+    // q_thr < 2^16 and the factors below fit 24 bits (|delta| < 2^17, weights
+    // <= 85, widths <= 8): 24-bit multiplies
+    const int clampv = (int)__umul24(q_thr, (uint32_t)byte_at(kQThreshMults, width - 1));
     const int mneg = byte_at(kWMult, wn - 1), mpos = byte_at(kWMult, wp - 1);
+    int delta[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int L[16];
+    for (int i = 0; i < 4; i++)
+        delta[i] = dmax(dmin(4 * (3 * (L[i][8] - L[i][7]) - (L[i][9] - L[i][6])), clampv), -clampv);
+    // tap j's correction (delta * w * (width - j) + 1024) >> 11 from a running
+    // sum that steps down by delta * w per tap; the width test is shared by
+    // the four lines
+    auto put = [&](int o, const int (&v)[4]) {
+        if constexpr (SA == 1) {
+            *(uint2 *)&p[o * SB] = make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+        } else {
 #pragma unroll
-        for (int k = 0; k < 16; k++) L[k] = i == 0 ? l0[k] : (i == 3 ? l3[k] : (int)p[i * SA + (k - 8) * SB]);
-        uint16_t *d = p + i * SA;
-        const int delta = iclip(4 * (3 * (L[8] - L[7]) - (L[9] - L[6])), -clampv, clampv);
-        if (!neg_ll) {
-            const int dn = delta * mneg;
-#pragma unroll
-            for (int j = 0; j < 8; j++)
-                if (j < wn) d[(-j - 1) * SB] = (uint16_t)iclip(L[7 - j] + ((dn * (wn - j) + 1024) >> 11), 0, bdmax);
+            for (int i = 0; i < 4; i++) p[i * SA + o] = (uint16_t)v[i];
         }
-        if (!pos_ll) {
-            const int dp = delta * mpos;
+    };
+    if (!neg_ll) {
+        int dn[4], acc[4];
 #pragma unroll
-            for (int j = 0; j < 8; j++)
-                if (j < wp) d[j * SB] = (uint16_t)iclip(L[8 + j] - ((dp * (wp - j) + 1024) >> 11), 0, bdmax);
+        for (int i = 0; i < 4; i++) {
+            dn[i] = __mul24(delta[i], mneg);
+            acc[i] = __mul24(dn[i], wn) + 1024;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (j < wn) {
+                int v[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) v[i] = dmax(dmin(L[i][7 - j] + (acc[i] >> 11), bdmax), 0);
+                put(-j - 1, v);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) acc[i] -= dn[i];
+        }
+    }
+    if (!pos_ll) {
+        int dp[4], acc[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            dp[i] = __mul24(delta[i], mpos);
+            acc[i] = __mul24(dp[i], wp) + 1024;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (j < wp) {
+                int v[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) v[i] = dmax(dmin(L[i][8 + j] - (acc[i] >> 11), bdmax), 0);
+                put(j, v);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) acc[i] -= dp[i];
         }
     }
 }
