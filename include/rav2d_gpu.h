@@ -401,8 +401,15 @@ typedef struct Rav2dBatchConfig {
                                               persistent grid that loops */
     int32_t xcd_remap[RAV2D_NUM_PASSES];   /* give each XCD a contiguous range
                                               of work items */
+    int32_t deblock_rows;                  /* 0: the two deblocking passes are
+                                              separate launches; 32 or 64: one
+                                              tile pass of 64 x deblock_rows
+                                              tiles filters vertical, then
+                                              horizontal edges (launch shape in
+                                              grid / xcd_remap[DEBLOCK_V]) */
 } Rav2dBatchConfig;
-/* use_graph = 1, apply_grain = 1, one workgroup per item, no remap. */
+/* use_graph = 1, apply_grain = 1, one workgroup per item, no remap, separate
+ * deblocking passes. */
 void rav2d_gpu_batch_default_config(Rav2dBatchConfig *cfg);
 int rav2d_gpu_batch_create(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n,
                            const Rav2dBatchConfig *cfg, Rav2dGpuBatch **out);

@@ -103,11 +103,13 @@ class BatchConfig(C.Structure):
         ("apply_grain", C.c_int32),
         ("grid", C.c_int32 * NUM_PASSES),
         ("xcd_remap", C.c_int32 * NUM_PASSES),
+        ("deblock_rows", C.c_int32),
     ]
 
     def as_dict(self):
         return {"use_graph": self.use_graph, "apply_grain": self.apply_grain,
-                "grid": list(self.grid), "xcd_remap": list(self.xcd_remap)}
+                "grid": list(self.grid), "xcd_remap": list(self.xcd_remap),
+                "deblock_rows": self.deblock_rows}
 
 
 class ServeConfig(C.Structure):

@@ -14,7 +14,12 @@ inline int group_depth(int g) { return 8 + 2 * g; }
 // per bit depth present, so every launch runs one instantiation of the kernel
 // body.
 struct HostBatch {
+    // deblock_rows > 0: both deblocking passes of a frame run as one tile pass
+    // (DeblockVHArgs, tiles of 64 x deblock_rows) in the vertical-edge pass's
+    // slot, and the horizontal-edge pass has no items
+    int deblock_rows = 0;
     std::vector<DeblockArgs> dbv[kDepths], dbh[kDepths];
+    std::vector<DeblockVHArgs> dbvh[kDepths];
     std::vector<CdefArgs> cdef[kDepths];
     std::vector<LrArgs> lr[kDepths];
     std::vector<GrainArgs> gapply[kDepths], gsetup;
@@ -32,7 +37,9 @@ struct HostBatch {
 
 // Device view of a packed HostBatch.
 struct DevBatch {
+    int deblock_rows = 0;
     DeblockArgs *dbv[kDepths] = {}, *dbh[kDepths] = {};
+    DeblockVHArgs *dbvh[kDepths] = {};
     CdefArgs *cdef[kDepths] = {};
     LrArgs *lr[kDepths] = {};
     GrainArgs *gapply[kDepths] = {}, *gsetup = nullptr;
@@ -54,8 +61,10 @@ struct LaunchShape {
 };
 
 // Plans the frames' chains (deblock, CDEF, LR into OUT; with apply_grain,
-// grain into DISP) and gathers their launch arguments.
-int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply_grain, HostBatch &hb);
+// grain into DISP) and gathers their launch arguments; deblock_rows 32 or 64
+// fuses the two deblocking passes (0: separate launches).
+int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply_grain, HostBatch &hb,
+               int deblock_rows = 0);
 // Bytes of the packed argument block.
 size_t packed_size(const HostBatch &hb);
 // An upper bound of packed_size for any batch of at most n frames.

@@ -32,7 +32,7 @@ namespace {
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 struct Layout {
-    size_t dbv[kDepths], dbh[kDepths], cdef[kDepths], lr[kDepths], gapply[kDepths], gsetup, prefix[kPasses][kDepths],
+    size_t dbv[kDepths], dbh[kDepths], dbvh[kDepths], cdef[kDepths], lr[kDepths], gapply[kDepths], gsetup, prefix[kPasses][kDepths],
         total;
 };
 
@@ -47,6 +47,7 @@ Layout layout_of(const HostBatch &hb) {
     for (int g = 0; g < kDepths; g++) {
         L.dbv[g] = reserve(hb.dbv[g].size() * sizeof(DeblockArgs));
         L.dbh[g] = reserve(hb.dbh[g].size() * sizeof(DeblockArgs));
+        L.dbvh[g] = reserve(hb.dbvh[g].size() * sizeof(DeblockVHArgs));
         L.cdef[g] = reserve(hb.cdef[g].size() * sizeof(CdefArgs));
         L.lr[g] = reserve(hb.lr[g].size() * sizeof(LrArgs));
         L.gapply[g] = reserve(hb.gapply[g].size() * sizeof(GrainArgs));
@@ -74,8 +75,14 @@ int batch_init_device() {
     return RAV2D_OK;
 }
 
-int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply_grain, HostBatch &hb) {
+int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply_grain, HostBatch &hb,
+               int deblock_rows) {
     hb = HostBatch();
+    if (deblock_rows && !deblock_vh_rows_ok(deblock_rows)) {
+        set_last_error("fused deblocking takes tiles of 32 or 64 rows");
+        return RAV2D_EINVAL;
+    }
+    hb.deblock_rows = deblock_rows;
     for (int i = 0; i < n; i++) {
         Rav2dGpuFrame *f = frames[i];
         ChainPlan cp;
@@ -91,11 +98,24 @@ int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply
             hb.bytes[pass] += fbytes;
             hb.frames[pass]++;
         };
-        if (cp.on[RAV2D_PASS_DEBLOCK_V]) {
+        if (deblock_rows && (cp.on[RAV2D_PASS_DEBLOCK_V] || cp.on[RAV2D_PASS_DEBLOCK_H])) {
+            // one tile pass from the first deblocking pass's source to the last one's output
+            const bool v = cp.on[RAV2D_PASS_DEBLOCK_V], h = cp.on[RAV2D_PASS_DEBLOCK_H];
+            DeblockVHArgs va;
+            va.a = v ? cp.dbv : cp.dbh;
+            for (int pl = 0; pl < 3; pl++) {
+                va.dst[pl] = h ? cp.dbh.dst[pl] : cp.dbv.dst[pl];
+                va.on_v[pl] = v && cp.dbv.plane_on[pl];
+                va.on_h[pl] = h && cp.dbh.plane_on[pl];
+            }
+            va.map = deblock_vh_tile_map(va.a, deblock_rows);
+            hb.dbvh[g].push_back(va);
+            add(RAV2D_PASS_DEBLOCK_V, va.map.first[3]);
+        } else if (cp.on[RAV2D_PASS_DEBLOCK_V]) {
             hb.dbv[g].push_back(cp.dbv);
             add(RAV2D_PASS_DEBLOCK_V, cp.dbv.map.first[3]);
         }
-        if (cp.on[RAV2D_PASS_DEBLOCK_H]) {
+        if (!deblock_rows && cp.on[RAV2D_PASS_DEBLOCK_H]) {
             hb.dbh[g].push_back(cp.dbh);
             add(RAV2D_PASS_DEBLOCK_H, cp.dbh.map.first[3]);
         }
@@ -136,12 +156,12 @@ int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply
 size_t packed_size(const HostBatch &hb) { return layout_of(hb).total; }
 
 size_t packed_bound(int n) {
-    // per frame at most: both deblock passes, CDEF, LR, grain apply and set-up,
+    // per frame at most: both deblock passes or the fused one, CDEF, LR, grain apply and set-up,
     // one prefix entry per pass; each array (5 + kPasses per bit-depth group,
     // and the set-ups) pads to 256 bytes
-    const size_t per = 2 * sizeof(DeblockArgs) + sizeof(CdefArgs) + sizeof(LrArgs) + 2 * sizeof(GrainArgs) +
+    const size_t per = 2 * sizeof(DeblockArgs) + sizeof(DeblockVHArgs) + sizeof(CdefArgs) + sizeof(LrArgs) + 2 * sizeof(GrainArgs) +
                        kPasses * sizeof(int32_t);
-    return (size_t)std::max(n, 1) * per + (kDepths * (5 + kPasses) + 1) * 256 + 256;
+    return (size_t)std::max(n, 1) * per + (kDepths * (6 + kPasses) + 1) * 256 + 256;
 }
 
 void pack_batch(const HostBatch &hb, uint8_t *host, uint8_t *dev, DevBatch &db) {
@@ -150,14 +170,17 @@ void pack_batch(const HostBatch &hb, uint8_t *host, uint8_t *dev, DevBatch &db) 
         if (bytes) memcpy(host + o, src, bytes);
     };
     db = DevBatch();
+    db.deblock_rows = hb.deblock_rows;
     for (int g = 0; g < kDepths; g++) {
         put(L.dbv[g], hb.dbv[g].data(), hb.dbv[g].size() * sizeof(DeblockArgs));
         put(L.dbh[g], hb.dbh[g].data(), hb.dbh[g].size() * sizeof(DeblockArgs));
+        put(L.dbvh[g], hb.dbvh[g].data(), hb.dbvh[g].size() * sizeof(DeblockVHArgs));
         put(L.cdef[g], hb.cdef[g].data(), hb.cdef[g].size() * sizeof(CdefArgs));
         put(L.lr[g], hb.lr[g].data(), hb.lr[g].size() * sizeof(LrArgs));
         put(L.gapply[g], hb.gapply[g].data(), hb.gapply[g].size() * sizeof(GrainArgs));
         db.dbv[g] = (DeblockArgs *)(dev + L.dbv[g]);
         db.dbh[g] = (DeblockArgs *)(dev + L.dbh[g]);
+        db.dbvh[g] = (DeblockVHArgs *)(dev + L.dbvh[g]);
         db.cdef[g] = (CdefArgs *)(dev + L.cdef[g]);
         db.lr[g] = (LrArgs *)(dev + L.lr[g]);
         db.gapply[g] = (GrainArgs *)(dev + L.gapply[g]);
@@ -216,7 +239,12 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
             const int grid = grid_of(db.total[p][g], shape, p);
             const int bd = group_depth(g);
             switch (p) {
-                case RAV2D_PASS_DEBLOCK_V: launch_deblock_batch(true, db.dbv[g], sc, grid, bd, sg); break;
+                case RAV2D_PASS_DEBLOCK_V:
+                    if (db.deblock_rows)
+                        launch_deblock_vh_batch(db.dbvh[g], sc, grid, bd, db.deblock_rows, sg);
+                    else
+                        launch_deblock_batch(true, db.dbv[g], sc, grid, bd, sg);
+                    break;
                 case RAV2D_PASS_DEBLOCK_H: launch_deblock_batch(false, db.dbh[g], sc, grid, bd, sg); break;
                 case RAV2D_PASS_CDEF: launch_cdef_batch(db.cdef[g], sc, grid, bd, sg); break;
                 case RAV2D_PASS_LR: launch_lr_batch(db.lr[g], sc, grid, bd, sg); break;
@@ -287,7 +315,8 @@ int build(Rav2dGpuBatch *b) {
     drop_args(b);
     int r = batch_init_device();
     if (r) return r;
-    r = plan_batch(b->ctx, b->frames.data(), (int)b->frames.size(), b->cfg.apply_grain != 0, b->hb);
+    r = plan_batch(b->ctx, b->frames.data(), (int)b->frames.size(), b->cfg.apply_grain != 0, b->hb,
+                   b->cfg.deblock_rows);
     if (r) return r;
     const size_t bytes = packed_size(b->hb);
     {
@@ -363,6 +392,16 @@ int prepare(Rav2dGpuBatch *b) {
     return RAV2D_OK;
 }
 
+bool config_ok(const Rav2dBatchConfig &c) {
+    for (int p = 0; p < kPasses; p++)
+        if (c.grid[p] < 0) return false;
+    if (c.deblock_rows && !deblock_vh_rows_ok(c.deblock_rows)) {
+        set_last_error("batch config: deblock_rows must be 0, 32 or 64");
+        return false;
+    }
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -398,6 +437,10 @@ int rav2d_gpu_batch_create(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n
         b->cfg = *cfg;
     else
         rav2d_gpu_batch_default_config(&b->cfg);
+    if (!config_ok(b->cfg)) {
+        delete b;
+        return RAV2D_EINVAL;
+    }
     if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->side2, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
@@ -455,11 +498,10 @@ int rav2d_gpu_batch_get_config(const Rav2dGpuBatch *b, Rav2dBatchConfig *out) {
 
 int rav2d_gpu_batch_set_config(Rav2dGpuBatch *b, const Rav2dBatchConfig *cfg) {
     if (!b || !cfg) return RAV2D_EINVAL;
-    for (int p = 0; p < kPasses; p++)
-        if (cfg->grid[p] < 0) return RAV2D_EINVAL;
+    if (!config_ok(*cfg)) return RAV2D_EINVAL;
     HIP_TRY(hipSetDevice(b->ctx->device));
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
-    bool rebuild = cfg->apply_grain != b->cfg.apply_grain;
+    bool rebuild = cfg->apply_grain != b->cfg.apply_grain || cfg->deblock_rows != b->cfg.deblock_rows;
     b->cfg = *cfg;
     drop_graph(b);
     if (rebuild) drop_args(b);
@@ -483,26 +525,29 @@ int rav2d_gpu_batch_autotune(Rav2dGpuBatch *b, int iters, Rav2dBatchConfig *chos
     if (r) return r;
     hipStream_t s = b->ctx->stream;
     Rav2dBatchConfig best = b->cfg;
-    for (int p = 0; p < kPasses; p++) {
-        if (b->hb.items(p) == 0) continue;
-        if (p == RAV2D_PASS_GRAIN && !b->cfg.apply_grain) continue;
+    // Times pass p under every launch shape with the rest of `best` (the
+    // batch's arguments must be built for best's deblocking form); keeps the
+    // fastest shape in best and returns its time (sum over iters runs).
+    auto tune_pass = [&](int p, double &best_ms) -> int {
+        best_ms = 0;
+        if (b->hb.items(p) == 0) return RAV2D_OK;
         std::vector<std::pair<int, int>> cands = {{0, 0}, {0, 1}};
         for (int g : {256, 512, 1024, 2048, 4096})
             if (g < b->hb.items(p)) {
                 cands.push_back({g, 0});
                 cands.push_back({g, 1});
             }
-        double best_ms = 1e30;
+        best_ms = 1e30;
         for (auto c : cands) {
             b->cfg = best;
             b->cfg.grid[p] = c.first;
             b->cfg.xcd_remap[p] = c.second;
-            r = enqueue(b, s, -1, false);  // warm-up
-            if (r) return r;
+            int e = enqueue(b, s, -1, false);  // warm-up
+            if (e) return e;
             double sum = 0;
             for (int it = 0; it < iters; it++) {
-                r = enqueue(b, s, p, false);
-                if (r) return r;
+                e = enqueue(b, s, p, false);
+                if (e) return e;
                 HIP_TRY(hipEventSynchronize(b->ev_b));
                 float ms = 0.f;
                 HIP_TRY(hipEventElapsedTime(&ms, b->ev_a, b->ev_b));
@@ -514,6 +559,46 @@ int rav2d_gpu_batch_autotune(Rav2dGpuBatch *b, int iters, Rav2dBatchConfig *chos
                 best.xcd_remap[p] = c.second;
             }
         }
+        return RAV2D_OK;
+    };
+    // re-plans the batch for configuration c (its deblocking form), eager launches
+    auto replan = [&](const Rav2dBatchConfig &c) -> int {
+        HIP_TRY(hipStreamSynchronize(s));
+        b->cfg = c;
+        return build(b);
+    };
+    // deblocking: the two separate passes, or one tile pass of 32 or 64 rows
+    Rav2dBatchConfig sep = best;
+    sep.deblock_rows = 0;
+    r = replan(sep);
+    if (r) return r;
+    best = sep;
+    double tv = 0, th = 0;
+    if ((r = tune_pass(RAV2D_PASS_DEBLOCK_V, tv)) || (r = tune_pass(RAV2D_PASS_DEBLOCK_H, th))) return r;
+    double best_db = tv + th;
+    Rav2dBatchConfig best_cfg = best;
+    for (int rows : {32, 64}) {
+        Rav2dBatchConfig fc = best_cfg;
+        fc.deblock_rows = rows;
+        fc.grid[RAV2D_PASS_DEBLOCK_V] = fc.xcd_remap[RAV2D_PASS_DEBLOCK_V] = 0;
+        fc.grid[RAV2D_PASS_DEBLOCK_H] = fc.xcd_remap[RAV2D_PASS_DEBLOCK_H] = 0;
+        r = replan(fc);
+        if (r) return r;
+        best = fc;
+        double tf = 0;
+        if ((r = tune_pass(RAV2D_PASS_DEBLOCK_V, tf))) return r;
+        if (tf < best_db) {
+            best_db = tf;
+            best_cfg = best;
+        }
+    }
+    r = replan(best_cfg);
+    if (r) return r;
+    best = best_cfg;
+    for (int p = RAV2D_PASS_CDEF; p < kPasses; p++) {
+        if (p == RAV2D_PASS_GRAIN && !b->cfg.apply_grain) continue;
+        double t = 0;
+        if ((r = tune_pass(p, t))) return r;
     }
     HIP_TRY(hipStreamSynchronize(s));
     b->cfg = best;

@@ -298,6 +298,116 @@ __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id
     }
 }
 
+// Both passes on one tile of TH output rows.  The horizontal-edge filters
+// whose footprint reaches rows [y0, y0 + TH) sit at rows y0 - 4 .. y0 + TH + 4
+// and read 8 rows on each side, so the vertical-edge pass's output is needed
+// on rows [y0 - 12, y0 + TH + 12): those rows of F0 are staged (columns with
+// the vertical pass's 16-sample halo), their vertical edges filtered in LDS
+// exactly as the vertical-edge pass filters them (halo rows included), then
+// the horizontal edges, and the tile interior is written.  Rows outside the
+// plane and its padding stage as zero, which is what the separate
+// horizontal-edge pass reads there.
+constexpr int kVHHalo = 12;
+
+template <int TH> struct VHTile {
+    static constexpr int LW = kTW + 2 * kHalo, R = TH + 2 * kVHHalo;
+    uint16_t px[R][LW];
+    uint16_t lut[4][512];  // q / side thresholds of the vertical-edge kind, then of the horizontal-edge kind
+};
+
+template <typename P, int TH>
+__device__ __forceinline__ void deblock_vh_body(const DeblockVHArgs &f, int tile_id, VHTile<TH> &sm) {
+    const DeblockArgs &a = f.a;
+    constexpr int LW = VHTile<TH>::LW, R = VHTile<TH>::R;
+    int pl, tx, ty;
+    decode_tile(f.map, tile_id, pl, tx, ty);
+    const DevPlane &src = a.src[pl];
+    const DevPlane &dst = f.dst[pl];
+    const int x0 = tx * kTW, y0 = ty * TH, ys = y0 - kVHHalo;  // ys: plane row of staged row 0
+    const int W = src.w, H = src.h;
+    const int WP = W + a.pad_cols, HP = H + a.pad_rows;
+    const int t = threadIdx.x;
+    constexpr int QW = LW / 4, NQ = R * QW, PV = (NQ + kThreads - 1) / kThreads;
+    uint2 v[PV];
+#pragma unroll
+    for (int j = 0; j < PV; j++) {
+        const int i = dmin(t + j * kThreads, NQ - 1);
+        const int r = i / QW, gx = x0 - kHalo + (i - r * QW) * 4, gy = ys + r;
+        const uint2 l = load4(row_ptr<const P>(src, iclip(gy, 0, HP - 1)) + iclip(gx, 0, WP - 4));
+        v[j] = (gy >= 0 && gy < HP && gx >= 0 && gx < WP) ? l : make_uint2(0, 0);
+    }
+    // the four LUT rows, 16 bytes per lane
+    const int kv = pl ? 1 + pl : 0, kh = pl ? 1 + pl : 1, w4 = t >> 6;
+    const uint16_t *lrow = w4 == 0 ? a.lut->q[kv] : (w4 == 1 ? a.lut->side[kv] : (w4 == 2 ? a.lut->q[kh] : a.lut->side[kh]));
+    const uint4 lutv = ((const uint4 *)lrow)[t & 63];
+    // vertical edges: columns x0 - 4 .. x0 + TW + 4, 4-row segments of every staged row
+    constexpr int NEV = kTW / 4 + 3, NSV = NEV * (R / 4), NV = (NSV + kThreads - 1) / kThreads;
+    EdgeDesc ev[NV];
+#pragma unroll
+    for (int s = 0; s < NV; s++) {
+        ev[s].on = false;
+        const int idx = t + s * kThreads;
+        const int rs = idx / NEV, ex = x0 - 4 + (idx - rs * NEV) * 4, ey = ys + rs * 4;
+        if (f.on_v[pl] && idx < NSV && ex > 0 && ex < W && ey >= 0 && ey < H) ev[s] = edge_desc(a, pl, true, ex >> 2, ey >> 2);
+    }
+#pragma unroll
+    for (int j = 0; j < PV; j++)
+        if (t + j * kThreads < NQ) ((uint2 *)&sm.px[0][0])[t + j * kThreads] = v[j];
+    ((uint4 *)&sm.lut[0][0])[t] = lutv;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NV; s++) {
+        const EdgeDesc &e = ev[s];
+        if (e.on) {
+            const int idx = t + s * kThreads;
+            const int rs = idx / NEV, ec = idx - rs * NEV;
+            const uint32_t q = (uint32_t)(edge_thr(sm.lut[0][e.qc], sm.lut[0][e.qp]) >> e.subpu);
+            const uint32_t side = (uint32_t)(edge_thr(sm.lut[1][e.qc], sm.lut[1][e.qp]) >> e.subpu);
+            filter_edge<LW, 1>(&sm.px[rs * 4][ec * 4 - 4 + kHalo], a.g.bdmax, q, side, e.mwp, e.mwn, e.pos_ll, e.neg_ll);
+        }
+    }
+    __syncthreads();
+    // horizontal edges: rows y0 - 4 .. y0 + TH + 4, 4-column segments of the tile
+    constexpr int NEH = TH / 4 + 3, NSH = NEH * (kTW / 4), NH = (NSH + kThreads - 1) / kThreads;
+#pragma unroll
+    for (int s = 0; s < NH; s++) {
+        const int idx = t + s * kThreads;
+        const int er = idx / (kTW / 4), cs = idx - er * (kTW / 4);
+        const int ey = y0 - 4 + er * 4, ex = x0 + cs * 4;
+        if (f.on_h[pl] && idx < NSH && ey > 0 && ey < H && ex < W) {
+            const EdgeDesc e = edge_desc(a, pl, false, ex >> 2, ey >> 2);
+            if (e.on) {
+                const uint32_t q = (uint32_t)(edge_thr(sm.lut[2][e.qc], sm.lut[2][e.qp]) >> e.subpu);
+                const uint32_t side = (uint32_t)(edge_thr(sm.lut[3][e.qc], sm.lut[3][e.qp]) >> e.subpu);
+                filter_edge<1, LW>(&sm.px[ey - ys][cs * 4 + kHalo], a.g.bdmax, q, side, e.mwp, e.mwn, e.pos_ll,
+                                   e.neg_ll);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < TH * kTW / 4 / kThreads; j++) {
+        const int i = t + j * kThreads;
+        const int r = i / (kTW / 4), c = (i - r * (kTW / 4)) * 4;
+        const int gx = x0 + c, gy = y0 + r;
+        if (gy < H && gx < W) store4(row_ptr<P>(dst, gy) + gx, *(const uint2 *)&sm.px[r + kVHHalo][c + kHalo]);
+    }
+}
+
+template <typename P, int TH> __global__ __launch_bounds__(kThreads) void deblock_vh_kernel(DeblockVHArgs a) {
+    __shared__ __align__(16) VHTile<TH> sm;
+    deblock_vh_body<P, TH>(a, blockIdx.x, sm);
+}
+
+template <typename P, int TH, bool kLoop>
+__global__ __launch_bounds__(kThreads) void deblock_vh_batch(const DeblockVHArgs *__restrict__ args, BatchSched s) {
+    __shared__ __align__(16) VHTile<TH> sm;
+    for_each_item<kLoop>(s, [&](int f, int t) {
+        deblock_vh_body<P, TH>(args[f], t, sm);
+        __syncthreads();
+    });
+}
+
 template <typename P> __global__ __launch_bounds__(kThreads) void deblock_v_kernel(DeblockArgs a) {
     __shared__ __align__(16) uint16_t tile[kTH][kTW + 2 * kHalo];
     __shared__ __align__(16) uint16_t lut[2][512];
@@ -356,6 +466,58 @@ void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched
         R2G_DB_PICK(deblock_h_batch);
 #undef R2G_DB_PICK
 #undef R2G_DB_LAUNCH
+}
+
+TileMap deblock_vh_tile_map(const DeblockArgs &a, int rows) {
+    TileMap m;
+    m.first[0] = 0;
+    for (int p = 0; p < 3; p++) {
+        m.tiles_x[p] = (a.src[p].w + kTW - 1) / kTW;
+        m.tiles_y[p] = (a.src[p].h + rows - 1) / rows;
+        m.first[p + 1] = m.first[p] + m.tiles_x[p] * m.tiles_y[p];
+    }
+    return m;
+}
+
+void launch_deblock_vh(const DeblockVHArgs &a, int bitdepth, int rows, hipStream_t s) {
+    if (a.map.first[3] == 0) return;
+    const dim3 grid(a.map.first[3]), block(kThreads);
+    if (bitdepth > 8) {
+        if (rows == 64)
+            R2G_LAUNCH((deblock_vh_kernel<uint16_t, 64>), grid, block, 0, s, a);
+        else
+            R2G_LAUNCH((deblock_vh_kernel<uint16_t, 32>), grid, block, 0, s, a);
+    } else {
+        if (rows == 64)
+            R2G_LAUNCH((deblock_vh_kernel<uint8_t, 64>), grid, block, 0, s, a);
+        else
+            R2G_LAUNCH((deblock_vh_kernel<uint8_t, 32>), grid, block, 0, s, a);
+    }
+}
+
+void launch_deblock_vh_batch(const DeblockVHArgs *d_args, const BatchSched &s, int grid, int bitdepth, int rows,
+                             hipStream_t st) {
+    if (s.total <= 0) return;
+    const bool loop = grid < s.total;
+#define R2G_VH(P, TH)                                                                                      \
+    do {                                                                                                   \
+        if (loop)                                                                                          \
+            R2G_LAUNCH((deblock_vh_batch<P, TH, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);   \
+        else                                                                                               \
+            R2G_LAUNCH((deblock_vh_batch<P, TH, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);  \
+    } while (0)
+    if (bitdepth > 8) {
+        if (rows == 64)
+            R2G_VH(uint16_t, 64);
+        else
+            R2G_VH(uint16_t, 32);
+    } else {
+        if (rows == 64)
+            R2G_VH(uint8_t, 64);
+        else
+            R2G_VH(uint8_t, 32);
+    }
+#undef R2G_VH
 }
 
 TileMap deblock_tile_map(const DeblockArgs &a, bool vert) {

@@ -146,8 +146,25 @@ bool grain_init_tables();           // Gaussian table and jump matrices of the c
 int launch_grain_generate(const Rav2dFilmGrainData &fg, uint32_t seed, int bitdepth, int plane, int16_t *out,
                           const int16_t *luma, hipStream_t s);
 
+// Both deblocking passes in one tile pass: F0 -> F2 without the intermediate
+// plane.  `a` carries the vertical-edge pass's arguments (source F0, masks,
+// LUT, padding, segmentation), which the horizontal-edge pass shares; on_v /
+// on_h say which planes each pass filters (a plane neither filters is
+// copied).  The tiles are 64 x rows (32 or 64) of the output planes.
+struct DeblockVHArgs {
+    DeblockArgs a;
+    DevPlane dst[3];
+    int32_t on_v[3], on_h[3];
+    TileMap map;
+};
 void launch_deblock_v(const DeblockArgs &a, int bitdepth, hipStream_t s);
 void launch_deblock_h(const DeblockArgs &a, int bitdepth, hipStream_t s);
+// Tile rows the fused pass supports.
+inline bool deblock_vh_rows_ok(int rows) { return rows == 32 || rows == 64; }
+TileMap deblock_vh_tile_map(const DeblockArgs &a, int rows);
+void launch_deblock_vh(const DeblockVHArgs &a, int bitdepth, int rows, hipStream_t s);
+void launch_deblock_vh_batch(const DeblockVHArgs *d_args, const BatchSched &s, int grid, int bitdepth, int rows,
+                             hipStream_t st);
 void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, int bitdepth,
                           hipStream_t st);
 // 64x32 tiles of the three planes (the vertical-edge pass: down to the padding rows)

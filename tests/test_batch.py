@@ -48,18 +48,24 @@ def _check_all(frames, refs, tag):
         _compare(ref_disp, f.download(2), "%s frame %d %s (display)" % (tag, i, FRAMES[i][:3]))
 
 
-def _config(use_graph, grid, xcd):
+def _config(use_graph, grid, xcd, deblock_rows=0):
     cfg = R.default_batch_config()
     cfg.use_graph = use_graph
     for p in range(R.NUM_PASSES):
         cfg.grid[p] = grid
         cfg.xcd_remap[p] = xcd
+    cfg.deblock_rows = deblock_rows
     return cfg
 
 
+# (use_graph, grid, xcd_remap, deblock_rows): deblock_rows 32 / 64 runs both
+# deblocking passes as one tile pass
+SHAPES = [(1, 0, 0, 0), (1, 0, 1, 0), (1, 64, 0, 0), (1, 24, 1, 0), (0, 0, 0, 0), (0, 7, 1, 0),
+          (1, 0, 0, 32), (1, 0, 1, 64), (0, 24, 1, 32), (1, 64, 0, 64)]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(1, 0, 0), (1, 0, 1), (1, 64, 0), (1, 24, 1), (0, 0, 0), (0, 7, 1)],
-                         ids=lambda s: "graph%d_grid%d_xcd%d" % s)
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "graph%d_grid%d_xcd%d_dbrows%d" % s)
 def test_batch_matches_oracle(ctx, synth, shape):
     sfs, refs = synth
     frames = [R.GpuFrame.from_synth(ctx, sf) for sf in sfs]
@@ -136,6 +142,35 @@ def test_batch_without_grain_leaves_display_alone(ctx, synth):
         for i, f in enumerate(frames):
             _compare(refs[i][0], f.download(1), "no-grain batch frame %d" % i)
         assert b.info(R.PASS_GRAIN)[0] == 0
+    finally:
+        b.close()
+        for f in frames:
+            f.close()
+
+
+@pytest.mark.gpu
+def test_batch_switches_deblocking_form(ctx, synth):
+    """set_config between separate and fused deblocking re-plans the batch;
+    every form gives the oracle's planes, and tile rows other than 32 / 64
+    are rejected with the configuration unchanged."""
+    sfs, refs = synth
+    frames = [R.GpuFrame.from_synth(ctx, sf) for sf in sfs]
+    b = R.GpuBatch(ctx, frames)
+    try:
+        for rows in (64, 0, 32):
+            cfg = b.config()
+            cfg.deblock_rows = rows
+            b.set_config(cfg)
+            b.run()
+            ctx.synchronize()
+            _check_all(frames, refs, "deblock_rows %d" % rows)
+        bad = b.config()
+        bad.deblock_rows = 48
+        with pytest.raises(R.Rav2dError):
+            b.set_config(bad)
+        assert b.config().deblock_rows == 32
+        with pytest.raises(R.Rav2dError):
+            R.GpuBatch(ctx, frames[:1], bad)
     finally:
         b.close()
         for f in frames:
