@@ -407,9 +407,15 @@ typedef struct Rav2dBatchConfig {
                                               tiles filters vertical, then
                                               horizontal edges (launch shape in
                                               grid / xcd_remap[DEBLOCK_V]) */
+    int32_t mixed_depths;                  /* 1: one launch per pass over the
+                                              frames of every bit depth (the
+                                              bit depth dispatched per work
+                                              item); 0: one launch per pass and
+                                              bit depth, the 8-bit frames' chain
+                                              on a parallel graph branch */
 } Rav2dBatchConfig;
 /* use_graph = 1, apply_grain = 1, one workgroup per item, no remap, separate
- * deblocking passes. */
+ * deblocking passes, one launch per bit depth. */
 void rav2d_gpu_batch_default_config(Rav2dBatchConfig *cfg);
 int rav2d_gpu_batch_create(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n,
                            const Rav2dBatchConfig *cfg, Rav2dGpuBatch **out);

@@ -104,12 +104,13 @@ class BatchConfig(C.Structure):
         ("grid", C.c_int32 * NUM_PASSES),
         ("xcd_remap", C.c_int32 * NUM_PASSES),
         ("deblock_rows", C.c_int32),
+        ("mixed_depths", C.c_int32),
     ]
 
     def as_dict(self):
         return {"use_graph": self.use_graph, "apply_grain": self.apply_grain,
                 "grid": list(self.grid), "xcd_remap": list(self.xcd_remap),
-                "deblock_rows": self.deblock_rows}
+                "deblock_rows": self.deblock_rows, "mixed_depths": self.mixed_depths}
 
 
 class ServeConfig(C.Structure):

@@ -10,9 +10,8 @@ constexpr int kDepths = 3;
 inline int depth_group(int bitdepth) { return (bitdepth - 8) >> 1; }
 inline int group_depth(int g) { return 8 + 2 * g; }
 
-// Host-side launch arguments of a set of frames.  Each pass has one launch
-// per bit depth present, so every launch runs one instantiation of the kernel
-// body.
+// Host-side launch arguments of a set of frames, per pass and bit-depth
+// group (8-, 10-, 12-bit).
 struct HostBatch {
     // deblock_rows > 0: both deblocking passes of a frame run as one tile pass
     // (DeblockVHArgs, tiles of 64 x deblock_rows) in the vertical-edge pass's
@@ -54,10 +53,13 @@ struct DevBatch {
     }
 };
 
-// Launch shape of each pass: workgroups (0 = one per item) and XCD remap.
+// Launch shape of each pass: workgroups (0 = one per item) and XCD remap;
+// mixed_depths: one launch per pass over every bit-depth group (else one per
+// group present).
 struct LaunchShape {
     int32_t grid[kPasses];
     int32_t xcd_remap[kPasses];
+    int32_t mixed_depths = 0;
 };
 
 // Plans the frames' chains (deblock, CDEF, LR into OUT; with apply_grain,

@@ -207,7 +207,7 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
     // a second stream they run side by side and fill each other's launch gaps and tails
     bool both = false;
     for (int p = 0; p < kPasses; p++) both = both || (db.total[p][0] > 0 && db.items(p) > db.total[p][0]);
-    const bool par = both && ec.side2 && ec.time_pass < 0 && !ec.profile;
+    const bool par = both && !shape.mixed_depths && ec.side2 && ec.time_pass < 0 && !ec.profile;
     if (par) {
         HIP_TRY(hipEventRecord(ec.ev_fork2, s));
         HIP_TRY(hipStreamWaitEvent(ec.side2, ec.ev_fork2, 0));
@@ -227,6 +227,36 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
             if (par) HIP_TRY(hipStreamWaitEvent(ec.side2, ec.ev_join, 0));
         }
         if (p == ec.time_pass) HIP_TRY(hipEventRecord(ec.ev_a, s));
+        if (shape.mixed_depths) {
+            // one launch over every group's items
+            PassTimer pt(ec.profile, p, s);
+            MixSched ms;
+            ms.first[0] = 0;
+            for (int g = 0; g < kDepths; g++) {
+                ms.g[g].prefix = db.prefix[p][g];
+                ms.g[g].n_frames = db.nf[p][g];
+                ms.g[g].total = db.total[p][g];
+                ms.g[g].xcd_remap = 0;
+                ms.first[g + 1] = ms.first[g] + db.total[p][g];
+            }
+            ms.xcd_remap = shape.xcd_remap[p] != 0;
+            const int grid = grid_of(ms.first[3], shape, p);
+            switch (p) {
+                case RAV2D_PASS_DEBLOCK_V:
+                    if (db.deblock_rows)
+                        launch_deblock_vh_mix({{db.dbvh[0], db.dbvh[1], db.dbvh[2]}}, ms, grid, db.deblock_rows, s);
+                    else
+                        launch_deblock_mix(true, {{db.dbv[0], db.dbv[1], db.dbv[2]}}, ms, grid, s);
+                    break;
+                case RAV2D_PASS_DEBLOCK_H: launch_deblock_mix(false, {{db.dbh[0], db.dbh[1], db.dbh[2]}}, ms, grid, s); break;
+                case RAV2D_PASS_CDEF: launch_cdef_mix({{db.cdef[0], db.cdef[1], db.cdef[2]}}, ms, grid, s); break;
+                case RAV2D_PASS_LR: launch_lr_mix({{db.lr[0], db.lr[1], db.lr[2]}}, ms, grid, s); break;
+                default: launch_grain_apply_mix({{db.gapply[0], db.gapply[1], db.gapply[2]}}, ms, grid, s); break;
+            }
+            HIP_TRY(hipGetLastError());
+            if (p == ec.time_pass) HIP_TRY(hipEventRecord(ec.ev_b, s));
+            continue;
+        }
         for (int g = 0; g < kDepths; g++) {
             if (db.total[p][g] == 0) continue;
             const hipStream_t sg = (par && g == 0) ? ec.side2 : s;
@@ -306,6 +336,7 @@ LaunchShape shape_of(const Rav2dBatchConfig &c) {
         sh.grid[p] = c.grid[p];
         sh.xcd_remap[p] = c.xcd_remap[p];
     }
+    sh.mixed_depths = c.mixed_depths;
     return sh;
 }
 
@@ -397,6 +428,10 @@ bool config_ok(const Rav2dBatchConfig &c) {
         if (c.grid[p] < 0) return false;
     if (c.deblock_rows && !deblock_vh_rows_ok(c.deblock_rows)) {
         set_last_error("batch config: deblock_rows must be 0, 32 or 64");
+        return false;
+    }
+    if (c.mixed_depths != 0 && c.mixed_depths != 1) {
+        set_last_error("batch config: mixed_depths must be 0 or 1");
         return false;
     }
     return true;
@@ -595,10 +630,49 @@ int rav2d_gpu_batch_autotune(Rav2dGpuBatch *b, int iters, Rav2dBatchConfig *chos
     r = replan(best_cfg);
     if (r) return r;
     best = best_cfg;
+    best.mixed_depths = 0;
     for (int p = RAV2D_PASS_CDEF; p < kPasses; p++) {
         if (p == RAV2D_PASS_GRAIN && !b->cfg.apply_grain) continue;
         double t = 0;
         if ((r = tune_pass(p, t))) return r;
+    }
+    // one launch per pass over every bit depth (shapes tuned again for it)
+    // against one launch per bit depth on parallel branches: whole-batch time
+    bool several = false;
+    for (int p = 0; p < kPasses; p++) {
+        int groups = 0;
+        for (int g = 0; g < kDepths; g++) groups += b->hb.total[p][g] > 0;
+        several = several || groups > 1;
+    }
+    if (several) {
+        auto whole = [&](const Rav2dBatchConfig &c, double &ms_out) -> int {
+            b->cfg = c;
+            int e = enqueue(b, s, -1, false);  // warm-up
+            if (e) return e;
+            ms_out = 0;
+            for (int it = 0; it < iters; it++) {
+                HIP_TRY(hipEventRecord(b->ev_a, s));
+                e = enqueue(b, s, -1, false);
+                if (e) return e;
+                HIP_TRY(hipEventRecord(b->ev_b, s));
+                HIP_TRY(hipEventSynchronize(b->ev_b));
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, b->ev_a, b->ev_b));
+                ms_out += ms;
+            }
+            return RAV2D_OK;
+        };
+        const Rav2dBatchConfig split = best;
+        best.mixed_depths = 1;
+        for (int p = 0; p < kPasses; p++) {
+            if (p == RAV2D_PASS_GRAIN && !b->cfg.apply_grain) continue;
+            double t = 0;
+            if ((r = tune_pass(p, t))) return r;
+        }
+        const Rav2dBatchConfig mixed = best;
+        double t_split = 0, t_mixed = 0;
+        if ((r = whole(split, t_split)) || (r = whole(mixed, t_mixed))) return r;
+        best = t_mixed < t_split ? mixed : split;
     }
     HIP_TRY(hipStreamSynchronize(s));
     b->cfg = best;

@@ -621,6 +621,31 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
     });
 }
 
+// Mixed launch: the superblocks of every bit-depth group's frames.
+template <bool kLoop>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void cdef_mix(DevArgs3<CdefArgs> args,
+                                                                                      MixSched s) {
+    __shared__ CdefSmem sm;
+    for_each_mixed_item<kLoop>(s, [&](int g, int f, int t) {
+        const CdefArgs &a = args.p[g][f];
+        const int sbw = (a.g.bw * 4 + 63) >> 6;
+        const int sby = t / sbw, sbx = t - sby * sbw;
+        if (g == 0)
+            cdef_body<uint8_t>(a, sbx, sby, sm);
+        else
+            cdef_body<uint16_t>(a, sbx, sby, sm);
+        __syncthreads();
+    });
+}
+
+void launch_cdef_mix(DevArgs3<CdefArgs> args, const MixSched &s, int grid, hipStream_t st) {
+    if (s.first[3] <= 0) return;
+    if (grid < s.first[3])
+        R2G_LAUNCH((cdef_mix<true>), dim3(grid), dim3(kThreads), 0, st, args, s);
+    else
+        R2G_LAUNCH((cdef_mix<false>), dim3(grid), dim3(kThreads), 0, st, args, s);
+}
+
 int cdef_items(const CdefArgs &a) { return ((a.g.bw * 4 + 63) >> 6) * ((a.g.bh * 4 + 63) >> 6); }
 
 // Per device: the direction offsets as (dy, dx) and the CCSO offset table.

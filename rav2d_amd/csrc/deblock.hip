@@ -441,6 +441,74 @@ __global__ __launch_bounds__(kThreads) void deblock_h_batch(const DeblockArgs *_
     });
 }
 
+// Mixed launches: the tiles of every bit-depth group's frames in one grid;
+// group 0 (8-bit) runs the 8-bit body, groups 1 and 2 the 16-bit one.
+template <bool kVert, bool kLoop>
+__global__ __launch_bounds__(kThreads) void deblock_mix(DevArgs3<DeblockArgs> args, MixSched s) {
+    __shared__ __align__(16) uint16_t tile[kVert ? kTH : kTH + 2 * kHalo][kVert ? kTW + 2 * kHalo : kTW];
+    __shared__ __align__(16) uint16_t lut[2][512];
+    for_each_mixed_item<kLoop>(s, [&](int g, int f, int t) {
+        if constexpr (kVert) {
+            if (g == 0)
+                deblock_v_body<uint8_t>(args.p[0][f], t, tile, lut);
+            else
+                deblock_v_body<uint16_t>(args.p[g][f], t, tile, lut);
+        } else {
+            if (g == 0)
+                deblock_h_body<uint8_t>(args.p[0][f], t, tile, lut);
+            else
+                deblock_h_body<uint16_t>(args.p[g][f], t, tile, lut);
+        }
+        __syncthreads();
+    });
+}
+
+template <int TH, bool kLoop>
+__global__ __launch_bounds__(kThreads) void deblock_vh_mix(DevArgs3<DeblockVHArgs> args, MixSched s) {
+    __shared__ __align__(16) VHTile<TH> sm;
+    for_each_mixed_item<kLoop>(s, [&](int g, int f, int t) {
+        if (g == 0)
+            deblock_vh_body<uint8_t, TH>(args.p[0][f], t, sm);
+        else
+            deblock_vh_body<uint16_t, TH>(args.p[g][f], t, sm);
+        __syncthreads();
+    });
+}
+
+void launch_deblock_mix(bool vert, DevArgs3<DeblockArgs> args, const MixSched &s, int grid, hipStream_t st) {
+    if (s.first[3] <= 0) return;
+    const bool loop = grid < s.first[3];
+    const dim3 G(grid), B(kThreads);
+    if (vert) {
+        if (loop)
+            R2G_LAUNCH((deblock_mix<true, true>), G, B, 0, st, args, s);
+        else
+            R2G_LAUNCH((deblock_mix<true, false>), G, B, 0, st, args, s);
+    } else {
+        if (loop)
+            R2G_LAUNCH((deblock_mix<false, true>), G, B, 0, st, args, s);
+        else
+            R2G_LAUNCH((deblock_mix<false, false>), G, B, 0, st, args, s);
+    }
+}
+
+void launch_deblock_vh_mix(DevArgs3<DeblockVHArgs> args, const MixSched &s, int grid, int rows, hipStream_t st) {
+    if (s.first[3] <= 0) return;
+    const bool loop = grid < s.first[3];
+    const dim3 G(grid), B(kThreads);
+    if (rows == 64) {
+        if (loop)
+            R2G_LAUNCH((deblock_vh_mix<64, true>), G, B, 0, st, args, s);
+        else
+            R2G_LAUNCH((deblock_vh_mix<64, false>), G, B, 0, st, args, s);
+    } else {
+        if (loop)
+            R2G_LAUNCH((deblock_vh_mix<32, true>), G, B, 0, st, args, s);
+        else
+            R2G_LAUNCH((deblock_vh_mix<32, false>), G, B, 0, st, args, s);
+    }
+}
+
 void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, int bitdepth,
                           hipStream_t st) {
     if (s.total <= 0) return;

@@ -154,4 +154,34 @@ template <bool kLoop, typename F> __device__ __forceinline__ void for_each_item(
     }
 }
 
+// Schedule of one launch over the frames of every bit-depth group (8-, 10-
+// and 12-bit): group g's items follow those of the groups before it, from
+// item first[g]; g[i] is group i's own schedule (prefix of its frames).
+struct MixSched {
+    BatchSched g[3];
+    int32_t first[4];  // first[3] = total
+    int32_t xcd_remap;
+};
+
+// Visits the items of workgroup blockIdx.x of a mixed launch in order:
+// fn(group, frame, local item), all workgroup-uniform.
+template <bool kLoop, typename F> __device__ __forceinline__ void for_each_mixed_item(const MixSched &s, F fn) {
+    const int G = gridDim.x, b = blockIdx.x, total = s.first[3];
+    int hi = total, step = G, first = b;
+    if (s.xcd_remap && (G & 7) == 0) {
+        const int per_xcd = G >> 3, xcd = b & 7, slot = b >> 3;
+        const int chunk = (total + 7) >> 3;
+        first = xcd * chunk + slot;
+        hi = min(total, xcd * chunk + chunk);
+        step = per_xcd;
+    }
+    for (int it = first; it < hi; it += step) {
+        const int g = __builtin_amdgcn_readfirstlane(it >= s.first[2] ? 2 : (it >= s.first[1] ? 1 : 0));
+        const int li = it - s.first[g];
+        const int f = __builtin_amdgcn_readfirstlane(find_frame(s.g[g].prefix, s.g[g].n_frames, li));
+        fn(g, f, __builtin_amdgcn_readfirstlane(li - s.g[g].prefix[f]));
+        if (!kLoop) break;
+    }
+}
+
 }  // namespace r2g

@@ -666,6 +666,29 @@ void launch_grain_setup_batch(const GrainArgs *d_args, int n, hipStream_t s) {
     R2G_LAUNCH(grain_setup_batch, dim3(n), dim3(kSetupThreads), 0, s, d_args, gauss);
 }
 
+// Mixed launch: the tiles of every bit-depth group's frames.
+template <bool kLoop>
+__global__ __launch_bounds__(256) void grain_apply_mix(DevArgs3<GrainArgs> args, MixSched s) {
+    __shared__ __align__(4) uint8_t scl[1024 * scl_words<12>()];
+    for_each_mixed_item<kLoop>(s, [&](int g, int f, int t) {
+        if (g == 0)
+            grain_apply_body<uint8_t, 8>(args.p[0][f], t, scl);
+        else if (g == 1)
+            grain_apply_body<uint16_t, 10>(args.p[1][f], t, scl);
+        else
+            grain_apply_body<uint16_t, 12>(args.p[2][f], t, scl);
+        if (kLoop) __syncthreads();  // the next item restages scl
+    });
+}
+
+void launch_grain_apply_mix(DevArgs3<GrainArgs> args, const MixSched &s, int grid, hipStream_t st) {
+    if (s.first[3] <= 0) return;
+    if (grid < s.first[3])
+        R2G_LAUNCH((grain_apply_mix<true>), dim3(grid), dim3(256), 0, st, args, s);
+    else
+        R2G_LAUNCH((grain_apply_mix<false>), dim3(grid), dim3(256), 0, st, args, s);
+}
+
 void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &sc, int grid, int bitdepth, hipStream_t s) {
     if (sc.total <= 0) return;
     const bool loop = grid < sc.total;

@@ -10,6 +10,12 @@
 
 namespace r2g {
 
+// Argument arrays of a mixed launch: frame argument blocks of each bit-depth
+// group (8-, 10-, 12-bit) in device memory.
+template <typename T> struct DevArgs3 {
+    const T *p[3];
+};
+
 // Per-frame deblock threshold LUT: [kind][qidx] -> (q_thr, side_thr); kinds
 // 0 = luma vertical edges, 1 = luma horizontal edges, 2 = U, 3 = V.
 // init_deblock_thr_lut_y/uv (crates/rav2d/src/deblock.rs:13-57) evaluated for
@@ -165,6 +171,12 @@ TileMap deblock_vh_tile_map(const DeblockArgs &a, int rows);
 void launch_deblock_vh(const DeblockVHArgs &a, int bitdepth, int rows, hipStream_t s);
 void launch_deblock_vh_batch(const DeblockVHArgs *d_args, const BatchSched &s, int grid, int bitdepth, int rows,
                              hipStream_t st);
+// One launch over the frames of every bit-depth group (MixSched).
+void launch_deblock_mix(bool vert, DevArgs3<DeblockArgs> args, const MixSched &s, int grid, hipStream_t st);
+void launch_deblock_vh_mix(DevArgs3<DeblockVHArgs> args, const MixSched &s, int grid, int rows, hipStream_t st);
+void launch_cdef_mix(DevArgs3<CdefArgs> args, const MixSched &s, int grid, hipStream_t st);
+void launch_lr_mix(DevArgs3<LrArgs> args, const MixSched &s, int grid, hipStream_t st);
+void launch_grain_apply_mix(DevArgs3<GrainArgs> args, const MixSched &s, int grid, hipStream_t st);
 void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, int bitdepth,
                           hipStream_t st);
 // 64x32 tiles of the three planes (the vertical-edge pass: down to the padding rows)

@@ -641,6 +641,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 8))
     });
 }
 
+// Mixed launch: the items of every bit-depth group's frames (8-, 10-, 12-bit bodies).
+template <bool kLoop>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 8))) void lr_mix(DevArgs3<LrArgs> args,
+                                                                                    MixSched s) {
+    __shared__ LrSmem sm;
+    for_each_mixed_item<kLoop>(s, [&](int g, int f, int t) {
+        if (g == 0)
+            lr_body<uint8_t, 8>(const_arg(args.p[0], f), t, sm);
+        else if (g == 1)
+            lr_body<uint16_t, 10>(const_arg(args.p[1], f), t, sm);
+        else
+            lr_body<uint16_t, 12>(const_arg(args.p[2], f), t, sm);
+        __syncthreads();
+    });
+}
+
+void launch_lr_mix(DevArgs3<LrArgs> args, const MixSched &s, int grid, hipStream_t st) {
+    if (s.first[3] <= 0) return;
+    if (grid < s.first[3])
+        R2G_LAUNCH((lr_mix<true>), dim3(grid), dim3(kThreads), 0, st, args, s);
+    else
+        R2G_LAUNCH((lr_mix<false>), dim3(grid), dim3(kThreads), 0, st, args, s);
+}
+
 void launch_lr_batch(const LrArgs *d_args, const BatchSched &sc, int grid, int bitdepth, hipStream_t s) {
     if (sc.total <= 0) return;
     const bool loop = grid < sc.total;

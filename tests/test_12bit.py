@@ -90,13 +90,23 @@ def test_12bit_chain_matches_oracle(ctx, case):
         sf.close()
 
 
+# batch configurations: (mixed_depths, deblock_rows, grid, xcd_remap); mixed_depths 1 runs
+# one launch per pass over the three bit depths
+BATCH_CONFIGS = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 64, 0, 1), (1, 32, 40, 1), (0, 64, 24, 0)]
+
+
 @pytest.mark.gpu
-def test_batch_mixes_8_10_and_12bit(ctx):
+@pytest.mark.parametrize("conf", BATCH_CONFIGS, ids=lambda c: "mixed%d_dbrows%d_grid%d_xcd%d" % c)
+def test_batch_mixes_8_10_and_12bit(ctx, conf):
     spec = [(640, 360, 8, 0, 91), (352, 288, 12, 1, 92), (640, 360, 10, 1, 93), (264, 136, 12, 0, 94),
             (1280, 720, 12, 0, 95)]
     sfs = [R.SynthFrame(w, h, bd, sb, s) for w, h, bd, sb, s in spec]
     frames = [R.GpuFrame.from_synth(ctx, sf) for sf in sfs]
-    b = R.GpuBatch(ctx, frames)
+    cfg = R.default_batch_config()
+    cfg.mixed_depths, cfg.deblock_rows = conf[0], conf[1]
+    for p in range(R.NUM_PASSES):
+        cfg.grid[p], cfg.xcd_remap[p] = conf[2], conf[3]
+    b = R.GpuBatch(ctx, frames, cfg)
     try:
         for _ in range(2):
             b.run()

@@ -48,24 +48,27 @@ def _check_all(frames, refs, tag):
         _compare(ref_disp, f.download(2), "%s frame %d %s (display)" % (tag, i, FRAMES[i][:3]))
 
 
-def _config(use_graph, grid, xcd, deblock_rows=0):
+def _config(use_graph, grid, xcd, deblock_rows=0, mixed=0):
     cfg = R.default_batch_config()
     cfg.use_graph = use_graph
     for p in range(R.NUM_PASSES):
         cfg.grid[p] = grid
         cfg.xcd_remap[p] = xcd
     cfg.deblock_rows = deblock_rows
+    cfg.mixed_depths = mixed
     return cfg
 
 
-# (use_graph, grid, xcd_remap, deblock_rows): deblock_rows 32 / 64 runs both
-# deblocking passes as one tile pass
-SHAPES = [(1, 0, 0, 0), (1, 0, 1, 0), (1, 64, 0, 0), (1, 24, 1, 0), (0, 0, 0, 0), (0, 7, 1, 0),
-          (1, 0, 0, 32), (1, 0, 1, 64), (0, 24, 1, 32), (1, 64, 0, 64)]
+# (use_graph, grid, xcd_remap, deblock_rows, mixed_depths): deblock_rows 32 / 64
+# runs both deblocking passes as one tile pass; mixed_depths 1 one launch per
+# pass over the 8- and 10-bit frames
+SHAPES = [(1, 0, 0, 0, 0), (1, 0, 1, 0, 0), (1, 64, 0, 0, 0), (1, 24, 1, 0, 0), (0, 0, 0, 0, 0), (0, 7, 1, 0, 0),
+          (1, 0, 0, 32, 0), (1, 0, 1, 64, 0), (0, 24, 1, 32, 0), (1, 64, 0, 64, 0),
+          (1, 0, 0, 0, 1), (1, 0, 1, 64, 1), (0, 24, 1, 32, 1), (1, 7, 0, 0, 1)]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "graph%d_grid%d_xcd%d_dbrows%d" % s)
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "graph%d_grid%d_xcd%d_dbrows%d_mixed%d" % s)
 def test_batch_matches_oracle(ctx, synth, shape):
     sfs, refs = synth
     frames = [R.GpuFrame.from_synth(ctx, sf) for sf in sfs]
