@@ -54,17 +54,17 @@ KERNEL_NAMES = ["deblock_v_kernel<u16>", "deblock_h_kernel<u16>", "cdef_kernel<u
                 "grain_apply_kernel<u16>"]
 
 # Per-launch counters of the 4k10 workload's kernels from rocprofv3 --pmc runs
-# of `bench.py --steps 3 --warmup 1 --no-cpu --batch 0 --serve-jobs 0`
+# of `bench.py --steps 5 --warmup 2 --no-cpu --batch 0 --serve-jobs 0`
 # (profiles/hbm_traffic_4k10_pmc.md): FETCH_SIZE and WRITE_SIZE in KiB per
 # dispatch and SQ_INSTS_VALU in wave instructions.  HBM bytes = FETCH x 2.0 +
 # WRITE x 1.0 KiB: the factors make the copy probe's counters equal its known
 # 256 MiB read and 256 MiB written.  Valid for the kernels profiled there.
 PMC_4K10 = {
-    "deblock_v": (37637.2, 24300.0, 8.84e6),
-    "deblock_h": (26609.6, 24300.0, 8.37e6),
-    "cdef_ccso": (14787.1, 24801.0, 19.27e6),
-    "loop_restoration": (46017.7, 24300.0, 23.47e6),
-    "film_grain": (19939.8, 24301.5, 7.24e6),
+    "deblock_v": (37574.0, 24420.0, 8.523e6),
+    "deblock_h": (26399.5, 24300.0, 8.156e6),
+    "cdef_ccso": (14863.0, 24812.8, 19.56e6),
+    "loop_restoration": (45991.4, 24300.0, 23.47e6),
+    "film_grain": (19949.7, 24301.9, 6.759e6),
 }
 
 
@@ -201,6 +201,21 @@ def bench_batch(ctx, spec, sfs, steps, warmup):
     for _ in range(steps):
         per_frame()
     pf_ms = ctx.timer_stop() / steps
+    # the per-frame form by frame size: each size's frames launched one by one
+    per_size = {}
+    for size in BATCH_SIZES:
+        idx = [i for i, c in enumerate(spec) if (c[0], c[1]) == size]
+        if not idx:
+            continue
+        ctx.synchronize()
+        ctx.timer_start()
+        for _ in range(steps):
+            for i in idx:
+                A[i].filter()
+                A[i].apply_grain()
+        ms = ctx.timer_stop() / steps
+        px = sum(spec[i][0] * spec[i][1] for i in idx)
+        per_size["%dx%d" % size] = {"frames": len(idx), "ms": round(ms, 3), "mpix_s": round(px / 1e6 / (ms / 1e3), 1)}
 
     b = R.GpuBatch(ctx, B)
     t0 = time.perf_counter()
@@ -241,6 +256,7 @@ def bench_batch(ctx, spec, sfs, steps, warmup):
         "grained_frames": sum(grain),
         "mpix_per_batch": round(pix / 1e6, 2),
         "per_frame_ms": round(pf_ms, 3),
+        "per_frame_by_size": per_size,
         "batched_ms": round(bt_ms, 3),
         "per_frame_mpix_s": round(pix / 1e6 / (pf_ms / 1e3), 1),
         "batched_mpix_s": round(pix / 1e6 / (bt_ms / 1e3), 1),
