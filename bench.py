@@ -13,7 +13,9 @@ launches and as one launch per pass (autotuned, hipGraph replay).  A third leg
 serves 10,000 jobs from 32 streams with Poisson arrivals (60 fps each) under an
 80 GB HBM budget through the continuous-batching server and reports p50/p99
 latency; every job's planes are hashed on the device and compared with the
-same frame filtered alone.
+same frame filtered alone.  A second serving run offers the same streams at
+5x the frame rate (300 fps each), where jobs arrive while batches run and the
+server forms larger batches.
 
 The CPU-baseline leg runs the C restatement under oracle/ on the same seeded
 inputs: the 4K frame (one thread, whole frames for ~12 s) and the 64 batch
@@ -334,6 +336,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-jobs", type=int, default=10000, help="jobs of the serving leg (0 = skip)")
     ap.add_argument("--serve-streams", type=int, default=32)
+    ap.add_argument("--serve-fps-high", type=int, default=300, help="frame rate per stream of the high-load serving run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the filter stage has no multi-GPU path); "
@@ -410,6 +413,10 @@ def main():
             print("bench: serving %d jobs from %d streams" % (args.serve_jobs, args.serve_streams),
                   file=sys.stderr, flush=True)
             serve = bench_serve(spec, sfs, args.serve_streams, 2, 60.0, args.serve_jobs, args.seed, 80 * 10**9)
+            # the same streams at 5x the frame rate: batches form while others run
+            print("bench: serving at %d fps per stream" % args.serve_fps_high, file=sys.stderr, flush=True)
+            serve["high_load"] = bench_serve(spec, sfs, args.serve_streams, 2, float(args.serve_fps_high),
+                                             args.serve_jobs, args.seed + 1, 80 * 10**9)
         for x in sfs:
             x.close()
 
