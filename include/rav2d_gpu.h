@@ -163,7 +163,10 @@ typedef struct Rav2dFrameParams {
     int32_t width;           /* frame width in luma samples            */
     int32_t height;          /* frame height in luma samples           */
     int32_t bitdepth;        /* 8 or 10                                 */
-    int32_t layout;          /* RAV2D_PIXEL_LAYOUT_*; I420 supported    */
+    int32_t layout;          /* RAV2D_PIXEL_LAYOUT_*: I420, or I400 (luma
+                                only: no chroma planes, pointers or
+                                strides needed; chroma side info and
+                                chroma parameters are ignored)           */
     int32_t sb128;           /* superblock size 128 (1) or 64 (0)       */
     uint32_t inloop_filters; /* RAV2D_INLOOPFILTER_* mask               */
     /* sequence header */
@@ -328,7 +331,8 @@ int rav2d_gpu_apply_grain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, void *stream);
  * A PicAllocator backed by device memory (picture.rs:38-41, 77-125): pictures
  * in the library's plane layout (rav2d_gpu_frame_planes' strides for the same
  * size and depth: rows padded to 256 B, +256 B when the stride is a multiple
- * of 1 KiB, 8 spare rows), one allocation against the HBM budget.  With them
+ * of 1 KiB, 8 spare rows), one allocation against the HBM budget.  I400
+ * pictures have one plane (data[1], data[2] and stride[1] are 0).  With them
  * the decoder's reconstruction, the in-loop filters and film grain share
  * device pictures and the copies at decode.rs:4549 and :4681 go away. */
 int rav2d_gpu_picture_alloc(Rav2dGpuCtx *ctx, int width, int height, int bitdepth, int layout, Rav2dPlanes *out);

@@ -305,8 +305,10 @@ static void backup2x8(P (*dst)[2], const P *src, ptrdiff_t base, ptrdiff_t strid
 
 /* cdef.rs:611-645 (positive strides) */
 template <typename P>
-static void backup2lines_bank(std::vector<P> bank[3], const Frame<P> &f, ptrdiff_t y_off, ptrdiff_t uv_off) {
+static void backup2lines_bank(std::vector<P> bank[3], const Frame<P> &f, ptrdiff_t y_off, ptrdiff_t uv_off,
+                              bool have_chroma) {
     for (ptrdiff_t i = 0; i < 2 * f.ys; i++) bank[0][i] = f.y[y_off + 6 * f.ys + i];
+    if (!have_chroma) return;  // cdef.rs:525 (I400)
     ptrdiff_t s = uv_off + 2 * f.uvs;  // 4:2:0: rows 2,3 of the 4-row chroma band
     for (ptrdiff_t i = 0; i < 2 * f.uvs; i++) {
         bank[1][i] = f.u[s + i];
@@ -335,7 +337,7 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
         int by_idx = (by & 0x3e) >> 1;
         int edges = edge_top | CDEF_HAVE_BOTTOM;
         if (by + 2 >= p.bh) edges &= ~CDEF_HAVE_BOTTOM;
-        if (edges & CDEF_HAVE_BOTTOM) backup2lines_bank(L.bank[1 - tf], f, row_y, row_uv);
+        if (edges & CDEF_HAVE_BOTTOM) backup2lines_bank(L.bank[1 - tf], f, row_y, row_uv, p.have_chroma);
         P lr_bak[2][3][8][2];
         memset(lr_bak, 0, sizeof(lr_bak));
         int bit = 0;
@@ -358,13 +360,14 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
                 prev_flag |= flag;
                 if (do_left && (edges & CDEF_HAVE_LEFT)) {
                     if (do_left & BACKUP_2X8_Y) backup2x8(lr_bak[bit][0], f.y, sb_y, y_ls, 0, 8);
-                    if (do_left & BACKUP_2X8_UV) {
+                    if (p.have_chroma && (do_left & BACKUP_2X8_UV)) {
                         backup2x8(lr_bak[bit][1], f.u, sb_uv, uv_ls, 0, 8 >> ss_ver);
                         backup2x8(lr_bak[bit][2], f.v, sb_uv, uv_ls, 0, 8 >> ss_ver);
                     }
                 }
                 for (int pl = 0; pl < 3; pl++) {
-                    if (!ccm[pl]) continue;
+                    // a monochrome stream signals no chroma CCSO (its planes do not exist)
+                    if (!ccm[pl] || (pl && !p.have_chroma)) continue;
                     const CcsoCfg &cfg = p.ccso[pl];
                     int phor = pl ? ss_hor : 0, pver = pl ? ss_ver : 0;
                     int sb_edges = edges;
@@ -421,7 +424,7 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
                     prev_flag = flag;
                     if (do_left && (edges & CDEF_HAVE_LEFT)) {
                         if (do_left & BACKUP_2X8_Y) backup2x8(lr_bak[bit][0], f.y, b_y, y_ls, 0, 8);
-                        if (do_left & BACKUP_2X8_UV) {
+                        if (p.have_chroma && (do_left & BACKUP_2X8_UV)) {
                             backup2x8(lr_bak[bit][1], f.u, b_uv, uv_ls, 0, 8 >> ss_ver);
                             backup2x8(lr_bak[bit][2], f.v, b_uv, uv_ls, 0, 8 >> ss_ver);
                         }
@@ -429,7 +432,7 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
                     if (edges & CDEF_HAVE_RIGHT) {
                         int other = 1 - bit;
                         if (flag & BACKUP_2X8_Y) backup2x8(lr_bak[other][0], f.y, b_y, y_ls, 8, 8);
-                        if (flag & BACKUP_2X8_UV) {
+                        if (p.have_chroma && (flag & BACKUP_2X8_UV)) {
                             backup2x8(lr_bak[other][1], f.u, b_uv, uv_ls, 8 >> ss_hor, 8 >> ss_ver);
                             backup2x8(lr_bak[other][2], f.v, b_uv, uv_ls, 8 >> ss_hor, 8 >> ss_ver);
                         }
@@ -447,7 +450,7 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
                         cdef_filter_block(f.y, y_ls, b_y, lr_bak[bit][0], L.bank[tf][0].data(), top_col, f.y, bot_y, 0,
                                           y_sec, 0, damping, 8, 8, edges, bdm8);
                     }
-                    if (uv_lvl && !uv_lossless) {
+                    if (uv_lvl && p.have_chroma && !uv_lossless) {
                         int uvdir = uv_pri ? UV_DIRS_420[dir] : 0;
                         int cw = 8 >> ss_hor, ch = 8 >> ss_ver;
                         ptrdiff_t top_col_uv = ((ptrdiff_t)bx * 4) >> ss_hor;
@@ -474,14 +477,14 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
                 int do_right = flag & ~prev_flag;
                 if (do_right && (sbx + 1) * sbsz < p.bw) {
                     if (do_right & BACKUP_2X8_Y) backup2x8(lr_bak[bit][0], f.y, sb_y, y_ls, sbsz * 4, 8);
-                    if (do_right & BACKUP_2X8_UV) {
+                    if (p.have_chroma && (do_right & BACKUP_2X8_UV)) {
                         backup2x8(lr_bak[bit][1], f.u, sb_uv, uv_ls, (sbsz * 4) >> ss_hor, 8 >> ss_ver);
                         backup2x8(lr_bak[bit][2], f.v, sb_uv, uv_ls, (sbsz * 4) >> ss_hor, 8 >> ss_ver);
                     }
                     prev_flag |= do_right;
                 }
                 for (int pl = 0; pl < 3; pl++) {
-                    if (!ccm[pl]) continue;
+                    if (!ccm[pl] || (pl && !p.have_chroma)) continue;
                     const CcsoCfg &cfg = p.ccso[pl];
                     int phor = pl ? ss_hor : 0, pver = pl ? ss_ver : 0;
                     int w_full = imin(sbsz, p.bw - sbx * sbsz) * 4;

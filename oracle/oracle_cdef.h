@@ -20,6 +20,7 @@ struct CdefBrowParams {
     int bw, bh, damping;
     bool on_skip_tx, cdef_on, ccso_on;
     bool any_lossless; /* a segment is lossless: the masks' lossless 4x4 units are skipped */
+    bool have_chroma; /* false for I400 (cdef.rs:728): chroma is neither backed up nor filtered */
     const uint8_t *y_strength, *uv_strength;
     const Rav2dAv2Filter *mask;
     size_t n_mask, row;

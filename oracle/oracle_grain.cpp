@@ -306,11 +306,13 @@ static void fguv(P *dst, const P *src, ptrdiff_t stride, const Rav2dFilmGrainDat
     }
 }
 
-/* filmgrain.rs:617-941 + decode.rs:4672-4775, 4:2:0, one thread */
+/* filmgrain.rs:617-941 + decode.rs:4672-4775, 4:2:0 or I400, one thread */
 template <typename P>
 static void apply_grain(const Rav2dFrameParams *p, P *dy, P *du, P *dv, const P *sy_, const P *su, const P *sv_,
                         ptrdiff_t ys, ptrdiff_t uvs) {
     const Rav2dFilmGrainData &d = p->grain;
+    /* I400: the chroma slices are empty, so apply_grain_row skips chroma (filmgrain.rs:745-780) */
+    const bool has_chroma = p->layout != RAV2D_PIXEL_LAYOUT_I400;
     const int bitdepth = p->bitdepth, w = p->width, h = p->height;
     const uint32_t seed = p->grain_seed;
     const int ssx = 1, ssy = 1;
@@ -345,7 +347,7 @@ static void apply_grain(const Rav2dFrameParams *p, P *dy, P *du, P *dv, const P 
         int cw = (w + ssx) >> ssx;
         ptrdiff_t uv_off = (ptrdiff_t)(row_start >> ssy) * uvs;
         ptrdiff_t l_off = (ptrdiff_t)row_start * ys;
-        for (int uv = 0; uv < 2; uv++) {
+        for (int uv = 0; uv < 2 && has_chroma; uv++) {
             const std::vector<uint8_t> &sc = d.chroma_scaling_from_luma ? scaling[0] : scaling[uv + 1];
             bool has = (d.num_points[uv + 1] > 0 || d.chroma_scaling_from_luma) && !sc.empty();
             if (!has) continue;
@@ -372,9 +374,10 @@ extern "C" int oracle_picture_has_grain(const Rav2dFrameParams *p) {
 extern "C" int oracle_apply_grain(const Rav2dFrameParams *p, const void *src_y, const void *src_u,
                                   const void *src_v, void *dst_y, void *dst_u, void *dst_v, ptrdiff_t y_stride,
                                   ptrdiff_t uv_stride) {
-    if (!p || p->layout != RAV2D_PIXEL_LAYOUT_I420) return RAV2D_EINVAL;
+    if (!p || (p->layout != RAV2D_PIXEL_LAYOUT_I420 && p->layout != RAV2D_PIXEL_LAYOUT_I400)) return RAV2D_EINVAL;
     int bps = p->bitdepth > 8 ? 2 : 1;
-    int cw = (p->width + 1) >> 1, ch = (p->height + 1) >> 1;
+    const bool mono = p->layout == RAV2D_PIXEL_LAYOUT_I400;
+    int cw = mono ? 0 : (p->width + 1) >> 1, ch = mono ? 0 : (p->height + 1) >> 1;
     /* the display copy starts as the ungrained picture */
     for (int y = 0; y < p->height; y++)
         memcpy((uint8_t *)dst_y + y * y_stride, (const uint8_t *)src_y + y * y_stride, (size_t)p->width * bps);

@@ -777,6 +777,7 @@ template <typename P> void copy_db(LrCtx<P> &c, const Frame<P> &f, int sby) {
     int y_stripe = (sby << (6 + g.sb128)) - offset;
     int row_h = imin((sby + 1) << (6 + g.sb128), h - 1);
     if (y_stripe < row_h) backup_db(c.lr_db[0].data(), f.y + y_stripe * f.ys, f.ys, 0, g.sb128, y_stripe, row_h, w);
+    if (g.layout == RAV2D_PIXEL_LAYOUT_I400) return;  // no chroma planes (zero chroma stride)
     int cw = w >> g.ss_hor, ch = h >> g.ss_ver;
     int ouv = offset >> g.ss_ver;
     int cy = (sby << ((6 - g.ss_ver) + g.sb128)) - ouv;
@@ -809,6 +810,8 @@ template <typename P> void lr_init(LrCtx<P> &c, const Rav2dFrameParams *p) {
     int has_u = p->restoration[1].restoration_type != RAV2D_RESTORATION_NONE;
     int has_v = p->restoration[2].restoration_type != RAV2D_RESTORATION_NONE;
     c.restore_planes = has_y | (has_u << 1) | (has_v << 2);
+    // without chroma planes only luma is restored (looprestoration.rs:2401-2409)
+    if (p->layout == RAV2D_PIXEL_LAYOUT_I400) c.restore_planes &= LR_RESTORE_Y;
 }
 
 template void lr_init<uint8_t>(LrCtx<uint8_t> &, const Rav2dFrameParams *);

@@ -24,6 +24,7 @@ def inject_failure(what, count):
 INLOOP_DEBLOCK, INLOOP_CDEF, INLOOP_CCSO, INLOOP_WIENER, INLOOP_GDF = 1, 2, 4, 8, 16
 INLOOP_ALL = 31
 PASS_DEBLOCK_V, PASS_DEBLOCK_H, PASS_CDEF, PASS_LR, PASS_GRAIN = range(5)
+PIXEL_LAYOUT_I400, PIXEL_LAYOUT_I420, PIXEL_LAYOUT_I422, PIXEL_LAYOUT_I444 = range(4)
 
 TOOL_DEBLOCK, TOOL_CDEF, TOOL_CCSO, TOOL_WIENER, TOOL_GDF, TOOL_GRAIN = 1, 2, 4, 8, 16, 32
 TOOL_ALL = 63
@@ -139,6 +140,7 @@ class SynthConfig(C.Structure):
         ("width", C.c_int32), ("height", C.c_int32), ("bitdepth", C.c_int32), ("sb128", C.c_int32),
         ("seed", C.c_uint64), ("tools", C.c_uint32), ("lossless_pct", C.c_int32),
         ("skip_pct", C.c_int32), ("pc_gdf_pct", C.c_int32), ("mask_flags", C.c_uint32),
+        ("monochrome", C.c_int32),
     ]
 
 
@@ -306,15 +308,16 @@ class SynthFrame:
     """A seeded synthetic reconstructed frame plus its filter side info."""
 
     def __init__(self, width, height, bitdepth=10, sb128=0, seed=1, tools=TOOL_ALL,
-                 lossless_pct=0, skip_pct=30, pc_gdf_pct=50, mask_flags=0):
+                 lossless_pct=0, skip_pct=30, pc_gdf_pct=50, mask_flags=0, monochrome=False):
         lib = load_synth()
         cfg = SynthConfig(width, height, bitdepth, sb128, seed, tools, lossless_pct, skip_pct, pc_gdf_pct,
-                          mask_flags)
+                          mask_flags, 1 if monochrome else 0)
         self._lib = lib
         self._h = lib.r2s_frame_create(C.byref(cfg))
         if not self._h:
             raise ValueError("invalid synthetic frame config")
         self.width, self.height, self.bitdepth = width, height, bitdepth
+        self.monochrome = bool(monochrome)
         self.params = lib.r2s_params(self._h).contents
         n = C.c_size_t()
         self.mask_ptr = lib.r2s_mask(self._h, C.byref(n))
@@ -338,6 +341,9 @@ class SynthFrame:
         for pl in range(3):
             h, w = self.plane_shape(pl)
             n = h * w * np.dtype(self.dtype).itemsize
+            if n == 0:  # no chroma planes (monochrome)
+                out.append(np.zeros((h, w), dtype=self.dtype))
+                continue
             buf = (C.c_uint8 * n).from_address(self.plane_ptr(pl))
             out.append(np.frombuffer(buf, dtype=self.dtype).reshape(h, w).copy())
         return out
@@ -355,6 +361,9 @@ class SynthFrame:
         out = []
         for pl in range(3):
             h, w = self.plane_shape(pl)
+            if h * w == 0:  # no chroma planes (monochrome)
+                out.append(np.zeros((0, 0), dtype=self.dtype))
+                continue
             st = C.c_ssize_t()
             ptr = self._lib.r2s_plane_padded(self._h, pl, C.byref(st))
             isz = np.dtype(self.dtype).itemsize
@@ -465,6 +474,7 @@ class Picture:
     def __init__(self, ctx, width, height, bitdepth, layout=1):
         self.ctx, self.lib = ctx, ctx.lib
         self.width, self.height, self.bitdepth = width, height, bitdepth
+        self.layout = layout
         self.planes = Planes()
         _check(self.lib.rav2d_gpu_picture_alloc(ctx.h, width, height, bitdepth, layout, C.byref(self.planes)),
                "picture_alloc")
@@ -474,13 +484,19 @@ class Picture:
         return np.uint16 if self.bitdepth > 8 else np.uint8
 
     def _shape(self, pl):
-        return (self.height, self.width) if pl == 0 else ((self.height + 1) // 2, (self.width + 1) // 2)
+        if pl == 0:
+            return (self.height, self.width)
+        if self.layout == PIXEL_LAYOUT_I400:
+            return (0, 0)
+        return ((self.height + 1) // 2, (self.width + 1) // 2)
 
     def write(self, planes):
         """Upload three numpy planes (rows of any stride are packed here); planes
         with picture padding (up to PICTURE_PAD more rows and columns) fill the
-        picture's padding too."""
+        picture's padding too.  Monochrome pictures take the luma plane only."""
         for pl, a in enumerate(planes):
+            if not self.planes.data[pl]:
+                continue
             h, w = self._shape(pl)
             s = self.planes.stride[1 if pl else 0]
             cols = s // np.dtype(self.dtype).itemsize
@@ -494,6 +510,9 @@ class Picture:
         out = []
         for pl in range(3):
             h, w = self._shape(pl)
+            if not self.planes.data[pl]:
+                out.append(np.zeros((h, w), dtype=self.dtype))
+                continue
             s = self.planes.stride[1 if pl else 0]
             buf = np.empty((h, s // np.dtype(self.dtype).itemsize), dtype=self.dtype)
             _check(self.lib.rav2d_gpu_copy(self.ctx.h, buf.ctypes.data, self.planes.data[pl], buf.nbytes, None),
@@ -531,6 +550,7 @@ class GpuFrame:
         _check(self.lib.rav2d_gpu_frame_create(ctx.h, C.byref(params), C.byref(h)), "frame_create")
         self.h = h
         self.width, self.height, self.bitdepth = params.width, params.height, params.bitdepth
+        self.monochrome = params.layout == PIXEL_LAYOUT_I400
 
     @classmethod
     def from_synth(cls, ctx, sf):
@@ -544,7 +564,7 @@ class GpuFrame:
     def upload_padded(self, sf):
         """The SynthFrame's planes with their picture padding (8 rows and columns)."""
         src = Planes()
-        for pl in range(3):
+        for pl in range(1 if self.monochrome else 3):
             st = C.c_ssize_t()
             src.data[pl] = sf._lib.r2s_plane_padded(sf._h, pl, C.byref(st))
             src.stride[1 if pl else 0] = st.value
@@ -574,11 +594,10 @@ class GpuFrame:
 
     def download(self, which=1):
         dt = np.uint16 if self.bitdepth > 8 else np.uint8
-        shapes = [(self.height, self.width), (self.height // 2, self.width // 2),
-                  (self.height // 2, self.width // 2)]
-        arrs = [np.empty(s, dtype=dt) for s in shapes]
-        _check(self.lib.rav2d_gpu_frame_download(self.h, which, arrs[0].ctypes.data, arrs[1].ctypes.data,
-                                                 arrs[2].ctypes.data, None), "download")
+        c = (0, 0) if self.monochrome else (self.height // 2, self.width // 2)
+        arrs = [np.empty(s, dtype=dt) for s in [(self.height, self.width), c, c]]
+        ptrs = [a.ctypes.data if a.size else None for a in arrs]
+        _check(self.lib.rav2d_gpu_frame_download(self.h, which, ptrs[0], ptrs[1], ptrs[2], None), "download")
         return arrs
 
     def checksum(self, which=1):

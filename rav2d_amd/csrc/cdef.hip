@@ -357,7 +357,9 @@ __device__ __forceinline__ void cdef_load(const CdefArgs &a, int sbx, int sby, C
         const int pl = i >= LCR * kQC;
         const int k = i - pl * LCR * kQC;
         const int r = k / kQC, gx = (x0 >> 1) - CX + (k - r * kQC) * 4, gy = (y0 >> 1) - 2 + r;
-        const uint2 v = load4(row_ptr<const P>(a.src[1 + (pl & 1)], iclip(gy, 0, ch - 1)) + iclip(gx, 0, cw - 4));
+        uint2 v = make_uint2(kOutside, kOutside);
+        if (a.has_chroma)  // monochrome frames have no chroma planes
+            v = load4(row_ptr<const P>(a.src[1 + (pl & 1)], iclip(gy, 0, ch - 1)) + iclip(gx, 0, cw - 4));
         const bool in = gx >= 0 && gx < cw && gy >= 0 && gy < ch;
         L.vc[j].x = in ? v.x : kOutside;
         L.vc[j].y = in ? v.y : kOutside;
@@ -557,7 +559,7 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
             os1[k] = sm.off[1][dir][k];
         }
         const bool cc = a.ccso_on && ((ccso_pl >> (1 + pl)) & 1);
-        if (gx < cw) {
+        if (a.has_chroma && gx < cw) {
 #pragma unroll 2
             for (int yy = by; yy < by + 4; yy++) {
                 const int gy = (y0 >> 1) + yy;

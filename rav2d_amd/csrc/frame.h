@@ -159,6 +159,8 @@ struct ChainPlan {
 
 int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp);
 DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl);
+// Planes of a picture of the given layout: 1 for monochrome (I400), else 3.
+inline int n_planes(int layout) { return layout == RAV2D_PIXEL_LAYOUT_I400 ? 1 : 3; }
 // Samples of picture padding right of each plane of a width-w picture: blocks
 // can overhang the right edge only when w is not a multiple of 128 (the
 // largest superblock).

@@ -62,6 +62,7 @@ struct CdefArgs {
     const int32_t *order;  // superblock of each launch slot (null: raster order)
     int32_t cdef_on, ccso_on, on_skiptx, damping;
     int32_t any_ll;  // a segment is lossless: CDEF and CCSO skip the masks' lossless 4x4 units
+    int32_t has_chroma;  // 0: monochrome (I400), no chroma planes to stage or filter
     alignas(8) uint8_t y_strength[8];
     alignas(8) uint8_t uv_strength[8];
     CcsoDev ccso[3];

@@ -115,6 +115,8 @@ void lr_frame_state(const Rav2dFrameParams &p, int &base_q, int &widx, int &ns_i
     int has_u = p.restoration[1].restoration_type != RAV2D_RESTORATION_NONE;
     int has_v = p.restoration[2].restoration_type != RAV2D_RESTORATION_NONE;
     restore_planes = has_y | (has_u << 1) | (has_v << 2);
+    // without chroma planes only luma is restored (looprestoration.rs:2401-2409)
+    if (p.layout == RAV2D_PIXEL_LAYOUT_I400) restore_planes &= 1;
 }
 
 // The stripe walk of lr_stripe (looprestoration.rs:1847-2274) for one unit
@@ -452,6 +454,8 @@ int grain_args(Rav2dGpuFrame *f, GrainArgs &a) {
         a.src[pl] = plane_of(f, BUF_OUT, pl);
         a.dst[pl] = plane_of(f, BUF_DISP, pl);
     }
+    // I400: no chroma planes to grain (decode.rs:4693, filmgrain.rs:745-780)
+    if (P.layout == RAV2D_PIXEL_LAYOUT_I400) a.plane_on[1] = a.plane_on[2] = 0;
     return RAV2D_OK;
 }
 
@@ -528,8 +532,8 @@ int validate(const Rav2dFrameParams *p) {
         set_last_error("bit depth must be 8, 10 or 12");
         return RAV2D_EINVAL;
     }
-    if (p->layout != RAV2D_PIXEL_LAYOUT_I420) {
-        set_last_error("only 4:2:0 is supported");
+    if (p->layout != RAV2D_PIXEL_LAYOUT_I420 && p->layout != RAV2D_PIXEL_LAYOUT_I400) {
+        set_last_error("only 4:2:0 and monochrome (I400) are supported");
         return RAV2D_EINVAL;
     }
     if (p->sb128 != 0 && p->sb128 != 1) {
@@ -557,6 +561,7 @@ DevPlane plane_of(const Rav2dGpuFrame *f, int b, int pl) {
 namespace {
 
 int copy_plane(Rav2dGpuFrame *f, int from, int to, int pl, hipStream_t s) {
+    if (!f->pw[pl]) return RAV2D_OK;  // a plane the layout does not have
     HIP_TRY(hipMemcpy2DAsync(plane_of(f, to, pl).ptr, f->stride[pl], plane_of(f, from, pl).ptr, f->stride[pl],
                              (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyDeviceToDevice, s));
     return RAV2D_OK;
@@ -567,13 +572,20 @@ int copy_plane(Rav2dGpuFrame *f, int from, int to, int pl, hipStream_t s) {
 // samples right of it (the picture padding the deblocking reads), rows padded
 // to 256 B (+256 B when the stride is a multiple of 1 KiB, keeping rows off
 // one channel).
-void picture_layout(int w, int h, int bps, int64_t stride[3], size_t bytes[3], int pw[3], int ph[3]) {
+// Monochrome (I400) pictures have no chroma planes: size, stride and bytes 0.
+void picture_layout(int w, int h, int bps, int layout, int64_t stride[3], size_t bytes[3], int pw[3], int ph[3]) {
     pw[0] = w;
     ph[0] = h;
     pw[1] = pw[2] = (w + 1) >> 1;
     ph[1] = ph[2] = (h + 1) >> 1;
     const int pad_cols = pad_columns(w);
     for (int pl = 0; pl < 3; pl++) {
+        if (pl && layout == RAV2D_PIXEL_LAYOUT_I400) {
+            pw[pl] = ph[pl] = 0;
+            stride[pl] = 0;
+            bytes[pl] = 0;
+            continue;
+        }
         int64_t s = ((int64_t)(pw[pl] + pad_cols) * bps + 255) & ~(int64_t)255;
         if ((s & 1023) == 0) s += 256;
         stride[pl] = s;
@@ -585,7 +597,7 @@ void picture_layout(int w, int h, int bps, int64_t stride[3], size_t bytes[3], i
 // buffer of a plane with one stride) and 8-byte alignment (4-sample groups).
 bool planes_match(const Rav2dGpuFrame *f, const Rav2dPlanes *p) {
     if (!p || p->stride[0] != f->stride[0] || p->stride[1] != f->stride[1]) return false;
-    for (int pl = 0; pl < 3; pl++)
+    for (int pl = 0; pl < n_planes(f->params.layout); pl++)
         if (!p->data[pl] || ((uintptr_t)p->data[pl] & 7)) return false;
     return true;
 }
@@ -606,8 +618,10 @@ Plan make_plan(const Rav2dFrameParams &p) {
     bool db_on = (p.inloop_filters & RAV2D_INLOOPFILTER_DEBLOCK) && (p.db_level_y[0] || p.db_level_y[1]);
     pl.dbv[0] = db_on && p.db_level_y[0];
     pl.dbh[0] = db_on && p.db_level_y[1];
+    // no chroma deblocking without chroma planes (deblock.rs:1745, 2002)
+    const bool uv = p.layout != RAV2D_PIXEL_LAYOUT_I400;
     for (int c = 1; c < 3; c++) {
-        bool lv = c == 1 ? p.db_level_u : p.db_level_v;
+        bool lv = uv && (c == 1 ? p.db_level_u : p.db_level_v);
         pl.dbv[c] = db_on && lv;
         pl.dbh[c] = db_on && lv;
     }
@@ -688,6 +702,7 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
         a.damping = P.cdef_damping;
         a.any_ll = 0;
         for (int i = 0; i < 8; i++) a.any_ll |= P.seg_enabled && P.seg_lossless[i];
+        a.has_chroma = P.layout != RAV2D_PIXEL_LAYOUT_I400;  // cdef.rs:728
         memcpy(a.y_strength, P.cdef_y_strength, 8);
         memcpy(a.uv_strength, P.cdef_uv_strength, 8);
         static const int8_t pos[7][2] = {{-1, 0}, {0, -1}, {-1, -1}, {-1, 1}, {-1, -2}, {1, -2}, {0, 2}};  // ccso.rs:5-13
@@ -957,7 +972,7 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
     g.ss_ver = 1;
     f->bps = params->bitdepth > 8 ? 2 : 1;
     size_t total = 0;
-    picture_layout(params->width, params->height, f->bps, f->stride, f->plane_bytes, f->pw, f->ph);
+    picture_layout(params->width, params->height, f->bps, params->layout, f->stride, f->plane_bytes, f->pw, f->ph);
     for (int pl = 0; pl < 3; pl++) total += f->plane_bytes[pl];
     const size_t grain_bytes = grain_scratch_bytes(params->width, params->height);
     // side info and the LR work list are reserved here too, so that the budget
@@ -1213,18 +1228,19 @@ int rav2d_gpu_frame_set_segmap(Rav2dGpuFrame *f, const uint8_t *segmap, ptrdiff_
 int rav2d_gpu_frame_planes(const Rav2dGpuFrame *f, int which, Rav2dPlanes *out) {
     if (!f || !out || which < 0 || which > 2) return RAV2D_EINVAL;
     int b = which == 0 ? BUF_IN : which == 1 ? BUF_OUT : BUF_DISP;
-    for (int pl = 0; pl < 3; pl++) out->data[pl] = f->buf[b][pl];
+    for (int pl = 0; pl < 3; pl++) out->data[pl] = f->pw[pl] ? f->buf[b][pl] : nullptr;
     out->stride[0] = f->stride[0];
     out->stride[1] = f->stride[1];
     return RAV2D_OK;
 }
 
 int rav2d_gpu_frame_upload(Rav2dGpuFrame *f, const void *y, const void *u, const void *v, void *stream) {
-    if (!f || !y || !u || !v) return RAV2D_EINVAL;
+    const int np = f ? n_planes(f->params.layout) : 3;
+    if (!f || !y || (np == 3 && (!u || !v))) return RAV2D_EINVAL;
     HIP_TRY(hipSetDevice(f->ctx->device));
     hipStream_t s = pick_stream(f->ctx, stream);
     const void *src[3] = {y, u, v};
-    for (int pl = 0; pl < 3; pl++)
+    for (int pl = 0; pl < np; pl++)
         HIP_TRY(hipMemcpy2DAsync(f->buf[BUF_IN][pl], f->stride[pl], src[pl], (size_t)f->pw[pl] * f->bps,
                                  (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1235,7 +1251,8 @@ int rav2d_gpu_frame_upload_padded(Rav2dGpuFrame *f, const Rav2dPlanes *src, int 
                                   void *stream) {
     if (!f || !src || pad_rows < 0 || pad_rows > RAV2D_PICTURE_PAD || pad_cols < 0 || pad_cols > RAV2D_PICTURE_PAD)
         return RAV2D_EINVAL;
-    for (int pl = 0; pl < 3; pl++)
+    const int np = n_planes(f->params.layout);
+    for (int pl = 0; pl < np; pl++)
         if (!src->data[pl] || src->stride[pl ? 1 : 0] < (ptrdiff_t)(f->pw[pl] + pad_cols) * f->bps) {
             set_last_error("upload_padded: null plane or stride below (width + pad_cols) samples");
             return RAV2D_EINVAL;
@@ -1243,7 +1260,7 @@ int rav2d_gpu_frame_upload_padded(Rav2dGpuFrame *f, const Rav2dPlanes *src, int 
     HIP_TRY(hipSetDevice(f->ctx->device));
     hipStream_t s = pick_stream(f->ctx, stream);
     const int cols = std::min(pad_cols, pad_columns(f->pw[0]));
-    for (int pl = 0; pl < 3; pl++)
+    for (int pl = 0; pl < np; pl++)
         HIP_TRY(hipMemcpy2DAsync(f->buf[BUF_IN][pl], f->stride[pl], src->data[pl], (size_t)src->stride[pl ? 1 : 0],
                                  (size_t)(f->pw[pl] + cols) * f->bps, f->ph[pl] + pad_rows, hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1251,12 +1268,13 @@ int rav2d_gpu_frame_upload_padded(Rav2dGpuFrame *f, const Rav2dPlanes *src, int 
 }
 
 int rav2d_gpu_frame_download(Rav2dGpuFrame *f, int which, void *y, void *u, void *v, void *stream) {
-    if (!f || !y || !u || !v || which < 0 || which > 2) return RAV2D_EINVAL;
+    const int np = f ? n_planes(f->params.layout) : 3;
+    if (!f || !y || (np == 3 && (!u || !v)) || which < 0 || which > 2) return RAV2D_EINVAL;
     HIP_TRY(hipSetDevice(f->ctx->device));
     hipStream_t s = pick_stream(f->ctx, stream);
     int b = which == 0 ? BUF_IN : which == 1 ? BUF_OUT : BUF_DISP;
     void *dst[3] = {y, u, v};
-    for (int pl = 0; pl < 3; pl++)
+    for (int pl = 0; pl < np; pl++)
         HIP_TRY(hipMemcpy2DAsync(dst[pl], (size_t)f->pw[pl] * f->bps, f->buf[b][pl], f->stride[pl],
                                  (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1328,7 +1346,7 @@ int rav2d_gpu_picture_alloc(Rav2dGpuCtx *ctx, int width, int height, int bitdept
     int64_t stride[3];
     size_t bytes[3];
     int pw[3], ph[3];
-    picture_layout(width, height, bitdepth > 8 ? 2 : 1, stride, bytes, pw, ph);
+    picture_layout(width, height, bitdepth > 8 ? 2 : 1, layout, stride, bytes, pw, ph);
     void *mem = nullptr;
     r = rav2d_gpu_alloc(ctx, bytes[0] + bytes[1] + bytes[2], &mem);
     if (r) return r;
@@ -1339,8 +1357,8 @@ int rav2d_gpu_picture_alloc(Rav2dGpuCtx *ctx, int width, int height, int bitdept
         return RAV2D_EDEVICE;
     }
     out->data[0] = mem;
-    out->data[1] = (uint8_t *)mem + bytes[0];
-    out->data[2] = (uint8_t *)mem + bytes[0] + bytes[1];
+    out->data[1] = bytes[1] ? (uint8_t *)mem + bytes[0] : nullptr;
+    out->data[2] = bytes[2] ? (uint8_t *)mem + bytes[0] + bytes[1] : nullptr;
     out->stride[0] = stride[0];
     out->stride[1] = stride[1];
     return RAV2D_OK;
@@ -1371,8 +1389,9 @@ int rav2d_gpu_filter_planes(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, const Rav2dPlane
     // written the output (loop restoration's stripe-edge rows when a plane is
     // not deblocked) take it from the frame's own input copy
     bool alias = false;
-    for (int pl = 0; pl < 3; pl++) alias = alias || in->data[pl] == out->data[pl];
-    for (int pl = 0; pl < 3; pl++) {
+    const int np = n_planes(f->params.layout);
+    for (int pl = 0; pl < np; pl++) alias = alias || in->data[pl] == out->data[pl];
+    for (int pl = 0; pl < np; pl++) {
         if (alias)
             HIP_TRY(hipMemcpy2DAsync(f->buf[BUF_IN][pl], f->stride[pl], in->data[pl], f->stride[pl],
                                      (size_t)f->pw[pl] * f->bps, f->ph[pl], hipMemcpyDeviceToDevice, s));
@@ -1395,11 +1414,12 @@ int grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *ds
                  int bitdepth, int layout, const Rav2dFilmGrainData *data, uint32_t seed, uint8_t *scratch,
                  hipStream_t s) {
     const int bps = bitdepth > 8 ? 2 : 1;
-    const int pw[3] = {width, (width + 1) >> 1, (width + 1) >> 1};
-    const int ph[3] = {height, (height + 1) >> 1, (height + 1) >> 1};
+    const int np = n_planes(layout), cw = np == 3 ? (width + 1) >> 1 : 0, ch = np == 3 ? (height + 1) >> 1 : 0;
+    const int pw[3] = {width, cw, cw};
+    const int ph[3] = {height, ch, ch};
     if (!data_has_grain(*data)) {
         // the display copy of a picture without grain (whatever the other fields)
-        for (int pl = 0; pl < 3; pl++)
+        for (int pl = 0; pl < np; pl++)
             HIP_TRY(hipMemcpy2DAsync(dst->data[pl], dst->stride[pl ? 1 : 0], src->data[pl], src->stride[pl ? 1 : 0],
                                      (size_t)pw[pl] * bps, ph[pl], hipMemcpyDeviceToDevice, s));
         return RAV2D_OK;
@@ -1411,6 +1431,7 @@ int grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *ds
         a.src[pl] = DevPlane{(uint8_t *)src->data[pl], src->stride[pl ? 1 : 0], pw[pl], ph[pl]};
         a.dst[pl] = DevPlane{(uint8_t *)dst->data[pl], dst->stride[pl ? 1 : 0], pw[pl], ph[pl]};
     }
+    if (np == 1) a.plane_on[1] = a.plane_on[2] = 0;  // I400: luma only
     if (!grain_init_tables()) {
         set_last_error("film grain tables: device set-up failed");
         return RAV2D_EDEVICE;
@@ -1419,7 +1440,6 @@ int grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *ds
     HIP_TRY(hipGetLastError());
     launch_grain_apply(a, bitdepth, s);
     HIP_TRY(hipGetLastError());
-    (void)layout;
     return RAV2D_OK;
 }
 
@@ -1435,14 +1455,16 @@ int check_grain_planes(const Rav2dPlanes *src, const Rav2dPlanes *dst, int width
     int r = validate(&p);
     if (r) return r;
     const int bps = bitdepth > 8 ? 2 : 1;
+    const int np = n_planes(layout);
     for (const Rav2dPlanes *q : {src, dst}) {
-        for (int pl = 0; pl < 3; pl++)
+        for (int pl = 0; pl < np; pl++)
             if (!q->data[pl] || ((uintptr_t)q->data[pl] & 7)) {
                 set_last_error("grain planes: plane pointers must be 8-byte aligned");
                 return RAV2D_EINVAL;
             }
-        if ((q->stride[0] & 7) || (q->stride[1] & 7) || q->stride[0] < (ptrdiff_t)width * bps ||
-            q->stride[1] < (ptrdiff_t)((width + 1) >> 1) * bps || q->stride[0] >= (1 << 24) ||
+        const bool uv = np == 3;
+        if ((q->stride[0] & 7) || (uv && (q->stride[1] & 7)) || q->stride[0] < (ptrdiff_t)width * bps ||
+            (uv && q->stride[1] < (ptrdiff_t)((width + 1) >> 1) * bps) || q->stride[0] >= (1 << 24) ||
             q->stride[1] >= (1 << 24)) {
             set_last_error("grain planes: strides must be multiples of 8 covering a row, below 16 MiB");
             return RAV2D_EINVAL;

@@ -385,6 +385,29 @@ void gen_params(Rng &r, R2SFrame *f) {
     g.block_size = r.range(0, 1);
 }
 
+// A monochrome sequence's frame: no chroma planes, so no chroma CCSO,
+// restoration or grain.  Applied after every draw, so the luma planes, luma
+// side info and luma parameters equal those of the 4:2:0 frame of the seed.
+void make_monochrome(R2SFrame *f) {
+    Rav2dFrameParams &p = f->params;
+    p.layout = RAV2D_PIXEL_LAYOUT_I400;
+    for (int pl = 1; pl < 3; pl++) {
+        p.ccso[pl].enabled = 0;
+        p.restoration[pl].restoration_type = RAV2D_RESTORATION_NONE;
+        p.grain.num_points[pl] = 0;
+    }
+    p.grain.chroma_scaling_from_luma = 0;
+    for (Rav2dAv2Filter &m : f->mask) m.ccso[1] = m.ccso[2] = 0;
+    for (Rav2dAv2Restoration &lr : f->lr)
+        for (int pl = 1; pl < 3; pl++)
+            for (int u = 0; u < 16; u++) lr.lr[pl][u].restoration_type = RAV2D_RESTORATION_NONE;
+    for (int pl = 1; pl < 3; pl++) {
+        f->pw[pl] = f->ph[pl] = 0;
+        f->planes[pl].clear();
+        f->padded[pl].clear();
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -524,6 +547,7 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
             fill_plane(r, rp, f->planes[pl].data(), f->padded[pl].data(), f->pw[pl], f->ph[pl], pl ? 1 : 0, blocks,
                        bd, sigma);
     }
+    if (cfg->monochrome) make_monochrome(f);
     return f;
 }
 

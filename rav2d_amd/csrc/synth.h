@@ -52,6 +52,8 @@ typedef struct R2SConfig {
     int32_t skip_pct;        /* percent of blocks without residual (noskip = 0)   */
     int32_t pc_gdf_pct;      /* percent of restoration units / GDF blocks enabled */
     uint32_t mask_flags;     /* R2S_MASK_*; 0 = square transforms only            */
+    int32_t monochrome;      /* 1: I400 (luma only; no chroma planes, chroma tools
+                                off, as a monochrome stream's headers give); 0: 4:2:0 */
 } R2SConfig;
 
 typedef struct R2SFrame R2SFrame;

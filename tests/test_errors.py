@@ -36,6 +36,7 @@ def _params(sf):
     ("bitdepth", 9, "bit depth"),
     ("bitdepth", 16, "bit depth"),
     ("layout", 3, "4:2:0"),
+    ("layout", 2, "4:2:0"),
     ("sb128", 2, "sb128"),
     ("width", 16392, "at most 16384"),
     ("height", 32768, "at most 16384"),
