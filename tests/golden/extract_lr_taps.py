@@ -2,8 +2,10 @@
 """Writes lr_taps.json: the tap-offset tables of rav2d's loop restoration
 (crates/rav2d/src/looprestoration.rs:333-406: GDF_COORDS,
 WIENER_NS_CONFIG_UV, WIENER_NS_CONFIG_UV_FROM_Y, PC_WIENER_CONFIG,
-WIENER_NS_CONFIG_Y), read as data from the reference checkout.  Each table is
-a list of (dy, dx) pairs.  Run once where the checkout exists:
+WIENER_NS_CONFIG_Y; each a list of (dy, dx) pairs) and the constants of the
+PC-Wiener / multi-class classifier (looprestoration.rs:13-22:
+PC_WIENER_NORMALIZER, MODE_WEIGHTS, MODE_OFFSETS; flat lists of integers),
+read as data from the reference checkout.  Run once where the checkout exists:
 
     python tests/golden/extract_lr_taps.py /root/reference
 """
@@ -14,6 +16,8 @@ import sys
 
 NAMES = ["WIENER_NS_CONFIG_Y", "WIENER_NS_CONFIG_UV", "WIENER_NS_CONFIG_UV_FROM_Y", "PC_WIENER_CONFIG",
          "GDF_COORDS"]
+# (name, element count)
+CONSTANTS = [("PC_WIENER_NORMALIZER", 4), ("MODE_WEIGHTS", 12), ("MODE_OFFSETS", 4)]
 
 
 def main():
@@ -28,6 +32,14 @@ def main():
         if len(pairs) != int(m.group(1)):
             raise SystemExit("table %s: %d entries, expected %s" % (name, len(pairs), m.group(1)))
         out[name] = pairs
+    for name, count in CONSTANTS:
+        m = re.search(r"(?:static|const) %s: [^=]*= \[(.*?)\];" % name, text, re.S)
+        if not m:
+            raise SystemExit("table %s not found" % name)
+        vals = [int(v) for v in re.findall(r"-?\d+", m.group(1))]
+        if len(vals) != count:
+            raise SystemExit("table %s: %d entries, expected %d" % (name, len(vals), count))
+        out[name] = vals
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lr_taps.json")
     with open(path, "w") as f:
         json.dump(out, f)
