@@ -306,17 +306,19 @@ static void backup2x8(P (*dst)[2], const P *src, ptrdiff_t base, ptrdiff_t strid
 /* cdef.rs:611-645 (positive strides) */
 template <typename P>
 static void backup2lines_bank(std::vector<P> bank[3], const Frame<P> &f, ptrdiff_t y_off, ptrdiff_t uv_off,
-                              bool have_chroma) {
+                              int layout) {
     for (ptrdiff_t i = 0; i < 2 * f.ys; i++) bank[0][i] = f.y[y_off + 6 * f.ys + i];
-    if (!have_chroma) return;  // cdef.rs:525 (I400)
-    ptrdiff_t s = uv_off + 2 * f.uvs;  // 4:2:0: rows 2,3 of the 4-row chroma band
+    if (layout == RAV2D_PIXEL_LAYOUT_I400) return;  // cdef.rs:525 (I400)
+    // the last two rows of the chroma band: rows 2,3 of 4 (4:2:0), 6,7 of 8
+    ptrdiff_t s = uv_off + (layout == RAV2D_PIXEL_LAYOUT_I420 ? 2 : 6) * f.uvs;
     for (ptrdiff_t i = 0; i < 2 * f.uvs; i++) {
         bank[1][i] = f.u[s + i];
         bank[2][i] = f.v[s + i];
     }
 }
 
-static const uint8_t UV_DIRS_420[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+/* cdef.rs:605: chroma direction from the luma direction, identity except 4:2:2 */
+static const uint8_t UV_DIRS[2][8] = {{0, 1, 2, 3, 4, 5, 6, 7}, {7, 0, 2, 4, 5, 6, 6, 6}};
 
 /* cdef.rs:702-1182 */
 template <typename P>
@@ -325,7 +327,8 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
     const int bdmax = (1 << bitdepth) - 1;
     const int damping = p.damping + bdm8;
     const ptrdiff_t y_ls = f.ys, uv_ls = f.uvs;
-    const int ss_hor = 1, ss_ver = 1;
+    const int ss_hor = p.layout != RAV2D_PIXEL_LAYOUT_I444, ss_ver = p.layout == RAV2D_PIXEL_LAYOUT_I420;
+    const uint8_t *uv_dir = UV_DIRS[p.layout == RAV2D_PIXEL_LAYOUT_I422];
     const int sbsz = 16;
     const int sb64w = (p.bw + sbsz - 1) >> 4;
     ptrdiff_t row_y = (ptrdiff_t)by_start * 4 * y_ls;
@@ -337,7 +340,7 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
         int by_idx = (by & 0x3e) >> 1;
         int edges = edge_top | CDEF_HAVE_BOTTOM;
         if (by + 2 >= p.bh) edges &= ~CDEF_HAVE_BOTTOM;
-        if (edges & CDEF_HAVE_BOTTOM) backup2lines_bank(L.bank[1 - tf], f, row_y, row_uv, p.have_chroma);
+        if (edges & CDEF_HAVE_BOTTOM) backup2lines_bank(L.bank[1 - tf], f, row_y, row_uv, p.layout);
         P lr_bak[2][3][8][2];
         memset(lr_bak, 0, sizeof(lr_bak));
         int bit = 0;
@@ -451,7 +454,7 @@ void cdef_brow(const CdefBrowParams &p, Frame<P> &f, CdefLines<P> &L, int by_sta
                                           y_sec, 0, damping, 8, 8, edges, bdm8);
                     }
                     if (uv_lvl && p.have_chroma && !uv_lossless) {
-                        int uvdir = uv_pri ? UV_DIRS_420[dir] : 0;
+                        int uvdir = uv_pri ? uv_dir[dir] : 0;
                         int cw = 8 >> ss_hor, ch = 8 >> ss_ver;
                         ptrdiff_t top_col_uv = ((ptrdiff_t)bx * 4) >> ss_hor;
                         ptrdiff_t bot_uv = b_uv + ch * uv_ls;

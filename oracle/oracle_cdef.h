@@ -21,6 +21,7 @@ struct CdefBrowParams {
     bool on_skip_tx, cdef_on, ccso_on;
     bool any_lossless; /* a segment is lossless: the masks' lossless 4x4 units are skipped */
     bool have_chroma; /* false for I400 (cdef.rs:728): chroma is neither backed up nor filtered */
+    int layout;       /* RAV2D_PIXEL_LAYOUT_*: chroma subsampling and the 4:2:2 direction remap (cdef.rs:722-725) */
     const uint8_t *y_strength, *uv_strength;
     const Rav2dAv2Filter *mask;
     size_t n_mask, row;

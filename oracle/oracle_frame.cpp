@@ -46,6 +46,7 @@ static int run_frame(const Rav2dFrameParams *p, const Rav2dAv2Filter *mask_in, s
     cp.on_skip_tx = p->cdef_on_skiptx != 0;
     cp.cdef_on = (inloop & RAV2D_INLOOPFILTER_CDEF) != 0;
     cp.have_chroma = p->layout != RAV2D_PIXEL_LAYOUT_I400;
+    cp.layout = p->layout;
     /* decode.rs:3868-3869 */
     cp.any_lossless = false;
     for (int i = 0; i < 8; i++) cp.any_lossless = cp.any_lossless || (p->seg_enabled && p->seg_lossless[i]);
@@ -121,8 +122,7 @@ extern "C" int oracle_filter_frame_seg(const Rav2dFrameParams *p, const Rav2dAv2
                                        ptrdiff_t y_stride, ptrdiff_t uv_stride, uint32_t inloop_mask,
                                        const uint8_t *seg, ptrdiff_t seg_stride, const uint8_t *seg_uv,
                                        ptrdiff_t uv_seg_stride) {
-    if (!p || !mask || (p->layout != RAV2D_PIXEL_LAYOUT_I420 && p->layout != RAV2D_PIXEL_LAYOUT_I400))
-        return RAV2D_EINVAL;
+    if (!p || !mask || p->layout < RAV2D_PIXEL_LAYOUT_I400 || p->layout > RAV2D_PIXEL_LAYOUT_I444) return RAV2D_EINVAL;
     if (p->seg_enabled && (!seg || !seg_uv)) return RAV2D_EINVAL;
     if (p->bitdepth == 8)
         return run_frame<uint8_t>(p, mask, n_mask, lr, n_lr, (uint8_t *)y, (uint8_t *)u, (uint8_t *)v, y_stride,

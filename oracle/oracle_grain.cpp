@@ -306,7 +306,7 @@ static void fguv(P *dst, const P *src, ptrdiff_t stride, const Rav2dFilmGrainDat
     }
 }
 
-/* filmgrain.rs:617-941 + decode.rs:4672-4775, 4:2:0 or I400, one thread */
+/* filmgrain.rs:617-941 + decode.rs:4672-4775, one thread */
 template <typename P>
 static void apply_grain(const Rav2dFrameParams *p, P *dy, P *du, P *dv, const P *sy_, const P *su, const P *sv_,
                         ptrdiff_t ys, ptrdiff_t uvs) {
@@ -315,7 +315,7 @@ static void apply_grain(const Rav2dFrameParams *p, P *dy, P *du, P *dv, const P 
     const bool has_chroma = p->layout != RAV2D_PIXEL_LAYOUT_I400;
     const int bitdepth = p->bitdepth, w = p->width, h = p->height;
     const uint32_t seed = p->grain_seed;
-    const int ssx = 1, ssy = 1;
+    const int ssx = p->layout != RAV2D_PIXEL_LAYOUT_I444, ssy = p->layout == RAV2D_PIXEL_LAYOUT_I420;
     static thread_local int16_t lut_y[GH][GW], lut_u[GH][GW], lut_v[GH][GW];
     memset(lut_u, 0, sizeof(lut_u));
     memset(lut_v, 0, sizeof(lut_v));
@@ -374,10 +374,11 @@ extern "C" int oracle_picture_has_grain(const Rav2dFrameParams *p) {
 extern "C" int oracle_apply_grain(const Rav2dFrameParams *p, const void *src_y, const void *src_u,
                                   const void *src_v, void *dst_y, void *dst_u, void *dst_v, ptrdiff_t y_stride,
                                   ptrdiff_t uv_stride) {
-    if (!p || (p->layout != RAV2D_PIXEL_LAYOUT_I420 && p->layout != RAV2D_PIXEL_LAYOUT_I400)) return RAV2D_EINVAL;
+    if (!p || p->layout < RAV2D_PIXEL_LAYOUT_I400 || p->layout > RAV2D_PIXEL_LAYOUT_I444) return RAV2D_EINVAL;
     int bps = p->bitdepth > 8 ? 2 : 1;
     const bool mono = p->layout == RAV2D_PIXEL_LAYOUT_I400;
-    int cw = mono ? 0 : (p->width + 1) >> 1, ch = mono ? 0 : (p->height + 1) >> 1;
+    const int ssx = p->layout != RAV2D_PIXEL_LAYOUT_I444, ssy = p->layout == RAV2D_PIXEL_LAYOUT_I420;
+    int cw = mono ? 0 : (p->width + ssx) >> ssx, ch = mono ? 0 : (p->height + ssy) >> ssy;
     /* the display copy starts as the ungrained picture */
     for (int y = 0; y < p->height; y++)
         memcpy((uint8_t *)dst_y + y * y_stride, (const uint8_t *)src_y + y * y_stride, (size_t)p->width * bps);

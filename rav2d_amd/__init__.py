@@ -26,6 +26,15 @@ INLOOP_ALL = 31
 PASS_DEBLOCK_V, PASS_DEBLOCK_H, PASS_CDEF, PASS_LR, PASS_GRAIN = range(5)
 PIXEL_LAYOUT_I400, PIXEL_LAYOUT_I420, PIXEL_LAYOUT_I422, PIXEL_LAYOUT_I444 = range(4)
 
+
+def chroma_shape(layout, width, height):
+    """(rows, columns) of a chroma plane of a width x height frame: (0, 0) for
+    I400, halved horizontally for 4:2:0 and 4:2:2, vertically for 4:2:0."""
+    if layout == PIXEL_LAYOUT_I400:
+        return (0, 0)
+    ssx, ssy = int(layout != PIXEL_LAYOUT_I444), int(layout == PIXEL_LAYOUT_I420)
+    return ((height + ssy) >> ssy, (width + ssx) >> ssx)
+
 TOOL_DEBLOCK, TOOL_CDEF, TOOL_CCSO, TOOL_WIENER, TOOL_GDF, TOOL_GRAIN = 1, 2, 4, 8, 16, 32
 TOOL_ALL = 63
 # synthetic mask shapes: transform partitions (incl. H5/V5), sub-PU / TIP edges, segmentation
@@ -140,7 +149,7 @@ class SynthConfig(C.Structure):
         ("width", C.c_int32), ("height", C.c_int32), ("bitdepth", C.c_int32), ("sb128", C.c_int32),
         ("seed", C.c_uint64), ("tools", C.c_uint32), ("lossless_pct", C.c_int32),
         ("skip_pct", C.c_int32), ("pc_gdf_pct", C.c_int32), ("mask_flags", C.c_uint32),
-        ("monochrome", C.c_int32),
+        ("monochrome", C.c_int32), ("chroma_format", C.c_int32),
     ]
 
 
@@ -308,10 +317,16 @@ class SynthFrame:
     """A seeded synthetic reconstructed frame plus its filter side info."""
 
     def __init__(self, width, height, bitdepth=10, sb128=0, seed=1, tools=TOOL_ALL,
-                 lossless_pct=0, skip_pct=30, pc_gdf_pct=50, mask_flags=0, monochrome=False):
+                 lossless_pct=0, skip_pct=30, pc_gdf_pct=50, mask_flags=0, monochrome=False, layout=None):
+        """layout: PIXEL_LAYOUT_I420 (default), _I422, _I444 or _I400 (the same as
+        monochrome=True).  The luma of every layout equals that of the 4:2:0
+        frame of the same seed."""
+        if layout == PIXEL_LAYOUT_I400:
+            monochrome = True
+        fmt = {None: 0, PIXEL_LAYOUT_I400: 0, PIXEL_LAYOUT_I420: 0, PIXEL_LAYOUT_I422: 1, PIXEL_LAYOUT_I444: 2}[layout]
         lib = load_synth()
         cfg = SynthConfig(width, height, bitdepth, sb128, seed, tools, lossless_pct, skip_pct, pc_gdf_pct,
-                          mask_flags, 1 if monochrome else 0)
+                          mask_flags, 1 if monochrome else 0, fmt)
         self._lib = lib
         self._h = lib.r2s_frame_create(C.byref(cfg))
         if not self._h:
@@ -319,6 +334,7 @@ class SynthFrame:
         self.width, self.height, self.bitdepth = width, height, bitdepth
         self.monochrome = bool(monochrome)
         self.params = lib.r2s_params(self._h).contents
+        self.layout = self.params.layout
         n = C.c_size_t()
         self.mask_ptr = lib.r2s_mask(self._h, C.byref(n))
         self.n_mask = n.value
@@ -486,9 +502,7 @@ class Picture:
     def _shape(self, pl):
         if pl == 0:
             return (self.height, self.width)
-        if self.layout == PIXEL_LAYOUT_I400:
-            return (0, 0)
-        return ((self.height + 1) // 2, (self.width + 1) // 2)
+        return chroma_shape(self.layout, self.width, self.height)
 
     def write(self, planes):
         """Upload three numpy planes (rows of any stride are packed here); planes
@@ -550,6 +564,7 @@ class GpuFrame:
         _check(self.lib.rav2d_gpu_frame_create(ctx.h, C.byref(params), C.byref(h)), "frame_create")
         self.h = h
         self.width, self.height, self.bitdepth = params.width, params.height, params.bitdepth
+        self.layout = params.layout
         self.monochrome = params.layout == PIXEL_LAYOUT_I400
 
     @classmethod
@@ -594,7 +609,7 @@ class GpuFrame:
 
     def download(self, which=1):
         dt = np.uint16 if self.bitdepth > 8 else np.uint8
-        c = (0, 0) if self.monochrome else (self.height // 2, self.width // 2)
+        c = chroma_shape(self.layout, self.width, self.height)
         arrs = [np.empty(s, dtype=dt) for s in [(self.height, self.width), c, c]]
         ptrs = [a.ctypes.data if a.size else None for a in arrs]
         _check(self.lib.rav2d_gpu_frame_download(self.h, which, ptrs[0], ptrs[1], ptrs[2], None), "download")

@@ -70,18 +70,18 @@ void outer_edge(uint16_t (*masks)[4], int pos4, int len4, int lvl_c, uint8_t *ct
     for (int i = 0; i < len4; i++) ctx[i] = (uint8_t)lvl_c;
 }
 
-// Edge masks of one block (create_db_mask for an intra block, TxPartition::None).
-// `dir0[x4]` / `dir1[y4]` are the SB256's filter_{y,uv}[0] / [1]; pos are in the
-// plane's 4px units relative to the SB256.
-void block_edges(uint16_t (*dir0)[5][4], uint16_t (*dir1)[5][4], int bx4, int by4, int w4, int h4, int tx4,
+// Edge masks of one block (create_db_mask for an intra block, TxPartition::None)
+// with tw4 x th4 transforms.  `dir0[x4]` / `dir1[y4]` are the SB256's
+// filter_{y,uv}[0] / [1]; pos are in the plane's 4px units relative to the SB256.
+void block_edges(uint16_t (*dir0)[5][4], uint16_t (*dir1)[5][4], int bx4, int by4, int w4, int h4, int tw4, int th4,
                  int lim, uint8_t *actx, uint8_t *lctx) {
-    int lw = std::min(lim, ilog2(tx4));
+    int lw = std::min(lim, ilog2(tw4)), lh = std::min(lim, ilog2(th4));
     outer_edge(dir0[bx4], by4, h4, lw, lctx);
-    outer_edge(dir1[by4], bx4, w4, lw, actx);
-    for (int x = tx4; x < w4; x += tx4)
+    outer_edge(dir1[by4], bx4, w4, lh, actx);
+    for (int x = tw4; x < w4; x += tw4)
         for (int y = 0; y < h4; y++) dir0[bx4 + x][lw][(by4 + y) >> 4] |= (uint16_t)(1u << ((by4 + y) & 15));
-    for (int y = tx4; y < h4; y += tx4)
-        for (int x = 0; x < w4; x++) dir1[by4 + y][lw][(bx4 + x) >> 4] |= (uint16_t)(1u << ((bx4 + x) & 15));
+    for (int y = th4; y < h4; y += th4)
+        for (int x = 0; x < w4; x++) dir1[by4 + y][lh][(bx4 + x) >> 4] |= (uint16_t)(1u << ((bx4 + x) & 15));
 }
 
 // Luma transform size (4px units) of TxPartition `part` (None, Split, H, V, H4,
@@ -161,10 +161,10 @@ void subpu_edges(uint16_t (*dir0)[5][4], uint16_t (*dir1)[5][4], int by4, int bx
 }
 
 // create_db_mask (lf_mask.rs:399-502) for one plane of a block; bx4/by4/w4/h4
-// are in the plane's 4px units within the SB256, `ss` the plane's subsampling.
-void db_mask(uint16_t (*dir0)[5][4], uint16_t (*dir1)[5][4], const Block &b, int ss, int bx4, int by4, int w4,
-             int h4, int ds_mask, uint8_t *a, uint8_t *l) {
-    const int chroma = ss;
+// are in the plane's 4px units within the SB256, ssx / ssy the plane's
+// subsampling (a chroma plane is one with `chroma` set).
+void db_mask(uint16_t (*dir0)[5][4], uint16_t (*dir1)[5][4], const Block &b, int chroma, int ssx, int ssy, int bx4,
+             int by4, int w4, int h4, int ds_mask, uint8_t *a, uint8_t *l) {
     const int s = b.subpu, bl2 = ilog2(b.bs4);
     int twl4c, thl4c;
     if (!b.inter || !b.skip) {
@@ -172,25 +172,29 @@ void db_mask(uint16_t (*dir0)[5][4], uint16_t (*dir1)[5][4], const Block &b, int
         if (b.lossless) {
             tw4 = th4 = 1;
         } else if (chroma) {
-            tw4 = th4 = std::max(1, std::min(b.bs4 >> 1, 8));  // largest chroma transform of the block
+            // the largest transform of the chroma block, at most 32 samples a side
+            tw4 = std::max(1, std::min(b.bs4 >> ssx, 8));
+            th4 = std::max(1, std::min(b.bs4 >> ssy, 8));
         } else {
             part = b.part;
             tw4 = b.tw4;
             th4 = b.th4;
         }
-        const int lim = std::min(std::max(s - ss, 0), 3 - chroma);
-        edges_part(dir0, dir1, by4, bx4, w4, h4, part, tw4, th4, lim, lim, a, l);
+        const int hlim = std::min(std::max(s - ssx, 0), 3 - chroma);
+        const int vlim = std::min(std::max(s - ssy, 0), 3 - chroma);
+        edges_part(dir0, dir1, by4, bx4, w4, h4, part, tw4, th4, hlim, vlim, a, l);
         twl4c = std::min(s, ilog2(tw4));
         thl4c = std::min(s, ilog2(th4));
     } else {
-        const int lim = b.lossless ? 0 : std::min(std::max(std::min(s, bl2) - ss, 0), 3 - chroma);
-        outer_edge(dir0[bx4], by4, h4, lim, l);
-        outer_edge(dir1[by4], bx4, w4, lim, a);
+        const int hlim = b.lossless ? 0 : std::min(std::max(std::min(s, bl2) - ssx, 0), 3 - chroma);
+        const int vlim = b.lossless ? 0 : std::min(std::max(std::min(s, bl2) - ssy, 0), 3 - chroma);
+        outer_edge(dir0[bx4], by4, h4, hlim, l);
+        outer_edge(dir1[by4], bx4, w4, vlim, a);
         twl4c = thl4c = s;
     }
     if (s != 3) {
-        const int hl = twl4c - (ss && twl4c), vl = thl4c - (ss && thl4c);
-        subpu_edges(dir0, dir1, by4, bx4, w4, h4, hl, vl, (1 << s) >> ss, (1 << s) >> ss, ds_mask);
+        const int hl = twl4c - (ssx && twl4c), vl = thl4c - (ssy && thl4c);
+        subpu_edges(dir0, dir1, by4, bx4, w4, h4, hl, vl, (1 << s) >> ssx, (1 << s) >> ssy, ds_mask);
     }
 }
 
@@ -238,8 +242,8 @@ constexpr int kPad = 8;
 // padding ((w + kPad) samples per row), whose padding continues the plane's
 // content (its blocks' DC offsets and noise from the padding's own stream `rp`).
 template <typename P>
-void fill_plane(Rng &r, Rng &rp, P *pl, P *pad, int w, int h, int ss, const std::vector<Block> &blocks, int bd,
-                double sigma) {
+void fill_plane(Rng &r, Rng &rp, P *pl, P *pad, int w, int h, int ssx, int ssy, const std::vector<Block> &blocks,
+                int bd, double sigma) {
     int bdmax = (1 << bd) - 1;
     double gx = r.uniform() * 2.0 - 1.0, gy = r.uniform() * 2.0 - 1.0;
     double base = (0.3 + 0.4 * r.uniform()) * bdmax;
@@ -248,10 +252,11 @@ void fill_plane(Rng &r, Rng &rp, P *pl, P *pad, int w, int h, int ss, const std:
     int scale = 1 << (bd - 8);
     for (const Block &b : blocks) {
         // DC offset per transform block
-        int tw = std::max(4, (b.tw4 * 4) >> ss), th = std::max(4, (b.th4 * 4) >> ss), bs = (b.bs4 * 4) >> ss;
-        int x0 = (b.bx * 4) >> ss, y0 = (b.by * 4) >> ss;
-        for (int ty = 0; ty < bs; ty += th)
-            for (int tx = 0; tx < bs; tx += tw) {
+        int tw = std::max(4, (b.tw4 * 4) >> ssx), th = std::max(4, (b.th4 * 4) >> ssy);
+        int bsw = (b.bs4 * 4) >> ssx, bsh = (b.bs4 * 4) >> ssy;
+        int x0 = (b.bx * 4) >> ssx, y0 = (b.by * 4) >> ssy;
+        for (int ty = 0; ty < bsh; ty += th)
+            for (int tx = 0; tx < bsw; tx += tw) {
                 int off = (b.dc + r.range(-6, 6)) * scale;
                 for (int yy = y0 + ty; yy < std::min(h, y0 + ty + th); yy++)
                     for (int xx = x0 + tx; xx < std::min(w, x0 + tx + tw); xx++) dc[(size_t)yy * w + xx] = off;
@@ -421,10 +426,14 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
     Rng r(cfg->seed);
     gen_params(r, f);
     const int W = cfg->width, H = cfg->height;
+    // chroma subsampling of the format (the I400 frame is made from the 4:2:0 one)
+    const int cf = cfg->monochrome ? 0 : cfg->chroma_format;
+    const int ssx = cf < 2, ssy = cf == 0;
+    f->params.layout = cf == 0 ? RAV2D_PIXEL_LAYOUT_I420 : cf == 1 ? RAV2D_PIXEL_LAYOUT_I422 : RAV2D_PIXEL_LAYOUT_I444;
     f->pw[0] = W;
     f->ph[0] = H;
-    f->pw[1] = f->pw[2] = (W + 1) >> 1;
-    f->ph[1] = f->ph[2] = (H + 1) >> 1;
+    f->pw[1] = f->pw[2] = (W + ssx) >> ssx;
+    f->ph[1] = f->ph[2] = (H + ssy) >> ssy;
     const int bw = ((W + 7) >> 3) << 1, bh = ((H + 7) >> 3) << 1;
     const int sb256w = (bw + 63) >> 6, sb256h = (bh + 63) >> 6;
     f->mask.assign((size_t)sb256w * sb256h, Rav2dAv2Filter());
@@ -448,16 +457,18 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
         int bx4 = b.bx & 63, by4 = b.by & 63;
         int w4 = std::min(bw - b.bx, b.bs4), h4 = std::min(bh - b.by, b.bs4);
         if (cfg->mask_flags) {
-            db_mask(m.filter_y[0], m.filter_y[1], b, 0, bx4, by4, w4, h4, ds_mask, &actx_y[b.bx], &lctx_y[b.by]);
-            db_mask(m.filter_uv[0], m.filter_uv[1], b, 1, bx4 >> 1, by4 >> 1, w4 >> 1, h4 >> 1, ds_mask,
-                    &actx_uv[b.bx >> 1], &lctx_uv[b.by >> 1]);
+            db_mask(m.filter_y[0], m.filter_y[1], b, 0, 0, 0, bx4, by4, w4, h4, ds_mask, &actx_y[b.bx],
+                    &lctx_y[b.by]);
+            db_mask(m.filter_uv[0], m.filter_uv[1], b, 1, ssx, ssy, bx4 >> ssx, by4 >> ssy, w4 >> ssx, h4 >> ssy,
+                    ds_mask, &actx_uv[b.bx >> ssx], &lctx_uv[b.by >> ssy]);
         } else {
-            block_edges(m.filter_y[0], m.filter_y[1], bx4, by4, w4, h4, b.tx4, 3, &actx_y[b.bx], &lctx_y[b.by]);
-            // chroma 4:2:0: chroma block is half size; transform = chroma block, at most 32
-            int ctx4 = b.lossless ? 1 : std::min(b.bs4 >> 1, 8);
-            if (ctx4 < 1) ctx4 = 1;
-            block_edges(m.filter_uv[0], m.filter_uv[1], bx4 >> 1, by4 >> 1, w4 >> 1, h4 >> 1, ctx4, 2,
-                        &actx_uv[b.bx >> 1], &lctx_uv[b.by >> 1]);
+            block_edges(m.filter_y[0], m.filter_y[1], bx4, by4, w4, h4, b.tx4, b.tx4, 3, &actx_y[b.bx],
+                        &lctx_y[b.by]);
+            // chroma transform = the chroma block, at most 32 samples a side
+            int ctw4 = b.lossless ? 1 : std::max(1, std::min(b.bs4 >> ssx, 8));
+            int cth4 = b.lossless ? 1 : std::max(1, std::min(b.bs4 >> ssy, 8));
+            block_edges(m.filter_uv[0], m.filter_uv[1], bx4 >> ssx, by4 >> ssy, w4 >> ssx, h4 >> ssy, ctw4, cth4, 2,
+                        &actx_uv[b.bx >> ssx], &lctx_uv[b.by >> ssy]);
         }
         for (int y = 0; y < h4; y++)
             for (int x = 0; x < w4; x++) {
@@ -469,10 +480,11 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
                 if (b.lossless) m.lossless_mask_y[yy][xx >> 4] |= (uint16_t)(1u << (xx & 15));
             }
         if (b.lossless)
-            for (int y = 0; y < (h4 >> 1); y++)
-                for (int x = 0; x < (w4 >> 1); x++) {
-                    int yy = (by4 >> 1) + y, xx = (bx4 >> 1) + x;  // chroma 4px units in SB256
-                    m.lossless_mask_uv[yy][xx >> 3] |= (uint16_t)(1u << (xx & 7));
+            for (int y = 0; y < (h4 >> ssy); y++)
+                for (int x = 0; x < (w4 >> ssx); x++) {
+                    // chroma 4px units in the SB256: a word per 64 luma columns
+                    int yy = (by4 >> ssy) + y, xx = (bx4 >> ssx) + x;
+                    m.lossless_mask_uv[yy][xx >> (4 - ssx)] |= (uint16_t)(1u << (xx & ((16 >> ssx) - 1)));
                 }
     }
 
@@ -485,16 +497,17 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
         sp.seg_delta_q[7] = (int16_t)-sp.yac;
         sp.seg_lossless[7] = cfg->lossless_pct > 0;
         f->seg_stride[0] = bw;
-        f->seg_stride[1] = bw >> 1;
+        const int cbw = bw >> ssx;
+        f->seg_stride[1] = cbw;
         f->seg[0].assign((size_t)bw * bh, 0);
-        f->seg[1].assign((size_t)(bw >> 1) * (bh >> 1), 0);
+        f->seg[1].assign((size_t)cbw * (bh >> ssy), 0);
         for (const Block &b : blocks) {
             int w4 = std::min(bw - b.bx, b.bs4), h4 = std::min(bh - b.by, b.bs4);
             for (int y = 0; y < h4; y++)
                 for (int x = 0; x < w4; x++) f->seg[0][(size_t)(b.by + y) * bw + b.bx + x] = (uint8_t)b.seg;
-            for (int y = 0; y < (h4 >> 1); y++)
-                for (int x = 0; x < (w4 >> 1); x++)
-                    f->seg[1][(size_t)((b.by >> 1) + y) * (bw >> 1) + (b.bx >> 1) + x] = (uint8_t)b.seg;
+            for (int y = 0; y < (h4 >> ssy); y++)
+                for (int x = 0; x < (w4 >> ssx); x++)
+                    f->seg[1][(size_t)((b.by >> ssy) + y) * cbw + (b.bx >> ssx) + x] = (uint8_t)b.seg;
         }
     }
 
@@ -542,10 +555,10 @@ R2SFrame *r2s_frame_create(const R2SConfig *cfg) {
         f->padded[pl].assign((size_t)(f->pw[pl] + kPad) * (f->ph[pl] + kPad) * bps, 0);
         if (bd > 8)
             fill_plane(r, rp, (uint16_t *)f->planes[pl].data(), (uint16_t *)f->padded[pl].data(), f->pw[pl],
-                       f->ph[pl], pl ? 1 : 0, blocks, bd, sigma);
+                       f->ph[pl], pl ? ssx : 0, pl ? ssy : 0, blocks, bd, sigma);
         else
-            fill_plane(r, rp, f->planes[pl].data(), f->padded[pl].data(), f->pw[pl], f->ph[pl], pl ? 1 : 0, blocks,
-                       bd, sigma);
+            fill_plane(r, rp, f->planes[pl].data(), f->padded[pl].data(), f->pw[pl], f->ph[pl], pl ? ssx : 0,
+                       pl ? ssy : 0, blocks, bd, sigma);
     }
     if (cfg->monochrome) make_monochrome(f);
     return f;

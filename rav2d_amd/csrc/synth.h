@@ -53,7 +53,10 @@ typedef struct R2SConfig {
     int32_t pc_gdf_pct;      /* percent of restoration units / GDF blocks enabled */
     uint32_t mask_flags;     /* R2S_MASK_*; 0 = square transforms only            */
     int32_t monochrome;      /* 1: I400 (luma only; no chroma planes, chroma tools
-                                off, as a monochrome stream's headers give); 0: 4:2:0 */
+                                off, as a monochrome stream's headers give); 0: chroma_format */
+    int32_t chroma_format;   /* 0: 4:2:0, 1: 4:2:2, 2: 4:4:4.  Every draw is the same in
+                                each format, so the luma planes, luma side info and luma
+                                parameters equal those of the 4:2:0 frame of the seed */
 } R2SConfig;
 
 typedef struct R2SFrame R2SFrame;

@@ -1,7 +1,9 @@
 """Monochrome (I400) frames: luma only, as the reference filters a monochrome
 sequence (no chroma planes: deblock.rs:1745, 2002; cdef.rs:728; the chroma
 restore bits masked off, looprestoration.rs:2401-2409; no chroma grain,
-decode.rs:4693 / filmgrain.rs:745-780).
+decode.rs:4693 / filmgrain.rs:745-780).  4:2:2 and 4:4:4 frames exist in the
+oracle and the synthetic generator only (the HIP library rejects them); their
+luma, which no filter derives from chroma, equals the 4:2:0 frame's.
 
 Pins that do not rest on the oracle's own chroma handling: a synthetic I400
 frame carries exactly the luma planes, luma side info and luma parameters of
@@ -46,6 +48,35 @@ def test_oracle_mono_luma_equals_420_luma(case):
         assert np.array_equal(oa[0], ob[0])
         assert (ob[0] != b.planes()[0]).mean() > 0.3  # the chain changed most samples
         assert np.array_equal(O.apply_grain(a, oa)[0], O.apply_grain(b, ob)[0])
+    finally:
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("layout", [R.PIXEL_LAYOUT_I422, R.PIXEL_LAYOUT_I444], ids=["i422", "i444"])
+@pytest.mark.parametrize("case", CASES[:4], ids=lambda c: "%dx%d_%db" % c[:3])
+def test_oracle_422_444_luma_equals_420_luma(case, layout):
+    """The oracle's chroma paths for 4:2:2 / 4:4:4 (subsampling of the masks,
+    CDEF blocks and the 4:2:2 direction remap, CCSO, LR stripes, grain) run
+    on frames whose luma, luma side info and luma parameters are the 4:2:0
+    frame's: the filtered and grained luma must be the 4:2:0 frame's, and
+    every chroma plane has the layout's size and is changed by the chain."""
+    w, h, bd, sb, seed, mf = case
+    a = R.SynthFrame(w, h, bd, sb, seed, mask_flags=mf)
+    b = R.SynthFrame(w, h, bd, sb, seed, mask_flags=mf, layout=layout)
+    try:
+        assert b.params.layout == layout
+        ssx, ssy = int(layout != R.PIXEL_LAYOUT_I444), 0
+        assert b.plane_shape(1) == (h >> ssy, w >> ssx) == R.chroma_shape(layout, w, h)
+        assert np.array_equal(a.planes()[0], b.planes()[0])
+        oa, ob = O.filter_frame(a), O.filter_frame(b)
+        assert np.array_equal(oa[0], ob[0])
+        ga, gb = O.apply_grain(a, oa), O.apply_grain(b, ob)
+        assert np.array_equal(ga[0], gb[0])
+        for pl in (1, 2):
+            assert ob[pl].shape == b.plane_shape(pl)
+            assert (ob[pl] != b.planes()[pl]).mean() > 0.3
+            assert (gb[pl] != ob[pl]).mean() > 0.3
     finally:
         a.close()
         b.close()
