@@ -162,11 +162,11 @@ typedef struct Rav2dRestorationPlane {
 typedef struct Rav2dFrameParams {
     int32_t width;           /* frame width in luma samples            */
     int32_t height;          /* frame height in luma samples           */
-    int32_t bitdepth;        /* 8 or 10                                 */
-    int32_t layout;          /* RAV2D_PIXEL_LAYOUT_*: I420, or I400 (luma
-                                only: no chroma planes, pointers or
-                                strides needed; chroma side info and
-                                chroma parameters are ignored)           */
+    int32_t bitdepth;        /* 8, 10 or 12                             */
+    int32_t layout;          /* RAV2D_PIXEL_LAYOUT_*: I420, I422, I444, or
+                                I400 (luma only: no chroma planes,
+                                pointers or strides needed; chroma side
+                                info and chroma parameters are ignored)  */
     int32_t sb128;           /* superblock size 128 (1) or 64 (0)       */
     uint32_t inloop_filters; /* RAV2D_INLOOPFILTER_* mask               */
     /* sequence header */
@@ -262,11 +262,15 @@ int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f);
 /* Upload side info: n_mask = sb256 rows * sb256 cols Av2Filter records,
  * n_lr = the same number of Av2Restoration records (lr may be NULL when no
  * plane uses Wiener restoration).  The records are packed on the host
- * (rav2d_gpu_pack_filter) and only the packed form goes to the device. */
+ * (rav2d_gpu_pack_filter) and only the packed form goes to the device; for
+ * 4:2:2 and 4:4:4 frames the chroma edge and lossless masks go to the device
+ * at full size beside it. */
 int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask,
                                   size_t n_mask, const Rav2dAv2Restoration *lr,
                                   size_t n_lr, void *stream);
-/* The same from records already in the device form (no repacking). */
+/* The same from records already in the device form (no repacking): 4:2:0 and
+ * I400 frames (the packed form holds 4:2:0 chroma masks; RAV2D_EINVAL for
+ * 4:2:2 and 4:4:4 frames, which take rav2d_gpu_frame_set_side_info). */
 int rav2d_gpu_frame_set_side_info_packed(Rav2dGpuFrame *f, const Rav2dPackedFilter *mask,
                                          size_t n_mask, const Rav2dAv2Restoration *lr,
                                          size_t n_lr, void *stream);
@@ -277,7 +281,7 @@ int rav2d_gpu_pack_filter(const Rav2dAv2Filter *in, size_t n, Rav2dPackedFilter 
  * segmap_uv (decode.rs:285-310, 2355-2367), as deblock64_cols/rows read them
  * (deblock.rs:1623-2140).  segmap: one segment id (0..7) per luma 4x4 unit,
  * `stride` bytes per row, for the frame's ((W+7)>>3)*2 x ((H+7)>>3)*2 units;
- * segmap_uv: one id per chroma 4x4 unit (4:2:0: half as many each way),
+ * segmap_uv: one id per chroma 4x4 unit (subsampled as the chroma planes),
  * uv_stride bytes per row.  Read by the deblocking thresholds when
  * params.seg_enabled is set: a segmented frame with deblocking on must have
  * its maps set before it is filtered (RAV2D_EINVAL otherwise).  Ids above 7
@@ -347,7 +351,7 @@ int rav2d_gpu_filter_planes(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, const Rav2dPlane
 /* apply_grain_to_picture_mt (decode.rs:4672) on caller-owned device pictures:
  * dst = src with film grain `data` / `seed` (a copy when the data carries no
  * grain, picture_has_grain decode.rs:4641-4648).  Strides are multiples of 8
- * covering a row; 4:2:0.  The grain templates, scaling LUTs and block offsets
+ * covering a row; any layout.  The grain templates, scaling LUTs and block offsets
  * live in a scratch area this call allocates and frees, stream-ordered
  * (hipMallocAsync / hipFreeAsync on `stream`). */
 int rav2d_gpu_apply_grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *dst, int width,

@@ -17,6 +17,7 @@ struct HostBatch {
     // (DeblockVHArgs, tiles of 64 x deblock_rows) in the vertical-edge pass's
     // slot, and the horizontal-edge pass has no items
     int deblock_rows = 0;
+    bool any_ss = false;  // a frame has 4:2:2 / 4:4:4 chroma: the general deblock / CDEF / grain kernels
     std::vector<DeblockArgs> dbv[kDepths], dbh[kDepths];
     std::vector<DeblockVHArgs> dbvh[kDepths];
     std::vector<CdefArgs> cdef[kDepths];
@@ -37,6 +38,7 @@ struct HostBatch {
 // Device view of a packed HostBatch.
 struct DevBatch {
     int deblock_rows = 0;
+    bool any_ss = false;  // a frame has 4:2:2 / 4:4:4 chroma: the general deblock / CDEF / grain kernels
     DeblockArgs *dbv[kDepths] = {}, *dbh[kDepths] = {};
     DeblockVHArgs *dbvh[kDepths] = {};
     CdefArgs *cdef[kDepths] = {};

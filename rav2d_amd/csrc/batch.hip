@@ -89,6 +89,7 @@ int plan_batch(Rav2dGpuCtx *ctx, Rav2dGpuFrame *const *frames, int n, bool apply
         int r = plan_chain(ctx, f, RAV2D_PASS_LR, cp);
         if (r) return r;
         const int g = depth_group(f->g.bitdepth);
+        hb.any_ss = hb.any_ss || f->d_uvm != nullptr;
         // one read and one write of the frame's 1.5 * W * H samples
         const int64_t fbytes = 3 * (int64_t)f->pw[0] * f->ph[0] * f->bps;
         hb.pixels += (int64_t)f->pw[0] * f->ph[0];
@@ -171,6 +172,7 @@ void pack_batch(const HostBatch &hb, uint8_t *host, uint8_t *dev, DevBatch &db) 
     };
     db = DevBatch();
     db.deblock_rows = hb.deblock_rows;
+    db.any_ss = hb.any_ss;
     for (int g = 0; g < kDepths; g++) {
         put(L.dbv[g], hb.dbv[g].data(), hb.dbv[g].size() * sizeof(DeblockArgs));
         put(L.dbh[g], hb.dbh[g].data(), hb.dbh[g].size() * sizeof(DeblockArgs));
@@ -244,14 +246,17 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
             switch (p) {
                 case RAV2D_PASS_DEBLOCK_V:
                     if (db.deblock_rows)
-                        launch_deblock_vh_mix({{db.dbvh[0], db.dbvh[1], db.dbvh[2]}}, ms, grid, db.deblock_rows, s);
+                        launch_deblock_vh_mix({{db.dbvh[0], db.dbvh[1], db.dbvh[2]}}, ms, grid, db.deblock_rows,
+                                              db.any_ss, s);
                     else
-                        launch_deblock_mix(true, {{db.dbv[0], db.dbv[1], db.dbv[2]}}, ms, grid, s);
+                        launch_deblock_mix(true, {{db.dbv[0], db.dbv[1], db.dbv[2]}}, ms, grid, db.any_ss, s);
                     break;
-                case RAV2D_PASS_DEBLOCK_H: launch_deblock_mix(false, {{db.dbh[0], db.dbh[1], db.dbh[2]}}, ms, grid, s); break;
-                case RAV2D_PASS_CDEF: launch_cdef_mix({{db.cdef[0], db.cdef[1], db.cdef[2]}}, ms, grid, s); break;
+                case RAV2D_PASS_DEBLOCK_H:
+                    launch_deblock_mix(false, {{db.dbh[0], db.dbh[1], db.dbh[2]}}, ms, grid, db.any_ss, s);
+                    break;
+                case RAV2D_PASS_CDEF: launch_cdef_mix({{db.cdef[0], db.cdef[1], db.cdef[2]}}, ms, grid, db.any_ss, s); break;
                 case RAV2D_PASS_LR: launch_lr_mix({{db.lr[0], db.lr[1], db.lr[2]}}, ms, grid, s); break;
-                default: launch_grain_apply_mix({{db.gapply[0], db.gapply[1], db.gapply[2]}}, ms, grid, s); break;
+                default: launch_grain_apply_mix({{db.gapply[0], db.gapply[1], db.gapply[2]}}, ms, grid, db.any_ss, s); break;
             }
             HIP_TRY(hipGetLastError());
             if (p == ec.time_pass) HIP_TRY(hipEventRecord(ec.ev_b, s));
@@ -271,14 +276,14 @@ int enqueue_batch(const DevBatch &db, const LaunchShape &shape, bool apply_grain
             switch (p) {
                 case RAV2D_PASS_DEBLOCK_V:
                     if (db.deblock_rows)
-                        launch_deblock_vh_batch(db.dbvh[g], sc, grid, bd, db.deblock_rows, sg);
+                        launch_deblock_vh_batch(db.dbvh[g], sc, grid, bd, db.deblock_rows, db.any_ss, sg);
                     else
-                        launch_deblock_batch(true, db.dbv[g], sc, grid, bd, sg);
+                        launch_deblock_batch(true, db.dbv[g], sc, grid, bd, db.any_ss, sg);
                     break;
-                case RAV2D_PASS_DEBLOCK_H: launch_deblock_batch(false, db.dbh[g], sc, grid, bd, sg); break;
-                case RAV2D_PASS_CDEF: launch_cdef_batch(db.cdef[g], sc, grid, bd, sg); break;
+                case RAV2D_PASS_DEBLOCK_H: launch_deblock_batch(false, db.dbh[g], sc, grid, bd, db.any_ss, sg); break;
+                case RAV2D_PASS_CDEF: launch_cdef_batch(db.cdef[g], sc, grid, bd, db.any_ss, sg); break;
                 case RAV2D_PASS_LR: launch_lr_batch(db.lr[g], sc, grid, bd, sg); break;
-                default: launch_grain_apply_batch(db.gapply[g], sc, grid, bd, sg); break;
+                default: launch_grain_apply_batch(db.gapply[g], sc, grid, bd, db.any_ss, sg); break;
             }
             HIP_TRY(hipGetLastError());
         }

@@ -1,5 +1,5 @@
-// CDEF with CCSO on gfx950: one workgroup per 64x64 luma superblock (and its
-// two 32x32 chroma blocks), F2 -> F3.
+// CDEF with CCSO on gfx950: one workgroup per 64x64 luma superblock (and the
+// chroma of its area, in 32x32 sub-tiles), F2 -> F3.
 //
 // Arithmetic follows crates/rav2d/src/cdef.rs: cdef_find_dir_bd (:134-216),
 // adjust_strength (:226-236), constrain (:117-120), cdef_filter_block with its
@@ -259,7 +259,7 @@ struct CdefSmem {
     alignas(4) int16_t off[2][12][2];    // CDEF_DIRECTIONS as offsets in the luma / chroma tile
     alignas(4) int8_t ccso_lut[3][256];  // CCSO offset per band/edge index, per plane
     uint32_t dcost[8][64]; // direction-search cost of each direction per block
-    uint16_t ll[24];       // lossless 4x4 units of the superblock: 16 luma rows, 8 chroma rows
+    uint16_t ll[32];       // lossless 4x4 units of the superblock: 16 luma rows, 16 >> ss_ver chroma rows
 };
 
 // CCSO offset of one sample: `lt` = staged luma at the collocated position
@@ -319,29 +319,57 @@ struct CdefLoads {
     uint32_t lutw, offw;     // words of the CCSO offset tables and of the tap offsets
     int cdef_idx, ccso_pl;   // side info of the superblock
     uint32_t noskip_b;       // noskip row of the 8x8 block of lane t & 63 (block row (t >> 3) & 7)
-    uint32_t llw;            // lanes 0..23: one row of the superblock's lossless masks
+    uint32_t llw;            // lanes 0..31: one row of the superblock's lossless masks
 };
+
+constexpr uint32_t kOutside = 0x80008000u;  // INT16_MIN x 2
+
+// F2 groups of one 32x32 chroma sub-tile of both chroma planes (with its
+// 2-row / 4-column halo), at chroma sample (cx0, cy0), for LDS tile lc.
+template <typename P>
+__device__ __forceinline__ void cdef_load_chroma(const CdefArgs &a, int cx0, int cy0, int cw, int ch, uint2 (&vc)[kPC]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kPC; j++) {
+        const int i = t + j * kThreads;
+        const int pl = i >= LCR * kQC;
+        const int k = i - pl * LCR * kQC;
+        const int r = k / kQC, gx = cx0 - CX + (k - r * kQC) * 4, gy = cy0 - 2 + r;
+        uint2 v = make_uint2(kOutside, kOutside);
+        if (a.has_chroma)  // monochrome frames have no chroma planes
+            v = load4(row_ptr<const P>(a.src[1 + (pl & 1)], iclip(gy, 0, ch - 1)) + iclip(gx, 0, cw - 4));
+        const bool in = gx >= 0 && gx < cw && gy >= 0 && gy < ch;
+        vc[j].x = in ? v.x : kOutside;
+        vc[j].y = in ? v.y : kOutside;
+    }
+}
 
 // Every global load of a lane for superblock (sbx, sby), issued together.
 // F2 is read in 4-sample groups (tile origins and plane widths are multiples
 // of 4, so a group is wholly inside or outside the frame) from clamped,
 // always valid addresses, then selected.
-template <typename P>
+// kAny: a frame of any chroma subsampling (4:2:2 / 4:4:4 masks in a.uvm); the
+// 4:2:0 / I400 instantiation has the subsampling as constants.
+template <typename P, bool kAny>
 __device__ __forceinline__ void cdef_load(const CdefArgs &a, int sbx, int sby, CdefLoads &L) {
     const int t = threadIdx.x;
     const int x0 = sbx * 64, y0 = sby * 64;
     const int fw = a.g.bw * 4, fh = a.g.bh * 4;
-    const int cw = fw >> 1, ch = fh >> 1;
-    const Rav2dPackedFilter &m = a.mask[(sby >> 2) * a.g.sb256w + (sbx >> 2)];
+    const int ssx = kAny ? a.g.ss_hor : 1, ssy = kAny ? a.g.ss_ver : 1;
+    const int cw = fw >> ssx, ch = fh >> ssy;
+    const int sbi = (sby >> 2) * a.g.sb256w + (sbx >> 2);
+    const Rav2dPackedFilter &m = a.mask[sbi];
     L.cdef_idx = m.cdef_idx[((sby & 3) << 2) + (sbx & 3)];
     L.ccso_pl = (m.ccso[0] != 0) | ((m.ccso[1] != 0) << 1) | ((m.ccso[2] != 0) << 2);
     L.noskip_b = a.on_skiptx ? 0xffffu : m.noskip_mask[(((y0 >> 3) + ((t >> 3) & 7)) & 31)][sbx & 3];
-    // the lossless rows are read only when a segment is lossless (cdef.rs:894-911)
+    // the lossless rows are read only when a segment is lossless (cdef.rs:894-911):
+    // lanes 0..15 luma, 16.. the 16 >> ss_ver chroma rows of the superblock
     L.llw = 0;
-    if (a.any_ll && t < 24)
-        L.llw = t < 16 ? m.lossless_mask_y[((sby & 3) << 4) + t][sbx & 3]
-                       : m.lossless_mask_uv[((sby & 3) << 3) + t - 16][sbx & 3];
-    constexpr uint32_t kOutside = 0x80008000u;  // INT16_MIN x 2
+    const int uvr = 16 >> ssy;
+    if (a.any_ll && t < 16 + uvr) {
+        const uint16_t(*uvll)[4] = (kAny && a.uvm) ? a.uvm[sbi].lossless_mask_uv : m.lossless_mask_uv;
+        L.llw = t < 16 ? m.lossless_mask_y[((sby & 3) << 4) + t][sbx & 3] : uvll[(sby & 3) * uvr + t - 16][sbx & 3];
+    }
 #pragma unroll
     for (int j = 0; j < kPY; j++) {
         const int i = t + j * kThreads;
@@ -351,19 +379,7 @@ __device__ __forceinline__ void cdef_load(const CdefArgs &a, int sbx, int sby, C
         L.vy[j].x = in ? v.x : kOutside;  // per word: a vector select would go through scratch
         L.vy[j].y = in ? v.y : kOutside;
     }
-#pragma unroll
-    for (int j = 0; j < kPC; j++) {
-        const int i = t + j * kThreads;
-        const int pl = i >= LCR * kQC;
-        const int k = i - pl * LCR * kQC;
-        const int r = k / kQC, gx = (x0 >> 1) - CX + (k - r * kQC) * 4, gy = (y0 >> 1) - 2 + r;
-        uint2 v = make_uint2(kOutside, kOutside);
-        if (a.has_chroma)  // monochrome frames have no chroma planes
-            v = load4(row_ptr<const P>(a.src[1 + (pl & 1)], iclip(gy, 0, ch - 1)) + iclip(gx, 0, cw - 4));
-        const bool in = gx >= 0 && gx < cw && gy >= 0 && gy < ch;
-        L.vc[j].x = in ? v.x : kOutside;
-        L.vc[j].y = in ? v.y : kOutside;
-    }
+    cdef_load_chroma<P>(a, x0 >> ssx, y0 >> ssy, cw, ch, L.vc);
     // the CCSO offset tables (3 x 256 B) and the tap offsets (96 B) of the argument block
     L.lutw = t < 3 * 64 ? ((const uint32_t *)a.ccso[t >> 6].lut)[t & 63] : 0u;
     L.offw = t < 24 ? ((const uint32_t *)&a.off[0][0][0])[t] : 0u;
@@ -380,14 +396,12 @@ __device__ __forceinline__ void cdef_store(const CdefLoads &L, CdefSmem &sm) {
     for (int j = 0; j < kPC; j++)
         if (t + j * kThreads < kNC) ((uint2 *)&sm.lc[0][0])[t + j * kThreads] = L.vc[j];
     if (t < 3 * 64) ((uint32_t *)&sm.ccso_lut[0][0])[t] = L.lutw;
-    if (t < 24) {
-        ((uint32_t *)&sm.off[0][0][0])[t] = L.offw;
-        sm.ll[t] = (uint16_t)L.llw;
-    }
+    if (t < 24) ((uint32_t *)&sm.off[0][0][0])[t] = L.offw;
+    if (t < 32) sm.ll[t] = (uint16_t)L.llw;
 }
 
 // Direction search, filtering and CCSO of a superblock staged by cdef_store.
-template <typename P>
+template <typename P, bool kAny>
 __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby, const CdefLoads &L, CdefSmem &sm) {
     int16_t *ly = sm.ly;
     int16_t(*lc)[LCR * LC] = sm.lc;
@@ -396,7 +410,8 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
     const int t = threadIdx.x;
     const int x0 = sbx * 64, y0 = sby * 64;
     const int fw = a.g.bw * 4, fh = a.g.bh * 4;
-    const int cw = fw >> 1, ch = fh >> 1;
+    const int ssx = kAny ? a.g.ss_hor : 1, ssy = kAny ? a.g.ss_ver : 1;
+    const int cw = fw >> ssx, ch = fh >> ssy;
     const int bdm8 = a.g.bitdepth - 8;
     const uint64_t y_str = *(const uint64_t *)a.y_strength, uv_str = *(const uint64_t *)a.uv_strength;
     const uint32_t noskip_b = L.noskip_b;
@@ -473,7 +488,16 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
                 // (cdef.rs:922-926, 984-1025)
                 const int r = (t >> 3) * 2, sh = (t & 7) * 2;
                 const bool y_ll = ((sm.ll[r] | sm.ll[r + 1]) >> sh) & 3;
-                const bool uv_ll = (sm.ll[16 + (t >> 3)] >> (t & 7)) & 1;
+                bool uv_ll;
+                if constexpr (!kAny) {
+                    uv_ll = (sm.ll[16 + (t >> 3)] >> (t & 7)) & 1;
+                } else {
+                    // the block's chroma 4x4 units: rows r >> ss_ver (two without vertical
+                    // subsampling), columns sh >> ss_hor (two without horizontal)
+                    const int cr = r >> ssy;
+                    const uint32_t crow = sm.ll[16 + cr] | (ssy ? 0u : (uint32_t)sm.ll[16 + cr + 1]);
+                    uv_ll = (crow >> (sh >> ssx)) & (ssx ? 1u : 3u);
+                }
                 if (y_pri) {
                     int adj = adjust_strength(y_pri, var);
                     if ((adj || y_sec) && !y_ll) st |= 1;
@@ -540,65 +564,147 @@ __device__ __forceinline__ void cdef_filter(const CdefArgs &a, int sbx, int sby,
             }
         }
     }
-    // ---- chroma: lane t filters the column pair 2 (t & 15), +1 of the 4 rows of
-    // block row (t >> 4) & 7 of plane t >> 7 (one 4x4 block per lane)
-    {
-        const int pl = __builtin_amdgcn_readfirstlane(t >> 7), x = 2 * (t & 15), by = 4 * ((t >> 4) & 7);
-        const int gx = (x0 >> 1) + x;
-        const int bi = (by >> 2) * 8 + (x >> 2);
-        const bool on = b_state[bi] & 2;
-        const int dir = uv_pri ? b_dir[bi] : 0;
-        const int pri_tap = 4 - ((uv_pri >> bdm8) & 1);
-        const int pri_shift = uv_pri ? dmax(0, damping - 1 - ulog2((uint32_t)uv_pri)) : 0;
-        const int sec_shift = uv_sec ? damping - 1 - ulog2((uint32_t)uv_sec) : 0;
-        int op[2], os0[2], os1[2];
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            op[k] = sm.off[1][dir + 2][k];
-            os0[k] = sm.off[1][dir + 4][k];
-            os1[k] = sm.off[1][dir][k];
-        }
-        const bool cc = a.ccso_on && ((ccso_pl >> (1 + pl)) & 1);
-        if (a.has_chroma && gx < cw) {
-#pragma unroll 2
-            for (int yy = by; yy < by + 4; yy++) {
-                const int gy = (y0 >> 1) + yy;
-                if (gy >= ch) break;
-                const int16_t *tp = &lc[pl][(yy + 2) * LC + x + CX];
-                int v0 = tp[0], v1 = tp[1];
-                if (on) {
-                    const s16x2 r = cdef_px2(tp, op, os0, os1, uv_pri, pri_tap, pri_shift, uv_sec, sec_shift);
-                    v0 = (int)(P)(uint16_t)r.x;
-                    v1 = (int)(P)(uint16_t)r.y;
-                }
-                if (cc && !((sm.ll[16 + (yy >> 2)] >> (x >> 2)) & 1)) {
-                    const CcsoDev &c = a.ccso[1 + pl];
-                    const int8_t *lut = sm.ccso_lut[1 + pl];
-                    int o0, o1;
-                    if (cc_inner) {
-                        ccso_pair_inner<2>(c, lut, &ly[(2 * yy + 2) * LY + 2 * x + CX], c.dy * LY + c.dx, o0, o1);
-                    } else {
-                        o0 = ccso_offset(c, lut, ly, gx << 1, gy << 1, x0, y0, fw, fh);
-                        o1 = ccso_offset(c, lut, ly, (gx + 1) << 1, gy << 1, x0, y0, fw, fh);
+    if constexpr (!kAny) {
+        // ---- chroma: lane t filters the column pair 2 (t & 15), +1 of the 4 rows of
+        // block row (t >> 4) & 7 of plane t >> 7 (one 4x4 block per lane)
+        {
+            const int pl = __builtin_amdgcn_readfirstlane(t >> 7), x = 2 * (t & 15), by = 4 * ((t >> 4) & 7);
+            const int gx = (x0 >> 1) + x;
+            const int bi = (by >> 2) * 8 + (x >> 2);
+            const bool on = b_state[bi] & 2;
+            const int dir = uv_pri ? b_dir[bi] : 0;
+            const int pri_tap = 4 - ((uv_pri >> bdm8) & 1);
+            const int pri_shift = uv_pri ? dmax(0, damping - 1 - ulog2((uint32_t)uv_pri)) : 0;
+            const int sec_shift = uv_sec ? damping - 1 - ulog2((uint32_t)uv_sec) : 0;
+            int op[2], os0[2], os1[2];
+    #pragma unroll
+            for (int k = 0; k < 2; k++) {
+                op[k] = sm.off[1][dir + 2][k];
+                os0[k] = sm.off[1][dir + 4][k];
+                os1[k] = sm.off[1][dir][k];
+            }
+            const bool cc = a.ccso_on && ((ccso_pl >> (1 + pl)) & 1);
+            if (a.has_chroma && gx < cw) {
+    #pragma unroll 2
+                for (int yy = by; yy < by + 4; yy++) {
+                    const int gy = (y0 >> 1) + yy;
+                    if (gy >= ch) break;
+                    const int16_t *tp = &lc[pl][(yy + 2) * LC + x + CX];
+                    int v0 = tp[0], v1 = tp[1];
+                    if (on) {
+                        const s16x2 r = cdef_px2(tp, op, os0, os1, uv_pri, pri_tap, pri_shift, uv_sec, sec_shift);
+                        v0 = (int)(P)(uint16_t)r.x;
+                        v1 = (int)(P)(uint16_t)r.y;
                     }
-                    v0 = iclip(v0 + o0, 0, bdmax);
-                    v1 = iclip(v1 + o1, 0, bdmax);
+                    if (cc && !((sm.ll[16 + (yy >> 2)] >> (x >> 2)) & 1)) {
+                        const CcsoDev &c = a.ccso[1 + pl];
+                        const int8_t *lut = sm.ccso_lut[1 + pl];
+                        int o0, o1;
+                        if (cc_inner) {
+                            ccso_pair_inner<2>(c, lut, &ly[(2 * yy + 2) * LY + 2 * x + CX], c.dy * LY + c.dx, o0, o1);
+                        } else {
+                            o0 = ccso_offset(c, lut, ly, gx << 1, gy << 1, x0, y0, fw, fh);
+                            o1 = ccso_offset(c, lut, ly, (gx + 1) << 1, gy << 1, x0, y0, fw, fh);
+                        }
+                        v0 = iclip(v0 + o0, 0, bdmax);
+                        v1 = iclip(v1 + o1, 0, bdmax);
+                    }
+                    store2(row_ptr<P>(a.dst[1 + pl], gy) + gx, v0, v1);
                 }
-                store2(row_ptr<P>(a.dst[1 + pl], gy) + gx, v0, v1);
+            }
+        }
+    } else {
+        // ---- chroma, in 32x32 sub-tiles of each plane's (64 >> ss_hor) x (64 >> ss_ver)
+        // region (one for 4:2:0, two for 4:2:2, four for 4:4:4), each staged in lc in
+        // turn: lane t filters the column pair 2 (t & 15), +1 of the 4 rows 4 ((t >> 4) & 7)..
+        // of plane t >> 7, inside one luma 8x8 block
+        const int nsx = 2 >> ssx, nsub = nsx * (2 >> ssy);
+        // chroma direction from the luma direction: identity, or the 4:2:2 remap
+        // (cdef.rs:605) {7, 0, 2, 4, 5, 6, 6, 6} as nibbles
+        const uint32_t uvdir_map = (ssx && !ssy) ? 0x66654207u : 0x76543210u;
+        for (int st = 0; st < nsub; st++) {
+            const int sxo = (st % nsx) * 32, syo = (st / nsx) * 32;  // sub-tile origin in the region
+            if (st > 0) {
+                // once every lane is done with the previous sub-tile's LDS samples,
+                // stage this one (4:2:2 / 4:4:4 only; loaded and stored group by group)
+                __syncthreads();
+                const int cx0 = (x0 >> ssx) + sxo, cy0 = (y0 >> ssy) + syo;
+                for (int i = t; i < kNC; i += kThreads) {
+                    const int cp = i >= LCR * kQC;
+                    const int k = i - cp * LCR * kQC;
+                    const int r = k / kQC, gx = cx0 - CX + (k - r * kQC) * 4, gy = cy0 - 2 + r;
+                    uint2 v = make_uint2(kOutside, kOutside);
+                    if (gx >= 0 && gx < cw && gy >= 0 && gy < ch) v = load4(row_ptr<const P>(a.src[1 + cp], gy) + gx);
+                    ((uint2 *)&lc[0][0])[i] = v;
+                }
+                __syncthreads();
+            }
+            const int pl = __builtin_amdgcn_readfirstlane(t >> 7), x = 2 * (t & 15), by = 4 * ((t >> 4) & 7);
+            const int cx = sxo + x, cyb = syo + by;  // chroma position in the superblock's region
+            const int gx = (x0 >> ssx) + cx;
+            const int bi = ((cyb << ssy) >> 3) * 8 + ((cx << ssx) >> 3);  // the luma 8x8 block
+            const bool on = b_state[bi] & 2;
+            const int dir = uv_pri ? (int)((uvdir_map >> (4 * b_dir[bi])) & 15) : 0;
+            const int pri_tap = 4 - ((uv_pri >> bdm8) & 1);
+            const int pri_shift = uv_pri ? dmax(0, damping - 1 - ulog2((uint32_t)uv_pri)) : 0;
+            const int sec_shift = uv_sec ? damping - 1 - ulog2((uint32_t)uv_sec) : 0;
+            int op[2], os0[2], os1[2];
+    #pragma unroll
+            for (int k = 0; k < 2; k++) {
+                op[k] = sm.off[1][dir + 2][k];
+                os0[k] = sm.off[1][dir + 4][k];
+                os1[k] = sm.off[1][dir][k];
+            }
+            const bool cc = a.ccso_on && ((ccso_pl >> (1 + pl)) & 1);
+            if (a.has_chroma && gx < cw) {
+    #pragma unroll 2
+                for (int yy = by; yy < by + 4; yy++) {
+                    const int cy = syo + yy;
+                    const int gy = (y0 >> ssy) + cy;
+                    if (gy >= ch) break;
+                    const int16_t *tp = &lc[pl][(yy + 2) * LC + x + CX];
+                    int v0 = tp[0], v1 = tp[1];
+                    if (on) {
+                        const s16x2 r = cdef_px2(tp, op, os0, os1, uv_pri, pri_tap, pri_shift, uv_sec, sec_shift);
+                        v0 = (int)(P)(uint16_t)r.x;
+                        v1 = (int)(P)(uint16_t)r.y;
+                    }
+                    if (cc && !((sm.ll[16 + (cy >> 2)] >> (cx >> 2)) & 1)) {
+                        const CcsoDev &c = a.ccso[1 + pl];
+                        const int8_t *lut = sm.ccso_lut[1 + pl];
+                        int o0, o1;
+                        if (cc_inner) {
+                            // collocated luma of the left sample; the right one's is 1 << ss_hor further
+                            const int16_t *lp = &ly[((cy << ssy) + 2) * LY + (cx << ssx) + CX];
+                            if (ssx)
+                                ccso_pair_inner<2>(c, lut, lp, c.dy * LY + c.dx, o0, o1);
+                            else
+                                ccso_pair_inner<1>(c, lut, lp, c.dy * LY + c.dx, o0, o1);
+                        } else {
+                            o0 = ccso_offset(c, lut, ly, gx << ssx, gy << ssy, x0, y0, fw, fh);
+                            o1 = ccso_offset(c, lut, ly, (gx + 1) << ssx, gy << ssy, x0, y0, fw, fh);
+                        }
+                        v0 = iclip(v0 + o0, 0, bdmax);
+                        v1 = iclip(v1 + o1, 0, bdmax);
+                    }
+                    store2(row_ptr<P>(a.dst[1 + pl], gy) + gx, v0, v1);
+                }
             }
         }
     }
 }
 
-template <typename P>
+// kAny instantiations for frames of any chroma subsampling (4:2:2 / 4:4:4),
+// picked by the launchers
+template <typename P, bool kAny>
 __device__ __forceinline__ void cdef_body(const CdefArgs &a, int sbx, int sby, CdefSmem &sm) {
     CdefLoads L;
-    cdef_load<P>(a, sbx, sby, L);
+    cdef_load<P, kAny>(a, sbx, sby, L);
     cdef_store(L, sm);
-    cdef_filter<P>(a, sbx, sby, L, sm);
+    cdef_filter<P, kAny>(a, sbx, sby, L, sm);
 }
 
-template <typename P>
+template <typename P, bool kAny>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void cdef_kernel(CdefArgs a) {
     __shared__ CdefSmem sm;
     // XCD-contiguous superblock runs (neighbours share halo rows and columns in
@@ -606,11 +712,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
     const int sbw = (a.g.bw * 4 + 63) / 64;
     const int slot = xcd_item(blockIdx.x, gridDim.x);
     const int sb = a.order ? a.order[slot] : slot;
-    cdef_body<P>(a, sb % sbw, sb / sbw, sm);
+    cdef_body<P, kAny>(a, sb % sbw, sb / sbw, sm);
 }
 
 // Batched launch: one item per 64x64 superblock of every frame of one bit depth.
-template <typename P, bool kLoop>
+template <typename P, bool kLoop, bool kAny>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void cdef_batch(const CdefArgs *__restrict__ args,
                                                                                         BatchSched s) {
     __shared__ CdefSmem sm;
@@ -618,13 +724,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
         const CdefArgs &a = args[f];
         const int sbw = (a.g.bw * 4 + 63) >> 6;
         const int sby = t / sbw, sbx = t - sby * sbw;
-        cdef_body<P>(a, sbx, sby, sm);
+        cdef_body<P, kAny>(a, sbx, sby, sm);
         __syncthreads();
     });
 }
 
 // Mixed launch: the superblocks of every bit-depth group's frames.
-template <bool kLoop>
+template <bool kLoop, bool kAny>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void cdef_mix(DevArgs3<CdefArgs> args,
                                                                                       MixSched s) {
     __shared__ CdefSmem sm;
@@ -633,19 +739,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
         const int sbw = (a.g.bw * 4 + 63) >> 6;
         const int sby = t / sbw, sbx = t - sby * sbw;
         if (g == 0)
-            cdef_body<uint8_t>(a, sbx, sby, sm);
+            cdef_body<uint8_t, kAny>(a, sbx, sby, sm);
         else
-            cdef_body<uint16_t>(a, sbx, sby, sm);
+            cdef_body<uint16_t, kAny>(a, sbx, sby, sm);
         __syncthreads();
     });
 }
 
-void launch_cdef_mix(DevArgs3<CdefArgs> args, const MixSched &s, int grid, hipStream_t st) {
+void launch_cdef_mix(DevArgs3<CdefArgs> args, const MixSched &s, int grid, bool any, hipStream_t st) {
     if (s.first[3] <= 0) return;
-    if (grid < s.first[3])
-        R2G_LAUNCH((cdef_mix<true>), dim3(grid), dim3(kThreads), 0, st, args, s);
-    else
-        R2G_LAUNCH((cdef_mix<false>), dim3(grid), dim3(kThreads), 0, st, args, s);
+    const bool loop = grid < s.first[3];
+    if (any) {
+        if (loop)
+            R2G_LAUNCH((cdef_mix<true, true>), dim3(grid), dim3(kThreads), 0, st, args, s);
+        else
+            R2G_LAUNCH((cdef_mix<false, true>), dim3(grid), dim3(kThreads), 0, st, args, s);
+    } else {
+        if (loop)
+            R2G_LAUNCH((cdef_mix<true, false>), dim3(grid), dim3(kThreads), 0, st, args, s);
+        else
+            R2G_LAUNCH((cdef_mix<false, false>), dim3(grid), dim3(kThreads), 0, st, args, s);
+    }
 }
 
 int cdef_items(const CdefArgs &a) { return ((a.g.bw * 4 + 63) >> 6) * ((a.g.bh * 4 + 63) >> 6); }
@@ -669,28 +783,45 @@ void cdef_tap_offsets(CdefArgs &a) {
         }
 }
 
-void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st) {
+void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, int bitdepth, bool any,
+                       hipStream_t st) {
     if (s.total <= 0) return;
     const bool loop = grid < s.total;
+#define R2G_CB(P, L)                                                                                       \
+    do {                                                                                                   \
+        if (any)                                                                                           \
+            R2G_LAUNCH((cdef_batch<P, L, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);          \
+        else                                                                                               \
+            R2G_LAUNCH((cdef_batch<P, L, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);         \
+    } while (0)
     if (bitdepth > 8) {
         if (loop)
-            R2G_LAUNCH((cdef_batch<uint16_t, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+            R2G_CB(uint16_t, true);
         else
-            R2G_LAUNCH((cdef_batch<uint16_t, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+            R2G_CB(uint16_t, false);
     } else {
         if (loop)
-            R2G_LAUNCH((cdef_batch<uint8_t, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+            R2G_CB(uint8_t, true);
         else
-            R2G_LAUNCH((cdef_batch<uint8_t, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);
+            R2G_CB(uint8_t, false);
     }
+#undef R2G_CB
 }
 
 void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s) {
     dim3 grid(((a.g.bw * 4 + 63) / 64) * ((a.g.bh * 4 + 63) / 64));
-    if (bitdepth > 8)
-        R2G_LAUNCH(cdef_kernel<uint16_t>, grid, dim3(kThreads), 0, s, a);
-    else
-        R2G_LAUNCH(cdef_kernel<uint8_t>, grid, dim3(kThreads), 0, s, a);
+    const bool any = a.uvm != nullptr;
+    if (bitdepth > 8) {
+        if (any)
+            R2G_LAUNCH((cdef_kernel<uint16_t, true>), grid, dim3(kThreads), 0, s, a);
+        else
+            R2G_LAUNCH((cdef_kernel<uint16_t, false>), grid, dim3(kThreads), 0, s, a);
+    } else {
+        if (any)
+            R2G_LAUNCH((cdef_kernel<uint8_t, true>), grid, dim3(kThreads), 0, s, a);
+        else
+            R2G_LAUNCH((cdef_kernel<uint8_t, false>), grid, dim3(kThreads), 0, s, a);
+    }
 }
 
 }  // namespace r2g

@@ -138,51 +138,114 @@ struct EdgeDesc {
 // unit (x4, y4); pl selects luma or chroma indexing.  Every side-info load is
 // issued unconditionally (one round of loads); the thresholds come later from
 // the LUT rows staged in LDS.
+// kAny: a frame of any chroma subsampling (4:2:2 / 4:4:4 masks in a.uvm); the
+// 4:2:0 / I400 instantiation has the subsampling as constants.
+template <bool kAny>
 __device__ __forceinline__ EdgeDesc edge_desc(const DeblockArgs &a, int pl, bool vert, int x4, int y4) {
-    EdgeDesc e;
-    const int sbw = a.g.sb256w;
-    const bool ch = pl != 0;
-    // luma 4px units of this sample (4:2:0 chroma)
-    const int lx4 = ch ? x4 * 2 : x4, ly4 = ch ? y4 * 2 : y4;
-    const Rav2dPackedFilter &f = sb_at(a.mask, sbw, lx4, ly4);
-    const int along = vert ? x4 : y4;   // index of the edge line within the SB256
-    const int across = vert ? y4 : x4;  // position along the edge
-    const int line = ch ? (along & 31) : (along & 63);
-    const int sidx = ch ? ((across >> 4) & 1) : ((across >> 4) & 3);
-    const uint32_t bit = 1u << (across & 15);
-    // level words 0..4 of the edge line (chroma lines hold two words per level)
-    const uint16_t *lw = ch ? &f.filter_uv[vert ? 0 : 1][line][0][sidx] : &f.filter_y[vert ? 0 : 1][line][0][sidx];
-    const int ls = ch ? 2 : 4;
-    const uint32_t m0 = lw[0], m1 = lw[ls], m2 = lw[2 * ls];
-    const uint32_t m3 = ch ? 0u : (uint32_t)lw[3 * ls], m4 = lw[4 * ls];
-    const int nx4 = vert ? x4 - 1 : x4, ny4 = vert ? y4 : y4 - 1;
-    const int lnx4 = ch ? nx4 * 2 : nx4, lny4 = ch ? ny4 * 2 : ny4;
-    e.qc = qidx_at(a.mask, sbw, lx4, ly4);
-    e.qp = qidx_at(a.mask, sbw, lnx4, lny4);
-    if (a.seg[0]) {
-        // per-segment qidx of this unit and of its neighbour (setup_thr_cols/rows
-        // read the segment map at both, deblock.rs:1477-1567)
-        const uint8_t *sm = a.seg[ch ? 1 : 0];
-        const int st = a.seg_stride[ch ? 1 : 0];
-        e.qc = iclip(e.qc + a.seg_dq[sm[y4 * st + x4] & 7], 0, a.qmax);
-        e.qp = iclip(e.qp + a.seg_dq[sm[ny4 * st + nx4] & 7], 0, a.qmax);
-    }
-    const Rav2dPackedFilter &fn = sb_at(a.mask, sbw, lnx4, lny4);
-    if (ch) {
-        e.pos_ll = (f.lossless_mask_uv[y4 & 31][(x4 >> 3) & 3] >> (x4 & 7)) & 1;
-        e.neg_ll = (fn.lossless_mask_uv[ny4 & 31][(nx4 >> 3) & 3] >> (nx4 & 7)) & 1;
+    if constexpr (!kAny) {
+        // 4:2:0 / I400: chroma units are luma units >> 1; masks in the packed record
+        EdgeDesc e;
+        const int sbw = a.g.sb256w;
+        const bool ch = pl != 0;
+        // luma 4px units of this sample (4:2:0 chroma)
+        const int lx4 = ch ? x4 * 2 : x4, ly4 = ch ? y4 * 2 : y4;
+        const Rav2dPackedFilter &f = sb_at(a.mask, sbw, lx4, ly4);
+        const int along = vert ? x4 : y4;   // index of the edge line within the SB256
+        const int across = vert ? y4 : x4;  // position along the edge
+        const int line = ch ? (along & 31) : (along & 63);
+        const int sidx = ch ? ((across >> 4) & 1) : ((across >> 4) & 3);
+        const uint32_t bit = 1u << (across & 15);
+        // level words 0..4 of the edge line (chroma lines hold two words per level)
+        const uint16_t *lw = ch ? &f.filter_uv[vert ? 0 : 1][line][0][sidx] : &f.filter_y[vert ? 0 : 1][line][0][sidx];
+        const int ls = ch ? 2 : 4;
+        const uint32_t m0 = lw[0], m1 = lw[ls], m2 = lw[2 * ls];
+        const uint32_t m3 = ch ? 0u : (uint32_t)lw[3 * ls], m4 = lw[4 * ls];
+        const int nx4 = vert ? x4 - 1 : x4, ny4 = vert ? y4 : y4 - 1;
+        const int lnx4 = ch ? nx4 * 2 : nx4, lny4 = ch ? ny4 * 2 : ny4;
+        e.qc = qidx_at(a.mask, sbw, lx4, ly4);
+        e.qp = qidx_at(a.mask, sbw, lnx4, lny4);
+        if (a.seg[0]) {
+            // per-segment qidx of this unit and of its neighbour (setup_thr_cols/rows
+            // read the segment map at both, deblock.rs:1477-1567)
+            const uint8_t *sm = a.seg[ch ? 1 : 0];
+            const int st = a.seg_stride[ch ? 1 : 0];
+            e.qc = iclip(e.qc + a.seg_dq[sm[y4 * st + x4] & 7], 0, a.qmax);
+            e.qp = iclip(e.qp + a.seg_dq[sm[ny4 * st + nx4] & 7], 0, a.qmax);
+        }
+        const Rav2dPackedFilter &fn = sb_at(a.mask, sbw, lnx4, lny4);
+        if (ch) {
+            e.pos_ll = (f.lossless_mask_uv[y4 & 31][(x4 >> 3) & 3] >> (x4 & 7)) & 1;
+            e.neg_ll = (fn.lossless_mask_uv[ny4 & 31][(nx4 >> 3) & 3] >> (nx4 & 7)) & 1;
+        } else {
+            e.pos_ll = (f.lossless_mask_y[y4 & 63][(x4 >> 4) & 3] >> (x4 & 15)) & 1;
+            e.neg_ll = (fn.lossless_mask_y[ny4 & 63][(nx4 >> 4) & 3] >> (nx4 & 15)) & 1;
+        }
+        e.on = ((m0 | m1 | m2 | m3) & bit) != 0;
+        const int idx = (m3 & bit) ? 3 : (m2 & bit) ? 2 : ((m1 & bit) ? 1 : 0);
+        e.mwp = byte_at(ch ? kMaxWidthUV : kMaxWidthY, idx);
+        e.subpu = (m4 & bit) ? 3 : 0;
+        // the rows pass clamps the negative side at the top of every 64-luma-row band
+        const bool band_top = !vert && ((ch ? (y4 & 7) : (y4 & 15)) == 0);
+        e.mwn = band_top ? dmin(ch ? 2 : 6, e.mwp) : e.mwp;
+        return e;
     } else {
-        e.pos_ll = (f.lossless_mask_y[y4 & 63][(x4 >> 4) & 3] >> (x4 & 15)) & 1;
-        e.neg_ll = (fn.lossless_mask_y[ny4 & 63][(nx4 >> 4) & 3] >> (nx4 & 15)) & 1;
+        EdgeDesc e;
+        const int sbw = a.g.sb256w;
+        const bool ch = pl != 0;
+        // chroma subsampling (luma: none); the packed record holds 4:2:0 chroma
+        // masks, 4:2:2 / 4:4:4 frames carry theirs at full size in a.uvm
+        const int sx = ch ? (kAny ? a.g.ss_hor : 1) : 0, sy = ch ? (kAny ? a.g.ss_ver : 1) : 0;
+        // luma 4px units of this sample
+        const int lx4 = x4 << sx, ly4 = y4 << sy;
+        const int sbi = (ly4 >> 6) * sbw + (lx4 >> 6);
+        const Rav2dPackedFilter &f = a.mask[sbi];
+        const ChromaMasks *um = (kAny && ch) ? a.uvm : nullptr;
+        const int along = vert ? x4 : y4;   // index of the edge line within the SB256
+        const int across = vert ? y4 : x4;  // position along the edge
+        const int s_al = vert ? sx : sy, s_ac = vert ? sy : sx;
+        const int line = along & ((64 >> s_al) - 1);
+        const int sidx = (across >> 4) & ((4 >> s_ac) - 1);
+        const uint32_t bit = 1u << (across & 15);
+        // level words 0..4 of the edge line (4:2:0 chroma lines hold two words per level)
+        const uint16_t *lw = um ? &um[sbi].filter_uv[vert ? 0 : 1][line][0][sidx]
+                                : (ch ? &f.filter_uv[vert ? 0 : 1][line][0][sidx] : &f.filter_y[vert ? 0 : 1][line][0][sidx]);
+        const int ls = (ch && !um) ? 2 : 4;
+        const uint32_t m0 = lw[0], m1 = lw[ls], m2 = lw[2 * ls];
+        const uint32_t m3 = ch ? 0u : (uint32_t)lw[3 * ls], m4 = lw[4 * ls];
+        const int nx4 = vert ? x4 - 1 : x4, ny4 = vert ? y4 : y4 - 1;
+        const int lnx4 = nx4 << sx, lny4 = ny4 << sy;
+        const int nsbi = (lny4 >> 6) * sbw + (lnx4 >> 6);
+        e.qc = qidx_at(a.mask, sbw, lx4, ly4);
+        e.qp = qidx_at(a.mask, sbw, lnx4, lny4);
+        if (a.seg[0]) {
+            // per-segment qidx of this unit and of its neighbour (setup_thr_cols/rows
+            // read the segment map at both, deblock.rs:1477-1567)
+            const uint8_t *sm = a.seg[ch ? 1 : 0];
+            const int st = a.seg_stride[ch ? 1 : 0];
+            e.qc = iclip(e.qc + a.seg_dq[sm[y4 * st + x4] & 7], 0, a.qmax);
+            e.qp = iclip(e.qp + a.seg_dq[sm[ny4 * st + nx4] & 7], 0, a.qmax);
+        }
+        const Rav2dPackedFilter &fn = a.mask[nsbi];
+        if (ch) {
+            // chroma lossless rows: one word per 64 luma columns, 16 >> sx units each
+            const int rows = (64 >> sy) - 1, wsh = 4 - sx, bm = (16 >> sx) - 1;
+            const uint16_t(*ll)[4] = um ? um[sbi].lossless_mask_uv : f.lossless_mask_uv;
+            const uint16_t(*nll)[4] = um ? um[nsbi].lossless_mask_uv : fn.lossless_mask_uv;
+            e.pos_ll = (ll[y4 & rows][(x4 >> wsh) & 3] >> (x4 & bm)) & 1;
+            e.neg_ll = (nll[ny4 & rows][(nx4 >> wsh) & 3] >> (nx4 & bm)) & 1;
+        } else {
+            e.pos_ll = (f.lossless_mask_y[y4 & 63][(x4 >> 4) & 3] >> (x4 & 15)) & 1;
+            e.neg_ll = (fn.lossless_mask_y[ny4 & 63][(nx4 >> 4) & 3] >> (nx4 & 15)) & 1;
+        }
+        e.on = ((m0 | m1 | m2 | m3) & bit) != 0;
+        const int idx = (m3 & bit) ? 3 : (m2 & bit) ? 2 : ((m1 & bit) ? 1 : 0);
+        e.mwp = byte_at(ch ? kMaxWidthUV : kMaxWidthY, idx);
+        e.subpu = (m4 & bit) ? 3 : 0;
+        // the rows pass clamps the negative side at the top of every 64-luma-row band
+        const bool band_top = !vert && ((y4 & ((16 >> sy) - 1)) == 0);
+        e.mwn = band_top ? dmin(ch ? 2 : 6, e.mwp) : e.mwp;
+        return e;
     }
-    e.on = ((m0 | m1 | m2 | m3) & bit) != 0;
-    const int idx = (m3 & bit) ? 3 : (m2 & bit) ? 2 : ((m1 & bit) ? 1 : 0);
-    e.mwp = byte_at(ch ? kMaxWidthUV : kMaxWidthY, idx);
-    e.subpu = (m4 & bit) ? 3 : 0;
-    // the rows pass clamps the negative side at the top of every 64-luma-row band
-    const bool band_top = !vert && ((ch ? (y4 & 7) : (y4 & 15)) == 0);
-    e.mwn = band_top ? dmin(ch ? 2 : 6, e.mwp) : e.mwp;
-    return e;
 }
 
 constexpr int kTW = 64, kTH = 32, kHalo = 16, kThreads = 256;
@@ -191,7 +254,7 @@ constexpr int kTW = 64, kTH = 32, kHalo = 16, kThreads = 256;
 // rows and its edge's side-info loads in one round, then stores the tile and
 // the LUT rows to LDS; thresholds are looked up there.  Every (edge line, 4-line segment) of a
 // tile is one lane (152 for V, 176 for H).
-template <typename P>
+template <typename P, bool kAny>
 __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW + 2 * kHalo],
                                                uint16_t (*lut)[512]) {
     int pl, tx, ty;
@@ -228,7 +291,7 @@ __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id
     const int rs = t / NE, ec = t - rs * NE;
     const int ex = x0 - 4 + ec * 4, ey = y0 + rs * 4;
     // no filtering at the frame's left edge
-    if (a.plane_on[pl] && t < NE * (kTH / 4) && ex > 0 && ex < W && ey < H) e = edge_desc(a, pl, true, ex >> 2, ey >> 2);
+    if (a.plane_on[pl] && t < NE * (kTH / 4) && ex > 0 && ex < W && ey < H) e = edge_desc<kAny>(a, pl, true, ex >> 2, ey >> 2);
 #pragma unroll
     for (int j = 0; j < PV; j++) ((uint2 *)&tile[0][0])[t + j * kThreads] = v[j];
     ((uint2 *)&lut[0][0])[t] = lutv;
@@ -248,7 +311,7 @@ __device__ __forceinline__ void deblock_v_body(const DeblockArgs &a, int tile_id
     }
 }
 
-template <typename P>
+template <typename P, bool kAny>
 __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id, uint16_t (*tile)[kTW],
                                                uint16_t (*lut)[512]) {
     int pl, tx, ty;
@@ -278,7 +341,7 @@ __device__ __forceinline__ void deblock_h_body(const DeblockArgs &a, int tile_id
     const int er = t / (kTW / 4), cs = t - er * (kTW / 4);
     const int ey = y0 - 4 + er * 4, ex = x0 + cs * 4;
     // no filtering at the frame's top edge
-    if (a.plane_on[pl] && t < NE * (kTW / 4) && ey > 0 && ey < H && ex < W) e = edge_desc(a, pl, false, ex >> 2, ey >> 2);
+    if (a.plane_on[pl] && t < NE * (kTW / 4) && ey > 0 && ey < H && ex < W) e = edge_desc<kAny>(a, pl, false, ex >> 2, ey >> 2);
 #pragma unroll
     for (int j = 0; j < PV; j++) ((uint2 *)&tile[0][0])[t + j * kThreads] = v[j];
     ((uint2 *)&lut[0][0])[t] = lutv;
@@ -315,7 +378,7 @@ template <int TH> struct VHTile {
     uint16_t lut[4][512];  // q / side thresholds of the vertical-edge kind, then of the horizontal-edge kind
 };
 
-template <typename P, int TH>
+template <typename P, int TH, bool kAny>
 __device__ __forceinline__ void deblock_vh_body(const DeblockVHArgs &f, int tile_id, VHTile<TH> &sm) {
     const DeblockArgs &a = f.a;
     constexpr int LW = VHTile<TH>::LW, R = VHTile<TH>::R;
@@ -348,7 +411,7 @@ __device__ __forceinline__ void deblock_vh_body(const DeblockVHArgs &f, int tile
         ev[s].on = false;
         const int idx = t + s * kThreads;
         const int rs = idx / NEV, ex = x0 - 4 + (idx - rs * NEV) * 4, ey = ys + rs * 4;
-        if (f.on_v[pl] && idx < NSV && ex > 0 && ex < W && ey >= 0 && ey < H) ev[s] = edge_desc(a, pl, true, ex >> 2, ey >> 2);
+        if (f.on_v[pl] && idx < NSV && ex > 0 && ex < W && ey >= 0 && ey < H) ev[s] = edge_desc<kAny>(a, pl, true, ex >> 2, ey >> 2);
     }
 #pragma unroll
     for (int j = 0; j < PV; j++)
@@ -375,7 +438,7 @@ __device__ __forceinline__ void deblock_vh_body(const DeblockVHArgs &f, int tile
         const int er = idx / (kTW / 4), cs = idx - er * (kTW / 4);
         const int ey = y0 - 4 + er * 4, ex = x0 + cs * 4;
         if (f.on_h[pl] && idx < NSH && ey > 0 && ey < H && ex < W) {
-            const EdgeDesc e = edge_desc(a, pl, false, ex >> 2, ey >> 2);
+            const EdgeDesc e = edge_desc<kAny>(a, pl, false, ex >> 2, ey >> 2);
             if (e.on) {
                 const uint32_t q = (uint32_t)(edge_thr(sm.lut[2][e.qc], sm.lut[2][e.qp]) >> e.subpu);
                 const uint32_t side = (uint32_t)(edge_thr(sm.lut[3][e.qc], sm.lut[3][e.qp]) >> e.subpu);
@@ -394,126 +457,143 @@ __device__ __forceinline__ void deblock_vh_body(const DeblockVHArgs &f, int tile
     }
 }
 
-template <typename P, int TH> __global__ __launch_bounds__(kThreads) void deblock_vh_kernel(DeblockVHArgs a) {
+// Kernels: kAny instantiations for frames of any chroma subsampling (the
+// launchers pick them for 4:2:2 / 4:4:4 frames; a body holding both forms
+// would cost the 4:2:0 form registers and occupancy).
+template <typename P, int TH, bool kAny>
+__global__ __launch_bounds__(kThreads) void deblock_vh_kernel(DeblockVHArgs a) {
     __shared__ __align__(16) VHTile<TH> sm;
-    deblock_vh_body<P, TH>(a, blockIdx.x, sm);
+    deblock_vh_body<P, TH, kAny>(a, blockIdx.x, sm);
 }
 
-template <typename P, int TH, bool kLoop>
+template <typename P, int TH, bool kLoop, bool kAny>
 __global__ __launch_bounds__(kThreads) void deblock_vh_batch(const DeblockVHArgs *__restrict__ args, BatchSched s) {
     __shared__ __align__(16) VHTile<TH> sm;
     for_each_item<kLoop>(s, [&](int f, int t) {
-        deblock_vh_body<P, TH>(args[f], t, sm);
+        deblock_vh_body<P, TH, kAny>(args[f], t, sm);
         __syncthreads();
     });
 }
 
-template <typename P> __global__ __launch_bounds__(kThreads) void deblock_v_kernel(DeblockArgs a) {
+template <typename P, bool kAny> __global__ __launch_bounds__(kThreads) void deblock_v_kernel(DeblockArgs a) {
     __shared__ __align__(16) uint16_t tile[kTH][kTW + 2 * kHalo];
     __shared__ __align__(16) uint16_t lut[2][512];
-    deblock_v_body<P>(a, blockIdx.x, tile, lut);
+    deblock_v_body<P, kAny>(a, blockIdx.x, tile, lut);
 }
 
-template <typename P> __global__ __launch_bounds__(kThreads) void deblock_h_kernel(DeblockArgs a) {
+template <typename P, bool kAny> __global__ __launch_bounds__(kThreads) void deblock_h_kernel(DeblockArgs a) {
     __shared__ __align__(16) uint16_t tile[kTH + 2 * kHalo][kTW];
     __shared__ __align__(16) uint16_t lut[2][512];
-    deblock_h_body<P>(a, blockIdx.x, tile, lut);
+    deblock_h_body<P, kAny>(a, blockIdx.x, tile, lut);
 }
 
 // Batched launches: every tile of every frame of one bit depth.
-template <typename P, bool kLoop>
+template <typename P, bool kLoop, bool kAny>
 __global__ __launch_bounds__(kThreads) void deblock_v_batch(const DeblockArgs *__restrict__ args, BatchSched s) {
     __shared__ __align__(16) uint16_t tile[kTH][kTW + 2 * kHalo];
     __shared__ __align__(16) uint16_t lut[2][512];
     for_each_item<kLoop>(s, [&](int f, int t) {
-        deblock_v_body<P>(args[f], t, tile, lut);
+        deblock_v_body<P, kAny>(args[f], t, tile, lut);
         __syncthreads();
     });
 }
 
-template <typename P, bool kLoop>
+template <typename P, bool kLoop, bool kAny>
 __global__ __launch_bounds__(kThreads) void deblock_h_batch(const DeblockArgs *__restrict__ args, BatchSched s) {
     __shared__ __align__(16) uint16_t tile[kTH + 2 * kHalo][kTW];
     __shared__ __align__(16) uint16_t lut[2][512];
     for_each_item<kLoop>(s, [&](int f, int t) {
-        deblock_h_body<P>(args[f], t, tile, lut);
+        deblock_h_body<P, kAny>(args[f], t, tile, lut);
         __syncthreads();
     });
 }
 
 // Mixed launches: the tiles of every bit-depth group's frames in one grid;
 // group 0 (8-bit) runs the 8-bit body, groups 1 and 2 the 16-bit one.
-template <bool kVert, bool kLoop>
+template <bool kVert, bool kLoop, bool kAny>
 __global__ __launch_bounds__(kThreads) void deblock_mix(DevArgs3<DeblockArgs> args, MixSched s) {
     __shared__ __align__(16) uint16_t tile[kVert ? kTH : kTH + 2 * kHalo][kVert ? kTW + 2 * kHalo : kTW];
     __shared__ __align__(16) uint16_t lut[2][512];
     for_each_mixed_item<kLoop>(s, [&](int g, int f, int t) {
         if constexpr (kVert) {
             if (g == 0)
-                deblock_v_body<uint8_t>(args.p[0][f], t, tile, lut);
+                deblock_v_body<uint8_t, kAny>(args.p[0][f], t, tile, lut);
             else
-                deblock_v_body<uint16_t>(args.p[g][f], t, tile, lut);
+                deblock_v_body<uint16_t, kAny>(args.p[g][f], t, tile, lut);
         } else {
             if (g == 0)
-                deblock_h_body<uint8_t>(args.p[0][f], t, tile, lut);
+                deblock_h_body<uint8_t, kAny>(args.p[0][f], t, tile, lut);
             else
-                deblock_h_body<uint16_t>(args.p[g][f], t, tile, lut);
+                deblock_h_body<uint16_t, kAny>(args.p[g][f], t, tile, lut);
         }
         __syncthreads();
     });
 }
 
-template <int TH, bool kLoop>
+template <int TH, bool kLoop, bool kAny>
 __global__ __launch_bounds__(kThreads) void deblock_vh_mix(DevArgs3<DeblockVHArgs> args, MixSched s) {
     __shared__ __align__(16) VHTile<TH> sm;
     for_each_mixed_item<kLoop>(s, [&](int g, int f, int t) {
         if (g == 0)
-            deblock_vh_body<uint8_t, TH>(args.p[0][f], t, sm);
+            deblock_vh_body<uint8_t, TH, kAny>(args.p[0][f], t, sm);
         else
-            deblock_vh_body<uint16_t, TH>(args.p[g][f], t, sm);
+            deblock_vh_body<uint16_t, TH, kAny>(args.p[g][f], t, sm);
         __syncthreads();
     });
 }
 
-void launch_deblock_mix(bool vert, DevArgs3<DeblockArgs> args, const MixSched &s, int grid, hipStream_t st) {
+// Picks the instantiation of launch expression L for the flags given.
+#define R2G_ANY(any, L)              \
+    do {                             \
+        if (any) {                   \
+            constexpr bool kA = true;  \
+            L;                       \
+        } else {                     \
+            constexpr bool kA = false; \
+            L;                       \
+        }                            \
+    } while (0)
+
+void launch_deblock_mix(bool vert, DevArgs3<DeblockArgs> args, const MixSched &s, int grid, bool any, hipStream_t st) {
     if (s.first[3] <= 0) return;
     const bool loop = grid < s.first[3];
     const dim3 G(grid), B(kThreads);
     if (vert) {
         if (loop)
-            R2G_LAUNCH((deblock_mix<true, true>), G, B, 0, st, args, s);
+            R2G_ANY(any, R2G_LAUNCH((deblock_mix<true, true, kA>), G, B, 0, st, args, s));
         else
-            R2G_LAUNCH((deblock_mix<true, false>), G, B, 0, st, args, s);
+            R2G_ANY(any, R2G_LAUNCH((deblock_mix<true, false, kA>), G, B, 0, st, args, s));
     } else {
         if (loop)
-            R2G_LAUNCH((deblock_mix<false, true>), G, B, 0, st, args, s);
+            R2G_ANY(any, R2G_LAUNCH((deblock_mix<false, true, kA>), G, B, 0, st, args, s));
         else
-            R2G_LAUNCH((deblock_mix<false, false>), G, B, 0, st, args, s);
+            R2G_ANY(any, R2G_LAUNCH((deblock_mix<false, false, kA>), G, B, 0, st, args, s));
     }
 }
 
-void launch_deblock_vh_mix(DevArgs3<DeblockVHArgs> args, const MixSched &s, int grid, int rows, hipStream_t st) {
+void launch_deblock_vh_mix(DevArgs3<DeblockVHArgs> args, const MixSched &s, int grid, int rows, bool any,
+                           hipStream_t st) {
     if (s.first[3] <= 0) return;
     const bool loop = grid < s.first[3];
     const dim3 G(grid), B(kThreads);
     if (rows == 64) {
         if (loop)
-            R2G_LAUNCH((deblock_vh_mix<64, true>), G, B, 0, st, args, s);
+            R2G_ANY(any, R2G_LAUNCH((deblock_vh_mix<64, true, kA>), G, B, 0, st, args, s));
         else
-            R2G_LAUNCH((deblock_vh_mix<64, false>), G, B, 0, st, args, s);
+            R2G_ANY(any, R2G_LAUNCH((deblock_vh_mix<64, false, kA>), G, B, 0, st, args, s));
     } else {
         if (loop)
-            R2G_LAUNCH((deblock_vh_mix<32, true>), G, B, 0, st, args, s);
+            R2G_ANY(any, R2G_LAUNCH((deblock_vh_mix<32, true, kA>), G, B, 0, st, args, s));
         else
-            R2G_LAUNCH((deblock_vh_mix<32, false>), G, B, 0, st, args, s);
+            R2G_ANY(any, R2G_LAUNCH((deblock_vh_mix<32, false, kA>), G, B, 0, st, args, s));
     }
 }
 
-void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, int bitdepth,
+void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, int bitdepth, bool any,
                           hipStream_t st) {
     if (s.total <= 0) return;
     const bool loop = grid < s.total;
-#define R2G_DB_LAUNCH(K, P, L) R2G_LAUNCH((K<P, L>), dim3(grid), dim3(kThreads), 0, st, d_args, s)
+#define R2G_DB_LAUNCH(K, P, L) R2G_ANY(any, R2G_LAUNCH((K<P, L, kA>), dim3(grid), dim3(kThreads), 0, st, d_args, s))
 #define R2G_DB_PICK(K)                                     \
     do {                                                   \
         if (bitdepth > 8) {                                \
@@ -550,29 +630,32 @@ TileMap deblock_vh_tile_map(const DeblockArgs &a, int rows) {
 void launch_deblock_vh(const DeblockVHArgs &a, int bitdepth, int rows, hipStream_t s) {
     if (a.map.first[3] == 0) return;
     const dim3 grid(a.map.first[3]), block(kThreads);
+    const bool any = a.a.uvm != nullptr;
     if (bitdepth > 8) {
         if (rows == 64)
-            R2G_LAUNCH((deblock_vh_kernel<uint16_t, 64>), grid, block, 0, s, a);
+            R2G_ANY(any, R2G_LAUNCH((deblock_vh_kernel<uint16_t, 64, kA>), grid, block, 0, s, a));
         else
-            R2G_LAUNCH((deblock_vh_kernel<uint16_t, 32>), grid, block, 0, s, a);
+            R2G_ANY(any, R2G_LAUNCH((deblock_vh_kernel<uint16_t, 32, kA>), grid, block, 0, s, a));
     } else {
         if (rows == 64)
-            R2G_LAUNCH((deblock_vh_kernel<uint8_t, 64>), grid, block, 0, s, a);
+            R2G_ANY(any, R2G_LAUNCH((deblock_vh_kernel<uint8_t, 64, kA>), grid, block, 0, s, a));
         else
-            R2G_LAUNCH((deblock_vh_kernel<uint8_t, 32>), grid, block, 0, s, a);
+            R2G_ANY(any, R2G_LAUNCH((deblock_vh_kernel<uint8_t, 32, kA>), grid, block, 0, s, a));
     }
 }
 
 void launch_deblock_vh_batch(const DeblockVHArgs *d_args, const BatchSched &s, int grid, int bitdepth, int rows,
-                             hipStream_t st) {
+                             bool any, hipStream_t st) {
     if (s.total <= 0) return;
     const bool loop = grid < s.total;
-#define R2G_VH(P, TH)                                                                                      \
-    do {                                                                                                   \
-        if (loop)                                                                                          \
-            R2G_LAUNCH((deblock_vh_batch<P, TH, true>), dim3(grid), dim3(kThreads), 0, st, d_args, s);   \
-        else                                                                                               \
-            R2G_LAUNCH((deblock_vh_batch<P, TH, false>), dim3(grid), dim3(kThreads), 0, st, d_args, s);  \
+#define R2G_VH(P, TH)                                                                                        \
+    do {                                                                                                     \
+        if (loop)                                                                                            \
+            R2G_ANY(any, R2G_LAUNCH((deblock_vh_batch<P, TH, true, kA>), dim3(grid), dim3(kThreads), 0, st,  \
+                                    d_args, s));                                                             \
+        else                                                                                                 \
+            R2G_ANY(any, R2G_LAUNCH((deblock_vh_batch<P, TH, false, kA>), dim3(grid), dim3(kThreads), 0, st, \
+                                    d_args, s));                                                             \
     } while (0)
     if (bitdepth > 8) {
         if (rows == 64)
@@ -603,20 +686,22 @@ void launch_deblock_v(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
     DeblockArgs a = a0;
     a.map = deblock_tile_map(a, true);
     if (a.map.first[3] == 0) return;
+    const bool any = a.uvm != nullptr;
     if (bitdepth > 8)
-        R2G_LAUNCH(deblock_v_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+        R2G_ANY(any, R2G_LAUNCH((deblock_v_kernel<uint16_t, kA>), dim3(a.map.first[3]), dim3(kThreads), 0, s, a));
     else
-        R2G_LAUNCH(deblock_v_kernel<uint8_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+        R2G_ANY(any, R2G_LAUNCH((deblock_v_kernel<uint8_t, kA>), dim3(a.map.first[3]), dim3(kThreads), 0, s, a));
 }
 
 void launch_deblock_h(const DeblockArgs &a0, int bitdepth, hipStream_t s) {
     DeblockArgs a = a0;
     a.map = deblock_tile_map(a, false);
     if (a.map.first[3] == 0) return;
+    const bool any = a.uvm != nullptr;
     if (bitdepth > 8)
-        R2G_LAUNCH(deblock_h_kernel<uint16_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+        R2G_ANY(any, R2G_LAUNCH((deblock_h_kernel<uint16_t, kA>), dim3(a.map.first[3]), dim3(kThreads), 0, s, a));
     else
-        R2G_LAUNCH(deblock_h_kernel<uint8_t>, dim3(a.map.first[3]), dim3(kThreads), 0, s, a);
+        R2G_ANY(any, R2G_LAUNCH((deblock_h_kernel<uint8_t, kA>), dim3(a.map.first[3]), dim3(kThreads), 0, s, a));
 }
 
 }  // namespace r2g

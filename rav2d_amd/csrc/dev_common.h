@@ -7,6 +7,21 @@
 
 namespace r2g {
 
+// The chroma edge and lossless masks of one SB256 at full size, the device
+// copy of a 4:2:2 or 4:4:4 frame's Av2Filter chroma fields (lf_mask.rs:9, 18):
+// lines and rows are the plane's 4x4 units in the SB256 (64 >> subsampling),
+// filter_uv words hold 16 units each, lossless words one 64-luma-column band.
+struct ChromaMasks {
+    uint16_t filter_uv[2][64][5][4];
+    uint16_t lossless_mask_uv[64][4];
+};
+
+// Chroma subsampling of a layout (I400 and I420: both; I422: horizontal).
+__host__ __device__ inline int layout_ss_hor(int layout) { return layout != RAV2D_PIXEL_LAYOUT_I444; }
+__host__ __device__ inline int layout_ss_ver(int layout) {
+    return layout == RAV2D_PIXEL_LAYOUT_I420 || layout == RAV2D_PIXEL_LAYOUT_I400;
+}
+
 // One plane of one frame as the kernels see it.
 struct DevPlane {
     uint8_t *ptr;     // first sample

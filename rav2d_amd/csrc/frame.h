@@ -115,17 +115,19 @@ struct Rav2dGpuFrame {
     // (rav2d_gpu_filter_planes); null = the frame's own
     uint8_t *ext[NBUF][3] = {};
     Rav2dPackedFilter *d_mask = nullptr;
+    r2g::ChromaMasks *d_uvm = nullptr;  // 4:2:2 / 4:4:4 frames: full-size chroma masks per SB256
     size_t n_mask = 0;
     size_t n_lr = 0;
     r2g::DbLut *d_dblut = nullptr;
     // segment maps (rav2d_gpu_frame_set_segmap): luma ids bw x bh, then chroma
-    // ids (bw / 2) x (bh / 2), packed; allocated at the first set
+    // ids (bw >> ss_hor) x (bh >> ss_ver), packed; allocated at the first set
     uint8_t *d_seg = nullptr;
     bool seg_ok = false;
     size_t bytes = 0;
     bool side_ok = false;
     // loop-restoration work list (rebuilt when params or side info change)
     std::vector<Rav2dPackedFilter> h_mask;  // host copy (after the bottom crop)
+    std::vector<r2g::ChromaMasks> h_uvm;    // host copy of d_uvm (4:2:2 / 4:4:4)
     std::vector<Rav2dAv2Restoration> h_lr;
     std::vector<r2g::LrItem> lr_items;
     r2g::LrItem *d_lr_items = nullptr;

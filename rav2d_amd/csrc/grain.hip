@@ -88,14 +88,19 @@ __device__ __forceinline__ int lane_up(int v) { return __builtin_amdgcn_mov_dpp(
 
 constexpr int kLumaSteps = (GW - 1) + 4 * (GH - 1) + 1;
 
-template <bool kChroma>
+// kChroma: a chroma plane of subsampling (SX, SY) (template (SY ? 38 : 73) x
+// (SX ? 44 : 82)), whose AR filter adds the average of the co-located luma
+// template samples; 4:2:0 chroma runs alongside the luma wave, 4:2:2 / 4:4:4
+// chroma waits for the whole luma template.
+template <bool kChroma, int SX = 1, int SY = 1>
 __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7], int ar_shift, int gmin, int gmax,
                                         const int16_t (*luma)[GW], int wl, int *done, uint32_t *err) {
-    constexpr int W = kChroma ? SGW : GW, H = kChroma ? SGH : GH;
+    constexpr int W = kChroma ? (SX ? SGW : GW) : GW, H = kChroma ? (SY ? SGH : GH) : GH;
     constexpr int kSteps = (W - 1) + 4 * (H - 1) + 1;
-    static_assert(!kChroma || 32 * ((kSteps + 15) / 16 - 1) + 21 <= 16 * ((kLumaSteps + 15) / 16),
+    constexpr int kLumaPublished = 16 * ((kLumaSteps + 15) / 16);
+    static_assert(!kChroma || !(SX && SY) || 32 * ((kSteps + 15) / 16 - 1) + 21 <= kLumaPublished,
                   "every chroma block's luma steps get published");
-    static_assert(kChroma || (4 * 8 + GW - 1 < 128 && H - 64 <= 9), "luma rows 64.. start after the switch");
+    static_assert(4 * 8 + W - 1 < 128 && (H <= 64 || H - 64 <= 9), "rows 64.. start after the switch");
     const int lane = threadIdx.x & 63;
     int cf[4][7];
 #pragma unroll
@@ -110,13 +115,13 @@ __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7]
     const int rnd = (1 << ar_shift) >> 1;
     int y = lane;
     for (int b = 0; b < (kSteps + 15) / 16; b++) {
-        if (!kChroma && b == 8 && lane < H - 64) y = lane + 64;
+        if (H > 64 && b == 8 && lane < H - 64) y = lane + 64;
         if (kChroma && wl != 0) {
             // bounded wait (the luma wave never waits, so the count always
             // arrives); should it not, the error word tells the host, which then
             // fails the frame's next synchronizing call instead of returning
             // grain built from unfinished luma rows
-            const int need = 32 * b + 21;
+            const int need = (SX && SY) ? 32 * b + 21 : kLumaPublished;
             bool arrived = false;
             for (int spin = 0; spin < (1 << 20); spin++) {
                 if (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= need) {
@@ -132,8 +137,12 @@ __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7]
         // (y, s - 4y) is element y (GW - 4) + s of the plane's rows; lanes
         // without a sample at step s read (and never write) other LDS entries
         const int base = y * (GW - 4), xm = -4 * y - 3;
-        // co-located luma of chroma sample (y, x): rows 2y-3, 2y-2, columns 2x-3, 2x-2
-        const int lbase = kChroma ? dmax(y * (2 * GW - 8) - 3 * GW - 3, 0) : 0;
+        // co-located luma of chroma sample (y, x) = step s - 4y: rows ((y - 3) << SY) + 3
+        // (and the next with SY), columns ((x - 3) << SX) + 3 (and the next with SX)
+        const int lbase = !kChroma ? 0
+                          : (SX && SY) ? dmax(y * (2 * GW - 8) - 3 * GW - 3, 0)
+                          : SX         ? dmax(y * (GW - 8) - 3, 0)
+                                       : y * (GW - 4);
 #pragma unroll
         for (int u = 0; u < 16; u++) {
             const int s = b * 16 + u;
@@ -143,8 +152,15 @@ __device__ __forceinline__ void ar_wave(int16_t (*L)[GW], const int8_t (*win)[7]
             const int raw = Lf[base + s];
             int lt = 0;
             if constexpr (kChroma) {
-                const int i = lbase + 2 * s;
-                lt = __mul24((lf[i] + lf[i + 1] + lf[i + GW] + lf[i + GW + 1] + 2) >> 2, wl);
+                if constexpr (SX && SY) {
+                    const int i = lbase + 2 * s;
+                    lt = __mul24((lf[i] + lf[i + 1] + lf[i + GW] + lf[i + GW + 1] + 2) >> 2, wl);
+                } else if constexpr (SX) {
+                    const int i = lbase + 2 * s;
+                    lt = __mul24((lf[i] + lf[i + 1] + 1) >> 1, wl);
+                } else {
+                    lt = __mul24((int)lf[lbase + s], wl);
+                }
             }
             // four partial sums: rows y-3, y-2, y-1 and the own row
             int a0 = 0, a1 = 0, a2 = 0, a3 = lt;
@@ -275,13 +291,16 @@ __device__ __forceinline__ void grain_setup_body(const GrainArgs &a, const int16
     }
     if (t == 0) sm.luma_done = 0;
     __syncthreads();
-    // Gaussian draws, kChunk per lane: lanes 0..127 the luma sequence (73x82),
-    // 128..163 and 164..199 the U and V sequences (38x44 each)
-    {
-        const int pl = t < 128 ? 0 : (t < 164 ? 1 : 2);
-        const int k = pl == 0 ? t : (pl == 1 ? t - 128 : t - 164);
-        const int w = pl ? SGW : GW, n = pl ? SGW * SGH : GW * GH;
-        const bool on = t < 200 && (pl == 0 || (pl == 1 ? want_u : want_v));
+    // Gaussian draws, kChunk per job: jobs 0..127 the luma sequence (73x82), then
+    // the U and V sequences (chroma templates cw x ch: 44x38 for 4:2:0, 44x73
+    // for 4:2:2, 82x73 for 4:4:4), one job per lane and round
+    const int cw = a.ss_hor ? SGW : GW, chh = a.ss_ver ? SGH : GH;
+    const int cjobs = (cw * chh + kChunk - 1) / kChunk;
+    for (int job = t; job < 128 + 2 * cjobs; job += kSetupThreads) {
+        const int pl = job < 128 ? 0 : (job < 128 + cjobs ? 1 : 2);
+        const int k = pl == 0 ? job : (pl == 1 ? job - 128 : job - 128 - cjobs);
+        const int w = pl ? cw : GW, n = pl ? cw * chh : GW * GH;
+        const bool on = pl == 0 || (pl == 1 ? want_u : want_v);
         const int first = k * kChunk, last = dmin(first + kChunk, n);
         if (on && first < n) {
             uint32_t s = a.seed;
@@ -305,18 +324,27 @@ __device__ __forceinline__ void grain_setup_body(const GrainArgs &a, const int16
     } else if (wave < 3) {
         const int uv = wave - 1;
         const int wl = d.num_points[0] > 0 ? d.ar_coeffs[1 + uv][2 * lag * (lag + 1)] : 0;
-        if (uv == 0 ? want_u : want_v)
-            ar_wave<true>(lut[1 + uv], win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl, &sm.luma_done,
-                          a.err);
+        if (uv == 0 ? want_u : want_v) {
+            if (a.ss_ver)
+                ar_wave<true, 1, 1>(lut[1 + uv], win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl,
+                                    &sm.luma_done, a.err);
+            else if (a.ss_hor)
+                ar_wave<true, 1, 0>(lut[1 + uv], win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl,
+                                    &sm.luma_done, a.err);
+            else
+                ar_wave<true, 0, 0>(lut[1 + uv], win[1 + uv], (int)d.ar_coeff_shift, gmin, gmax, lut[0], wl,
+                                    &sm.luma_done, a.err);
+        }
     } else {
         scaling_luts(d, bd, sm.scl, lane);
         band_offsets(a, lane);
     }
     __syncthreads();
     // out: the scaling LUTs and the grain LUTs' sample areas (luma rows 0..72,
-    // chroma rows 0..37 at the same row stride), four bytes at a time
+    // chroma rows 0..chh-1 at the same row stride), four bytes at a time
     for (int i = t; i < 3 * 4096 / 4; i += kSetupThreads) ((uint32_t *)a.scaling)[i] = ((const uint32_t *)sm.scl)[i];
-    constexpr int kLw = GH * GW / 2, kCw = SGH * GW / 2;
+    constexpr int kLw = GH * GW / 2;
+    const int kCw = chh * GW / 2;
     for (int i = t; i < kLw + 2 * kCw; i += kSetupThreads) {
         const int pl = i < kLw ? 0 : (i < kLw + kCw ? 1 : 2);
         const int j = pl * kLw + (pl ? (i - kLw - (pl - 1) * kCw) : i);
@@ -352,7 +380,9 @@ __global__ __launch_bounds__(kSetupThreads) void grain_setup_batch(const GrainAr
 // one word per lane).
 template <int BD> constexpr int scl_words() { return (1 << BD) > 1024 ? (1 << BD) / 1024 : 1; }
 
-template <typename P, int BD>
+// kAny: chroma of any subsampling (a.ss_hor / a.ss_ver); the 4:2:0 / I400
+// instantiation has it as a constant.
+template <typename P, int BD, bool kAny>
 __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id, uint8_t *scl) {
     int pl, tx, ty;
     decode_tile(a.map, tile_id, pl, tx, ty);
@@ -363,9 +393,12 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
     constexpr int bd = BD, bdm8 = bd - 8, bdmax = (1 << bd) - 1;
     constexpr int ctr = 128 << bdm8, gmin = -ctr, gmax = ctr - 1;
     constexpr int NW = scl_words<BD>();
-    const int sub = pl ? 1 : 0;
-    const int lg = 4 + d.block_size - sub;  // log2 of the block size in this plane (16 << block_size luma)
-    const int pbs = 1 << lg;
+    // the subsampling of chroma planes (cssx, cssy: constant 1 for 4:2:0) and of this plane
+    const int cssx = kAny ? a.ss_hor : 1, cssy = kAny ? a.ss_ver : 1;
+    const int ssx = pl ? cssx : 0, ssy = pl ? cssy : 0;
+    // log2 of the block width / height in this plane (16 << block_size luma)
+    const int lgx = 4 + d.block_size - ssx, lgy = 4 + d.block_size - ssy;
+    const int pbsx = 1 << lgx, pbsy = 1 << lgy;
     const int16_t *lut = a.lut + pl * GH * GW;
     const uint8_t *scaling = a.scaling + (pl && d.chroma_scaling_from_luma ? 0 : pl) * 4096;
     const bool on = a.plane_on[pl];
@@ -381,7 +414,7 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
     const uint8_t *sbase = src.ptr + (uint64_t)(uint32_t)y0 * sstride + x0 * sizeof(P);
     uint8_t *dbase = dst.ptr + (uint64_t)(uint32_t)y0 * dstride + x0 * sizeof(P);
     const uint32_t lstride = (uint32_t)a.src[0].stride;
-    const uint8_t *lbase = a.src[0].ptr + (uint64_t)(uint32_t)(2 * y0) * lstride + 2 * x0 * sizeof(P);
+    const uint8_t *lbase = a.src[0].ptr + (uint64_t)(uint32_t)(y0 << cssy) * lstride + (x0 << cssx) * sizeof(P);
     const int t = threadIdx.x;
     const int c4 = (t & 15) * 4;  // column of the group in the tile
     // first loads: the scaling LUT words of this lane
@@ -401,12 +434,12 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         const int rc = dmin(r, H - 1 - y0), cc = dmin(c4, W - 4 - x0);  // clamped: always valid addresses
         sq[u] = load4((const P *)(sbase + __umul24(rc, sstride)) + cc);
         if (pl) {
-            const P *lrow = (const P *)(lbase + __umul24(2 * rc, lstride)) + 2 * cc;
+            const P *lrow = (const P *)(lbase + __umul24(rc << cssy, lstride)) + (cc << cssx);
             lq0[u] = load4(lrow);
-            lq1[u] = load4(lrow + 4);
+            lq1[u] = cssx ? load4(lrow + 4) : lq0[u];
         }
         if (on) {
-            const int band = (y0 + rc) >> lg, b = (x0 + cc) >> lg;
+            const int band = (y0 + rc) >> lgy, b = (x0 + cc) >> lgx;
             const int bl = dmax(b - 1, 0), bu = dmax(band - 1, 0);
             const uint16_t *ob = (const uint16_t *)a.offsets;
             const int rb = __umul24(band, a.n_blocks), ru = __umul24(bu, a.n_blocks);
@@ -425,11 +458,12 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
 #pragma unroll
     for (int k = 0; k < NW; k++) ((uint32_t *)scl)[t + 256 * k] = sclw[k];  // on is tile-uniform: every lane reaches the barrier
     __syncthreads();
-    auto Wt = [&](int k, int j) -> int { return sub ? (k == 0 ? (j == 0 ? 23 : 22) : 0) : ((k == j) ? 27 : 17); };
-    const int os = 2 >> sub;
+    // overlap weights of a direction with / without subsampling (filmgrain.rs:248-592)
+    auto Wt = [&](int ss, int k, int j) -> int { return ss ? (k == 0 ? (j == 0 ? 23 : 22) : 0) : ((k == j) ? 27 : 17); };
+    const int osx = 2 >> ssx, osy = 2 >> ssy;
     // template index of block offsets `o` (x byte high, y byte low) at (lx, ly)
     auto tix = [&](uint32_t o, int lx, int ly) -> int {
-        return __umul24(3 + os * (3 + (int)(o & 0xff)) + ly, GW) + 3 + os * (3 + (int)(o >> 8)) + lx;
+        return __umul24(3 + osy * (3 + (int)(o & 0xff)) + ly, GW) + 3 + osx * (3 + (int)(o >> 8)) + lx;
     };
 #pragma unroll
     for (int u = 0; u < NG; u++) {
@@ -442,7 +476,7 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
                                (int)(lq1[u].y & 0xffff), (int)(lq1[u].y >> 16)};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const int avg = (ls[2 * k] + ls[2 * k + 1] + 1) >> 1;
+                const int avg = cssx ? (ls[2 * k] + ls[2 * k + 1] + 1) >> 1 : ls[k];
                 if (!d.chroma_scaling_from_luma) {
                     const int uv = pl - 1;
                     const int combined = __mul24(avg, d.uv_luma_mult[uv]) + __mul24(sv[k], d.uv_mult[uv]);
@@ -460,33 +494,33 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
         int sc[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) sc[k] = scl[dmin(val[k], bdmax)];
-        const int band = y >> lg, b = x >> lg;
-        const int ly = y & (pbs - 1), lx0 = x & (pbs - 1);
+        const int band = y >> lgy, b = x >> lgx;
+        const int ly = y & (pbsy - 1), lx0 = x & (pbsx - 1);
         const int16_t *g0 = lut + tix(o00[u], lx0, ly);
         int g[4] = {g0[0], g0[1], g0[2], g0[3]};
-        // overlap with the left neighbour block (its first os <= 2 columns)
-        // and/or the upper one (first os rows).  The reference clips the
+        // overlap with the left neighbour block (its first osx <= 2 columns)
+        // and/or the upper one (first osy rows).  The reference clips the
         // overlap to the block's size at the plane's edge (min(os, bw)); plane
         // sizes are multiples of 4 (frames are multiples of 8), so a clipped
         // block still spans at least 4 > os samples.  lx0 is a multiple of 4:
         // only a group that starts a block blends horizontally, and only its
-        // samples 0 and 1 (sample 1 for luma only, os = 2).
+        // samples 0 and 1 (sample 1 without horizontal subsampling, osx = 2).
         const bool hov = d.overlap_flag && b > 0 && lx0 == 0;
-        const bool vov = d.overlap_flag && band > 0 && ly < os;
+        const bool vov = d.overlap_flag && band > 0 && ly < osy;
         if (hov || vov) {
             auto blend = [&](int n, int c, int wn, int wc) {
                 return iclip(round2(__mul24(n, wn) + __mul24(c, wc), 5), gmin, gmax);
             };
             auto hblend = [&](const int16_t *n, int (&c)[4]) {
-                c[0] = blend(n[0], c[0], Wt(0, 0), Wt(0, 1));
-                if (!sub) c[1] = blend(n[1], c[1], Wt(1, 0), Wt(1, 1));
+                c[0] = blend(n[0], c[0], Wt(ssx, 0, 0), Wt(ssx, 0, 1));
+                if (!ssx) c[1] = blend(n[1], c[1], Wt(ssx, 1, 0), Wt(ssx, 1, 1));
             };
-            if (hov) hblend(lut + tix(o10[u], pbs, ly), g);
+            if (hov) hblend(lut + tix(o10[u], pbsx, ly), g);
             if (vov) {
-                const int16_t *gt = lut + tix(o01[u], lx0, ly + pbs);
+                const int16_t *gt = lut + tix(o01[u], lx0, ly + pbsy);
                 int top[4] = {gt[0], gt[1], gt[2], gt[3]};
-                if (hov) hblend(lut + tix(o11[u], pbs, ly + pbs), top);
-                const int w0 = Wt(ly, 0), w1 = Wt(ly, 1);
+                if (hov) hblend(lut + tix(o11[u], pbsx, ly + pbsy), top);
+                const int w0 = Wt(ssy, ly, 0), w1 = Wt(ssy, ly, 1);
 #pragma unroll
                 for (int k = 0; k < 4; k++) g[k] = blend(top[k], g[k], w0, w1);
             }
@@ -503,16 +537,16 @@ __device__ __forceinline__ void grain_apply_body(const GrainArgs &a, int tile_id
     }
 }
 
-template <typename P, int BD> __global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
+template <typename P, int BD, bool kAny> __global__ __launch_bounds__(256) void grain_apply_kernel(GrainArgs a) {
     __shared__ __align__(4) uint8_t scl[1024 * scl_words<BD>()];
-    grain_apply_body<P, BD>(a, blockIdx.x, scl);
+    grain_apply_body<P, BD, kAny>(a, blockIdx.x, scl);
 }
 
-template <typename P, int BD, bool kLoop>
+template <typename P, int BD, bool kLoop, bool kAny>
 __global__ __launch_bounds__(256) void grain_apply_batch(const GrainArgs *__restrict__ args, BatchSched s) {
     __shared__ __align__(4) uint8_t scl[1024 * scl_words<BD>()];
     for_each_item<kLoop>(s, [&](int f, int t) {
-        grain_apply_body<P, BD>(args[f], t, scl);
+        grain_apply_body<P, BD, kAny>(args[f], t, scl);
         if (kLoop) __syncthreads();  // the next item restages scl
     });
 }
@@ -667,44 +701,59 @@ void launch_grain_setup_batch(const GrainArgs *d_args, int n, hipStream_t s) {
 }
 
 // Mixed launch: the tiles of every bit-depth group's frames.
-template <bool kLoop>
+template <bool kLoop, bool kAny>
 __global__ __launch_bounds__(256) void grain_apply_mix(DevArgs3<GrainArgs> args, MixSched s) {
     __shared__ __align__(4) uint8_t scl[1024 * scl_words<12>()];
     for_each_mixed_item<kLoop>(s, [&](int g, int f, int t) {
+        const GrainArgs &fa = args.p[g][f];
         if (g == 0)
-            grain_apply_body<uint8_t, 8>(args.p[0][f], t, scl);
+            grain_apply_body<uint8_t, 8, kAny>(fa, t, scl);
         else if (g == 1)
-            grain_apply_body<uint16_t, 10>(args.p[1][f], t, scl);
+            grain_apply_body<uint16_t, 10, kAny>(fa, t, scl);
         else
-            grain_apply_body<uint16_t, 12>(args.p[2][f], t, scl);
+            grain_apply_body<uint16_t, 12, kAny>(fa, t, scl);
         if (kLoop) __syncthreads();  // the next item restages scl
     });
 }
 
-void launch_grain_apply_mix(DevArgs3<GrainArgs> args, const MixSched &s, int grid, hipStream_t st) {
+void launch_grain_apply_mix(DevArgs3<GrainArgs> args, const MixSched &s, int grid, bool any, hipStream_t st) {
     if (s.first[3] <= 0) return;
-    if (grid < s.first[3])
-        R2G_LAUNCH((grain_apply_mix<true>), dim3(grid), dim3(256), 0, st, args, s);
+    const bool loop = grid < s.first[3];
+    if (loop && any)
+        R2G_LAUNCH((grain_apply_mix<true, true>), dim3(grid), dim3(256), 0, st, args, s);
+    else if (loop)
+        R2G_LAUNCH((grain_apply_mix<true, false>), dim3(grid), dim3(256), 0, st, args, s);
+    else if (any)
+        R2G_LAUNCH((grain_apply_mix<false, true>), dim3(grid), dim3(256), 0, st, args, s);
     else
-        R2G_LAUNCH((grain_apply_mix<false>), dim3(grid), dim3(256), 0, st, args, s);
+        R2G_LAUNCH((grain_apply_mix<false, false>), dim3(grid), dim3(256), 0, st, args, s);
 }
 
-void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &sc, int grid, int bitdepth, hipStream_t s) {
+void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &sc, int grid, int bitdepth, bool any,
+                              hipStream_t s) {
     if (sc.total <= 0) return;
     const bool loop = grid < sc.total;
-#define R2G_GRAIN_BATCH(P, BD)                                                                  \
-    do {                                                                                        \
-        if (loop)                                                                               \
-            R2G_LAUNCH((grain_apply_batch<P, BD, true>), dim3(grid), dim3(256), 0, s, d_args, sc);  \
-        else                                                                                    \
-            R2G_LAUNCH((grain_apply_batch<P, BD, false>), dim3(grid), dim3(256), 0, s, d_args, sc); \
+#define R2G_GRAIN_BATCH(P, BD, A)                                                                  \
+    do {                                                                                           \
+        if (loop)                                                                                  \
+            R2G_LAUNCH((grain_apply_batch<P, BD, true, A>), dim3(grid), dim3(256), 0, s, d_args, sc);  \
+        else                                                                                       \
+            R2G_LAUNCH((grain_apply_batch<P, BD, false, A>), dim3(grid), dim3(256), 0, s, d_args, sc); \
     } while (0)
-    if (bitdepth == 12)
-        R2G_GRAIN_BATCH(uint16_t, 12);
-    else if (bitdepth == 10)
-        R2G_GRAIN_BATCH(uint16_t, 10);
+#define R2G_GRAIN_BATCH_BD(A)              \
+    do {                                   \
+        if (bitdepth == 12)                \
+            R2G_GRAIN_BATCH(uint16_t, 12, A); \
+        else if (bitdepth == 10)           \
+            R2G_GRAIN_BATCH(uint16_t, 10, A); \
+        else                               \
+            R2G_GRAIN_BATCH(uint8_t, 8, A);   \
+    } while (0)
+    if (any)
+        R2G_GRAIN_BATCH_BD(true);
     else
-        R2G_GRAIN_BATCH(uint8_t, 8);
+        R2G_GRAIN_BATCH_BD(false);
+#undef R2G_GRAIN_BATCH_BD
 #undef R2G_GRAIN_BATCH
 }
 
@@ -719,12 +768,21 @@ void grain_tile_map(GrainArgs &a) {
 
 void launch_grain_apply(GrainArgs a, int bitdepth, hipStream_t s) {
     grain_tile_map(a);
-    if (bitdepth == 12)
-        R2G_LAUNCH((grain_apply_kernel<uint16_t, 12>), dim3(a.map.first[3]), dim3(256), 0, s, a);
-    else if (bitdepth == 10)
-        R2G_LAUNCH((grain_apply_kernel<uint16_t, 10>), dim3(a.map.first[3]), dim3(256), 0, s, a);
+    const dim3 grid(a.map.first[3]);
+#define R2G_GRAIN_ONE(A)                                                                      \
+    do {                                                                                      \
+        if (bitdepth == 12)                                                                   \
+            R2G_LAUNCH((grain_apply_kernel<uint16_t, 12, A>), grid, dim3(256), 0, s, a);      \
+        else if (bitdepth == 10)                                                              \
+            R2G_LAUNCH((grain_apply_kernel<uint16_t, 10, A>), grid, dim3(256), 0, s, a);      \
+        else                                                                                  \
+            R2G_LAUNCH((grain_apply_kernel<uint8_t, 8, A>), grid, dim3(256), 0, s, a);        \
+    } while (0)
+    if (a.ss_hor && a.ss_ver)
+        R2G_GRAIN_ONE(false);
     else
-        R2G_LAUNCH((grain_apply_kernel<uint8_t, 8>), dim3(a.map.first[3]), dim3(256), 0, s, a);
+        R2G_GRAIN_ONE(true);
+#undef R2G_GRAIN_ONE
 }
 
 }  // namespace r2g

@@ -29,6 +29,7 @@ struct DeblockArgs {
     DevPlane src[3], dst[3];
     DevGeom g;
     const Rav2dPackedFilter *mask;
+    const ChromaMasks *uvm;  // 4:2:2 / 4:4:4: the chroma masks (null: the packed record's 4:2:0 ones)
     const DbLut *lut;
     int32_t plane_on[3];  // plane filtered in this pass
     TileMap map;
@@ -59,6 +60,7 @@ struct CdefArgs {
     DevPlane src[3], dst[3];
     DevGeom g;
     const Rav2dPackedFilter *mask;
+    const ChromaMasks *uvm;  // 4:2:2 / 4:4:4: the chroma masks (null: the packed record's 4:2:0 ones)
     const int32_t *order;  // superblock of each launch slot (null: raster order)
     int32_t cdef_on, ccso_on, on_skiptx, damping;
     int32_t any_ll;  // a segment is lossless: CDEF and CCSO skip the masks' lossless 4x4 units
@@ -70,7 +72,7 @@ struct CdefArgs {
 };
 
 void launch_cdef(const CdefArgs &a, int bitdepth, hipStream_t s);
-void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st);
+void launch_cdef_batch(const CdefArgs *d_args, const BatchSched &s, int grid, int bitdepth, bool any, hipStream_t st);
 int cdef_items(const CdefArgs &a);  // 64x64 superblocks of the frame
 // Host side of the argument block: CCSO's offset table of one plane (from the
 // plane's packed filter_off nibbles and scale_idx), and the direction offsets.
@@ -89,6 +91,12 @@ struct LrItem {
     int8_t filt[18];              // single-filter taps (luma 16 / chroma 18)
     uint16_t noskip[17];          // lr_noskip rows of the stripe (multi-class kinds)
     uint16_t ll[17];              // lossless rows of the stripe, bit per 4x4 column
+    // chroma NS-Wiener items of 4:2:2 / 4:4:4 frames: the deblocked luma rows the
+    // reference's luma line store holds where the item reads it (above the
+    // stripe, its wrap-around row, below it twice, that one's wrap-around row);
+    // -1: a store row never written (zeros)
+    int16_t lsrc[5];
+    int16_t pad2;
 };
 
 // Device copies of the loop-restoration / GDF tables (tables.h).
@@ -116,6 +124,7 @@ struct LrArgs {
     int32_t base_q;
     int32_t widx, ns_idx;
     int32_t gdf_ref, gdf_qp, gdf_scale;
+    int32_t lstore_stride;    // samples per row of the emulated luma line store (the luma plane's stride)
     int8_t mfilt[3][16][18];  // frame filters of each plane (multi-class NS)
 };
 
@@ -138,6 +147,7 @@ struct GrainArgs {
     uint8_t *offsets;    // [n_bands][n_blocks][2]
     uint32_t *err;       // error flags the set-up raises (kGrainErr*); may be null
     int32_t n_bands, n_blocks;
+    int32_t ss_hor, ss_ver;  // chroma subsampling
     int32_t plane_on[3];
     TileMap map;
 };
@@ -145,7 +155,8 @@ struct GrainArgs {
 void launch_grain_setup(const GrainArgs &a, hipStream_t s);
 void launch_grain_apply(GrainArgs a, int bitdepth, hipStream_t s);
 void launch_grain_setup_batch(const GrainArgs *d_args, int n, hipStream_t s);
-void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &s, int grid, int bitdepth, hipStream_t st);
+void launch_grain_apply_batch(const GrainArgs *d_args, const BatchSched &s, int grid, int bitdepth, bool any,
+                              hipStream_t st);
 void grain_tile_map(GrainArgs &a);  // 64x32 tiles of the three planes
 bool grain_init_tables();           // Gaussian table and jump matrices of the current device
 // generate_grain_y (plane 0) / generate_grain_uv (plane 1 U, 2 V; 4:2:0, from
@@ -170,16 +181,18 @@ void launch_deblock_h(const DeblockArgs &a, int bitdepth, hipStream_t s);
 inline bool deblock_vh_rows_ok(int rows) { return rows == 32 || rows == 64; }
 TileMap deblock_vh_tile_map(const DeblockArgs &a, int rows);
 void launch_deblock_vh(const DeblockVHArgs &a, int bitdepth, int rows, hipStream_t s);
+// `any`: some frame of the launch has 4:2:2 / 4:4:4 chroma (the general kernels)
 void launch_deblock_vh_batch(const DeblockVHArgs *d_args, const BatchSched &s, int grid, int bitdepth, int rows,
-                             hipStream_t st);
+                             bool any, hipStream_t st);
 // One launch over the frames of every bit-depth group (MixSched).
-void launch_deblock_mix(bool vert, DevArgs3<DeblockArgs> args, const MixSched &s, int grid, hipStream_t st);
-void launch_deblock_vh_mix(DevArgs3<DeblockVHArgs> args, const MixSched &s, int grid, int rows, hipStream_t st);
-void launch_cdef_mix(DevArgs3<CdefArgs> args, const MixSched &s, int grid, hipStream_t st);
+void launch_deblock_mix(bool vert, DevArgs3<DeblockArgs> args, const MixSched &s, int grid, bool any, hipStream_t st);
+void launch_deblock_vh_mix(DevArgs3<DeblockVHArgs> args, const MixSched &s, int grid, int rows, bool any,
+                           hipStream_t st);
+void launch_cdef_mix(DevArgs3<CdefArgs> args, const MixSched &s, int grid, bool any, hipStream_t st);
 void launch_lr_mix(DevArgs3<LrArgs> args, const MixSched &s, int grid, hipStream_t st);
-void launch_grain_apply_mix(DevArgs3<GrainArgs> args, const MixSched &s, int grid, hipStream_t st);
+void launch_grain_apply_mix(DevArgs3<GrainArgs> args, const MixSched &s, int grid, bool any, hipStream_t st);
 void launch_deblock_batch(bool vert, const DeblockArgs *d_args, const BatchSched &s, int grid, int bitdepth,
-                          hipStream_t st);
+                          bool any, hipStream_t st);
 // 64x32 tiles of the three planes (the vertical-edge pass: down to the padding rows)
 TileMap deblock_tile_map(const DeblockArgs &a, bool vert);
 

@@ -129,7 +129,7 @@ constexpr int kGdfErrBase8 = 8 * (4 * 64 + 2 * 8 + 1), kGdfErrBase5 = 5 * (4 * 2
 struct LrSmem {
     alignas(16) uint16_t V[VH * VW];  // virtual image, rows [-E, 64 + E), columns [-VC, 64 + VC)
     union {
-        uint16_t Ld[(32 + 4) * LS];  // downsampled luma of a chroma stripe (chroma items)
+        uint16_t Ld[(64 + 4) * LS];  // the luma taps of a chroma stripe (chroma items): rows -2 .. h + 1
         struct {
             uint16_t cells[33 * CS * 4];     // GDF gradient cells (luma items)
             alignas(16) int8_t s_err[4096];  // GDF error table slice of (ref, qp)
@@ -247,12 +247,15 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
             erv2 = ((const uint2 *)(a.t.gdf_inter + (a.gdf_ref - 1) * 6000 + a.gdf_qp * 1000))[t];
     }
     const bool uv_single = kind == LR_KIND_SINGLE && pl != 0;
+    // 4:2:0 chroma averages luma row pairs, staged with the image below;
+    // 4:2:2 / 4:4:4 chroma (no vertical subsampling) is staged after it
+    const bool uv_ss_ver = a.g.ss_ver != 0;
     // lane t < LRP * LS owns column t % LS of the rows t / LS + LRP j
     constexpr int LRP = kThreads / LS, PERL = (32 + 4 + LRP - 1) / LRP;
     const int lc = t % LS, lr0 = t / LS;
     const bool llane = t < LRP * LS;
     uint16_t lvals[PERL];
-    if (uv_single) {
+    if (uv_single && uv_ss_ver) {
         // downsampled luma rows of the chroma stripe (cfl_ds_filter_index):
         // luma rows 2y, 2y + 1 inside the stripe (F3); across a stripe boundary
         // the deblocked rows (F2): above -2 -> (2y0-2, 2y0-2), -1 -> (2y0-2, 2y0-1),
@@ -299,11 +302,57 @@ __device__ __forceinline__ void lr_body(Args &a, int item, LrSmem &sm) {
             *(uint2 *)&V[i * 4] = u;
         }
     }
-    if (uv_single) {
+    if (uv_single && uv_ss_ver) {
 #pragma unroll
         for (int j = 0; j < PERL; j++) {
             const int gr = lr0 + LRP * j;
             if (llane && gr < h + 4) Ld[gr * LS + lc] = lvals[j];
+        }
+    }
+    if (uv_single && !uv_ss_ver) {
+        // backup_row_luma with ss_ver = 0 copies luma rows (ns_wiener_single_uv,
+        // looprestoration.rs:408-425, 1145-1430): chroma (r, c) reads luma row
+        // y0 + r at column (x0 + c) << ss_hor; the unit's luma row extends 4
+        // samples each side, replicating its first / last luma column at a
+        // frame edge.  Above the stripe both rows read the line store's row 0
+        // (the deblocked row y0 - 2), below it rows 6 and 7; no stripe beyond
+        // the frame: the first / last stripe row repeats.  The store is read at
+        // column 2 x0 + (c << ss_hor) whatever the subsampling (llpf_off = x * 2,
+        // :1957-1959): for 4:4:4 its columns past the frame width are zeros up to
+        // the store's row length and then continue in its next row.
+        auto &l3 = a.f3[0];
+        auto &l2 = a.f2[0];
+        const int ssx = a.g.ss_hor, lw = w << ssx, lx0 = x0 << ssx;
+        const int W = l3.w, S = a.lstore_stride;
+        for (int i = t; i < (h + 4) * LS; i += kThreads) {
+            const int gr = i / LS, c = i - gr * LS - 2;
+            uint16_t v = 0;
+            if (c < w + 2) {
+                const int r = gr - 2;
+                // luma offset of the tap in the unit's extended row
+                const int o = c << ssx;
+                const int oc = (o < 0 && !(edges & LRE_LEFT)) ? 0 : ((o >= lw && !(edges & LRE_RIGHT)) ? lw - 1 : o);
+                int k = -1;  // line-store row slot: 0 above, 2 / 3 below
+                int row = y0 + r;
+                if (r < 0) {
+                    if (top) k = 0; else row = y0;
+                } else if (r >= h) {
+                    if (bottom) k = r == h ? 2 : 3; else row = y0 + h - 1;
+                }
+                if (k < 0) {
+                    v = (uint16_t)row_ptr<const P>(l3, row)[lx0 + oc];
+                } else {
+                    int sc = 2 * x0 + oc, src = it.lsrc[k];
+                    if (sc >= S) {
+                        sc -= S;
+                        src = it.lsrc[k + 1];  // the store's next row
+                    } else if (sc >= W) {
+                        src = -1;  // past the frame width: never written
+                    }
+                    if (src >= 0) v = (uint16_t)row_ptr<const P>(l2, src)[sc];
+                }
+            }
+            Ld[i] = v;
         }
     }
     if (kind == LR_KIND_PC && t < 64 * 8 * 4 / 16) ((uint4 *)sm.s_pcp)[t] = pcv;

@@ -121,8 +121,12 @@ void lr_frame_state(const Rav2dFrameParams &p, int &base_q, int &widx, int &ns_i
 
 // The stripe walk of lr_stripe (looprestoration.rs:1847-2274) for one unit
 // column, emitting one work item per stripe.
+// rows of the luma line store (lr_db_line: 20 rows of the plane stride, decode.rs:3818-3835;
+// zero where not written, as in a fresh LoopFilterState)
+constexpr int kLumaStoreRows = 20;
+
 void lr_column_items(const Rav2dGpuFrame *f, int pl, int x, int y, int w, int row_h, const Rav2dRestorationUnit *unit,
-                     uint8_t edges, bool first_sby, std::vector<LrItem> &out) {
+                     uint8_t edges, bool first_sby, const int *lstore, std::vector<LrItem> &out) {
     const Rav2dFrameParams &p = f->params;
     const DevGeom &g = f->g;
     const int chroma = pl != 0, ss_ver = chroma ? g.ss_ver : 0, ss_hor = chroma ? g.ss_hor : 0;
@@ -156,6 +160,7 @@ void lr_column_items(const Rav2dGpuFrame *f, int pl, int x, int y, int w, int ro
     }
     int noskip_off = 0;
     const int hp = (g.bh * 4) >> ss_ver;
+    int store_row = 0;  // first luma line-store row of the stripe (lr_stripe's llpf_off, 4 rows per stripe)
     while (y + stripe_h <= row_h) {
         bool bottom = (sby + 1) != sbh || (y + stripe_h != row_h);
         edges = (uint8_t)((edges & ~LRE_BOTTOM) | (bottom ? LRE_BOTTOM : 0));
@@ -176,6 +181,13 @@ void lr_column_items(const Rav2dGpuFrame *f, int pl, int x, int y, int w, int ro
             const int8_t *src = pd.frame_filters_on ? pd.filter[0] : unit->ns_filter[0];
             memcpy(it.filt, src, chroma ? 18 : 16);
         }
+        // store rows 0, 1 above the stripe and 6, 7, 8 below it (looprestoration.rs:1957-1959,
+        // 1180-1360 with ss_ver = 0)
+        static const int kStoreRow[5] = {0, 1, 6, 7, 8};
+        for (int k = 0; k < 5; k++) {
+            const int r = store_row + kStoreRow[k];
+            it.lsrc[k] = (int16_t)(r < kLumaStoreRows ? lstore[r] : -1);
+        }
         if (multi)
             for (int r = 0; r < 17 && noskip_off + r < 66; r++) it.noskip[r] = noskip[noskip_off + r];
         // lossless rows (looprestoration.rs:2029-2074)
@@ -183,18 +195,22 @@ void lr_column_items(const Rav2dGpuFrame *f, int pl, int x, int y, int w, int ro
         int ll_rows = std::max(stripe_h >> 2, 1) + 1;
         if (sbi < f->h_mask.size()) {
             const Rav2dPackedFilter &m = f->h_mask[sbi];
+            // chroma lossless rows: the packed record's 32 (4:2:0) or the full 64
+            const bool full = !f->h_uvm.empty();
+            const uint16_t(*uvll)[4] = full ? f->h_uvm[sbi].lossless_mask_uv : m.lossless_mask_uv;
+            const int uv_rows = full ? 64 : 32;
             if (pl == 0 || ss_hor == 0) {
-                const uint16_t(*src)[4] = pl == 0 ? m.lossless_mask_y : m.lossless_mask_uv;
+                const uint16_t(*src)[4] = pl == 0 ? m.lossless_mask_y : uvll;
                 for (int r = 0; r < ll_rows && r < 17; r++)
-                    if (y4 + r < 64) it.ll[r] = src[y4 + r][sb64x_idx];
+                    if (y4 + r < (pl == 0 ? 64 : uv_rows)) it.ll[r] = src[y4 + r][sb64x_idx];
             } else {
                 int init_y = y >> 2, end_y = (y + stripe_h) >> 2, base = y4 >> ss_ver;
                 for (int yy = init_y, r = 0; yy < end_y; yy++, r++) {
                     int sy = base + (yy - init_y);
-                    // rows past the 4:2:0 extent (32) hold no chroma unit: zero
-                    if (sy < 32 && r < ll_rows && r < 17) {
-                        uint16_t lo = m.lossless_mask_uv[sy][sb64x_idx];
-                        uint16_t hi = sb64x_idx + 1 < 4 ? m.lossless_mask_uv[sy][sb64x_idx + 1] : 0;
+                    // rows past the chroma extent hold no chroma unit: zero
+                    if (sy < uv_rows && r < ll_rows && r < 17) {
+                        uint16_t lo = uvll[sy][sb64x_idx];
+                        uint16_t hi = sb64x_idx + 1 < 4 ? uvll[sy][sb64x_idx + 1] : 0;
                         it.ll[r] = (uint16_t)((lo & 0xff) | ((hi & 0xff) << 8));
                     }
                 }
@@ -202,6 +218,7 @@ void lr_column_items(const Rav2dGpuFrame *f, int pl, int x, int y, int w, int ro
         }
         out.push_back(it);
         if (multi) noskip_off += stripe_h >> 2;
+        store_row += 4;
         y += stripe_h;
         edges |= LRE_TOP;
         stripe_h = std::min(64 >> ss_ver, row_h - y);
@@ -211,7 +228,7 @@ void lr_column_items(const Rav2dGpuFrame *f, int pl, int x, int y, int w, int ro
 
 // lr_sbrow_plane (looprestoration.rs:2276-2399): unit selection per 64 columns.
 void lr_plane_items(const Rav2dGpuFrame *f, int pl, int y, int w, int h, int row_h, bool first_sby,
-                    std::vector<LrItem> &out) {
+                    const int *lstore, std::vector<LrItem> &out) {
     const DevGeom &g = f->g;
     const int chroma = pl != 0, ss_ver = chroma ? g.ss_ver : 0, ss_hor = chroma ? g.ss_hor : 0;
     int usl = f->params.lr_unit_size[chroma ? 1 : 0];
@@ -233,7 +250,7 @@ void lr_plane_items(const Rav2dGpuFrame *f, int pl, int y, int w, int h, int row
     auto emit = [&](int cx, int cw, uint8_t e) {
         const Rav2dRestorationUnit *ru = unit_at(lr_sb, lr_u);
         if (ru) {
-            lr_column_items(f, pl, cx, y, cw, row_h, ru, e, first_sby, out);
+            lr_column_items(f, pl, cx, y, cw, row_h, ru, e, first_sby, lstore, out);
         } else {
             // the reference skips the column: output equals input
             LrItem it;
@@ -286,9 +303,30 @@ void build_lr_items(Rav2dGpuFrame *f) {
     f->lr_items.clear();
     const int rp = f->lr_restore_planes;
     const int sbh = (g.bh + (16 << g.sb128) - 1) >> (4 + g.sb128);
+    // the luma line store (lr_db_line[0]) as copy_db / backup_db (deblock.rs:688-729,
+    // 2192-2264) leave it for each superblock row: the deblocked luma row of each
+    // store row, -1 where none was written
+    int lstore[kLumaStoreRows];
+    for (int &r : lstore) r = -1;
     for (int sby = 0; sby < sbh; sby++) {
         int not_last = sby + 1 < sbh;
         bool first = sby == 0;
+        {
+            const int H = g.bh * 4, off = 8 * (sby != 0);
+            int row = (sby << (6 + g.sb128)) - off;
+            const int rh = std::min((sby + 1) << (6 + g.sb128), H - 1);
+            if (row < rh) {
+                int sh = 64 - 8 * (row == 0), dst = 4;
+                if (row > 0)
+                    for (int i = 0; i < 4; i++) lstore[i] = lstore[(4 << g.sb128) + i];
+                while (row + sh <= rh && dst + 4 <= kLumaStoreRows) {
+                    for (int i = 0; i < 4; i++) lstore[dst + i] = row + sh - 2 + i;
+                    dst += 4;
+                    row += sh;
+                    sh = 64;
+                }
+            }
+        }
         for (int pl = 1; pl < 3; pl++) {
             if (!(rp & (1 << pl))) continue;
             int h = (g.bh * 4) >> g.ss_ver, w = (g.bw * 4) >> g.ss_hor;
@@ -296,14 +334,14 @@ void build_lr_items(Rav2dGpuFrame *f) {
             int row_h = std::min(next_row_y - (8 >> g.ss_ver) * not_last, h);
             int off = (8 * (sby != 0)) >> g.ss_ver;
             int ys = (sby << ((6 - g.ss_ver) + g.sb128)) - off;
-            lr_plane_items(f, pl, ys, w, h, row_h, first, f->lr_items);
+            lr_plane_items(f, pl, ys, w, h, row_h, first, lstore, f->lr_items);
         }
         if (rp & 1) {
             int h = g.bh * 4, w = g.bw * 4;
             int next_row_y = (sby + 1) << (6 + g.sb128);
             int row_h = std::min(next_row_y - 8 * not_last, h);
             int ys = (sby << (6 + g.sb128)) - 8 * (sby != 0);
-            lr_plane_items(f, 0, ys, w, h, row_h, first, f->lr_items);
+            lr_plane_items(f, 0, ys, w, h, row_h, first, lstore, f->lr_items);
         }
     }
     // planes the LR launch does not restore get copy items, so that one launch
@@ -438,6 +476,7 @@ int grain_args_for(const Rav2dFilmGrainData &d, uint32_t seed, int bitdepth, int
     int bs = 16 << d.block_size;
     a.n_bands = (height + bs - 1) / bs;
     a.n_blocks = (width + bs - 1) / bs;
+    a.ss_hor = a.ss_ver = 1;
     // apply_grain_row (filmgrain.rs:713, 744-747): a plane is grained only when
     // its scaling LUT exists; chroma-from-luma uses the luma LUT
     a.plane_on[0] = d.num_points[0] > 0;
@@ -456,6 +495,8 @@ int grain_args(Rav2dGpuFrame *f, GrainArgs &a) {
     }
     // I400: no chroma planes to grain (decode.rs:4693, filmgrain.rs:745-780)
     if (P.layout == RAV2D_PIXEL_LAYOUT_I400) a.plane_on[1] = a.plane_on[2] = 0;
+    a.ss_hor = f->g.ss_hor;
+    a.ss_ver = f->g.ss_ver;
     return RAV2D_OK;
 }
 
@@ -532,8 +573,8 @@ int validate(const Rav2dFrameParams *p) {
         set_last_error("bit depth must be 8, 10 or 12");
         return RAV2D_EINVAL;
     }
-    if (p->layout != RAV2D_PIXEL_LAYOUT_I420 && p->layout != RAV2D_PIXEL_LAYOUT_I400) {
-        set_last_error("only 4:2:0 and monochrome (I400) are supported");
+    if (p->layout < RAV2D_PIXEL_LAYOUT_I400 || p->layout > RAV2D_PIXEL_LAYOUT_I444) {
+        set_last_error("layout must be I400, I420, I422 or I444");
         return RAV2D_EINVAL;
     }
     if (p->sb128 != 0 && p->sb128 != 1) {
@@ -574,10 +615,11 @@ int copy_plane(Rav2dGpuFrame *f, int from, int to, int pl, hipStream_t s) {
 // one channel).
 // Monochrome (I400) pictures have no chroma planes: size, stride and bytes 0.
 void picture_layout(int w, int h, int bps, int layout, int64_t stride[3], size_t bytes[3], int pw[3], int ph[3]) {
+    const int sx = layout_ss_hor(layout), sy = layout_ss_ver(layout);
     pw[0] = w;
     ph[0] = h;
-    pw[1] = pw[2] = (w + 1) >> 1;
-    ph[1] = ph[2] = (h + 1) >> 1;
+    pw[1] = pw[2] = (w + sx) >> sx;
+    ph[1] = ph[2] = (h + sy) >> sy;
     const int pad_cols = pad_columns(w);
     for (int pl = 0; pl < 3; pl++) {
         if (pl && layout == RAV2D_PIXEL_LAYOUT_I400) {
@@ -652,6 +694,7 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
     auto deblock = [&](DeblockArgs &a, const bool *on, bool vert) {
         a.g = f->g;
         a.mask = f->d_mask;
+        a.uvm = f->d_uvm;
         a.lut = f->d_dblut;
         for (int pl = 0; pl < 3; pl++) {
             a.plane_on[pl] = on[pl];
@@ -668,7 +711,7 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
             a.seg[0] = f->d_seg;
             a.seg[1] = f->d_seg + (size_t)bw * bh;
             a.seg_stride[0] = bw;
-            a.seg_stride[1] = bw >> 1;
+            a.seg_stride[1] = bw >> f->g.ss_hor;
             a.qmax = 255 + 48 * f->g.hbd;
             memcpy(a.seg_dq, f->params.seg_delta_q, sizeof(a.seg_dq));
         }
@@ -694,6 +737,7 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
         CdefArgs &a = cp.cdef;
         a.g = f->g;
         a.mask = f->d_mask;
+        a.uvm = f->d_uvm;
         a.order = f->d_cdef_order;
         a.cdef_on = (P.inloop_filters & RAV2D_INLOOPFILTER_CDEF) != 0;
         a.ccso_on = (P.inloop_filters & RAV2D_INLOOPFILTER_CCSO) &&
@@ -748,6 +792,7 @@ int plan_chain(Rav2dGpuCtx *ctx, Rav2dGpuFrame *f, int upto, ChainPlan &cp) {
         a.gdf_ref = P.gdf_ref_dst_idx;
         a.gdf_qp = P.gdf_qp_idx;
         a.gdf_scale = P.gdf_scale;
+        a.lstore_stride = (int32_t)(f->stride[0] / f->bps);
         for (int pl = 0; pl < 3; pl++) {
             memcpy(a.mfilt[pl], P.restoration[pl].filter, sizeof(a.mfilt[pl]));
             a.f3[pl] = plane_of(f, cur[pl], pl);
@@ -968,8 +1013,8 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
     g.bitdepth = params->bitdepth;
     g.bdmax = (1 << params->bitdepth) - 1;
     g.hbd = (params->bitdepth - 8) >> 1;  // SequenceHeader.hbd: 0, 1, 2 for 8, 10, 12 bit
-    g.ss_hor = 1;
-    g.ss_ver = 1;
+    g.ss_hor = layout_ss_hor(params->layout);
+    g.ss_ver = layout_ss_ver(params->layout);
     f->bps = params->bitdepth > 8 ? 2 : 1;
     size_t total = 0;
     picture_layout(params->width, params->height, f->bps, params->layout, f->stride, f->plane_bytes, f->pw, f->ph);
@@ -983,7 +1028,8 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
     for (int pl = 0; pl < 3; pl++)
         lr_cap += (size_t)((f->pw[pl] + 63) / 64) * sbrows * 3 +
                   (size_t)((f->pw[pl] + 255) / 256) * ((f->ph[pl] + 63) / 64);
-    const size_t side_bytes = n_sb * sizeof(Rav2dPackedFilter);
+    const bool full_uv = params->layout == RAV2D_PIXEL_LAYOUT_I422 || params->layout == RAV2D_PIXEL_LAYOUT_I444;
+    const size_t side_bytes = n_sb * (sizeof(Rav2dPackedFilter) + (full_uv ? sizeof(ChromaMasks) : 0));
     const size_t n_cdef = (size_t)((params->width + 63) / 64) * ((params->height + 63) / 64);
     size_t need = total * NBUF + sizeof(DbLut) + grain_bytes + side_bytes + lr_cap * sizeof(LrItem) +
                   n_cdef * sizeof(int32_t);
@@ -1002,9 +1048,10 @@ int rav2d_gpu_frame_create(Rav2dGpuCtx *ctx, const Rav2dFrameParams *params, Rav
         hipMalloc((void **)&f->d_grain, grain_bytes) != hipSuccess ||
         hipMalloc((void **)&f->d_mask, n_sb * sizeof(Rav2dPackedFilter)) != hipSuccess ||
         hipMalloc((void **)&f->d_lr_items, lr_cap * sizeof(LrItem)) != hipSuccess ||
-        hipMalloc((void **)&f->d_cdef_order, n_cdef * sizeof(int32_t)) != hipSuccess) {
+        hipMalloc((void **)&f->d_cdef_order, n_cdef * sizeof(int32_t)) != hipSuccess ||
+        (full_uv && hipMalloc((void **)&f->d_uvm, n_sb * sizeof(ChromaMasks)) != hipSuccess)) {
         for (void *p : {(void *)f->block, (void *)f->d_dblut, (void *)f->d_grain, (void *)f->d_mask,
-                        (void *)f->d_lr_items, (void *)f->d_cdef_order})
+                        (void *)f->d_lr_items, (void *)f->d_cdef_order, (void *)f->d_uvm})
             if (p) (void)hipFree(p);
         {
             std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1056,6 +1103,7 @@ int rav2d_gpu_frame_destroy(Rav2dGpuFrame *f) {
     (void)hipStreamSynchronize(ctx->aux);
     (void)hipFree(f->block);
     (void)hipFree(f->d_dblut);
+    (void)hipFree(f->d_uvm);
     if (f->ev_setup) (void)hipEventDestroy(f->ev_setup);
     if (f->ev_free) (void)hipEventDestroy(f->ev_free);
     if (f->ev_done) (void)hipEventDestroy(f->ev_done);
@@ -1162,6 +1210,19 @@ int rav2d_gpu_frame_set_side_info(Rav2dGpuFrame *f, const Rav2dAv2Filter *mask, 
     }
     std::vector<Rav2dPackedFilter> packed(n_mask);
     rav2d_gpu_pack_filter(mask, n_mask, packed.data());
+    if (f->d_uvm) {
+        // 4:2:2 / 4:4:4: the chroma masks go to the device at full size
+        std::vector<ChromaMasks> uvm(n_mask);
+        for (size_t i = 0; i < n_mask; i++) {
+            memcpy(uvm[i].filter_uv, mask[i].filter_uv, sizeof(uvm[i].filter_uv));
+            memcpy(uvm[i].lossless_mask_uv, mask[i].lossless_mask_uv, sizeof(uvm[i].lossless_mask_uv));
+        }
+        HIP_TRY(hipSetDevice(f->ctx->device));
+        int r = frame_wait_idle(f);
+        if (r) return r;
+        HIP_TRY(hipMemcpy(f->d_uvm, uvm.data(), n_mask * sizeof(ChromaMasks), hipMemcpyHostToDevice));
+        f->h_uvm.swap(uvm);
+    }
     return set_side_info_common(f, packed, lr, n_lr, stream);
 }
 
@@ -1172,6 +1233,11 @@ int rav2d_gpu_frame_set_side_info_packed(Rav2dGpuFrame *f, const Rav2dPackedFilt
         set_last_error("side info count must equal the SB256 grid size");
         return RAV2D_EINVAL;
     }
+    if (f->d_uvm) {
+        set_last_error("the packed side-info form holds 4:2:0 chroma masks: 4:2:2 and 4:4:4 frames take "
+                       "rav2d_gpu_frame_set_side_info");
+        return RAV2D_EINVAL;
+    }
     std::vector<Rav2dPackedFilter> copy(mask, mask + n_mask);
     return set_side_info_common(f, copy, lr, n_lr, stream);
 }
@@ -1179,7 +1245,7 @@ int rav2d_gpu_frame_set_side_info_packed(Rav2dGpuFrame *f, const Rav2dPackedFilt
 int rav2d_gpu_frame_set_segmap(Rav2dGpuFrame *f, const uint8_t *segmap, ptrdiff_t stride,
                                const uint8_t *segmap_uv, ptrdiff_t uv_stride, void *stream) {
     if (!f) return RAV2D_EINVAL;
-    const int bw = f->g.bw, bh = f->g.bh, cw = bw >> 1, ch = bh >> 1;
+    const int bw = f->g.bw, bh = f->g.bh, cw = bw >> f->g.ss_hor, ch = bh >> f->g.ss_ver;
     if (!segmap) {
         f->seg_ok = false;
         return RAV2D_OK;
@@ -1414,7 +1480,8 @@ int grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *ds
                  int bitdepth, int layout, const Rav2dFilmGrainData *data, uint32_t seed, uint8_t *scratch,
                  hipStream_t s) {
     const int bps = bitdepth > 8 ? 2 : 1;
-    const int np = n_planes(layout), cw = np == 3 ? (width + 1) >> 1 : 0, ch = np == 3 ? (height + 1) >> 1 : 0;
+    const int sx = layout_ss_hor(layout), sy = layout_ss_ver(layout);
+    const int np = n_planes(layout), cw = np == 3 ? (width + sx) >> sx : 0, ch = np == 3 ? (height + sy) >> sy : 0;
     const int pw[3] = {width, cw, cw};
     const int ph[3] = {height, ch, ch};
     if (!data_has_grain(*data)) {
@@ -1432,6 +1499,8 @@ int grain_planes(Rav2dGpuCtx *ctx, const Rav2dPlanes *src, const Rav2dPlanes *ds
         a.dst[pl] = DevPlane{(uint8_t *)dst->data[pl], dst->stride[pl ? 1 : 0], pw[pl], ph[pl]};
     }
     if (np == 1) a.plane_on[1] = a.plane_on[2] = 0;  // I400: luma only
+    a.ss_hor = sx;
+    a.ss_ver = sy;
     if (!grain_init_tables()) {
         set_last_error("film grain tables: device set-up failed");
         return RAV2D_EDEVICE;
@@ -1464,7 +1533,8 @@ int check_grain_planes(const Rav2dPlanes *src, const Rav2dPlanes *dst, int width
             }
         const bool uv = np == 3;
         if ((q->stride[0] & 7) || (uv && (q->stride[1] & 7)) || q->stride[0] < (ptrdiff_t)width * bps ||
-            (uv && q->stride[1] < (ptrdiff_t)((width + 1) >> 1) * bps) || q->stride[0] >= (1 << 24) ||
+            (uv && q->stride[1] < (ptrdiff_t)((width + layout_ss_hor(layout)) >> layout_ss_hor(layout)) * bps) ||
+            q->stride[0] >= (1 << 24) ||
             q->stride[1] >= (1 << 24)) {
             set_last_error("grain planes: strides must be multiples of 8 covering a row, below 16 MiB");
             return RAV2D_EINVAL;

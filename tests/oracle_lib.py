@@ -75,12 +75,23 @@ def ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def filter_frame(sf, inloop_mask=R.INLOOP_ALL, params=None):
-    """Run the oracle chain on copies of a SynthFrame's planes; returns [y, u, v]."""
+def filter_frame(sf, inloop_mask=R.INLOOP_ALL, params=None, y_stride=None):
+    """Run the oracle chain on copies of a SynthFrame's planes; returns [y, u, v].
+    y_stride: samples per luma row of the oracle's picture (default: the
+    padded plane's width); the reference's 4:4:4 chroma restoration reads its
+    luma line store, whose rows have that length, past the frame width."""
     lib = load()
     # the planes inside their picture padding (8 rows below, 8 columns right),
     # which the deblocking reads where transforms overhang the frame edge
     padded = sf.padded_planes()
+    if y_stride is not None and y_stride != padded[0].shape[1]:
+        # rows of exactly y_stride samples (narrower than the padded plane only
+        # for widths that are multiples of 128, which read no padding to the right)
+        assert y_stride >= sf.width
+        rows = np.zeros((padded[0].shape[0], y_stride), dtype=padded[0].dtype)
+        n = min(y_stride, padded[0].shape[1])
+        rows[:, :n] = padded[0][:, :n]
+        padded[0] = rows
     bps = padded[0].itemsize
     p = params if params is not None else sf.params
     seg, seg_stride, seg_uv, uv_seg_stride = sf.segmaps()

@@ -35,8 +35,8 @@ def _params(sf):
     ("width", 0, "multiple of 8"),
     ("bitdepth", 9, "bit depth"),
     ("bitdepth", 16, "bit depth"),
-    ("layout", 3, "4:2:0"),
-    ("layout", 2, "4:2:0"),
+    ("layout", 4, "layout must be"),
+    ("layout", -1, "layout must be"),
     ("sb128", 2, "sb128"),
     ("width", 16392, "at most 16384"),
     ("height", 32768, "at most 16384"),

@@ -1,9 +1,10 @@
 """Monochrome (I400) frames: luma only, as the reference filters a monochrome
 sequence (no chroma planes: deblock.rs:1745, 2002; cdef.rs:728; the chroma
 restore bits masked off, looprestoration.rs:2401-2409; no chroma grain,
-decode.rs:4693 / filmgrain.rs:745-780).  4:2:2 and 4:4:4 frames exist in the
-oracle and the synthetic generator only (the HIP library rejects them); their
-luma, which no filter derives from chroma, equals the 4:2:0 frame's.
+decode.rs:4693 / filmgrain.rs:745-780).  The oracle's 4:2:2 and 4:4:4 frames
+are pinned here the same way: their luma, which no filter derives from chroma,
+equals the 4:2:0 frame's (the GPU against the oracle in those layouts:
+test_layouts_422_444.py).
 
 Pins that do not rest on the oracle's own chroma handling: a synthetic I400
 frame carries exactly the luma planes, luma side info and luma parameters of

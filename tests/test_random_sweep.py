@@ -1,7 +1,7 @@
 """Seeded random sweep against the oracle: frames of random size, bit depth,
 superblock size, tool subset, mask features (transform partitions, sub-PU
 and TIP masks, segmentation), lossless / skip / restoration percentages and
-layout (4:2:0 or I400), each filtered and grained on the GPU per frame, and
+layout (4:2:0, 4:2:2, 4:4:4 or I400), each filtered and grained on the GPU per frame, and
 all of them together in batches (one launch per pass over every frame)."""
 import numpy as np
 import pytest
@@ -9,6 +9,7 @@ import pytest
 import rav2d_amd as R
 import oracle_lib as O
 from test_parity_deblock import _compare
+from test_layouts_422_444 import _y_stride
 
 SWEEP = 40
 
@@ -29,6 +30,8 @@ def _cases():
             skip=int(rng.integers(0, 80)),
             pcg=int(rng.integers(0, 101)),
             mono=bool(rng.random() < 0.15),
+            layout=int(rng.choice([R.PIXEL_LAYOUT_I420, R.PIXEL_LAYOUT_I420, R.PIXEL_LAYOUT_I422,
+                                   R.PIXEL_LAYOUT_I444])),
         ))
     return out
 
@@ -38,12 +41,15 @@ CASES = _cases()
 
 def _frame(c):
     return R.SynthFrame(c["w"], c["h"], c["bd"], c["sb"], c["seed"], tools=c["tools"], lossless_pct=c["ll"],
-                        skip_pct=c["skip"], pc_gdf_pct=c["pcg"], mask_flags=c["mask_flags"], monochrome=c["mono"])
+                        skip_pct=c["skip"], pc_gdf_pct=c["pcg"], mask_flags=c["mask_flags"],
+                        layout=R.PIXEL_LAYOUT_I400 if c["mono"] else c["layout"])
 
 
 def _tag(c):
-    return "%dx%d_%db_sb%d_t%d_m%d_ll%d_s%d%s" % (c["w"], c["h"], c["bd"], 64 << c["sb"], c["tools"], c["mask_flags"],
-                                                 c["ll"], c["seed"], "_i400" if c["mono"] else "")
+    lay = "i400" if c["mono"] else {R.PIXEL_LAYOUT_I420: "i420", R.PIXEL_LAYOUT_I422: "i422",
+                                     R.PIXEL_LAYOUT_I444: "i444"}[c["layout"]]
+    return "%dx%d_%db_sb%d_t%d_m%d_ll%d_s%d_%s" % (c["w"], c["h"], c["bd"], 64 << c["sb"], c["tools"], c["mask_flags"],
+                                                  c["ll"], c["seed"], lay)
 
 
 @pytest.fixture(scope="module")
@@ -64,6 +70,8 @@ def test_sweep_covers_the_space():
     for bit in (R.TOOL_DEBLOCK, R.TOOL_CDEF, R.TOOL_CCSO, R.TOOL_WIENER, R.TOOL_GDF, R.TOOL_GRAIN):
         assert any(c["tools"] & bit for c in CASES) and any(not c["tools"] & bit for c in CASES)
     assert any(c["ll"] for c in CASES) and any(c["mono"] for c in CASES)
+    assert {c["layout"] for c in CASES if not c["mono"]} == {R.PIXEL_LAYOUT_I420, R.PIXEL_LAYOUT_I422,
+                                                             R.PIXEL_LAYOUT_I444}
 
 
 @pytest.mark.gpu
@@ -74,7 +82,7 @@ def test_sweep_frame_matches_oracle(ctx, c):
     try:
         f.filter()
         f.apply_grain()
-        ref = O.filter_frame(sf)
+        ref = O.filter_frame(sf, y_stride=_y_stride(f))
         _compare(ref, f.download(1), _tag(c) + " output")
         _compare(O.apply_grain(sf, ref), f.download(2), _tag(c) + " display")
     finally:
@@ -95,7 +103,7 @@ def test_sweep_batch_matches_oracle(ctx, conf):
         b.run()
         ctx.synchronize()
         for c, sf, f in zip(CASES, sfs, frames):
-            ref = O.filter_frame(sf)
+            ref = O.filter_frame(sf, y_stride=_y_stride(f))
             _compare(ref, f.download(1), "batch " + _tag(c))
             _compare(O.apply_grain(sf, ref), f.download(2), "batch " + _tag(c) + " display")
     finally:
