@@ -634,20 +634,22 @@ int rav2d_gpu_gdf_prep(Rav2dGpuCtx *ctx, int8_t *dst, ptrdiff_t dst_stride, cons
 int rav2d_gpu_gdf_add(Rav2dGpuCtx *ctx, void *p, ptrdiff_t stride, const int8_t *err, ptrdiff_t err_stride, int w,
                       int h, int scale, const uint16_t (*ll_mask)[4], int bitdepth, void *stream);
 
-/* FilmGrainDSPContext.generate_grain_y / generate_grain_uv[I420]
- * (filmgrain.rs:115-229): the 73 x 82 luma template, or a 38 x 44 chroma
- * template (rows of 82 entries, subx = suby = 1 only) built from the luma
- * template buf_y; buffers int16_t, device. */
+/* FilmGrainDSPContext.generate_grain_y / generate_grain_uv[layout]
+ * (filmgrain.rs:115-229): the 73 x 82 luma template, or a chroma template
+ * built from the luma template buf_y: 38 x 44 for (subx, suby) = (1, 1)
+ * (4:2:0), 73 x 44 for (1, 0) (4:2:2), 73 x 82 for (0, 0) (4:4:4); rows of
+ * 82 entries; buffers int16_t, device. */
 int rav2d_gpu_generate_grain_y(Rav2dGpuCtx *ctx, int16_t *buf, const Rav2dFilmGrainData *data, uint32_t seed,
                                int bitdepth, void *stream);
 int rav2d_gpu_generate_grain_uv(Rav2dGpuCtx *ctx, int16_t *buf, const int16_t *buf_y,
                                 const Rav2dFilmGrainData *data, uint32_t seed, int uv, int subx, int suby,
                                 int bitdepth, void *stream);
-/* FilmGrainDSPContext.fgy_32x32xn / fguv_32x32xn[I420] (filmgrain.rs:248-592):
+/* FilmGrainDSPContext.fgy_32x32xn / fguv_32x32xn[layout] (filmgrain.rs:248-592):
  * one row band (bh rows, band row_num) of a plane, pw samples wide; dst, src
  * at stride; scaling: the plane's 1 << bitdepth LUT (device); grain_lut: the
  * plane's template (device, rows of 82); fguv also reads the band's luma rows
- * (luma_row at luma_stride), is_id = mc_identity, sx / sy = 1 (4:2:0 only). */
+ * (luma_row at luma_stride), is_id = mc_identity, (sx, sy) = (1, 1) 4:2:0,
+ * (1, 0) 4:2:2 or (0, 0) 4:4:4. */
 int rav2d_gpu_fgy_32x32xn(Rav2dGpuCtx *ctx, void *dst, const void *src, ptrdiff_t stride,
                           const Rav2dFilmGrainData *data, uint32_t seed, int pw, const uint8_t *scaling,
                           const int16_t *grain_lut, int bh, int row_num, int bitdepth, void *stream);

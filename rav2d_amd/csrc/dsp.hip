@@ -601,7 +601,8 @@ __device__ __forceinline__ uint32_t lfsr_draw(uint32_t bits, uint32_t &s) {
 template <typename P> __global__ __launch_bounds__(256) void dsp_grain_band(DspGrainArgs a) {
     __shared__ uint8_t offs[2][12][2];  // [band: 0 this, 1 previous][block - first][y, x]
     const Rav2dFilmGrainData &d = a.fg;
-    const int bs = 16 << d.block_size, pbs = bs >> a.sx;
+    // block width / height in this plane
+    const int bs = 16 << d.block_size, pbs = bs >> a.sx, pbsy = bs >> a.sy;
     const int x0 = blockIdx.x * 64, x1 = dmin(x0 + 64, a.pw);
     const int b_first = dmax(x0 / pbs - 1, 0), b_last = (x1 - 1) / pbs;
     const int rows = 1 + (d.overlap_flag && a.row_num > 0);
@@ -629,7 +630,7 @@ template <typename P> __global__ __launch_bounds__(256) void dsp_grain_band(DspG
     auto lut_at = [&](int b, int bx, int by, int lx, int ly) -> int {
         const uint8_t *o = offs[by][b - bx - b_first];
         const int offx = 3 + (2 >> a.sx) * (3 + o[1]), offy = 3 + (2 >> a.sy) * (3 + o[0]);
-        return a.lut[(offy + ly + pbs * by) * 82 + offx + lx + pbs * bx];
+        return a.lut[(offy + ly + pbsy * by) * 82 + offx + lx + pbs * bx];
     };
     auto blend = [&](int old, int cur, int w0, int w1) { return iclip(round2i(old * w0 + cur * w1, 5), gmin, gmax); };
     const int wts[2][2][2] = {{{27, 17}, {17, 27}}, {{23, 22}, {0, 0}}};
@@ -966,8 +967,8 @@ static int grain_band(Rav2dGpuCtx *ctx, void *dst, const void *src, ptrdiff_t st
                       int bitdepth, void *stream) {
     if (!ctx || !dst || !src || !data || !scaling || !grain_lut || pw <= 0 || pw > RAV2D_MAX_FRAME_DIM || bh <= 0 ||
         row_num < 0 || data->block_size < 0 || data->block_size > 1 || data->scaling_shift < 8 ||
-        data->scaling_shift > 11 || bh > (16 << data->block_size) || (chroma && (!luma_row || sx != 1 || sy != 1 ||
-                                                                                   uv < 0 || uv > 1)))
+        data->scaling_shift > 11 || bh > (16 << data->block_size) ||
+        (chroma && (!luma_row || sx < sy || sx < 0 || sx > 1 || sy < 0 || uv < 0 || uv > 1)))
         return bad("film grain band: invalid arguments");
     if (bad_depth(bitdepth)) return RAV2D_EINVAL;
     HIP_TRY(hipSetDevice(ctx->device));
@@ -1005,17 +1006,19 @@ int rav2d_gpu_generate_grain_y(Rav2dGpuCtx *ctx, int16_t *buf, const Rav2dFilmGr
     if (!ctx || !buf || !grain_params_ok(data)) return bad("generate_grain_y: invalid arguments");
     if (bad_depth(bitdepth)) return RAV2D_EINVAL;
     HIP_TRY(hipSetDevice(ctx->device));
-    return launch_grain_generate(*data, seed, bitdepth, 0, buf, nullptr, pick_stream(ctx, stream));
+    return launch_grain_generate(*data, seed, bitdepth, 0, 0, 0, buf, nullptr, pick_stream(ctx, stream));
 }
 
 int rav2d_gpu_generate_grain_uv(Rav2dGpuCtx *ctx, int16_t *buf, const int16_t *buf_y,
                                 const Rav2dFilmGrainData *data, uint32_t seed, int uv, int subx, int suby,
                                 int bitdepth, void *stream) {
-    if (!ctx || !buf || !buf_y || !grain_params_ok(data) || uv < 0 || uv > 1 || subx != 1 || suby != 1)
-        return bad("generate_grain_uv: invalid arguments (4:2:0 only)");
+    // (subx, suby): (1, 1) 4:2:0, (1, 0) 4:2:2, (0, 0) 4:4:4
+    if (!ctx || !buf || !buf_y || !grain_params_ok(data) || uv < 0 || uv > 1 || subx < suby || subx < 0 || subx > 1 ||
+        suby < 0)
+        return bad("generate_grain_uv: invalid arguments");
     if (bad_depth(bitdepth)) return RAV2D_EINVAL;
     HIP_TRY(hipSetDevice(ctx->device));
-    return launch_grain_generate(*data, seed, bitdepth, 1 + uv, buf, buf_y, pick_stream(ctx, stream));
+    return launch_grain_generate(*data, seed, bitdepth, 1 + uv, subx, suby, buf, buf_y, pick_stream(ctx, stream));
 }
 
 int rav2d_gpu_fgy_32x32xn(Rav2dGpuCtx *ctx, void *dst, const void *src, ptrdiff_t stride,

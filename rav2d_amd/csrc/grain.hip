@@ -609,12 +609,13 @@ static const int16_t *grain_tables() {
 
 // generate_grain_y / generate_grain_uv (filmgrain.rs:115-229) of one plane
 // into a caller's template buffer (rows of 82): the set-up's Gaussian draws
-// and wavefront AR filter for that plane alone; a chroma plane reads the
-// caller's luma template, complete before the filter starts.
+// and wavefront AR filter for that plane alone; a chroma plane (subsampling
+// ss_hor, ss_ver) reads the caller's luma template, complete before the
+// filter starts.
 struct GrainGenArgs {
     Rav2dFilmGrainData fg;
     uint32_t seed;
-    int32_t bitdepth, plane;
+    int32_t bitdepth, plane, ss_hor, ss_ver;
     int16_t *out;
     const int16_t *luma;
 };
@@ -636,8 +637,10 @@ __global__ __launch_bounds__(kSetupThreads) void grain_generate_kernel(GrainGenA
         for (int i = t; i < GH * GW; i += kSetupThreads) sm.lut[0][i / GW][i % GW] = a.luma[i];
     if (t == 0) sm.luma_done = 1 << 30;  // the luma template is complete
     __syncthreads();
+    // the plane's template: 73 x 82 luma; chroma 38 x 44 (4:2:0), 73 x 44 (4:2:2), 73 x 82 (4:4:4)
+    const int rows = pl && a.ss_ver ? SGH : GH, cols = pl && a.ss_hor ? SGW : GW;
     {
-        const int w = pl ? SGW : GW, n = pl ? SGW * SGH : GW * GH, lanes = (n + kChunk - 1) / kChunk;
+        const int w = cols, n = cols * rows, lanes = (n + kChunk - 1) / kChunk;  // at most 128 (kJump)
         const int first = t * kChunk, last = dmin(first + kChunk, n);
         if (t < lanes) {
             uint32_t s = a.seed;
@@ -659,20 +662,26 @@ __global__ __launch_bounds__(kSetupThreads) void grain_generate_kernel(GrainGenA
             ar_wave<false>(sm.lut[0], sm.win[0], (int)d.ar_coeff_shift, gmin, gmax, nullptr, 0, &sm.luma_done, nullptr);
         } else {
             const int wl = d.num_points[0] > 0 ? d.ar_coeffs[pl][2 * lag * (lag + 1)] : 0;
-            ar_wave<true>(sm.lut[pl], sm.win[pl], (int)d.ar_coeff_shift, gmin, gmax, sm.lut[0], wl, &sm.luma_done,
-                          nullptr);
+            if (a.ss_ver)
+                ar_wave<true, 1, 1>(sm.lut[pl], sm.win[pl], (int)d.ar_coeff_shift, gmin, gmax, sm.lut[0], wl,
+                                    &sm.luma_done, nullptr);
+            else if (a.ss_hor)
+                ar_wave<true, 1, 0>(sm.lut[pl], sm.win[pl], (int)d.ar_coeff_shift, gmin, gmax, sm.lut[0], wl,
+                                    &sm.luma_done, nullptr);
+            else
+                ar_wave<true, 0, 0>(sm.lut[pl], sm.win[pl], (int)d.ar_coeff_shift, gmin, gmax, sm.lut[0], wl,
+                                    &sm.luma_done, nullptr);
         }
     }
     __syncthreads();
-    const int rows = pl ? SGH : GH, cols = pl ? SGW : GW;
     for (int i = t; i < rows * cols; i += kSetupThreads) {
         const int y = i / cols, x = i - y * cols;
         a.out[y * GW + x] = sm.lut[pl][y][x];
     }
 }
 
-int launch_grain_generate(const Rav2dFilmGrainData &fg, uint32_t seed, int bitdepth, int plane, int16_t *out,
-                          const int16_t *luma, hipStream_t s) {
+int launch_grain_generate(const Rav2dFilmGrainData &fg, uint32_t seed, int bitdepth, int plane, int ss_hor, int ss_ver,
+                          int16_t *out, const int16_t *luma, hipStream_t s) {
     const int16_t *gauss = grain_tables();
     if (!gauss) return RAV2D_ENOMEM;
     GrainGenArgs a;
@@ -680,6 +689,8 @@ int launch_grain_generate(const Rav2dFilmGrainData &fg, uint32_t seed, int bitde
     a.seed = seed;
     a.bitdepth = bitdepth;
     a.plane = plane;
+    a.ss_hor = ss_hor;
+    a.ss_ver = ss_ver;
     a.out = out;
     a.luma = luma;
     hipLaunchKernelGGL(grain_generate_kernel, dim3(1), dim3(kSetupThreads), 0, s, a, gauss);

@@ -161,8 +161,8 @@ void grain_tile_map(GrainArgs &a);  // 64x32 tiles of the three planes
 bool grain_init_tables();           // Gaussian table and jump matrices of the current device
 // generate_grain_y (plane 0) / generate_grain_uv (plane 1 U, 2 V; 4:2:0, from
 // the luma template `luma`) of one plane into `out` (rows of 82, device).
-int launch_grain_generate(const Rav2dFilmGrainData &fg, uint32_t seed, int bitdepth, int plane, int16_t *out,
-                          const int16_t *luma, hipStream_t s);
+int launch_grain_generate(const Rav2dFilmGrainData &fg, uint32_t seed, int bitdepth, int plane, int ss_hor, int ss_ver,
+                          int16_t *out, const int16_t *luma, hipStream_t s);
 
 // Both deblocking passes in one tile pass: F0 -> F2 without the intermediate
 // plane.  `a` carries the vertical-edge pass's arguments (source F0, masks,

@@ -318,6 +318,10 @@ def test_gdf_prep_and_add(ctx, bd):
             d.close()
 
 
+# chroma subsampling (sx, sy) of 4:2:0, 4:2:2 and 4:4:4
+SUBSAMPLINGS = [(1, 1), (1, 0), (0, 0)]
+
+
 def _grain_data(rng, lag=3):
     d = R.FilmGrainData()
     d.ar_coeff_lag = lag
@@ -346,22 +350,25 @@ def test_generate_grain(ctx, bd, lag):
         gy = ctx.device_array(np.zeros((73, 82), dtype=np.int16))
         R._check(ctx.lib.rav2d_gpu_generate_grain_y(ctx.h, gy.ptr, C.byref(d), seed, bd, None), "grain_y")
         assert np.array_equal(gy.read(), want_y), ("y", it)
-        for uv in range(2):
-            want = np.zeros((73, 82), dtype=np.int16)
-            L.oracle_generate_grain_uv(O.ptr(want), O.ptr(want_y), C.byref(d), seed, uv, 1, 1, bd)
-            g = ctx.device_array(np.zeros((73, 82), dtype=np.int16))
-            R._check(ctx.lib.rav2d_gpu_generate_grain_uv(ctx.h, g.ptr, gy.ptr, C.byref(d), seed, uv, 1, 1, bd, None),
-                     "grain_uv")
-            assert np.array_equal(g.read()[:38, :44], want[:38, :44]), ("uv", uv, it)
-            g.close()
+        for (sx, sy) in SUBSAMPLINGS:
+            th, tw = (38 if sy else 73), (44 if sx else 82)  # the chroma template's size
+            for uv in range(2):
+                want = np.zeros((73, 82), dtype=np.int16)
+                L.oracle_generate_grain_uv(O.ptr(want), O.ptr(want_y), C.byref(d), seed, uv, sx, sy, bd)
+                g = ctx.device_array(np.zeros((73, 82), dtype=np.int16))
+                R._check(ctx.lib.rav2d_gpu_generate_grain_uv(ctx.h, g.ptr, gy.ptr, C.byref(d), seed, uv, sx, sy, bd,
+                                                             None), "grain_uv")
+                assert np.array_equal(g.read()[:th, :tw], want[:th, :tw]), ("uv", uv, it, sx, sy)
+                g.close()
         gy.close()
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("bd", [8, 10, 12])
 @pytest.mark.parametrize("bs,ov", [(0, 1), (1, 1), (0, 0), (1, 0)])
-def test_fgy_fguv(ctx, bd, bs, ov):
-    rng = np.random.default_rng(500 + bd * 8 + bs * 2 + ov)
+@pytest.mark.parametrize("sx,sy", SUBSAMPLINGS, ids=["420", "422", "444"])
+def test_fgy_fguv(ctx, bd, bs, ov, sx, sy):
+    rng = np.random.default_rng(500 + bd * 8 + bs * 2 + ov + 1000 * (sx + sy))
     L = O.load()
     lib = _olib()
     for it in range(4):
@@ -390,20 +397,20 @@ def test_fgy_fguv(ctx, bd, bs, ov):
         R._check(ctx.lib.rav2d_gpu_fgy_32x32xn(ctx.h, dd.ptr, ds.ptr, pw * it_, C.byref(d), seed, pw, dsc.ptr, dlut.ptr,
                                                bh, row_num, bd, None), "fgy")
         assert np.array_equal(dd.read().astype(np.uint16), hd), ("fgy", it)
-        # chroma of the same band: bh / 2 rows, pw / 2 wide, luma = src
-        cw, ch = pw // 2, (bh + 1) // 2
+        # chroma of the same band: bh >> sy rows, pw >> sx wide, luma = src
+        cw, ch = pw >> sx, (bh + sy) >> sy
         csrc = rng.integers(0, 1 << bd, size=(ch, cw))
         dcs, hcs = _gpu_and_host(ctx, csrc, bd)
         dcd, hcd = _gpu_and_host(ctx, np.zeros_like(csrc), bd)
         for uv in range(2):
             L.oracle_generate_grain_uv(O.ptr(lut), O.ptr(np.zeros((73, 82), dtype=np.int16)), C.byref(d), seed, uv,
-                                       1, 1, bd)
+                                       sx, sy, bd)
             dlut.write(lut)
             lib.oracle_fguv_16(_p(hcd), _p(hcs), cw, C.byref(d), seed, cw, _p(scaling), _p(lut), ch, row_num, _p(hs),
-                               pw, uv, d.mc_identity, 1, 1, bd)
+                               pw, uv, d.mc_identity, sx, sy, bd)
             R._check(ctx.lib.rav2d_gpu_fguv_32x32xn(ctx.h, dcd.ptr, dcs.ptr, cw * it_, C.byref(d), seed, cw, dsc.ptr,
-                                                    dlut.ptr, ch, row_num, ds.ptr, pw * it_, uv, d.mc_identity, 1, 1,
+                                                    dlut.ptr, ch, row_num, ds.ptr, pw * it_, uv, d.mc_identity, sx, sy,
                                                     bd, None), "fguv")
-            assert np.array_equal(dcd.read().astype(np.uint16), hcd), ("fguv", it, uv)
+            assert np.array_equal(dcd.read().astype(np.uint16), hcd), ("fguv", it, uv, sx, sy)
         for x in (ds, dd, dsc, dlut, dcs, dcd):
             x.close()
