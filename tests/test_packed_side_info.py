@@ -61,3 +61,27 @@ def test_packed_side_info_count_checked(ctx):
     finally:
         f.close()
         sf.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("layout", [R.PIXEL_LAYOUT_I422, R.PIXEL_LAYOUT_I444], ids=["i422", "i444"])
+def test_packed_side_info_rejected_for_full_chroma_masks(ctx, layout):
+    # the packed form holds 4:2:0 chroma masks: 4:2:2 / 4:4:4 frames are refused
+    # on the host, and the frame then filters from Av2Filter records as usual
+    from test_layouts_422_444 import _y_stride
+    from test_parity_deblock import _compare
+    sf = R.SynthFrame(136, 72, 10, 0, seed=64, layout=layout)
+    f = R.GpuFrame(ctx, sf.params)
+    try:
+        packed = R.pack_filter(sf.mask_ptr, sf.n_mask)
+        rc = ctx.lib.rav2d_gpu_frame_set_side_info_packed(f.h, packed.ctypes.data, sf.n_mask, sf.lr_ptr, sf.n_lr,
+                                                          None)
+        assert rc == R.RAV2D_EINVAL
+        assert b"4:2:0 chroma masks" in ctx.lib.rav2d_gpu_last_error()
+        f.set_side_info(sf.mask_ptr, sf.n_mask, sf.lr_ptr, sf.n_lr)
+        f.upload_padded(sf)
+        f.filter()
+        _compare(O.filter_frame(sf, y_stride=_y_stride(f)), f.download(1), "after the refused packed form")
+    finally:
+        f.close()
+        sf.close()
